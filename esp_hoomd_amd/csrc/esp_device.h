@@ -1,0 +1,261 @@
+// Device-side building blocks shared by the ESP kernels (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "esp_types.h"
+
+namespace esp
+{
+namespace dev
+{
+
+constexpr int WAVE = 64;
+
+template<typename T> __device__ __forceinline__ T wave_sum(T v)
+    {
+#pragma unroll
+    for (int off = WAVE / 2; off > 0; off >>= 1)
+        v += __shfl_xor(v, off, WAVE);
+    return v;
+    }
+
+//! Sum over aligned groups of G lanes (G a power of two <= 64).
+template<int G, typename T> __device__ __forceinline__ T group_sum(T v)
+    {
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1)
+        v += __shfl_xor(v, off, WAVE);
+    return v;
+    }
+
+//! Horner in z over D+1 coefficients c[0..D] (c[j] multiplies z^j), value and d/dz.
+template<typename R, int D> __device__ __forceinline__ R horner(const R* c, R z)
+    {
+    R v = c[D];
+#pragma unroll
+    for (int j = D - 1; j >= 0; --j)
+        v = v * z + c[j];
+    return v;
+    }
+
+template<typename R, int D> __device__ __forceinline__ R horner_d(const R* c, R z, R& dv)
+    {
+    R v = c[D];
+    R d = R(0);
+#pragma unroll
+    for (int j = D - 1; j >= 0; --j)
+        {
+        d = d * z + v;
+        v = v * z + c[j];
+        }
+    dv = d;
+    return v;
+    }
+
+// ---------------------------------------------------------------------------
+// Complex helpers on (x, y) pairs
+// ---------------------------------------------------------------------------
+template<typename R> struct C2
+    {
+    R x, y;
+    };
+
+template<typename R> __device__ __forceinline__ C2<R> cmul(C2<R> a, C2<R> b)
+    {
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+    }
+template<typename R> __device__ __forceinline__ C2<R> cadd(C2<R> a, C2<R> b)
+    {
+    return {a.x + b.x, a.y + b.y};
+    }
+template<typename R> __device__ __forceinline__ C2<R> csub(C2<R> a, C2<R> b)
+    {
+    return {a.x - b.x, a.y - b.y};
+    }
+//! multiply by -i (forward, sign = -1) or +i (inverse, sign = +1)
+template<typename R> __device__ __forceinline__ C2<R> cmul_si(C2<R> a, int sign)
+    {
+    return sign < 0 ? C2<R> {a.y, -a.x} : C2<R> {-a.y, a.x};
+    }
+
+// ---------------------------------------------------------------------------
+// Radix-R butterflies, in place on v[0..R), sign -1 forward / +1 inverse.
+// ---------------------------------------------------------------------------
+template<typename R> __device__ __forceinline__ void bfly2(C2<R>* v)
+    {
+    const C2<R> a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+    }
+
+template<typename R> __device__ __forceinline__ void bfly4(C2<R>* v, int sign)
+    {
+    const C2<R> a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+    const C2<R> b0 = cadd(v[1], v[3]), b1 = cmul_si(csub(v[1], v[3]), sign);
+    v[0] = cadd(a0, b0);
+    v[2] = csub(a0, b0);
+    v[1] = cadd(a1, b1);
+    v[3] = csub(a1, b1);
+    }
+
+template<typename R> __device__ __forceinline__ void bfly3(C2<R>* v, int sign)
+    {
+    // exp(sign * 2 pi i / 3) = -1/2 + sign * i sqrt(3)/2
+    const R c1 = R(-0.5);
+    const R s1 = R(sign) * R(0.86602540378443864676);
+    const C2<R> t = cadd(v[1], v[2]);
+    const C2<R> d = csub(v[1], v[2]);
+    const C2<R> m = {v[0].x + c1 * t.x, v[0].y + c1 * t.y};
+    // i * s1 * d
+    const C2<R> j = {-s1 * d.y, s1 * d.x};
+    v[0] = cadd(v[0], t);
+    v[1] = cadd(m, j);
+    v[2] = csub(m, j);
+    }
+
+template<typename R> __device__ __forceinline__ void bfly5(C2<R>* v, int sign)
+    {
+    const R c1 = R(0.30901699437494742410);   // cos(2pi/5)
+    const R c2 = R(-0.80901699437494742410);  // cos(4pi/5)
+    const R s1 = R(sign) * R(0.95105651629515357212);
+    const R s2 = R(sign) * R(0.58778525229247312917);
+    const C2<R> t1 = cadd(v[1], v[4]), d1 = csub(v[1], v[4]);
+    const C2<R> t2 = cadd(v[2], v[3]), d2 = csub(v[2], v[3]);
+    const C2<R> a1 = {v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y};
+    const C2<R> a2 = {v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y};
+    // i * (s1 d1 + s2 d2), i * (s2 d1 - s1 d2)
+    const C2<R> b1 = {-(s1 * d1.y + s2 * d2.y), s1 * d1.x + s2 * d2.x};
+    const C2<R> b2 = {-(s2 * d1.y - s1 * d2.y), s2 * d1.x - s1 * d2.x};
+    v[0] = {v[0].x + t1.x + t2.x, v[0].y + t1.y + t2.y};
+    v[1] = cadd(a1, b1);
+    v[4] = csub(a1, b1);
+    v[2] = cadd(a2, b2);
+    v[3] = csub(a2, b2);
+    }
+
+//! One Stockham stage over nlines lines of length N stored with line stride ls in LDS.
+//! tw[m] = exp(-2 pi i m / N); the inverse uses its conjugate.
+template<typename R, int RAD>
+__device__ __forceinline__ void stockham_stage(const C2<R>* __restrict__ in,
+                                               C2<R>* __restrict__ out,
+                                               const C2<R>* __restrict__ tw,
+                                               int N,
+                                               int ls,
+                                               int nlines,
+                                               int Ns,
+                                               int sign,
+                                               int tid,
+                                               int nthreads)
+    {
+    const int nb = N / RAD;
+    const int twstep = N / (Ns * RAD);
+    for (int idx = tid; idx < nlines * nb; idx += nthreads)
+        {
+        const int line = idx / nb;
+        const int j = idx - line * nb;
+        const C2<R>* src = in + line * ls;
+        C2<R>* dst = out + line * ls;
+        C2<R> v[RAD];
+        const int k = j % Ns;
+#pragma unroll
+        for (int r = 0; r < RAD; ++r)
+            v[r] = src[j + r * nb];
+#pragma unroll
+        for (int r = 1; r < RAD; ++r)
+            {
+            C2<R> w = tw[(k * r * twstep) % N];
+            if (sign > 0)
+                w.y = -w.y;
+            v[r] = cmul(v[r], w);
+            }
+        if constexpr (RAD == 2)
+            bfly2(v);
+        else if constexpr (RAD == 3)
+            bfly3(v, sign);
+        else if constexpr (RAD == 4)
+            bfly4(v, sign);
+        else
+            bfly5(v, sign);
+        const int base = (j / Ns) * Ns * RAD + k;
+#pragma unroll
+        for (int r = 0; r < RAD; ++r)
+            dst[base + r * Ns] = v[r];
+        }
+    }
+
+//! Full 1-D FFT of nlines lines in LDS (ping-pong between a and b).  Returns the buffer holding
+//! the result.  radices[0..nstages) multiply to N.
+template<typename R>
+__device__ C2<R>* lds_fft(C2<R>* a,
+                          C2<R>* b,
+                          const C2<R>* tw,
+                          const FFTPlan& plan,
+                          int ls,
+                          int nlines,
+                          int sign,
+                          int tid,
+                          int nthreads)
+    {
+    int Ns = 1;
+    C2<R>* src = a;
+    C2<R>* dst = b;
+    for (int s = 0; s < plan.nstages; ++s)
+        {
+        const int rad = plan.radix[s];
+        if (rad == 4)
+            stockham_stage<R, 4>(src, dst, tw, plan.n, ls, nlines, Ns, sign, tid, nthreads);
+        else if (rad == 2)
+            stockham_stage<R, 2>(src, dst, tw, plan.n, ls, nlines, Ns, sign, tid, nthreads);
+        else if (rad == 3)
+            stockham_stage<R, 3>(src, dst, tw, plan.n, ls, nlines, Ns, sign, tid, nthreads);
+        else
+            stockham_stage<R, 5>(src, dst, tw, plan.n, ls, nlines, Ns, sign, tid, nthreads);
+        __syncthreads();
+        Ns *= rad;
+        C2<R>* t = src;
+        src = dst;
+        dst = t;
+        }
+    return src;
+    }
+
+// ---------------------------------------------------------------------------
+// Split-table lookup (EvaluatorPairPSWF.h:113-143, reduced s = r / rc).
+// Record per segment: r_lo, dr_inv, S c0..c4, dS/ds c0..c4 (12 reals).
+// ---------------------------------------------------------------------------
+template<typename R>
+__device__ __forceinline__ void split_lookup(const R* __restrict__ tab, unsigned int n_segs, R r, R rc, R inv_rc, R& L, R& negdLdr)
+    {
+    if (r < R(1.0e-12) || r > rc)
+        {
+        L = R(0);
+        negdLdr = R(0);
+        return;
+        }
+    R s = r * inv_rc;
+    s = s < R(0) ? R(0) : (s > R(1) - R(1.0e-9) ? R(1) - R(1.0e-9) : s);
+    unsigned int seg = static_cast<unsigned int>(s * R(n_segs));
+    seg = seg >= n_segs ? n_segs - 1u : seg;
+    const R* c = tab + size_t(seg) * 12u;
+    const R t = (s - c[0]) * c[1];
+    R S = c[6];
+    S = c[5] + t * S;
+    S = c[4] + t * S;
+    S = c[3] + t * S;
+    S = c[2] + t * S;
+    R dS = c[11];
+    dS = c[10] + t * dS;
+    dS = c[9] + t * dS;
+    dS = c[8] + t * dS;
+    dS = c[7] + t * dS;
+    dS *= inv_rc;
+    const R inv_4pi_r = R(0.079577471545947673) / r;
+    const R inv_4pi_r2 = inv_4pi_r / r;
+    L = S * inv_4pi_r;
+    negdLdr = S * inv_4pi_r2 - dS * inv_4pi_r;
+    }
+
+    } // namespace dev
+    } // namespace esp

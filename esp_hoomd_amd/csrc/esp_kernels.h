@@ -1,0 +1,95 @@
+// Kernel-launcher layer: host-callable functions returning hipError_t, one per device stage
+// (the role of ESPForceComputeGPU.cuh:49-144 in the reference).  Every launcher enqueues on the
+// given stream and performs no synchronisation.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "esp_types.h"
+
+namespace esp
+{
+namespace kernel
+{
+
+// ---- binning / sorting -----------------------------------------------------
+hipError_t bin_particles(unsigned int n,
+                         const unsigned int* group,
+                         const double* postype,
+                         const double* charge,
+                         const MeshGeom& g,
+                         int P,
+                         unsigned int* tile_count,
+                         unsigned int* key,
+                         unsigned int* slot,
+                         double* qpart,
+                         hipStream_t s);
+hipError_t scan_tiles(unsigned int n_tiles, unsigned int* count, unsigned int* start, hipStream_t s);
+template<typename R>
+hipError_t scatter_particles(unsigned int n,
+                             const unsigned int* group,
+                             const double* postype,
+                             const double* charge,
+                             const MeshGeom& g,
+                             int P,
+                             const unsigned int* key,
+                             const unsigned int* slot,
+                             const unsigned int* start,
+                             void* rec,
+                             SortedIdx* sidx,
+                             hipStream_t s);
+
+// ---- spread / gather ------------------------------------------------------
+size_t spread_lds_bytes(const MeshGeom& g, int P, int D, bool fp64);
+template<typename R>
+hipError_t spread(const MeshGeom& g, int P, int D, unsigned int n_tiles, const unsigned int* start, const void* rec,
+                  const SortedIdx* sidx, const void* wcoef, void* mesh, hipStream_t s);
+template<typename R>
+hipError_t gather(const MeshGeom& g, int P, int D, unsigned int n_tiles, const unsigned int* start, const void* rec,
+                  const SortedIdx* sidx, const void* wcoef, const void* phi, double* force, hipStream_t s);
+
+// ---- FFT passes -------------------------------------------------------------
+int fft_x_pairs_per_block(int nx, bool fp64);
+int fft_cols_per_block(int n, bool fp64);
+unsigned int fft_z_blocks(const MeshGeom& g, bool fp64);
+template<typename R>
+hipError_t fft_forward_x(const MeshGeom& g, const FFTPlan& px, const void* tw, void* mesh, void* spec, hipStream_t s);
+template<typename R>
+hipError_t fft_inverse_x(const MeshGeom& g, const FFTPlan& px, const void* tw, const void* spec, double scale, void* phi, hipStream_t s);
+template<typename R> hipError_t fft_y(const MeshGeom& g, const FFTPlan& py, const void* tw, void* spec, int sign, hipStream_t s);
+template<typename R>
+hipError_t fft_z_convolve(const MeshGeom& g, const FFTPlan& pz, const void* tw, void* spec, const double* G, const double* Gv,
+                          int want_ev, double* ev_part, hipStream_t s);
+
+// ---- influence function and reductions ------------------------------------
+hipError_t build_influence(const MeshGeom& g, int split_kind, double kappa, double rc, int nq, const double* qx, const double* qw,
+                           const double* w2x, const double* w2y, const double* w2z, double* G, double* Gv, hipStream_t s);
+hipError_t finalize(int n_ev, const double* ev_part, int n_q, const double* qpart, double inv_2V, double self_coeff, double bg_coeff,
+                    double V, int want_ev, double* out, double* qsum, hipStream_t s);
+
+// ---- short-range pair correction (EvaluatorPairPSWF over a full neighbour list) ----
+struct PairArgs
+    {
+    unsigned int N;
+    const double* postype;
+    const double* charge;
+    double Lx, Ly, Lz;
+    const unsigned int* n_neigh;
+    const unsigned int* nlist;
+    const uint64_t* head_list;
+    const unsigned int* n_ex;
+    const unsigned int* ex_list;
+    unsigned int ex_pitch;
+    const void* split_tab;   // [n_segs][12] reals
+    unsigned int n_segs;
+    double rc;
+    double* force;           // Scalar4[N], overwritten
+    double* virial;          // [6][pitch], overwritten (may be null)
+    size_t virial_pitch;
+    };
+template<typename R> hipError_t pair_forces(const PairArgs& a, int lanes_per_particle, hipStream_t s);
+
+    } // namespace kernel
+    } // namespace esp

@@ -1,0 +1,476 @@
+// Host orchestration of one ESP solve; see esp_solver.h.
+#include "esp_solver.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <sstream>
+
+#include "esp_kernels.h"
+
+namespace esp
+{
+
+void check_hip(hipError_t e, const char* what)
+    {
+    if (e != hipSuccess)
+        {
+        std::ostringstream oss;
+        oss << what << ": " << hipGetErrorString(e);
+        throw EspError(e == hipErrorOutOfMemory ? STATUS_OOM : STATUS_HIP, oss.str());
+        }
+    }
+
+DeviceBuffer::~DeviceBuffer()
+    {
+    release();
+    }
+
+void DeviceBuffer::release()
+    {
+    if (m_ptr)
+        (void)hipFree(m_ptr);
+    m_ptr = nullptr;
+    m_bytes = 0;
+    }
+
+void DeviceBuffer::resize(size_t bytes, bool zero)
+    {
+    if (bytes > m_bytes)
+        {
+        release();
+        if (bytes > 0)
+            {
+            check_hip(hipMalloc(&m_ptr, bytes), "hipMalloc");
+            m_bytes = bytes;
+            }
+        if (zero && m_ptr)
+            check_hip(hipMemset(m_ptr, 0, m_bytes), "hipMemset");
+        }
+    else if (zero && m_ptr)
+        check_hip(hipMemset(m_ptr, 0, m_bytes), "hipMemset");
+    }
+
+FFTPlan make_fft_plan(int n)
+    {
+    FFTPlan p {};
+    p.n = n;
+    p.nstages = 0;
+    int m = n;
+    if (n < 1)
+        throw EspError(STATUS_ARG, "mesh dimension must be >= 1");
+    while (m % 4 == 0 && p.nstages < MAX_FFT_STAGES)
+        {
+        p.radix[p.nstages++] = 4;
+        m /= 4;
+        }
+    const int rads[3] = {2, 3, 5};
+    for (int r : rads)
+        while (m % r == 0 && p.nstages < MAX_FFT_STAGES)
+            {
+            p.radix[p.nstages++] = r;
+            m /= r;
+            }
+    if (m != 1)
+        {
+        std::ostringstream oss;
+        oss << "mesh dimension " << n << " has a prime factor other than 2, 3, 5";
+        throw EspError(STATUS_UNSUPPORTED, oss.str());
+        }
+    return p;
+    }
+
+EspSolver::EspSolver() = default;
+EspSolver::~EspSolver() = default;
+
+void EspSolver::set_params(const SolverParams& p)
+    {
+    if (p.nx < 2 || p.ny < 2 || p.nz < 2)
+        throw EspError(STATUS_ARG, "mesh resolution must be >= 2 along each axis");
+    if (p.nx > 1024 || p.ny > 1024 || p.nz > 1024)
+        throw EspError(STATUS_ARG, "mesh resolution must be <= 1024 along each axis");
+    if (p.order < 2 || p.order > unsigned(MAX_ORDER))
+        throw EspError(STATUS_ARG, "order must be in [2, 16]");
+    if (!(p.rcut > 0.0))
+        throw EspError(STATUS_ARG, "rcut must be > 0");
+    if (!(p.kappa > 0.0))
+        throw EspError(STATUS_ARG, "kappa must be > 0");
+    if (p.alpha != 0.0)
+        throw EspError(STATUS_UNSUPPORTED, "Debye screening alpha != 0 is not supported");
+    if (p.n_table < 16 || p.n_table > (1u << 20))
+        throw EspError(STATUS_ARG, "n_table must be in [16, 2^20]");
+    if (p.precision != 0 && p.precision != 1)
+        throw EspError(STATUS_ARG, "precision must be 0 (FP64) or 1 (mixed)");
+    if (p.split != 0 && p.split != 1)
+        throw EspError(STATUS_ARG, "split must be 0 (PSWF) or 1 (Gaussian)");
+    if (p.split == 0 && p.kappa * p.rcut > 200.0)
+        throw EspError(STATUS_ARG, "PSWF bandwidth kappa * rcut must be <= 200");
+    if (p.window_degree != 0 && (p.window_degree < 2 || p.window_degree > unsigned(MAX_WINDOW_DEG)))
+        throw EspError(STATUS_ARG, "window_degree must be 0 or in [2, 15]");
+    if (p.pair_lanes != 0 && p.pair_lanes != 1 && p.pair_lanes != 2 && p.pair_lanes != 4 && p.pair_lanes != 8 && p.pair_lanes != 16
+        && p.pair_lanes != 32)
+        throw EspError(STATUS_ARG, "pair_lanes must be 0 or a power of two <= 32");
+    // validate FFT factorisation eagerly
+    make_fft_plan(int(p.nx));
+    make_fft_plan(int(p.ny));
+    make_fft_plan(int(p.nz));
+    m_p = p;
+    m_need_init = true;
+    m_tables_built = false;
+    m_external_table = false;
+    }
+
+void EspSolver::set_split_table(unsigned int n_segs, const double* screen, const double* dscreen, const double* r_lo, const double* dr_inv)
+    {
+    if (n_segs < 1 || !screen || !dscreen || !r_lo || !dr_inv)
+        throw EspError(STATUS_ARG, "set_split_table: invalid table");
+    m_screen.assign(screen, screen + size_t(n_segs) * 5);
+    m_dscreen.assign(dscreen, dscreen + size_t(n_segs) * 5);
+    m_r_lo.assign(r_lo, r_lo + n_segs);
+    m_dr_inv.assign(dr_inv, dr_inv + n_segs);
+    m_external_table = true;
+    m_need_init = true;
+    }
+
+void EspSolver::build_host_tables()
+    {
+    const math::SplitKind kind = m_p.split == 1 ? math::SplitKind::GAUSSIAN : math::SplitKind::PSWF;
+    math::SplitFunction split(kind, m_p.kappa, m_p.rcut);
+    if (!m_external_table)
+        math::build_split_table(split, m_p.n_table, m_screen, m_dscreen, m_r_lo, m_dr_inv);
+    m_self_coeff = split.self_coeff();
+    m_bg_coeff = split.background_coeff();
+
+    // window: oversampling sigma = (pi / h) / K, K the effective band limit of F
+    const unsigned int P = m_p.order;
+    m_D = m_p.window_degree ? m_p.window_degree : math::default_window_degree(P);
+    double cw = m_p.window_c;
+    if (cw <= 0.0)
+        {
+        // mesh spacing is only known with the box; use the finest axis of the last box if any
+        const double Lmax = std::max({m_box[0] / m_p.nx, m_box[1] / m_p.ny, m_box[2] / m_p.nz});
+        const double h = Lmax > 0 ? Lmax : 1.0;
+        const double K = (kind == math::SplitKind::PSWF) ? m_p.kappa : 2.0 * m_p.kappa * 3.7;
+        const double sigma = math::PI / (h * K);
+        cw = math::default_window_c(P, sigma);
+        }
+    m_window = math::build_window(P, m_D, cw);
+    m_tables_built = true;
+    }
+
+size_t EspSolver::memory_bytes() const
+    {
+    size_t b = 0;
+    const DeviceBuffer* bufs[] = {&m_split_tab, &m_wcoef, &m_tw[0], &m_tw[1], &m_tw[2], &m_G, &m_Gv, &m_qxd, &m_qwd, &m_w2[0], &m_w2[1],
+                                  &m_w2[2], &m_mesh, &m_spec, &m_phi, &m_tile_count, &m_tile_start, &m_key, &m_slot, &m_rec,
+                                  &m_sidx, &m_qpart, &m_evpart, &m_qsum, &m_ev_internal};
+    for (const DeviceBuffer* d : bufs)
+        b += d->bytes();
+    return b;
+    }
+
+template<typename R> static void upload_as(DeviceBuffer& buf, const std::vector<double>& v)
+    {
+    std::vector<R> tmp(v.begin(), v.end());
+    buf.resize(tmp.size() * sizeof(R));
+    check_hip(hipMemcpy(buf.get(), tmp.data(), tmp.size() * sizeof(R), hipMemcpyHostToDevice), "hipMemcpy tables");
+    }
+
+void EspSolver::initialize(const SystemView& sys, hipStream_t s)
+    {
+    if (sys.box[3] != 0.0 || sys.box[4] != 0.0 || sys.box[5] != 0.0)
+        throw EspError(STATUS_UNSUPPORTED, "triclinic boxes (non-zero tilt) are not supported");
+    if (!(sys.box[0] > 0 && sys.box[1] > 0 && sys.box[2] > 0))
+        throw EspError(STATUS_ARG, "box lengths must be > 0");
+    m_box[0] = sys.box[0];
+    m_box[1] = sys.box[1];
+    m_box[2] = sys.box[2];
+    const bool fp64 = m_p.precision == 0;
+    const unsigned int P = m_p.order;
+
+    build_host_tables();
+
+    // mesh geometry and tiles: 8-cell tiles unless the LDS box would exceed 64 KiB or the
+    // mesh is small, then 4
+    MeshGeom& g = m_geom;
+    g.nx = int(m_p.nx);
+    g.ny = int(m_p.ny);
+    g.nz = int(m_p.nz);
+    g.nxh = g.nx / 2 + 1;
+    int T = 8;
+    const size_t el = fp64 ? 8 : 4;
+    if (size_t(T + P - 1) * (T + P - 1) * (T + P - 1) * el > 64 * 1024)
+        T = 4;
+    if (T == 8 && size_t((g.nx + 7) / 8) * ((g.ny + 7) / 8) * ((g.nz + 7) / 8) < 256)
+        T = 4;
+    g.T = T;
+    g.tx = (g.nx + T - 1) / T;
+    g.ty = (g.ny + T - 1) / T;
+    g.tz = (g.nz + T - 1) / T;
+    g.B = T + int(P) - 1;
+    g.Lx = m_box[0];
+    g.Ly = m_box[1];
+    g.Lz = m_box[2];
+    g.hx = g.Lx / g.nx;
+    g.hy = g.Ly / g.ny;
+    g.hz = g.Lz / g.nz;
+    m_n_tiles = unsigned(g.tx) * g.ty * g.tz;
+    if (m_n_tiles > (1u << 24))
+        throw EspError(STATUS_ARG, "too many spreading tiles");
+    m_plan[0] = make_fft_plan(g.nx);
+    m_plan[1] = make_fft_plan(g.ny);
+    m_plan[2] = make_fft_plan(g.nz);
+
+    // split table records [r_lo, dr_inv, S c0..c4, dS c0..c4]
+    const unsigned int nseg = unsigned(m_r_lo.size());
+    std::vector<double> rec(size_t(nseg) * 12);
+    for (unsigned int i = 0; i < nseg; ++i)
+        {
+        rec[size_t(i) * 12 + 0] = m_r_lo[i];
+        rec[size_t(i) * 12 + 1] = m_dr_inv[i];
+        for (int c = 0; c < 5; ++c)
+            {
+            rec[size_t(i) * 12 + 2 + c] = m_screen[size_t(i) * 5 + c];
+            rec[size_t(i) * 12 + 7 + c] = m_dscreen[size_t(i) * 5 + c];
+            }
+        }
+    if (fp64)
+        upload_as<double>(m_split_tab, rec);
+    else
+        upload_as<float>(m_split_tab, rec);
+    if (fp64)
+        upload_as<double>(m_wcoef, m_window.coef);
+    else
+        upload_as<float>(m_wcoef, m_window.coef);
+
+    // twiddles exp(-2 pi i m / n)
+    const int dims[3] = {g.nx, g.ny, g.nz};
+    for (int a = 0; a < 3; ++a)
+        {
+        std::vector<double> tw(size_t(dims[a]) * 2);
+        for (int m = 0; m < dims[a]; ++m)
+            {
+            const double ang = -2.0 * math::PI * double(m) / double(dims[a]);
+            tw[2 * m] = std::cos(ang);
+            tw[2 * m + 1] = std::sin(ang);
+            }
+        if (fp64)
+            upload_as<double>(m_tw[a], tw);
+        else
+            upload_as<float>(m_tw[a], tw);
+        }
+
+    // device buffers
+    const size_t M = size_t(g.nx) * g.ny * g.nz;
+    const size_t Mh = size_t(g.nxh) * g.ny * g.nz;
+    m_mesh.resize(M * 8, true);
+    m_spec.resize(Mh * 2 * (fp64 ? 8 : 4));
+    m_phi.resize(M * (fp64 ? 8 : 4));
+    m_G.resize(Mh * 8);
+    m_Gv.resize(Mh * 8);
+    m_tile_count.resize((size_t(m_n_tiles) + 1) * 4, true);
+    m_tile_start.resize((size_t(m_n_tiles) + 1) * 4, true);
+    m_evpart.resize(size_t(kernel::fft_z_blocks(g, fp64)) * 8 * 8, true);
+    m_qsum.resize(2 * 8, true);
+    m_ev_internal.resize(8 * 8, true);
+    m_cap_particles = 0;
+
+    build_influence(s);
+    m_need_init = false;
+    }
+
+void EspSolver::build_influence(hipStream_t s)
+    {
+    const MeshGeom& g = m_geom;
+    const math::SplitKind kind = m_p.split == 1 ? math::SplitKind::GAUSSIAN : math::SplitKind::PSWF;
+    // |w(theta)|^2 per axis
+    std::vector<double> w2x(g.nxh), w2y(g.ny), w2z(g.nz);
+    for (int x = 0; x < g.nxh; ++x)
+        {
+        const double f = m_window.fourier(2.0 * math::PI * x / g.nx);
+        w2x[x] = f * f;
+        }
+    for (int y = 0; y < g.ny; ++y)
+        {
+        const int k = (y <= g.ny / 2) ? y : y - g.ny;
+        const double f = m_window.fourier(2.0 * math::PI * k / g.ny);
+        w2y[y] = f * f;
+        }
+    for (int z = 0; z < g.nz; ++z)
+        {
+        const int k = (z <= g.nz / 2) ? z : z - g.nz;
+        const double f = m_window.fourier(2.0 * math::PI * k / g.nz);
+        w2z[z] = f * f;
+        }
+    upload_as<double>(m_w2[0], w2x);
+    upload_as<double>(m_w2[1], w2y);
+    upload_as<double>(m_w2[2], w2z);
+    int nq = 0;
+    if (kind == math::SplitKind::PSWF)
+        {
+        math::SplitFunction split(kind, m_p.kappa, m_p.rcut);
+        const double kmax = math::PI * std::sqrt(1.0 / (g.hx * g.hx) + 1.0 / (g.hy * g.hy) + 1.0 / (g.hz * g.hz));
+        nq = math::SplitFunction::quadrature_nodes(m_p.kappa * m_p.rcut, kmax * m_p.rcut);
+        split.quadrature(nq, m_qx, m_qw);
+        upload_as<double>(m_qxd, m_qx);
+        upload_as<double>(m_qwd, m_qw);
+        }
+    else
+        {
+        m_qxd.resize(8);
+        m_qwd.resize(8);
+        }
+    check_hip(kernel::build_influence(g, m_p.split, m_p.kappa, m_p.rcut, nq, m_qxd.as<double>(), m_qwd.as<double>(), m_w2[0].as<double>(),
+                                      m_w2[1].as<double>(), m_w2[2].as<double>(), m_G.as<double>(), m_Gv.as<double>(), s),
+              "build_influence");
+    }
+
+void EspSolver::ensure_particles(unsigned int n)
+    {
+    if (n <= m_cap_particles)
+        return;
+    const unsigned int cap = std::max(n, m_cap_particles + m_cap_particles / 2);
+    const bool fp64 = m_p.precision == 0;
+    m_key.resize(size_t(cap) * 4);
+    m_slot.resize(size_t(cap) * 4);
+    m_rec.resize(size_t(cap) * 4 * (fp64 ? 8 : 4));
+    m_sidx.resize(size_t(cap) * sizeof(SortedIdx));
+    m_qpart.resize(size_t((cap + 255) / 256 + 1) * 2 * 8);
+    m_cap_particles = cap;
+    }
+
+template<typename R> void EspSolver::run_pair(const SystemView& sys, const OutputView& out, hipStream_t s)
+    {
+    kernel::PairArgs a {};
+    a.N = sys.N;
+    a.postype = sys.postype;
+    a.charge = sys.charge;
+    a.Lx = sys.box[0];
+    a.Ly = sys.box[1];
+    a.Lz = sys.box[2];
+    a.n_neigh = sys.n_neigh;
+    a.nlist = sys.nlist;
+    a.head_list = sys.head_list;
+    a.n_ex = sys.n_ex;
+    a.ex_list = sys.ex_list;
+    a.ex_pitch = sys.ex_pitch;
+    a.split_tab = m_split_tab.get();
+    a.n_segs = unsigned(m_r_lo.size());
+    a.rc = m_p.rcut;
+    a.force = out.force;
+    a.virial = out.virial;
+    a.virial_pitch = out.virial_pitch;
+    int lanes = m_p.pair_lanes ? m_p.pair_lanes : 8;
+    check_hip(kernel::pair_forces<R>(a, lanes, s), "pair_forces");
+    }
+
+template<typename R>
+void EspSolver::run_mesh(const SystemView& sys, const OutputView& out, unsigned int n, hipStream_t s, bool want_ev)
+    {
+    const MeshGeom& g = m_geom;
+    const int P = int(m_p.order);
+    const int D = int(m_D);
+    const unsigned int nqb = (n + 255) / 256;
+    check_hip(kernel::bin_particles(n, sys.group_idx, sys.postype, sys.charge, g, P, m_tile_count.as<unsigned int>(),
+                                    m_key.as<unsigned int>(), m_slot.as<unsigned int>(), m_qpart.as<double>(), s),
+              "bin_particles");
+    check_hip(kernel::scan_tiles(m_n_tiles, m_tile_count.as<unsigned int>(), m_tile_start.as<unsigned int>(), s), "scan_tiles");
+    check_hip(kernel::scatter_particles<R>(n, sys.group_idx, sys.postype, sys.charge, g, P, m_key.as<unsigned int>(),
+                                           m_slot.as<unsigned int>(), m_tile_start.as<unsigned int>(), m_rec.get(),
+                                           m_sidx.as<SortedIdx>(), s),
+              "scatter_particles");
+    check_hip(kernel::spread<R>(g, P, D, m_n_tiles, m_tile_start.as<unsigned int>(), m_rec.get(), m_sidx.as<SortedIdx>(), m_wcoef.get(),
+                                m_mesh.get(), s),
+              "spread");
+    check_hip(kernel::fft_forward_x<R>(g, m_plan[0], m_tw[0].get(), m_mesh.get(), m_spec.get(), s), "fft_forward_x");
+    check_hip(kernel::fft_y<R>(g, m_plan[1], m_tw[1].get(), m_spec.get(), -1, s), "fft_y");
+    check_hip(kernel::fft_z_convolve<R>(g, m_plan[2], m_tw[2].get(), m_spec.get(), m_G.as<double>(), m_Gv.as<double>(), want_ev ? 1 : 0,
+                                        m_evpart.as<double>(), s),
+              "fft_z_convolve");
+    check_hip(kernel::fft_y<R>(g, m_plan[1], m_tw[1].get(), m_spec.get(), +1, s), "fft_y inverse");
+    const double V = g.Lx * g.Ly * g.Lz;
+    check_hip(kernel::fft_inverse_x<R>(g, m_plan[0], m_tw[0].get(), m_spec.get(), 1.0 / V, m_phi.get(), s), "fft_inverse_x");
+    check_hip(kernel::gather<R>(g, P, D, m_n_tiles, m_tile_start.as<unsigned int>(), m_rec.get(), m_sidx.as<SortedIdx>(), m_wcoef.get(),
+                                m_phi.get(), out.force, s),
+              "gather");
+    double* evout = out.ext_energy_virial ? out.ext_energy_virial : m_ev_internal.as<double>();
+    check_hip(kernel::finalize(int(kernel::fft_z_blocks(g, sizeof(R) == 8)), m_evpart.as<double>(), int(nqb), m_qpart.as<double>(),
+                               0.5 / V, m_self_coeff, m_bg_coeff, V, want_ev ? 1 : 0, evout, m_qsum.as<double>(), s),
+              "finalize");
+    }
+
+void EspSolver::compute(const SystemView& sys, const OutputView& out, hipStream_t s)
+    {
+    if (!sys.postype || !sys.charge)
+        throw EspError(STATUS_ARG, "postype and charge must be device pointers");
+    if (!out.force)
+        throw EspError(STATUS_ARG, "force output must be a device pointer");
+    if (out.virial && out.virial_pitch < sys.N)
+        throw EspError(STATUS_ARG, "virial_pitch must be >= N");
+    if (sys.nlist && (!sys.n_neigh || !sys.head_list))
+        throw EspError(STATUS_ARG, "neighbour list needs n_neigh, nlist and head_list");
+    if (sys.n_ex && (!sys.ex_list || sys.ex_pitch == 0))
+        throw EspError(STATUS_ARG, "exclusions need ex_list and ex_pitch");
+    if (sys.group_idx && sys.group_size > sys.N)
+        throw EspError(STATUS_ARG, "group_size exceeds N");
+    if (m_need_init || sys.box[0] != m_box[0] || sys.box[1] != m_box[1] || sys.box[2] != m_box[2])
+        {
+        if (m_need_init)
+            initialize(sys, s);
+        else
+            {
+            if (sys.box[3] != 0.0 || sys.box[4] != 0.0 || sys.box[5] != 0.0)
+                throw EspError(STATUS_UNSUPPORTED, "triclinic boxes (non-zero tilt) are not supported");
+            m_box[0] = sys.box[0];
+            m_box[1] = sys.box[1];
+            m_box[2] = sys.box[2];
+            m_geom.Lx = m_box[0];
+            m_geom.Ly = m_box[1];
+            m_geom.Lz = m_box[2];
+            m_geom.hx = m_geom.Lx / m_geom.nx;
+            m_geom.hy = m_geom.Ly / m_geom.ny;
+            m_geom.hz = m_geom.Lz / m_geom.nz;
+            build_influence(s);
+            }
+        }
+    const unsigned int n = sys.group_idx ? sys.group_size : sys.N;
+    ensure_particles(std::max(n, 1u));
+    const bool fp64 = m_p.precision == 0;
+    const unsigned int flags = out.flags ? out.flags : COMPUTE_ALL;
+    if (flags & COMPUTE_PAIR)
+        {
+        if (fp64)
+            run_pair<double>(sys, out, s);
+        else
+            run_pair<float>(sys, out, s);
+        }
+    else
+        {
+        check_hip(hipMemsetAsync(out.force, 0, size_t(sys.N) * 32, s), "hipMemsetAsync force");
+        if (out.virial)
+            for (int c = 0; c < 6; ++c)
+                check_hip(hipMemsetAsync(out.virial + size_t(c) * out.virial_pitch, 0, size_t(sys.N) * 8, s), "hipMemsetAsync virial");
+        }
+    if (flags & COMPUTE_MESH)
+        {
+        const bool want_ev = (flags & COMPUTE_ENERGY_VIRIAL) != 0;
+        if (fp64)
+            run_mesh<double>(sys, out, n, s, want_ev);
+        else
+            run_mesh<float>(sys, out, n, s, want_ev);
+        }
+    }
+
+void EspSolver::charge_sums(double& q, double& q2)
+    {
+    double h[2] = {0, 0};
+    if (m_qsum.get())
+        {
+        check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        check_hip(hipMemcpy(h, m_qsum.get(), 16, hipMemcpyDeviceToHost), "hipMemcpy qsum");
+        }
+    q = h[0];
+    q2 = h[1];
+    }
+
+    } // namespace esp

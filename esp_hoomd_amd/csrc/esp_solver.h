@@ -1,0 +1,190 @@
+// Host orchestration of one ESP solve on one GPU: parameters, tables, device buffers, FFT
+// plans and the per-step launch sequence.  Mirrors the stage order of
+// ESPForceCompute::computeForces (ESPForceCompute.cc:96-127): one-time initialisation
+// (setupMesh, initializeFFT, setupCoeffs/influence, tables), then per step the pair
+// correction, assignParticles, forward FFT, influence + energy/virial, inverse FFT,
+// interpolateForces and the final reductions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "esp_types.h"
+#include "pswf_math.h"
+
+namespace esp
+{
+
+//! Error carrying a C-ABI status code.
+class EspError : public std::runtime_error
+    {
+    public:
+    EspError(int code, const std::string& msg) : std::runtime_error(msg), m_code(code) { }
+    int code() const { return m_code; }
+
+    private:
+    int m_code;
+    };
+
+enum Status
+    {
+    STATUS_OK = 0,
+    STATUS_ARG = -1,
+    STATUS_HIP = -2,
+    STATUS_OOM = -3,
+    STATUS_STATE = -4,
+    STATUS_RANGE = -5,
+    STATUS_UNSUPPORTED = -6
+    };
+
+void check_hip(hipError_t e, const char* what);
+
+//! RAII device allocation.
+class DeviceBuffer
+    {
+    public:
+    DeviceBuffer() = default;
+    ~DeviceBuffer();
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    void resize(size_t bytes, bool zero = false);  //!< reallocates only when growing
+    void* get() const { return m_ptr; }
+    template<typename T> T* as() const { return static_cast<T*>(m_ptr); }
+    size_t bytes() const { return m_bytes; }
+    void release();
+
+    private:
+    void* m_ptr = nullptr;
+    size_t m_bytes = 0;
+    };
+
+struct SolverParams
+    {
+    unsigned int nx = 32, ny = 32, nz = 32;
+    unsigned int order = 6;
+    double kappa = 0.0;
+    double rcut = 0.0;
+    double alpha = 0.0;
+    unsigned int n_table = 4096;
+    int precision = 0;       // 0 FP64, 1 mixed
+    int split = 0;           // 0 PSWF, 1 Gaussian
+    double window_c = 0.0;   // 0 = automatic
+    unsigned int window_degree = 0;  // 0 = automatic
+    int pair_lanes = 0;      // lanes per particle in the pair kernel (0 = automatic)
+    };
+
+//! Device-side description of one system for one call (caller-owned device pointers).
+struct SystemView
+    {
+    unsigned int N = 0;
+    const double* postype = nullptr;
+    const double* charge = nullptr;
+    unsigned int group_size = 0;
+    const unsigned int* group_idx = nullptr;
+    double box[6] = {0, 0, 0, 0, 0, 0};
+    const unsigned int* n_neigh = nullptr;
+    const unsigned int* nlist = nullptr;
+    const uint64_t* head_list = nullptr;
+    const unsigned int* n_ex = nullptr;
+    const unsigned int* ex_list = nullptr;
+    unsigned int ex_pitch = 0;
+    };
+
+struct OutputView
+    {
+    double* force = nullptr;
+    double* virial = nullptr;
+    size_t virial_pitch = 0;
+    double* ext_energy_virial = nullptr;  // device double[7] or null
+    unsigned int flags = 0;
+    };
+
+enum ComputeFlags : unsigned int
+    {
+    COMPUTE_PAIR = 1u,
+    COMPUTE_MESH = 2u,
+    COMPUTE_ENERGY_VIRIAL = 4u,
+    COMPUTE_ALL = 7u
+    };
+
+class EspSolver
+    {
+    public:
+    EspSolver();
+    ~EspSolver();
+
+    void set_params(const SolverParams& p);
+    const SolverParams& params() const { return m_p; }
+    void invalidate() { m_need_init = true; }
+
+    //! Full ESP evaluation: pair correction + mesh; outputs overwritten.
+    void compute(const SystemView& sys, const OutputView& out, hipStream_t s);
+
+    //! Host-side tables (built at initialisation).
+    const std::vector<double>& split_screen() const { return m_screen; }
+    const std::vector<double>& split_dscreen() const { return m_dscreen; }
+    const std::vector<double>& split_r_lo() const { return m_r_lo; }
+    const std::vector<double>& split_dr_inv() const { return m_dr_inv; }
+    const math::WindowTable& window() const { return m_window; }
+    //! Replace the split table by an external one in the PSWF_Coeffs.h format (reduced s).
+    void set_split_table(unsigned int n_segs, const double* screen, const double* dscreen, const double* r_lo, const double* dr_inv);
+    //! Build host tables now (no device work); used by introspection before the first step.
+    void build_host_tables();
+    double self_coeff() const { return m_self_coeff; }
+    double background_coeff() const { return m_bg_coeff; }
+
+    //! Charge sums of the last step (synchronises with the device).
+    void charge_sums(double& q, double& q2);
+    //! Tile edge and tile count chosen for the mesh.
+    const MeshGeom& geometry() const { return m_geom; }
+    //! Device pointers of the influence tables (half spectrum), for tests.
+    const double* device_green() const { return m_G.as<double>(); }
+    size_t memory_bytes() const;
+
+    private:
+    void initialize(const SystemView& sys, hipStream_t s);
+    void build_influence(hipStream_t s);
+    void ensure_particles(unsigned int n);
+    template<typename R> void run_mesh(const SystemView& sys, const OutputView& out, unsigned int n, hipStream_t s, bool want_ev);
+    template<typename R> void run_pair(const SystemView& sys, const OutputView& out, hipStream_t s);
+
+    SolverParams m_p;
+    bool m_need_init = true;
+    bool m_tables_built = false;
+    bool m_external_table = false;
+    double m_box[3] = {0, 0, 0};
+    MeshGeom m_geom {};
+    FFTPlan m_plan[3] {};
+    unsigned int m_n_tiles = 0;
+    unsigned int m_D = 0;
+    double m_self_coeff = 0.0;
+    double m_bg_coeff = 0.0;
+
+    std::vector<double> m_screen, m_dscreen, m_r_lo, m_dr_inv;
+    math::WindowTable m_window;
+    std::vector<double> m_qx, m_qw;
+
+    // device
+    DeviceBuffer m_split_tab;   // [n_segs][12] R
+    DeviceBuffer m_wcoef;       // [P][D+1] R
+    DeviceBuffer m_tw[3];       // twiddles per axis, complex R
+    DeviceBuffer m_G, m_Gv;     // half spectrum FP64
+    DeviceBuffer m_qxd, m_qwd, m_w2[3];
+    DeviceBuffer m_mesh;        // spread accumulator (FP64 or int64 fixed point)
+    DeviceBuffer m_spec;        // half spectrum complex R
+    DeviceBuffer m_phi;         // potential mesh R
+    DeviceBuffer m_tile_count, m_tile_start;
+    DeviceBuffer m_key, m_slot, m_rec, m_sidx;
+    DeviceBuffer m_qpart, m_evpart, m_qsum, m_ev_internal;
+    unsigned int m_cap_particles = 0;
+    };
+
+//! Mixed-radix plan for length n using radices 4, 2, 3, 5; throws for other factors.
+FFTPlan make_fft_plan(int n);
+
+    } // namespace esp

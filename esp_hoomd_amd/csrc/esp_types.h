@@ -1,0 +1,49 @@
+// POD descriptors shared by the host orchestration and the HIP kernels.
+#pragma once
+
+#include <stdint.h>
+
+namespace esp
+{
+
+constexpr int MAX_FFT_STAGES = 16;
+constexpr int MAX_ORDER = 16;      // largest stencil order P with a kernel instantiation
+constexpr int MAX_WINDOW_DEG = 15; // largest window polynomial degree
+
+//! Mixed-radix (4, 2, 3, 5) Stockham plan for one 1-D length.
+struct FFTPlan
+    {
+    int n;
+    int nstages;
+    int radix[MAX_FFT_STAGES];
+    };
+
+//! Geometry of one periodic system and its mesh.
+struct MeshGeom
+    {
+    int nx, ny, nz;     // mesh points per axis
+    int nxh;            // nx / 2 + 1 (half spectrum along x)
+    int T;              // spreading tile edge (cells)
+    int tx, ty, tz;     // tiles per axis
+    int B;              // LDS box edge T + P - 1
+    double Lx, Ly, Lz;  // orthorhombic box lengths (box centred on the origin)
+    double hx, hy, hz;  // mesh spacing
+    };
+
+//! Particle record in tile-sorted order (one per charged group member).
+template<typename R> struct SortedRec
+    {
+    R tx, ty, tz, q;  // stencil fractional offsets t in [0,1) and charge
+    };
+
+struct SortedIdx
+    {
+    uint32_t local;   // origin within the tile: ox | oy << 8 | oz << 16
+    uint32_t idx;     // particle index into postype / force
+    };
+
+//! Fixed-point scale for the deterministic mixed-precision mesh accumulation.
+constexpr double FIXED_SCALE = 8388608.0;  // 2^23
+constexpr double FIXED_INV_SCALE = 1.0 / 8388608.0;
+
+    } // namespace esp

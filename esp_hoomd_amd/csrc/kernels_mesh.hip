@@ -1,0 +1,1014 @@
+// ESP mesh kernels for gfx950: tile binning, LDS-tiled PSWF spreading, Stockham FFT passes with
+// the influence multiply fused into the z pass, the mirrored gather, influence tabulation and
+// the FP64 energy/virial reduction.
+//
+// Reference counterparts: gpu_esp_assign_particles_kernel (ESPForceComputeGPU.cu:206-272),
+// hipfftExecC2C (ESPForceComputeGPU.cc:254-285), gpu_esp_apply_influence_kernel (.cu:277-287),
+// gpu_esp_interpolate_forces_kernel (.cu:292-373), gpu_esp_build_influence_kernel (.cu:157-201).
+#include "esp_device.h"
+#include "esp_kernels.h"
+
+namespace esp
+{
+namespace dev
+{
+
+//! Accumulator types: FP64 mode accumulates the mesh in FP64; the mixed mode accumulates it
+//! in fixed point (int32 in LDS, int64 in HBM), which is exact and order independent.
+template<typename R> struct Acc;
+template<> struct Acc<double>
+    {
+    using lds_t = double;
+    using glob_t = double;
+    };
+template<> struct Acc<float>
+    {
+    using lds_t = int;
+    using glob_t = unsigned long long;
+    };
+
+__device__ __forceinline__ void lds_add(double* p, double v)
+    {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+__device__ __forceinline__ void lds_add(int* p, float v)
+    {
+    __hip_atomic_fetch_add(p, __float2int_rn(v * float(FIXED_SCALE)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+__device__ __forceinline__ void glob_add(double* p, double v)
+    {
+    unsafeAtomicAdd(p, v);
+    }
+__device__ __forceinline__ void glob_add(unsigned long long* p, int v)
+    {
+    atomicAdd(p, static_cast<unsigned long long>(static_cast<long long>(v)));
+    }
+__device__ __forceinline__ double mesh_load_clear(double* p)
+    {
+    const double v = *p;
+    *p = 0.0;
+    return v;
+    }
+__device__ __forceinline__ float mesh_load_clear(unsigned long long* p)
+    {
+    const long long v = static_cast<long long>(*p);
+    *p = 0ull;
+    return static_cast<float>(static_cast<double>(v) * FIXED_INV_SCALE);
+    }
+
+//! Stencil origin o = ceil(g - P/2) (wrapped into [0, n)) and offset t = o - (g - P/2) in [0, 1)
+//! for one axis; g = makeFraction(x) * n for a box centred on the origin.
+__device__ __forceinline__ int stencil_origin(double x, double L, int n, int P, double& t)
+    {
+    double frac = x / L + 0.5;
+    frac -= floor(frac);
+    const double gg = frac * double(n) - 0.5 * double(P);
+    const double o = ceil(gg);
+    t = o - gg;
+    int oi = int(o) % n;
+    if (oi < 0)
+        oi += n;
+    return oi;
+    }
+
+// ---------------------------------------------------------------------------
+// Binning into spreading tiles (counting sort: count, scan, scatter)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_bin(unsigned int n,
+                                             const unsigned int* __restrict__ group,
+                                             const double4* __restrict__ postype,
+                                             const double* __restrict__ charge,
+                                             MeshGeom g,
+                                             int P,
+                                             unsigned int* __restrict__ tile_count,
+                                             unsigned int* __restrict__ key,
+                                             unsigned int* __restrict__ slot,
+                                             double* __restrict__ qpart)
+    {
+    const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double q = 0.0;
+    if (i < n)
+        {
+        const unsigned int p = group ? group[i] : i;
+        const double4 r = postype[p];
+        q = charge[p];
+        unsigned int k = 0xffffffffu;
+        if (q != 0.0)
+            {
+            double t;
+            const int ox = stencil_origin(r.x, g.Lx, g.nx, P, t);
+            const int oy = stencil_origin(r.y, g.Ly, g.ny, P, t);
+            const int oz = stencil_origin(r.z, g.Lz, g.nz, P, t);
+            k = unsigned((oz / g.T) * g.ty + oy / g.T) * unsigned(g.tx) + unsigned(ox / g.T);
+            slot[i] = atomicAdd(&tile_count[k], 1u);
+            }
+        key[i] = k;
+        }
+    // block partial sums of q and q^2 (deterministic order)
+    __shared__ double sq[8], sq2[8];
+    double a = wave_sum(q), b = wave_sum(q * q);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0)
+        {
+        sq[wave] = a;
+        sq2[wave] = b;
+        }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        {
+        double s = 0, s2 = 0;
+        for (int w = 0; w < int(blockDim.x >> 6); ++w)
+            {
+            s += sq[w];
+            s2 += sq2[w];
+            }
+        qpart[2 * blockIdx.x] = s;
+        qpart[2 * blockIdx.x + 1] = s2;
+        }
+    }
+
+//! Exclusive scan of tile counts by one workgroup; clears the counts for the next step.
+__global__ void __launch_bounds__(1024) k_scan(unsigned int n_tiles, unsigned int* __restrict__ count, unsigned int* __restrict__ start)
+    {
+    __shared__ unsigned int part[1024];
+    const unsigned int tid = threadIdx.x;
+    const unsigned int chunk = (n_tiles + 1023u) / 1024u;
+    const unsigned int b0 = tid * chunk;
+    const unsigned int b1 = min(b0 + chunk, n_tiles);
+    unsigned int s = 0;
+    for (unsigned int i = b0; i < b1; ++i)
+        s += count[i];
+    part[tid] = s;
+    __syncthreads();
+    // Hillis-Steele inclusive scan over 1024 partials
+    for (unsigned int off = 1; off < 1024u; off <<= 1)
+        {
+        const unsigned int v = (tid >= off) ? part[tid - off] : 0u;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+        }
+    unsigned int run = part[tid] - s;
+    for (unsigned int i = b0; i < b1; ++i)
+        {
+        const unsigned int c = count[i];
+        start[i] = run;
+        run += c;
+        count[i] = 0u;
+        }
+    if (tid == 1023u)
+        start[n_tiles] = part[1023];
+    }
+
+template<typename R>
+__global__ void __launch_bounds__(256) k_scatter(unsigned int n,
+                                                 const unsigned int* __restrict__ group,
+                                                 const double4* __restrict__ postype,
+                                                 const double* __restrict__ charge,
+                                                 MeshGeom g,
+                                                 int P,
+                                                 const unsigned int* __restrict__ key,
+                                                 const unsigned int* __restrict__ slot,
+                                                 const unsigned int* __restrict__ start,
+                                                 SortedRec<R>* __restrict__ rec,
+                                                 SortedIdx* __restrict__ sidx)
+    {
+    const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const unsigned int k = key[i];
+    if (k == 0xffffffffu)
+        return;
+    const unsigned int p = group ? group[i] : i;
+    const double4 r = postype[p];
+    double tx, ty, tz;
+    const int ox = stencil_origin(r.x, g.Lx, g.nx, P, tx);
+    const int oy = stencil_origin(r.y, g.Ly, g.ny, P, ty);
+    const int oz = stencil_origin(r.z, g.Lz, g.nz, P, tz);
+    const unsigned int pos = start[k] + slot[i];
+    SortedRec<R> o;
+    o.tx = R(tx);
+    o.ty = R(ty);
+    o.tz = R(tz);
+    o.q = R(charge[p]);
+    rec[pos] = o;
+    SortedIdx s;
+    s.local = unsigned(ox % g.T) | (unsigned(oy % g.T) << 8) | (unsigned(oz % g.T) << 16);
+    s.idx = p;
+    sidx[pos] = s;
+    }
+
+// ---------------------------------------------------------------------------
+// Spreading: one workgroup per tile, P^3 stencil accumulated in an LDS box of
+// (T+P-1)^3 cells, then added to the mesh.  One wave per particle; each lane
+// owns (a, b) stencil columns and walks c.
+// ---------------------------------------------------------------------------
+template<typename R, int P>
+__global__ void __launch_bounds__(256) k_spread(MeshGeom g,
+                                                int D,
+                                                const unsigned int* __restrict__ tile_start,
+                                                const SortedRec<R>* __restrict__ rec,
+                                                const SortedIdx* __restrict__ sidx,
+                                                const R* __restrict__ wcoef,
+                                                typename Acc<R>::glob_t* __restrict__ mesh)
+    {
+    using L = typename Acc<R>::lds_t;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int B = g.B;
+    const int B2 = B * B;
+    const int B3 = B2 * B;
+    L* box = reinterpret_cast<L*>(smem);
+    R* wc = reinterpret_cast<R*>(smem + ((size_t(B3) * sizeof(L) + 15) & ~size_t(15)));
+
+    const unsigned int tile = blockIdx.x;
+    const unsigned int s0 = tile_start[tile], s1 = tile_start[tile + 1];
+    if (s0 == s1)
+        return;
+    for (int c = threadIdx.x; c < B3; c += blockDim.x)
+        box[c] = L(0);
+    for (int c = threadIdx.x; c < P * (D + 1); c += blockDim.x)
+        wc[c] = wcoef[c];
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int nw = blockDim.x >> 6;
+    const int axis = lane / P;
+    const int kt = lane - axis * P;
+    for (unsigned int p = s0 + wave; p < s1; p += nw)
+        {
+        const SortedRec<R> r = rec[p];
+        const unsigned int loc = sidx[p].local;
+        const int ox = loc & 0xff, oy = (loc >> 8) & 0xff, oz = loc >> 16;
+        R w = R(0);
+        if (lane < 3 * P)
+            {
+            const R t = axis == 0 ? r.tx : (axis == 1 ? r.ty : r.tz);
+            const R z = R(2) * t - R(1);
+            const R* c = wc + kt * (D + 1);
+            R v = c[D];
+            for (int j = D - 1; j >= 0; --j)
+                v = v * z + c[j];
+            w = v;
+            }
+        R wz[P];
+#pragma unroll
+        for (int c = 0; c < P; ++c)
+            wz[c] = __shfl(w, 2 * P + c, WAVE);
+#pragma unroll
+        for (int it = 0; it < (P * P + 63) / 64; ++it)
+            {
+            const int s = lane + it * 64;
+            const int a = s % P;
+            const int b = s / P;
+            const R wx = __shfl(w, a, WAVE);
+            const R wy = __shfl(w, P + (b < P ? b : 0), WAVE);
+            if (s < P * P)
+                {
+                const R qw = r.q * wx * wy;
+                L* dst = box + (oz * B + oy + b) * B + ox + a;
+#pragma unroll
+                for (int c = 0; c < P; ++c)
+                    lds_add(dst + c * B2, qw * wz[c]);
+                }
+            }
+        }
+    __syncthreads();
+
+    const int txy = g.tx * g.ty;
+    const int tz = int(tile) / txy;
+    const int ty = (int(tile) - tz * txy) / g.tx;
+    const int tx = int(tile) - tz * txy - ty * g.tx;
+    for (int c = threadIdx.x; c < B3; c += blockDim.x)
+        {
+        const L v = box[c];
+        if (v != L(0))
+            {
+            const int lz = c / B2;
+            const int ly = (c - lz * B2) / B;
+            const int lx = c - lz * B2 - ly * B;
+            const int gx = (tx * g.T + lx) % g.nx;
+            const int gy = (ty * g.T + ly) % g.ny;
+            const int gz = (tz * g.T + lz) % g.nz;
+            glob_add(&mesh[(size_t(gz) * g.ny + gy) * g.nx + gx], v);
+            }
+        }
+    }
+
+// ---------------------------------------------------------------------------
+// Gather: mirror of the spread.  Loads the (T+P-1)^3 potential box into LDS and
+// evaluates F_j = q_j sum grad W(x_m - r_j) phi(m) with the analytic stencil
+// derivative; adds to force[j].xyz.
+// ---------------------------------------------------------------------------
+template<typename R, int P>
+__global__ void __launch_bounds__(256) k_gather(MeshGeom g,
+                                                int D,
+                                                const unsigned int* __restrict__ tile_start,
+                                                const SortedRec<R>* __restrict__ rec,
+                                                const SortedIdx* __restrict__ sidx,
+                                                const R* __restrict__ wcoef,
+                                                const R* __restrict__ phi,
+                                                double4* __restrict__ force)
+    {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int B = g.B;
+    const int B2 = B * B;
+    const int B3 = B2 * B;
+    R* box = reinterpret_cast<R*>(smem);
+    R* wc = reinterpret_cast<R*>(smem + ((size_t(B3) * sizeof(R) + 15) & ~size_t(15)));
+
+    const unsigned int tile = blockIdx.x;
+    const unsigned int s0 = tile_start[tile], s1 = tile_start[tile + 1];
+    if (s0 == s1)
+        return;
+    const int txy = g.tx * g.ty;
+    const int tz = int(tile) / txy;
+    const int ty = (int(tile) - tz * txy) / g.tx;
+    const int tx = int(tile) - tz * txy - ty * g.tx;
+    for (int c = threadIdx.x; c < B3; c += blockDim.x)
+        {
+        const int lz = c / B2;
+        const int ly = (c - lz * B2) / B;
+        const int lx = c - lz * B2 - ly * B;
+        const int gx = (tx * g.T + lx) % g.nx;
+        const int gy = (ty * g.T + ly) % g.ny;
+        const int gz = (tz * g.T + lz) % g.nz;
+        box[c] = phi[(size_t(gz) * g.ny + gy) * g.nx + gx];
+        }
+    for (int c = threadIdx.x; c < P * (D + 1); c += blockDim.x)
+        wc[c] = wcoef[c];
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int nw = blockDim.x >> 6;
+    const int axis = lane / P;
+    const int kt = lane - axis * P;
+    const R inv_h[3] = {R(1.0 / g.hx), R(1.0 / g.hy), R(1.0 / g.hz)};
+    for (unsigned int p = s0 + wave; p < s1; p += nw)
+        {
+        const SortedRec<R> r = rec[p];
+        const SortedIdx id = sidx[p];
+        const int ox = id.local & 0xff, oy = (id.local >> 8) & 0xff, oz = id.local >> 16;
+        R w = R(0), dw = R(0);
+        if (lane < 3 * P)
+            {
+            const R t = axis == 0 ? r.tx : (axis == 1 ? r.ty : r.tz);
+            const R z = R(2) * t - R(1);
+            const R* c = wc + kt * (D + 1);
+            R v = c[D], d = R(0);
+            for (int j = D - 1; j >= 0; --j)
+                {
+                d = d * z + v;
+                v = v * z + c[j];
+                }
+            w = v;
+            dw = R(2) * d;  // d/dt = 2 d/dz
+            }
+        R wz[P], dwz[P];
+#pragma unroll
+        for (int c = 0; c < P; ++c)
+            {
+            wz[c] = __shfl(w, 2 * P + c, WAVE);
+            dwz[c] = __shfl(dw, 2 * P + c, WAVE);
+            }
+        R ex = R(0), ey = R(0), ez = R(0);
+#pragma unroll
+        for (int it = 0; it < (P * P + 63) / 64; ++it)
+            {
+            const int s = lane + it * 64;
+            const int a = s % P;
+            const int b = s / P;
+            const int bb = b < P ? b : 0;
+            const R wx = __shfl(w, a, WAVE);
+            const R dwx = __shfl(dw, a, WAVE);
+            const R wy = __shfl(w, P + bb, WAVE);
+            const R dwy = __shfl(dw, P + bb, WAVE);
+            if (s < P * P)
+                {
+                const R* src = box + (oz * B + oy + b) * B + ox + a;
+                R A = R(0), Bz = R(0);
+#pragma unroll
+                for (int c = 0; c < P; ++c)
+                    {
+                    const R ph = src[c * B2];
+                    A += wz[c] * ph;
+                    Bz += dwz[c] * ph;
+                    }
+                ex += dwx * wy * A;
+                ey += wx * dwy * A;
+                ez += wx * wy * Bz;
+                }
+            }
+        ex = wave_sum(ex);
+        ey = wave_sum(ey);
+        ez = wave_sum(ez);
+        if (lane == 0)
+            {
+            double4 f = force[id.idx];
+            f.x += double(r.q) * double(ex * inv_h[0]);
+            f.y += double(r.q) * double(ey * inv_h[1]);
+            f.z += double(r.q) * double(ez * inv_h[2]);
+            force[id.idx] = f;
+            }
+        }
+    }
+
+// ---------------------------------------------------------------------------
+// FFT passes.  Layout everywhere: [z][y][x], x fastest; the spectrum is the
+// half spectrum along x (nxh = nx/2 + 1 columns).
+// ---------------------------------------------------------------------------
+
+//! R2C along x for pairs of real lines packed into one complex FFT; reads and clears the mesh.
+template<typename R>
+__global__ void __launch_bounds__(256) k_fft_x_r2c(MeshGeom g,
+                                                   FFTPlan plan,
+                                                   int pairs_per_block,
+                                                   const C2<R>* __restrict__ tw_g,
+                                                   typename Acc<R>::glob_t* __restrict__ mesh,
+                                                   C2<R>* __restrict__ spec)
+    {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int nx = g.nx;
+    const int ls = nx + 1;
+    C2<R>* tw = reinterpret_cast<C2<R>*>(smem);
+    C2<R>* a = tw + nx;
+    C2<R>* b = a + pairs_per_block * ls;
+    const int nlines = g.ny * g.nz;
+    const int pair0 = blockIdx.x * pairs_per_block;
+    for (int i = threadIdx.x; i < nx; i += blockDim.x)
+        tw[i] = tw_g[i];
+    for (int idx = threadIdx.x; idx < pairs_per_block * nx; idx += blockDim.x)
+        {
+        const int pr = idx / nx;
+        const int x = idx - pr * nx;
+        const int l0 = 2 * (pair0 + pr);
+        const int l1 = l0 + 1;
+        R v0 = R(0), v1 = R(0);
+        if (l0 < nlines)
+            v0 = R(mesh_load_clear(&mesh[size_t(l0) * nx + x]));
+        if (l1 < nlines)
+            v1 = R(mesh_load_clear(&mesh[size_t(l1) * nx + x]));
+        a[pr * ls + x] = C2<R> {v0, v1};
+        }
+    __syncthreads();
+    C2<R>* res = lds_fft<R>(a, b, tw, plan, ls, pairs_per_block, -1, threadIdx.x, blockDim.x);
+    const int nxh = g.nxh;
+    for (int idx = threadIdx.x; idx < pairs_per_block * nxh; idx += blockDim.x)
+        {
+        const int pr = idx / nxh;
+        const int k = idx - pr * nxh;
+        const int l0 = 2 * (pair0 + pr);
+        const int l1 = l0 + 1;
+        const C2<R> z = res[pr * ls + k];
+        const C2<R> zc = res[pr * ls + (k == 0 ? 0 : nx - k)];
+        // X0 = (Z[k] + conj(Z[-k])) / 2, X1 = (Z[k] - conj(Z[-k])) / (2 i)
+        const C2<R> x0 = {R(0.5) * (z.x + zc.x), R(0.5) * (z.y - zc.y)};
+        const C2<R> x1 = {R(0.5) * (z.y + zc.y), R(-0.5) * (z.x - zc.x)};
+        if (l0 < nlines)
+            spec[size_t(l0) * nxh + k] = x0;
+        if (l1 < nlines)
+            spec[size_t(l1) * nxh + k] = x1;
+        }
+    }
+
+//! C2R along x for pairs of half-spectrum lines; phi = scale * real(IFFT).
+template<typename R>
+__global__ void __launch_bounds__(256) k_fft_x_c2r(MeshGeom g,
+                                                   FFTPlan plan,
+                                                   int pairs_per_block,
+                                                   const C2<R>* __restrict__ tw_g,
+                                                   const C2<R>* __restrict__ spec,
+                                                   R scale,
+                                                   R* __restrict__ phi)
+    {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int nx = g.nx;
+    const int nxh = g.nxh;
+    const int ls = nx + 1;
+    C2<R>* tw = reinterpret_cast<C2<R>*>(smem);
+    C2<R>* a = tw + nx;
+    C2<R>* b = a + pairs_per_block * ls;
+    const int nlines = g.ny * g.nz;
+    const int pair0 = blockIdx.x * pairs_per_block;
+    for (int i = threadIdx.x; i < nx; i += blockDim.x)
+        tw[i] = tw_g[i];
+    for (int idx = threadIdx.x; idx < pairs_per_block * nxh; idx += blockDim.x)
+        {
+        const int pr = idx / nxh;
+        const int k = idx - pr * nxh;
+        const int l0 = 2 * (pair0 + pr);
+        const int l1 = l0 + 1;
+        C2<R> x0 = {R(0), R(0)}, x1 = {R(0), R(0)};
+        if (l0 < nlines)
+            x0 = spec[size_t(l0) * nxh + k];
+        if (l1 < nlines)
+            x1 = spec[size_t(l1) * nxh + k];
+        // Z[k] = X0[k] + i X1[k]; Z[n-k] = conj(X0[k]) + i conj(X1[k])
+        a[pr * ls + k] = C2<R> {x0.x - x1.y, x0.y + x1.x};
+        if (k != 0 && k != nx - k)
+            a[pr * ls + nx - k] = C2<R> {x0.x + x1.y, -x0.y + x1.x};
+        }
+    __syncthreads();
+    C2<R>* res = lds_fft<R>(a, b, tw, plan, ls, pairs_per_block, +1, threadIdx.x, blockDim.x);
+    for (int idx = threadIdx.x; idx < pairs_per_block * nx; idx += blockDim.x)
+        {
+        const int pr = idx / nx;
+        const int x = idx - pr * nx;
+        const int l0 = 2 * (pair0 + pr);
+        const int l1 = l0 + 1;
+        const C2<R> z = res[pr * ls + x];
+        if (l0 < nlines)
+            phi[size_t(l0) * nx + x] = z.x * scale;
+        if (l1 < nlines)
+            phi[size_t(l1) * nx + x] = z.y * scale;
+        }
+    }
+
+//! Complex FFT along y (sign -1 forward, +1 inverse) on blocks of ncol consecutive x columns.
+template<typename R>
+__global__ void __launch_bounds__(256) k_fft_y(MeshGeom g, FFTPlan plan, int ncol, const C2<R>* __restrict__ tw_g, C2<R>* __restrict__ spec, int sign)
+    {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int ny = g.ny;
+    const int nxh = g.nxh;
+    const int ls = ny + 1;
+    C2<R>* tw = reinterpret_cast<C2<R>*>(smem);
+    C2<R>* a = tw + ny;
+    C2<R>* b = a + ncol * ls;
+    const int nbx = (nxh + ncol - 1) / ncol;
+    const int z = blockIdx.x / nbx;
+    const int x0 = (blockIdx.x - z * nbx) * ncol;
+    const int nc = min(ncol, nxh - x0);
+    for (int i = threadIdx.x; i < ny; i += blockDim.x)
+        tw[i] = tw_g[i];
+    for (int idx = threadIdx.x; idx < ny * ncol; idx += blockDim.x)
+        {
+        const int y = idx / ncol;
+        const int xi = idx - y * ncol;
+        if (xi < nc)
+            a[xi * ls + y] = spec[(size_t(z) * ny + y) * nxh + x0 + xi];
+        }
+    __syncthreads();
+    C2<R>* res = lds_fft<R>(a, b, tw, plan, ls, nc, sign, threadIdx.x, blockDim.x);
+    for (int idx = threadIdx.x; idx < ny * ncol; idx += blockDim.x)
+        {
+        const int y = idx / ncol;
+        const int xi = idx - y * ncol;
+        if (xi < nc)
+            spec[(size_t(z) * ny + y) * nxh + x0 + xi] = res[xi * ls + y];
+        }
+    }
+
+//! Forward FFT along z, multiply by the influence function G(k), accumulate the FP64
+//! reciprocal energy and virial, inverse FFT along z.  One block per (y, column block).
+template<typename R>
+__global__ void __launch_bounds__(256) k_fft_z_conv(MeshGeom g,
+                                                    FFTPlan plan,
+                                                    int ncol,
+                                                    const C2<R>* __restrict__ tw_g,
+                                                    C2<R>* __restrict__ spec,
+                                                    const double* __restrict__ G,
+                                                    const double* __restrict__ Gv,
+                                                    int want_ev,
+                                                    double* __restrict__ ev_part)
+    {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int ny = g.ny;
+    const int nz = g.nz;
+    const int nxh = g.nxh;
+    const int ls = nz + 1;
+    C2<R>* tw = reinterpret_cast<C2<R>*>(smem);
+    C2<R>* a = tw + nz;
+    C2<R>* b = a + ncol * ls;
+    const int nbx = (nxh + ncol - 1) / ncol;
+    const int y = blockIdx.x / nbx;
+    const int x0 = (blockIdx.x - y * nbx) * ncol;
+    const int nc = min(ncol, nxh - x0);
+    for (int i = threadIdx.x; i < nz; i += blockDim.x)
+        tw[i] = tw_g[i];
+    for (int idx = threadIdx.x; idx < nz * ncol; idx += blockDim.x)
+        {
+        const int z = idx / ncol;
+        const int xi = idx - z * ncol;
+        if (xi < nc)
+            a[xi * ls + z] = spec[(size_t(z) * ny + y) * nxh + x0 + xi];
+        }
+    __syncthreads();
+    C2<R>* res = lds_fft<R>(a, b, tw, plan, ls, nc, -1, threadIdx.x, blockDim.x);
+
+    double e = 0, v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
+    const double two_pi = 6.283185307179586476925286766559;
+    const int ky_i = (y <= ny / 2) ? y : y - ny;
+    const double ky = two_pi * double(ky_i) / g.Ly;
+    for (int idx = threadIdx.x; idx < nz * ncol; idx += blockDim.x)
+        {
+        const int z = idx / ncol;
+        const int xi = idx - z * ncol;
+        if (xi >= nc)
+            continue;
+        const int x = x0 + xi;
+        const size_t m = (size_t(z) * ny + y) * nxh + x;
+        const double gk = G[m];
+        C2<R> c = res[xi * ls + z];
+        if (want_ev)
+            {
+            const double s2 = double(c.x) * double(c.x) + double(c.y) * double(c.y);
+            const double wgt = (x == 0 || 2 * x == g.nx) ? 1.0 : 2.0;
+            const double ee = wgt * s2;
+            const double gv = Gv[m];
+            const int kz_i = (z <= nz / 2) ? z : z - nz;
+            const double kx = two_pi * double(x) / g.Lx;
+            const double kz = two_pi * double(kz_i) / g.Lz;
+            e += ee * gk;
+            v0 += ee * (gk + gv * kx * kx);
+            v1 += ee * (gv * kx * ky);
+            v2 += ee * (gv * kx * kz);
+            v3 += ee * (gk + gv * ky * ky);
+            v4 += ee * (gv * ky * kz);
+            v5 += ee * (gk + gv * kz * kz);
+            }
+        const R gr = R(gk);
+        c.x *= gr;
+        c.y *= gr;
+        res[xi * ls + z] = c;
+        }
+    __syncthreads();
+    C2<R>* other = (res == a) ? b : a;
+    C2<R>* res2 = lds_fft<R>(res, other, tw, plan, ls, nc, +1, threadIdx.x, blockDim.x);
+    for (int idx = threadIdx.x; idx < nz * ncol; idx += blockDim.x)
+        {
+        const int z = idx / ncol;
+        const int xi = idx - z * ncol;
+        if (xi < nc)
+            spec[(size_t(z) * ny + y) * nxh + x0 + xi] = res2[xi * ls + z];
+        }
+    if (want_ev)
+        {
+        __shared__ double red[4][7];
+        double vals[7] = {e, v0, v1, v2, v3, v4, v5};
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+        for (int c = 0; c < 7; ++c)
+            {
+            const double s = wave_sum(vals[c]);
+            if (lane == 0)
+                red[wave][c] = s;
+            }
+        __syncthreads();
+        if (threadIdx.x < 7)
+            {
+            double s = 0;
+            for (int w = 0; w < int(blockDim.x >> 6); ++w)
+                s += red[w][threadIdx.x];
+            ev_part[size_t(blockIdx.x) * 8 + threadIdx.x] = s;
+            }
+        }
+    }
+
+// ---------------------------------------------------------------------------
+// Influence function tabulation (FP64): G = F(|k|) / (k^2 |w(kx)|^2 |w(ky)|^2 |w(kz)|^2),
+// Gv = (dF/dk / k^3 - 2 F / k^4) / (same window product), both 0 at k = 0.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_build_influence(MeshGeom g,
+                                                         int split_kind,
+                                                         double kappa,
+                                                         double rc,
+                                                         int nq,
+                                                         const double* __restrict__ qx,
+                                                         const double* __restrict__ qw,
+                                                         const double* __restrict__ w2x,
+                                                         const double* __restrict__ w2y,
+                                                         const double* __restrict__ w2z,
+                                                         double* __restrict__ G,
+                                                         double* __restrict__ Gv)
+    {
+    const size_t m = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const size_t total = size_t(g.nz) * g.ny * g.nxh;
+    if (m >= total)
+        return;
+    const int x = int(m % g.nxh);
+    const int y = int((m / g.nxh) % g.ny);
+    const int z = int(m / (size_t(g.nxh) * g.ny));
+    const int ky_i = (y <= g.ny / 2) ? y : y - g.ny;
+    const int kz_i = (z <= g.nz / 2) ? z : z - g.nz;
+    const double two_pi = 6.283185307179586476925286766559;
+    const double kx = two_pi * x / g.Lx, ky = two_pi * ky_i / g.Ly, kz = two_pi * kz_i / g.Lz;
+    const double k2 = kx * kx + ky * ky + kz * kz;
+    if (k2 == 0.0)
+        {
+        G[m] = 0.0;
+        Gv[m] = 0.0;
+        return;
+        }
+    const double k = sqrt(k2);
+    double F = 0.0, dF = 0.0;
+    if (split_kind == 1)
+        {
+        F = exp(-k2 / (4.0 * kappa * kappa));
+        dF = -k / (2.0 * kappa * kappa) * F;
+        }
+    else
+        {
+        for (int i = 0; i < nq; ++i)
+            {
+            double s, c;
+            sincos(k * rc * qx[i], &s, &c);
+            F += qw[i] * c;
+            dF -= qw[i] * rc * qx[i] * s;
+            }
+        }
+    const double wp = w2x[x] * w2y[y] * w2z[z];
+    G[m] = F / (k2 * wp);
+    Gv[m] = (dF / (k2 * k) - 2.0 * F / (k2 * k2)) / wp;
+    }
+
+//! Final FP64 reduction: out[0] = E_rec + E_self + E_bg, out[1..6] = W_rec + background;
+//! qsum[0..1] = (sum q, sum q^2).
+__global__ void __launch_bounds__(256) k_finalize(int n_ev,
+                                                  const double* __restrict__ ev_part,
+                                                  int n_q,
+                                                  const double* __restrict__ qpart,
+                                                  double inv_2V,
+                                                  double self_coeff,
+                                                  double bg_coeff,
+                                                  double V,
+                                                  int want_ev,
+                                                  double* __restrict__ out,
+                                                  double* __restrict__ qsum)
+    {
+    __shared__ double red[4][9];
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (want_ev)
+        for (int i = threadIdx.x; i < n_ev; i += blockDim.x)
+            for (int c = 0; c < 7; ++c)
+                acc[c] += ev_part[size_t(i) * 8 + c];
+    for (int i = threadIdx.x; i < n_q; i += blockDim.x)
+        {
+        acc[7] += qpart[2 * i];
+        acc[8] += qpart[2 * i + 1];
+        }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < 9; ++c)
+        {
+        const double s = wave_sum(acc[c]);
+        if (lane == 0)
+            red[wave][c] = s;
+        }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        {
+        double t[9];
+        for (int c = 0; c < 9; ++c)
+            {
+            t[c] = 0;
+            for (int w = 0; w < int(blockDim.x >> 6); ++w)
+                t[c] += red[w][c];
+            }
+        const double Q = t[7], Q2 = t[8];
+        qsum[0] = Q;
+        qsum[1] = Q2;
+        if (want_ev && out)
+            {
+            const double Eself = -0.5 * 0.079577471545947673 * self_coeff * Q2;
+            const double Ebg = -Q * Q / (2.0 * V) * bg_coeff;
+            out[0] = inv_2V * t[0] + Eself + Ebg;
+            for (int c = 1; c < 7; ++c)
+                out[c] = inv_2V * t[c];
+            out[1] += Ebg;
+            out[4] += Ebg;
+            out[6] += Ebg;
+            }
+        }
+    }
+
+    } // namespace dev
+
+// ===========================================================================
+// Host launchers (the kernel-launcher layer; cf. ESPForceComputeGPU.cuh:49-144)
+// ===========================================================================
+namespace kernel
+{
+using namespace esp::dev;
+
+hipError_t bin_particles(unsigned int n,
+                         const unsigned int* group,
+                         const double* postype,
+                         const double* charge,
+                         const MeshGeom& g,
+                         int P,
+                         unsigned int* tile_count,
+                         unsigned int* key,
+                         unsigned int* slot,
+                         double* qpart,
+                         hipStream_t s)
+    {
+    const unsigned int nb = (n + 255) / 256;
+    if (nb == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_bin, dim3(nb), dim3(256), 0, s, n, group, reinterpret_cast<const double4*>(postype), charge, g, P,
+                       tile_count, key, slot, qpart);
+    return hipGetLastError();
+    }
+
+hipError_t scan_tiles(unsigned int n_tiles, unsigned int* count, unsigned int* start, hipStream_t s)
+    {
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, n_tiles, count, start);
+    return hipGetLastError();
+    }
+
+template<typename R>
+hipError_t scatter_particles(unsigned int n,
+                             const unsigned int* group,
+                             const double* postype,
+                             const double* charge,
+                             const MeshGeom& g,
+                             int P,
+                             const unsigned int* key,
+                             const unsigned int* slot,
+                             const unsigned int* start,
+                             void* rec,
+                             SortedIdx* sidx,
+                             hipStream_t s)
+    {
+    const unsigned int nb = (n + 255) / 256;
+    if (nb == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_scatter<R>, dim3(nb), dim3(256), 0, s, n, group, reinterpret_cast<const double4*>(postype), charge, g,
+                       P, key, slot, start, reinterpret_cast<SortedRec<R>*>(rec), sidx);
+    return hipGetLastError();
+    }
+template hipError_t scatter_particles<double>(unsigned int, const unsigned int*, const double*, const double*, const MeshGeom&, int,
+                                              const unsigned int*, const unsigned int*, const unsigned int*, void*, SortedIdx*,
+                                              hipStream_t);
+template hipError_t scatter_particles<float>(unsigned int, const unsigned int*, const double*, const double*, const MeshGeom&, int,
+                                             const unsigned int*, const unsigned int*, const unsigned int*, void*, SortedIdx*,
+                                             hipStream_t);
+
+size_t spread_lds_bytes(const MeshGeom& g, int P, int D, bool fp64)
+    {
+    const size_t B3 = size_t(g.B) * g.B * g.B;
+    const size_t el = fp64 ? 8 : 4;
+    return ((B3 * el + 15) & ~size_t(15)) + size_t(P) * (D + 1) * el;
+    }
+
+template<typename R, int P>
+static hipError_t launch_spread_P(const MeshGeom& g, int D, unsigned int n_tiles, const unsigned int* start, const void* rec,
+                                  const SortedIdx* sidx, const void* wcoef, void* mesh, hipStream_t s)
+    {
+    const size_t lds = spread_lds_bytes(g, P, D, sizeof(R) == 8);
+    hipLaunchKernelGGL((k_spread<R, P>), dim3(n_tiles), dim3(256), lds, s, g, D, start, reinterpret_cast<const SortedRec<R>*>(rec),
+                       sidx, reinterpret_cast<const R*>(wcoef), reinterpret_cast<typename Acc<R>::glob_t*>(mesh));
+    return hipGetLastError();
+    }
+
+template<typename R, int P>
+static hipError_t launch_gather_P(const MeshGeom& g, int D, unsigned int n_tiles, const unsigned int* start, const void* rec,
+                                  const SortedIdx* sidx, const void* wcoef, const void* phi, double* force, hipStream_t s)
+    {
+    const size_t lds = spread_lds_bytes(g, P, D, sizeof(R) == 8);
+    hipLaunchKernelGGL((k_gather<R, P>), dim3(n_tiles), dim3(256), lds, s, g, D, start, reinterpret_cast<const SortedRec<R>*>(rec),
+                       sidx, reinterpret_cast<const R*>(wcoef), reinterpret_cast<const R*>(phi), reinterpret_cast<double4*>(force));
+    return hipGetLastError();
+    }
+
+#define ESP_P_CASES(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)
+
+template<typename R>
+hipError_t spread(const MeshGeom& g, int P, int D, unsigned int n_tiles, const unsigned int* start, const void* rec,
+                  const SortedIdx* sidx, const void* wcoef, void* mesh, hipStream_t s)
+    {
+    switch (P)
+        {
+#define ESP_CASE(PP) \
+    case PP: return launch_spread_P<R, PP>(g, D, n_tiles, start, rec, sidx, wcoef, mesh, s);
+        ESP_P_CASES(ESP_CASE)
+#undef ESP_CASE
+        default: return hipErrorInvalidValue;
+        }
+    }
+template hipError_t spread<double>(const MeshGeom&, int, int, unsigned int, const unsigned int*, const void*, const SortedIdx*,
+                                   const void*, void*, hipStream_t);
+template hipError_t spread<float>(const MeshGeom&, int, int, unsigned int, const unsigned int*, const void*, const SortedIdx*,
+                                  const void*, void*, hipStream_t);
+
+template<typename R>
+hipError_t gather(const MeshGeom& g, int P, int D, unsigned int n_tiles, const unsigned int* start, const void* rec,
+                  const SortedIdx* sidx, const void* wcoef, const void* phi, double* force, hipStream_t s)
+    {
+    switch (P)
+        {
+#define ESP_CASE(PP) \
+    case PP: return launch_gather_P<R, PP>(g, D, n_tiles, start, rec, sidx, wcoef, phi, force, s);
+        ESP_P_CASES(ESP_CASE)
+#undef ESP_CASE
+        default: return hipErrorInvalidValue;
+        }
+    }
+template hipError_t gather<double>(const MeshGeom&, int, int, unsigned int, const unsigned int*, const void*, const SortedIdx*,
+                                   const void*, const void*, double*, hipStream_t);
+template hipError_t gather<float>(const MeshGeom&, int, int, unsigned int, const unsigned int*, const void*, const SortedIdx*,
+                                  const void*, const void*, double*, hipStream_t);
+
+int fft_x_pairs_per_block(int nx, bool fp64)
+    {
+    const size_t el = fp64 ? 16 : 8;
+    int pairs = 8;
+    while (pairs > 1 && (size_t(nx) + 2 * size_t(pairs) * (nx + 1)) * el > 64 * 1024)
+        pairs >>= 1;
+    return pairs;
+    }
+
+int fft_cols_per_block(int n, bool fp64)
+    {
+    const size_t el = fp64 ? 16 : 8;
+    int cols = 16;
+    while (cols > 1 && (size_t(n) + 2 * size_t(cols) * (n + 1)) * el > 64 * 1024)
+        cols >>= 1;
+    return cols;
+    }
+
+template<typename R>
+hipError_t fft_forward_x(const MeshGeom& g, const FFTPlan& px, const void* tw, void* mesh, void* spec, hipStream_t s)
+    {
+    const int ppb = fft_x_pairs_per_block(g.nx, sizeof(R) == 8);
+    const int npairs = (g.ny * g.nz + 1) / 2;
+    const int nb = (npairs + ppb - 1) / ppb;
+    const size_t lds = (size_t(g.nx) + 2 * size_t(ppb) * (g.nx + 1)) * sizeof(C2<R>);
+    hipLaunchKernelGGL(k_fft_x_r2c<R>, dim3(nb), dim3(256), lds, s, g, px, ppb, reinterpret_cast<const C2<R>*>(tw),
+                       reinterpret_cast<typename Acc<R>::glob_t*>(mesh), reinterpret_cast<C2<R>*>(spec));
+    return hipGetLastError();
+    }
+
+template<typename R>
+hipError_t fft_inverse_x(const MeshGeom& g, const FFTPlan& px, const void* tw, const void* spec, double scale, void* phi, hipStream_t s)
+    {
+    const int ppb = fft_x_pairs_per_block(g.nx, sizeof(R) == 8);
+    const int npairs = (g.ny * g.nz + 1) / 2;
+    const int nb = (npairs + ppb - 1) / ppb;
+    const size_t lds = (size_t(g.nx) + 2 * size_t(ppb) * (g.nx + 1)) * sizeof(C2<R>);
+    hipLaunchKernelGGL(k_fft_x_c2r<R>, dim3(nb), dim3(256), lds, s, g, px, ppb, reinterpret_cast<const C2<R>*>(tw),
+                       reinterpret_cast<const C2<R>*>(spec), R(scale), reinterpret_cast<R*>(phi));
+    return hipGetLastError();
+    }
+
+template<typename R>
+hipError_t fft_y(const MeshGeom& g, const FFTPlan& py, const void* tw, void* spec, int sign, hipStream_t s)
+    {
+    const int ncol = fft_cols_per_block(g.ny, sizeof(R) == 8);
+    const int nbx = (g.nxh + ncol - 1) / ncol;
+    const size_t lds = (size_t(g.ny) + 2 * size_t(ncol) * (g.ny + 1)) * sizeof(C2<R>);
+    hipLaunchKernelGGL(k_fft_y<R>, dim3(nbx * g.nz), dim3(256), lds, s, g, py, ncol, reinterpret_cast<const C2<R>*>(tw),
+                       reinterpret_cast<C2<R>*>(spec), sign);
+    return hipGetLastError();
+    }
+
+unsigned int fft_z_blocks(const MeshGeom& g, bool fp64)
+    {
+    const int ncol = fft_cols_per_block(g.nz, fp64);
+    const int nbx = (g.nxh + ncol - 1) / ncol;
+    return unsigned(nbx * g.ny);
+    }
+
+template<typename R>
+hipError_t fft_z_convolve(const MeshGeom& g, const FFTPlan& pz, const void* tw, void* spec, const double* G, const double* Gv,
+                          int want_ev, double* ev_part, hipStream_t s)
+    {
+    const int ncol = fft_cols_per_block(g.nz, sizeof(R) == 8);
+    const int nbx = (g.nxh + ncol - 1) / ncol;
+    const size_t lds = (size_t(g.nz) + 2 * size_t(ncol) * (g.nz + 1)) * sizeof(C2<R>);
+    hipLaunchKernelGGL(k_fft_z_conv<R>, dim3(nbx * g.ny), dim3(256), lds, s, g, pz, ncol, reinterpret_cast<const C2<R>*>(tw),
+                       reinterpret_cast<C2<R>*>(spec), G, Gv, want_ev, ev_part);
+    return hipGetLastError();
+    }
+
+#define ESP_INST_FFT(R)                                                                                                          \
+    template hipError_t fft_forward_x<R>(const MeshGeom&, const FFTPlan&, const void*, void*, void*, hipStream_t);                \
+    template hipError_t fft_inverse_x<R>(const MeshGeom&, const FFTPlan&, const void*, const void*, double, void*, hipStream_t);  \
+    template hipError_t fft_y<R>(const MeshGeom&, const FFTPlan&, const void*, void*, int, hipStream_t);                        \
+    template hipError_t fft_z_convolve<R>(const MeshGeom&, const FFTPlan&, const void*, void*, const double*, const double*, int, \
+                                          double*, hipStream_t);
+ESP_INST_FFT(double)
+ESP_INST_FFT(float)
+#undef ESP_INST_FFT
+
+hipError_t build_influence(const MeshGeom& g, int split_kind, double kappa, double rc, int nq, const double* qx, const double* qw,
+                           const double* w2x, const double* w2y, const double* w2z, double* G, double* Gv, hipStream_t s)
+    {
+    const size_t total = size_t(g.nz) * g.ny * g.nxh;
+    const unsigned int nb = unsigned((total + 255) / 256);
+    hipLaunchKernelGGL(k_build_influence, dim3(nb), dim3(256), 0, s, g, split_kind, kappa, rc, nq, qx, qw, w2x, w2y, w2z, G, Gv);
+    return hipGetLastError();
+    }
+
+hipError_t finalize(int n_ev, const double* ev_part, int n_q, const double* qpart, double inv_2V, double self_coeff, double bg_coeff,
+                    double V, int want_ev, double* out, double* qsum, hipStream_t s)
+    {
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, n_ev, ev_part, n_q, qpart, inv_2V, self_coeff, bg_coeff, V, want_ev, out,
+                       qsum);
+    return hipGetLastError();
+    }
+
+    } // namespace kernel
+    } // namespace esp

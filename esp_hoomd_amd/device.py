@@ -1,0 +1,105 @@
+"""Minimal device-memory plumbing over the C ABI (no PyTorch needed)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+
+class DeviceArray:
+    """A device allocation with a numpy dtype and shape."""
+
+    def __init__(self, shape, dtype=np.float64, zero: bool = False):
+        lib = _lib.load()
+        self.shape = tuple(int(s) for s in (shape if isinstance(shape, (tuple, list)) else (shape,)))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        _lib.check(lib.esp_device_malloc(C.byref(p), max(self.nbytes, 8)))
+        self._ptr = p
+        if zero:
+            _lib.check(lib.esp_memset(self._ptr, 0, max(self.nbytes, 8)))
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+    @property
+    def ptr(self) -> int:
+        return self._ptr.value
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if a.nbytes != self.nbytes:
+            raise ValueError(f"size mismatch: {a.nbytes} != {self.nbytes}")
+        if self.nbytes:
+            _lib.check(_lib.load().esp_memcpy_htod(self._ptr, a.ctypes.data_as(C.c_void_p), self.nbytes))
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            _lib.check(_lib.load().esp_memcpy_dtoh(out.ctypes.data_as(C.c_void_p), self._ptr, self.nbytes))
+        return out
+
+    def zero(self) -> None:
+        if self.nbytes:
+            _lib.check(_lib.load().esp_memset(self._ptr, 0, self.nbytes))
+
+    def free(self) -> None:
+        if getattr(self, "_ptr", None) is not None and self._ptr.value:
+            _lib.load().esp_device_free(self._ptr)
+        self._ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        s = C.c_void_p()
+        _lib.check(_lib.load().esp_stream_create(C.byref(s)))
+        self.handle = s
+
+    def synchronize(self) -> None:
+        _lib.check(_lib.load().esp_stream_synchronize(self.handle))
+
+    def __del__(self):
+        try:
+            if self.handle.value:
+                _lib.load().esp_stream_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        e = C.c_void_p()
+        _lib.check(_lib.load().esp_event_create(C.byref(e)))
+        self.handle = e
+
+    def record(self, stream: Stream | None = None) -> None:
+        _lib.check(_lib.load().esp_event_record(self.handle, stream.handle if stream else None))
+
+    def elapsed_ms(self, end: "Event") -> float:
+        ms = C.c_float()
+        _lib.check(_lib.load().esp_event_elapsed_ms(self.handle, end.handle, C.byref(ms)))
+        return float(ms.value)
+
+    def __del__(self):
+        try:
+            if self.handle.value:
+                _lib.load().esp_event_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def synchronize() -> None:
+    _lib.check(_lib.load().esp_device_synchronize())

@@ -1,0 +1,199 @@
+/*
+ * esp_mi355x.h -- C ABI of the MI355X-native ESP (Ewald summation with prolates) solver.
+ *
+ * Plain C types only: device pointers, sizes and status codes.  No C++ exceptions cross
+ * this boundary.  A context is single threaded and enqueues all work on the caller's
+ * stream; distinct contexts may run concurrently.  The caller owns the system and output
+ * buffers; the library owns meshes, tables and plans.
+ *
+ * Units and layouts are those of the reference plugin:
+ *   Coulomb kernel 1/(4 pi r)                         (ESPForceCompute.cc:36)
+ *   postype Scalar4[N] = (x, y, z, type), box centred on the origin, charge Scalar[N]
+ *   force   Scalar4[N] = (fx, fy, fz, per-particle energy)
+ *   virial  Scalar[6 * pitch] SoA xx, xy, xz, yy, yz, zz  (ESPForceComputeGPU.cu:453-458)
+ *   mesh index (cz * Ny + cy) * Nx + cx                   (ESPForceComputeGPU.cu:267)
+ *   neighbour list: full list, nlist[head_list[i] + k] for k < n_neigh[i] (HOOMD layout)
+ *   exclusions: ex_list[i * ex_pitch + k] for k < n_ex[i]  (Index2D(ex, i), ESPForceComputeGPU.cu:418)
+ * Scalar is double.
+ */
+#ifndef ESP_MI355X_H_
+#define ESP_MI355X_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define ESP_OK 0
+#define ESP_ERR_ARG (-1)
+#define ESP_ERR_HIP (-2)
+#define ESP_ERR_OOM (-3)
+#define ESP_ERR_STATE (-4)
+#define ESP_ERR_RANGE (-5)
+#define ESP_ERR_UNSUPPORTED (-6)
+
+/* precision modes */
+#define ESP_PREC_FP64 0   /* every stage in FP64 */
+#define ESP_PREC_MIXED 1  /* FP32 spread/FFT/gather/pair, fixed-point mesh accumulation,
+                             FP64 influence function, self/k=0 terms and reductions */
+
+/* split kinds */
+#define ESP_SPLIT_PSWF 0      /* the ESP prolate split (compactly supported short range) */
+#define ESP_SPLIT_GAUSSIAN 1  /* erfc split (textbook Ewald / SPME), for validation */
+
+/* compute flags (esp_out.flags; 0 means all) */
+#define ESP_COMPUTE_PAIR 1u
+#define ESP_COMPUTE_MESH 2u
+#define ESP_COMPUTE_ENERGY_VIRIAL 4u
+#define ESP_COMPUTE_ALL 7u
+
+typedef struct esp_ctx esp_ctx;
+typedef void* esp_stream_t; /* a hipStream_t; NULL is the default stream */
+
+/* Parameters of ESPForceCompute::setParams(nx, ny, nz, order, kappa, rcut, alpha, n_table)
+ * (ESPForceCompute.h:99-106) plus the MI355X-specific options.  kappa is the splitting
+ * parameter in 1/length: for the PSWF split the prolate bandwidth is c = kappa * rcut and
+ * the long-range part is band limited near |k| = kappa. */
+typedef struct esp_params {
+    unsigned int nx, ny, nz;       /* mesh resolution; each a product of 2, 3, 5 */
+    unsigned int order;            /* stencil order P, 2..16 */
+    double kappa;                  /* splitting parameter [1/length] */
+    double rcut;                   /* real-space cutoff r_c */
+    double alpha;                  /* Debye screening; must be 0 */
+    unsigned int n_table;          /* split-table segments (PSWF_N_SEGS = 4096 default) */
+    int precision;                 /* ESP_PREC_* */
+    int split;                     /* ESP_SPLIT_* */
+    double window_c;               /* window prolate bandwidth; 0 = automatic */
+    unsigned int window_degree;    /* window polynomial degree; 0 = automatic */
+    int pair_lanes;                /* lanes per particle in the pair kernel; 0 = automatic */
+} esp_params;
+
+/* One system: device pointers owned by the caller (HOOMD's ParticleData, NeighborList and
+ * ParticleGroup arrays, ESPForceComputeGPU.cc:221-224, 397-402). */
+typedef struct esp_system {
+    unsigned int N;                 /* particles in postype/charge/force */
+    const double* postype;          /* Scalar4[N] */
+    const double* charge;           /* Scalar[N] */
+    unsigned int group_size;        /* members of the mesh group (ignored if group_idx NULL) */
+    const unsigned int* group_idx;  /* group member indices, or NULL for all N */
+    double box[6];                  /* Lx, Ly, Lz, xy, xz, yz (tilts must be 0) */
+    const unsigned int* n_neigh;    /* full neighbour list, or NULL for no pair part */
+    const unsigned int* nlist;
+    const uint64_t* head_list;
+    const unsigned int* n_ex;       /* exclusions, or NULL */
+    const unsigned int* ex_list;
+    unsigned int ex_pitch;
+} esp_system;
+
+/* Outputs, overwritten each call (ForceCompute m_force / m_virial / external energy). */
+typedef struct esp_out {
+    double* force;                  /* Scalar4[N] */
+    double* virial;                 /* Scalar[6 * virial_pitch], or NULL */
+    size_t virial_pitch;            /* >= N */
+    double* ext_energy_virial;      /* device double[7]: E_ext, W_xx, W_xy, W_xz, W_yy, W_yz, W_zz; or NULL */
+    unsigned int flags;             /* ESP_COMPUTE_*; 0 = all */
+} esp_out;
+
+/* ESPTableEntry (ESPForceCompute.h:78-88): introspection table of L(r) and -dL/dr. */
+typedef struct esp_table_entry {
+    double potential_coeffs[4];
+    double force_coeffs[4];
+    double potential_coeff4;
+    double force_coeff4;
+    double r_lo;
+    double dr_inv;
+} esp_table_entry;
+
+/* Introspection (getters of ESPForceCompute.h:108-118). */
+typedef struct esp_info {
+    unsigned int nx, ny, nz;
+    unsigned int order;
+    double kappa, rcut, alpha;
+    unsigned int n_table;
+    int precision, split;
+    unsigned int window_degree;
+    double window_c;
+    double self_coeff;              /* lim (1 - S(r/rc)) / r */
+    double background_coeff;        /* lim (1 - F(k)) / k^2 */
+    unsigned int tile;              /* spreading tile edge (0 before the first step) */
+    size_t device_bytes;            /* device memory held by the context */
+} esp_info;
+
+/* ---- lifetime and parameters --------------------------------------------------------- */
+/* ESPForceCompute ctor + setParams (ESPForceCompute.cc:53-89; pybind ctor :454-467) */
+int esp_create(esp_ctx** ctx, const esp_params* params);
+/* ESPForceCompute::setParams (ESPForceCompute.cc:71-89); takes effect on the next compute */
+int esp_set_params(esp_ctx* ctx, const esp_params* params);
+/* ESPForceCompute::invalidate (ESPForceCompute.h:120) */
+int esp_invalidate(esp_ctx* ctx);
+/* destructor (ESPForceCompute.cc:69) */
+int esp_destroy(esp_ctx* ctx);
+/* last error message of this context (or of the last failed esp_create when ctx is NULL) */
+const char* esp_last_error(const esp_ctx* ctx);
+
+/* ---- compute --------------------------------------------------------------------------- */
+/* ESPForceCompute::computeForces (ESPForceCompute.cc:96-127) on the GPU: pair correction
+ * (EvaluatorPairPSWF over the neighbour list, EvaluatorPairPSWF.h:274-295) + exclusion
+ * correction (ESPForceComputeGPU.cu:391-471) + mesh part (.cu:206-373) + energy/virial. */
+int esp_compute_forces(esp_ctx* ctx, uint64_t timestep, const esp_system* sys, esp_out* out, esp_stream_t stream);
+/* the PotentialPair<EvaluatorPairPSWF> path alone (module.cc:73,79): pair + exclusions */
+int esp_pair_forces(esp_ctx* ctx, const esp_system* sys, esp_out* out, esp_stream_t stream);
+
+/* ---- introspection ------------------------------------------------------------------- */
+int esp_get_info(esp_ctx* ctx, esp_info* info);
+/* getQSum / getQ2Sum (ESPForceCompute.h:113-114); synchronises with the device */
+int esp_get_charge_sums(esp_ctx* ctx, double* q_sum, double* q2_sum);
+/* getTableSize / getTableEntry (ESPForceCompute.cc:129-141, 336-372) */
+int esp_get_table_size(esp_ctx* ctx, unsigned int* n);
+int esp_get_table_entry(esp_ctx* ctx, unsigned int i, esp_table_entry* entry);
+/* split table in the PSWF_Coeffs.h format (PSWF_Coeffs.h:24-9599), reduced s = r / rc:
+ * screen[n][5], dscreen[n][5] (dS/ds), r_lo[n], dr_inv[n]; pass NULL arrays to query n */
+int esp_get_split_table(esp_ctx* ctx, unsigned int* n_segs, double* screen, double* dscreen, double* r_lo, double* dr_inv);
+/* replace the split table (e.g. the arrays of a PSWF_Coeffs.h header) */
+int esp_set_split_table(esp_ctx* ctx, unsigned int n_segs, const double* screen, const double* dscreen, const double* r_lo,
+                        const double* dr_inv);
+/* spreading window: P taps x (D+1) monomial coefficients in z = 2t - 1 (replaces
+ * PSWF_STENCIL_COEFFS / _X_LO / _DX_INV, PSWF_Coeffs.h:9606-9624) */
+int esp_get_window(esp_ctx* ctx, unsigned int* P, unsigned int* D, double* coef);
+
+/* ---- host-only table mathematics (no GPU needed) ------------------------------------- */
+/* table generator (replaces generate_pswf_table.py:540-595 with a valid split, S(0) = 1) */
+int esp_build_split_table(int split, double kappa, double rcut, unsigned int n_segs, double* screen, double* dscreen, double* r_lo,
+                          double* dr_inv);
+int esp_split_eval(int split, double kappa, double rcut, int n, const double* s, double* S, double* dSds);
+int esp_split_fourier(int split, double kappa, double rcut, int n, const double* k, double* F, double* dF);
+int esp_split_constants(int split, double kappa, double rcut, double* self_coeff, double* background_coeff);
+int esp_prolate_eval(double c, int n, const double* x, double* psi, double* integral);
+int esp_build_window(unsigned int P, unsigned int D, double window_c, double* coef);
+int esp_window_fourier(unsigned int P, unsigned int D, const double* coef, int n, const double* theta, double* what);
+double esp_default_window_c(unsigned int P, double sigma);
+unsigned int esp_default_window_degree(unsigned int P);
+
+/* ---- device utilities (plumbing for callers without a device allocator) ---------------- */
+int esp_device_count(int* n);
+int esp_set_device(int device);
+int esp_device_malloc(void** ptr, size_t bytes);
+int esp_device_free(void* ptr);
+int esp_memcpy_htod(void* dst, const void* src, size_t bytes);
+int esp_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+int esp_memcpy_dtod(void* dst, const void* src, size_t bytes, esp_stream_t stream);
+int esp_memset(void* dst, int value, size_t bytes);
+int esp_stream_create(esp_stream_t* stream);
+int esp_stream_destroy(esp_stream_t stream);
+int esp_stream_synchronize(esp_stream_t stream);
+int esp_device_synchronize(void);
+/* HIP events for timing on a given stream */
+int esp_event_create(void** ev);
+int esp_event_destroy(void* ev);
+int esp_event_record(void* ev, esp_stream_t stream);
+int esp_event_elapsed_ms(void* start, void* stop, float* ms);
+const char* esp_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ESP_MI355X_H_ */

@@ -1,0 +1,739 @@
+/*
+ * esp_oracle.c -- TEST INFRASTRUCTURE ONLY.
+ *
+ * CPU FP64 restatement of the ESP long-range electrostatics path, used as the
+ * checker for the HIP implementation.  Nothing in the product links or calls
+ * this file; only tests/, the smoke check and the CPU-baseline leg of the
+ * benchmark load it.
+ *
+ * Contents
+ *   oracle_split_eval   S(r)/S'(r) table lookup and L, -dL/dr
+ *                       (EvaluatorPairPSWF.h:98-143, ESPForceCompute.cc:41-50,258-296)
+ *   oracle_fft3d        mixed-radix complex 3-D FFT (x fastest), forward sign e^{-i}
+ *   oracle_ewald        FP64 direct Ewald sum in 1/(4 pi r) units (Gaussian split)
+ *   oracle_esp          the full ESP evaluation: spread -> FFT -> influence -> IFFT ->
+ *                       analytic-derivative gather, pair correction, exclusions, self and
+ *                       background terms, energy and virial.  Stage order follows
+ *                       ESPForceCompute::computeForces (ESPForceCompute.cc:96-127); the stubs there
+ *                       are filled from the mathematics of the split.
+ *
+ * Conventions kept from the reference: ESP_INV_4PI (ESPForceCompute.cc:36), mesh index
+ * (cz*Ny + cy)*Nx + cx (ESPForceComputeGPU.cu:267), k ordering i <= N/2 -> +i else i - N
+ * (ESPForceComputeGPU.cu:173-175), per-particle virial SoA xx,xy,xz,yy,yz,zz with a pitch
+ * (ESPForceComputeGPU.cu:453-458,469-470), exclusion list ex_list[i*pitch + ex] with the
+ * excluded-pair energy/force/virial formulas of ESPForceComputeGPU.cu:441-458.
+ *
+ * Parity status: every function here is pinned by tests (golden table vectors from
+ * PSWF_Coeffs.h, numpy.fft, Madelung constants, two Ewald splitting parameters, the
+ * Gaussian-split special case against Ewald, finite differences and brute force).
+ */
+#include <complex.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORACLE_INV_4PI 0.079577471545947673
+#define ORACLE_PI 3.14159265358979323846
+
+/* ------------------------------------------------------------------------- */
+/* Table lookup: EvaluatorPairPSWF.h:113-143 with the reduced variable s = r/rc */
+/* ------------------------------------------------------------------------- */
+static double horner5(const double* coeffs, unsigned int seg, double t)
+{
+    /* ESPForceCompute.cc:41-50 */
+    const double* c = coeffs + (size_t)seg * 5u;
+    double v = c[4];
+    v = c[3] + t * v;
+    v = c[2] + t * v;
+    v = c[1] + t * v;
+    v = c[0] + t * v;
+    return v;
+}
+
+static void lookup(const double* screen, const double* dscreen, const double* r_lo, const double* dr_inv,
+                   unsigned int n_segs, double rc, double r, double* L, double* negdLdr, double* S, double* dS)
+{
+    if (r < 1.0e-12 || r > rc) {
+        *L = 0.0;
+        *negdLdr = 0.0;
+        if (S) *S = 0.0;
+        if (dS) *dS = 0.0;
+        return;
+    }
+    double s = r / rc;
+    s = s < 0.0 ? 0.0 : (s > 1.0 - 1.0e-9 ? 1.0 - 1.0e-9 : s);
+    unsigned int seg = (unsigned int)(s * (double)n_segs);
+    if (seg >= n_segs) seg = n_segs - 1u;
+    const double t = (s - r_lo[seg]) * dr_inv[seg];
+    const double Sv = horner5(screen, seg, t);
+    const double dSv = horner5(dscreen, seg, t) / rc; /* dS/dr from dS/ds */
+    const double inv_4pi_r = ORACLE_INV_4PI / r;
+    const double inv_4pi_r2 = inv_4pi_r / r;
+    *L = Sv * inv_4pi_r;
+    *negdLdr = Sv * inv_4pi_r2 - dSv * inv_4pi_r;
+    if (S) *S = Sv;
+    if (dS) *dS = dSv;
+}
+
+void oracle_split_eval(const double* screen, const double* dscreen, const double* r_lo, const double* dr_inv,
+                       unsigned int n_segs, double rc, int n, const double* r, double* L, double* negdLdr,
+                       double* S, double* dS)
+{
+    for (int i = 0; i < n; ++i)
+        lookup(screen, dscreen, r_lo, dr_inv, n_segs, rc, r[i], &L[i], &negdLdr[i], S ? &S[i] : NULL,
+               dS ? &dS[i] : NULL);
+}
+
+/* ------------------------------------------------------------------------- */
+/* Mixed-radix FFT (recursive decimation in time over the smallest prime)     */
+/* ------------------------------------------------------------------------- */
+static int smallest_factor(int n)
+{
+    for (int p = 2; p * p <= n; ++p)
+        if (n % p == 0) return p;
+    return n;
+}
+
+static void fft_rec(int n, int stride, const double complex* in, double complex* out, int sign, double complex* tmp)
+{
+    if (n == 1) {
+        out[0] = in[0];
+        return;
+    }
+    const int p = smallest_factor(n);
+    const int m = n / p;
+    if (p == n) { /* prime: direct DFT */
+        for (int k = 0; k < n; ++k) {
+            double complex s = 0;
+            for (int j = 0; j < n; ++j) {
+                const double ang = sign * 2.0 * ORACLE_PI * (double)((long)j * k % n) / (double)n;
+                s += in[(size_t)j * stride] * (cos(ang) + I * sin(ang));
+            }
+            out[k] = s;
+        }
+        return;
+    }
+    /* p sub-transforms of length m on in[r*stride + j*p*stride] */
+    for (int r = 0; r < p; ++r)
+        fft_rec(m, stride * p, in + (size_t)r * stride, tmp + (size_t)r * m, sign, out);
+    for (int k = 0; k < m; ++k) {
+        for (int q = 0; q < p; ++q) {
+            const int kk = k + q * m;
+            double complex s = 0;
+            for (int r = 0; r < p; ++r) {
+                const double ang = sign * 2.0 * ORACLE_PI * (double)((long)r * kk % n) / (double)n;
+                s += tmp[(size_t)r * m + k] * (cos(ang) + I * sin(ang));
+            }
+            out[kk] = s;
+        }
+    }
+}
+
+static void fft1d(int n, double complex* line, int sign)
+{
+    double complex* in = malloc(sizeof(double complex) * n);
+    double complex* tmp = malloc(sizeof(double complex) * n);
+    memcpy(in, line, sizeof(double complex) * n);
+    fft_rec(n, 1, in, line, sign, tmp);
+    free(in);
+    free(tmp);
+}
+
+/* data: complex interleaved, index (z*ny + y)*nx + x; sign -1 forward, +1 backward (unnormalised) */
+int oracle_fft3d(int nx, int ny, int nz, double* data, int sign)
+{
+    double complex* a = (double complex*)data;
+    int nmax = nx > ny ? nx : ny;
+    nmax = nmax > nz ? nmax : nz;
+    double complex* line = malloc(sizeof(double complex) * nmax);
+    for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y) {
+            double complex* p = a + ((size_t)z * ny + y) * nx;
+            fft1d(nx, p, sign);
+        }
+    for (int z = 0; z < nz; ++z)
+        for (int x = 0; x < nx; ++x) {
+            for (int y = 0; y < ny; ++y) line[y] = a[((size_t)z * ny + y) * nx + x];
+            fft1d(ny, line, sign);
+            for (int y = 0; y < ny; ++y) a[((size_t)z * ny + y) * nx + x] = line[y];
+        }
+    for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+            for (int z = 0; z < nz; ++z) line[z] = a[((size_t)z * ny + y) * nx + x];
+            fft1d(nz, line, sign);
+            for (int z = 0; z < nz; ++z) a[((size_t)z * ny + y) * nx + x] = line[z];
+        }
+    free(line);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Direct Ewald sum (Gaussian split), orthorhombic box centred on the origin */
+/* ------------------------------------------------------------------------- */
+static void min_image(double* d, const double* L)
+{
+    for (int a = 0; a < 3; ++a) d[a] -= L[a] * nearbyint(d[a] / L[a]);
+}
+
+/*
+ * Forces on the n_targets particles listed in targets (all if targets == NULL) from all N
+ * particles; energy and virial of the whole system when energy != NULL (needs all targets).
+ * Real space: all images within rcut (rcut may exceed L/2).  k space: |k| <= kcut.
+ */
+int oracle_ewald(int N, const double* postype, const double* q, const double* Lbox, double alpha, double rcut,
+                 double kcut, int n_targets, const int* targets, double* force, double* energy, double* virial)
+{
+    const double L[3] = {Lbox[0], Lbox[1], Lbox[2]};
+    const double V = L[0] * L[1] * L[2];
+    const int nt = targets ? n_targets : N;
+    double E = 0.0, W[6] = {0, 0, 0, 0, 0, 0};
+    const int want_ev = (energy != NULL) || (virial != NULL);
+    const int ix = (int)ceil(rcut / L[0]), iy = (int)ceil(rcut / L[1]), iz = (int)ceil(rcut / L[2]);
+    const double rc2 = rcut * rcut;
+    const double two_a_spi = 2.0 * alpha / sqrt(ORACLE_PI);
+
+    memset(force, 0, sizeof(double) * 3 * (size_t)nt);
+    /* real space */
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : E) reduction(+ : W[:6])
+    for (int t = 0; t < nt; ++t) {
+        const int i = targets ? targets[t] : t;
+        double fi[3] = {0, 0, 0};
+        for (int j = 0; j < N; ++j) {
+            double d0[3] = {postype[4 * i] - postype[4 * j], postype[4 * i + 1] - postype[4 * j + 1],
+                            postype[4 * i + 2] - postype[4 * j + 2]};
+            min_image(d0, L);
+            for (int a = -ix; a <= ix; ++a)
+                for (int b = -iy; b <= iy; ++b)
+                    for (int c = -iz; c <= iz; ++c) {
+                        if (i == j && a == 0 && b == 0 && c == 0) continue;
+                        const double d[3] = {d0[0] + a * L[0], d0[1] + b * L[1], d0[2] + c * L[2]};
+                        const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                        if (r2 >= rc2) continue;
+                        const double r = sqrt(r2);
+                        const double qq = q[i] * q[j] * ORACLE_INV_4PI;
+                        const double er = erfc(alpha * r);
+                        const double fdivr = qq * (er / r + two_a_spi * exp(-alpha * alpha * r2)) / r2;
+                        for (int k = 0; k < 3; ++k) fi[k] += fdivr * d[k];
+                        if (want_ev) {
+                            E += 0.5 * qq * er / r;
+                            W[0] += 0.5 * d[0] * d[0] * fdivr;
+                            W[1] += 0.5 * d[0] * d[1] * fdivr;
+                            W[2] += 0.5 * d[0] * d[2] * fdivr;
+                            W[3] += 0.5 * d[1] * d[1] * fdivr;
+                            W[4] += 0.5 * d[1] * d[2] * fdivr;
+                            W[5] += 0.5 * d[2] * d[2] * fdivr;
+                        }
+                    }
+        }
+        for (int k = 0; k < 3; ++k) force[3 * t + k] = fi[k];
+    }
+    /* reciprocal space over the half space k_z > 0 or (k_z == 0 and (k_y > 0 or (k_y == 0 and k_x > 0))) */
+    const int mx = (int)floor(kcut * L[0] / (2 * ORACLE_PI)), my = (int)floor(kcut * L[1] / (2 * ORACLE_PI)),
+              mz = (int)floor(kcut * L[2] / (2 * ORACLE_PI));
+    const double kc2 = kcut * kcut;
+    /* collect k vectors */
+    long nk = 0;
+    for (int c = 0; c <= mz; ++c)
+        for (int b = -my; b <= my; ++b)
+            for (int a = -mx; a <= mx; ++a) {
+                if (c == 0 && (b < 0 || (b == 0 && a <= 0))) continue;
+                const double kx = 2 * ORACLE_PI * a / L[0], ky = 2 * ORACLE_PI * b / L[1], kz = 2 * ORACLE_PI * c / L[2];
+                if (kx * kx + ky * ky + kz * kz > kc2) continue;
+                ++nk;
+            }
+    double* kv = malloc(sizeof(double) * 3 * (size_t)(nk > 0 ? nk : 1));
+    nk = 0;
+    for (int c = 0; c <= mz; ++c)
+        for (int b = -my; b <= my; ++b)
+            for (int a = -mx; a <= mx; ++a) {
+                if (c == 0 && (b < 0 || (b == 0 && a <= 0))) continue;
+                const double kx = 2 * ORACLE_PI * a / L[0], ky = 2 * ORACLE_PI * b / L[1], kz = 2 * ORACLE_PI * c / L[2];
+                if (kx * kx + ky * ky + kz * kz > kc2) continue;
+                kv[3 * nk] = kx;
+                kv[3 * nk + 1] = ky;
+                kv[3 * nk + 2] = kz;
+                ++nk;
+            }
+    double* Sre = calloc((size_t)(nk > 0 ? nk : 1), sizeof(double));
+    double* Sim = calloc((size_t)(nk > 0 ? nk : 1), sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (long k = 0; k < nk; ++k) {
+        double sr = 0, si = 0;
+        for (int j = 0; j < N; ++j) {
+            const double ph = kv[3 * k] * postype[4 * j] + kv[3 * k + 1] * postype[4 * j + 1] + kv[3 * k + 2] * postype[4 * j + 2];
+            sr += q[j] * cos(ph);
+            si -= q[j] * sin(ph);
+        }
+        Sre[k] = sr;
+        Sim[k] = si;
+    }
+    double Ek = 0.0, Wk[6] = {0, 0, 0, 0, 0, 0};
+    if (want_ev) {
+        for (long k = 0; k < nk; ++k) {
+            const double kx = kv[3 * k], ky = kv[3 * k + 1], kz = kv[3 * k + 2];
+            const double k2 = kx * kx + ky * ky + kz * kz;
+            const double g = exp(-k2 / (4 * alpha * alpha)) / k2;
+            const double s2 = Sre[k] * Sre[k] + Sim[k] * Sim[k];
+            /* half space: factor 2 */
+            const double e = g * s2 / V;
+            Ek += e;
+            const double fac = -2.0 / k2 * (1.0 + k2 / (4 * alpha * alpha));
+            Wk[0] += e * (1.0 + fac * kx * kx);
+            Wk[1] += e * (fac * kx * ky);
+            Wk[2] += e * (fac * kx * kz);
+            Wk[3] += e * (1.0 + fac * ky * ky);
+            Wk[4] += e * (fac * ky * kz);
+            Wk[5] += e * (1.0 + fac * kz * kz);
+        }
+    }
+#pragma omp parallel for schedule(static)
+    for (int t = 0; t < nt; ++t) {
+        const int j = targets ? targets[t] : t;
+        double f[3] = {0, 0, 0};
+        for (long k = 0; k < nk; ++k) {
+            const double kx = kv[3 * k], ky = kv[3 * k + 1], kz = kv[3 * k + 2];
+            const double k2 = kx * kx + ky * ky + kz * kz;
+            const double g = exp(-k2 / (4 * alpha * alpha)) / k2;
+            const double ph = kx * postype[4 * j] + ky * postype[4 * j + 1] + kz * postype[4 * j + 2];
+            /* Im(e^{i k r_j} S(k)), times 2 for the half space */
+            const double im = sin(ph) * Sre[k] + cos(ph) * Sim[k];
+            const double s = 2.0 * q[j] * g / V * im;
+            f[0] += s * kx;
+            f[1] += s * ky;
+            f[2] += s * kz;
+        }
+        for (int k = 0; k < 3; ++k) force[3 * t + k] += f[k];
+    }
+    if (want_ev) {
+        double q2 = 0, qs = 0;
+        for (int j = 0; j < N; ++j) {
+            q2 += q[j] * q[j];
+            qs += q[j];
+        }
+        const double Eself = -alpha / sqrt(ORACLE_PI) * ORACLE_INV_4PI * q2;
+        const double Ebg = -qs * qs / (8.0 * V * alpha * alpha);
+        if (energy) *energy = E + Ek + Eself + Ebg;
+        if (virial) {
+            for (int k = 0; k < 6; ++k) virial[k] = W[k] + Wk[k];
+            /* background term: E_bg scales as 1/V -> -dE/d eps_aa = E_bg */
+            virial[0] += Ebg;
+            virial[3] += Ebg;
+            virial[5] += Ebg;
+        }
+    }
+    free(kv);
+    free(Sre);
+    free(Sim);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Full ESP evaluation                                                        */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    int N;
+    const double* postype; /* Scalar4[N], box centred on the origin */
+    const double* q;
+    double L[3];
+    int n[3];              /* mesh Nx, Ny, Nz */
+    int P;                 /* stencil order */
+    int D;                 /* window degree */
+    const double* wcoef;   /* [P][D+1] monomials in z = 2t - 1 */
+    int n_segs;
+    const double* screen;  /* [n_segs][5] S(s) */
+    const double* dscreen; /* [n_segs][5] dS/ds */
+    const double* r_lo;
+    const double* dr_inv;
+    double rc;
+    int split_kind;        /* 0 PSWF (F from the table), 1 Gaussian erfc(kappa r) */
+    double kappa;
+    const unsigned int* n_ex; /* may be NULL */
+    const unsigned int* ex_list;
+    unsigned int ex_pitch;
+} oracle_esp_in;
+
+typedef struct {
+    double* force;        /* [N][4]  (fx, fy, fz, per-particle pair + exclusion energy) */
+    double* virial;       /* [6][pitch] per-particle pair + exclusion virial */
+    unsigned int virial_pitch;
+    double ext_energy;    /* reciprocal + self + background */
+    double ext_virial[6]; /* reciprocal + background */
+    double* mesh;         /* optional [Nz][Ny][Nx] spread charge */
+    double* phi;          /* optional [Nz][Ny][Nx] potential mesh */
+    double* green;        /* optional [Nz][Ny][Nx] influence function G(k) (full spectrum) */
+} oracle_esp_out;
+
+static double window_eval(const double* c, int D, double t, double* deriv)
+{
+    const double z = 2.0 * t - 1.0;
+    double v = c[D], dv = 0.0;
+    for (int j = D - 1; j >= 0; --j) {
+        dv = dv * z + v;
+        v = v * z + c[j];
+    }
+    if (deriv) *deriv = 2.0 * dv; /* d/dt = 2 d/dz */
+    return v;
+}
+
+/* |int W(d) e^{-i theta d} dd|^2 by 32-point Gauss per tap */
+static double window_fourier2(const double* wc, int P, int D, double theta)
+{
+    static const int NG = 32;
+    static double gx[32], gw[32];
+    static int init = 0;
+    if (!init) {
+        /* Gauss-Legendre on [0,1] by Newton */
+        for (int i = 0; i < NG; ++i) {
+            double z = cos(ORACLE_PI * (i + 0.75) / (NG + 0.5));
+            for (int it = 0; it < 100; ++it) {
+                double p0 = 1, p1 = z;
+                for (int k = 2; k <= NG; ++k) {
+                    double p2 = ((2.0 * k - 1) * z * p1 - (k - 1.0) * p0) / k;
+                    p0 = p1;
+                    p1 = p2;
+                }
+                double dp = NG * (z * p1 - p0) / (z * z - 1);
+                double dz = p1 / dp;
+                z -= dz;
+                if (fabs(dz) < 1e-16) {
+                    p0 = 1;
+                    p1 = z;
+                    for (int k = 2; k <= NG; ++k) {
+                        double p2 = ((2.0 * k - 1) * z * p1 - (k - 1.0) * p0) / k;
+                        p0 = p1;
+                        p1 = p2;
+                    }
+                    dp = NG * (z * p1 - p0) / (z * z - 1);
+                    gx[i] = 0.5 * (1 - z);
+                    gw[i] = 1.0 / ((1 - z * z) * dp * dp);
+                    break;
+                }
+            }
+        }
+        init = 1;
+    }
+    double re = 0, im = 0;
+    for (int k = 0; k < P; ++k)
+        for (int g = 0; g < NG; ++g) {
+            const double t = gx[g];
+            const double d = k - 0.5 * P + t;
+            const double w = gw[g] * window_eval(wc + (size_t)k * (D + 1), D, t, NULL);
+            re += w * cos(theta * d);
+            im -= w * sin(theta * d);
+        }
+    return re * re + im * im;
+}
+
+/* F(k) and dF/dk of the long-range part from the dS/ds table: F(k) = int_0^1 (-dS/ds) cos(k rc s) ds */
+static void split_F_from_table(const oracle_esp_in* in, double k, double* F, double* dF)
+{
+    if (in->split_kind == 1) {
+        const double a2 = in->kappa * in->kappa;
+        *F = exp(-k * k / (4 * a2));
+        *dF = -k / (2 * a2) * (*F);
+        return;
+    }
+    /* 3-point Gauss per segment */
+    static const double gx[3] = {0.1127016653792583, 0.5, 0.8872983346207417};
+    static const double gw[3] = {0.2777777777777778, 0.4444444444444444, 0.2777777777777778};
+    double f = 0, df = 0;
+    const double h = 1.0 / in->n_segs;
+    for (int s = 0; s < in->n_segs; ++s)
+        for (int g = 0; g < 3; ++g) {
+            const double t = gx[g];
+            const double x = in->r_lo[s] + t / in->dr_inv[s];
+            const double w = -gw[g] * h * horner5(in->dscreen, (unsigned)s, t);
+            f += w * cos(k * in->rc * x);
+            df -= w * in->rc * x * sin(k * in->rc * x);
+        }
+    *F = f;
+    *dF = df;
+}
+
+static double split_background(const oracle_esp_in* in)
+{
+    if (in->split_kind == 1) return 1.0 / (4 * in->kappa * in->kappa);
+    static const double gx[3] = {0.1127016653792583, 0.5, 0.8872983346207417};
+    static const double gw[3] = {0.2777777777777778, 0.4444444444444444, 0.2777777777777778};
+    double m2 = 0;
+    const double h = 1.0 / in->n_segs;
+    for (int s = 0; s < in->n_segs; ++s)
+        for (int g = 0; g < 3; ++g) {
+            const double t = gx[g];
+            const double x = in->r_lo[s] + t / in->dr_inv[s];
+            m2 += -gw[g] * h * horner5(in->dscreen, (unsigned)s, t) * x * x;
+        }
+    return in->rc * in->rc * m2 / 2.0;
+}
+
+typedef struct {
+    double key;
+    double F, dF;
+} fcache_t;
+
+static int cmp_double(const void* a, const void* b)
+{
+    const double x = *(const double*)a, y = *(const double*)b;
+    return (x > y) - (x < y);
+}
+
+int oracle_esp(const oracle_esp_in* in, oracle_esp_out* out)
+{
+    const int N = in->N, P = in->P, D = in->D;
+    const int nx = in->n[0], ny = in->n[1], nz = in->n[2];
+    const size_t M = (size_t)nx * ny * nz;
+    const double* L = in->L;
+    const double V = L[0] * L[1] * L[2];
+    const double h[3] = {L[0] / nx, L[1] / ny, L[2] / nz};
+    const int nm[3] = {nx, ny, nz};
+
+    /* ---- assignParticles: spread q_j W(m - g_j) ---- */
+    int* i0 = malloc(sizeof(int) * 3 * (size_t)N);
+    double* wts = malloc(sizeof(double) * 3 * (size_t)N * P);
+    double* dwts = malloc(sizeof(double) * 3 * (size_t)N * P);
+    for (int j = 0; j < N; ++j)
+        for (int a = 0; a < 3; ++a) {
+            double frac = in->postype[4 * j + a] / L[a] + 0.5; /* makeFraction for a box centred at 0 */
+            frac -= floor(frac);
+            const double g = frac * nm[a];
+            const double o = ceil(g - 0.5 * P);
+            const double t = o - (g - 0.5 * P);
+            int oi = (int)o;
+            oi = ((oi % nm[a]) + nm[a]) % nm[a];
+            i0[3 * j + a] = oi;
+            for (int k = 0; k < P; ++k) {
+                double dw;
+                wts[((size_t)j * 3 + a) * P + k] = window_eval(in->wcoef + (size_t)k * (D + 1), D, t, &dw);
+                dwts[((size_t)j * 3 + a) * P + k] = dw;
+            }
+        }
+    double complex* mesh = calloc(M, sizeof(double complex));
+    for (int j = 0; j < N; ++j) {
+        const double* wx = &wts[((size_t)j * 3 + 0) * P];
+        const double* wy = &wts[((size_t)j * 3 + 1) * P];
+        const double* wz = &wts[((size_t)j * 3 + 2) * P];
+        for (int c = 0; c < P; ++c) {
+            const int cz = (i0[3 * j + 2] + c) % nz;
+            for (int b = 0; b < P; ++b) {
+                const int cy = (i0[3 * j + 1] + b) % ny;
+                for (int a = 0; a < P; ++a) {
+                    const int cx = (i0[3 * j] + a) % nx;
+                    mesh[((size_t)cz * ny + cy) * nx + cx] += in->q[j] * wx[a] * wy[b] * wz[c];
+                }
+            }
+        }
+    }
+    if (out->mesh)
+        for (size_t m = 0; m < M; ++m) out->mesh[m] = creal(mesh[m]);
+
+    /* ---- updateMeshes: forward FFT ---- */
+    oracle_fft3d(nx, ny, nz, (double*)mesh, -1);
+
+    /* ---- influence function G(k) = F(|k|) / (k^2 |w(kx)|^2 |w(ky)|^2 |w(kz)|^2) ---- */
+    double* w2[3];
+    for (int a = 0; a < 3; ++a) {
+        w2[a] = malloc(sizeof(double) * nm[a]);
+        for (int i = 0; i < nm[a]; ++i) {
+            const int ki = (i <= nm[a] / 2) ? i : i - nm[a];
+            w2[a][i] = window_fourier2(in->wcoef, P, D, 2 * ORACLE_PI * ki / nm[a]);
+        }
+    }
+    /* distinct k^2 values -> F cache */
+    double* k2s = malloc(sizeof(double) * M);
+    for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x) {
+                const int kx = (x <= nx / 2) ? x : x - nx, ky = (y <= ny / 2) ? y : y - ny, kz = (z <= nz / 2) ? z : z - nz;
+                const double a = 2 * ORACLE_PI * kx / L[0], b = 2 * ORACLE_PI * ky / L[1], c = 2 * ORACLE_PI * kz / L[2];
+                k2s[((size_t)z * ny + y) * nx + x] = a * a + b * b + c * c;
+            }
+    double* uniq = malloc(sizeof(double) * M);
+    memcpy(uniq, k2s, sizeof(double) * M);
+    qsort(uniq, M, sizeof(double), cmp_double);
+    size_t nu = 0;
+    for (size_t m = 0; m < M; ++m)
+        if (nu == 0 || uniq[m] != uniq[nu - 1]) uniq[nu++] = uniq[m];
+    double* uF = malloc(sizeof(double) * nu);
+    double* udF = malloc(sizeof(double) * nu);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (size_t u = 0; u < nu; ++u) split_F_from_table(in, sqrt(uniq[u]), &uF[u], &udF[u]);
+
+    double Erec = 0, Wrec[6] = {0, 0, 0, 0, 0, 0};
+    for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x) {
+                const size_t m = ((size_t)z * ny + y) * nx + x;
+                const double k2 = k2s[m];
+                if (k2 == 0.0) {
+                    mesh[m] = 0;
+                    if (out->green) out->green[m] = 0;
+                    continue;
+                }
+                /* binary search in uniq */
+                size_t lo = 0, hi = nu - 1;
+                while (lo < hi) {
+                    size_t mid = (lo + hi) / 2;
+                    if (uniq[mid] < k2) lo = mid + 1; else hi = mid;
+                }
+                const double F = uF[lo], dF = udF[lo];
+                const double kk = sqrt(k2);
+                const double wprod = w2[0][x] * w2[1][y] * w2[2][z];
+                const double G = F / (k2 * wprod);
+                /* d/dk (F/k^2) / k  / wprod */
+                const double Gv = (dF / (k2 * kk) - 2.0 * F / (k2 * k2)) / wprod;
+                if (out->green) out->green[m] = G;
+                const double s2 = creal(mesh[m]) * creal(mesh[m]) + cimag(mesh[m]) * cimag(mesh[m]);
+                const int kx = (x <= nx / 2) ? x : x - nx, ky = (y <= ny / 2) ? y : y - ny, kz = (z <= nz / 2) ? z : z - nz;
+                const double k3[3] = {2 * ORACLE_PI * kx / L[0], 2 * ORACLE_PI * ky / L[1], 2 * ORACLE_PI * kz / L[2]};
+                const double e = 0.5 / V * s2;
+                Erec += e * G;
+                Wrec[0] += e * (G + Gv * k3[0] * k3[0]);
+                Wrec[1] += e * (Gv * k3[0] * k3[1]);
+                Wrec[2] += e * (Gv * k3[0] * k3[2]);
+                Wrec[3] += e * (G + Gv * k3[1] * k3[1]);
+                Wrec[4] += e * (Gv * k3[1] * k3[2]);
+                Wrec[5] += e * (G + Gv * k3[2] * k3[2]);
+                mesh[m] *= G;
+            }
+
+    /* ---- inverse FFT: phi(m) = (1/V) sum_k G M^(k) e^{i k x_m} ---- */
+    oracle_fft3d(nx, ny, nz, (double*)mesh, +1);
+    double* phi = malloc(sizeof(double) * M);
+    for (size_t m = 0; m < M; ++m) phi[m] = creal(mesh[m]) / V;
+    if (out->phi) memcpy(out->phi, phi, sizeof(double) * M);
+
+    /* ---- interpolateForces: F_j = q_j sum_m grad W(x_m - r_j) phi(m) ---- */
+    memset(out->force, 0, sizeof(double) * 4 * (size_t)N);
+    for (int c = 0; c < 6; ++c) memset(out->virial + (size_t)c * out->virial_pitch, 0, sizeof(double) * N);
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < N; ++j) {
+        const double* wx = &wts[((size_t)j * 3 + 0) * P];
+        const double* wy = &wts[((size_t)j * 3 + 1) * P];
+        const double* wz = &wts[((size_t)j * 3 + 2) * P];
+        const double* dx = &dwts[((size_t)j * 3 + 0) * P];
+        const double* dy = &dwts[((size_t)j * 3 + 1) * P];
+        const double* dz = &dwts[((size_t)j * 3 + 2) * P];
+        double ex = 0, ey = 0, ez = 0;
+        for (int c = 0; c < P; ++c) {
+            const int cz = (i0[3 * j + 2] + c) % nz;
+            for (int b = 0; b < P; ++b) {
+                const int cy = (i0[3 * j + 1] + b) % ny;
+                for (int a = 0; a < P; ++a) {
+                    const int cx = (i0[3 * j] + a) % nx;
+                    const double p = phi[((size_t)cz * ny + cy) * nx + cx];
+                    ex += dx[a] * wy[b] * wz[c] * p;
+                    ey += wx[a] * dy[b] * wz[c] * p;
+                    ez += wx[a] * wy[b] * dz[c] * p;
+                }
+            }
+        }
+        out->force[4 * j + 0] = in->q[j] * ex / h[0];
+        out->force[4 * j + 1] = in->q[j] * ey / h[1];
+        out->force[4 * j + 2] = in->q[j] * ez / h[2];
+    }
+
+    /* ---- computeRealSpaceCorrection: pair sum of q_i q_j L(r), all pairs within rc ---- */
+    /* exclusion lookup helper: pair (i,j) excluded? */
+#pragma omp parallel for schedule(dynamic, 8)
+    for (int i = 0; i < N; ++i) {
+        double f[3] = {0, 0, 0}, e = 0, v[6] = {0, 0, 0, 0, 0, 0};
+        for (int j = 0; j < N; ++j) {
+            if (j == i) continue;
+            int excluded = 0;
+            if (in->n_ex)
+                for (unsigned int k = 0; k < in->n_ex[i]; ++k)
+                    if (in->ex_list[(size_t)i * in->ex_pitch + k] == (unsigned)j) excluded = 1;
+            double d[3] = {in->postype[4 * i] - in->postype[4 * j], in->postype[4 * i + 1] - in->postype[4 * j + 1],
+                           in->postype[4 * i + 2] - in->postype[4 * j + 2]};
+            min_image(d, L);
+            const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            const double r = sqrt(r2);
+            const double qq = in->q[i] * in->q[j];
+            if (!excluded) {
+                if (r2 >= in->rc * in->rc || qq == 0.0) continue;
+                double Lr, nd;
+                lookup(in->screen, in->dscreen, in->r_lo, in->dr_inv, (unsigned)in->n_segs, in->rc, r, &Lr, &nd, NULL, NULL);
+                const double fdivr = qq * nd / r;
+                for (int a = 0; a < 3; ++a) f[a] += fdivr * d[a];
+                e += 0.5 * qq * Lr;
+                v[0] += 0.5 * d[0] * d[0] * fdivr;
+                v[1] += 0.5 * d[0] * d[1] * fdivr;
+                v[2] += 0.5 * d[0] * d[2] * fdivr;
+                v[3] += 0.5 * d[1] * d[1] * fdivr;
+                v[4] += 0.5 * d[1] * d[2] * fdivr;
+                v[5] += 0.5 * d[2] * d[2] * fdivr;
+            } else {
+                /* fixExclusions (ESPForceComputeGPU.cu:441-458): remove the mesh's M(r) = 1/(4 pi r) - L(r)
+                 * for the excluded pair, at any distance */
+                if (r2 < 1.0e-20) continue;
+                double Lr = 0, nd = 0;
+                if (r <= in->rc)
+                    lookup(in->screen, in->dscreen, in->r_lo, in->dr_inv, (unsigned)in->n_segs, in->rc, r, &Lr, &nd, NULL, NULL);
+                const double dE = qq * (Lr - ORACLE_INV_4PI / r);
+                const double fdivr = qq * (nd - ORACLE_INV_4PI / r2) / r;
+                for (int a = 0; a < 3; ++a) f[a] += fdivr * d[a];
+                e += 0.5 * dE;
+                v[0] += 0.5 * d[0] * d[0] * fdivr;
+                v[1] += 0.5 * d[0] * d[1] * fdivr;
+                v[2] += 0.5 * d[0] * d[2] * fdivr;
+                v[3] += 0.5 * d[1] * d[1] * fdivr;
+                v[4] += 0.5 * d[1] * d[2] * fdivr;
+                v[5] += 0.5 * d[2] * d[2] * fdivr;
+            }
+        }
+        out->force[4 * i + 0] += f[0];
+        out->force[4 * i + 1] += f[1];
+        out->force[4 * i + 2] += f[2];
+        out->force[4 * i + 3] += e;
+        for (int c = 0; c < 6; ++c) out->virial[(size_t)c * out->virial_pitch + i] += v[c];
+    }
+
+    /* ---- self and neutralising-background terms ---- */
+    double q2 = 0, qs = 0;
+    for (int j = 0; j < N; ++j) {
+        q2 += in->q[j] * in->q[j];
+        qs += in->q[j];
+    }
+    /* lim_{r->0} (1 - S)/r = -dS/dr(0) */
+    double self_coeff;
+    if (in->split_kind == 1)
+        self_coeff = 2.0 * in->kappa / sqrt(ORACLE_PI);
+    else
+        self_coeff = -horner5(in->dscreen, 0, 0.0) / in->rc;
+    const double Eself = -0.5 * ORACLE_INV_4PI * self_coeff * q2;
+    const double Ebg = -qs * qs / (2.0 * V) * split_background(in);
+    out->ext_energy = Erec + Eself + Ebg;
+    for (int c = 0; c < 6; ++c) out->ext_virial[c] = Wrec[c];
+    out->ext_virial[0] += Ebg;
+    out->ext_virial[3] += Ebg;
+    out->ext_virial[5] += Ebg;
+
+    free(i0);
+    free(wts);
+    free(dwts);
+    free(mesh);
+    for (int a = 0; a < 3; ++a) free(w2[a]);
+    free(k2s);
+    free(uniq);
+    free(uF);
+    free(udF);
+    free(phi);
+    return 0;
+}
+
+/* Window / influence pieces exposed for unit tests */
+double oracle_window_fourier2(const double* wc, int P, int D, double theta)
+{
+    return window_fourier2(wc, P, D, theta);
+}
+
+void oracle_window_eval(const double* wc, int P, int D, double t, double* w, double* dw)
+{
+    for (int k = 0; k < P; ++k) w[k] = window_eval(wc + (size_t)k * (D + 1), D, t, &dw[k]);
+}
+
+void oracle_split_F(const oracle_esp_in* in, double k, double* F, double* dF)
+{
+    split_F_from_table(in, k, F, dF);
+}
