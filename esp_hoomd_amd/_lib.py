@@ -31,6 +31,7 @@ ESP_COMPUTE_PAIR = 1
 ESP_COMPUTE_MESH = 2
 ESP_COMPUTE_ENERGY_VIRIAL = 4
 ESP_COMPUTE_ALL = 7
+ESP_N_STAGES = 12
 
 
 class EspParams(C.Structure):
@@ -127,6 +128,10 @@ def load() -> C.CDLL:
     _sig(lib, "esp_get_split_table", C.c_int, [_VP, C.POINTER(C.c_uint), _DP, _DP, _DP, _DP])
     _sig(lib, "esp_set_split_table", C.c_int, [_VP, C.c_uint, _DP, _DP, _DP, _DP])
     _sig(lib, "esp_get_window", C.c_int, [_VP, C.POINTER(C.c_uint), C.POINTER(C.c_uint), _DP])
+    _sig(lib, "esp_set_profiling", C.c_int, [_VP, C.c_int])
+    _sig(lib, "esp_get_stage_times", C.c_int, [_VP, _DP, C.POINTER(C.c_uint)])
+    _sig(lib, "esp_reset_stage_times", C.c_int, [_VP])
+    _sig(lib, "esp_stage_name", C.c_char_p, [C.c_int])
     _sig(lib, "esp_build_split_table", C.c_int, [C.c_int, C.c_double, C.c_double, C.c_uint, _DP, _DP, _DP, _DP])
     _sig(lib, "esp_split_eval", C.c_int, [C.c_int, C.c_double, C.c_double, C.c_int, _DP, _DP, _DP])
     _sig(lib, "esp_split_fourier", C.c_int, [C.c_int, C.c_double, C.c_double, C.c_int, _DP, _DP, _DP])

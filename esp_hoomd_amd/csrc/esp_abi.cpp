@@ -345,6 +345,34 @@ int esp_get_window(esp_ctx* ctx, unsigned int* P, unsigned int* D, double* coef)
                    });
     }
 
+// ---- tracing ------------------------------------------------------------------------------
+int esp_set_profiling(esp_ctx* ctx, int enable)
+    {
+    if (!ctx)
+        return ESP_ERR_ARG;
+    ctx->solver.set_profiling(enable != 0);
+    return ESP_OK;
+    }
+
+int esp_get_stage_times(esp_ctx* ctx, double* ms, unsigned int* calls)
+    {
+    if (!ctx || !ms || !calls)
+        return ESP_ERR_ARG;
+    return guarded(ctx, [&] { ctx->solver.stage_times(ms, calls); });
+    }
+
+int esp_reset_stage_times(esp_ctx* ctx)
+    {
+    if (!ctx)
+        return ESP_ERR_ARG;
+    return guarded(ctx, [&] { ctx->solver.reset_stage_times(); });
+    }
+
+const char* esp_stage_name(int i)
+    {
+    return esp::EspSolver::stage_name(i);
+    }
+
 // ---- host-only mathematics -------------------------------------------------------------
 int esp_build_split_table(int split, double kappa, double rcut, unsigned int n_segs, double* screen, double* dscreen, double* r_lo,
                           double* dr_inv)

@@ -234,12 +234,13 @@ __device__ __forceinline__ void split_lookup(const R* __restrict__ tab, unsigned
         negdLdr = R(0);
         return;
         }
-    R s = r * inv_rc;
-    s = s < R(0) ? R(0) : (s > R(1) - R(1.0e-9) ? R(1) - R(1.0e-9) : s);
+    // segment from the clamped reduced distance, offset t from the unclamped one
+    const R su = r * inv_rc;
+    R s = su < R(0) ? R(0) : (su > R(1) - R(1.0e-9) ? R(1) - R(1.0e-9) : su);
     unsigned int seg = static_cast<unsigned int>(s * R(n_segs));
     seg = seg >= n_segs ? n_segs - 1u : seg;
     const R* c = tab + size_t(seg) * 12u;
-    const R t = (s - c[0]) * c[1];
+    const R t = (su - c[0]) * c[1];
     R S = c[6];
     S = c[5] + t * S;
     S = c[4] + t * S;

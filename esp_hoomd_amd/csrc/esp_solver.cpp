@@ -81,7 +81,95 @@ FFTPlan make_fft_plan(int n)
     }
 
 EspSolver::EspSolver() = default;
-EspSolver::~EspSolver() = default;
+
+EspSolver::~EspSolver()
+    {
+    for (auto& v : m_ev_pool)
+        for (hipEvent_t e : v)
+            (void)hipEventDestroy(e);
+    for (auto& v : m_ev_pending)
+        for (hipEvent_t e : v)
+            (void)hipEventDestroy(e);
+    }
+
+const char* EspSolver::stage_name(int i)
+    {
+    static const char* names[N_STAGES] = {"pair",   "bin",          "scan",       "scatter", "spread",  "fft_x_r2c",
+                                          "fft_y", "fft_z_convolve", "ifft_y", "ifft_x_c2r", "gather", "finalize"};
+    return (i >= 0 && i < N_STAGES) ? names[i] : "";
+    }
+
+void EspSolver::set_profiling(bool on)
+    {
+    m_profile = on;
+    }
+
+void EspSolver::reset_stage_times()
+    {
+    harvest();
+    for (int i = 0; i < N_STAGES; ++i)
+        {
+        m_stage_ms[i] = 0.0;
+        m_stage_calls[i] = 0;
+        }
+    }
+
+//! Record stage boundary b (0..N_STAGES) of the current call on stream s.
+void EspSolver::mark(int b, hipStream_t s)
+    {
+    if (!m_profile)
+        return;
+    if (b == 0)
+        {
+        if (m_ev_pool.empty())
+            {
+            std::vector<hipEvent_t> v(N_STAGES + 1);
+            for (hipEvent_t& e : v)
+                check_hip(hipEventCreate(&e), "hipEventCreate");
+            m_ev_pool.push_back(std::move(v));
+            }
+        m_ev_pending.push_back(std::move(m_ev_pool.back()));
+        m_ev_pool.pop_back();
+        m_ev_marked.push_back(false);
+        m_ev_cur = &m_ev_pending.back();
+        // boundaries of skipped stages are recorded at the next boundary
+        }
+    if (!m_ev_cur)
+        return;
+    check_hip(hipEventRecord((*m_ev_cur)[b], s), "hipEventRecord");
+    if (b == N_STAGES)
+        m_ev_cur = nullptr;
+    }
+
+void EspSolver::harvest()
+    {
+    for (auto& v : m_ev_pending)
+        {
+        check_hip(hipEventSynchronize(v[N_STAGES]), "hipEventSynchronize");
+        for (int i = 0; i < N_STAGES; ++i)
+            {
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, v[i], v[i + 1]) == hipSuccess)
+                {
+                m_stage_ms[i] += ms;
+                m_stage_calls[i] += 1;
+                }
+            }
+        m_ev_pool.push_back(std::move(v));
+        }
+    m_ev_pending.clear();
+    m_ev_marked.clear();
+    }
+
+void EspSolver::stage_times(double* ms, unsigned int* calls)
+    {
+    harvest();
+    for (int i = 0; i < N_STAGES; ++i)
+        {
+        ms[i] = m_stage_ms[i];
+        calls[i] = m_stage_calls[i];
+        }
+    }
 
 void EspSolver::set_params(const SolverParams& p)
     {
@@ -371,28 +459,39 @@ void EspSolver::run_mesh(const SystemView& sys, const OutputView& out, unsigned 
     const int P = int(m_p.order);
     const int D = int(m_D);
     const unsigned int nqb = (n + 255) / 256;
+    mark(1, s);
     check_hip(kernel::bin_particles(n, sys.group_idx, sys.postype, sys.charge, g, P, m_tile_count.as<unsigned int>(),
                                     m_key.as<unsigned int>(), m_slot.as<unsigned int>(), m_qpart.as<double>(), s),
               "bin_particles");
+    mark(2, s);
     check_hip(kernel::scan_tiles(m_n_tiles, m_tile_count.as<unsigned int>(), m_tile_start.as<unsigned int>(), s), "scan_tiles");
+    mark(3, s);
     check_hip(kernel::scatter_particles<R>(n, sys.group_idx, sys.postype, sys.charge, g, P, m_key.as<unsigned int>(),
                                            m_slot.as<unsigned int>(), m_tile_start.as<unsigned int>(), m_rec.get(),
                                            m_sidx.as<SortedIdx>(), s),
               "scatter_particles");
+    mark(4, s);
     check_hip(kernel::spread<R>(g, P, D, m_n_tiles, m_tile_start.as<unsigned int>(), m_rec.get(), m_sidx.as<SortedIdx>(), m_wcoef.get(),
                                 m_mesh.get(), s),
               "spread");
+    mark(5, s);
     check_hip(kernel::fft_forward_x<R>(g, m_plan[0], m_tw[0].get(), m_mesh.get(), m_spec.get(), s), "fft_forward_x");
+    mark(6, s);
     check_hip(kernel::fft_y<R>(g, m_plan[1], m_tw[1].get(), m_spec.get(), -1, s), "fft_y");
+    mark(7, s);
     check_hip(kernel::fft_z_convolve<R>(g, m_plan[2], m_tw[2].get(), m_spec.get(), m_G.as<double>(), m_Gv.as<double>(), want_ev ? 1 : 0,
                                         m_evpart.as<double>(), s),
               "fft_z_convolve");
+    mark(8, s);
     check_hip(kernel::fft_y<R>(g, m_plan[1], m_tw[1].get(), m_spec.get(), +1, s), "fft_y inverse");
+    mark(9, s);
     const double V = g.Lx * g.Ly * g.Lz;
     check_hip(kernel::fft_inverse_x<R>(g, m_plan[0], m_tw[0].get(), m_spec.get(), 1.0 / V, m_phi.get(), s), "fft_inverse_x");
+    mark(10, s);
     check_hip(kernel::gather<R>(g, P, D, m_n_tiles, m_tile_start.as<unsigned int>(), m_rec.get(), m_sidx.as<SortedIdx>(), m_wcoef.get(),
                                 m_phi.get(), out.force, s),
               "gather");
+    mark(11, s);
     double* evout = out.ext_energy_virial ? out.ext_energy_virial : m_ev_internal.as<double>();
     check_hip(kernel::finalize(int(kernel::fft_z_blocks(g, sizeof(R) == 8)), m_evpart.as<double>(), int(nqb), m_qpart.as<double>(),
                                0.5 / V, m_self_coeff, m_bg_coeff, V, want_ev ? 1 : 0, evout, m_qsum.as<double>(), s),
@@ -437,6 +536,7 @@ void EspSolver::compute(const SystemView& sys, const OutputView& out, hipStream_
     ensure_particles(std::max(n, 1u));
     const bool fp64 = m_p.precision == 0;
     const unsigned int flags = out.flags ? out.flags : COMPUTE_ALL;
+    mark(0, s);
     if (flags & COMPUTE_PAIR)
         {
         if (fp64)
@@ -459,6 +559,10 @@ void EspSolver::compute(const SystemView& sys, const OutputView& out, hipStream_
         else
             run_mesh<float>(sys, out, n, s, want_ev);
         }
+    else
+        for (int b = 1; b <= 11; ++b)
+            mark(b, s);
+    mark(N_STAGES, s);
     }
 
 void EspSolver::charge_sums(double& q, double& q2)

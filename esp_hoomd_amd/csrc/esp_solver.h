@@ -146,7 +146,25 @@ class EspSolver
     const double* device_green() const { return m_G.as<double>(); }
     size_t memory_bytes() const;
 
+    //! Per-stage HIP-event timing (stage boundaries recorded on the compute stream).
+    static constexpr int N_STAGES = 12;
+    static const char* stage_name(int i);
+    void set_profiling(bool on);
+    //! Totals in ms and call counts per stage since the last reset (synchronises).
+    void stage_times(double* ms, unsigned int* calls);
+    void reset_stage_times();
+
     private:
+    void mark(int boundary, hipStream_t s);
+    void harvest();
+    bool m_profile = false;
+    std::vector<std::vector<hipEvent_t>> m_ev_pool;   // free event sets (N_STAGES + 1 each)
+    std::vector<std::vector<hipEvent_t>> m_ev_pending; // recorded, not yet harvested
+    std::vector<hipEvent_t>* m_ev_cur = nullptr;
+    std::vector<bool> m_ev_marked;
+    double m_stage_ms[N_STAGES] = {};
+    unsigned int m_stage_calls[N_STAGES] = {};
+
     void initialize(const SystemView& sys, hipStream_t s);
     void build_influence(hipStream_t s);
     void ensure_particles(unsigned int n);

@@ -614,19 +614,32 @@ __global__ void __launch_bounds__(256) k_fft_z_conv(MeshGeom g,
         if (want_ev)
             {
             const double s2 = double(c.x) * double(c.x) + double(c.y) * double(c.y);
-            const double wgt = (x == 0 || 2 * x == g.nx) ? 1.0 : 2.0;
-            const double ee = wgt * s2;
             const double gv = Gv[m];
             const int kz_i = (z <= nz / 2) ? z : z - nz;
             const double kx = two_pi * double(x) / g.Lx;
             const double kz = two_pi * double(kz_i) / g.Lz;
-            e += ee * gk;
-            v0 += ee * (gk + gv * kx * kx);
-            v1 += ee * (gv * kx * ky);
-            v2 += ee * (gv * kx * kz);
-            v3 += ee * (gk + gv * ky * ky);
-            v4 += ee * (gv * ky * kz);
-            v5 += ee * (gk + gv * kz * kz);
+            e += s2 * gk;
+            v0 += s2 * (gk + gv * kx * kx);
+            v1 += s2 * (gv * kx * ky);
+            v2 += s2 * (gv * kx * kz);
+            v3 += s2 * (gk + gv * ky * ky);
+            v4 += s2 * (gv * ky * kz);
+            v5 += s2 * (gk + gv * kz * kz);
+            if (x != 0 && 2 * x != g.nx)
+                {
+                // the conjugate mode (-k) outside the half spectrum; a Nyquist index keeps
+                // its +N/2 wavenumber (k ordering i <= N/2 -> +i, ESPForceComputeGPU.cu:173-175)
+                const double kxp = -kx;
+                const double kyp = (2 * y == ny) ? ky : -ky;
+                const double kzp = (2 * z == nz) ? kz : -kz;
+                e += s2 * gk;
+                v0 += s2 * (gk + gv * kxp * kxp);
+                v1 += s2 * (gv * kxp * kyp);
+                v2 += s2 * (gv * kxp * kzp);
+                v3 += s2 * (gk + gv * kyp * kyp);
+                v4 += s2 * (gv * kyp * kzp);
+                v5 += s2 * (gk + gv * kzp * kzp);
+                }
             }
         const R gr = R(gk);
         c.x *= gr;

@@ -470,6 +470,13 @@ WindowTable build_window(unsigned int P, unsigned int D, double cw)
             w.coef[size_t(km) * (D + 1) + j] = avg * ((j % 2) ? -1.0 : 1.0);
             }
         }
+    // renormalise the fitted pieces exactly: int_0^1 z^j dt = 1/(j+1) for even j, 0 for odd j
+    double total = 0.0;
+    for (unsigned int k = 0; k < P; ++k)
+        for (unsigned int j = 0; j <= D; j += 2)
+            total += w.coef[size_t(k) * (D + 1) + j] / double(j + 1);
+    for (double& c : w.coef)
+        c /= total;
     return w;
     }
 

@@ -164,6 +164,20 @@ class Solver:
         _lib.check(_lib.load().esp_device_synchronize())
         return {"force": system.force.numpy(), "virial": system.virial.numpy()[:, : system.N]}
 
+    def set_profiling(self, on: bool = True) -> None:
+        _lib.check(_lib.load().esp_set_profiling(self._ctx, 1 if on else 0), self._ctx)
+
+    def reset_stage_times(self) -> None:
+        _lib.check(_lib.load().esp_reset_stage_times(self._ctx), self._ctx)
+
+    def stage_times(self):
+        """{stage name: (total ms, calls)} since the last reset (synchronises)."""
+        lib = _lib.load()
+        ms = np.zeros(_lib.ESP_N_STAGES)
+        calls = (C.c_uint * _lib.ESP_N_STAGES)()
+        _lib.check(lib.esp_get_stage_times(self._ctx, _lib.dptr(ms), calls), self._ctx)
+        return {lib.esp_stage_name(i).decode(): (float(ms[i]), int(calls[i])) for i in range(_lib.ESP_N_STAGES)}
+
     def info(self) -> _lib.EspInfo:
         inf = _lib.EspInfo()
         _lib.check(_lib.load().esp_get_info(self._ctx, C.byref(inf)), self._ctx)

@@ -159,6 +159,16 @@ int esp_set_split_table(esp_ctx* ctx, unsigned int n_segs, const double* screen,
  * PSWF_STENCIL_COEFFS / _X_LO / _DX_INV, PSWF_Coeffs.h:9606-9624) */
 int esp_get_window(esp_ctx* ctx, unsigned int* P, unsigned int* D, double* coef);
 
+/* ---- tracing -------------------------------------------------------------------------- */
+/* Per-stage HIP-event timing on the compute stream (no counterpart in the reference, which
+ * only brackets kernels with HOOMD Autotuner begin/end, ESPForceComputeGPU.cc:230-250). */
+#define ESP_N_STAGES 12
+int esp_set_profiling(esp_ctx* ctx, int enable);
+/* totals in ms and call counts for the ESP_N_STAGES stages since the last reset; synchronises */
+int esp_get_stage_times(esp_ctx* ctx, double* ms, unsigned int* calls);
+int esp_reset_stage_times(esp_ctx* ctx);
+const char* esp_stage_name(int i);
+
 /* ---- host-only table mathematics (no GPU needed) ------------------------------------- */
 /* table generator (replaces generate_pswf_table.py:540-595 with a valid split, S(0) = 1) */
 int esp_build_split_table(int split, double kappa, double rcut, unsigned int n_segs, double* screen, double* dscreen, double* r_lo,

@@ -61,11 +61,14 @@ static void lookup(const double* screen, const double* dscreen, const double* r_
         if (dS) *dS = 0.0;
         return;
     }
-    double s = r / rc;
+    /* segment from the clamped reduced distance, offset from the unclamped one
+     * (EvaluatorPairPSWF.h:126-133 uses the unclamped r in t) */
+    const double su = r / rc;
+    double s = su;
     s = s < 0.0 ? 0.0 : (s > 1.0 - 1.0e-9 ? 1.0 - 1.0e-9 : s);
     unsigned int seg = (unsigned int)(s * (double)n_segs);
     if (seg >= n_segs) seg = n_segs - 1u;
-    const double t = (s - r_lo[seg]) * dr_inv[seg];
+    const double t = (su - r_lo[seg]) * dr_inv[seg];
     const double Sv = horner5(screen, seg, t);
     const double dSv = horner5(dscreen, seg, t) / rc; /* dS/dr from dS/ds */
     const double inv_4pi_r = ORACLE_INV_4PI / r;

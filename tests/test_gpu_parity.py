@@ -1,0 +1,105 @@
+"""GPU parity: the HIP path against the CPU oracle on identical seeded inputs.
+
+FP64 mode: forces, per-particle energies and virials, external energy and virial within
+1e-10 relative RMS.  Mixed mode: within 2e-6 of the FP64 oracle at the same parameters.
+"""
+import numpy as np
+import pytest
+
+from conftest import random_electrolyte, rel_rms
+from esp_cases import make_params, oracle_for, run_gpu
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+
+
+def _compare(res, ref, tol):
+    assert rel_rms(res["force"][:, :3], ref["force"][:, :3]) < tol
+    assert rel_rms(res["force"][:, 3], ref["force"][:, 3]) < tol
+    assert rel_rms(res["virial"], ref["virial"]) < tol
+    assert abs(res["ext"][0] - ref["ext_energy"]) <= tol * abs(ref["ext_energy"])
+    assert rel_rms(res["ext"][1:7], ref["ext_virial"]) < tol
+    assert abs(res["energy"] - ref["energy"]) <= tol * abs(ref["energy"])
+
+
+@pytest.mark.parametrize("N,L,mesh,P,c,rc", [
+    (100, 8.0, 32, 8, 12.0, 3.0),
+    (300, 10.0, (24, 30, 20), 7, 10.0, 3.5),
+    (64, 6.0, 16, 4, 6.0, 2.5),
+    (500, 12.0, 32, 12, 14.0, 3.0),
+])
+def test_fp64_pswf_matches_oracle(N, L, mesh, P, c, rc):
+    pos, q = random_electrolyte(N, L, seed=N)
+    box = (L, L, L)
+    params = make_params(mesh, P, c, rc)
+    solver, _, res = run_gpu(params, pos, q, box)
+    ref = oracle_for(solver, box, params).compute(pos, q)
+    print(f"case N={N} mesh={mesh} P={P}", flush=True)
+    _compare(res, ref, FP64_TOL)
+
+
+def test_fp64_gaussian_split_matches_oracle():
+    pos, q = random_electrolyte(200, 9.0, seed=7)
+    box = (9.0, 9.0, 9.0)
+    params = make_params(36, 9, None, 4.0, split=1, kappa=1.6)
+    solver, _, res = run_gpu(params, pos, q, box)
+    ref = oracle_for(solver, box, params).compute(pos, q)
+    _compare(res, ref, FP64_TOL)
+
+
+def test_fp64_rectangular_box_and_exclusions():
+    pos, q = random_electrolyte(150, 9.0, seed=3)
+    box = (9.0, 10.0, 11.0)
+    excl = [(0, 1), (2, 3), (4, 9), (10, 100)]
+    params = make_params((30, 32, 36), 6, 10.0, 3.0)
+    solver, _, res = run_gpu(params, pos, q, box, exclusions=excl)
+    ref = oracle_for(solver, box, params).compute(pos, q, exclusions=excl)
+    _compare(res, ref, FP64_TOL)
+
+
+def test_fp64_group_and_zero_charges():
+    pos, q = random_electrolyte(120, 8.0, seed=5)
+    q[::7] = 0.0
+    q -= q.mean()  # keep neutral with fractional charges
+    box = (8.0, 8.0, 8.0)
+    params = make_params(32, 8, 12.0, 3.0)
+    solver, _, res = run_gpu(params, pos, q, box, group=np.random.default_rng(0).permutation(120).astype(np.uint32))
+    ref = oracle_for(solver, box, params).compute(pos, q)
+    _compare(res, ref, FP64_TOL)
+    qs, q2s = solver.charge_sums()
+    assert abs(qs - q.sum()) < 1e-12 and abs(q2s - (q * q).sum()) < 1e-10
+
+
+def test_fp64_non_neutral_background():
+    pos, q = random_electrolyte(90, 8.0, seed=11)
+    q[0] = 2.0  # net charge
+    box = (8.0, 8.0, 8.0)
+    params = make_params(32, 8, 12.0, 3.0)
+    solver, _, res = run_gpu(params, pos, q, box)
+    ref = oracle_for(solver, box, params).compute(pos, q)
+    _compare(res, ref, FP64_TOL)
+
+
+@pytest.mark.parametrize("P", [6, 8, 10])
+def test_mixed_precision_close_to_fp64_oracle(P):
+    pos, q = random_electrolyte(400, 12.0, seed=P)
+    box = (12.0, 12.0, 12.0)
+    params = make_params(40, P, 12.0, 3.5, precision=1)
+    solver, _, res = run_gpu(params, pos, q, box)
+    ref = oracle_for(solver, box, params).compute(pos, q)
+    err = rel_rms(res["force"][:, :3], ref["force"][:, :3])
+    print(f"mixed P={P} force err {err:.2e}", flush=True)
+    assert err < 2e-6
+    assert abs(res["energy"] - ref["energy"]) < 1e-5 * abs(ref["energy"])
+
+
+def test_empty_and_tiny_systems():
+    params = make_params(16, 4, 6.0, 2.5)
+    pos, q = random_electrolyte(2, 6.0, seed=1)
+    solver, _, res = run_gpu(params, pos, q, (6.0, 6.0, 6.0))
+    ref = oracle_for(solver, (6.0, 6.0, 6.0), params).compute(pos, q)
+    _compare(res, ref, FP64_TOL)
+    # all charges zero: forces and energies vanish
+    solver2, _, res2 = run_gpu(params, pos, np.zeros(2), (6.0, 6.0, 6.0))
+    assert np.all(res2["force"] == 0.0) and res2["ext"][0] == 0.0
