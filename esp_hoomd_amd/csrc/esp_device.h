@@ -222,6 +222,39 @@ __device__ C2<R>* lds_fft(C2<R>* a,
     }
 
 // ---------------------------------------------------------------------------
+// Mesh accumulation
+// ---------------------------------------------------------------------------
+//! Accumulator types: FP64 mode accumulates the mesh in FP64; the mixed mode accumulates it
+//! in fixed point (int32 in LDS, int64 in HBM), which is exact and order independent.
+template<typename R> struct Acc;
+template<> struct Acc<double>
+    {
+    using lds_t = double;
+    using glob_t = double;
+    };
+template<> struct Acc<float>
+    {
+    using lds_t = int;
+    using glob_t = unsigned long long;
+    };
+
+__device__ __forceinline__ void lds_add(double* p, double v)
+    {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+__device__ __forceinline__ void lds_add(int* p, float v)
+    {
+    __hip_atomic_fetch_add(p, __float2int_rn(v * float(FIXED_SCALE)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+__device__ __forceinline__ void glob_add(double* p, double v)
+    {
+    unsafeAtomicAdd(p, v);
+    }
+__device__ __forceinline__ void glob_add(unsigned long long* p, int v)
+    {
+    atomicAdd(p, static_cast<unsigned long long>(static_cast<long long>(v)));
+    }
+// ---------------------------------------------------------------------------
 // Split-table lookup (EvaluatorPairPSWF.h:113-143, reduced s = r / rc).
 // Record per segment: r_lo, dr_inv, S c0..c4, dS/ds c0..c4 (12 reals).
 // ---------------------------------------------------------------------------

@@ -43,6 +43,7 @@ hipError_t scatter_particles(unsigned int n,
 
 // ---- spread / gather ------------------------------------------------------
 size_t spread_lds_bytes(const MeshGeom& g, int P, int D, bool fp64);
+size_t gather_lds_bytes(const MeshGeom& g, int P, int D, bool fp64);
 template<typename R>
 hipError_t spread(const MeshGeom& g, int P, int D, unsigned int n_tiles, const unsigned int* start, const void* rec,
                   const SortedIdx* sidx, const void* wcoef, void* mesh, hipStream_t s);
@@ -82,8 +83,11 @@ struct PairArgs
     const unsigned int* n_ex;
     const unsigned int* ex_list;
     unsigned int ex_pitch;
-    const void* split_tab;   // [n_segs][12] reals
+    const void* split_tab;   // [n_segs][12] reals (reference-format lookup)
     unsigned int n_segs;
+    const void* compact_tab; // [n_compact][compact_deg + 1] reals of S(s), or null
+    unsigned int n_compact;
+    unsigned int compact_deg;
     double rc;
     double* force;           // Scalar4[N], overwritten
     double* virial;          // [6][pitch], overwritten (may be null)

@@ -90,6 +90,13 @@ EspSolver::~EspSolver()
     for (auto& v : m_ev_pending)
         for (hipEvent_t e : v)
             (void)hipEventDestroy(e);
+    if (m_side)
+        {
+        (void)hipStreamSynchronize(m_side);
+        (void)hipStreamDestroy(m_side);
+        (void)hipEventDestroy(m_ev_fork);
+        (void)hipEventDestroy(m_ev_join);
+        }
     }
 
 const char* EspSolver::stage_name(int i)
@@ -114,7 +121,8 @@ void EspSolver::reset_stage_times()
         }
     }
 
-//! Record stage boundary b (0..N_STAGES) of the current call on stream s.
+//! Record mesh-pipeline boundary b (0..N_STAGES) of the current call on stream s; stage i in
+//! 1..N_STAGES-1 spans boundaries i and i+1.  Boundary 0 opens a new event set.
 void EspSolver::mark(int b, hipStream_t s)
     {
     if (!m_profile)
@@ -123,7 +131,7 @@ void EspSolver::mark(int b, hipStream_t s)
         {
         if (m_ev_pool.empty())
             {
-            std::vector<hipEvent_t> v(N_STAGES + 1);
+            std::vector<hipEvent_t> v(N_STAGES + 3);
             for (hipEvent_t& e : v)
                 check_hip(hipEventCreate(&e), "hipEventCreate");
             m_ev_pool.push_back(std::move(v));
@@ -132,7 +140,6 @@ void EspSolver::mark(int b, hipStream_t s)
         m_ev_pool.pop_back();
         m_ev_marked.push_back(false);
         m_ev_cur = &m_ev_pending.back();
-        // boundaries of skipped stages are recorded at the next boundary
         }
     if (!m_ev_cur)
         return;
@@ -141,20 +148,38 @@ void EspSolver::mark(int b, hipStream_t s)
         m_ev_cur = nullptr;
     }
 
+//! Record the start (end = false) or the end of the pair stage on the stream it runs on.
+void EspSolver::pair_mark(bool end, hipStream_t s)
+    {
+    if (!m_profile || !m_ev_cur)
+        return;
+    check_hip(hipEventRecord((*m_ev_cur)[N_STAGES + (end ? 2 : 1)], s), "hipEventRecord");
+    if (end)
+        m_ev_marked.back() = true;
+    }
+
 void EspSolver::harvest()
     {
-    for (auto& v : m_ev_pending)
+    for (size_t c = 0; c < m_ev_pending.size(); ++c)
         {
+        std::vector<hipEvent_t>& v = m_ev_pending[c];
         check_hip(hipEventSynchronize(v[N_STAGES]), "hipEventSynchronize");
-        for (int i = 0; i < N_STAGES; ++i)
+        float ms = 0.0f;
+        if (m_ev_marked[c])
             {
-            float ms = 0.0f;
+            check_hip(hipEventSynchronize(v[N_STAGES + 2]), "hipEventSynchronize");
+            if (hipEventElapsedTime(&ms, v[N_STAGES + 1], v[N_STAGES + 2]) == hipSuccess)
+                {
+                m_stage_ms[0] += ms;
+                m_stage_calls[0] += 1;
+                }
+            }
+        for (int i = 1; i < N_STAGES; ++i)
             if (hipEventElapsedTime(&ms, v[i], v[i + 1]) == hipSuccess)
                 {
                 m_stage_ms[i] += ms;
                 m_stage_calls[i] += 1;
                 }
-            }
         m_ev_pool.push_back(std::move(v));
         }
     m_ev_pending.clear();
@@ -285,12 +310,19 @@ void EspSolver::initialize(const SystemView& sys, hipStream_t s)
     g.ny = int(m_p.ny);
     g.nz = int(m_p.nz);
     g.nxh = g.nx / 2 + 1;
+    // largest tile edge in {8, 4, 2} whose LDS box fits 64 KiB; 4 on small meshes for parallelism
     int T = 8;
-    const size_t el = fp64 ? 8 : 4;
-    if (size_t(T + P - 1) * (T + P - 1) * (T + P - 1) * el > 64 * 1024)
-        T = 4;
+    for (; T > 2; T /= 2)
+        {
+        g.B = T + int(P) - 1;
+        if (kernel::gather_lds_bytes(g, int(P), 0, fp64) <= 64 * 1024)
+            break;
+        }
     if (T == 8 && size_t((g.nx + 7) / 8) * ((g.ny + 7) / 8) * ((g.nz + 7) / 8) < 256)
         T = 4;
+    g.B = T + int(P) - 1;
+    if (kernel::gather_lds_bytes(g, int(P), 0, fp64) > 64 * 1024)
+        throw EspError(STATUS_UNSUPPORTED, "stencil order too large for the LDS tile at this precision");
     g.T = T;
     g.tx = (g.nx + T - 1) / T;
     g.ty = (g.ny + T - 1) / T;
@@ -326,6 +358,19 @@ void EspSolver::initialize(const SystemView& sys, hipStream_t s)
         upload_as<double>(m_split_tab, rec);
     else
         upload_as<float>(m_split_tab, rec);
+    // compact LDS-resident table of S for the pair kernel, unless an external
+    // reference-format table was supplied (then the reference lookup is used verbatim)
+    if (!m_external_table)
+        {
+        const math::SplitKind kind = m_p.split == 1 ? math::SplitKind::GAUSSIAN : math::SplitKind::PSWF;
+        math::SplitFunction split(kind, m_p.kappa, m_p.rcut);
+        std::vector<double> ct;
+        math::build_compact_split(split, COMPACT_SEGS, COMPACT_DEG, ct);
+        if (fp64)
+            upload_as<double>(m_compact, ct);
+        else
+            upload_as<float>(m_compact, ct);
+        }
     if (fp64)
         upload_as<double>(m_wcoef, m_window.coef);
     else
@@ -444,6 +489,9 @@ template<typename R> void EspSolver::run_pair(const SystemView& sys, const Outpu
     a.ex_pitch = sys.ex_pitch;
     a.split_tab = m_split_tab.get();
     a.n_segs = unsigned(m_r_lo.size());
+    a.compact_tab = m_external_table ? nullptr : m_compact.get();
+    a.n_compact = COMPACT_SEGS;
+    a.compact_deg = COMPACT_DEG;
     a.rc = m_p.rcut;
     a.force = out.force;
     a.virial = out.virial;
@@ -453,7 +501,7 @@ template<typename R> void EspSolver::run_pair(const SystemView& sys, const Outpu
     }
 
 template<typename R>
-void EspSolver::run_mesh(const SystemView& sys, const OutputView& out, unsigned int n, hipStream_t s, bool want_ev)
+void EspSolver::run_mesh(const SystemView& sys, const OutputView& out, unsigned int n, hipStream_t s, bool want_ev, hipEvent_t join)
     {
     const MeshGeom& g = m_geom;
     const int P = int(m_p.order);
@@ -488,6 +536,8 @@ void EspSolver::run_mesh(const SystemView& sys, const OutputView& out, unsigned 
     const double V = g.Lx * g.Ly * g.Lz;
     check_hip(kernel::fft_inverse_x<R>(g, m_plan[0], m_tw[0].get(), m_spec.get(), 1.0 / V, m_phi.get(), s), "fft_inverse_x");
     mark(10, s);
+    if (join)
+        check_hip(hipStreamWaitEvent(s, join, 0), "hipStreamWaitEvent");
     check_hip(kernel::gather<R>(g, P, D, m_n_tiles, m_tile_start.as<unsigned int>(), m_rec.get(), m_sidx.as<SortedIdx>(), m_wcoef.get(),
                                 m_phi.get(), out.force, s),
               "gather");
@@ -537,12 +587,33 @@ void EspSolver::compute(const SystemView& sys, const OutputView& out, hipStream_
     const bool fp64 = m_p.precision == 0;
     const unsigned int flags = out.flags ? out.flags : COMPUTE_ALL;
     mark(0, s);
+    // the pair correction runs on a side stream, overlapping binning, spreading and the FFTs;
+    // the gather (which adds into the same force array) waits for it
+    hipEvent_t join = nullptr;
     if (flags & COMPUTE_PAIR)
         {
+        hipStream_t ps = s;
+        if ((flags & COMPUTE_MESH) && m_overlap)
+            {
+            if (!m_side)
+                {
+                check_hip(hipStreamCreateWithFlags(&m_side, hipStreamNonBlocking), "hipStreamCreate");
+                check_hip(hipEventCreateWithFlags(&m_ev_fork, hipEventDisableTiming), "hipEventCreate");
+                check_hip(hipEventCreateWithFlags(&m_ev_join, hipEventDisableTiming), "hipEventCreate");
+                }
+            check_hip(hipEventRecord(m_ev_fork, s), "hipEventRecord");
+            check_hip(hipStreamWaitEvent(m_side, m_ev_fork, 0), "hipStreamWaitEvent");
+            ps = m_side;
+            join = m_ev_join;
+            }
+        pair_mark(false, ps);
         if (fp64)
-            run_pair<double>(sys, out, s);
+            run_pair<double>(sys, out, ps);
         else
-            run_pair<float>(sys, out, s);
+            run_pair<float>(sys, out, ps);
+        pair_mark(true, ps);
+        if (join)
+            check_hip(hipEventRecord(join, ps), "hipEventRecord");
         }
     else
         {
@@ -555,9 +626,9 @@ void EspSolver::compute(const SystemView& sys, const OutputView& out, hipStream_
         {
         const bool want_ev = (flags & COMPUTE_ENERGY_VIRIAL) != 0;
         if (fp64)
-            run_mesh<double>(sys, out, n, s, want_ev);
+            run_mesh<double>(sys, out, n, s, want_ev, join);
         else
-            run_mesh<float>(sys, out, n, s, want_ev);
+            run_mesh<float>(sys, out, n, s, want_ev, join);
         }
     else
         for (int b = 1; b <= 11; ++b)

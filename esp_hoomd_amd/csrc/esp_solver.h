@@ -156,7 +156,13 @@ class EspSolver
 
     private:
     void mark(int boundary, hipStream_t s);
+    void pair_mark(bool end, hipStream_t s);
     void harvest();
+    // side stream for the pair correction, overlapped with the mesh pipeline
+    bool m_overlap = true;
+    hipStream_t m_side = nullptr;
+    hipEvent_t m_ev_fork = nullptr;
+    hipEvent_t m_ev_join = nullptr;
     bool m_profile = false;
     std::vector<std::vector<hipEvent_t>> m_ev_pool;   // free event sets (N_STAGES + 1 each)
     std::vector<std::vector<hipEvent_t>> m_ev_pending; // recorded, not yet harvested
@@ -168,7 +174,8 @@ class EspSolver
     void initialize(const SystemView& sys, hipStream_t s);
     void build_influence(hipStream_t s);
     void ensure_particles(unsigned int n);
-    template<typename R> void run_mesh(const SystemView& sys, const OutputView& out, unsigned int n, hipStream_t s, bool want_ev);
+    template<typename R>
+    void run_mesh(const SystemView& sys, const OutputView& out, unsigned int n, hipStream_t s, bool want_ev, hipEvent_t join);
     template<typename R> void run_pair(const SystemView& sys, const OutputView& out, hipStream_t s);
 
     SolverParams m_p;
@@ -188,7 +195,10 @@ class EspSolver
     std::vector<double> m_qx, m_qw;
 
     // device
+    static constexpr unsigned int COMPACT_SEGS = 64;
+    static constexpr unsigned int COMPACT_DEG = 11;
     DeviceBuffer m_split_tab;   // [n_segs][12] R
+    DeviceBuffer m_compact;     // [COMPACT_SEGS][COMPACT_DEG + 1] R
     DeviceBuffer m_wcoef;       // [P][D+1] R
     DeviceBuffer m_tw[3];       // twiddles per axis, complex R
     DeviceBuffer m_G, m_Gv;     // half spectrum FP64

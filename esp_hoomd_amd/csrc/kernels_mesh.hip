@@ -13,36 +13,6 @@ namespace esp
 namespace dev
 {
 
-//! Accumulator types: FP64 mode accumulates the mesh in FP64; the mixed mode accumulates it
-//! in fixed point (int32 in LDS, int64 in HBM), which is exact and order independent.
-template<typename R> struct Acc;
-template<> struct Acc<double>
-    {
-    using lds_t = double;
-    using glob_t = double;
-    };
-template<> struct Acc<float>
-    {
-    using lds_t = int;
-    using glob_t = unsigned long long;
-    };
-
-__device__ __forceinline__ void lds_add(double* p, double v)
-    {
-    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    }
-__device__ __forceinline__ void lds_add(int* p, float v)
-    {
-    __hip_atomic_fetch_add(p, __float2int_rn(v * float(FIXED_SCALE)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    }
-__device__ __forceinline__ void glob_add(double* p, double v)
-    {
-    unsafeAtomicAdd(p, v);
-    }
-__device__ __forceinline__ void glob_add(unsigned long long* p, int v)
-    {
-    atomicAdd(p, static_cast<unsigned long long>(static_cast<long long>(v)));
-    }
 __device__ __forceinline__ double mesh_load_clear(double* p)
     {
     const double v = *p;
@@ -196,222 +166,6 @@ __global__ void __launch_bounds__(256) k_scatter(unsigned int n,
     s.local = unsigned(ox % g.T) | (unsigned(oy % g.T) << 8) | (unsigned(oz % g.T) << 16);
     s.idx = p;
     sidx[pos] = s;
-    }
-
-// ---------------------------------------------------------------------------
-// Spreading: one workgroup per tile, P^3 stencil accumulated in an LDS box of
-// (T+P-1)^3 cells, then added to the mesh.  One wave per particle; each lane
-// owns (a, b) stencil columns and walks c.
-// ---------------------------------------------------------------------------
-template<typename R, int P>
-__global__ void __launch_bounds__(256) k_spread(MeshGeom g,
-                                                int D,
-                                                const unsigned int* __restrict__ tile_start,
-                                                const SortedRec<R>* __restrict__ rec,
-                                                const SortedIdx* __restrict__ sidx,
-                                                const R* __restrict__ wcoef,
-                                                typename Acc<R>::glob_t* __restrict__ mesh)
-    {
-    using L = typename Acc<R>::lds_t;
-    extern __shared__ __align__(16) unsigned char smem[];
-    const int B = g.B;
-    const int B2 = B * B;
-    const int B3 = B2 * B;
-    L* box = reinterpret_cast<L*>(smem);
-    R* wc = reinterpret_cast<R*>(smem + ((size_t(B3) * sizeof(L) + 15) & ~size_t(15)));
-
-    const unsigned int tile = blockIdx.x;
-    const unsigned int s0 = tile_start[tile], s1 = tile_start[tile + 1];
-    if (s0 == s1)
-        return;
-    for (int c = threadIdx.x; c < B3; c += blockDim.x)
-        box[c] = L(0);
-    for (int c = threadIdx.x; c < P * (D + 1); c += blockDim.x)
-        wc[c] = wcoef[c];
-    __syncthreads();
-
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int nw = blockDim.x >> 6;
-    const int axis = lane / P;
-    const int kt = lane - axis * P;
-    for (unsigned int p = s0 + wave; p < s1; p += nw)
-        {
-        const SortedRec<R> r = rec[p];
-        const unsigned int loc = sidx[p].local;
-        const int ox = loc & 0xff, oy = (loc >> 8) & 0xff, oz = loc >> 16;
-        R w = R(0);
-        if (lane < 3 * P)
-            {
-            const R t = axis == 0 ? r.tx : (axis == 1 ? r.ty : r.tz);
-            const R z = R(2) * t - R(1);
-            const R* c = wc + kt * (D + 1);
-            R v = c[D];
-            for (int j = D - 1; j >= 0; --j)
-                v = v * z + c[j];
-            w = v;
-            }
-        R wz[P];
-#pragma unroll
-        for (int c = 0; c < P; ++c)
-            wz[c] = __shfl(w, 2 * P + c, WAVE);
-#pragma unroll
-        for (int it = 0; it < (P * P + 63) / 64; ++it)
-            {
-            const int s = lane + it * 64;
-            const int a = s % P;
-            const int b = s / P;
-            const R wx = __shfl(w, a, WAVE);
-            const R wy = __shfl(w, P + (b < P ? b : 0), WAVE);
-            if (s < P * P)
-                {
-                const R qw = r.q * wx * wy;
-                L* dst = box + (oz * B + oy + b) * B + ox + a;
-#pragma unroll
-                for (int c = 0; c < P; ++c)
-                    lds_add(dst + c * B2, qw * wz[c]);
-                }
-            }
-        }
-    __syncthreads();
-
-    const int txy = g.tx * g.ty;
-    const int tz = int(tile) / txy;
-    const int ty = (int(tile) - tz * txy) / g.tx;
-    const int tx = int(tile) - tz * txy - ty * g.tx;
-    for (int c = threadIdx.x; c < B3; c += blockDim.x)
-        {
-        const L v = box[c];
-        if (v != L(0))
-            {
-            const int lz = c / B2;
-            const int ly = (c - lz * B2) / B;
-            const int lx = c - lz * B2 - ly * B;
-            const int gx = (tx * g.T + lx) % g.nx;
-            const int gy = (ty * g.T + ly) % g.ny;
-            const int gz = (tz * g.T + lz) % g.nz;
-            glob_add(&mesh[(size_t(gz) * g.ny + gy) * g.nx + gx], v);
-            }
-        }
-    }
-
-// ---------------------------------------------------------------------------
-// Gather: mirror of the spread.  Loads the (T+P-1)^3 potential box into LDS and
-// evaluates F_j = q_j sum grad W(x_m - r_j) phi(m) with the analytic stencil
-// derivative; adds to force[j].xyz.
-// ---------------------------------------------------------------------------
-template<typename R, int P>
-__global__ void __launch_bounds__(256) k_gather(MeshGeom g,
-                                                int D,
-                                                const unsigned int* __restrict__ tile_start,
-                                                const SortedRec<R>* __restrict__ rec,
-                                                const SortedIdx* __restrict__ sidx,
-                                                const R* __restrict__ wcoef,
-                                                const R* __restrict__ phi,
-                                                double4* __restrict__ force)
-    {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const int B = g.B;
-    const int B2 = B * B;
-    const int B3 = B2 * B;
-    R* box = reinterpret_cast<R*>(smem);
-    R* wc = reinterpret_cast<R*>(smem + ((size_t(B3) * sizeof(R) + 15) & ~size_t(15)));
-
-    const unsigned int tile = blockIdx.x;
-    const unsigned int s0 = tile_start[tile], s1 = tile_start[tile + 1];
-    if (s0 == s1)
-        return;
-    const int txy = g.tx * g.ty;
-    const int tz = int(tile) / txy;
-    const int ty = (int(tile) - tz * txy) / g.tx;
-    const int tx = int(tile) - tz * txy - ty * g.tx;
-    for (int c = threadIdx.x; c < B3; c += blockDim.x)
-        {
-        const int lz = c / B2;
-        const int ly = (c - lz * B2) / B;
-        const int lx = c - lz * B2 - ly * B;
-        const int gx = (tx * g.T + lx) % g.nx;
-        const int gy = (ty * g.T + ly) % g.ny;
-        const int gz = (tz * g.T + lz) % g.nz;
-        box[c] = phi[(size_t(gz) * g.ny + gy) * g.nx + gx];
-        }
-    for (int c = threadIdx.x; c < P * (D + 1); c += blockDim.x)
-        wc[c] = wcoef[c];
-    __syncthreads();
-
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int nw = blockDim.x >> 6;
-    const int axis = lane / P;
-    const int kt = lane - axis * P;
-    const R inv_h[3] = {R(1.0 / g.hx), R(1.0 / g.hy), R(1.0 / g.hz)};
-    for (unsigned int p = s0 + wave; p < s1; p += nw)
-        {
-        const SortedRec<R> r = rec[p];
-        const SortedIdx id = sidx[p];
-        const int ox = id.local & 0xff, oy = (id.local >> 8) & 0xff, oz = id.local >> 16;
-        R w = R(0), dw = R(0);
-        if (lane < 3 * P)
-            {
-            const R t = axis == 0 ? r.tx : (axis == 1 ? r.ty : r.tz);
-            const R z = R(2) * t - R(1);
-            const R* c = wc + kt * (D + 1);
-            R v = c[D], d = R(0);
-            for (int j = D - 1; j >= 0; --j)
-                {
-                d = d * z + v;
-                v = v * z + c[j];
-                }
-            w = v;
-            dw = R(2) * d;  // d/dt = 2 d/dz
-            }
-        R wz[P], dwz[P];
-#pragma unroll
-        for (int c = 0; c < P; ++c)
-            {
-            wz[c] = __shfl(w, 2 * P + c, WAVE);
-            dwz[c] = __shfl(dw, 2 * P + c, WAVE);
-            }
-        R ex = R(0), ey = R(0), ez = R(0);
-#pragma unroll
-        for (int it = 0; it < (P * P + 63) / 64; ++it)
-            {
-            const int s = lane + it * 64;
-            const int a = s % P;
-            const int b = s / P;
-            const int bb = b < P ? b : 0;
-            const R wx = __shfl(w, a, WAVE);
-            const R dwx = __shfl(dw, a, WAVE);
-            const R wy = __shfl(w, P + bb, WAVE);
-            const R dwy = __shfl(dw, P + bb, WAVE);
-            if (s < P * P)
-                {
-                const R* src = box + (oz * B + oy + b) * B + ox + a;
-                R A = R(0), Bz = R(0);
-#pragma unroll
-                for (int c = 0; c < P; ++c)
-                    {
-                    const R ph = src[c * B2];
-                    A += wz[c] * ph;
-                    Bz += dwz[c] * ph;
-                    }
-                ex += dwx * wy * A;
-                ey += wx * dwy * A;
-                ez += wx * wy * Bz;
-                }
-            }
-        ex = wave_sum(ex);
-        ey = wave_sum(ey);
-        ez = wave_sum(ez);
-        if (lane == 0)
-            {
-            double4 f = force[id.idx];
-            f.x += double(r.q) * double(ex * inv_h[0]);
-            f.y += double(r.q) * double(ey * inv_h[1]);
-            f.z += double(r.q) * double(ez * inv_h[2]);
-            force[id.idx] = f;
-            }
-        }
     }
 
 // ---------------------------------------------------------------------------
@@ -858,71 +612,6 @@ template hipError_t scatter_particles<double>(unsigned int, const unsigned int*,
 template hipError_t scatter_particles<float>(unsigned int, const unsigned int*, const double*, const double*, const MeshGeom&, int,
                                              const unsigned int*, const unsigned int*, const unsigned int*, void*, SortedIdx*,
                                              hipStream_t);
-
-size_t spread_lds_bytes(const MeshGeom& g, int P, int D, bool fp64)
-    {
-    const size_t B3 = size_t(g.B) * g.B * g.B;
-    const size_t el = fp64 ? 8 : 4;
-    return ((B3 * el + 15) & ~size_t(15)) + size_t(P) * (D + 1) * el;
-    }
-
-template<typename R, int P>
-static hipError_t launch_spread_P(const MeshGeom& g, int D, unsigned int n_tiles, const unsigned int* start, const void* rec,
-                                  const SortedIdx* sidx, const void* wcoef, void* mesh, hipStream_t s)
-    {
-    const size_t lds = spread_lds_bytes(g, P, D, sizeof(R) == 8);
-    hipLaunchKernelGGL((k_spread<R, P>), dim3(n_tiles), dim3(256), lds, s, g, D, start, reinterpret_cast<const SortedRec<R>*>(rec),
-                       sidx, reinterpret_cast<const R*>(wcoef), reinterpret_cast<typename Acc<R>::glob_t*>(mesh));
-    return hipGetLastError();
-    }
-
-template<typename R, int P>
-static hipError_t launch_gather_P(const MeshGeom& g, int D, unsigned int n_tiles, const unsigned int* start, const void* rec,
-                                  const SortedIdx* sidx, const void* wcoef, const void* phi, double* force, hipStream_t s)
-    {
-    const size_t lds = spread_lds_bytes(g, P, D, sizeof(R) == 8);
-    hipLaunchKernelGGL((k_gather<R, P>), dim3(n_tiles), dim3(256), lds, s, g, D, start, reinterpret_cast<const SortedRec<R>*>(rec),
-                       sidx, reinterpret_cast<const R*>(wcoef), reinterpret_cast<const R*>(phi), reinterpret_cast<double4*>(force));
-    return hipGetLastError();
-    }
-
-#define ESP_P_CASES(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)
-
-template<typename R>
-hipError_t spread(const MeshGeom& g, int P, int D, unsigned int n_tiles, const unsigned int* start, const void* rec,
-                  const SortedIdx* sidx, const void* wcoef, void* mesh, hipStream_t s)
-    {
-    switch (P)
-        {
-#define ESP_CASE(PP) \
-    case PP: return launch_spread_P<R, PP>(g, D, n_tiles, start, rec, sidx, wcoef, mesh, s);
-        ESP_P_CASES(ESP_CASE)
-#undef ESP_CASE
-        default: return hipErrorInvalidValue;
-        }
-    }
-template hipError_t spread<double>(const MeshGeom&, int, int, unsigned int, const unsigned int*, const void*, const SortedIdx*,
-                                   const void*, void*, hipStream_t);
-template hipError_t spread<float>(const MeshGeom&, int, int, unsigned int, const unsigned int*, const void*, const SortedIdx*,
-                                  const void*, void*, hipStream_t);
-
-template<typename R>
-hipError_t gather(const MeshGeom& g, int P, int D, unsigned int n_tiles, const unsigned int* start, const void* rec,
-                  const SortedIdx* sidx, const void* wcoef, const void* phi, double* force, hipStream_t s)
-    {
-    switch (P)
-        {
-#define ESP_CASE(PP) \
-    case PP: return launch_gather_P<R, PP>(g, D, n_tiles, start, rec, sidx, wcoef, phi, force, s);
-        ESP_P_CASES(ESP_CASE)
-#undef ESP_CASE
-        default: return hipErrorInvalidValue;
-        }
-    }
-template hipError_t gather<double>(const MeshGeom&, int, int, unsigned int, const unsigned int*, const void*, const SortedIdx*,
-                                   const void*, const void*, double*, hipStream_t);
-template hipError_t gather<float>(const MeshGeom&, int, int, unsigned int, const unsigned int*, const void*, const SortedIdx*,
-                                  const void*, const void*, double*, hipStream_t);
 
 int fft_x_pairs_per_block(int nx, bool fp64)
     {

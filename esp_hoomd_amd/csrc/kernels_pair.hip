@@ -2,11 +2,20 @@
 // correction fused in.  Each particle is handled by a group of G lanes; the ten per-particle
 // sums (force, energy, virial) are reduced with cross-lane shuffles inside the wave.
 //
+// Two evaluations of the split function S(r / r_c):
+//  * compact (default): a 64-segment, degree-11 polynomial table of S held in LDS; dS/ds is
+//    the analytic derivative of the same polynomial, so the pair force is exactly the
+//    gradient of the pair energy.  The grid is persistent so each block loads the table once.
+//  * reference format: the n_segs x 5 Horner tables of PSWF_Coeffs.h read from global memory
+//    with the lookup of EvaluatorPairPSWF.h:113-143 (used when an external table is set).
+//
 // Reference counterparts: EvaluatorPairPSWF::evalForceAndEnergy (EvaluatorPairPSWF.h:274-295)
 // called by HOOMD's PotentialPairGPU, and gpu_esp_fix_exclusions_kernel
 // (ESPForceComputeGPU.cu:391-471).
 #include "esp_device.h"
 #include "esp_kernels.h"
+
+#include <algorithm>
 
 namespace esp
 {
@@ -18,24 +27,84 @@ __device__ __forceinline__ double min_image(double d, double L, double invL)
     return d - L * rint(d * invL);
     }
 
-template<typename R, int G>
+//! S and dS/ds from the compact table: segment = floor(s * NSEG), z = 2 t - 1.
+template<typename R>
+__device__ __forceinline__ void compact_split(const R* __restrict__ tab, int nseg, int deg, R s, R& S, R& dS)
+    {
+    const R u = s * R(nseg);
+    int seg = int(u);
+    seg = seg < 0 ? 0 : (seg >= nseg ? nseg - 1 : seg);
+    const R z = R(2) * (u - R(seg)) - R(1);
+    const R* c = tab + seg * (deg + 1);
+    R v = c[deg], d = R(0);
+    for (int j = deg - 1; j >= 0; --j)
+        {
+        d = d * z + v;
+        v = v * z + c[j];
+        }
+    S = v;
+    dS = d * R(2 * nseg);  // dS/ds = dS/dz * dz/ds
+    }
+
+//! L(r) and (-dL/dr) / r for 0 < r <= rc (zero outside), given rinv = 1 / r.
+template<typename R, bool COMPACT>
+__device__ __forceinline__ void pair_terms(const R* __restrict__ tab,
+                                           const kernel::PairArgs& a,
+                                           R r,
+                                           R rinv,
+                                           R rc,
+                                           R inv_rc,
+                                           R& L,
+                                           R& ndr)
+    {
+    if constexpr (COMPACT)
+        {
+        if (r < R(1.0e-12) || r > rc)
+            {
+            L = R(0);
+            ndr = R(0);
+            return;
+            }
+        R S, dS;
+        compact_split<R>(tab, int(a.n_compact), int(a.compact_deg), r * inv_rc, S, dS);
+        const R inv4pir = R(0.079577471545947673) * rinv;
+        L = S * inv4pir;
+        ndr = (S * rinv - dS * inv_rc) * inv4pir * rinv;
+        }
+    else
+        {
+        R nd;
+        split_lookup<R>(tab, a.n_segs, r, rc, inv_rc, L, nd);
+        ndr = nd * rinv;
+        }
+    }
+
+template<typename R, int G, bool COMPACT>
 __global__ void __launch_bounds__(256) k_pair(kernel::PairArgs a)
     {
-    const unsigned int gid = blockIdx.x * blockDim.x + threadIdx.x;
-    const unsigned int i = gid / G;
-    const int sub = int(gid % G);
-    const bool active = i < a.N;
-    const R* __restrict__ tab = reinterpret_cast<const R*>(a.split_tab);
+    extern __shared__ __align__(16) unsigned char smem[];
+    const R* tab = reinterpret_cast<const R*>(a.split_tab);
+    if constexpr (COMPACT)
+        {
+        R* t = reinterpret_cast<R*>(smem);
+        const R* src = reinterpret_cast<const R*>(a.compact_tab);
+        const int n = int(a.n_compact * (a.compact_deg + 1));
+        for (int i = threadIdx.x; i < n; i += blockDim.x)
+            t[i] = src[i];
+        __syncthreads();
+        tab = t;
+        }
     const double invLx = 1.0 / a.Lx, invLy = 1.0 / a.Ly, invLz = 1.0 / a.Lz;
     const R rc = R(a.rc);
     const R inv_rc = R(1.0 / a.rc);
     const R rc2 = rc * rc;
-
-    R f[3] = {R(0), R(0), R(0)};
-    R e = R(0);
-    R v[6] = {R(0), R(0), R(0), R(0), R(0), R(0)};
-    if (active)
+    const int sub = int(threadIdx.x % G);
+    const unsigned int groups_per_grid = gridDim.x * (blockDim.x / G);
+    for (unsigned int i = (blockIdx.x * blockDim.x + threadIdx.x) / G; i < a.N + 0u; i += groups_per_grid)
         {
+        R f[3] = {R(0), R(0), R(0)};
+        R e = R(0);
+        R v[6] = {R(0), R(0), R(0), R(0), R(0), R(0)};
         const double4 pi = reinterpret_cast<const double4*>(a.postype)[i];
         const double qi = a.charge[i];
         if (a.nlist != nullptr && qi != 0.0 && a.rc >= 1.0e-12)
@@ -54,9 +123,10 @@ __global__ void __launch_bounds__(256) k_pair(kernel::PairArgs a)
                 if (rsq >= rc2 || qiqj == R(0))
                     continue;
                 const R r = sqrt(rsq);
-                R Lr, nd;
-                split_lookup<R>(tab, a.n_segs, r, rc, inv_rc, Lr, nd);
-                const R fdivr = qiqj * nd / r;
+                const R rinv = R(1) / r;
+                R Lr, ndr;
+                pair_terms<R, COMPACT>(tab, a, r, rinv, rc, inv_rc, Lr, ndr);
+                const R fdivr = qiqj * ndr;
                 f[0] += fdivr * dx;
                 f[1] += fdivr * dy;
                 f[2] += fdivr * dz;
@@ -88,12 +158,13 @@ __global__ void __launch_bounds__(256) k_pair(kernel::PairArgs a)
                 if (rsq < R(1.0e-20))
                     continue;
                 const R r = sqrt(rsq);
-                R Lr = R(0), nd = R(0);
+                const R rinv = R(1) / r;
+                R Lr = R(0), ndr = R(0);
                 if (r <= rc)
-                    split_lookup<R>(tab, a.n_segs, r, rc, inv_rc, Lr, nd);
-                const R inv4pir = R(0.079577471545947673) / r;
+                    pair_terms<R, COMPACT>(tab, a, r, rinv, rc, inv_rc, Lr, ndr);
+                const R inv4pir = R(0.079577471545947673) * rinv;
                 const R dE = qiqj * (Lr - inv4pir);
-                const R fdivr = qiqj * (nd - inv4pir / r) / r;
+                const R fdivr = qiqj * (ndr - inv4pir * rinv * rinv);
                 f[0] += fdivr * dx;
                 f[1] += fdivr * dy;
                 f[2] += fdivr * dz;
@@ -107,28 +178,28 @@ __global__ void __launch_bounds__(256) k_pair(kernel::PairArgs a)
                 v[5] += hf * dz * dz;
                 }
             }
-        }
-    // reduce over the G lanes of this particle (all lanes of the wave take part)
+        // reduce over the G lanes of this particle (every lane of the group takes part)
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
-        f[c] = group_sum<G>(f[c]);
-    e = group_sum<G>(e);
+        for (int c = 0; c < 3; ++c)
+            f[c] = group_sum<G>(f[c]);
+        e = group_sum<G>(e);
 #pragma unroll
-    for (int c = 0; c < 6; ++c)
-        v[c] = group_sum<G>(v[c]);
-    if (active && sub == 0)
-        {
-        double4 o;
-        o.x = double(f[0]);
-        o.y = double(f[1]);
-        o.z = double(f[2]);
-        o.w = double(e);
-        reinterpret_cast<double4*>(a.force)[i] = o;
-        if (a.virial != nullptr)
+        for (int c = 0; c < 6; ++c)
+            v[c] = group_sum<G>(v[c]);
+        if (sub == 0)
             {
+            double4 o;
+            o.x = double(f[0]);
+            o.y = double(f[1]);
+            o.z = double(f[2]);
+            o.w = double(e);
+            reinterpret_cast<double4*>(a.force)[i] = o;
+            if (a.virial != nullptr)
+                {
 #pragma unroll
-            for (int c = 0; c < 6; ++c)
-                a.virial[size_t(c) * a.virial_pitch + i] = double(v[c]);
+                for (int c = 0; c < 6; ++c)
+                    a.virial[size_t(c) * a.virial_pitch + i] = double(v[c]);
+                }
             }
         }
     }
@@ -137,20 +208,32 @@ __global__ void __launch_bounds__(256) k_pair(kernel::PairArgs a)
 
 namespace kernel
 {
+template<typename R, int G> static void launch_pair(const PairArgs& a, unsigned int nb, hipStream_t s)
+    {
+    if (a.compact_tab != nullptr)
+        {
+        const size_t lds = size_t(a.n_compact) * (a.compact_deg + 1) * sizeof(R);
+        hipLaunchKernelGGL((dev::k_pair<R, G, true>), dim3(nb), dim3(256), lds, s, a);
+        }
+    else
+        hipLaunchKernelGGL((dev::k_pair<R, G, false>), dim3(nb), dim3(256), 0, s, a);
+    }
+
 template<typename R> hipError_t pair_forces(const PairArgs& a, int lanes_per_particle, hipStream_t s)
     {
-    const size_t threads = size_t(a.N) * lanes_per_particle;
-    const unsigned int nb = unsigned((threads + 255) / 256);
-    if (nb == 0)
+    if (a.N == 0)
         return hipSuccess;
+    const size_t threads = size_t(a.N) * lanes_per_particle;
+    // persistent grid: at most 8 blocks per CU of a 256-CU device
+    const unsigned int nb = unsigned(std::min<size_t>((threads + 255) / 256, 2048));
     switch (lanes_per_particle)
         {
-        case 1: hipLaunchKernelGGL((dev::k_pair<R, 1>), dim3(nb), dim3(256), 0, s, a); break;
-        case 2: hipLaunchKernelGGL((dev::k_pair<R, 2>), dim3(nb), dim3(256), 0, s, a); break;
-        case 4: hipLaunchKernelGGL((dev::k_pair<R, 4>), dim3(nb), dim3(256), 0, s, a); break;
-        case 8: hipLaunchKernelGGL((dev::k_pair<R, 8>), dim3(nb), dim3(256), 0, s, a); break;
-        case 16: hipLaunchKernelGGL((dev::k_pair<R, 16>), dim3(nb), dim3(256), 0, s, a); break;
-        case 32: hipLaunchKernelGGL((dev::k_pair<R, 32>), dim3(nb), dim3(256), 0, s, a); break;
+        case 1: launch_pair<R, 1>(a, nb, s); break;
+        case 2: launch_pair<R, 2>(a, nb, s); break;
+        case 4: launch_pair<R, 4>(a, nb, s); break;
+        case 8: launch_pair<R, 8>(a, nb, s); break;
+        case 16: launch_pair<R, 16>(a, nb, s); break;
+        case 32: launch_pair<R, 32>(a, nb, s); break;
         default: return hipErrorInvalidValue;
         }
     return hipGetLastError();

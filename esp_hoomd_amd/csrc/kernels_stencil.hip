@@ -1,0 +1,814 @@
+// PSWF charge spreading and force gather on tile-sorted particles (gfx950, wave64).
+//
+// One workgroup owns one spreading tile of T^3 mesh cells and the LDS box of (T+P-1)^3 cells
+// its particles' stencils touch.  Particles are processed in chunks of CH: the workgroup first
+// stages the chunk's charges, origins and all 3P (and, for the gather, 3P derivative) 1-D
+// stencil weights in LDS in parallel, then each wave walks particles with G lanes per particle;
+// a lane owns stencil columns (a, b) -- pairs of x offsets in the gather, so one ds_read2 feeds
+// two columns -- and runs the P-long z loop with the z weights held in registers.
+//
+// Spreading accumulates in the LDS box with LDS atomics and adds the box to the mesh in HBM
+// (FP64 atomics, or int64 fixed point in the mixed mode).  The gather reduces its per-column
+// partial sums with cross-lane shuffles inside each G-lane segment and adds the particle's
+// force with no-return global atomics, so no wave waits on a global load per particle.
+//
+// Reference counterparts: gpu_esp_assign_particles_kernel (ESPForceComputeGPU.cu:206-272) and
+// gpu_esp_interpolate_forces_kernel (ESPForceComputeGPU.cu:292-373).
+#include "esp_device.h"
+#include "esp_kernels.h"
+
+#include <cstdlib>
+
+namespace esp
+{
+namespace dev
+{
+
+//! Particles staged per chunk (keeps the FP64 gather within 64 KiB of LDS at T = 8, P <= 11).
+template<typename R> constexpr int chunk_size()
+    {
+    return sizeof(R) == 8 ? 32 : 64;
+    }
+
+//! Lanes per particle segment: a power of two in {8, 16, 32, 64} maximising lane use for
+//! `items` work items per particle (ties go to the larger segment).
+constexpr int choose_group(int items)
+    {
+    int best = 64;
+    double bu = 0.0;
+    const int gs[4] = {64, 32, 16, 8};
+    for (int i = 0; i < 4; ++i)
+        {
+        const int g = gs[i];
+        if (g < 16 && items > 8)
+            continue;
+        const int it = (items + g - 1) / g;
+        const double u = double(items) / double(it * g);
+        if (u > bu + 1e-9)
+            {
+            bu = u;
+            best = g;
+            }
+        }
+    return best;
+    }
+
+//! Stage the chunk [c0, c0 + n): charges, origins, indices and 1-D weights (and derivatives).
+//! W layout: [(p * 3 + axis) * (P + 1) + k], entry k = P is a zero pad.
+template<typename R, int P, bool DERIV>
+__device__ __forceinline__ void stage_chunk(int c0,
+                                            int n,
+                                            int D,
+                                            const SortedRec<R>* __restrict__ rec,
+                                            const SortedIdx* __restrict__ sidx,
+                                            const R* __restrict__ wc,
+                                            R* __restrict__ W,
+                                            R* __restrict__ DW,
+                                            R* __restrict__ qs,
+                                            unsigned int* __restrict__ locs,
+                                            unsigned int* __restrict__ idxs)
+    {
+    for (int item = threadIdx.x; item < n * 3 * P; item += blockDim.x)
+        {
+        const int p = item / (3 * P);
+        const int r = item - p * (3 * P);
+        const int axis = r / P;
+        const int k = r - axis * P;
+        const SortedRec<R> rr = rec[c0 + p];
+        const R t = axis == 0 ? rr.tx : (axis == 1 ? rr.ty : rr.tz);
+        const R z = R(2) * t - R(1);
+        const R* c = wc + k * (D + 1);
+        R v = c[D], d = R(0);
+        for (int j = D - 1; j >= 0; --j)
+            {
+            if (DERIV)
+                d = d * z + v;
+            v = v * z + c[j];
+            }
+        const int o = (p * 3 + axis) * (P + 1);
+        W[o + k] = v;
+        if (DERIV)
+            DW[o + k] = R(2) * d;  // d/dt = 2 d/dz
+        if (k == 0)
+            {
+            W[o + P] = R(0);
+            if (DERIV)
+                DW[o + P] = R(0);
+            }
+        }
+    for (int p = threadIdx.x; p < n; p += blockDim.x)
+        {
+        qs[p] = rec[c0 + p].q;
+        const SortedIdx s = sidx[c0 + p];
+        locs[p] = s.local;
+        idxs[p] = s.idx;
+        }
+    }
+
+__device__ __forceinline__ void tile_coords(const MeshGeom& g, unsigned int tile, int& tx, int& ty, int& tz)
+    {
+    const int txy = g.tx * g.ty;
+    tz = int(tile) / txy;
+    ty = (int(tile) - tz * txy) / g.tx;
+    tx = int(tile) - tz * txy - ty * g.tx;
+    }
+
+// ---------------------------------------------------------------------------
+// Spread
+// ---------------------------------------------------------------------------
+template<typename R, int P>
+__global__ void __launch_bounds__(256) k_spread(MeshGeom g,
+                                                int D,
+                                                const unsigned int* __restrict__ tile_start,
+                                                const SortedRec<R>* __restrict__ rec,
+                                                const SortedIdx* __restrict__ sidx,
+                                                const R* __restrict__ wcoef,
+                                                typename Acc<R>::glob_t* __restrict__ mesh)
+    {
+    using L = typename Acc<R>::lds_t;
+    constexpr int CH = chunk_size<R>();
+    constexpr int ITEMS = P * P;                 // (a, b) columns per particle
+    constexpr int G = choose_group(ITEMS);
+    constexpr int ITER = (ITEMS + G - 1) / G;
+    constexpr int PPW = WAVE / G;                // particles per wave pass
+
+    const unsigned int tile = blockIdx.x;
+    const unsigned int s0 = tile_start[tile], s1 = tile_start[tile + 1];
+    if (s0 == s1)
+        return;
+
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int B = g.B;
+    const int B2 = B * B;
+    const int B3 = B2 * B;
+    L* box = reinterpret_cast<L*>(smem);
+    R* W = reinterpret_cast<R*>(smem + ((size_t(B3 + 1) * sizeof(L) + 15) & ~size_t(15)));
+    R* wc = W + CH * 3 * (P + 1);
+    R* qs = wc + P * (MAX_WINDOW_DEG + 1);
+    unsigned int* locs = reinterpret_cast<unsigned int*>(qs + CH);
+    unsigned int* idxs = locs + CH;
+
+    for (int c = threadIdx.x; c <= B3; c += blockDim.x)
+        box[c] = L(0);
+    for (int c = threadIdx.x; c < P * (D + 1); c += blockDim.x)
+        wc[c] = wcoef[c];
+    __syncthreads();
+
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = threadIdx.x / WAVE;
+    const int nw = blockDim.x / WAVE;
+    const int seg = lane / G;
+    const int sl = lane - seg * G;
+
+    for (unsigned int c0 = s0; c0 < s1; c0 += CH)
+        {
+        const int n = int(min(unsigned(CH), s1 - c0));
+        stage_chunk<R, P, false>(int(c0), n, D, rec, sidx, wc, W, nullptr, qs, locs, idxs);
+        __syncthreads();
+        for (int pb = wave * PPW; pb < n; pb += nw * PPW)
+            {
+            const int p = pb + seg;
+            if (p < n)
+                {
+                const unsigned int loc = locs[p];
+                const int ox = loc & 0xff, oy = (loc >> 8) & 0xff, oz = loc >> 16;
+                const R q = qs[p];
+                const R* wp = W + p * 3 * (P + 1);
+                R wz[P];
+#pragma unroll
+                for (int c = 0; c < P; ++c)
+                    wz[c] = wp[2 * (P + 1) + c];
+#pragma unroll
+                for (int it = 0; it < ITER; ++it)
+                    {
+                    const int item = sl + it * G;
+                    if (item < ITEMS)
+                        {
+                        const int b = item / P;
+                        const int a = item - b * P;
+                        const R qw = q * wp[a] * wp[(P + 1) + b];
+                        L* dst = box + (oz * B + oy + b) * B + ox + a;
+#pragma unroll
+                        for (int c = 0; c < P; ++c)
+                            lds_add(dst + c * B2, qw * wz[c]);
+                        }
+                    }
+                }
+            }
+        __syncthreads();
+        }
+
+    int tx, ty, tz;
+    tile_coords(g, tile, tx, ty, tz);
+    for (int c = threadIdx.x; c < B3; c += blockDim.x)
+        {
+        const L v = box[c];
+        if (v != L(0))
+            {
+            const int lz = c / B2;
+            const int ly = (c - lz * B2) / B;
+            const int lx = c - lz * B2 - ly * B;
+            const int gx = (tx * g.T + lx) % g.nx;
+            const int gy = (ty * g.T + ly) % g.ny;
+            const int gz = (tz * g.T + lz) % g.nz;
+            glob_add(&mesh[(size_t(gz) * g.ny + gy) * g.nx + gx], v);
+            }
+        }
+    }
+
+// ---------------------------------------------------------------------------
+// Gather
+// ---------------------------------------------------------------------------
+template<typename R, int P>
+__global__ void __launch_bounds__(256) k_gather(MeshGeom g,
+                                                int D,
+                                                const unsigned int* __restrict__ tile_start,
+                                                const SortedRec<R>* __restrict__ rec,
+                                                const SortedIdx* __restrict__ sidx,
+                                                const R* __restrict__ wcoef,
+                                                const R* __restrict__ phi,
+                                                double4* __restrict__ force)
+    {
+    constexpr int CH = chunk_size<R>();
+    constexpr int NPAIR = (P + 1) / 2;           // x offsets handled two at a time
+    constexpr int ITEMS = NPAIR * P;             // (a pair, b) columns per particle
+    constexpr int G = choose_group(ITEMS);
+    constexpr int ITER = (ITEMS + G - 1) / G;
+    constexpr int PPW = WAVE / G;
+
+    const unsigned int tile = blockIdx.x;
+    const unsigned int s0 = tile_start[tile], s1 = tile_start[tile + 1];
+    if (s0 == s1)
+        return;
+
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int B = g.B;
+    const int B2 = B * B;
+    const int B3 = B2 * B;
+    R* box = reinterpret_cast<R*>(smem);
+    R* W = reinterpret_cast<R*>(smem + ((size_t(B3 + 1) * sizeof(R) + 15) & ~size_t(15)));
+    R* DW = W + CH * 3 * (P + 1);
+    R* wc = DW + CH * 3 * (P + 1);
+    R* qs = wc + P * (MAX_WINDOW_DEG + 1);
+    unsigned int* locs = reinterpret_cast<unsigned int*>(qs + CH);
+    unsigned int* idxs = locs + CH;
+
+    int tx, ty, tz;
+    tile_coords(g, tile, tx, ty, tz);
+    for (int c = threadIdx.x; c < B3; c += blockDim.x)
+        {
+        const int lz = c / B2;
+        const int ly = (c - lz * B2) / B;
+        const int lx = c - lz * B2 - ly * B;
+        const int gx = (tx * g.T + lx) % g.nx;
+        const int gy = (ty * g.T + ly) % g.ny;
+        const int gz = (tz * g.T + lz) % g.nz;
+        box[c] = phi[(size_t(gz) * g.ny + gy) * g.nx + gx];
+        }
+    if (threadIdx.x == 0)
+        box[B3] = R(0);
+    for (int c = threadIdx.x; c < P * (D + 1); c += blockDim.x)
+        wc[c] = wcoef[c];
+    __syncthreads();
+
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int wave = threadIdx.x / WAVE;
+    const int nw = blockDim.x / WAVE;
+    const int seg = lane / G;
+    const int sl = lane - seg * G;
+    const R inv_hx = R(1.0 / g.hx), inv_hy = R(1.0 / g.hy), inv_hz = R(1.0 / g.hz);
+
+    for (unsigned int c0 = s0; c0 < s1; c0 += CH)
+        {
+        const int n = int(min(unsigned(CH), s1 - c0));
+        stage_chunk<R, P, true>(int(c0), n, D, rec, sidx, wc, W, DW, qs, locs, idxs);
+        __syncthreads();
+        for (int pb = wave * PPW; pb < n; pb += nw * PPW)
+            {
+            const int p = pb + seg;
+            R ex = R(0), ey = R(0), ez = R(0);
+            if (p < n)
+                {
+                const unsigned int loc = locs[p];
+                const int ox = loc & 0xff, oy = (loc >> 8) & 0xff, oz = loc >> 16;
+                const R* wp = W + p * 3 * (P + 1);
+                const R* dp = DW + p * 3 * (P + 1);
+                R wz[P], dwz[P];
+#pragma unroll
+                for (int c = 0; c < P; ++c)
+                    {
+                    wz[c] = wp[2 * (P + 1) + c];
+                    dwz[c] = dp[2 * (P + 1) + c];
+                    }
+#pragma unroll
+                for (int it = 0; it < ITER; ++it)
+                    {
+                    const int item = sl + it * G;
+                    if (item < ITEMS)
+                        {
+                        const int b = item / NPAIR;
+                        const int a = 2 * (item - b * NPAIR);
+                        const R wx0 = wp[a], wx1 = wp[a + 1];
+                        const R dx0 = dp[a], dx1 = dp[a + 1];
+                        const R wy = wp[(P + 1) + b], dy = dp[(P + 1) + b];
+                        const R* src = box + (oz * B + oy + b) * B + ox + a;
+                        R A0 = R(0), A1 = R(0), Z0 = R(0), Z1 = R(0);
+#pragma unroll
+                        for (int c = 0; c < P; ++c)
+                            {
+                            const R f0 = src[c * B2];
+                            const R f1 = src[c * B2 + 1];
+                            A0 += wz[c] * f0;
+                            A1 += wz[c] * f1;
+                            Z0 += dwz[c] * f0;
+                            Z1 += dwz[c] * f1;
+                            }
+                        ex += wy * (dx0 * A0 + dx1 * A1);
+                        ey += dy * (wx0 * A0 + wx1 * A1);
+                        ez += wy * (wx0 * Z0 + wx1 * Z1);
+                        }
+                    }
+                }
+            ex = group_sum<G>(ex);
+            ey = group_sum<G>(ey);
+            ez = group_sum<G>(ez);
+            if (p < n && sl == 0)
+                {
+                const double q = double(qs[p]);
+                double* f = reinterpret_cast<double*>(&force[idxs[p]]);
+                unsafeAtomicAdd(f + 0, q * double(ex * inv_hx));
+                unsafeAtomicAdd(f + 1, q * double(ey * inv_hy));
+                unsafeAtomicAdd(f + 2, q * double(ez * inv_hz));
+                }
+            }
+        __syncthreads();
+        }
+    }
+
+// ---------------------------------------------------------------------------
+// Thread-per-particle variants: each thread owns one particle, keeps its x and z
+// weights (and derivatives) in registers, evaluates the y weight per row and walks
+// the P^2 rows of P contiguous LDS cells.  No cross-lane traffic at all.
+// ---------------------------------------------------------------------------
+template<typename R, int P>
+__device__ __forceinline__ void eval_taps(const R* __restrict__ wc, int D, R t, R* w, R* dw)
+    {
+    const R z = R(2) * t - R(1);
+#pragma unroll
+    for (int k = 0; k < P; ++k)
+        {
+        const R* c = wc + k * (D + 1);
+        R v = c[D], d = R(0);
+        for (int j = D - 1; j >= 0; --j)
+            {
+            if (dw)
+                d = d * z + v;
+            v = v * z + c[j];
+            }
+        w[k] = v;
+        if (dw)
+            dw[k] = R(2) * d;
+        }
+    }
+
+template<typename R>
+__device__ __forceinline__ R eval_tap(const R* __restrict__ wc, int D, int k, R t, R& dw)
+    {
+    const R z = R(2) * t - R(1);
+    const R* c = wc + k * (D + 1);
+    R v = c[D], d = R(0);
+    for (int j = D - 1; j >= 0; --j)
+        {
+        d = d * z + v;
+        v = v * z + c[j];
+        }
+    dw = R(2) * d;
+    return v;
+    }
+
+template<typename R, int P>
+__global__ void __launch_bounds__(256) k_spread_tp(MeshGeom g,
+                                                   int D,
+                                                   const unsigned int* __restrict__ tile_start,
+                                                   const SortedRec<R>* __restrict__ rec,
+                                                   const SortedIdx* __restrict__ sidx,
+                                                   const R* __restrict__ wcoef,
+                                                   typename Acc<R>::glob_t* __restrict__ mesh)
+    {
+    using L = typename Acc<R>::lds_t;
+    const unsigned int tile = blockIdx.x;
+    const unsigned int s0 = tile_start[tile], s1 = tile_start[tile + 1];
+    if (s0 == s1)
+        return;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int B = g.B;
+    const int B2 = B * B;
+    const int B3 = B2 * B;
+    L* box = reinterpret_cast<L*>(smem);
+    R* wc = reinterpret_cast<R*>(smem + ((size_t(B3 + 1) * sizeof(L) + 15) & ~size_t(15)));
+    for (int c = threadIdx.x; c <= B3; c += blockDim.x)
+        box[c] = L(0);
+    for (int c = threadIdx.x; c < P * (D + 1); c += blockDim.x)
+        wc[c] = wcoef[c];
+    __syncthreads();
+
+    for (unsigned int p = s0 + threadIdx.x; p < s1; p += blockDim.x)
+        {
+        const SortedRec<R> r = rec[p];
+        const unsigned int loc = sidx[p].local;
+        const int ox = loc & 0xff, oy = (loc >> 8) & 0xff, oz = loc >> 16;
+        R wx[P], wz[P];
+        eval_taps<R, P>(wc, D, r.tx, wx, nullptr);
+        eval_taps<R, P>(wc, D, r.tz, wz, nullptr);
+        L* base = box + (oz * B + oy) * B + ox;
+        for (int b = 0; b < P; ++b)
+            {
+            R dwy;
+            const R qy = r.q * eval_tap<R>(wc, D, b, r.ty, dwy);
+            L* row = base + b * B;
+#pragma unroll
+            for (int c = 0; c < P; ++c)
+                {
+                const R v = qy * wz[c];
+#pragma unroll
+                for (int a = 0; a < P; ++a)
+                    lds_add(row + c * B2 + a, v * wx[a]);
+                }
+            }
+        }
+    __syncthreads();
+
+    int tx, ty, tz;
+    tile_coords(g, tile, tx, ty, tz);
+    for (int c = threadIdx.x; c < B3; c += blockDim.x)
+        {
+        const L v = box[c];
+        if (v != L(0))
+            {
+            const int lz = c / B2;
+            const int ly = (c - lz * B2) / B;
+            const int lx = c - lz * B2 - ly * B;
+            const int gx = (tx * g.T + lx) % g.nx;
+            const int gy = (ty * g.T + ly) % g.ny;
+            const int gz = (tz * g.T + lz) % g.nz;
+            glob_add(&mesh[(size_t(gz) * g.ny + gy) * g.nx + gx], v);
+            }
+        }
+    }
+
+template<typename R, int P>
+__global__ void __launch_bounds__(256) k_gather_tp(MeshGeom g,
+                                                   int D,
+                                                   const unsigned int* __restrict__ tile_start,
+                                                   const SortedRec<R>* __restrict__ rec,
+                                                   const SortedIdx* __restrict__ sidx,
+                                                   const R* __restrict__ wcoef,
+                                                   const R* __restrict__ phi,
+                                                   double4* __restrict__ force)
+    {
+    const unsigned int tile = blockIdx.x;
+    const unsigned int s0 = tile_start[tile], s1 = tile_start[tile + 1];
+    if (s0 == s1)
+        return;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int B = g.B;
+    const int B2 = B * B;
+    const int B3 = B2 * B;
+    R* box = reinterpret_cast<R*>(smem);
+    R* wc = reinterpret_cast<R*>(smem + ((size_t(B3 + 1) * sizeof(R) + 15) & ~size_t(15)));
+    int tx, ty, tz;
+    tile_coords(g, tile, tx, ty, tz);
+    for (int c = threadIdx.x; c < B3; c += blockDim.x)
+        {
+        const int lz = c / B2;
+        const int ly = (c - lz * B2) / B;
+        const int lx = c - lz * B2 - ly * B;
+        const int gx = (tx * g.T + lx) % g.nx;
+        const int gy = (ty * g.T + ly) % g.ny;
+        const int gz = (tz * g.T + lz) % g.nz;
+        box[c] = phi[(size_t(gz) * g.ny + gy) * g.nx + gx];
+        }
+    if (threadIdx.x == 0)
+        box[B3] = R(0);
+    for (int c = threadIdx.x; c < P * (D + 1); c += blockDim.x)
+        wc[c] = wcoef[c];
+    __syncthreads();
+
+    const R inv_hx = R(1.0 / g.hx), inv_hy = R(1.0 / g.hy), inv_hz = R(1.0 / g.hz);
+    for (unsigned int p = s0 + threadIdx.x; p < s1; p += blockDim.x)
+        {
+        const SortedRec<R> r = rec[p];
+        const SortedIdx id = sidx[p];
+        const int ox = id.local & 0xff, oy = (id.local >> 8) & 0xff, oz = id.local >> 16;
+        R wx[P], dwx[P], wz[P], dwz[P];
+        eval_taps<R, P>(wc, D, r.tx, wx, dwx);
+        eval_taps<R, P>(wc, D, r.tz, wz, dwz);
+        const R* base = box + (oz * B + oy) * B + ox;
+        R ex = R(0), ey = R(0), ez = R(0);
+        for (int b = 0; b < P; ++b)
+            {
+            R dwy;
+            const R wy = eval_tap<R>(wc, D, b, r.ty, dwy);
+            const R* row = base + b * B;
+            R Sx = R(0), Sdx = R(0), Sdz = R(0);
+#pragma unroll
+            for (int c = 0; c < P; ++c)
+                {
+                R sx = R(0), sdx = R(0);
+#pragma unroll
+                for (int a = 0; a < P; ++a)
+                    {
+                    const R f = row[c * B2 + a];
+                    sx += wx[a] * f;
+                    sdx += dwx[a] * f;
+                    }
+                Sx += wz[c] * sx;
+                Sdx += wz[c] * sdx;
+                Sdz += dwz[c] * sx;
+                }
+            ex += wy * Sdx;
+            ey += dwy * Sx;
+            ez += wy * Sdz;
+            }
+        const double q = double(r.q);
+        double* f = reinterpret_cast<double*>(&force[id.idx]);
+        unsafeAtomicAdd(f + 0, q * double(ex * inv_hx));
+        unsafeAtomicAdd(f + 1, q * double(ey * inv_hy));
+        unsafeAtomicAdd(f + 2, q * double(ez * inv_hz));
+        }
+    }
+
+// ---------------------------------------------------------------------------
+// Column-ownership spreading: lane (cx, cy) owns the z column of the tile's
+// (T+P-1)^3 box in registers.  Particles are broadcast through LDS: x and y
+// weights, and the z weights already shifted to box coordinates (zero outside
+// the stencil).  A lane whose column lies in a particle's xy footprint adds
+// B = T+P-1 products; no LDS atomics, fixed accumulation order within a tile.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void glob_add_real(double* p, double v)
+    {
+    unsafeAtomicAdd(p, v);
+    }
+__device__ __forceinline__ void glob_add_real(unsigned long long* p, float v)
+    {
+    atomicAdd(p, static_cast<unsigned long long>(__double2ll_rn(double(v) * FIXED_SCALE)));
+    }
+
+template<typename R, int P, int T>
+__global__ void __launch_bounds__(1024) k_spread_own(MeshGeom g,
+                                                     int D,
+                                                     const unsigned int* __restrict__ tile_start,
+                                                     const SortedRec<R>* __restrict__ rec,
+                                                     const SortedIdx* __restrict__ sidx,
+                                                     const R* __restrict__ wcoef,
+                                                     typename Acc<R>::glob_t* __restrict__ mesh)
+    {
+    constexpr int B = T + P - 1;
+    constexpr int BZ = (B + 3) & ~3;             // z weights padded to a multiple of 4
+    constexpr int CH = 64;
+    const unsigned int tile = blockIdx.x;
+    const unsigned int s0 = tile_start[tile], s1 = tile_start[tile + 1];
+    if (s0 == s1)
+        return;
+    extern __shared__ __align__(16) unsigned char smem[];
+    R* Wz = reinterpret_cast<R*>(smem);          // [CH][BZ] shifted z weights
+    R* Wxy = Wz + CH * BZ;                        // [CH][2][P] x and y weights (times q for x)
+    R* wc = Wxy + CH * 2 * P;                     // [P][D+1]
+    unsigned int* org = reinterpret_cast<unsigned int*>(wc + P * (MAX_WINDOW_DEG + 1));  // [CH] ox | oy << 8
+
+    for (int c = threadIdx.x; c < P * (D + 1); c += blockDim.x)
+        wc[c] = wcoef[c];
+
+    const int cx = int(threadIdx.x) % B;
+    const int cy = int(threadIdx.x) / B;
+    const bool owner = int(threadIdx.x) < B * B;
+    R col[B];
+#pragma unroll
+    for (int z = 0; z < B; ++z)
+        col[z] = R(0);
+    __syncthreads();
+
+    for (unsigned int c0 = s0; c0 < s1; c0 += CH)
+        {
+        const int n = int(min(unsigned(CH), s1 - c0));
+        // stage: items (p, j) with j < BZ for z, BZ <= j < BZ + 2P for x and y
+        constexpr int PER = BZ + 2 * P;
+        for (int item = threadIdx.x; item < n * PER; item += blockDim.x)
+            {
+            const int p = item / PER;
+            const int j = item - p * PER;
+            const SortedRec<R> r = rec[c0 + p];
+            const unsigned int loc = sidx[c0 + p].local;
+            int k;
+            R t;
+            R scale = R(1);
+            if (j < BZ)
+                {
+                k = j - int(loc >> 16);           // z - oz
+                t = r.tz;
+                }
+            else if (j < BZ + P)
+                {
+                k = j - BZ;
+                t = r.tx;
+                scale = r.q;
+                }
+            else
+                {
+                k = j - BZ - P;
+                t = r.ty;
+                }
+            R v = R(0);
+            if (k >= 0 && k < P)
+                {
+                const R zz = R(2) * t - R(1);
+                const R* c = wc + k * (D + 1);
+                v = c[D];
+                for (int i = D - 1; i >= 0; --i)
+                    v = v * zz + c[i];
+                v *= scale;
+                }
+            if (j < BZ)
+                Wz[p * BZ + j] = v;
+            else
+                Wxy[p * 2 * P + (j - BZ)] = v;
+            if (j == 0)
+                org[p] = loc & 0xffffu;
+            }
+        __syncthreads();
+        for (int p = 0; p < n; ++p)
+            {
+            const unsigned int o = org[p];
+            const int dx = cx - int(o & 0xff);
+            const int dy = cy - int(o >> 8);
+            if (owner && dx >= 0 && dx < P && dy >= 0 && dy < P)
+                {
+                const R w = Wxy[p * 2 * P + dx] * Wxy[p * 2 * P + P + dy];
+                const R* wz = Wz + p * BZ;
+#pragma unroll
+                for (int z = 0; z < B; ++z)
+                    col[z] += w * wz[z];
+                }
+            }
+        __syncthreads();
+        }
+
+    if (owner)
+        {
+        int tx, ty, tz;
+        tile_coords(g, tile, tx, ty, tz);
+        const int gx = (tx * T + cx) % g.nx;
+        const int gy = (ty * T + cy) % g.ny;
+#pragma unroll
+        for (int z = 0; z < B; ++z)
+            if (col[z] != R(0))
+                {
+                const int gz = (tz * T + z) % g.nz;
+                glob_add_real(&mesh[(size_t(gz) * g.ny + gy) * g.nx + gx], col[z]);
+                }
+        }
+    }
+
+    } // namespace dev
+
+namespace kernel
+{
+using namespace esp::dev;
+
+size_t own_lds_bytes(int P, bool fp64)
+    {
+    const size_t el = fp64 ? 8 : 4;
+    const int B = 8 + P - 1;  // largest box edge (T = 8)
+    const size_t BZ = size_t((B + 3) & ~3);
+    return 64 * BZ * el + 64 * 2 * size_t(P) * el + size_t(P) * (MAX_WINDOW_DEG + 1) * el + 64 * 4;
+    }
+
+template<typename R, int P>
+static void launch_spread_own(const MeshGeom& g, int D, unsigned int n_tiles, const unsigned int* start, const void* rec,
+                              const SortedIdx* sidx, const void* wcoef, void* mesh, hipStream_t s)
+    {
+    const size_t lds = own_lds_bytes(P, sizeof(R) == 8);
+    const unsigned int threads = unsigned((g.B * g.B + 63) / 64 * 64);
+    const auto* rr = reinterpret_cast<const SortedRec<R>*>(rec);
+    const auto* wc = reinterpret_cast<const R*>(wcoef);
+    auto* m = reinterpret_cast<typename Acc<R>::glob_t*>(mesh);
+    if (g.T == 8)
+        hipLaunchKernelGGL((k_spread_own<R, P, 8>), dim3(n_tiles), dim3(threads), lds, s, g, D, start, rr, sidx, wc, m);
+    else if (g.T == 4)
+        hipLaunchKernelGGL((k_spread_own<R, P, 4>), dim3(n_tiles), dim3(threads), lds, s, g, D, start, rr, sidx, wc, m);
+    else
+        hipLaunchKernelGGL((k_spread_own<R, P, 2>), dim3(n_tiles), dim3(threads), lds, s, g, D, start, rr, sidx, wc, m);
+    }
+
+//! Stencil kernel variant: 0 = wave segments per particle, 1 = thread per particle,
+//! 2 = column ownership (spread only).
+//! Defaults (measured on MI355X at N = 1e6, 128^3, P = 10): FP64 spreads by column ownership,
+//! FP32 by thread-per-particle LDS atomics; both gather thread-per-particle.  The environment
+//! variables ESP_SPREAD_VARIANT / ESP_GATHER_VARIANT override them.
+static int stencil_variant(int which, bool fp64)
+    {
+    static int v[2] = {-2, -2};
+    if (v[which] == -2)
+        {
+        const char* e = getenv(which == 0 ? "ESP_SPREAD_VARIANT" : "ESP_GATHER_VARIANT");
+        v[which] = e ? atoi(e) : -1;
+        }
+    if (v[which] >= 0)
+        return v[which];
+    return which == 0 ? (fp64 ? 2 : 1) : 1;
+    }
+
+size_t tp_lds_bytes(const MeshGeom& g, int P, bool fp64)
+    {
+    const size_t B3 = size_t(g.B) * g.B * g.B;
+    const size_t el = fp64 ? 8 : 4;
+    return (((B3 + 1) * el + 15) & ~size_t(15)) + size_t(P) * (MAX_WINDOW_DEG + 1) * el;
+    }
+
+size_t spread_lds_bytes(const MeshGeom& g, int P, int D, bool fp64)
+    {
+    (void)D;
+    const size_t B3 = size_t(g.B) * g.B * g.B;
+    const size_t el = fp64 ? 8 : 4;
+    const size_t CH = fp64 ? chunk_size<double>() : chunk_size<float>();
+    const size_t box = ((B3 + 1) * el + 15) & ~size_t(15);
+    const size_t w = size_t(CH) * 3 * (P + 1) * el;
+    const size_t wc = size_t(P) * (MAX_WINDOW_DEG + 1) * el;
+    return box + w + wc + CH * el + 2 * CH * 4;
+    }
+
+size_t gather_lds_bytes(const MeshGeom& g, int P, int D, bool fp64)
+    {
+    const size_t el = fp64 ? 8 : 4;
+    const size_t CH = fp64 ? chunk_size<double>() : chunk_size<float>();
+    return spread_lds_bytes(g, P, D, fp64) + CH * 3 * (P + 1) * el;
+    }
+
+#define ESP_P_CASES(X) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16)
+
+template<typename R>
+hipError_t spread(const MeshGeom& g, int P, int D, unsigned int n_tiles, const unsigned int* start, const void* rec,
+                  const SortedIdx* sidx, const void* wcoef, void* mesh, hipStream_t s)
+    {
+    const int variant = stencil_variant(0, sizeof(R) == 8);
+    const bool tp = variant == 1;
+    const size_t lds = tp ? tp_lds_bytes(g, P, sizeof(R) == 8) : spread_lds_bytes(g, P, D, sizeof(R) == 8);
+    switch (P)
+        {
+#define ESP_CASE(PP)                                                                                                         \
+    case PP:                                                                                                                 \
+        if (variant == 2)                                                                                                    \
+            launch_spread_own<R, PP>(g, D, n_tiles, start, rec, sidx, wcoef, mesh, s);                                      \
+        else if (tp)                                                                                                         \
+            hipLaunchKernelGGL((k_spread_tp<R, PP>), dim3(n_tiles), dim3(256), lds, s, g, D, start,                         \
+                               reinterpret_cast<const SortedRec<R>*>(rec), sidx, reinterpret_cast<const R*>(wcoef),          \
+                               reinterpret_cast<typename Acc<R>::glob_t*>(mesh));                                            \
+        else                                                                                                                 \
+            hipLaunchKernelGGL((k_spread<R, PP>), dim3(n_tiles), dim3(256), lds, s, g, D, start,                            \
+                               reinterpret_cast<const SortedRec<R>*>(rec), sidx, reinterpret_cast<const R*>(wcoef),          \
+                               reinterpret_cast<typename Acc<R>::glob_t*>(mesh));                                            \
+        break;
+        ESP_P_CASES(ESP_CASE)
+#undef ESP_CASE
+        default: return hipErrorInvalidValue;
+        }
+    return hipGetLastError();
+    }
+
+template<typename R>
+hipError_t gather(const MeshGeom& g, int P, int D, unsigned int n_tiles, const unsigned int* start, const void* rec,
+                  const SortedIdx* sidx, const void* wcoef, const void* phi, double* force, hipStream_t s)
+    {
+    const bool tp = stencil_variant(1, sizeof(R) == 8) == 1;
+    const size_t lds = tp ? tp_lds_bytes(g, P, sizeof(R) == 8) : gather_lds_bytes(g, P, D, sizeof(R) == 8);
+    switch (P)
+        {
+#define ESP_CASE(PP)                                                                                                         \
+    case PP:                                                                                                                 \
+        if (tp)                                                                                                              \
+            hipLaunchKernelGGL((k_gather_tp<R, PP>), dim3(n_tiles), dim3(256), lds, s, g, D, start,                         \
+                               reinterpret_cast<const SortedRec<R>*>(rec), sidx, reinterpret_cast<const R*>(wcoef),          \
+                               reinterpret_cast<const R*>(phi), reinterpret_cast<double4*>(force));                          \
+        else                                                                                                                 \
+            hipLaunchKernelGGL((k_gather<R, PP>), dim3(n_tiles), dim3(256), lds, s, g, D, start,                            \
+                               reinterpret_cast<const SortedRec<R>*>(rec), sidx, reinterpret_cast<const R*>(wcoef),          \
+                               reinterpret_cast<const R*>(phi), reinterpret_cast<double4*>(force));                          \
+        break;
+        ESP_P_CASES(ESP_CASE)
+#undef ESP_CASE
+        default: return hipErrorInvalidValue;
+        }
+    return hipGetLastError();
+    }
+
+template hipError_t spread<double>(const MeshGeom&, int, int, unsigned int, const unsigned int*, const void*, const SortedIdx*,
+                                   const void*, void*, hipStream_t);
+template hipError_t spread<float>(const MeshGeom&, int, int, unsigned int, const unsigned int*, const void*, const SortedIdx*,
+                                  const void*, void*, hipStream_t);
+template hipError_t gather<double>(const MeshGeom&, int, int, unsigned int, const unsigned int*, const void*, const SortedIdx*,
+                                   const void*, const void*, double*, hipStream_t);
+template hipError_t gather<float>(const MeshGeom&, int, int, unsigned int, const unsigned int*, const void*, const SortedIdx*,
+                                  const void*, const void*, double*, hipStream_t);
+
+    } // namespace kernel
+    } // namespace esp

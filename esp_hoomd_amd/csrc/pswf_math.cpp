@@ -386,6 +386,24 @@ void build_split_table(const SplitFunction& split,
         }
     }
 
+void build_compact_split(const SplitFunction& split, unsigned int nseg, unsigned int deg, std::vector<double>& coef)
+    {
+    if (nseg < 1 || deg < 1 || deg > 15)
+        throw std::invalid_argument("build_compact_split: invalid segment count or degree");
+    coef.assign(size_t(nseg) * (deg + 1), 0.0);
+    for (unsigned int i = 0; i < nseg; ++i)
+        {
+        std::vector<double> m = cheb_fit_monomial(deg,
+                                                  [&](double z)
+                                                  {
+                                                      const double s = (double(i) + 0.5 * (z + 1.0)) / double(nseg);
+                                                      return split.S(s);
+                                                  });
+        for (unsigned int j = 0; j <= deg; ++j)
+            coef[size_t(i) * (deg + 1) + j] = m[j];
+        }
+    }
+
 // ---------------------------------------------------------------------------
 // Spreading window
 // ---------------------------------------------------------------------------

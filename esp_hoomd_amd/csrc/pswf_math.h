@@ -104,6 +104,10 @@ void build_split_table(const SplitFunction& split,
                        std::vector<double>& r_lo,
                        std::vector<double>& dr_inv);
 
+//! Compact device table of S(s): nseg segments, each a degree-deg polynomial in z = 2t - 1
+//! (t the offset inside the segment), coef[seg * (deg + 1) + j] for z^j.
+void build_compact_split(const SplitFunction& split, unsigned int nseg, unsigned int deg, std::vector<double>& coef);
+
 //! P-tap piecewise-polynomial spreading window.  Tap k (0 <= k < P) covers grid offsets
 //! d = k - P/2 + t, t in [0, 1); its weight is a polynomial of degree D in z = 2t - 1,
 //! stored as coef[k * (D + 1) + j] for z^j.  Normalised so that int W(d) dd = 1.
