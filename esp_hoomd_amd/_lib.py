@@ -141,6 +141,18 @@ def load() -> C.CDLL:
     _sig(lib, "esp_window_fourier", C.c_int, [C.c_uint, C.c_uint, _DP, C.c_int, _DP, _DP])
     _sig(lib, "esp_default_window_c", C.c_double, [C.c_uint, C.c_double])
     _sig(lib, "esp_default_window_degree", C.c_uint, [C.c_uint])
+    _sig(lib, "esp_nlist_create", C.c_int, [C.POINTER(_VP), C.c_double, C.c_double])
+    _sig(lib, "esp_nlist_destroy", C.c_int, [_VP])
+    _sig(lib, "esp_nlist_last_error", C.c_char_p, [_VP])
+    _sig(lib, "esp_nlist_build", C.c_int, [_VP, C.POINTER(EspSystem), _VP])
+    _sig(lib, "esp_nlist_update", C.c_int, [_VP, C.POINTER(EspSystem), C.POINTER(C.c_int), _VP])
+    _sig(lib, "esp_nlist_arrays", C.c_int, [_VP, C.POINTER(_VP), C.POINTER(_VP), C.POINTER(_VP), C.POINTER(C.c_uint)])
+    _sig(lib, "esp_nlist_stats", C.c_int, [_VP, C.POINTER(C.c_ulonglong)])
+    _sig(lib, "esp_wca_forces", C.c_int, [C.POINTER(EspSystem), C.c_double, C.c_double, C.POINTER(EspOut), _VP])
+    _sig(lib, "esp_integrate_first", C.c_int, [C.c_uint, _VP, _VP, _VP, C.c_double, _DP, _VP])
+    _sig(lib, "esp_integrate_second", C.c_int, [C.c_uint, _VP, _VP, C.c_double, C.c_double, C.c_double, C.c_uint64,
+                                                C.c_uint64, _VP])
+    _sig(lib, "esp_kinetic_potential", C.c_int, [C.c_uint, _VP, _VP, _DP, _DP])
     _sig(lib, "esp_device_count", C.c_int, [C.POINTER(C.c_int)])
     _sig(lib, "esp_set_device", C.c_int, [C.c_int])
     _sig(lib, "esp_device_malloc", C.c_int, [C.POINTER(_VP), C.c_size_t])

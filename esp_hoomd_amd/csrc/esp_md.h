@@ -1,0 +1,38 @@
+// Molecular-dynamics support around the ESP hot path: neighbour lists, WCA, integrators.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace esp
+{
+namespace md
+{
+
+struct CellGeom
+    {
+    int nx, ny, nz;
+    double Lx, Ly, Lz;
+    };
+
+hipError_t cell_count(unsigned int N, const double* pos, const CellGeom& cg, unsigned int* count, unsigned int* cell, unsigned int* slot,
+                      hipStream_t s);
+hipError_t cell_fill(unsigned int N, const unsigned int* cell, const unsigned int* slot, const unsigned int* start, unsigned int* members,
+                     hipStream_t s);
+hipError_t build_nlist(unsigned int N, const double* pos, const CellGeom& cg, const unsigned int* start, const unsigned int* members,
+                       double r_list, const unsigned int* n_ex, const unsigned int* ex_list, unsigned int ex_pitch, unsigned int nmax,
+                       unsigned int* n_neigh, unsigned int* nlist, uint64_t* head, unsigned int* overflow, hipStream_t s);
+hipError_t wca_forces(unsigned int N, const double* pos, double Lx, double Ly, double Lz, const unsigned int* n_neigh,
+                      const unsigned int* nlist, const uint64_t* head, double sigma, double epsilon, double* force, double* virial,
+                      size_t pitch, hipStream_t s);
+hipError_t vv_first(unsigned int N, double* pos, double* vel, const double* force, double dt, double Lx, double Ly, double Lz, hipStream_t s);
+hipError_t vv_second(unsigned int N, double* vel, const double* force, double dt, double gamma, double kT, uint64_t seed, uint64_t step,
+                     hipStream_t s);
+hipError_t max_displacement(unsigned int N, const double* pos, const double* last, double Lx, double Ly, double Lz, unsigned long long* out,
+                            hipStream_t s);
+hipError_t copy4(unsigned int N, const double* src, double* dst, hipStream_t s);
+hipError_t energies(unsigned int N, const double* vel, const double* force, double* part, hipStream_t s);
+
+    } // namespace md
+    } // namespace esp

@@ -1,0 +1,271 @@
+// C ABI of the MD support: neighbour-list objects, WCA forces, velocity-Verlet / Langevin
+// integration and energy reductions.
+#include "../../include/esp_mi355x.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+
+#include "esp_kernels.h"
+#include "esp_md.h"
+#include "esp_solver.h"
+
+struct esp_nlist
+    {
+    double r_cut = 0.0;
+    double r_buff = 0.0;
+    unsigned int nmax = 32;
+    unsigned int N = 0;
+    unsigned int n_cells = 0;
+    double box[3] = {0, 0, 0};
+    bool built = false;
+    unsigned long long builds = 0;
+    esp::DeviceBuffer n_neigh, nlist, head, cell, slot, count, start, members, overflow, last_pos, disp;
+    std::string last_error;
+    };
+
+namespace
+{
+thread_local std::string g_md_error;
+
+int fail(esp_nlist* nl, int code, const std::string& msg)
+    {
+    if (nl)
+        nl->last_error = msg;
+    g_md_error = msg;
+    return code;
+    }
+
+template<class Fn> int guard(esp_nlist* nl, Fn fn)
+    {
+    try
+        {
+        fn();
+        return ESP_OK;
+        }
+    catch (const esp::EspError& e)
+        {
+        return fail(nl, e.code(), e.what());
+        }
+    catch (const std::exception& e)
+        {
+        return fail(nl, ESP_ERR_STATE, e.what());
+        }
+    }
+
+void do_build(esp_nlist* nl, const esp_system* sys, hipStream_t s)
+    {
+    using esp::check_hip;
+    const unsigned int N = sys->N;
+    if (sys->box[3] != 0.0 || sys->box[4] != 0.0 || sys->box[5] != 0.0)
+        throw esp::EspError(ESP_ERR_UNSUPPORTED, "triclinic boxes are not supported");
+    const double r_list = nl->r_cut + nl->r_buff;
+    const double Lmin = std::min({sys->box[0], sys->box[1], sys->box[2]});
+    if (!(r_list > 0.0) || r_list >= 0.5 * Lmin)
+        throw esp::EspError(ESP_ERR_ARG, "r_cut + r_buff must be positive and below half the box length");
+    esp::md::CellGeom cg;
+    cg.Lx = sys->box[0];
+    cg.Ly = sys->box[1];
+    cg.Lz = sys->box[2];
+    cg.nx = std::max(1, int(std::floor(cg.Lx / r_list)));
+    cg.ny = std::max(1, int(std::floor(cg.Ly / r_list)));
+    cg.nz = std::max(1, int(std::floor(cg.Lz / r_list)));
+    // keep the cell count bounded for the single-workgroup scan
+    while (size_t(cg.nx) * cg.ny * cg.nz > (1u << 22))
+        {
+        cg.nx = std::max(1, cg.nx / 2);
+        cg.ny = std::max(1, cg.ny / 2);
+        cg.nz = std::max(1, cg.nz / 2);
+        }
+    const unsigned int nc = unsigned(cg.nx * cg.ny * cg.nz);
+    if (N > nl->N || nc != nl->n_cells || !nl->n_neigh.get())
+        {
+        nl->n_neigh.resize(size_t(std::max(N, 1u)) * 4);
+        nl->head.resize(size_t(std::max(N, 1u)) * 8);
+        nl->cell.resize(size_t(std::max(N, 1u)) * 4);
+        nl->slot.resize(size_t(std::max(N, 1u)) * 4);
+        nl->members.resize(size_t(std::max(N, 1u)) * 4);
+        nl->last_pos.resize(size_t(std::max(N, 1u)) * 32);
+        nl->count.resize(size_t(nc + 1) * 4, true);
+        nl->start.resize(size_t(nc + 1) * 4, true);
+        nl->overflow.resize(8, true);
+        nl->disp.resize(8, true);
+        nl->N = N;
+        nl->n_cells = nc;
+        }
+    for (int attempt = 0; attempt < 4; ++attempt)
+        {
+        nl->nlist.resize(size_t(std::max(N, 1u)) * nl->nmax * 4);
+        check_hip(hipMemsetAsync(nl->count.get(), 0, size_t(nc + 1) * 4, s), "hipMemsetAsync");
+        check_hip(hipMemsetAsync(nl->overflow.get(), 0, 4, s), "hipMemsetAsync");
+        check_hip(esp::md::cell_count(N, sys->postype, cg, nl->count.as<unsigned int>(), nl->cell.as<unsigned int>(),
+                                      nl->slot.as<unsigned int>(), s),
+                  "cell_count");
+        check_hip(esp::kernel::scan_tiles(nc, nl->count.as<unsigned int>(), nl->start.as<unsigned int>(), s), "scan");
+        check_hip(esp::md::cell_fill(N, nl->cell.as<unsigned int>(), nl->slot.as<unsigned int>(), nl->start.as<unsigned int>(),
+                                     nl->members.as<unsigned int>(), s),
+                  "cell_fill");
+        check_hip(esp::md::build_nlist(N, sys->postype, cg, nl->start.as<unsigned int>(), nl->members.as<unsigned int>(), r_list,
+                                       sys->n_ex, sys->ex_list, sys->ex_pitch, nl->nmax, nl->n_neigh.as<unsigned int>(),
+                                       nl->nlist.as<unsigned int>(), nl->head.as<uint64_t>(), nl->overflow.as<unsigned int>(), s),
+                  "build_nlist");
+        unsigned int over = 0;
+        check_hip(hipMemcpyAsync(&over, nl->overflow.get(), 4, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+        check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
+        if (over == 0)
+            {
+            check_hip(esp::md::copy4(N, sys->postype, nl->last_pos.as<double>(), s), "copy4");
+            nl->box[0] = sys->box[0];
+            nl->box[1] = sys->box[1];
+            nl->box[2] = sys->box[2];
+            nl->built = true;
+            nl->builds += 1;
+            return;
+            }
+        nl->nmax = ((over + over / 4) + 7) & ~7u;
+        }
+    throw esp::EspError(ESP_ERR_STATE, "neighbour list did not converge");
+    }
+} // namespace
+
+extern "C" {
+
+int esp_nlist_create(esp_nlist** nl, double r_cut, double r_buff)
+    {
+    if (!nl || !(r_cut > 0.0) || r_buff < 0.0)
+        return fail(nullptr, ESP_ERR_ARG, "esp_nlist_create: r_cut must be > 0 and r_buff >= 0");
+    *nl = new esp_nlist();
+    (*nl)->r_cut = r_cut;
+    (*nl)->r_buff = r_buff;
+    return ESP_OK;
+    }
+
+int esp_nlist_destroy(esp_nlist* nl)
+    {
+    delete nl;
+    return ESP_OK;
+    }
+
+const char* esp_nlist_last_error(const esp_nlist* nl)
+    {
+    return nl ? nl->last_error.c_str() : g_md_error.c_str();
+    }
+
+int esp_nlist_build(esp_nlist* nl, const esp_system* sys, esp_stream_t stream)
+    {
+    if (!nl || !sys || !sys->postype)
+        return fail(nl, ESP_ERR_ARG, "esp_nlist_build: null argument");
+    return guard(nl, [&] { do_build(nl, sys, static_cast<hipStream_t>(stream)); });
+    }
+
+int esp_nlist_update(esp_nlist* nl, const esp_system* sys, int* rebuilt, esp_stream_t stream)
+    {
+    if (!nl || !sys || !sys->postype)
+        return fail(nl, ESP_ERR_ARG, "esp_nlist_update: null argument");
+    return guard(nl,
+                 [&]
+                 {
+                     hipStream_t s = static_cast<hipStream_t>(stream);
+                     bool need = !nl->built || sys->N != nl->N || sys->box[0] != nl->box[0] || sys->box[1] != nl->box[1]
+                                 || sys->box[2] != nl->box[2];
+                     if (!need)
+                         {
+                         esp::check_hip(hipMemsetAsync(nl->disp.get(), 0, 8, s), "hipMemsetAsync");
+                         esp::check_hip(esp::md::max_displacement(sys->N, sys->postype, nl->last_pos.as<double>(), sys->box[0], sys->box[1],
+                                                                  sys->box[2], nl->disp.as<unsigned long long>(), s),
+                                        "max_displacement");
+                         double d2 = 0.0;
+                         esp::check_hip(hipMemcpyAsync(&d2, nl->disp.get(), 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+                         esp::check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
+                         need = d2 >= 0.25 * nl->r_buff * nl->r_buff;
+                         }
+                     if (need)
+                         do_build(nl, sys, s);
+                     if (rebuilt)
+                         *rebuilt = need ? 1 : 0;
+                 });
+    }
+
+int esp_nlist_arrays(esp_nlist* nl, const unsigned int** n_neigh, const unsigned int** nlist, const uint64_t** head_list, unsigned int* nmax)
+    {
+    if (!nl || !nl->built)
+        return fail(nl, ESP_ERR_STATE, "esp_nlist_arrays: list not built");
+    if (n_neigh)
+        *n_neigh = nl->n_neigh.as<unsigned int>();
+    if (nlist)
+        *nlist = nl->nlist.as<unsigned int>();
+    if (head_list)
+        *head_list = nl->head.as<uint64_t>();
+    if (nmax)
+        *nmax = nl->nmax;
+    return ESP_OK;
+    }
+
+int esp_nlist_stats(esp_nlist* nl, unsigned long long* builds)
+    {
+    if (!nl || !builds)
+        return ESP_ERR_ARG;
+    *builds = nl->builds;
+    return ESP_OK;
+    }
+
+int esp_wca_forces(const esp_system* sys, double sigma, double epsilon, esp_out* out, esp_stream_t stream)
+    {
+    if (!sys || !out || !out->force || !sys->nlist || !sys->n_neigh || !sys->head_list)
+        return fail(nullptr, ESP_ERR_ARG, "esp_wca_forces: needs a system with a neighbour list and a force output");
+    if (out->virial && out->virial_pitch < sys->N)
+        return fail(nullptr, ESP_ERR_ARG, "esp_wca_forces: virial_pitch < N");
+    const hipError_t e = esp::md::wca_forces(sys->N, sys->postype, sys->box[0], sys->box[1], sys->box[2], sys->n_neigh, sys->nlist,
+                                             sys->head_list, sigma, epsilon, out->force, out->virial, out->virial_pitch,
+                                             static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? ESP_OK : fail(nullptr, ESP_ERR_HIP, hipGetErrorString(e));
+    }
+
+int esp_integrate_first(unsigned int N, double* postype, double* vel, const double* force, double dt, const double* box, esp_stream_t stream)
+    {
+    if ((N > 0 && (!postype || !vel || !force)) || !box || !(dt > 0.0))
+        return fail(nullptr, ESP_ERR_ARG, "esp_integrate_first: invalid argument");
+    const hipError_t e = esp::md::vv_first(N, postype, vel, force, dt, box[0], box[1], box[2], static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? ESP_OK : fail(nullptr, ESP_ERR_HIP, hipGetErrorString(e));
+    }
+
+int esp_integrate_second(unsigned int N, double* vel, const double* force, double dt, double gamma, double kT, uint64_t seed,
+                         uint64_t step, esp_stream_t stream)
+    {
+    if ((N > 0 && (!vel || !force)) || !(dt > 0.0) || gamma < 0.0 || kT < 0.0)
+        return fail(nullptr, ESP_ERR_ARG, "esp_integrate_second: invalid argument");
+    const hipError_t e = esp::md::vv_second(N, vel, force, dt, gamma, kT, seed, step, static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? ESP_OK : fail(nullptr, ESP_ERR_HIP, hipGetErrorString(e));
+    }
+
+int esp_kinetic_potential(unsigned int N, const double* vel, const double* force, double* ke, double* pe)
+    {
+    if (!ke || !pe || (N > 0 && (!vel || !force)))
+        return fail(nullptr, ESP_ERR_ARG, "esp_kinetic_potential: invalid argument");
+    try
+        {
+        esp::DeviceBuffer part;
+        const unsigned int nb = (N + 255) / 256;
+        part.resize(size_t(std::max(nb, 1u)) * 16, true);
+        esp::check_hip(esp::md::energies(N, vel, force, part.as<double>(), nullptr), "energies");
+        std::vector<double> h(size_t(std::max(nb, 1u)) * 2, 0.0);
+        esp::check_hip(hipMemcpy(h.data(), part.get(), h.size() * 8, hipMemcpyDeviceToHost), "hipMemcpy");
+        double a = 0, b = 0;
+        for (unsigned int i = 0; i < nb; ++i)
+            {
+            a += h[2 * i];
+            b += h[2 * i + 1];
+            }
+        *ke = a;
+        *pe = b;
+        return ESP_OK;
+        }
+    catch (const esp::EspError& e)
+        {
+        return fail(nullptr, e.code(), e.what());
+        }
+    }
+
+} // extern "C"

@@ -1,0 +1,391 @@
+// Molecular-dynamics kernels around the ESP hot path (the roles HOOMD plays for the reference
+// plugin, ESPForceCompute.h:39,177): cell-list neighbour lists in the HOOMD full-list layout,
+// the WCA repulsion, velocity-Verlet NVE integration, a Langevin thermostat and the
+// displacement check that triggers neighbour-list rebuilds.
+#include "esp_device.h"
+#include "esp_md.h"
+
+namespace esp
+{
+namespace dev
+{
+
+__device__ __forceinline__ double wrap_min_image(double d, double L, double invL)
+    {
+    return d - L * rint(d * invL);
+    }
+
+__device__ __forceinline__ int cell_coord(double x, double L, int n)
+    {
+    double f = x / L + 0.5;
+    f -= floor(f);
+    int c = int(f * n);
+    return c >= n ? n - 1 : (c < 0 ? 0 : c);
+    }
+
+__global__ void __launch_bounds__(256) k_cell_count(unsigned int N, const double4* __restrict__ pos, md::CellGeom cg,
+                                                    unsigned int* __restrict__ count, unsigned int* __restrict__ cell,
+                                                    unsigned int* __restrict__ slot)
+    {
+    const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N)
+        return;
+    const double4 r = pos[i];
+    const int cx = cell_coord(r.x, cg.Lx, cg.nx), cy = cell_coord(r.y, cg.Ly, cg.ny), cz = cell_coord(r.z, cg.Lz, cg.nz);
+    const unsigned int c = unsigned((cz * cg.ny + cy) * cg.nx + cx);
+    cell[i] = c;
+    slot[i] = atomicAdd(&count[c], 1u);
+    }
+
+__global__ void __launch_bounds__(256) k_cell_fill(unsigned int N, const unsigned int* __restrict__ cell, const unsigned int* __restrict__ slot,
+                                                   const unsigned int* __restrict__ start, unsigned int* __restrict__ members)
+    {
+    const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N)
+        return;
+    members[start[cell[i]] + slot[i]] = i;
+    }
+
+//! Full neighbour list: for every particle, all others within r_list (minimum image), excluding
+//! the pairs in its exclusion list.  Rows of width nmax; overflow raises *overflow to the
+//! largest count seen so the host can grow nmax and rebuild.
+__global__ void __launch_bounds__(256) k_nlist(unsigned int N, const double4* __restrict__ pos, md::CellGeom cg,
+                                               const unsigned int* __restrict__ start, const unsigned int* __restrict__ members,
+                                               double r_list2, const unsigned int* __restrict__ n_ex,
+                                               const unsigned int* __restrict__ ex_list, unsigned int ex_pitch, unsigned int nmax,
+                                               unsigned int* __restrict__ n_neigh, unsigned int* __restrict__ nlist,
+                                               uint64_t* __restrict__ head, unsigned int* __restrict__ overflow)
+    {
+    const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N)
+        return;
+    const double4 ri = pos[i];
+    const int cx = cell_coord(ri.x, cg.Lx, cg.nx), cy = cell_coord(ri.y, cg.Ly, cg.ny), cz = cell_coord(ri.z, cg.Lz, cg.nz);
+    const double invLx = 1.0 / cg.Lx, invLy = 1.0 / cg.Ly, invLz = 1.0 / cg.Lz;
+    const unsigned int ne = n_ex ? n_ex[i] : 0u;
+    unsigned int k = 0;
+    // neighbour cells, each visited once even when an axis has fewer than 3 cells
+    const int rx = cg.nx >= 3 ? 1 : (cg.nx == 2 ? 1 : 0), ry = cg.ny >= 3 ? 1 : (cg.ny == 2 ? 1 : 0), rz = cg.nz >= 3 ? 1 : (cg.nz == 2 ? 1 : 0);
+    for (int dz = -rz; dz <= (cg.nz == 2 ? 0 : rz); ++dz)
+        for (int dy = -ry; dy <= (cg.ny == 2 ? 0 : ry); ++dy)
+            for (int dx = -rx; dx <= (cg.nx == 2 ? 0 : rx); ++dx)
+                {
+                const int x = (cx + dx + cg.nx) % cg.nx, y = (cy + dy + cg.ny) % cg.ny, z = (cz + dz + cg.nz) % cg.nz;
+                const unsigned int c = unsigned((z * cg.ny + y) * cg.nx + x);
+                for (unsigned int m = start[c]; m < start[c + 1]; ++m)
+                    {
+                    const unsigned int j = members[m];
+                    if (j == i)
+                        continue;
+                    const double4 rj = pos[j];
+                    const double ddx = wrap_min_image(ri.x - rj.x, cg.Lx, invLx);
+                    const double ddy = wrap_min_image(ri.y - rj.y, cg.Ly, invLy);
+                    const double ddz = wrap_min_image(ri.z - rj.z, cg.Lz, invLz);
+                    if (ddx * ddx + ddy * ddy + ddz * ddz >= r_list2)
+                        continue;
+                    bool excluded = false;
+                    for (unsigned int e = 0; e < ne; ++e)
+                        excluded |= (ex_list[size_t(i) * ex_pitch + e] == j);
+                    if (excluded)
+                        continue;
+                    if (k < nmax)
+                        nlist[size_t(i) * nmax + k] = j;
+                    ++k;
+                    }
+                }
+    n_neigh[i] = k < nmax ? k : nmax;
+    head[i] = uint64_t(i) * nmax;
+    if (k > nmax)
+        atomicMax(overflow, k);
+    }
+
+//! WCA (LJ truncated at 2^(1/6) sigma and shifted) over the full list; adds to force/virial.
+__global__ void __launch_bounds__(256) k_wca(unsigned int N, const double4* __restrict__ pos, double Lx, double Ly, double Lz,
+                                             const unsigned int* __restrict__ n_neigh, const unsigned int* __restrict__ nlist,
+                                             const uint64_t* __restrict__ head, double sigma, double epsilon, double4* __restrict__ force,
+                                             double* __restrict__ virial, size_t pitch)
+    {
+    const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N)
+        return;
+    const double4 ri = pos[i];
+    const double invLx = 1.0 / Lx, invLy = 1.0 / Ly, invLz = 1.0 / Lz;
+    const double rc2 = 1.2599210498948732 * sigma * sigma;  // (2^(1/6) sigma)^2
+    const double s2 = sigma * sigma;
+    double fx = 0, fy = 0, fz = 0, e = 0, v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
+    const unsigned int nn = n_neigh[i];
+    const uint64_t h = head[i];
+    for (unsigned int k = 0; k < nn; ++k)
+        {
+        const unsigned int j = nlist[h + k];
+        const double4 rj = pos[j];
+        const double dx = wrap_min_image(ri.x - rj.x, Lx, invLx);
+        const double dy = wrap_min_image(ri.y - rj.y, Ly, invLy);
+        const double dz = wrap_min_image(ri.z - rj.z, Lz, invLz);
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 >= rc2)
+            continue;
+        const double sr2 = s2 / r2;
+        const double sr6 = sr2 * sr2 * sr2;
+        const double fdivr = 24.0 * epsilon * sr6 * (2.0 * sr6 - 1.0) / r2;
+        fx += fdivr * dx;
+        fy += fdivr * dy;
+        fz += fdivr * dz;
+        e += 0.5 * (4.0 * epsilon * sr6 * (sr6 - 1.0) + epsilon);
+        const double hf = 0.5 * fdivr;
+        v0 += hf * dx * dx;
+        v1 += hf * dx * dy;
+        v2 += hf * dx * dz;
+        v3 += hf * dy * dy;
+        v4 += hf * dy * dz;
+        v5 += hf * dz * dz;
+        }
+    double4 f = force[i];
+    f.x += fx;
+    f.y += fy;
+    f.z += fz;
+    f.w += e;
+    force[i] = f;
+    if (virial)
+        {
+        virial[0 * pitch + i] += v0;
+        virial[1 * pitch + i] += v1;
+        virial[2 * pitch + i] += v2;
+        virial[3 * pitch + i] += v3;
+        virial[4 * pitch + i] += v4;
+        virial[5 * pitch + i] += v5;
+        }
+    }
+
+//! Counter-based uniform random number in (-1, 1) from (seed, step, particle, component).
+__device__ __forceinline__ double uniform_pm1(uint64_t seed, uint64_t step, uint64_t i, uint32_t c)
+    {
+    uint64_t x = seed * 0x9E3779B97F4A7C15ull ^ (step + 0x632BE59BD9B4E019ull) * 0xBF58476D1CE4E5B9ull;
+    x ^= (i * 4 + c + 1) * 0x94D049BB133111EBull;
+    // splitmix64 finaliser, twice
+    for (int r = 0; r < 2; ++r)
+        {
+        x ^= x >> 30;
+        x *= 0xBF58476D1CE4E5B9ull;
+        x ^= x >> 27;
+        x *= 0x94D049BB133111EBull;
+        x ^= x >> 31;
+        }
+    return (double(x >> 11) + 0.5) * (2.0 / 9007199254740992.0) - 1.0;
+    }
+
+//! Velocity-Verlet first half: v += f/(2m) dt; x += v dt (wrapped into the box).
+//! vel = (vx, vy, vz, mass).
+__global__ void __launch_bounds__(256) k_vv_first(unsigned int N, double4* __restrict__ pos, double4* __restrict__ vel,
+                                                  const double4* __restrict__ force, double dt, double Lx, double Ly, double Lz)
+    {
+    const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N)
+        return;
+    double4 v = vel[i];
+    const double4 f = force[i];
+    const double h = 0.5 * dt / v.w;
+    v.x += h * f.x;
+    v.y += h * f.y;
+    v.z += h * f.z;
+    double4 r = pos[i];
+    r.x += v.x * dt;
+    r.y += v.y * dt;
+    r.z += v.z * dt;
+    r.x -= Lx * rint(r.x / Lx);
+    r.y -= Ly * rint(r.y / Ly);
+    r.z -= Lz * rint(r.z / Lz);
+    pos[i] = r;
+    vel[i] = v;
+    }
+
+//! Velocity-Verlet second half with an optional Langevin bath (gamma > 0): the drag -gamma v and
+//! a uniform random force of variance 2 gamma kT / dt per component are added to f.
+__global__ void __launch_bounds__(256) k_vv_second(unsigned int N, double4* __restrict__ vel, const double4* __restrict__ force, double dt,
+                                                   double gamma, double kT, uint64_t seed, uint64_t step)
+    {
+    const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N)
+        return;
+    double4 v = vel[i];
+    const double4 f = force[i];
+    const double h = 0.5 * dt / v.w;
+    double bx = 0.0, by = 0.0, bz = 0.0;
+    if (gamma > 0.0)
+        {
+        // drag and noise act over the whole step (one Euler-Maruyama kick per step), so that
+        // the bath temperature is kT: per-step noise variance 2 gamma kT dt / m^2
+        const double amp = sqrt(6.0 * gamma * kT / dt);  // uniform(-1,1) has variance 1/3
+        bx = -gamma * v.x + amp * uniform_pm1(seed, step, i, 0);
+        by = -gamma * v.y + amp * uniform_pm1(seed, step, i, 1);
+        bz = -gamma * v.z + amp * uniform_pm1(seed, step, i, 2);
+        }
+    const double k = dt / v.w;
+    v.x += h * f.x + k * bx;
+    v.y += h * f.y + k * by;
+    v.z += h * f.z + k * bz;
+    vel[i] = v;
+    }
+
+//! Largest squared minimum-image displacement since the last neighbour-list build, as the bits
+//! of a non-negative double (monotone in unsigned comparison).
+__global__ void __launch_bounds__(256) k_max_disp(unsigned int N, const double4* __restrict__ pos, const double4* __restrict__ last,
+                                                  double Lx, double Ly, double Lz, unsigned long long* __restrict__ out)
+    {
+    const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double d2 = 0.0;
+    if (i < N)
+        {
+        const double4 a = pos[i], b = last[i];
+        const double dx = wrap_min_image(a.x - b.x, Lx, 1.0 / Lx);
+        const double dy = wrap_min_image(a.y - b.y, Ly, 1.0 / Ly);
+        const double dz = wrap_min_image(a.z - b.z, Lz, 1.0 / Lz);
+        d2 = dx * dx + dy * dy + dz * dz;
+        }
+    // wave maximum, then one atomic per wave
+    for (int off = 32; off > 0; off >>= 1)
+        d2 = fmax(d2, __shfl_xor(d2, off, 64));
+    if ((threadIdx.x & 63) == 0)
+        atomicMax(out, static_cast<unsigned long long>(__double_as_longlong(d2)));
+    }
+
+__global__ void __launch_bounds__(256) k_copy4(unsigned int N, const double4* __restrict__ src, double4* __restrict__ dst)
+    {
+    const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N)
+        dst[i] = src[i];
+    }
+
+//! Kinetic energy and potential energy (sum of force.w) partial sums per block.
+__global__ void __launch_bounds__(256) k_energies(unsigned int N, const double4* __restrict__ vel, const double4* __restrict__ force,
+                                                  double* __restrict__ part)
+    {
+    const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double ke = 0.0, pe = 0.0;
+    if (i < N)
+        {
+        const double4 v = vel[i];
+        ke = 0.5 * v.w * (v.x * v.x + v.y * v.y + v.z * v.z);
+        pe = force[i].w;
+        }
+    ke = wave_sum(ke);
+    pe = wave_sum(pe);
+    __shared__ double s[2][4];
+    if ((threadIdx.x & 63) == 0)
+        {
+        s[0][threadIdx.x >> 6] = ke;
+        s[1][threadIdx.x >> 6] = pe;
+        }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        {
+        double a = 0, b = 0;
+        for (int w = 0; w < int(blockDim.x >> 6); ++w)
+            {
+            a += s[0][w];
+            b += s[1][w];
+            }
+        part[2 * blockIdx.x] = a;
+        part[2 * blockIdx.x + 1] = b;
+        }
+    }
+
+    } // namespace dev
+
+namespace md
+{
+using namespace esp::dev;
+
+static unsigned int nblocks(unsigned int n)
+    {
+    return (n + 255u) / 256u;
+    }
+
+hipError_t cell_count(unsigned int N, const double* pos, const CellGeom& cg, unsigned int* count, unsigned int* cell, unsigned int* slot,
+                      hipStream_t s)
+    {
+    if (N == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_cell_count, dim3(nblocks(N)), dim3(256), 0, s, N, reinterpret_cast<const double4*>(pos), cg, count, cell, slot);
+    return hipGetLastError();
+    }
+
+hipError_t cell_fill(unsigned int N, const unsigned int* cell, const unsigned int* slot, const unsigned int* start, unsigned int* members,
+                     hipStream_t s)
+    {
+    if (N == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_cell_fill, dim3(nblocks(N)), dim3(256), 0, s, N, cell, slot, start, members);
+    return hipGetLastError();
+    }
+
+hipError_t build_nlist(unsigned int N, const double* pos, const CellGeom& cg, const unsigned int* start, const unsigned int* members,
+                       double r_list, const unsigned int* n_ex, const unsigned int* ex_list, unsigned int ex_pitch, unsigned int nmax,
+                       unsigned int* n_neigh, unsigned int* nlist, uint64_t* head, unsigned int* overflow, hipStream_t s)
+    {
+    if (N == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_nlist, dim3(nblocks(N)), dim3(256), 0, s, N, reinterpret_cast<const double4*>(pos), cg, start, members,
+                       r_list * r_list, n_ex, ex_list, ex_pitch, nmax, n_neigh, nlist, head, overflow);
+    return hipGetLastError();
+    }
+
+hipError_t wca_forces(unsigned int N, const double* pos, double Lx, double Ly, double Lz, const unsigned int* n_neigh,
+                      const unsigned int* nlist, const uint64_t* head, double sigma, double epsilon, double* force, double* virial,
+                      size_t pitch, hipStream_t s)
+    {
+    if (N == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_wca, dim3(nblocks(N)), dim3(256), 0, s, N, reinterpret_cast<const double4*>(pos), Lx, Ly, Lz, n_neigh, nlist, head,
+                       sigma, epsilon, reinterpret_cast<double4*>(force), virial, pitch);
+    return hipGetLastError();
+    }
+
+hipError_t vv_first(unsigned int N, double* pos, double* vel, const double* force, double dt, double Lx, double Ly, double Lz, hipStream_t s)
+    {
+    if (N == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_vv_first, dim3(nblocks(N)), dim3(256), 0, s, N, reinterpret_cast<double4*>(pos), reinterpret_cast<double4*>(vel),
+                       reinterpret_cast<const double4*>(force), dt, Lx, Ly, Lz);
+    return hipGetLastError();
+    }
+
+hipError_t vv_second(unsigned int N, double* vel, const double* force, double dt, double gamma, double kT, uint64_t seed, uint64_t step,
+                     hipStream_t s)
+    {
+    if (N == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_vv_second, dim3(nblocks(N)), dim3(256), 0, s, N, reinterpret_cast<double4*>(vel),
+                       reinterpret_cast<const double4*>(force), dt, gamma, kT, seed, step);
+    return hipGetLastError();
+    }
+
+hipError_t max_displacement(unsigned int N, const double* pos, const double* last, double Lx, double Ly, double Lz, unsigned long long* out,
+                            hipStream_t s)
+    {
+    if (N == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_max_disp, dim3(nblocks(N)), dim3(256), 0, s, N, reinterpret_cast<const double4*>(pos),
+                       reinterpret_cast<const double4*>(last), Lx, Ly, Lz, out);
+    return hipGetLastError();
+    }
+
+hipError_t copy4(unsigned int N, const double* src, double* dst, hipStream_t s)
+    {
+    if (N == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_copy4, dim3(nblocks(N)), dim3(256), 0, s, N, reinterpret_cast<const double4*>(src), reinterpret_cast<double4*>(dst));
+    return hipGetLastError();
+    }
+
+hipError_t energies(unsigned int N, const double* vel, const double* force, double* part, hipStream_t s)
+    {
+    if (N == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(k_energies, dim3(nblocks(N)), dim3(256), 0, s, N, reinterpret_cast<const double4*>(vel),
+                       reinterpret_cast<const double4*>(force), part);
+    return hipGetLastError();
+    }
+
+    } // namespace md
+    } // namespace esp

@@ -1,0 +1,309 @@
+"""Device-resident molecular dynamics around the ESP force (the roles HOOMD-blue plays for the
+reference plugin): particle state, a GPU cell-list neighbour list, the WCA repulsion and
+velocity-Verlet (NVE) / Langevin integration.
+
+Every per-step operation runs on the GPU through the C ABI; the host only sequences calls.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .device import DeviceArray, synchronize
+from .nlist import build_exclusion_list
+
+
+class State:
+    """Particle data in the HOOMD layout: postype (x, y, z, type), charge, velocity (vx, vy,
+    vz, mass), net force (fx, fy, fz, energy) and per-particle virial (6 x pitch)."""
+
+    def __init__(self, positions, charges, box, velocities=None, masses=1.0, types=None, exclusions=None):
+        pos = np.asarray(positions, dtype=np.float64)
+        self.N = len(pos)
+        box = list(box)
+        self.box = np.array(box[:3] + [0.0, 0.0, 0.0] if len(box) == 3 else box, dtype=np.float64)
+        pt = np.zeros((self.N, 4))
+        pt[:, :3] = pos
+        if types is not None:
+            pt[:, 3] = types
+        self.postype = DeviceArray.from_numpy(pt)
+        self.charge = DeviceArray.from_numpy(np.asarray(charges, dtype=np.float64))
+        v = np.zeros((self.N, 4))
+        if velocities is not None:
+            v[:, :3] = velocities
+        v[:, 3] = masses
+        self.vel = DeviceArray.from_numpy(v)
+        self.force = DeviceArray((self.N, 4), np.float64, zero=True)
+        self.virial_pitch = max(self.N, 1)
+        self.virial = DeviceArray((6, self.virial_pitch), np.float64, zero=True)
+        self.ext = DeviceArray((8,), np.float64, zero=True)
+        self.exclusions = list(exclusions) if exclusions else None
+        self.n_ex = self.ex_list = None
+        self.ex_pitch = 0
+        if self.exclusions:
+            n_ex, ex_list, pitch = build_exclusion_list(self.N, self.exclusions)
+            self.n_ex = DeviceArray.from_numpy(n_ex)
+            self.ex_list = DeviceArray.from_numpy(ex_list)
+            self.ex_pitch = pitch
+        self.nlist_ptrs = None  # (n_neigh, nlist, head) device pointers of the current list
+
+    @property
+    def volume(self) -> float:
+        return float(np.prod(self.box[:3]))
+
+    def c_system(self) -> _lib.EspSystem:
+        s = _lib.EspSystem()
+        s.N = self.N
+        s.postype = self.postype.ptr
+        s.charge = self.charge.ptr
+        for i in range(6):
+            s.box[i] = float(self.box[i])
+        if self.nlist_ptrs is not None:
+            s.n_neigh, s.nlist, s.head_list = self.nlist_ptrs
+        if self.n_ex is not None:
+            s.n_ex, s.ex_list, s.ex_pitch = self.n_ex.ptr, self.ex_list.ptr, self.ex_pitch
+        return s
+
+    def c_out(self, flags=0) -> _lib.EspOut:
+        o = _lib.EspOut()
+        o.force, o.virial, o.virial_pitch = self.force.ptr, self.virial.ptr, self.virial_pitch
+        o.ext_energy_virial = self.ext.ptr
+        o.flags = flags
+        return o
+
+    def positions(self) -> np.ndarray:
+        return self.postype.numpy()[:, :3]
+
+    def velocities(self) -> np.ndarray:
+        return self.vel.numpy()[:, :3]
+
+    def set_positions(self, pos) -> None:
+        pt = self.postype.numpy()
+        pt[:, :3] = pos
+        self.postype.upload(pt)
+
+    def set_velocities(self, v) -> None:
+        a = self.vel.numpy()
+        a[:, :3] = v
+        self.vel.upload(a)
+
+    def thermalize(self, kT: float, seed: int = 1) -> None:
+        """Maxwell-Boltzmann velocities at kT with zero net momentum."""
+        m = self.vel.numpy()[:, 3]
+        rng = np.random.default_rng(seed)
+        v = rng.standard_normal((self.N, 3)) * np.sqrt(kT / m)[:, None]
+        v -= (m[:, None] * v).sum(0) / m.sum()
+        self.set_velocities(v)
+
+
+class NeighborList:
+    """GPU cell-list neighbour list (full list, HOOMD layout) with a skin `buffer`; rebuilt when
+    a particle has moved more than buffer / 2 since the last build."""
+
+    def __init__(self, buffer: float = 0.4):
+        if buffer < 0:
+            raise ValueError("NeighborList: buffer must be >= 0")
+        self.buffer = float(buffer)
+        self.r_cut = 0.0
+        self._nl = None
+
+    def _ensure(self, r_cut: float):
+        lib = _lib.load()
+        if self._nl is None or r_cut > self.r_cut:
+            if self._nl is not None:
+                lib.esp_nlist_destroy(self._nl)
+            nl = C.c_void_p()
+            _lib.check(lib.esp_nlist_create(C.byref(nl), r_cut, self.buffer))
+            self._nl = nl
+            self.r_cut = r_cut
+
+    def update(self, state: State, r_cut: float, force_rebuild: bool = False) -> bool:
+        lib = _lib.load()
+        self._ensure(r_cut)
+        s = state.c_system()
+        rebuilt = C.c_int(0)
+        if force_rebuild:
+            rc = lib.esp_nlist_build(self._nl, C.byref(s), None)
+            rebuilt.value = 1
+        else:
+            rc = lib.esp_nlist_update(self._nl, C.byref(s), C.byref(rebuilt), None)
+        if rc != _lib.ESP_OK:
+            raise _lib.EspError(rc, lib.esp_nlist_last_error(self._nl).decode())
+        a, b, c = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        nmax = C.c_uint()
+        _lib.check(lib.esp_nlist_arrays(self._nl, C.byref(a), C.byref(b), C.byref(c), C.byref(nmax)))
+        state.nlist_ptrs = (a.value, b.value, c.value)
+        self.nmax = nmax.value
+        return bool(rebuilt.value)
+
+    @property
+    def builds(self) -> int:
+        n = C.c_ulonglong()
+        _lib.check(_lib.load().esp_nlist_stats(self._nl, C.byref(n)))
+        return int(n.value)
+
+    def host_arrays(self, state: State):
+        """(n_neigh, [neighbour indices per particle]) copied to the host (for tests)."""
+        import ctypes
+        n = np.empty(state.N, np.uint32)
+        _lib.check(_lib.load().esp_memcpy_dtoh(n.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(state.nlist_ptrs[0]),
+                                               n.nbytes))
+        flat = np.empty(state.N * self.nmax, np.uint32)
+        _lib.check(_lib.load().esp_memcpy_dtoh(flat.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(state.nlist_ptrs[1]),
+                                               flat.nbytes))
+        flat = flat.reshape(state.N, self.nmax)
+        return n, [flat[i, : n[i]] for i in range(state.N)]
+
+    def __del__(self):
+        try:
+            if self._nl is not None:
+                _lib.load().esp_nlist_destroy(self._nl)
+        except Exception:
+            pass
+
+
+class WCA:
+    """Weeks-Chandler-Andersen repulsion: LJ truncated at 2^(1/6) sigma and shifted by epsilon."""
+
+    def __init__(self, sigma: float = 1.0, epsilon: float = 1.0):
+        self.sigma = float(sigma)
+        self.epsilon = float(epsilon)
+
+    @property
+    def r_cut(self) -> float:
+        return 2.0 ** (1.0 / 6.0) * self.sigma
+
+    def add_to(self, state: State, stream=None) -> None:
+        s = state.c_system()
+        o = state.c_out()
+        _lib.check(_lib.load().esp_wca_forces(C.byref(s), self.sigma, self.epsilon, C.byref(o),
+                                              stream.handle if stream else None))
+
+
+class NVE:
+    """Velocity-Verlet integration."""
+
+    gamma = 0.0
+    kT = 0.0
+    seed = 0
+
+
+class Langevin:
+    """Velocity Verlet with a Langevin bath (drag gamma, temperature kT, counter-based RNG)."""
+
+    def __init__(self, kT: float, gamma: float = 1.0, seed: int = 4):
+        if kT < 0 or gamma <= 0:
+            raise ValueError("Langevin: kT must be >= 0 and gamma > 0")
+        self.kT, self.gamma, self.seed = float(kT), float(gamma), int(seed)
+
+
+class Simulation:
+    """Minimal MD driver: forces (ESP Spectral/Local, WCA) on a shared neighbour list and one
+    integration method, all device resident."""
+
+    def __init__(self, state: State, dt: float, method=None, nlist: NeighborList | None = None):
+        if dt <= 0:
+            raise ValueError("Simulation: dt must be > 0")
+        self.state = state
+        self.dt = float(dt)
+        self.method = method if method is not None else NVE()
+        self.nlist = nlist if nlist is not None else NeighborList(0.4)
+        self.forces = []
+        self.timestep = 0
+        self._forces_valid = False
+
+    def add_force(self, f) -> None:
+        self.forces.append(f)
+        if hasattr(f, "_attach"):
+            f._attach(self)
+        self._forces_valid = False
+
+    def _r_cut(self) -> float:
+        rc = 0.0
+        for f in self.forces:
+            rc = max(rc, getattr(f, "r_cut", 0.0) or 0.0)
+        return rc
+
+    def compute_forces(self) -> None:
+        st = self.state
+        if self.forces:
+            self.nlist.update(st, self._r_cut())
+        # the ESP objects (Spectral mesh part, Local pair part) share one solver: one call with
+        # the union of their flags overwrites the net force; other forces then add to it
+        flags = 0
+        solver = None
+        for f in self.forces:
+            if hasattr(f, "_esp_flags"):
+                flags |= f._esp_flags()
+                solver = solver or f._esp_solver()
+        if solver is not None:
+            solver.enqueue_state(st, self.timestep, flags)
+        else:
+            st.force.zero()
+            st.virial.zero()
+            st.ext.zero()
+        for f in self.forces:
+            if isinstance(f, WCA):
+                f.add_to(st)
+        self._forces_valid = True
+
+    def run(self, steps: int) -> None:
+        lib = _lib.load()
+        st = self.state
+        if not self._forces_valid:
+            self.compute_forces()
+        box = np.ascontiguousarray(st.box[:3])
+        for _ in range(int(steps)):
+            _lib.check(lib.esp_integrate_first(st.N, st.postype.ptr, st.vel.ptr, st.force.ptr, self.dt, _lib.dptr(box), None))
+            self.timestep += 1
+            self.compute_forces()
+            m = self.method
+            _lib.check(lib.esp_integrate_second(st.N, st.vel.ptr, st.force.ptr, self.dt, m.gamma, m.kT, m.seed,
+                                                self.timestep, None))
+        synchronize()
+
+    def energies(self):
+        """(kinetic, potential) with the ESP external (mesh, self, background) energy included."""
+        ke, pe = C.c_double(), C.c_double()
+        _lib.check(_lib.load().esp_kinetic_potential(self.state.N, self.state.vel.ptr, self.state.force.ptr,
+                                                      C.byref(ke), C.byref(pe)))
+        ext = self.state.ext.numpy()[0] if any(hasattr(f, "_esp_flags") for f in self.forces) else 0.0
+        return ke.value, pe.value + ext
+
+    def temperature(self) -> float:
+        ke, _ = self.energies()
+        return 2.0 * ke / (3.0 * (self.state.N - 1)) if self.state.N > 1 else 0.0
+
+
+def jittered_lattice(N: int, rho: float, seed: int = 2, jitter: float = 0.1):
+    """Seeded jittered simple-cubic lattice of N = m^3 sites at density rho (box centred on 0)."""
+    m = round(N ** (1.0 / 3.0))
+    if m ** 3 != N:
+        raise ValueError("N must be a perfect cube")
+    L = (N / rho) ** (1.0 / 3.0)
+    a = L / m
+    g = (np.arange(m) + 0.5) * a - L / 2
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    pos = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    pos += np.random.default_rng(seed).uniform(-jitter * a, jitter * a, pos.shape)
+    return pos, L
+
+
+def random_packing(N: int, L: float, min_dist: float, seed: int = 0):
+    """Uniform positions in a cubic box with every pair at least min_dist apart (rejection)."""
+    from scipy.spatial import cKDTree
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(-L / 2, L / 2, (N, 3))
+    for _ in range(1000):
+        t = cKDTree(pos + L / 2, boxsize=L)
+        pairs = t.query_pairs(min_dist, output_type="ndarray")
+        if len(pairs) == 0:
+            return pos
+        bad = np.unique(pairs[:, 1])
+        pos[bad] = rng.uniform(-L / 2, L / 2, (len(bad), 3))
+    raise RuntimeError("random_packing: could not place particles")
+
+
+__all__ = ["State", "NeighborList", "WCA", "NVE", "Langevin", "Simulation", "jittered_lattice", "random_packing"]

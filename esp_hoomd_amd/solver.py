@@ -147,6 +147,13 @@ class Solver:
         _lib.check(_lib.load().esp_compute_forces(self._ctx, timestep, C.byref(s), C.byref(o),
                                                   stream.handle if stream is not None else None), self._ctx)
 
+    def enqueue_state(self, state, timestep: int = 0, flags: int = 0, stream=None) -> None:
+        """Enqueue on an md.State (device arrays including its current neighbour list)."""
+        s = state.c_system()
+        o = state.c_out(flags)
+        _lib.check(_lib.load().esp_compute_forces(self._ctx, timestep, C.byref(s), C.byref(o),
+                                                  stream.handle if stream is not None else None), self._ctx)
+
     def compute(self, system: DeviceSystem, timestep: int = 0, flags: int = 0):
         """Run one evaluation and return host copies: force (N,4), virial (6,N), ext (7,)."""
         self.enqueue(system, timestep, flags)

@@ -182,6 +182,33 @@ int esp_window_fourier(unsigned int P, unsigned int D, const double* coef, int n
 double esp_default_window_c(unsigned int P, double sigma);
 unsigned int esp_default_window_degree(unsigned int P);
 
+/* ---- MD support: the roles HOOMD plays around the plugin ---------------------------------
+ * Neighbour list (HOOMD NeighborList consumed by ESPForceCompute, ESPForceCompute.h:39,177 and
+ * by PotentialPair<EvaluatorPairPSWF>, module.cc:73,79): cell-list build in the full-list
+ * layout, rebuilt when any particle moved more than r_buff / 2. */
+typedef struct esp_nlist esp_nlist;
+int esp_nlist_create(esp_nlist** nl, double r_cut, double r_buff);
+int esp_nlist_destroy(esp_nlist* nl);
+const char* esp_nlist_last_error(const esp_nlist* nl);
+/* build now (uses sys->postype, N, box and the exclusion list; synchronises the stream) */
+int esp_nlist_build(esp_nlist* nl, const esp_system* sys, esp_stream_t stream);
+/* rebuild if needed; *rebuilt tells whether it did (synchronises the stream) */
+int esp_nlist_update(esp_nlist* nl, const esp_system* sys, int* rebuilt, esp_stream_t stream);
+/* device arrays of the current list (HOOMD layout) */
+int esp_nlist_arrays(esp_nlist* nl, const unsigned int** n_neigh, const unsigned int** nlist, const uint64_t** head_list,
+                     unsigned int* nmax);
+int esp_nlist_stats(esp_nlist* nl, unsigned long long* builds);
+/* WCA repulsion (LJ truncated at 2^(1/6) sigma, shifted) added to out->force / out->virial */
+int esp_wca_forces(const esp_system* sys, double sigma, double epsilon, esp_out* out, esp_stream_t stream);
+/* velocity Verlet: vel = (vx, vy, vz, mass) per particle; box = (Lx, Ly, Lz) */
+int esp_integrate_first(unsigned int N, double* postype, double* vel, const double* force, double dt, const double* box,
+                        esp_stream_t stream);
+/* second half; gamma > 0 adds a Langevin bath at temperature kT (counter-based RNG: seed, step) */
+int esp_integrate_second(unsigned int N, double* vel, const double* force, double dt, double gamma, double kT, uint64_t seed,
+                         uint64_t step, esp_stream_t stream);
+/* total kinetic energy and sum of per-particle energies (force.w); synchronises */
+int esp_kinetic_potential(unsigned int N, const double* vel, const double* force, double* ke, double* pe);
+
 /* ---- device utilities (plumbing for callers without a device allocator) ---------------- */
 int esp_device_count(int* n);
 int esp_set_device(int device);
