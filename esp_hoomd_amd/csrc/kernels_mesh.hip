@@ -44,6 +44,34 @@ __device__ __forceinline__ int stencil_origin(double x, double L, int n, int P, 
 // ---------------------------------------------------------------------------
 // Binning into spreading tiles (counting sort: count, scan, scatter)
 // ---------------------------------------------------------------------------
+//! Bin group member i into its spreading tile; returns its charge.
+__device__ __forceinline__ double bin_one(unsigned int i,
+                                          const unsigned int* __restrict__ group,
+                                          const double4* __restrict__ postype,
+                                          const double* __restrict__ charge,
+                                          const MeshGeom& g,
+                                          int P,
+                                          unsigned int* __restrict__ tile_count,
+                                          unsigned int* __restrict__ key,
+                                          unsigned int* __restrict__ slot)
+    {
+    const unsigned int p = group ? group[i] : i;
+    const double4 r = postype[p];
+    const double q = charge[p];
+    unsigned int k = 0xffffffffu;
+    if (q != 0.0)
+        {
+        double t;
+        const int ox = stencil_origin(r.x, g.Lx, g.nx, P, t);
+        const int oy = stencil_origin(r.y, g.Ly, g.ny, P, t);
+        const int oz = stencil_origin(r.z, g.Lz, g.nz, P, t);
+        k = unsigned((oz / g.T) * g.ty + oy / g.T) * unsigned(g.tx) + unsigned(ox / g.T);
+        slot[i] = atomicAdd(&tile_count[k], 1u);
+        }
+    key[i] = k;
+    return q;
+    }
+
 __global__ void __launch_bounds__(256) k_bin(unsigned int n,
                                              const unsigned int* __restrict__ group,
                                              const double4* __restrict__ postype,
@@ -58,22 +86,7 @@ __global__ void __launch_bounds__(256) k_bin(unsigned int n,
     const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
     double q = 0.0;
     if (i < n)
-        {
-        const unsigned int p = group ? group[i] : i;
-        const double4 r = postype[p];
-        q = charge[p];
-        unsigned int k = 0xffffffffu;
-        if (q != 0.0)
-            {
-            double t;
-            const int ox = stencil_origin(r.x, g.Lx, g.nx, P, t);
-            const int oy = stencil_origin(r.y, g.Ly, g.ny, P, t);
-            const int oz = stencil_origin(r.z, g.Lz, g.nz, P, t);
-            k = unsigned((oz / g.T) * g.ty + oy / g.T) * unsigned(g.tx) + unsigned(ox / g.T);
-            slot[i] = atomicAdd(&tile_count[k], 1u);
-            }
-        key[i] = k;
-        }
+        q = bin_one(i, group, postype, charge, g, P, tile_count, key, slot);
     // block partial sums of q and q^2 (deterministic order)
     __shared__ double sq[8], sq2[8];
     double a = wave_sum(q), b = wave_sum(q * q);
@@ -97,8 +110,8 @@ __global__ void __launch_bounds__(256) k_bin(unsigned int n,
         }
     }
 
-//! Exclusive scan of tile counts by one workgroup; clears the counts for the next step.
-__global__ void __launch_bounds__(1024) k_scan(unsigned int n_tiles, unsigned int* __restrict__ count, unsigned int* __restrict__ start)
+//! Exclusive scan of tile counts by one 1024-thread workgroup; clears the counts for the next step.
+__device__ __forceinline__ void scan_one(unsigned int n_tiles, unsigned int* __restrict__ count, unsigned int* __restrict__ start)
     {
     __shared__ unsigned int part[1024];
     const unsigned int tid = threadIdx.x;
@@ -130,22 +143,25 @@ __global__ void __launch_bounds__(1024) k_scan(unsigned int n_tiles, unsigned in
         start[n_tiles] = part[1023];
     }
 
-template<typename R>
-__global__ void __launch_bounds__(256) k_scatter(unsigned int n,
-                                                 const unsigned int* __restrict__ group,
-                                                 const double4* __restrict__ postype,
-                                                 const double* __restrict__ charge,
-                                                 MeshGeom g,
-                                                 int P,
-                                                 const unsigned int* __restrict__ key,
-                                                 const unsigned int* __restrict__ slot,
-                                                 const unsigned int* __restrict__ start,
-                                                 SortedRec<R>* __restrict__ rec,
-                                                 SortedIdx* __restrict__ sidx)
+__global__ void __launch_bounds__(1024) k_scan(unsigned int n_tiles, unsigned int* __restrict__ count, unsigned int* __restrict__ start)
     {
-    const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
+    scan_one(n_tiles, count, start);
+    }
+
+//! Write group member i's record into its tile-sorted slot.
+template<typename R>
+__device__ __forceinline__ void scatter_one(unsigned int i,
+                                            const unsigned int* __restrict__ group,
+                                            const double4* __restrict__ postype,
+                                            const double* __restrict__ charge,
+                                            const MeshGeom& g,
+                                            int P,
+                                            const unsigned int* __restrict__ key,
+                                            const unsigned int* __restrict__ slot,
+                                            const unsigned int* __restrict__ start,
+                                            SortedRec<R>* __restrict__ rec,
+                                            SortedIdx* __restrict__ sidx)
+    {
     const unsigned int k = key[i];
     if (k == 0xffffffffu)
         return;
@@ -168,6 +184,24 @@ __global__ void __launch_bounds__(256) k_scatter(unsigned int n,
     sidx[pos] = s;
     }
 
+template<typename R>
+__global__ void __launch_bounds__(256) k_scatter(unsigned int n,
+                                                 const unsigned int* __restrict__ group,
+                                                 const double4* __restrict__ postype,
+                                                 const double* __restrict__ charge,
+                                                 MeshGeom g,
+                                                 int P,
+                                                 const unsigned int* __restrict__ key,
+                                                 const unsigned int* __restrict__ slot,
+                                                 const unsigned int* __restrict__ start,
+                                                 SortedRec<R>* __restrict__ rec,
+                                                 SortedIdx* __restrict__ sidx)
+    {
+    const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        scatter_one<R>(i, group, postype, charge, g, P, key, slot, start, rec, sidx);
+    }
+
 // ---------------------------------------------------------------------------
 // FFT passes.  Layout everywhere: [z][y][x], x fastest; the spectrum is the
 // half spectrum along x (nxh = nx/2 + 1 columns).
@@ -175,12 +209,13 @@ __global__ void __launch_bounds__(256) k_scatter(unsigned int n,
 
 //! R2C along x for pairs of real lines packed into one complex FFT; reads and clears the mesh.
 template<typename R>
-__global__ void __launch_bounds__(256) k_fft_x_r2c(MeshGeom g,
-                                                   FFTPlan plan,
-                                                   int pairs_per_block,
-                                                   const C2<R>* __restrict__ tw_g,
-                                                   typename Acc<R>::glob_t* __restrict__ mesh,
-                                                   C2<R>* __restrict__ spec)
+__device__ __forceinline__ void fft_x_r2c_block(int blk,
+                                                const MeshGeom& g,
+                                                const FFTPlan& plan,
+                                                int pairs_per_block,
+                                                const C2<R>* __restrict__ tw_g,
+                                                typename Acc<R>::glob_t* __restrict__ mesh,
+                                                C2<R>* __restrict__ spec)
     {
     extern __shared__ __align__(16) unsigned char smem[];
     const int nx = g.nx;
@@ -189,7 +224,7 @@ __global__ void __launch_bounds__(256) k_fft_x_r2c(MeshGeom g,
     C2<R>* a = tw + nx;
     C2<R>* b = a + pairs_per_block * ls;
     const int nlines = g.ny * g.nz;
-    const int pair0 = blockIdx.x * pairs_per_block;
+    const int pair0 = blk * pairs_per_block;
     for (int i = threadIdx.x; i < nx; i += blockDim.x)
         tw[i] = tw_g[i];
     for (int idx = threadIdx.x; idx < pairs_per_block * nx; idx += blockDim.x)
@@ -226,15 +261,27 @@ __global__ void __launch_bounds__(256) k_fft_x_r2c(MeshGeom g,
         }
     }
 
-//! C2R along x for pairs of half-spectrum lines; phi = scale * real(IFFT).
 template<typename R>
-__global__ void __launch_bounds__(256) k_fft_x_c2r(MeshGeom g,
+__global__ void __launch_bounds__(256) k_fft_x_r2c(MeshGeom g,
                                                    FFTPlan plan,
                                                    int pairs_per_block,
                                                    const C2<R>* __restrict__ tw_g,
-                                                   const C2<R>* __restrict__ spec,
-                                                   R scale,
-                                                   R* __restrict__ phi)
+                                                   typename Acc<R>::glob_t* __restrict__ mesh,
+                                                   C2<R>* __restrict__ spec)
+    {
+    fft_x_r2c_block<R>(blockIdx.x, g, plan, pairs_per_block, tw_g, mesh, spec);
+    }
+
+//! C2R along x for pairs of half-spectrum lines; phi = scale * real(IFFT).
+template<typename R>
+__device__ __forceinline__ void fft_x_c2r_block(int blk,
+                                                const MeshGeom& g,
+                                                const FFTPlan& plan,
+                                                int pairs_per_block,
+                                                const C2<R>* __restrict__ tw_g,
+                                                const C2<R>* __restrict__ spec,
+                                                R scale,
+                                                R* __restrict__ phi)
     {
     extern __shared__ __align__(16) unsigned char smem[];
     const int nx = g.nx;
@@ -244,7 +291,7 @@ __global__ void __launch_bounds__(256) k_fft_x_c2r(MeshGeom g,
     C2<R>* a = tw + nx;
     C2<R>* b = a + pairs_per_block * ls;
     const int nlines = g.ny * g.nz;
-    const int pair0 = blockIdx.x * pairs_per_block;
+    const int pair0 = blk * pairs_per_block;
     for (int i = threadIdx.x; i < nx; i += blockDim.x)
         tw[i] = tw_g[i];
     for (int idx = threadIdx.x; idx < pairs_per_block * nxh; idx += blockDim.x)
@@ -279,9 +326,27 @@ __global__ void __launch_bounds__(256) k_fft_x_c2r(MeshGeom g,
         }
     }
 
+template<typename R>
+__global__ void __launch_bounds__(256) k_fft_x_c2r(MeshGeom g,
+                                                   FFTPlan plan,
+                                                   int pairs_per_block,
+                                                   const C2<R>* __restrict__ tw_g,
+                                                   const C2<R>* __restrict__ spec,
+                                                   R scale,
+                                                   R* __restrict__ phi)
+    {
+    fft_x_c2r_block<R>(blockIdx.x, g, plan, pairs_per_block, tw_g, spec, scale, phi);
+    }
+
 //! Complex FFT along y (sign -1 forward, +1 inverse) on blocks of ncol consecutive x columns.
 template<typename R>
-__global__ void __launch_bounds__(256) k_fft_y(MeshGeom g, FFTPlan plan, int ncol, const C2<R>* __restrict__ tw_g, C2<R>* __restrict__ spec, int sign)
+__device__ __forceinline__ void fft_y_block(int blk,
+                                            const MeshGeom& g,
+                                            const FFTPlan& plan,
+                                            int ncol,
+                                            const C2<R>* __restrict__ tw_g,
+                                            C2<R>* __restrict__ spec,
+                                            int sign)
     {
     extern __shared__ __align__(16) unsigned char smem[];
     const int ny = g.ny;
@@ -291,8 +356,8 @@ __global__ void __launch_bounds__(256) k_fft_y(MeshGeom g, FFTPlan plan, int nco
     C2<R>* a = tw + ny;
     C2<R>* b = a + ncol * ls;
     const int nbx = (nxh + ncol - 1) / ncol;
-    const int z = blockIdx.x / nbx;
-    const int x0 = (blockIdx.x - z * nbx) * ncol;
+    const int z = blk / nbx;
+    const int x0 = (blk - z * nbx) * ncol;
     const int nc = min(ncol, nxh - x0);
     for (int i = threadIdx.x; i < ny; i += blockDim.x)
         tw[i] = tw_g[i];
@@ -314,18 +379,26 @@ __global__ void __launch_bounds__(256) k_fft_y(MeshGeom g, FFTPlan plan, int nco
         }
     }
 
-//! Forward FFT along z, multiply by the influence function G(k), accumulate the FP64
-//! reciprocal energy and virial, inverse FFT along z.  One block per (y, column block).
 template<typename R>
-__global__ void __launch_bounds__(256) k_fft_z_conv(MeshGeom g,
-                                                    FFTPlan plan,
-                                                    int ncol,
-                                                    const C2<R>* __restrict__ tw_g,
-                                                    C2<R>* __restrict__ spec,
-                                                    const double* __restrict__ G,
-                                                    const double* __restrict__ Gv,
-                                                    int want_ev,
-                                                    double* __restrict__ ev_part)
+__global__ void __launch_bounds__(256) k_fft_y(MeshGeom g, FFTPlan plan, int ncol, const C2<R>* __restrict__ tw_g, C2<R>* __restrict__ spec, int sign)
+    {
+    fft_y_block<R>(blockIdx.x, g, plan, ncol, tw_g, spec, sign);
+    }
+
+//! Forward FFT along z, multiply by the influence function G(k), accumulate the FP64
+//! reciprocal energy and virial, inverse FFT along z.  One block per (y, column block);
+//! its partial sums go to ev_out[0..6].
+template<typename R>
+__device__ __forceinline__ void fft_z_conv_block(int blk,
+                                                 const MeshGeom& g,
+                                                 const FFTPlan& plan,
+                                                 int ncol,
+                                                 const C2<R>* __restrict__ tw_g,
+                                                 C2<R>* __restrict__ spec,
+                                                 const double* __restrict__ G,
+                                                 const double* __restrict__ Gv,
+                                                 int want_ev,
+                                                 double* __restrict__ ev_out)
     {
     extern __shared__ __align__(16) unsigned char smem[];
     const int ny = g.ny;
@@ -336,8 +409,8 @@ __global__ void __launch_bounds__(256) k_fft_z_conv(MeshGeom g,
     C2<R>* a = tw + nz;
     C2<R>* b = a + ncol * ls;
     const int nbx = (nxh + ncol - 1) / ncol;
-    const int y = blockIdx.x / nbx;
-    const int x0 = (blockIdx.x - y * nbx) * ncol;
+    const int y = blk / nbx;
+    const int x0 = (blk - y * nbx) * ncol;
     const int nc = min(ncol, nxh - x0);
     for (int i = threadIdx.x; i < nz; i += blockDim.x)
         tw[i] = tw_g[i];
@@ -428,9 +501,23 @@ __global__ void __launch_bounds__(256) k_fft_z_conv(MeshGeom g,
             double s = 0;
             for (int w = 0; w < int(blockDim.x >> 6); ++w)
                 s += red[w][threadIdx.x];
-            ev_part[size_t(blockIdx.x) * 8 + threadIdx.x] = s;
+            ev_out[threadIdx.x] = s;
             }
         }
+    }
+
+template<typename R>
+__global__ void __launch_bounds__(256) k_fft_z_conv(MeshGeom g,
+                                                    FFTPlan plan,
+                                                    int ncol,
+                                                    const C2<R>* __restrict__ tw_g,
+                                                    C2<R>* __restrict__ spec,
+                                                    const double* __restrict__ G,
+                                                    const double* __restrict__ Gv,
+                                                    int want_ev,
+                                                    double* __restrict__ ev_part)
+    {
+    fft_z_conv_block<R>(blockIdx.x, g, plan, ncol, tw_g, spec, G, Gv, want_ev, ev_part + size_t(blockIdx.x) * 8);
     }
 
 // ---------------------------------------------------------------------------
@@ -492,19 +579,19 @@ __global__ void __launch_bounds__(256) k_build_influence(MeshGeom g,
 
 //! Final FP64 reduction: out[0] = E_rec + E_self + E_bg, out[1..6] = W_rec + background;
 //! qsum[0..1] = (sum q, sum q^2).
-__global__ void __launch_bounds__(256) k_finalize(int n_ev,
-                                                  const double* __restrict__ ev_part,
-                                                  int n_q,
-                                                  const double* __restrict__ qpart,
-                                                  double inv_2V,
-                                                  double self_coeff,
-                                                  double bg_coeff,
-                                                  double V,
-                                                  int want_ev,
-                                                  double* __restrict__ out,
-                                                  double* __restrict__ qsum)
+__device__ __forceinline__ void finalize_one(int n_ev,
+                                             const double* __restrict__ ev_part,
+                                             int n_q,
+                                             const double* __restrict__ qpart,
+                                             double inv_2V,
+                                             double self_coeff,
+                                             double bg_coeff,
+                                             double V,
+                                             int want_ev,
+                                             double* __restrict__ out,
+                                             double* __restrict__ qsum)
     {
-    __shared__ double red[4][9];
+    __shared__ double red[16][9];
     double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (want_ev)
         for (int i = threadIdx.x; i < n_ev; i += blockDim.x)
@@ -548,6 +635,21 @@ __global__ void __launch_bounds__(256) k_finalize(int n_ev,
             out[6] += Ebg;
             }
         }
+    }
+
+__global__ void __launch_bounds__(256) k_finalize(int n_ev,
+                                                  const double* __restrict__ ev_part,
+                                                  int n_q,
+                                                  const double* __restrict__ qpart,
+                                                  double inv_2V,
+                                                  double self_coeff,
+                                                  double bg_coeff,
+                                                  double V,
+                                                  int want_ev,
+                                                  double* __restrict__ out,
+                                                  double* __restrict__ qsum)
+    {
+    finalize_one(n_ev, ev_part, n_q, qpart, inv_2V, self_coeff, bg_coeff, V, want_ev, out, qsum);
     }
 
     } // namespace dev

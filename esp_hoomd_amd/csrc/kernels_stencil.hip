@@ -387,16 +387,16 @@ __device__ __forceinline__ R eval_tap(const R* __restrict__ wc, int D, int k, R 
     }
 
 template<typename R, int P>
-__global__ void __launch_bounds__(256) k_spread_tp(MeshGeom g,
-                                                   int D,
-                                                   const unsigned int* __restrict__ tile_start,
-                                                   const SortedRec<R>* __restrict__ rec,
-                                                   const SortedIdx* __restrict__ sidx,
-                                                   const R* __restrict__ wcoef,
-                                                   typename Acc<R>::glob_t* __restrict__ mesh)
+__device__ __forceinline__ void spread_tile_tp(unsigned int tile,
+                                               const MeshGeom& g,
+                                               int D,
+                                               const unsigned int* __restrict__ tile_start,
+                                               const SortedRec<R>* __restrict__ rec,
+                                               const SortedIdx* __restrict__ sidx,
+                                               const R* __restrict__ wcoef,
+                                               typename Acc<R>::glob_t* __restrict__ mesh)
     {
     using L = typename Acc<R>::lds_t;
-    const unsigned int tile = blockIdx.x;
     const unsigned int s0 = tile_start[tile], s1 = tile_start[tile + 1];
     if (s0 == s1)
         return;
@@ -457,16 +457,28 @@ __global__ void __launch_bounds__(256) k_spread_tp(MeshGeom g,
     }
 
 template<typename R, int P>
-__global__ void __launch_bounds__(256) k_gather_tp(MeshGeom g,
+__global__ void __launch_bounds__(256) k_spread_tp(MeshGeom g,
                                                    int D,
                                                    const unsigned int* __restrict__ tile_start,
                                                    const SortedRec<R>* __restrict__ rec,
                                                    const SortedIdx* __restrict__ sidx,
                                                    const R* __restrict__ wcoef,
-                                                   const R* __restrict__ phi,
-                                                   double4* __restrict__ force)
+                                                   typename Acc<R>::glob_t* __restrict__ mesh)
     {
-    const unsigned int tile = blockIdx.x;
+    spread_tile_tp<R, P>(blockIdx.x, g, D, tile_start, rec, sidx, wcoef, mesh);
+    }
+
+template<typename R, int P>
+__device__ __forceinline__ void gather_tile_tp(unsigned int tile,
+                                               const MeshGeom& g,
+                                               int D,
+                                               const unsigned int* __restrict__ tile_start,
+                                               const SortedRec<R>* __restrict__ rec,
+                                               const SortedIdx* __restrict__ sidx,
+                                               const R* __restrict__ wcoef,
+                                               const R* __restrict__ phi,
+                                               double4* __restrict__ force)
+    {
     const unsigned int s0 = tile_start[tile], s1 = tile_start[tile + 1];
     if (s0 == s1)
         return;
@@ -538,6 +550,19 @@ __global__ void __launch_bounds__(256) k_gather_tp(MeshGeom g,
         }
     }
 
+template<typename R, int P>
+__global__ void __launch_bounds__(256) k_gather_tp(MeshGeom g,
+                                                   int D,
+                                                   const unsigned int* __restrict__ tile_start,
+                                                   const SortedRec<R>* __restrict__ rec,
+                                                   const SortedIdx* __restrict__ sidx,
+                                                   const R* __restrict__ wcoef,
+                                                   const R* __restrict__ phi,
+                                                   double4* __restrict__ force)
+    {
+    gather_tile_tp<R, P>(blockIdx.x, g, D, tile_start, rec, sidx, wcoef, phi, force);
+    }
+
 // ---------------------------------------------------------------------------
 // Column-ownership spreading: lane (cx, cy) owns the z column of the tile's
 // (T+P-1)^3 box in registers.  Particles are broadcast through LDS: x and y
@@ -555,18 +580,18 @@ __device__ __forceinline__ void glob_add_real(unsigned long long* p, float v)
     }
 
 template<typename R, int P, int T>
-__global__ void __launch_bounds__(1024) k_spread_own(MeshGeom g,
-                                                     int D,
-                                                     const unsigned int* __restrict__ tile_start,
-                                                     const SortedRec<R>* __restrict__ rec,
-                                                     const SortedIdx* __restrict__ sidx,
-                                                     const R* __restrict__ wcoef,
-                                                     typename Acc<R>::glob_t* __restrict__ mesh)
+__device__ __forceinline__ void spread_tile_own(unsigned int tile,
+                                                const MeshGeom& g,
+                                                int D,
+                                                const unsigned int* __restrict__ tile_start,
+                                                const SortedRec<R>* __restrict__ rec,
+                                                const SortedIdx* __restrict__ sidx,
+                                                const R* __restrict__ wcoef,
+                                                typename Acc<R>::glob_t* __restrict__ mesh)
     {
     constexpr int B = T + P - 1;
     constexpr int BZ = (B + 3) & ~3;             // z weights padded to a multiple of 4
     constexpr int CH = 64;
-    const unsigned int tile = blockIdx.x;
     const unsigned int s0 = tile_start[tile], s1 = tile_start[tile + 1];
     if (s0 == s1)
         return;
@@ -667,6 +692,18 @@ __global__ void __launch_bounds__(1024) k_spread_own(MeshGeom g,
                 glob_add_real(&mesh[(size_t(gz) * g.ny + gy) * g.nx + gx], col[z]);
                 }
         }
+    }
+
+template<typename R, int P, int T>
+__global__ void __launch_bounds__(1024) k_spread_own(MeshGeom g,
+                                                     int D,
+                                                     const unsigned int* __restrict__ tile_start,
+                                                     const SortedRec<R>* __restrict__ rec,
+                                                     const SortedIdx* __restrict__ sidx,
+                                                     const R* __restrict__ wcoef,
+                                                     typename Acc<R>::glob_t* __restrict__ mesh)
+    {
+    spread_tile_own<R, P, T>(blockIdx.x, g, D, tile_start, rec, sidx, wcoef, mesh);
     }
 
     } // namespace dev
