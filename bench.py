@@ -67,6 +67,49 @@ def kernel_bytes(stage, N, n_charged, n_pairs, M, Mh, R, want_ev=True):
     }[stage]
 
 
+def stage_kernel(stage, precision, P):
+    """Device kernel name that one stage launches (as rocprofv3 prints it, template
+    arguments dropped)."""
+    return {
+        "pair": "k_pair", "bin": "k_bin", "scan": "k_scan", "scatter": "k_scatter",
+        "spread": "k_spread_own" if precision == "fp64" else "k_spread_tp",
+        "fft_x_r2c": "k_fft_x_r2c", "fft_y": "k_fft_y", "fft_z_convolve": "k_fft_z_conv",
+        "ifft_y": "k_fft_y", "ifft_x_c2r": "k_fft_x_c2r", "gather": "k_gather_tp", "finalize": "k_finalize",
+    }[stage]
+
+
+def measured_traffic(workload_key, kernel):
+    """HBM bytes per launch of `kernel` for `workload_key`, from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary kept in profiles/hbm_traffic.txt (lines
+    'TRAFFIC <workload_key> <kernel> <bytes_per_launch>'), or None when not measured."""
+    f = ROOT / "profiles" / "hbm_traffic.txt"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        parts = line.split()
+        if len(parts) == 4 and parts[0] == "TRAFFIC" and parts[1] == workload_key and parts[2] == kernel:
+            return float(parts[3])
+    return None
+
+
+def copy_calibration(mib, steps):
+    """Launch the library's plain record-copy kernel `steps` times over `mib` MiB (64 bytes of
+    HBM traffic per 32-byte record): the known-size kernel that scales the FETCH_SIZE and
+    WRITE_SIZE counters to bytes."""
+    from esp_hoomd_amd import _lib
+    from esp_hoomd_amd.device import DeviceArray, Stream
+    n_rec = int(mib * 2 ** 20) // 32
+    src = DeviceArray((n_rec, 4), np.float64, zero=True)
+    dst = DeviceArray((n_rec, 4), np.float64)
+    stream = Stream()
+    for _ in range(steps):
+        _lib.check(_lib.load().esp_copy_records(dst._ptr, src._ptr, n_rec, stream.handle))
+    stream.synchronize()
+    print(json.dumps({"copy_calibration": {"kernel": "k_copy4", "records": n_rec, "read_bytes": n_rec * 32,
+                                           "write_bytes": n_rec * 32, "launches": steps}}))
+    return 0
+
+
 def cpu_baseline(h, rc, P, kappa, precision_name, n_mesh=48, reps=2):
     """The oracle (FP64 CPU ESP) on a same-density sample with the same h, r_c, P."""
     sys.path.insert(0, str(ROOT / "oracle"))
@@ -111,13 +154,17 @@ def main(argv=None):
     ap.add_argument("--skin", type=float, default=0.4)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--profile-stages", action="store_true", default=True)
+    ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="MIB",
+                    help="only run the record-copy kernel over MIB MiB, --steps times (counter calibration)")
     args = ap.parse_args(argv)
 
     if args.gpus != 1:
         print(f"bench.py: only single-GPU runs are supported (got --gpus {args.gpus}); "
               "multi-GPU sharding is a design note, not part of this build", file=sys.stderr)
         return 2
+
+    if args.calibrate_copy > 0:
+        return copy_calibration(args.calibrate_copy, args.steps)
 
     from esp_hoomd_amd import _lib
     from esp_hoomd_amd.device import Event, Stream, synchronize
@@ -164,6 +211,9 @@ def main(argv=None):
     dom = max(avg, key=avg.get)
     b = kernel_bytes(dom, args.N, n_charged, n_pairs, M, Mh, R)
     achieved = b / (avg[dom] * 1e-3) / 1e9 if avg[dom] > 0 else 0.0
+    workload_key = f"N{args.N}_mesh{args.mesh}_P{args.order}_rc{args.rcut}_{args.precision}"
+    dom_kernel = stage_kernel(dom, args.precision, args.order)
+    traffic = measured_traffic(workload_key, dom_kernel)
     out = {
         "metric": METRIC,
         "value": value,
@@ -186,13 +236,13 @@ def main(argv=None):
             "neighbours_per_particle": n_pairs / args.N,
         },
         "roofline": {
-            "kernel": dom,
+            "kernel": f"{dom} ({dom_kernel})",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
             "algorithmic_bytes": b,
             "avg_launch_ms": avg[dom],
         },

@@ -161,6 +161,7 @@ def load() -> C.CDLL:
     _sig(lib, "esp_memcpy_dtoh", C.c_int, [_VP, _VP, C.c_size_t])
     _sig(lib, "esp_memcpy_dtod", C.c_int, [_VP, _VP, C.c_size_t, _VP])
     _sig(lib, "esp_memset", C.c_int, [_VP, C.c_int, C.c_size_t])
+    _sig(lib, "esp_copy_records", C.c_int, [_VP, _VP, C.c_uint, _VP])
     _sig(lib, "esp_stream_create", C.c_int, [C.POINTER(_VP)])
     _sig(lib, "esp_stream_destroy", C.c_int, [_VP])
     _sig(lib, "esp_stream_synchronize", C.c_int, [_VP])

@@ -9,6 +9,7 @@
 #include <new>
 #include <string>
 
+#include "esp_md.h"
 #include "esp_solver.h"
 #include "pswf_math.h"
 
@@ -534,6 +535,14 @@ int esp_memcpy_dtod(void* dst, const void* src, size_t bytes, esp_stream_t strea
 int esp_memset(void* dst, int value, size_t bytes)
     {
     return hip_status(hipMemset(dst, value, bytes));
+    }
+
+int esp_copy_records(void* dst, const void* src, unsigned int n_records, esp_stream_t stream)
+    {
+    if ((!dst || !src) && n_records)
+        return ESP_ERR_ARG;
+    return hip_status(esp::md::copy4(n_records, static_cast<const double*>(src), static_cast<double*>(dst),
+                                     static_cast<hipStream_t>(stream)));
     }
 
 int esp_stream_create(esp_stream_t* stream)

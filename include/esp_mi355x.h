@@ -218,6 +218,9 @@ int esp_memcpy_htod(void* dst, const void* src, size_t bytes);
 int esp_memcpy_dtoh(void* dst, const void* src, size_t bytes);
 int esp_memcpy_dtod(void* dst, const void* src, size_t bytes, esp_stream_t stream);
 int esp_memset(void* dst, int value, size_t bytes);
+/* copy n 32-byte records with one vector load and one vector store per record (a plain
+   streaming kernel of known traffic: 64 bytes of HBM per record) */
+int esp_copy_records(void* dst, const void* src, unsigned int n_records, esp_stream_t stream);
 int esp_stream_create(esp_stream_t* stream);
 int esp_stream_destroy(esp_stream_t stream);
 int esp_stream_synchronize(esp_stream_t stream);
