@@ -121,6 +121,12 @@ def load() -> C.CDLL:
     _sig(lib, "esp_last_error", C.c_char_p, [_VP])
     _sig(lib, "esp_compute_forces", C.c_int, [_VP, C.c_uint64, C.POINTER(EspSystem), C.POINTER(EspOut), _VP])
     _sig(lib, "esp_pair_forces", C.c_int, [_VP, C.POINTER(EspSystem), C.POINTER(EspOut), _VP])
+    _sig(lib, "esp_batch_create", C.c_int, [C.POINTER(_VP)])
+    _sig(lib, "esp_batch_destroy", C.c_int, [_VP])
+    _sig(lib, "esp_batch_last_error", C.c_char_p, [_VP])
+    _sig(lib, "esp_batch_set_options", C.c_int, [_VP, C.c_int, C.c_uint])
+    _sig(lib, "esp_batch_compute", C.c_int, [_VP, C.c_uint, C.POINTER(_VP), C.POINTER(EspSystem), C.POINTER(EspOut), _VP])
+    _sig(lib, "esp_batch_stats", C.c_int, [_VP, C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)])
     _sig(lib, "esp_get_info", C.c_int, [_VP, C.POINTER(EspInfo)])
     _sig(lib, "esp_get_charge_sums", C.c_int, [_VP, _DP, _DP])
     _sig(lib, "esp_get_table_size", C.c_int, [_VP, C.POINTER(C.c_uint)])
@@ -180,6 +186,12 @@ def check(rc: int, ctx=None) -> None:
         lib = load()
         msg = lib.esp_last_error(ctx).decode() if lib.esp_last_error(ctx) else ""
         raise EspError(rc, msg or "error")
+
+
+def check_batch(rc: int, batch) -> None:
+    if rc != ESP_OK:
+        msg = load().esp_batch_last_error(batch)
+        raise EspError(rc, (msg.decode() if msg else "") or "error")
 
 
 def dptr(a: np.ndarray):

@@ -8,7 +8,9 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
+#include "esp_batch.h"
 #include "esp_md.h"
 #include "esp_solver.h"
 #include "pswf_math.h"
@@ -17,6 +19,15 @@ struct esp_ctx
     {
     esp::EspSolver solver;
     std::string last_error;
+    };
+
+struct esp_batch
+    {
+    esp::EspBatch batch;
+    std::string last_error;
+    std::vector<esp::EspSolver*> solvers;
+    std::vector<esp::SystemView> views;
+    std::vector<esp::OutputView> outs;
     };
 
 namespace
@@ -42,55 +53,46 @@ esp::SolverParams to_solver(const esp_params* p)
     return s;
     }
 
-template<class Fn> int guarded(esp_ctx* ctx, Fn fn)
+template<class Fn> int guarded_err(std::string* err, Fn fn)
     {
+    std::string& msg = err ? *err : g_create_error;
     try
         {
         fn();
-        if (ctx)
-            ctx->last_error.clear();
+        if (err)
+            err->clear();
         return ESP_OK;
         }
     catch (const esp::EspError& e)
         {
-        if (ctx)
-            ctx->last_error = e.what();
-        else
-            g_create_error = e.what();
+        msg = e.what();
         return e.code();
         }
     catch (const std::bad_alloc&)
         {
-        if (ctx)
-            ctx->last_error = "host out of memory";
-        else
-            g_create_error = "host out of memory";
+        msg = "host out of memory";
         return ESP_ERR_OOM;
         }
     catch (const std::invalid_argument& e)
         {
-        if (ctx)
-            ctx->last_error = e.what();
-        else
-            g_create_error = e.what();
+        msg = e.what();
         return ESP_ERR_ARG;
         }
     catch (const std::exception& e)
         {
-        if (ctx)
-            ctx->last_error = e.what();
-        else
-            g_create_error = e.what();
+        msg = e.what();
         return ESP_ERR_STATE;
         }
     catch (...)
         {
-        if (ctx)
-            ctx->last_error = "unknown error";
-        else
-            g_create_error = "unknown error";
+        msg = "unknown error";
         return ESP_ERR_STATE;
         }
+    }
+
+template<class Fn> int guarded(esp_ctx* ctx, Fn fn)
+    {
+    return guarded_err(ctx ? &ctx->last_error : nullptr, fn);
     }
 
 esp::SystemView to_view(const esp_system* s)
@@ -223,6 +225,72 @@ int esp_pair_forces(esp_ctx* ctx, const esp_system* sys, esp_out* out, esp_strea
     esp::OutputView o = to_out(out);
     o.flags = esp::COMPUTE_PAIR;
     return guarded(ctx, [&] { ctx->solver.compute(to_view(sys), o, static_cast<hipStream_t>(stream)); });
+    }
+
+int esp_batch_create(esp_batch** b)
+    {
+    if (!b)
+        return ESP_ERR_ARG;
+    *b = nullptr;
+    return guarded_err(nullptr, [&] { *b = new esp_batch(); });
+    }
+
+int esp_batch_destroy(esp_batch* b)
+    {
+    return guarded_err(nullptr, [&] { delete b; });
+    }
+
+const char* esp_batch_last_error(const esp_batch* b)
+    {
+    return b ? b->last_error.c_str() : g_create_error.c_str();
+    }
+
+int esp_batch_set_options(esp_batch* b, int use_graph, unsigned int max_grid)
+    {
+    if (!b)
+        return ESP_ERR_ARG;
+    return guarded_err(&b->last_error, [&] { b->batch.set_options(use_graph != 0, max_grid); });
+    }
+
+int esp_batch_compute(esp_batch* b, unsigned int n, esp_ctx* const* ctxs, const esp_system* systems, const esp_out* outs,
+                      esp_stream_t stream)
+    {
+    if (!b)
+        return ESP_ERR_ARG;
+    if (n && (!ctxs || !systems || !outs))
+        {
+        b->last_error = "esp_batch_compute: null arrays";
+        return ESP_ERR_ARG;
+        }
+    return guarded_err(&b->last_error,
+                       [&]
+                       {
+                           b->solvers.resize(n);
+                           b->views.resize(n);
+                           b->outs.resize(n);
+                           for (unsigned int i = 0; i < n; ++i)
+                               {
+                               if (!ctxs[i])
+                                   throw esp::EspError(ESP_ERR_ARG, "esp_batch_compute: null context");
+                               b->solvers[i] = &ctxs[i]->solver;
+                               b->views[i] = to_view(&systems[i]);
+                               b->outs[i] = to_out(&outs[i]);
+                               }
+                           b->batch.compute(n, b->solvers.data(), b->views.data(), b->outs.data(), static_cast<hipStream_t>(stream));
+                       });
+    }
+
+int esp_batch_stats(esp_batch* b, unsigned long long* captures, unsigned long long* replays, unsigned long long* launches)
+    {
+    if (!b)
+        return ESP_ERR_ARG;
+    if (captures)
+        *captures = b->batch.captures();
+    if (replays)
+        *replays = b->batch.replays();
+    if (launches)
+        *launches = b->batch.launches();
+    return ESP_OK;
     }
 
 int esp_get_info(esp_ctx* ctx, esp_info* info)

@@ -95,5 +95,77 @@ struct PairArgs
     };
 template<typename R> hipError_t pair_forces(const PairArgs& a, int lanes_per_particle, hipStream_t s);
 
+// ---- batched solves: many independent systems per launch ------------------------------
+//! Everything one system contributes to a batched launch: its geometry and plans, the
+//! caller's arrays and the solver-owned tables and scratch (all device pointers).
+struct BatchSys
+    {
+    MeshGeom g;
+    FFTPlan plan[3];
+    int P, D;
+    unsigned int n;          // mesh-group members (group_size, or N without a group)
+    unsigned int n_tiles;
+    unsigned int flags;      // COMPUTE_* bits
+    int want_ev;
+    int ppb_x, ncol_y, ncol_z;
+    unsigned int n_qblk, n_zblk;
+    double inv_2V, self_coeff, bg_coeff, V;
+    const unsigned int* group;
+    const double* postype;
+    const double* charge;
+    double* force;
+    const void* wcoef;
+    const void* tw[3];
+    const double* G;
+    const double* Gv;
+    void* mesh;
+    void* spec;
+    void* phi;
+    unsigned int* tile_count;
+    unsigned int* tile_start;
+    unsigned int* key;
+    unsigned int* slot;
+    void* rec;
+    SortedIdx* sidx;
+    double* qpart;
+    double* evpart;
+    double* qsum;
+    double* ev_out;
+    PairArgs pair;
+    };
+
+//! One unit of batched work: (system, local block index) -- a particle chunk, a tile or an
+//! FFT block, depending on the stage.
+struct BatchItem
+    {
+    unsigned int sys;
+    unsigned int local;
+    };
+
+//! Work list of one stage; `grid` blocks walk `n` items with a grid stride.
+struct BatchList
+    {
+    const BatchItem* items;
+    unsigned int n;
+    unsigned int grid;
+    };
+
+constexpr int BATCH_PAIR_LANES = 8;
+constexpr unsigned int BATCH_CHUNK = 256;  // particles per bin/scatter item
+
+hipError_t batch_bin(const BatchSys* d, BatchList l, hipStream_t s);
+hipError_t batch_scan(const BatchSys* d, BatchList l, hipStream_t s);
+template<typename R> hipError_t batch_scatter(const BatchSys* d, BatchList l, hipStream_t s);
+template<typename R> hipError_t batch_spread(const BatchSys* d, BatchList l, size_t lds, hipStream_t s);
+template<typename R> hipError_t batch_gather(const BatchSys* d, BatchList l, size_t lds, hipStream_t s);
+template<typename R> hipError_t batch_fft_x_r2c(const BatchSys* d, BatchList l, size_t lds, hipStream_t s);
+template<typename R> hipError_t batch_fft_y(const BatchSys* d, BatchList l, int sign, size_t lds, hipStream_t s);
+template<typename R> hipError_t batch_fft_z_conv(const BatchSys* d, BatchList l, size_t lds, hipStream_t s);
+template<typename R> hipError_t batch_fft_x_c2r(const BatchSys* d, BatchList l, size_t lds, hipStream_t s);
+hipError_t batch_finalize(const BatchSys* d, BatchList l, hipStream_t s);
+template<typename R> hipError_t batch_pair(const BatchSys* d, BatchList l, size_t lds, hipStream_t s);
+size_t tp_lds_bytes(const MeshGeom& g, int P, bool fp64);
+bool batch_order_supported(int P);
+
     } // namespace kernel
     } // namespace esp

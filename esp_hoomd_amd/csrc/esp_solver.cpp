@@ -472,7 +472,7 @@ void EspSolver::ensure_particles(unsigned int n)
     m_cap_particles = cap;
     }
 
-template<typename R> void EspSolver::run_pair(const SystemView& sys, const OutputView& out, hipStream_t s)
+kernel::PairArgs EspSolver::pair_args(const SystemView& sys, const OutputView& out) const
     {
     kernel::PairArgs a {};
     a.N = sys.N;
@@ -496,6 +496,12 @@ template<typename R> void EspSolver::run_pair(const SystemView& sys, const Outpu
     a.force = out.force;
     a.virial = out.virial;
     a.virial_pitch = out.virial_pitch;
+    return a;
+    }
+
+template<typename R> void EspSolver::run_pair(const SystemView& sys, const OutputView& out, hipStream_t s)
+    {
+    const kernel::PairArgs a = pair_args(sys, out);
     int lanes = m_p.pair_lanes ? m_p.pair_lanes : 8;
     check_hip(kernel::pair_forces<R>(a, lanes, s), "pair_forces");
     }
@@ -548,7 +554,7 @@ void EspSolver::run_mesh(const SystemView& sys, const OutputView& out, unsigned 
               "finalize");
     }
 
-void EspSolver::compute(const SystemView& sys, const OutputView& out, hipStream_t s)
+unsigned int EspSolver::prepare(const SystemView& sys, const OutputView& out, hipStream_t s)
     {
     if (!sys.postype || !sys.charge)
         throw EspError(STATUS_ARG, "postype and charge must be device pointers");
@@ -584,6 +590,12 @@ void EspSolver::compute(const SystemView& sys, const OutputView& out, hipStream_
         }
     const unsigned int n = sys.group_idx ? sys.group_size : sys.N;
     ensure_particles(std::max(n, 1u));
+    return n;
+    }
+
+void EspSolver::compute(const SystemView& sys, const OutputView& out, hipStream_t s)
+    {
+    const unsigned int n = prepare(sys, out, s);
     const bool fp64 = m_p.precision == 0;
     const unsigned int flags = out.flags ? out.flags : COMPUTE_ALL;
     mark(0, s);
@@ -634,6 +646,57 @@ void EspSolver::compute(const SystemView& sys, const OutputView& out, hipStream_
         for (int b = 1; b <= 11; ++b)
             mark(b, s);
     mark(N_STAGES, s);
+    }
+
+void EspSolver::describe(const SystemView& sys, const OutputView& out, hipStream_t s, kernel::BatchSys& d)
+    {
+    if (!kernel::batch_order_supported(int(m_p.order)))
+        throw EspError(STATUS_UNSUPPORTED, "batched solves support stencil orders 4, 6, 8, 10 and 12");
+    const unsigned int n = prepare(sys, out, s);
+    const bool fp64 = m_p.precision == 0;
+    const MeshGeom& g = m_geom;
+    d = kernel::BatchSys {};
+    d.g = g;
+    for (int a = 0; a < 3; ++a)
+        d.plan[a] = m_plan[a];
+    d.P = int(m_p.order);
+    d.D = int(m_D);
+    d.n = n;
+    d.n_tiles = m_n_tiles;
+    d.flags = out.flags ? out.flags : COMPUTE_ALL;
+    d.want_ev = (d.flags & COMPUTE_ENERGY_VIRIAL) ? 1 : 0;
+    d.ppb_x = kernel::fft_x_pairs_per_block(g.nx, fp64);
+    d.ncol_y = kernel::fft_cols_per_block(g.ny, fp64);
+    d.ncol_z = kernel::fft_cols_per_block(g.nz, fp64);
+    d.n_qblk = (n + kernel::BATCH_CHUNK - 1) / kernel::BATCH_CHUNK;
+    d.n_zblk = kernel::fft_z_blocks(g, fp64);
+    d.V = g.Lx * g.Ly * g.Lz;
+    d.inv_2V = 0.5 / d.V;
+    d.self_coeff = m_self_coeff;
+    d.bg_coeff = m_bg_coeff;
+    d.group = sys.group_idx;
+    d.postype = sys.postype;
+    d.charge = sys.charge;
+    d.force = out.force;
+    d.wcoef = m_wcoef.get();
+    for (int a = 0; a < 3; ++a)
+        d.tw[a] = m_tw[a].get();
+    d.G = m_G.as<double>();
+    d.Gv = m_Gv.as<double>();
+    d.mesh = m_mesh.get();
+    d.spec = m_spec.get();
+    d.phi = m_phi.get();
+    d.tile_count = m_tile_count.as<unsigned int>();
+    d.tile_start = m_tile_start.as<unsigned int>();
+    d.key = m_key.as<unsigned int>();
+    d.slot = m_slot.as<unsigned int>();
+    d.rec = m_rec.get();
+    d.sidx = m_sidx.as<SortedIdx>();
+    d.qpart = m_qpart.as<double>();
+    d.evpart = m_evpart.as<double>();
+    d.qsum = m_qsum.as<double>();
+    d.ev_out = out.ext_energy_virial ? out.ext_energy_virial : m_ev_internal.as<double>();
+    d.pair = pair_args(sys, out);
     }
 
 void EspSolver::charge_sums(double& q, double& q2)

@@ -14,6 +14,7 @@
 #include <string>
 #include <vector>
 
+#include "esp_kernels.h"
 #include "esp_types.h"
 #include "pswf_math.h"
 
@@ -124,6 +125,10 @@ class EspSolver
 
     //! Full ESP evaluation: pair correction + mesh; outputs overwritten.
     void compute(const SystemView& sys, const OutputView& out, hipStream_t s);
+    //! Prepare this solver for a batched call on `sys` (initialisation, scratch sizing) and
+    //! describe its part of the batch in `d`; the batch launches the kernels.
+    void describe(const SystemView& sys, const OutputView& out, hipStream_t s, kernel::BatchSys& d);
+    bool fp64() const { return m_p.precision == 0; }
 
     //! Host-side tables (built at initialisation).
     const std::vector<double>& split_screen() const { return m_screen; }
@@ -172,6 +177,10 @@ class EspSolver
     unsigned int m_stage_calls[N_STAGES] = {};
 
     void initialize(const SystemView& sys, hipStream_t s);
+    //! Validate the call, (re)initialise when parameters or the box changed, size the
+    //! per-particle scratch; returns the number of mesh-group members.
+    unsigned int prepare(const SystemView& sys, const OutputView& out, hipStream_t s);
+    kernel::PairArgs pair_args(const SystemView& sys, const OutputView& out) const;
     void build_influence(hipStream_t s);
     void ensure_particles(unsigned int n);
     template<typename R>

@@ -652,6 +652,137 @@ __global__ void __launch_bounds__(256) k_finalize(int n_ev,
     finalize_one(n_ev, ev_part, n_q, qpart, inv_2V, self_coeff, bg_coeff, V, want_ev, out, qsum);
     }
 
+// ---------------------------------------------------------------------------
+// Batched stages: one launch covers the work items of every system in a batch.  Each block
+// walks the item list with a grid stride; an item names its system and a system-local
+// block index, so every item runs exactly the code of the single-system kernel.
+// ---------------------------------------------------------------------------
+using kernel::BatchItem;
+using kernel::BatchList;
+using kernel::BatchSys;
+
+__global__ void __launch_bounds__(256) kb_bin(const BatchSys* __restrict__ ds, BatchList l)
+    {
+    __shared__ double sq[4], sq2[4];
+    for (unsigned int it = blockIdx.x; it < l.n; it += gridDim.x)
+        {
+        const BatchItem w = l.items[it];
+        const BatchSys& d = ds[w.sys];
+        const unsigned int i = w.local * kernel::BATCH_CHUNK + threadIdx.x;
+        double q = 0.0;
+        if (i < d.n)
+            q = bin_one(i, d.group, reinterpret_cast<const double4*>(d.postype), d.charge, d.g, d.P, d.tile_count, d.key, d.slot);
+        const double a = wave_sum(q), b = wave_sum(q * q);
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        if (lane == 0)
+            {
+            sq[wave] = a;
+            sq2[wave] = b;
+            }
+        __syncthreads();
+        if (threadIdx.x == 0)
+            {
+            // same summation order as k_bin, so batched and single results agree bit for bit
+            double s = 0, s2 = 0;
+            for (int k = 0; k < 4; ++k)
+                {
+                s += sq[k];
+                s2 += sq2[k];
+                }
+            d.qpart[2 * w.local] = s;
+            d.qpart[2 * w.local + 1] = s2;
+            }
+        __syncthreads();
+        }
+    }
+
+__global__ void __launch_bounds__(1024) kb_scan(const BatchSys* __restrict__ ds, BatchList l)
+    {
+    for (unsigned int it = blockIdx.x; it < l.n; it += gridDim.x)
+        {
+        const BatchSys& d = ds[l.items[it].sys];
+        scan_one(d.n_tiles, d.tile_count, d.tile_start);
+        __syncthreads();
+        }
+    }
+
+template<typename R>
+__global__ void __launch_bounds__(256) kb_scatter(const BatchSys* __restrict__ ds, BatchList l)
+    {
+    for (unsigned int it = blockIdx.x; it < l.n; it += gridDim.x)
+        {
+        const BatchItem w = l.items[it];
+        const BatchSys& d = ds[w.sys];
+        const unsigned int i = w.local * kernel::BATCH_CHUNK + threadIdx.x;
+        if (i < d.n)
+            scatter_one<R>(i, d.group, reinterpret_cast<const double4*>(d.postype), d.charge, d.g, d.P, d.key, d.slot, d.tile_start,
+                           reinterpret_cast<SortedRec<R>*>(d.rec), d.sidx);
+        }
+    }
+
+template<typename R>
+__global__ void __launch_bounds__(256) kb_fft_x_r2c(const BatchSys* __restrict__ ds, BatchList l)
+    {
+    for (unsigned int it = blockIdx.x; it < l.n; it += gridDim.x)
+        {
+        const BatchItem w = l.items[it];
+        const BatchSys& d = ds[w.sys];
+        fft_x_r2c_block<R>(int(w.local), d.g, d.plan[0], d.ppb_x, reinterpret_cast<const C2<R>*>(d.tw[0]),
+                           reinterpret_cast<typename Acc<R>::glob_t*>(d.mesh), reinterpret_cast<C2<R>*>(d.spec));
+        __syncthreads();
+        }
+    }
+
+template<typename R>
+__global__ void __launch_bounds__(256) kb_fft_y(const BatchSys* __restrict__ ds, BatchList l, int sign)
+    {
+    for (unsigned int it = blockIdx.x; it < l.n; it += gridDim.x)
+        {
+        const BatchItem w = l.items[it];
+        const BatchSys& d = ds[w.sys];
+        fft_y_block<R>(int(w.local), d.g, d.plan[1], d.ncol_y, reinterpret_cast<const C2<R>*>(d.tw[1]), reinterpret_cast<C2<R>*>(d.spec),
+                       sign);
+        __syncthreads();
+        }
+    }
+
+template<typename R>
+__global__ void __launch_bounds__(256) kb_fft_z_conv(const BatchSys* __restrict__ ds, BatchList l)
+    {
+    for (unsigned int it = blockIdx.x; it < l.n; it += gridDim.x)
+        {
+        const BatchItem w = l.items[it];
+        const BatchSys& d = ds[w.sys];
+        fft_z_conv_block<R>(int(w.local), d.g, d.plan[2], d.ncol_z, reinterpret_cast<const C2<R>*>(d.tw[2]),
+                            reinterpret_cast<C2<R>*>(d.spec), d.G, d.Gv, d.want_ev, d.evpart + size_t(w.local) * 8);
+        __syncthreads();
+        }
+    }
+
+template<typename R>
+__global__ void __launch_bounds__(256) kb_fft_x_c2r(const BatchSys* __restrict__ ds, BatchList l)
+    {
+    for (unsigned int it = blockIdx.x; it < l.n; it += gridDim.x)
+        {
+        const BatchItem w = l.items[it];
+        const BatchSys& d = ds[w.sys];
+        fft_x_c2r_block<R>(int(w.local), d.g, d.plan[0], d.ppb_x, reinterpret_cast<const C2<R>*>(d.tw[0]),
+                           reinterpret_cast<const C2<R>*>(d.spec), R(1.0 / d.V), reinterpret_cast<R*>(d.phi));
+        __syncthreads();
+        }
+    }
+
+__global__ void __launch_bounds__(256) kb_finalize(const BatchSys* __restrict__ ds, BatchList l)
+    {
+    for (unsigned int it = blockIdx.x; it < l.n; it += gridDim.x)
+        {
+        const BatchSys& d = ds[l.items[it].sys];
+        finalize_one(int(d.n_zblk), d.evpart, int(d.n_qblk), d.qpart, d.inv_2V, d.self_coeff, d.bg_coeff, d.V, d.want_ev, d.ev_out,
+                     d.qsum);
+        __syncthreads();
+        }
+    }
+
     } // namespace dev
 
 // ===========================================================================
@@ -813,6 +944,81 @@ hipError_t finalize(int n_ev, const double* ev_part, int n_q, const double* qpar
                        qsum);
     return hipGetLastError();
     }
+
+// ---- batched launchers ---------------------------------------------------------------
+hipError_t batch_bin(const BatchSys* d, BatchList l, hipStream_t s)
+    {
+    if (l.n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(kb_bin, dim3(l.grid), dim3(256), 0, s, d, l);
+    return hipGetLastError();
+    }
+
+hipError_t batch_scan(const BatchSys* d, BatchList l, hipStream_t s)
+    {
+    if (l.n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(kb_scan, dim3(l.grid), dim3(1024), 0, s, d, l);
+    return hipGetLastError();
+    }
+
+template<typename R> hipError_t batch_scatter(const BatchSys* d, BatchList l, hipStream_t s)
+    {
+    if (l.n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(kb_scatter<R>, dim3(l.grid), dim3(256), 0, s, d, l);
+    return hipGetLastError();
+    }
+
+template<typename R> hipError_t batch_fft_x_r2c(const BatchSys* d, BatchList l, size_t lds, hipStream_t s)
+    {
+    if (l.n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(kb_fft_x_r2c<R>, dim3(l.grid), dim3(256), lds, s, d, l);
+    return hipGetLastError();
+    }
+
+template<typename R> hipError_t batch_fft_y(const BatchSys* d, BatchList l, int sign, size_t lds, hipStream_t s)
+    {
+    if (l.n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(kb_fft_y<R>, dim3(l.grid), dim3(256), lds, s, d, l, sign);
+    return hipGetLastError();
+    }
+
+template<typename R> hipError_t batch_fft_z_conv(const BatchSys* d, BatchList l, size_t lds, hipStream_t s)
+    {
+    if (l.n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(kb_fft_z_conv<R>, dim3(l.grid), dim3(256), lds, s, d, l);
+    return hipGetLastError();
+    }
+
+template<typename R> hipError_t batch_fft_x_c2r(const BatchSys* d, BatchList l, size_t lds, hipStream_t s)
+    {
+    if (l.n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(kb_fft_x_c2r<R>, dim3(l.grid), dim3(256), lds, s, d, l);
+    return hipGetLastError();
+    }
+
+hipError_t batch_finalize(const BatchSys* d, BatchList l, hipStream_t s)
+    {
+    if (l.n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(kb_finalize, dim3(l.grid), dim3(256), 0, s, d, l);
+    return hipGetLastError();
+    }
+
+#define ESP_INST_BATCH(R)                                                                     \
+    template hipError_t batch_scatter<R>(const BatchSys*, BatchList, hipStream_t);            \
+    template hipError_t batch_fft_x_r2c<R>(const BatchSys*, BatchList, size_t, hipStream_t);  \
+    template hipError_t batch_fft_y<R>(const BatchSys*, BatchList, int, size_t, hipStream_t); \
+    template hipError_t batch_fft_z_conv<R>(const BatchSys*, BatchList, size_t, hipStream_t); \
+    template hipError_t batch_fft_x_c2r<R>(const BatchSys*, BatchList, size_t, hipStream_t);
+ESP_INST_BATCH(double)
+ESP_INST_BATCH(float)
+#undef ESP_INST_BATCH
 
     } // namespace kernel
     } // namespace esp

@@ -79,6 +79,129 @@ __device__ __forceinline__ void pair_terms(const R* __restrict__ tab,
         }
     }
 
+//! Pair correction of particle i by the G lanes of its group (`sub` = lane within the group);
+//! every lane of the group must call it.  Writes force (fx, fy, fz, energy) and, when
+//! requested, the six per-particle virial components.
+template<typename R, int G, bool COMPACT>
+__device__ __forceinline__ void pair_particle(unsigned int i, int sub, const kernel::PairArgs& a, const R* __restrict__ tab)
+    {
+    const double invLx = 1.0 / a.Lx, invLy = 1.0 / a.Ly, invLz = 1.0 / a.Lz;
+    const R rc = R(a.rc);
+    const R inv_rc = R(1.0 / a.rc);
+    const R rc2 = rc * rc;
+    R f[3] = {R(0), R(0), R(0)};
+    R e = R(0);
+    R v[6] = {R(0), R(0), R(0), R(0), R(0), R(0)};
+    const double4 pi = reinterpret_cast<const double4*>(a.postype)[i];
+    const double qi = a.charge[i];
+    if (a.nlist != nullptr && qi != 0.0 && a.rc >= 1.0e-12)
+        {
+        const unsigned int nn = a.n_neigh[i];
+        const uint64_t head = a.head_list[i];
+        for (unsigned int k = sub; k < nn; k += G)
+            {
+            const unsigned int j = a.nlist[head + k];
+            const double4 pj = reinterpret_cast<const double4*>(a.postype)[j];
+            const R qiqj = R(qi * a.charge[j]);
+            const R dx = R(min_image(pi.x - pj.x, a.Lx, invLx));
+            const R dy = R(min_image(pi.y - pj.y, a.Ly, invLy));
+            const R dz = R(min_image(pi.z - pj.z, a.Lz, invLz));
+            const R rsq = dx * dx + dy * dy + dz * dz;
+            if (rsq >= rc2 || qiqj == R(0))
+                continue;
+            const R r = sqrt(rsq);
+            const R rinv = R(1) / r;
+            R Lr, ndr;
+            pair_terms<R, COMPACT>(tab, a, r, rinv, rc, inv_rc, Lr, ndr);
+            const R fdivr = qiqj * ndr;
+            f[0] += fdivr * dx;
+            f[1] += fdivr * dy;
+            f[2] += fdivr * dz;
+            e += R(0.5) * qiqj * Lr;
+            const R hf = R(0.5) * fdivr;
+            v[0] += hf * dx * dx;
+            v[1] += hf * dx * dy;
+            v[2] += hf * dx * dz;
+            v[3] += hf * dy * dy;
+            v[4] += hf * dy * dz;
+            v[5] += hf * dz * dz;
+            }
+        }
+    if (a.n_ex != nullptr && qi != 0.0)
+        {
+        // remove the mesh part M(r) = 1/(4 pi r) - L(r) of each excluded pair at any distance
+        const unsigned int ne = a.n_ex[i];
+        for (unsigned int k = sub; k < ne; k += G)
+            {
+            const unsigned int j = a.ex_list[size_t(i) * a.ex_pitch + k];
+            if (j >= a.N)
+                continue;
+            const double4 pj = reinterpret_cast<const double4*>(a.postype)[j];
+            const R qiqj = R(qi * a.charge[j]);
+            const R dx = R(min_image(pi.x - pj.x, a.Lx, invLx));
+            const R dy = R(min_image(pi.y - pj.y, a.Ly, invLy));
+            const R dz = R(min_image(pi.z - pj.z, a.Lz, invLz));
+            const R rsq = dx * dx + dy * dy + dz * dz;
+            if (rsq < R(1.0e-20))
+                continue;
+            const R r = sqrt(rsq);
+            const R rinv = R(1) / r;
+            R Lr = R(0), ndr = R(0);
+            if (r <= rc)
+                pair_terms<R, COMPACT>(tab, a, r, rinv, rc, inv_rc, Lr, ndr);
+            const R inv4pir = R(0.079577471545947673) * rinv;
+            const R dE = qiqj * (Lr - inv4pir);
+            const R fdivr = qiqj * (ndr - inv4pir * rinv * rinv);
+            f[0] += fdivr * dx;
+            f[1] += fdivr * dy;
+            f[2] += fdivr * dz;
+            e += R(0.5) * dE;
+            const R hf = R(0.5) * fdivr;
+            v[0] += hf * dx * dx;
+            v[1] += hf * dx * dy;
+            v[2] += hf * dx * dz;
+            v[3] += hf * dy * dy;
+            v[4] += hf * dy * dz;
+            v[5] += hf * dz * dz;
+            }
+        }
+    // reduce over the G lanes of this particle (every lane of the group takes part)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        f[c] = group_sum<G>(f[c]);
+    e = group_sum<G>(e);
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+        v[c] = group_sum<G>(v[c]);
+    if (sub == 0)
+        {
+        double4 o;
+        o.x = double(f[0]);
+        o.y = double(f[1]);
+        o.z = double(f[2]);
+        o.w = double(e);
+        reinterpret_cast<double4*>(a.force)[i] = o;
+        if (a.virial != nullptr)
+            {
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+                a.virial[size_t(c) * a.virial_pitch + i] = double(v[c]);
+            }
+        }
+    }
+
+//! Copy the compact split table of one system into LDS (callers synchronise around it).
+template<typename R>
+__device__ __forceinline__ const R* stage_compact_table(const kernel::PairArgs& a, unsigned char* smem)
+    {
+    R* t = reinterpret_cast<R*>(smem);
+    const R* src = reinterpret_cast<const R*>(a.compact_tab);
+    const int n = int(a.n_compact * (a.compact_deg + 1));
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+        t[i] = src[i];
+    return t;
+    }
+
 template<typename R, int G, bool COMPACT>
 __global__ void __launch_bounds__(256) k_pair(kernel::PairArgs a)
     {
@@ -86,121 +209,54 @@ __global__ void __launch_bounds__(256) k_pair(kernel::PairArgs a)
     const R* tab = reinterpret_cast<const R*>(a.split_tab);
     if constexpr (COMPACT)
         {
-        R* t = reinterpret_cast<R*>(smem);
-        const R* src = reinterpret_cast<const R*>(a.compact_tab);
-        const int n = int(a.n_compact * (a.compact_deg + 1));
-        for (int i = threadIdx.x; i < n; i += blockDim.x)
-            t[i] = src[i];
+        tab = stage_compact_table<R>(a, smem);
         __syncthreads();
-        tab = t;
         }
-    const double invLx = 1.0 / a.Lx, invLy = 1.0 / a.Ly, invLz = 1.0 / a.Lz;
-    const R rc = R(a.rc);
-    const R inv_rc = R(1.0 / a.rc);
-    const R rc2 = rc * rc;
     const int sub = int(threadIdx.x % G);
     const unsigned int groups_per_grid = gridDim.x * (blockDim.x / G);
     for (unsigned int i = (blockIdx.x * blockDim.x + threadIdx.x) / G; i < a.N + 0u; i += groups_per_grid)
+        pair_particle<R, G, COMPACT>(i, sub, a, tab);
+    }
+
+//! Batched pair correction: items are (system, chunk of 256 / G particles).  The compact
+//! table is restaged whenever the block moves to another system.  Systems whose flags
+//! exclude the pair part get zero force and virial (the outputs are always overwritten).
+template<typename R, int G>
+__global__ void __launch_bounds__(256) kb_pair(const kernel::BatchSys* __restrict__ ds, kernel::BatchList l)
+    {
+    extern __shared__ __align__(16) unsigned char smem[];
+    unsigned int staged = 0xffffffffu;
+    const int sub = int(threadIdx.x % G);
+    for (unsigned int it = blockIdx.x; it < l.n; it += gridDim.x)
         {
-        R f[3] = {R(0), R(0), R(0)};
-        R e = R(0);
-        R v[6] = {R(0), R(0), R(0), R(0), R(0), R(0)};
-        const double4 pi = reinterpret_cast<const double4*>(a.postype)[i];
-        const double qi = a.charge[i];
-        if (a.nlist != nullptr && qi != 0.0 && a.rc >= 1.0e-12)
+        const kernel::BatchItem w = l.items[it];
+        const kernel::PairArgs& a = ds[w.sys].pair;
+        const bool compact = a.compact_tab != nullptr;
+        if (compact && staged != w.sys)
             {
-            const unsigned int nn = a.n_neigh[i];
-            const uint64_t head = a.head_list[i];
-            for (unsigned int k = sub; k < nn; k += G)
-                {
-                const unsigned int j = a.nlist[head + k];
-                const double4 pj = reinterpret_cast<const double4*>(a.postype)[j];
-                const R qiqj = R(qi * a.charge[j]);
-                const R dx = R(min_image(pi.x - pj.x, a.Lx, invLx));
-                const R dy = R(min_image(pi.y - pj.y, a.Ly, invLy));
-                const R dz = R(min_image(pi.z - pj.z, a.Lz, invLz));
-                const R rsq = dx * dx + dy * dy + dz * dz;
-                if (rsq >= rc2 || qiqj == R(0))
-                    continue;
-                const R r = sqrt(rsq);
-                const R rinv = R(1) / r;
-                R Lr, ndr;
-                pair_terms<R, COMPACT>(tab, a, r, rinv, rc, inv_rc, Lr, ndr);
-                const R fdivr = qiqj * ndr;
-                f[0] += fdivr * dx;
-                f[1] += fdivr * dy;
-                f[2] += fdivr * dz;
-                e += R(0.5) * qiqj * Lr;
-                const R hf = R(0.5) * fdivr;
-                v[0] += hf * dx * dx;
-                v[1] += hf * dx * dy;
-                v[2] += hf * dx * dz;
-                v[3] += hf * dy * dy;
-                v[4] += hf * dy * dz;
-                v[5] += hf * dz * dz;
-                }
+            __syncthreads();
+            stage_compact_table<R>(a, smem);
+            __syncthreads();
+            staged = w.sys;
             }
-        if (a.n_ex != nullptr && qi != 0.0)
+        const unsigned int i = w.local * (blockDim.x / G) + threadIdx.x / G;
+        if (i >= a.N)
+            continue;
+        if ((ds[w.sys].flags & 1u) == 0u)
             {
-            // remove the mesh part M(r) = 1/(4 pi r) - L(r) of each excluded pair at any distance
-            const unsigned int ne = a.n_ex[i];
-            for (unsigned int k = sub; k < ne; k += G)
+            if (sub == 0)
                 {
-                const unsigned int j = a.ex_list[size_t(i) * a.ex_pitch + k];
-                if (j >= a.N)
-                    continue;
-                const double4 pj = reinterpret_cast<const double4*>(a.postype)[j];
-                const R qiqj = R(qi * a.charge[j]);
-                const R dx = R(min_image(pi.x - pj.x, a.Lx, invLx));
-                const R dy = R(min_image(pi.y - pj.y, a.Ly, invLy));
-                const R dz = R(min_image(pi.z - pj.z, a.Lz, invLz));
-                const R rsq = dx * dx + dy * dy + dz * dz;
-                if (rsq < R(1.0e-20))
-                    continue;
-                const R r = sqrt(rsq);
-                const R rinv = R(1) / r;
-                R Lr = R(0), ndr = R(0);
-                if (r <= rc)
-                    pair_terms<R, COMPACT>(tab, a, r, rinv, rc, inv_rc, Lr, ndr);
-                const R inv4pir = R(0.079577471545947673) * rinv;
-                const R dE = qiqj * (Lr - inv4pir);
-                const R fdivr = qiqj * (ndr - inv4pir * rinv * rinv);
-                f[0] += fdivr * dx;
-                f[1] += fdivr * dy;
-                f[2] += fdivr * dz;
-                e += R(0.5) * dE;
-                const R hf = R(0.5) * fdivr;
-                v[0] += hf * dx * dx;
-                v[1] += hf * dx * dy;
-                v[2] += hf * dx * dz;
-                v[3] += hf * dy * dy;
-                v[4] += hf * dy * dz;
-                v[5] += hf * dz * dz;
+                reinterpret_cast<double4*>(a.force)[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+                if (a.virial != nullptr)
+                    for (int c = 0; c < 6; ++c)
+                        a.virial[size_t(c) * a.virial_pitch + i] = 0.0;
                 }
+            continue;
             }
-        // reduce over the G lanes of this particle (every lane of the group takes part)
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-            f[c] = group_sum<G>(f[c]);
-        e = group_sum<G>(e);
-#pragma unroll
-        for (int c = 0; c < 6; ++c)
-            v[c] = group_sum<G>(v[c]);
-        if (sub == 0)
-            {
-            double4 o;
-            o.x = double(f[0]);
-            o.y = double(f[1]);
-            o.z = double(f[2]);
-            o.w = double(e);
-            reinterpret_cast<double4*>(a.force)[i] = o;
-            if (a.virial != nullptr)
-                {
-#pragma unroll
-                for (int c = 0; c < 6; ++c)
-                    a.virial[size_t(c) * a.virial_pitch + i] = double(v[c]);
-                }
-            }
+        if (compact)
+            pair_particle<R, G, true>(i, sub, a, reinterpret_cast<const R*>(smem));
+        else
+            pair_particle<R, G, false>(i, sub, a, reinterpret_cast<const R*>(a.split_tab));
         }
     }
 
@@ -240,5 +296,15 @@ template<typename R> hipError_t pair_forces(const PairArgs& a, int lanes_per_par
     }
 template hipError_t pair_forces<double>(const PairArgs&, int, hipStream_t);
 template hipError_t pair_forces<float>(const PairArgs&, int, hipStream_t);
+
+template<typename R> hipError_t batch_pair(const BatchSys* d, BatchList l, size_t lds, hipStream_t s)
+    {
+    if (l.n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL((dev::kb_pair<R, BATCH_PAIR_LANES>), dim3(l.grid), dim3(256), lds, s, d, l);
+    return hipGetLastError();
+    }
+template hipError_t batch_pair<double>(const BatchSys*, BatchList, size_t, hipStream_t);
+template hipError_t batch_pair<float>(const BatchSys*, BatchList, size_t, hipStream_t);
     } // namespace kernel
     } // namespace esp

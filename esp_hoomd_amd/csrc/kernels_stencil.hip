@@ -706,6 +706,61 @@ __global__ void __launch_bounds__(1024) k_spread_own(MeshGeom g,
     spread_tile_own<R, P, T>(blockIdx.x, g, D, tile_start, rec, sidx, wcoef, mesh);
     }
 
+// ---------------------------------------------------------------------------
+// Batched spreading and gathering: items are (system, tile); the stencil order is a
+// per-system runtime value dispatched to the compiled orders.
+// ---------------------------------------------------------------------------
+#define ESP_BATCH_P_CASES(X) X(4) X(6) X(8) X(10) X(12)
+
+template<typename R>
+__global__ void __launch_bounds__(256) kb_spread(const kernel::BatchSys* __restrict__ ds, kernel::BatchList l)
+    {
+    for (unsigned int it = blockIdx.x; it < l.n; it += gridDim.x)
+        {
+        const kernel::BatchItem w = l.items[it];
+        const kernel::BatchSys& d = ds[w.sys];
+        const auto* rec = reinterpret_cast<const SortedRec<R>*>(d.rec);
+        const auto* wc = reinterpret_cast<const R*>(d.wcoef);
+        auto* mesh = reinterpret_cast<typename Acc<R>::glob_t*>(d.mesh);
+        switch (d.P)
+            {
+#define ESP_CASE(PP)                                                                 \
+    case PP:                                                                         \
+        spread_tile_tp<R, PP>(w.local, d.g, d.D, d.tile_start, rec, d.sidx, wc, mesh); \
+        break;
+            ESP_BATCH_P_CASES(ESP_CASE)
+#undef ESP_CASE
+            default: break;
+            }
+        __syncthreads();
+        }
+    }
+
+template<typename R>
+__global__ void __launch_bounds__(256) kb_gather(const kernel::BatchSys* __restrict__ ds, kernel::BatchList l)
+    {
+    for (unsigned int it = blockIdx.x; it < l.n; it += gridDim.x)
+        {
+        const kernel::BatchItem w = l.items[it];
+        const kernel::BatchSys& d = ds[w.sys];
+        const auto* rec = reinterpret_cast<const SortedRec<R>*>(d.rec);
+        const auto* wc = reinterpret_cast<const R*>(d.wcoef);
+        const auto* phi = reinterpret_cast<const R*>(d.phi);
+        auto* force = reinterpret_cast<double4*>(d.force);
+        switch (d.P)
+            {
+#define ESP_CASE(PP)                                                                          \
+    case PP:                                                                                  \
+        gather_tile_tp<R, PP>(w.local, d.g, d.D, d.tile_start, rec, d.sidx, wc, phi, force); \
+        break;
+            ESP_BATCH_P_CASES(ESP_CASE)
+#undef ESP_CASE
+            default: break;
+            }
+        __syncthreads();
+        }
+    }
+
     } // namespace dev
 
 namespace kernel
@@ -846,6 +901,39 @@ template hipError_t gather<double>(const MeshGeom&, int, int, unsigned int, cons
                                    const void*, const void*, double*, hipStream_t);
 template hipError_t gather<float>(const MeshGeom&, int, int, unsigned int, const unsigned int*, const void*, const SortedIdx*,
                                   const void*, const void*, double*, hipStream_t);
+
+bool batch_order_supported(int P)
+    {
+    switch (P)
+        {
+#define ESP_CASE(PP) case PP:
+        ESP_BATCH_P_CASES(ESP_CASE)
+#undef ESP_CASE
+        return true;
+        default: return false;
+        }
+    }
+
+template<typename R> hipError_t batch_spread(const BatchSys* d, BatchList l, size_t lds, hipStream_t s)
+    {
+    if (l.n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(kb_spread<R>, dim3(l.grid), dim3(256), lds, s, d, l);
+    return hipGetLastError();
+    }
+
+template<typename R> hipError_t batch_gather(const BatchSys* d, BatchList l, size_t lds, hipStream_t s)
+    {
+    if (l.n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(kb_gather<R>, dim3(l.grid), dim3(256), lds, s, d, l);
+    return hipGetLastError();
+    }
+
+template hipError_t batch_spread<double>(const BatchSys*, BatchList, size_t, hipStream_t);
+template hipError_t batch_spread<float>(const BatchSys*, BatchList, size_t, hipStream_t);
+template hipError_t batch_gather<double>(const BatchSys*, BatchList, size_t, hipStream_t);
+template hipError_t batch_gather<float>(const BatchSys*, BatchList, size_t, hipStream_t);
 
     } // namespace kernel
     } // namespace esp

@@ -142,6 +142,26 @@ int esp_compute_forces(esp_ctx* ctx, uint64_t timestep, const esp_system* sys, e
 /* the PotentialPair<EvaluatorPairPSWF> path alone (module.cc:73,79): pair + exclusions */
 int esp_pair_forces(esp_ctx* ctx, const esp_system* sys, esp_out* out, esp_stream_t stream);
 
+/* ---- batched solves --------------------------------------------------------------------
+ * Many independent systems, one launch per stage (no counterpart in the reference, which
+ * evaluates one system per ESPForceCompute).  System i is evaluated with the parameters,
+ * tables and scratch of ctxs[i] exactly as esp_compute_forces(ctxs[i], ...) would, and its
+ * outputs are overwritten.  All contexts of a call share one precision, a context appears at
+ * most once per call, and stencil orders are 4, 6, 8, 10 or 12.  The launch sequence is
+ * captured into a hipGraph and replayed while the batch keeps its shape (same contexts,
+ * particle counts, meshes and flags); new device pointers are picked up without a new
+ * capture.  A batch is single threaded; its contexts must not be used concurrently. */
+typedef struct esp_batch esp_batch;
+int esp_batch_create(esp_batch** batch);
+int esp_batch_destroy(esp_batch* batch);
+const char* esp_batch_last_error(const esp_batch* batch);
+/* use_graph: capture and replay (default 1); max_grid: blocks per batched launch that walk
+ * the work items with a grid stride (default 2048; 0 = one block per item) */
+int esp_batch_set_options(esp_batch* batch, int use_graph, unsigned int max_grid);
+int esp_batch_compute(esp_batch* batch, unsigned int n, esp_ctx* const* ctxs, const esp_system* systems, const esp_out* outs,
+                      esp_stream_t stream);
+int esp_batch_stats(esp_batch* batch, unsigned long long* captures, unsigned long long* replays, unsigned long long* launches);
+
 /* ---- introspection ------------------------------------------------------------------- */
 int esp_get_info(esp_ctx* ctx, esp_info* info);
 /* getQSum / getQ2Sum (ESPForceCompute.h:113-114); synchronises with the device */
