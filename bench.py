@@ -124,7 +124,7 @@ def cpu_baseline(h, rc, P, kappa, precision_name, n_mesh=48, reps=2):
     D, cw = S.default_window(P, sigma)
     w = S.build_window(P, D, cw)
     scr, dscr, rlo, dri = S.build_split_table(0, kappa, rc, 4096)
-    o = O.OracleESP((L, L, L), (n_mesh,) * 3, P, w, scr, dscr, rlo, dri, rc)
+    o = O.OracleESP((L, L, L), (n_mesh,) * 3, P, w, scr, dscr, rlo, dri, rc, k_cut=1.2 * kappa)
     o.compute(pos[:64], q[:64])  # warm the library
     t0 = time.perf_counter()
     for _ in range(reps):

@@ -42,7 +42,7 @@ class EspParams(C.Structure):
         ("n_table", C.c_uint),
         ("precision", C.c_int), ("split", C.c_int),
         ("window_c", C.c_double), ("window_degree", C.c_uint),
-        ("pair_lanes", C.c_int),
+        ("pair_lanes", C.c_int), ("k_cut", C.c_double),
     ]
 
 
@@ -79,7 +79,7 @@ class EspInfo(C.Structure):
         ("n_table", C.c_uint), ("precision", C.c_int), ("split", C.c_int),
         ("window_degree", C.c_uint), ("window_c", C.c_double),
         ("self_coeff", C.c_double), ("background_coeff", C.c_double),
-        ("tile", C.c_uint), ("device_bytes", C.c_size_t),
+        ("tile", C.c_uint), ("device_bytes", C.c_size_t), ("k_cut", C.c_double),
     ]
 
 

@@ -50,6 +50,7 @@ esp::SolverParams to_solver(const esp_params* p)
     s.window_c = p->window_c;
     s.window_degree = p->window_degree;
     s.pair_lanes = p->pair_lanes;
+    s.k_cut = p->k_cut;
     return s;
     }
 
@@ -319,6 +320,7 @@ int esp_get_info(esp_ctx* ctx, esp_info* info)
                        info->background_coeff = ctx->solver.background_coeff();
                        info->tile = unsigned(ctx->solver.geometry().T);
                        info->device_bytes = ctx->solver.memory_bytes();
+                       info->k_cut = ctx->solver.effective_k_cut();
                    });
     }
 

@@ -223,6 +223,8 @@ void EspSolver::set_params(const SolverParams& p)
     if (p.pair_lanes != 0 && p.pair_lanes != 1 && p.pair_lanes != 2 && p.pair_lanes != 4 && p.pair_lanes != 8 && p.pair_lanes != 16
         && p.pair_lanes != 32)
         throw EspError(STATUS_ARG, "pair_lanes must be 0 or a power of two <= 32");
+    if (!std::isfinite(p.k_cut))
+        throw EspError(STATUS_ARG, "k_cut must be finite");
     // validate FFT factorisation eagerly
     make_fft_plan(int(p.nx));
     make_fft_plan(int(p.ny));
@@ -412,6 +414,15 @@ void EspSolver::initialize(const SystemView& sys, hipStream_t s)
     m_need_init = false;
     }
 
+double EspSolver::effective_k_cut() const
+    {
+    if (m_p.k_cut > 0.0)
+        return m_p.k_cut;
+    if (m_p.k_cut < 0.0 || m_p.split == 1)
+        return 0.0;
+    return 1.2 * m_p.kappa;
+    }
+
 void EspSolver::build_influence(hipStream_t s)
     {
     const MeshGeom& g = m_geom;
@@ -453,7 +464,7 @@ void EspSolver::build_influence(hipStream_t s)
         m_qxd.resize(8);
         m_qwd.resize(8);
         }
-    check_hip(kernel::build_influence(g, m_p.split, m_p.kappa, m_p.rcut, nq, m_qxd.as<double>(), m_qwd.as<double>(), m_w2[0].as<double>(),
+    check_hip(kernel::build_influence(g, m_p.split, m_p.kappa, m_p.rcut, effective_k_cut(), nq, m_qxd.as<double>(), m_qwd.as<double>(), m_w2[0].as<double>(),
                                       m_w2[1].as<double>(), m_w2[2].as<double>(), m_G.as<double>(), m_Gv.as<double>(), s),
               "build_influence");
     }

@@ -77,6 +77,7 @@ struct SolverParams
     double window_c = 0.0;   // 0 = automatic
     unsigned int window_degree = 0;  // 0 = automatic
     int pair_lanes = 0;      // lanes per particle in the pair kernel (0 = automatic)
+    double k_cut = 0.0;      // reciprocal-space cutoff (0 = automatic, < 0 = none)
     };
 
 //! Device-side description of one system for one call (caller-owned device pointers).
@@ -141,6 +142,9 @@ class EspSolver
     //! Build host tables now (no device work); used by introspection before the first step.
     void build_host_tables();
     double self_coeff() const { return m_self_coeff; }
+    //! Reciprocal-space cutoff in use: 1.2 kappa by default for the prolate split (beyond the
+    //! band limit, where deconvolving the window would only amplify round-off), 0 = none.
+    double effective_k_cut() const;
     double background_coeff() const { return m_bg_coeff; }
 
     //! Charge sums of the last step (synchronises with the device).

@@ -522,12 +522,15 @@ __global__ void __launch_bounds__(256) k_fft_z_conv(MeshGeom g,
 
 // ---------------------------------------------------------------------------
 // Influence function tabulation (FP64): G = F(|k|) / (k^2 |w(kx)|^2 |w(ky)|^2 |w(kz)|^2),
-// Gv = (dF/dk / k^3 - 2 F / k^4) / (same window product), both 0 at k = 0.
+// Gv = (dF/dk / k^3 - 2 F / k^4) / (same window product), both 0 at k = 0 and beyond the
+// spherical cutoff |k| > k_cut (k_cut > 0): past the band limit of the split the window
+// deconvolution only amplifies round-off of the spread mesh.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_build_influence(MeshGeom g,
                                                          int split_kind,
                                                          double kappa,
                                                          double rc,
+                                                         double k_cut,
                                                          int nq,
                                                          const double* __restrict__ qx,
                                                          const double* __restrict__ qw,
@@ -549,7 +552,7 @@ __global__ void __launch_bounds__(256) k_build_influence(MeshGeom g,
     const double two_pi = 6.283185307179586476925286766559;
     const double kx = two_pi * x / g.Lx, ky = two_pi * ky_i / g.Ly, kz = two_pi * kz_i / g.Lz;
     const double k2 = kx * kx + ky * ky + kz * kz;
-    if (k2 == 0.0)
+    if (k2 == 0.0 || (k_cut > 0.0 && k2 > k_cut * k_cut))
         {
         G[m] = 0.0;
         Gv[m] = 0.0;
@@ -928,12 +931,14 @@ ESP_INST_FFT(double)
 ESP_INST_FFT(float)
 #undef ESP_INST_FFT
 
-hipError_t build_influence(const MeshGeom& g, int split_kind, double kappa, double rc, int nq, const double* qx, const double* qw,
+hipError_t build_influence(const MeshGeom& g, int split_kind, double kappa, double rc, double k_cut, int nq, const double* qx,
+                           const double* qw,
                            const double* w2x, const double* w2y, const double* w2z, double* G, double* Gv, hipStream_t s)
     {
     const size_t total = size_t(g.nz) * g.ny * g.nxh;
     const unsigned int nb = unsigned((total + 255) / 256);
-    hipLaunchKernelGGL(k_build_influence, dim3(nb), dim3(256), 0, s, g, split_kind, kappa, rc, nq, qx, qw, w2x, w2y, w2z, G, Gv);
+    hipLaunchKernelGGL(k_build_influence, dim3(nb), dim3(256), 0, s, g, split_kind, kappa, rc, k_cut, nq, qx, qw, w2x, w2y, w2z, G,
+                       Gv);
     return hipGetLastError();
     }
 

@@ -33,6 +33,7 @@ class Params:
     window_c: float = 0.0
     window_degree: int = 0
     pair_lanes: int = 0
+    k_cut: float = 0.0
 
     def to_c(self) -> _lib.EspParams:
         p = _lib.EspParams()
@@ -43,6 +44,7 @@ class Params:
         p.precision, p.split = int(self.precision), int(self.split)
         p.window_c, p.window_degree = float(self.window_c), int(self.window_degree)
         p.pair_lanes = int(self.pair_lanes)
+        p.k_cut = float(self.k_cut)
         return p
 
 

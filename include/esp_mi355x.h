@@ -69,6 +69,9 @@ typedef struct esp_params {
     double window_c;               /* window prolate bandwidth; 0 = automatic */
     unsigned int window_degree;    /* window polynomial degree; 0 = automatic */
     int pair_lanes;                /* lanes per particle in the pair kernel; 0 = automatic */
+    double k_cut;                  /* spherical reciprocal-space cutoff: the influence function is 0
+                                      for |k| > k_cut.  0 = automatic (1.2 kappa for the PSWF split,
+                                      none for the Gaussian split); < 0 = none */
 } esp_params;
 
 /* One system: device pointers owned by the caller (HOOMD's ParticleData, NeighborList and
@@ -120,6 +123,7 @@ typedef struct esp_info {
     double background_coeff;        /* lim (1 - F(k)) / k^2 */
     unsigned int tile;              /* spreading tile edge (0 before the first step) */
     size_t device_bytes;            /* device memory held by the context */
+    double k_cut;                   /* effective reciprocal-space cutoff (0 = none) */
 } esp_info;
 
 /* ---- lifetime and parameters --------------------------------------------------------- */

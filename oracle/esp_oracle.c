@@ -354,6 +354,7 @@ typedef struct {
     const unsigned int* n_ex; /* may be NULL */
     const unsigned int* ex_list;
     unsigned int ex_pitch;
+    double k_cut;          /* spherical reciprocal-space cutoff: G = 0 for |k| > k_cut (0 = none) */
 } oracle_esp_in;
 
 typedef struct {
@@ -568,7 +569,7 @@ int oracle_esp(const oracle_esp_in* in, oracle_esp_out* out)
             for (int x = 0; x < nx; ++x) {
                 const size_t m = ((size_t)z * ny + y) * nx + x;
                 const double k2 = k2s[m];
-                if (k2 == 0.0) {
+                if (k2 == 0.0 || (in->k_cut > 0.0 && k2 > in->k_cut * in->k_cut)) {
                     mesh[m] = 0;
                     if (out->green) out->green[m] = 0;
                     continue;

@@ -69,7 +69,7 @@ def test_esp_objects_in_simulation_match_oracle():
     solver = spectral._cpp_obj
     scr, dscr, rlo, dri = solver.split_table()
     _, _, w = solver.window()
-    ref = O.OracleESP((L, L, L), (36, 36, 36), 8, w, scr, dscr, rlo, dri, 3.0).compute(pos, q)
+    ref = O.OracleESP((L, L, L), (36, 36, 36), 8, w, scr, dscr, rlo, dri, 3.0, k_cut=solver.info().k_cut).compute(pos, q)
     assert rel_rms(f[:, :3], ref["force"][:, :3]) < 1e-10
     ke, pe = sim.energies()
     assert abs(pe - ref["energy"]) < 1e-10 * abs(ref["energy"])

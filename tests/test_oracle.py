@@ -209,3 +209,26 @@ def test_oracle_exclusions_remove_the_coulomb_pair():
         f_direct[j] -= fv
     assert abs(r["energy"] - (ee - e_direct)) < 2e-6 * abs(ee)
     assert rel_rms(r["force"][:, :3], fe - f_direct) < 2e-6
+
+
+def test_oracle_reciprocal_cutoff_limits():
+    """k_cut beyond the largest mesh wavevector changes nothing; k_cut below the smallest
+    non-zero wavevector removes the whole reciprocal sum (only self and background remain)."""
+    L, n, P, c, rc = 8.0, 16, 6, 8.0, 3.0
+    pos, q = random_electrolyte(60, L, seed=4)
+    q[0] += 1.0  # net charge, so the background term is non-zero
+    base = esp_tables(c / rc, rc, P, L, n)
+    full = base.compute(pos, q)
+    kmax = np.sqrt(3) * np.pi * n / L
+    big = O.OracleESP((L, L, L), (n, n, n), P, base.wcoef, base.screen, base.dscreen, base.r_lo, base.dr_inv, rc,
+                      k_cut=1.01 * kmax).compute(pos, q)
+    assert np.array_equal(big["force"], full["force"]) and big["ext_energy"] == full["ext_energy"]
+    tiny = O.OracleESP((L, L, L), (n, n, n), P, base.wcoef, base.screen, base.dscreen, base.r_lo, base.dr_inv, rc,
+                       k_cut=0.99 * 2 * np.pi / L).compute(pos, q)
+    from esp_hoomd_amd import solver as S
+    self_c, bg_c = S.split_constants(0, c / rc, rc)
+    Q, Q2 = q.sum(), (q * q).sum()
+    expect = -0.5 * 0.079577471545947673 * self_c * Q2 - Q * Q / (2 * L ** 3) * bg_c
+    assert abs(tiny["ext_energy"] - expect) < 1e-12 * abs(expect)
+    # the pair part is untouched by the cutoff
+    np.testing.assert_array_equal(tiny["force"][:, 3], full["force"][:, 3])
