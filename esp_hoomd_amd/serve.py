@@ -1,0 +1,295 @@
+"""Serving ESP solves: continuous batching of MD requests on one GPU.
+
+A request is an independent periodic electrolyte (N particles, number density, charges) that
+asks for `steps` MD steps (velocity Verlet, WCA + ESP electrostatics).  The server
+
+* admits arrived requests while the HBM footprint of everything resident (particle state,
+  neighbour list, mesh, tables and scratch) stays under a budget and a slot is free —
+  requests that do not fit wait, in arrival order;
+* keeps every admitted request's particle, mesh and neighbour-list state resident on the
+  device until it finishes;
+* advances all resident requests by one MD step per tick, with the ESP forces of all of
+  them evaluated by one batched launch per stage (replayed from a captured hipGraph while
+  the set of resident requests is unchanged), so requests that arrive while others run join
+  the next tick and their work overlaps on the device;
+* records per-step latency (tick wall time, every resident request advancing one step),
+  queueing delay, completed requests and particle-steps.
+
+ESP parameters per request come from the autotuner's model at the force-error target
+(cached per size/density bucket).  The step is deterministic in mixed precision: a request
+computes the same trajectory alone or in any mix (the tests check this).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import heapq
+import math
+import time
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _lib
+from .batch import Batch
+from .device import synchronize
+from .md import NeighborList, State, WCA
+from .solver import Params, Solver
+from . import tune as _tune
+
+WCA_SIGMA = 1.0
+SKIN = 0.4
+
+
+def lattice_positions(N: int, density: float, seed: int, jitter: float = 0.1):
+    """N particles on randomly chosen sites of a jittered simple-cubic lattice (cubic box of
+    volume N / density, centred on the origin): no close contacts, so MD starts cleanly."""
+    L = (N / density) ** (1.0 / 3.0)
+    m = int(math.ceil(N ** (1.0 / 3.0)))
+    a = L / m
+    rng = np.random.default_rng(seed)
+    sites = rng.permutation(m ** 3)[:N]
+    idx = np.stack([sites % m, (sites // m) % m, sites // (m * m)], 1)
+    pos = (idx + 0.5) * a - L / 2 + rng.uniform(-jitter * a, jitter * a, (N, 3))
+    return pos, L
+
+
+def neutral_charges(N: int, q2_frac: float, seed: int) -> np.ndarray:
+    """+-1 charges with a fraction q2_frac of +-2, each valence class neutral (N even)."""
+    rng = np.random.default_rng(seed + 7919)
+    n2 = int(N * q2_frac) // 2 * 2
+    n1 = N - n2
+    q = np.concatenate([np.tile([2.0, -2.0], n2 // 2), np.tile([1.0, -1.0], n1 // 2), np.zeros(n1 % 2)])
+    return rng.permutation(q)
+
+
+@dataclass
+class Request:
+    rid: int
+    N: int
+    density: float
+    steps: int
+    arrival: float = 0.0          # seconds after the server starts
+    seed: int = 0
+    q2_frac: float = 0.1
+    kT: float = 1.0
+    # filled by the server
+    params: Optional[Params] = None
+    footprint: int = 0
+    admitted_at: Optional[float] = None
+    finished_at: Optional[float] = None
+    steps_done: int = 0
+    state: Optional[State] = field(default=None, repr=False)
+    solver: Optional[Solver] = field(default=None, repr=False)
+    nlist: Optional[NeighborList] = field(default=None, repr=False)
+    first_forces: Optional[np.ndarray] = field(default=None, repr=False)
+    final_positions: Optional[np.ndarray] = field(default=None, repr=False)
+    final_forces: Optional[np.ndarray] = field(default=None, repr=False)
+    prepared: Optional[tuple] = field(default=None, repr=False)
+
+    def setup(self) -> Tuple[np.ndarray, np.ndarray, float]:
+        N = self.N - (self.N % 2)
+        pos, L = lattice_positions(N, self.density, self.seed)
+        return pos, neutral_charges(N, self.q2_frac, self.seed), L
+
+
+def footprint_bytes(N: int, params: Params, neighbours: float) -> int:
+    """Device bytes a request holds while resident: particle arrays (position, charge,
+    velocity, force, virial: 152 B), neighbour list (4 B per entry at 1.3x the mean count,
+    plus counts and heads), sorting scratch (~48 B), and the solver's meshes and tables."""
+    M = params.nx * params.ny * params.nz
+    Mh = (params.nx // 2 + 1) * params.ny * params.nz
+    R = 8 if params.precision == _lib.ESP_PREC_FP64 else 4
+    per_particle = 152 + 48 + 12 + 4 * 1.3 * neighbours
+    mesh = M * 8 + 2 * Mh * R + M * R + 2 * Mh * 8
+    return int(N * per_particle + mesh + (1 << 20))
+
+
+class Server:
+    """Continuous-batching ESP/MD server (see the module docstring)."""
+
+    def __init__(self, hbm_budget: int = 32 << 30, max_active: int = 64, precision: int = _lib.ESP_PREC_MIXED,
+                 target: float = 1e-5, dt: float = 0.002, use_graph: bool = True, keep_results: bool = False,
+                 tuner_orders: Sequence[int] = (6, 8, 10), mesh_range: Tuple[int, int] = (16, 128)):
+        self.hbm_budget = int(hbm_budget)
+        self.max_active = int(max_active)
+        self.precision = precision
+        self.target = target
+        self.dt = dt
+        self.keep_results = keep_results
+        self.tuner_orders = tuple(tuner_orders)
+        self.mesh_range = mesh_range
+        self.batch = Batch(use_graph=use_graph)       # every resident request, every tick
+        self.batch_new = Batch(use_graph=False)       # initial forces of newly admitted requests
+        self.wca = WCA(WCA_SIGMA, 1.0)
+        self.pending: List[Tuple[float, int, Request]] = []
+        self.active: List[Request] = []
+        self.done: List[Request] = []
+        self.resident_bytes = 0
+        self.peak_bytes = 0
+        self.tick_ms: List[float] = []
+        self.tick_active: List[int] = []
+        self.tick_particles: List[int] = []
+        self.waited_for_memory = 0
+        self._tune_cache: Dict[Tuple[int, int], Params] = {}
+        self._lib = _lib.load()
+
+    # ---- request lifecycle ----------------------------------------------------------
+    def submit(self, req: Request) -> None:
+        heapq.heappush(self.pending, (req.arrival, req.rid, req))
+
+    def _params_for(self, N: int, L: float) -> Tuple[Params, float]:
+        density = N / L ** 3
+        key = (int(round(math.log(N) * 8)), int(round(math.log(density) * 16)))
+        if key not in self._tune_cache:
+            ch = _tune.tune(N, (L, L, L), self.target, self.precision, orders=self.tuner_orders,
+                            mesh_range=self.mesh_range)[0]
+            self._tune_cache[key] = (ch.params, ch.c, ch.sigma)
+        p, c, sigma = self._tune_cache[key]
+        # same c and sigma, mesh spacing and cutoff rescaled to this box
+        n = _tune.fft_size_at_least(max(2 * p.order, math.ceil(L / (math.pi * p.rcut / (sigma * c)) - 1e-9)))
+        h = L / n
+        rc = sigma * c * h / math.pi
+        if rc + SKIN > 0.5 * L:
+            ch = _tune.tune(N, (L, L, L), self.target, self.precision, orders=self.tuner_orders, mesh_range=self.mesh_range)[0]
+            return ch.params, ch.neighbours
+        params = Params(nx=n, ny=n, nz=n, order=p.order, kappa=c / rc, rcut=rc, precision=self.precision)
+        return params, density * 4.0 / 3.0 * math.pi * (rc + SKIN) ** 3
+
+    def _admit(self, now: float) -> bool:
+        changed = False
+        while self.pending and self.pending[0][0] <= now and len(self.active) < self.max_active:
+            req = self.pending[0][2]
+            if req.prepared is None:
+                pos, q, L = req.setup()
+                params, nbar = self._params_for(len(pos), L)
+                req.prepared = (pos, q, L, params, footprint_bytes(len(pos), params, nbar))
+            pos, q, L, params, need = req.prepared
+            if self.resident_bytes + need > self.hbm_budget and self.active:
+                self.waited_for_memory += 1
+                break
+            heapq.heappop(self.pending)
+            req.params, req.footprint, req.prepared = params, need, None
+            st = State(pos, q, [L, L, L])
+            st.thermalize(req.kT, seed=req.seed + 1)
+            req.state = st
+            req.nlist = NeighborList(buffer=SKIN)
+            req.nlist.update(st, max(params.rcut, self.wca.r_cut), force_rebuild=True)
+            req.solver = Solver(params)
+            req.admitted_at = now
+            self.active.append(req)
+            self.resident_bytes += need
+            self.peak_bytes = max(self.peak_bytes, self.resident_bytes)
+            changed = True
+        return changed
+
+    def _forces(self, reqs: List[Request], batch: Batch) -> None:
+        """Net force of the given requests: batched ESP (overwrites), then WCA (adds)."""
+        batch.enqueue([(r.solver, r.state) for r in reqs])
+        for r in reqs:
+            self.wca.add_to(r.state)
+
+    def _retire(self, now: float) -> None:
+        keep = []
+        for r in self.active:
+            if r.steps_done >= r.steps:
+                r.finished_at = now
+                if self.keep_results:
+                    r.final_positions = r.state.positions().copy()
+                    r.final_forces = r.state.force.numpy().copy()
+                self.resident_bytes -= r.footprint
+                r.state = r.solver = r.nlist = None
+                self.done.append(r)
+            else:
+                keep.append(r)
+        self.active = keep
+
+    def tick(self, now: float) -> None:
+        """One MD step of every resident request."""
+        lib = self._lib
+        t0 = time.perf_counter()
+        new = [r for r in self.active if r.steps_done == 0]
+        if new:
+            # forces at the initial positions of newly admitted requests (the first half kick
+            # needs them); a separate batch, so the resident set's captured graph stays valid
+            self._forces(new, self.batch_new)
+            if self.keep_results:
+                synchronize()
+                for r in new:
+                    r.first_forces = r.state.force.numpy().copy()
+        for r in self.active:
+            st = r.state
+            box = np.ascontiguousarray(st.box[:3])
+            _lib.check(lib.esp_integrate_first(st.N, st.postype.ptr, st.vel.ptr, st.force.ptr, self.dt, _lib.dptr(box), None))
+        for r in self.active:
+            r.nlist.update(r.state, max(r.params.rcut, self.wca.r_cut))
+        self._forces(self.active, self.batch)
+        for r in self.active:
+            st = r.state
+            _lib.check(lib.esp_integrate_second(st.N, st.vel.ptr, st.force.ptr, self.dt, 0.0, 0.0, 0, r.steps_done + 1, None))
+            r.steps_done += 1
+        synchronize()
+        self.tick_ms.append((time.perf_counter() - t0) * 1e3)
+        self.tick_active.append(len(self.active))
+        self.tick_particles.append(sum(r.state.N for r in self.active))
+
+    def run(self, time_limit: float = 1e9) -> dict:
+        """Serve until every submitted request is done (or time_limit seconds pass)."""
+        start = time.perf_counter()
+        while self.pending or self.active:
+            now = time.perf_counter() - start
+            if now > time_limit:
+                break
+            self._admit(now)
+            if not self.active:
+                # idle until the next arrival
+                wait = self.pending[0][0] - now if self.pending else 0.0
+                if wait > 0:
+                    time.sleep(min(wait, 0.01))
+                continue
+            self.tick(now)
+            self._retire(time.perf_counter() - start)
+        wall = time.perf_counter() - start
+        return self.stats(wall)
+
+    def stats(self, wall: float) -> dict:
+        lat = []
+        for ms, k in zip(self.tick_ms, self.tick_active):
+            lat.extend([ms] * k)  # every resident request waited this long for its step
+        lat = np.asarray(lat) if lat else np.zeros(1)
+        steps = int(sum(self.tick_active))
+        psteps = int(sum(self.tick_particles))
+        qdelay = [r.admitted_at - r.arrival for r in self.done if r.admitted_at is not None]
+        return {
+            "wall_s": wall,
+            "requests_done": len(self.done),
+            "requests_per_s": len(self.done) / wall if wall > 0 else 0.0,
+            "solves": steps,
+            "solves_per_s": steps / wall if wall > 0 else 0.0,
+            "particle_steps_per_s": psteps / wall if wall > 0 else 0.0,
+            "step_latency_ms_p50": float(np.percentile(lat, 50)),
+            "step_latency_ms_p99": float(np.percentile(lat, 99)),
+            "queue_delay_s_p50": float(np.percentile(qdelay, 50)) if qdelay else 0.0,
+            "queue_delay_s_p99": float(np.percentile(qdelay, 99)) if qdelay else 0.0,
+            "ticks": len(self.tick_ms),
+            "mean_active": float(np.mean(self.tick_active)) if self.tick_active else 0.0,
+            "peak_resident_bytes": self.peak_bytes,
+            "admissions_deferred_for_memory": self.waited_for_memory,
+            "batch": self.batch.stats(),
+        }
+
+
+def poisson_requests(n: int, rate: float, seed: int = 7, N_range=(1_000, 200_000), density_range=(0.05, 0.2),
+                     steps_range=(10, 1000), q2_frac: float = 0.1) -> List[Request]:
+    """n requests with Poisson arrivals at `rate` per second; N and steps log-uniform,
+    density uniform."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n))
+    N = np.exp(rng.uniform(np.log(N_range[0]), np.log(N_range[1]), n)).astype(int) // 2 * 2
+    dens = rng.uniform(*density_range, n)
+    steps = np.exp(rng.uniform(np.log(steps_range[0]), np.log(steps_range[1]), n)).astype(int)
+    return [Request(rid=i, N=int(N[i]), density=float(dens[i]), steps=int(steps[i]), arrival=float(t[i]),
+                    seed=int(seed * 100003 + i), q2_frac=q2_frac) for i in range(n)]
+
+
+__all__ = ["Request", "Server", "poisson_requests", "footprint_bytes", "lattice_positions", "neutral_charges"]

@@ -1,0 +1,71 @@
+"""Serving: continuous batching of MD requests.
+
+* A request's trajectory is the same alone and in a mix of concurrently resident requests
+  (mixed precision: forces within 1e-5 relative RMS at the first and last step, positions
+  within 1e-6 of the box; FP64: 1e-9).
+* Admission by HBM footprint: with a budget that holds one request at a time, requests are
+  deferred and served one after another, and every one finishes.
+* Latency and throughput statistics are consistent.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_rms
+from esp_hoomd_amd import _lib
+from esp_hoomd_amd.serve import Request, Server, footprint_bytes, poisson_requests
+
+pytestmark = pytest.mark.gpu
+
+
+def _req(rid, N, steps, arrival=0.0, seed=None, density=0.1):
+    return Request(rid=rid, N=N, density=density, steps=steps, arrival=arrival, seed=seed if seed is not None else 11 + rid)
+
+
+@pytest.mark.parametrize("precision,ftol,xtol", [(_lib.ESP_PREC_MIXED, 1e-5, 1e-6), (_lib.ESP_PREC_FP64, 1e-9, 1e-10)])
+def test_request_result_is_the_same_alone_and_in_the_mix(precision, ftol, xtol):
+    alone = Server(precision=precision, keep_results=True)
+    a = _req(0, 3000, 25)
+    alone.submit(a)
+    alone.run()
+    mix = Server(precision=precision, keep_results=True)
+    a2 = _req(0, 3000, 25)
+    others = [_req(1, 5000, 40, density=0.15), _req(2, 1200, 10, density=0.05), _req(3, 8000, 30, arrival=0.01)]
+    for r in [a2] + others:
+        mix.submit(r)
+    st = mix.run()
+    print(f"mix stats {st}", flush=True)
+    assert st["requests_done"] == 4 and st["mean_active"] > 1.5
+    L = (a.N / a.density) ** (1 / 3)
+    assert rel_rms(a2.first_forces[:, :3], a.first_forces[:, :3]) < ftol
+    assert rel_rms(a2.final_forces[:, :3], a.final_forces[:, :3]) < ftol
+    assert np.abs(a2.final_positions - a.final_positions).max() < xtol * L
+    same = np.array_equal(a2.final_forces, a.final_forces)
+    print(f"precision {precision}: bit-identical final forces alone vs mix: {same}", flush=True)
+
+
+def test_admission_by_footprint_serialises_when_memory_is_short():
+    reqs = [_req(i, 4000, 8, arrival=0.0) for i in range(3)]
+    srv0 = Server()
+    p, nbar = srv0._params_for(4000, (4000 / 0.1) ** (1 / 3))
+    one = footprint_bytes(4000, p, nbar)
+    srv = Server(hbm_budget=int(one * 1.5))
+    for r in reqs:
+        srv.submit(r)
+    st = srv.run()
+    assert st["requests_done"] == 3
+    assert st["peak_resident_bytes"] <= 1.5 * one
+    assert st["admissions_deferred_for_memory"] >= 1
+    assert max(srv.tick_active) == 1
+
+
+def test_poisson_stream_statistics():
+    reqs = poisson_requests(12, rate=200.0, seed=3, N_range=(1000, 20000), steps_range=(5, 40))
+    srv = Server()
+    for r in reqs:
+        srv.submit(r)
+    st = srv.run()
+    assert st["requests_done"] == 12
+    assert st["solves"] == sum(r.steps for r in reqs)
+    assert 0 < st["step_latency_ms_p50"] <= st["step_latency_ms_p99"]
+    assert st["particle_steps_per_s"] > 0
+    assert st["batch"]["replays"] >= 1
