@@ -115,6 +115,7 @@ class Choice:
     neighbours: float
     measured_ms: Optional[float] = None
     measured_error: Optional[float] = None
+    method_scale: float = 1.0
 
     def describe(self) -> dict:
         p = self.params
@@ -255,6 +256,7 @@ def autotune(positions: np.ndarray, charges: np.ndarray, box: Sequence[float], t
             cands.sort(key=lambda ch: ch.measured_ms)
         best = cands[0]
         best.measured_error = rel_rms_error(forces_with(best.params, positions, charges, box), ref)
+        best.method_scale = scale
         last = best
         if best.measured_error <= target:
             return best
