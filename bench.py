@@ -251,7 +251,7 @@ def batched_leg(n_sys, steps, warmup, seed=6):
     pairs, states, choices = [], [], []
     for pos, q, L in systems:
         ch = tune.tune(len(pos), (L, L, L), TARGET, _lib.ESP_PREC_MIXED, orders=(6, 8, 10), mesh_range=(16, 64),
-                       method_scale=scale, safety=1.1)[0]
+                       method_scale=scale, safety=1.1, batched=True)[0]
         st = State(pos, q, [L, L, L])
         nl = NeighborList(buffer=0.4)
         nl.update(st, ch.params.rcut, force_rebuild=True)

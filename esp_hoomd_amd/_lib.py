@@ -42,7 +42,7 @@ class EspParams(C.Structure):
         ("n_table", C.c_uint),
         ("precision", C.c_int), ("split", C.c_int),
         ("window_c", C.c_double), ("window_degree", C.c_uint),
-        ("pair_lanes", C.c_int), ("k_cut", C.c_double),
+        ("pair_lanes", C.c_int), ("k_cut", C.c_double), ("tile", C.c_uint),
     ]
 
 
@@ -159,6 +159,15 @@ def load() -> C.CDLL:
     _sig(lib, "esp_integrate_second", C.c_int, [C.c_uint, _VP, _VP, C.c_double, C.c_double, C.c_double, C.c_uint64,
                                                 C.c_uint64, _VP])
     _sig(lib, "esp_kinetic_potential", C.c_int, [C.c_uint, _VP, _VP, _DP, _DP])
+    _sig(lib, "esp_md_batch_create", C.c_int, [C.POINTER(_VP)])
+    _sig(lib, "esp_md_batch_destroy", C.c_int, [_VP])
+    _sig(lib, "esp_md_batch_last_error", C.c_char_p, [_VP])
+    _sig(lib, "esp_md_batch_set", C.c_int, [_VP, C.c_uint, C.POINTER(EspSystem), C.POINTER(_VP), C.POINTER(EspOut), C.POINTER(_VP),
+                                          C.POINTER(C.c_uint64)])
+    _sig(lib, "esp_md_batch_integrate_first", C.c_int, [_VP, C.c_double, _VP])
+    _sig(lib, "esp_md_batch_integrate_second", C.c_int, [_VP, C.c_double, C.c_double, C.c_double, C.c_uint64, _VP])
+    _sig(lib, "esp_md_batch_wca", C.c_int, [_VP, C.c_double, C.c_double, _VP])
+    _sig(lib, "esp_md_batch_update_nlists", C.c_int, [_VP, C.POINTER(C.c_uint), C.POINTER(C.c_ubyte), _VP])
     _sig(lib, "esp_device_count", C.c_int, [C.POINTER(C.c_int)])
     _sig(lib, "esp_set_device", C.c_int, [C.c_int])
     _sig(lib, "esp_device_malloc", C.c_int, [C.POINTER(_VP), C.c_size_t])

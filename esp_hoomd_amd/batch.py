@@ -38,6 +38,28 @@ class Batch:
         _lib.check_batch(_lib.load().esp_batch_compute(self._b, n, ctxs, systems, outs,
                                                        stream.handle if stream is not None else None), self._b)
 
+    def prepare(self, pairs: Sequence[Tuple[Solver, DeviceSystem]], flags: int = 0):
+        """Build the argument arrays once for a fixed set of systems (enqueue_prepared reuses
+        them; refresh(i) updates one system's entry, e.g. after its neighbour list moved)."""
+        n = len(pairs)
+        ctxs = (C.c_void_p * max(n, 1))()
+        systems = (_lib.EspSystem * max(n, 1))()
+        outs = (_lib.EspOut * max(n, 1))()
+        for i, (solver, system) in enumerate(pairs):
+            ctxs[i] = solver.ctx.value
+            systems[i] = system.c_system()
+            outs[i] = system.c_out(flags)
+        return {"n": n, "ctxs": ctxs, "systems": systems, "outs": outs, "pairs": list(pairs)}
+
+    @staticmethod
+    def refresh(prepared, i: int) -> None:
+        prepared["systems"][i] = prepared["pairs"][i][1].c_system()
+
+    def enqueue_prepared(self, prepared, stream=None) -> None:
+        _lib.check_batch(_lib.load().esp_batch_compute(self._b, prepared["n"], prepared["ctxs"], prepared["systems"],
+                                                       prepared["outs"], stream.handle if stream is not None else None),
+                         self._b)
+
     def compute(self, pairs: Sequence[Tuple[Solver, DeviceSystem]], flags: int = 0):
         """Evaluate and return host copies per system: force (N, 4), virial (6, N), ext (7,)."""
         self.enqueue(pairs, flags)

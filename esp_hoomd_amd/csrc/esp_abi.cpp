@@ -51,6 +51,7 @@ esp::SolverParams to_solver(const esp_params* p)
     s.window_degree = p->window_degree;
     s.pair_lanes = p->pair_lanes;
     s.k_cut = p->k_cut;
+    s.tile = p->tile;
     return s;
     }
 

@@ -166,6 +166,7 @@ hipError_t batch_finalize(const BatchSys* d, BatchList l, hipStream_t s);
 template<typename R> hipError_t batch_pair(const BatchSys* d, BatchList l, size_t lds, hipStream_t s);
 size_t tp_lds_bytes(const MeshGeom& g, int P, bool fp64);
 bool batch_order_supported(int P);
+bool stencil_tile_ok(int T, int P, bool fp64, bool batched);
 
     } // namespace kernel
     } // namespace esp

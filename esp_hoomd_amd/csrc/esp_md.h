@@ -16,6 +16,37 @@ struct CellGeom
     double Lx, Ly, Lz;
     };
 
+//! One system of a batched MD launch (device pointers; positions, velocities and forces are
+//! Scalar4 arrays).
+struct MDSys
+    {
+    unsigned int N;
+    double* postype;
+    double* vel;
+    double* force;
+    double* virial;         // [6][pitch] or null
+    size_t pitch;
+    double L[3];
+    const unsigned int* n_neigh;  // neighbour list (null: no WCA)
+    const unsigned int* nlist;
+    const uint64_t* head;
+    const double* last_pos; // positions at the last list build (null: no displacement check)
+    uint64_t seed;          // Langevin stream of this system
+    };
+
+//! (system, chunk of 256 particles)
+struct MDItem
+    {
+    unsigned int sys;
+    unsigned int chunk;
+    };
+
+hipError_t batch_vv_first(const MDSys* d, const MDItem* items, unsigned int n_items, double dt, hipStream_t s);
+hipError_t batch_vv_second(const MDSys* d, const MDItem* items, unsigned int n_items, double dt, double gamma, double kT, uint64_t step,
+                           hipStream_t s);
+hipError_t batch_wca(const MDSys* d, const MDItem* items, unsigned int n_items, double sigma, double epsilon, hipStream_t s);
+hipError_t batch_max_disp(const MDSys* d, const MDItem* items, unsigned int n_items, unsigned long long* out, hipStream_t s);
+
 hipError_t cell_count(unsigned int N, const double* pos, const CellGeom& cg, unsigned int* count, unsigned int* cell, unsigned int* slot,
                       hipStream_t s);
 hipError_t cell_fill(unsigned int N, const unsigned int* cell, const unsigned int* slot, const unsigned int* start, unsigned int* members,

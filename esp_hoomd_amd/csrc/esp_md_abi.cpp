@@ -6,7 +6,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <string>
+#include <vector>
 
 #include "esp_kernels.h"
 #include "esp_md.h"
@@ -147,6 +149,235 @@ int esp_nlist_destroy(esp_nlist* nl)
     delete nl;
     return ESP_OK;
     }
+
+} // extern "C"
+
+// ---- batched MD: many systems advanced by one launch per operation -----------------------
+struct esp_md_batch
+    {
+    std::vector<esp_system> sys;
+    std::vector<esp_nlist*> nls;
+    std::vector<esp::md::MDSys> desc;
+    std::vector<esp::md::MDItem> items;
+    esp::DeviceBuffer d_desc, d_items, d_disp;
+    std::vector<unsigned long long> h_disp;
+    std::string last_error;
+    };
+
+namespace
+{
+int md_batch_fail(esp_md_batch* b, int code, const std::string& msg)
+    {
+    if (b)
+        b->last_error = msg;
+    g_md_error = msg;
+    return code;
+    }
+
+template<class Fn> int md_batch_guard(esp_md_batch* b, Fn fn)
+    {
+    try
+        {
+        fn();
+        return ESP_OK;
+        }
+    catch (const esp::EspError& e)
+        {
+        return md_batch_fail(b, e.code(), e.what());
+        }
+    catch (const std::exception& e)
+        {
+        return md_batch_fail(b, ESP_ERR_STATE, e.what());
+        }
+    }
+
+//! Refresh system i's list pointers from its neighbour-list object.
+void md_batch_refresh(esp_md_batch* b, unsigned int i)
+    {
+    esp_nlist* nl = b->nls[i];
+    esp::md::MDSys& d = b->desc[i];
+    if (nl && nl->built)
+        {
+        d.n_neigh = nl->n_neigh.as<unsigned int>();
+        d.nlist = nl->nlist.as<unsigned int>();
+        d.head = nl->head.as<uint64_t>();
+        d.last_pos = nl->last_pos.as<double>();
+        }
+    else
+        {
+        d.n_neigh = nullptr;
+        d.nlist = nullptr;
+        d.head = nullptr;
+        d.last_pos = nullptr;
+        }
+    }
+
+void md_batch_upload(esp_md_batch* b)
+    {
+    esp::check_hip(hipMemcpy(b->d_desc.get(), b->desc.data(), b->desc.size() * sizeof(esp::md::MDSys), hipMemcpyHostToDevice),
+                   "hipMemcpy descriptors");
+    }
+} // namespace
+
+extern "C" {
+
+int esp_md_batch_create(esp_md_batch** b)
+    {
+    if (!b)
+        return md_batch_fail(nullptr, ESP_ERR_ARG, "esp_md_batch_create: null argument");
+    *b = new esp_md_batch();
+    return ESP_OK;
+    }
+
+int esp_md_batch_destroy(esp_md_batch* b)
+    {
+    delete b;
+    return ESP_OK;
+    }
+
+const char* esp_md_batch_last_error(const esp_md_batch* b)
+    {
+    return b ? b->last_error.c_str() : g_md_error.c_str();
+    }
+
+int esp_md_batch_set(esp_md_batch* b, unsigned int n, const esp_system* systems, double* const* vel, const esp_out* outs,
+                     esp_nlist* const* nls, const uint64_t* seeds)
+    {
+    if (!b || (n && (!systems || !vel || !outs)))
+        return md_batch_fail(b, ESP_ERR_ARG, "esp_md_batch_set: null argument");
+    return md_batch_guard(b,
+                          [&]
+                          {
+                              b->sys.assign(systems, systems + n);
+                              b->nls.assign(n, nullptr);
+                              b->desc.assign(n, esp::md::MDSys {});
+                              b->items.clear();
+                              for (unsigned int i = 0; i < n; ++i)
+                                  {
+                                  const esp_system& s = systems[i];
+                                  if (!s.postype || !vel[i] || !outs[i].force)
+                                      throw esp::EspError(ESP_ERR_ARG, "esp_md_batch_set: positions, velocities and forces are required");
+                                  if (s.box[3] != 0.0 || s.box[4] != 0.0 || s.box[5] != 0.0)
+                                      throw esp::EspError(ESP_ERR_UNSUPPORTED, "triclinic boxes are not supported");
+                                  if (nls)
+                                      b->nls[i] = nls[i];
+                                  esp::md::MDSys& d = b->desc[i];
+                                  d.N = s.N;
+                                  d.postype = const_cast<double*>(s.postype);
+                                  d.vel = vel[i];
+                                  d.force = outs[i].force;
+                                  d.virial = outs[i].virial;
+                                  d.pitch = outs[i].virial_pitch;
+                                  for (int a = 0; a < 3; ++a)
+                                      d.L[a] = s.box[a];
+                                  d.seed = seeds ? seeds[i] : 0;
+                                  md_batch_refresh(b, i);
+                                  for (unsigned int c = 0; c < (s.N + 255u) / 256u; ++c)
+                                      b->items.push_back({i, c});
+                                  }
+                              b->d_desc.resize(std::max<size_t>(n, 1) * sizeof(esp::md::MDSys));
+                              b->d_items.resize(std::max<size_t>(b->items.size(), 1) * sizeof(esp::md::MDItem));
+                              b->d_disp.resize(std::max<size_t>(n, 1) * 8);
+                              b->h_disp.assign(n, 0ull);
+                              if (n)
+                                  md_batch_upload(b);
+                              if (!b->items.empty())
+                                  esp::check_hip(hipMemcpy(b->d_items.get(), b->items.data(), b->items.size() * sizeof(esp::md::MDItem),
+                                                           hipMemcpyHostToDevice),
+                                                 "hipMemcpy items");
+                          });
+    }
+
+int esp_md_batch_integrate_first(esp_md_batch* b, double dt, esp_stream_t stream)
+    {
+    if (!b)
+        return ESP_ERR_ARG;
+    return md_batch_guard(b,
+                          [&]
+                          {
+                              esp::check_hip(esp::md::batch_vv_first(b->d_desc.as<esp::md::MDSys>(), b->d_items.as<esp::md::MDItem>(),
+                                                                     unsigned(b->items.size()), dt, static_cast<hipStream_t>(stream)),
+                                             "batch_vv_first");
+                          });
+    }
+
+int esp_md_batch_integrate_second(esp_md_batch* b, double dt, double gamma, double kT, uint64_t step, esp_stream_t stream)
+    {
+    if (!b)
+        return ESP_ERR_ARG;
+    return md_batch_guard(b,
+                          [&]
+                          {
+                              esp::check_hip(esp::md::batch_vv_second(b->d_desc.as<esp::md::MDSys>(), b->d_items.as<esp::md::MDItem>(),
+                                                                      unsigned(b->items.size()), dt, gamma, kT, step,
+                                                                      static_cast<hipStream_t>(stream)),
+                                             "batch_vv_second");
+                          });
+    }
+
+int esp_md_batch_wca(esp_md_batch* b, double sigma, double epsilon, esp_stream_t stream)
+    {
+    if (!b)
+        return ESP_ERR_ARG;
+    return md_batch_guard(b,
+                          [&]
+                          {
+                              esp::check_hip(esp::md::batch_wca(b->d_desc.as<esp::md::MDSys>(), b->d_items.as<esp::md::MDItem>(),
+                                                                unsigned(b->items.size()), sigma, epsilon, static_cast<hipStream_t>(stream)),
+                                             "batch_wca");
+                          });
+    }
+
+int esp_md_batch_update_nlists(esp_md_batch* b, unsigned int* rebuilt, unsigned char* which, esp_stream_t stream)
+    {
+    if (!b)
+        return ESP_ERR_ARG;
+    return md_batch_guard(b,
+                          [&]
+                          {
+                              hipStream_t s = static_cast<hipStream_t>(stream);
+                              const unsigned int n = unsigned(b->desc.size());
+                              unsigned int count = 0;
+                              if (n)
+                                  {
+                                  esp::check_hip(hipMemsetAsync(b->d_disp.get(), 0, size_t(n) * 8, s), "hipMemsetAsync");
+                                  esp::check_hip(esp::md::batch_max_disp(b->d_desc.as<esp::md::MDSys>(), b->d_items.as<esp::md::MDItem>(),
+                                                                         unsigned(b->items.size()), b->d_disp.as<unsigned long long>(), s),
+                                                 "batch_max_disp");
+                                  esp::check_hip(hipMemcpyAsync(b->h_disp.data(), b->d_disp.get(), size_t(n) * 8, hipMemcpyDeviceToHost, s),
+                                                 "hipMemcpyAsync");
+                                  esp::check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
+                                  }
+                              bool changed = false;
+                              for (unsigned int i = 0; i < n; ++i)
+                                  {
+                                  esp_nlist* nl = b->nls[i];
+                                  bool need = false;
+                                  if (nl)
+                                      {
+                                      double d2;
+                                      std::memcpy(&d2, &b->h_disp[i], 8);
+                                      const esp_system& sy = b->sys[i];
+                                      need = !nl->built || sy.N != nl->N || sy.box[0] != nl->box[0] || sy.box[1] != nl->box[1]
+                                             || sy.box[2] != nl->box[2] || d2 >= 0.25 * nl->r_buff * nl->r_buff;
+                                      if (need)
+                                          {
+                                          do_build(nl, &sy, s);
+                                          md_batch_refresh(b, i);
+                                          changed = true;
+                                          ++count;
+                                          }
+                                      }
+                                  if (which)
+                                      which[i] = need ? 1 : 0;
+                                  }
+                              if (changed)
+                                  md_batch_upload(b);
+                              if (rebuilt)
+                                  *rebuilt = count;
+                          });
+    }
+
 
 const char* esp_nlist_last_error(const esp_nlist* nl)
     {

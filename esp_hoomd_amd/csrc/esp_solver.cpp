@@ -225,6 +225,8 @@ void EspSolver::set_params(const SolverParams& p)
         throw EspError(STATUS_ARG, "pair_lanes must be 0 or a power of two <= 32");
     if (!std::isfinite(p.k_cut))
         throw EspError(STATUS_ARG, "k_cut must be finite");
+    if (p.tile != 0 && p.tile != 2 && p.tile != 4 && p.tile != 8 && p.tile != 16)
+        throw EspError(STATUS_ARG, "tile must be 0, 2, 4, 8 or 16");
     // validate FFT factorisation eagerly
     make_fft_plan(int(p.nx));
     make_fft_plan(int(p.ny));
@@ -312,19 +314,29 @@ void EspSolver::initialize(const SystemView& sys, hipStream_t s)
     g.ny = int(m_p.ny);
     g.nz = int(m_p.nz);
     g.nxh = g.nx / 2 + 1;
-    // largest tile edge in {8, 4, 2} whose LDS box fits 64 KiB; 4 on small meshes for parallelism
-    int T = 8;
-    for (; T > 2; T /= 2)
+    // tile edge: the requested one, else the largest of {8, 4, 2} whose kernels fit their LDS
+    // box, dropping to 4 on small meshes (fewer than 256 tiles of 8) for parallelism
+    auto tiles_of = [&](int t) { return size_t((g.nx + t - 1) / t) * ((g.ny + t - 1) / t) * ((g.nz + t - 1) / t); };
+    int T = 0;
+    if (m_p.tile)
         {
-        g.B = T + int(P) - 1;
-        if (kernel::gather_lds_bytes(g, int(P), 0, fp64) <= 64 * 1024)
-            break;
+        T = int(m_p.tile);
+        if (!kernel::stencil_tile_ok(T, int(P), fp64, false))
+            throw EspError(STATUS_UNSUPPORTED, "the requested tile does not fit the stencil kernels' LDS at this order and precision");
         }
-    if (T == 8 && size_t((g.nx + 7) / 8) * ((g.ny + 7) / 8) * ((g.nz + 7) / 8) < 256)
-        T = 4;
-    g.B = T + int(P) - 1;
-    if (kernel::gather_lds_bytes(g, int(P), 0, fp64) > 64 * 1024)
-        throw EspError(STATUS_UNSUPPORTED, "stencil order too large for the LDS tile at this precision");
+    else
+        {
+        for (int t : {8, 4, 2})
+            if (kernel::stencil_tile_ok(t, int(P), fp64, false))
+                {
+                T = t;
+                break;
+                }
+        if (T == 0)
+            throw EspError(STATUS_UNSUPPORTED, "stencil order too large for the LDS tile at this precision");
+        if (T == 8 && tiles_of(8) < 256 && kernel::stencil_tile_ok(4, int(P), fp64, false))
+            T = 4;
+        }
     g.T = T;
     g.tx = (g.nx + T - 1) / T;
     g.ty = (g.ny + T - 1) / T;
@@ -664,6 +676,8 @@ void EspSolver::describe(const SystemView& sys, const OutputView& out, hipStream
     if (!kernel::batch_order_supported(int(m_p.order)))
         throw EspError(STATUS_UNSUPPORTED, "batched solves support stencil orders 4, 6, 8, 10 and 12");
     const unsigned int n = prepare(sys, out, s);
+    if (!kernel::stencil_tile_ok(m_geom.T, int(m_p.order), m_p.precision == 0, true))
+        throw EspError(STATUS_UNSUPPORTED, "the tile of this system does not fit the batched stencil kernels' LDS");
     const bool fp64 = m_p.precision == 0;
     const MeshGeom& g = m_geom;
     d = kernel::BatchSys {};

@@ -78,6 +78,7 @@ struct SolverParams
     unsigned int window_degree = 0;  // 0 = automatic
     int pair_lanes = 0;      // lanes per particle in the pair kernel (0 = automatic)
     double k_cut = 0.0;      // reciprocal-space cutoff (0 = automatic, < 0 = none)
+    unsigned int tile = 0;   // spreading tile edge (0 = automatic; 2, 4, 8 or 16)
     };
 
 //! Device-side description of one system for one call (caller-owned device pointers).

@@ -99,15 +99,13 @@ __global__ void __launch_bounds__(256) k_nlist(unsigned int N, const double4* __
         atomicMax(overflow, k);
     }
 
-//! WCA (LJ truncated at 2^(1/6) sigma and shifted) over the full list; adds to force/virial.
-__global__ void __launch_bounds__(256) k_wca(unsigned int N, const double4* __restrict__ pos, double Lx, double Ly, double Lz,
-                                             const unsigned int* __restrict__ n_neigh, const unsigned int* __restrict__ nlist,
-                                             const uint64_t* __restrict__ head, double sigma, double epsilon, double4* __restrict__ force,
-                                             double* __restrict__ virial, size_t pitch)
+//! WCA (LJ truncated at 2^(1/6) sigma and shifted) of particle i over the full list; adds to
+//! its force/virial.
+__device__ __forceinline__ void wca_one(unsigned int i, const double4* __restrict__ pos, double Lx, double Ly, double Lz,
+                                        const unsigned int* __restrict__ n_neigh, const unsigned int* __restrict__ nlist,
+                                        const uint64_t* __restrict__ head, double sigma, double epsilon, double4* __restrict__ force,
+                                        double* __restrict__ virial, size_t pitch)
     {
-    const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N)
-        return;
     const double4 ri = pos[i];
     const double invLx = 1.0 / Lx, invLy = 1.0 / Ly, invLz = 1.0 / Lz;
     const double rc2 = 1.2599210498948732 * sigma * sigma;  // (2^(1/6) sigma)^2
@@ -157,6 +155,16 @@ __global__ void __launch_bounds__(256) k_wca(unsigned int N, const double4* __re
         }
     }
 
+__global__ void __launch_bounds__(256) k_wca(unsigned int N, const double4* __restrict__ pos, double Lx, double Ly, double Lz,
+                                             const unsigned int* __restrict__ n_neigh, const unsigned int* __restrict__ nlist,
+                                             const uint64_t* __restrict__ head, double sigma, double epsilon, double4* __restrict__ force,
+                                             double* __restrict__ virial, size_t pitch)
+    {
+    const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N)
+        wca_one(i, pos, Lx, Ly, Lz, n_neigh, nlist, head, sigma, epsilon, force, virial, pitch);
+    }
+
 //! Counter-based uniform random number in (-1, 1) from (seed, step, particle, component).
 __device__ __forceinline__ double uniform_pm1(uint64_t seed, uint64_t step, uint64_t i, uint32_t c)
     {
@@ -174,14 +182,11 @@ __device__ __forceinline__ double uniform_pm1(uint64_t seed, uint64_t step, uint
     return (double(x >> 11) + 0.5) * (2.0 / 9007199254740992.0) - 1.0;
     }
 
-//! Velocity-Verlet first half: v += f/(2m) dt; x += v dt (wrapped into the box).
+//! Velocity-Verlet first half for particle i: v += f/(2m) dt; x += v dt (wrapped into the box).
 //! vel = (vx, vy, vz, mass).
-__global__ void __launch_bounds__(256) k_vv_first(unsigned int N, double4* __restrict__ pos, double4* __restrict__ vel,
-                                                  const double4* __restrict__ force, double dt, double Lx, double Ly, double Lz)
+__device__ __forceinline__ void vv_first_one(unsigned int i, double4* __restrict__ pos, double4* __restrict__ vel,
+                                             const double4* __restrict__ force, double dt, double Lx, double Ly, double Lz)
     {
-    const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N)
-        return;
     double4 v = vel[i];
     const double4 f = force[i];
     const double h = 0.5 * dt / v.w;
@@ -199,14 +204,20 @@ __global__ void __launch_bounds__(256) k_vv_first(unsigned int N, double4* __res
     vel[i] = v;
     }
 
-//! Velocity-Verlet second half with an optional Langevin bath (gamma > 0): the drag -gamma v and
-//! a uniform random force of variance 2 gamma kT / dt per component are added to f.
-__global__ void __launch_bounds__(256) k_vv_second(unsigned int N, double4* __restrict__ vel, const double4* __restrict__ force, double dt,
-                                                   double gamma, double kT, uint64_t seed, uint64_t step)
+__global__ void __launch_bounds__(256) k_vv_first(unsigned int N, double4* __restrict__ pos, double4* __restrict__ vel,
+                                                  const double4* __restrict__ force, double dt, double Lx, double Ly, double Lz)
     {
     const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N)
-        return;
+    if (i < N)
+        vv_first_one(i, pos, vel, force, dt, Lx, Ly, Lz);
+    }
+
+//! Velocity-Verlet second half for particle i with an optional Langevin bath (gamma > 0): the
+//! drag -gamma v and a uniform random force of variance 2 gamma kT / dt per component are
+//! added to f.
+__device__ __forceinline__ void vv_second_one(unsigned int i, double4* __restrict__ vel, const double4* __restrict__ force, double dt,
+                                              double gamma, double kT, uint64_t seed, uint64_t step)
+    {
     double4 v = vel[i];
     const double4 f = force[i];
     const double h = 0.5 * dt / v.w;
@@ -225,6 +236,14 @@ __global__ void __launch_bounds__(256) k_vv_second(unsigned int N, double4* __re
     v.y += h * f.y + k * by;
     v.z += h * f.z + k * bz;
     vel[i] = v;
+    }
+
+__global__ void __launch_bounds__(256) k_vv_second(unsigned int N, double4* __restrict__ vel, const double4* __restrict__ force, double dt,
+                                                   double gamma, double kT, uint64_t seed, uint64_t step)
+    {
+    const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N)
+        vv_second_one(i, vel, force, dt, gamma, kT, seed, step);
     }
 
 //! Largest squared minimum-image displacement since the last neighbour-list build, as the bits
@@ -288,6 +307,65 @@ __global__ void __launch_bounds__(256) k_energies(unsigned int N, const double4*
         part[2 * blockIdx.x] = a;
         part[2 * blockIdx.x + 1] = b;
         }
+    }
+
+// ---------------------------------------------------------------------------
+// Batched MD: one launch advances many independent systems.  One workgroup per item
+// (system, chunk of 256 particles); each thread runs the single-system per-particle code.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) kb_vv_first(const md::MDSys* __restrict__ ds, const md::MDItem* __restrict__ items, double dt)
+    {
+    const md::MDItem w = items[blockIdx.x];
+    const md::MDSys& d = ds[w.sys];
+    const unsigned int i = w.chunk * 256u + threadIdx.x;
+    if (i < d.N)
+        vv_first_one(i, reinterpret_cast<double4*>(d.postype), reinterpret_cast<double4*>(d.vel),
+                     reinterpret_cast<const double4*>(d.force), dt, d.L[0], d.L[1], d.L[2]);
+    }
+
+__global__ void __launch_bounds__(256) kb_vv_second(const md::MDSys* __restrict__ ds, const md::MDItem* __restrict__ items, double dt,
+                                                    double gamma, double kT, uint64_t step)
+    {
+    const md::MDItem w = items[blockIdx.x];
+    const md::MDSys& d = ds[w.sys];
+    const unsigned int i = w.chunk * 256u + threadIdx.x;
+    if (i < d.N)
+        vv_second_one(i, reinterpret_cast<double4*>(d.vel), reinterpret_cast<const double4*>(d.force), dt, gamma, kT, d.seed, step);
+    }
+
+__global__ void __launch_bounds__(256) kb_wca(const md::MDSys* __restrict__ ds, const md::MDItem* __restrict__ items, double sigma,
+                                              double epsilon)
+    {
+    const md::MDItem w = items[blockIdx.x];
+    const md::MDSys& d = ds[w.sys];
+    const unsigned int i = w.chunk * 256u + threadIdx.x;
+    if (i < d.N && d.nlist != nullptr)
+        wca_one(i, reinterpret_cast<const double4*>(d.postype), d.L[0], d.L[1], d.L[2], d.n_neigh, d.nlist, d.head, sigma, epsilon,
+                reinterpret_cast<double4*>(d.force), d.virial, d.pitch);
+    }
+
+//! Per-system largest squared displacement since its last neighbour-list build (bits of a
+//! non-negative double in out[sys], which the caller zeroes).
+__global__ void __launch_bounds__(256) kb_max_disp(const md::MDSys* __restrict__ ds, const md::MDItem* __restrict__ items,
+                                                   unsigned long long* __restrict__ out)
+    {
+    const md::MDItem w = items[blockIdx.x];
+    const md::MDSys& d = ds[w.sys];
+    const unsigned int i = w.chunk * 256u + threadIdx.x;
+    double d2 = 0.0;
+    if (i < d.N && d.last_pos != nullptr)
+        {
+        const double4 a = reinterpret_cast<const double4*>(d.postype)[i];
+        const double4 b = reinterpret_cast<const double4*>(d.last_pos)[i];
+        const double dx = wrap_min_image(a.x - b.x, d.L[0], 1.0 / d.L[0]);
+        const double dy = wrap_min_image(a.y - b.y, d.L[1], 1.0 / d.L[1]);
+        const double dz = wrap_min_image(a.z - b.z, d.L[2], 1.0 / d.L[2]);
+        d2 = dx * dx + dy * dy + dz * dz;
+        }
+    for (int off = 32; off > 0; off >>= 1)
+        d2 = fmax(d2, __shfl_xor(d2, off, 64));
+    if ((threadIdx.x & 63) == 0)
+        atomicMax(out + w.sys, static_cast<unsigned long long>(__double_as_longlong(d2)));
     }
 
     } // namespace dev
@@ -384,6 +462,39 @@ hipError_t energies(unsigned int N, const double* vel, const double* force, doub
         return hipSuccess;
     hipLaunchKernelGGL(k_energies, dim3(nblocks(N)), dim3(256), 0, s, N, reinterpret_cast<const double4*>(vel),
                        reinterpret_cast<const double4*>(force), part);
+    return hipGetLastError();
+    }
+
+hipError_t batch_vv_first(const MDSys* d, const MDItem* items, unsigned int n_items, double dt, hipStream_t s)
+    {
+    if (n_items == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(kb_vv_first, dim3(n_items), dim3(256), 0, s, d, items, dt);
+    return hipGetLastError();
+    }
+
+hipError_t batch_vv_second(const MDSys* d, const MDItem* items, unsigned int n_items, double dt, double gamma, double kT, uint64_t step,
+                           hipStream_t s)
+    {
+    if (n_items == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(kb_vv_second, dim3(n_items), dim3(256), 0, s, d, items, dt, gamma, kT, step);
+    return hipGetLastError();
+    }
+
+hipError_t batch_wca(const MDSys* d, const MDItem* items, unsigned int n_items, double sigma, double epsilon, hipStream_t s)
+    {
+    if (n_items == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(kb_wca, dim3(n_items), dim3(256), 0, s, d, items, sigma, epsilon);
+    return hipGetLastError();
+    }
+
+hipError_t batch_max_disp(const MDSys* d, const MDItem* items, unsigned int n_items, unsigned long long* out, hipStream_t s)
+    {
+    if (n_items == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(kb_max_disp, dim3(n_items), dim3(256), 0, s, d, items, out);
     return hipGetLastError();
     }
 

@@ -16,6 +16,7 @@
 #include "esp_kernels.h"
 
 #include <algorithm>
+#include <cstdlib>
 
 namespace esp
 {
@@ -82,7 +83,7 @@ __device__ __forceinline__ void pair_terms(const R* __restrict__ tab,
 //! Pair correction of particle i by the G lanes of its group (`sub` = lane within the group);
 //! every lane of the group must call it.  Writes force (fx, fy, fz, energy) and, when
 //! requested, the six per-particle virial components.
-template<typename R, int G, bool COMPACT>
+template<typename R, int G, bool COMPACT, int U = 2>
 __device__ __forceinline__ void pair_particle(unsigned int i, int sub, const kernel::PairArgs& a, const R* __restrict__ tab)
     {
     const double invLx = 1.0 / a.Lx, invLy = 1.0 / a.Ly, invLz = 1.0 / a.Lz;
@@ -96,19 +97,16 @@ __device__ __forceinline__ void pair_particle(unsigned int i, int sub, const ker
     const double qi = a.charge[i];
     if (a.nlist != nullptr && qi != 0.0 && a.rc >= 1.0e-12)
         {
-        const unsigned int nn = a.n_neigh[i];
-        const uint64_t head = a.head_list[i];
-        for (unsigned int k = sub; k < nn; k += G)
+        const double4* __restrict__ pos = reinterpret_cast<const double4*>(a.postype);
+        auto add = [&](const double4& pj, double qj)
             {
-            const unsigned int j = a.nlist[head + k];
-            const double4 pj = reinterpret_cast<const double4*>(a.postype)[j];
-            const R qiqj = R(qi * a.charge[j]);
+            const R qiqj = R(qi * qj);
             const R dx = R(min_image(pi.x - pj.x, a.Lx, invLx));
             const R dy = R(min_image(pi.y - pj.y, a.Ly, invLy));
             const R dz = R(min_image(pi.z - pj.z, a.Lz, invLz));
             const R rsq = dx * dx + dy * dy + dz * dz;
             if (rsq >= rc2 || qiqj == R(0))
-                continue;
+                return;
             const R r = sqrt(rsq);
             const R rinv = R(1) / r;
             R Lr, ndr;
@@ -125,6 +123,34 @@ __device__ __forceinline__ void pair_particle(unsigned int i, int sub, const ker
             v[3] += hf * dy * dy;
             v[4] += hf * dy * dz;
             v[5] += hf * dz * dz;
+            };
+        const unsigned int nn = a.n_neigh[i];
+        const uint64_t head = a.head_list[i];
+        if constexpr (U == 2)
+            {
+            // two neighbours in flight per lane; they are accumulated in list order, so the
+            // sums are the same as one at a time
+            for (unsigned int k = sub; k < nn; k += 2 * G)
+                {
+                const bool two = k + G < nn;
+                const unsigned int j0 = a.nlist[head + k];
+                const unsigned int j1 = two ? a.nlist[head + k + G] : j0;
+                const double4 p0 = pos[j0];
+                const double4 p1 = pos[j1];
+                const double q0 = a.charge[j0];
+                const double q1 = a.charge[j1];
+                add(p0, q0);
+                if (two)
+                    add(p1, q1);
+                }
+            }
+        else
+            {
+            for (unsigned int k = sub; k < nn; k += G)
+                {
+                const unsigned int j = a.nlist[head + k];
+                add(pos[j], a.charge[j]);
+                }
             }
         }
     if (a.n_ex != nullptr && qi != 0.0)
@@ -202,7 +228,11 @@ __device__ __forceinline__ const R* stage_compact_table(const kernel::PairArgs& 
     return t;
     }
 
-template<typename R, int G, bool COMPACT>
+//! Single-system pair kernel.  MAP = 1: XCD-aware contiguous ranges -- block b runs on XCD
+//! b % 8, so blocks are renumbered to give each XCD one contiguous eighth of the particles
+//! (its L2 then holds the neighbour positions of one region instead of every region);
+//! MAP = 0: plain grid stride.
+template<typename R, int G, bool COMPACT, int U, int MAP>
 __global__ void __launch_bounds__(256) k_pair(kernel::PairArgs a)
     {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -213,9 +243,23 @@ __global__ void __launch_bounds__(256) k_pair(kernel::PairArgs a)
         __syncthreads();
         }
     const int sub = int(threadIdx.x % G);
-    const unsigned int groups_per_grid = gridDim.x * (blockDim.x / G);
-    for (unsigned int i = (blockIdx.x * blockDim.x + threadIdx.x) / G; i < a.N + 0u; i += groups_per_grid)
-        pair_particle<R, G, COMPACT>(i, sub, a, tab);
+    const unsigned int per_pass = blockDim.x / G;
+    if constexpr (MAP == 1)
+        {
+        const unsigned int nb = gridDim.x;
+        const unsigned int lb = (nb % 8u == 0u) ? (blockIdx.x % 8u) * (nb / 8u) + blockIdx.x / 8u : blockIdx.x;
+        const unsigned int span = (a.N + nb - 1u) / nb;
+        const unsigned int p0 = lb * span;
+        const unsigned int p1 = min(a.N, p0 + span);
+        for (unsigned int i = p0 + threadIdx.x / G; i < p1; i += per_pass)
+            pair_particle<R, G, COMPACT, U>(i, sub, a, tab);
+        }
+    else
+        {
+        const unsigned int groups_per_grid = gridDim.x * per_pass;
+        for (unsigned int i = (blockIdx.x * blockDim.x + threadIdx.x) / G; i < a.N + 0u; i += groups_per_grid)
+            pair_particle<R, G, COMPACT, U>(i, sub, a, tab);
+        }
     }
 
 //! Batched pair correction: items are (system, chunk of 256 / G particles).  The compact
@@ -264,15 +308,36 @@ __global__ void __launch_bounds__(256) kb_pair(const kernel::BatchSys* __restric
 
 namespace kernel
 {
+//! Pair-kernel variant (environment ESP_PAIR_VARIANT, default 3): bit 0 = XCD-aware
+//! contiguous ranges, bit 1 = two neighbours in flight per lane.
+static int pair_variant()
+    {
+    static int v = -1;
+    if (v < 0)
+        {
+        const char* e = getenv("ESP_PAIR_VARIANT");
+        v = e ? (atoi(e) & 3) : 3;
+        }
+    return v;
+    }
+
+template<typename R, int G, bool C> static void launch_pair_v(const PairArgs& a, unsigned int nb, size_t lds, hipStream_t s)
+    {
+    switch (pair_variant())
+        {
+        case 0: hipLaunchKernelGGL((dev::k_pair<R, G, C, 1, 0>), dim3(nb), dim3(256), lds, s, a); break;
+        case 1: hipLaunchKernelGGL((dev::k_pair<R, G, C, 1, 1>), dim3(nb), dim3(256), lds, s, a); break;
+        case 2: hipLaunchKernelGGL((dev::k_pair<R, G, C, 2, 0>), dim3(nb), dim3(256), lds, s, a); break;
+        default: hipLaunchKernelGGL((dev::k_pair<R, G, C, 2, 1>), dim3(nb), dim3(256), lds, s, a); break;
+        }
+    }
+
 template<typename R, int G> static void launch_pair(const PairArgs& a, unsigned int nb, hipStream_t s)
     {
     if (a.compact_tab != nullptr)
-        {
-        const size_t lds = size_t(a.n_compact) * (a.compact_deg + 1) * sizeof(R);
-        hipLaunchKernelGGL((dev::k_pair<R, G, true>), dim3(nb), dim3(256), lds, s, a);
-        }
+        launch_pair_v<R, G, true>(a, nb, size_t(a.n_compact) * (a.compact_deg + 1) * sizeof(R), s);
     else
-        hipLaunchKernelGGL((dev::k_pair<R, G, false>), dim3(nb), dim3(256), 0, s, a);
+        launch_pair_v<R, G, false>(a, nb, 0, s);
     }
 
 template<typename R> hipError_t pair_forces(const PairArgs& a, int lanes_per_particle, hipStream_t s)
@@ -281,7 +346,15 @@ template<typename R> hipError_t pair_forces(const PairArgs& a, int lanes_per_par
         return hipSuccess;
     const size_t threads = size_t(a.N) * lanes_per_particle;
     // persistent grid: at most 8 blocks per CU of a 256-CU device
-    const unsigned int nb = unsigned(std::min<size_t>((threads + 255) / 256, 2048));
+    static int max_blocks = -1;
+    if (max_blocks < 0)
+        {
+        const char* e = getenv("ESP_PAIR_MAX_BLOCKS");
+        max_blocks = e ? std::max(8, atoi(e)) : 2048;
+        }
+    unsigned int nb = unsigned(std::min<size_t>((threads + 255) / 256, size_t(max_blocks)));
+    if (nb >= 8)
+        nb &= ~7u;  // a multiple of the 8 XCDs for the XCD-aware mapping
     switch (lanes_per_particle)
         {
         case 1: launch_pair<R, 1>(a, nb, s); break;

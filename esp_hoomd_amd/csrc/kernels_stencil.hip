@@ -17,6 +17,7 @@
 #include "esp_device.h"
 #include "esp_kernels.h"
 
+#include <algorithm>
 #include <cstdlib>
 
 namespace esp
@@ -808,6 +809,34 @@ static int stencil_variant(int which, bool fp64)
     if (v[which] >= 0)
         return v[which];
     return which == 0 ? (fp64 ? 2 : 1) : 1;
+    }
+
+size_t tp_lds_bytes(const MeshGeom& g, int P, bool fp64);
+size_t spread_lds_bytes(const MeshGeom& g, int P, int D, bool fp64);
+size_t gather_lds_bytes(const MeshGeom& g, int P, int D, bool fp64);
+
+//! Whether the spreading/gathering kernels selected for this precision can run on tiles of
+//! edge T at stencil order P: their LDS box must fit 64 KiB, and column-ownership spreading
+//! has instantiations for T = 2, 4, 8 only.  `batched` asks for the batched kernels (which
+//! always use the thread-per-particle forms).
+bool stencil_tile_ok(int T, int P, bool fp64, bool batched)
+    {
+    MeshGeom g {};
+    g.T = T;
+    g.B = T + P - 1;
+    const int sv = batched ? 1 : stencil_variant(0, fp64);
+    const int gv = batched ? 1 : stencil_variant(1, fp64);
+    size_t lds = 0;
+    if (sv == 2)
+        {
+        if (T > 8)
+            return false;
+        lds = own_lds_bytes(P, fp64);
+        }
+    else
+        lds = sv == 1 ? tp_lds_bytes(g, P, fp64) : spread_lds_bytes(g, P, 0, fp64);
+    lds = std::max(lds, gv == 1 ? tp_lds_bytes(g, P, fp64) : gather_lds_bytes(g, P, 0, fp64));
+    return lds <= 64 * 1024;
     }
 
 size_t tp_lds_bytes(const MeshGeom& g, int P, bool fp64)

@@ -306,4 +306,69 @@ def random_packing(N: int, L: float, min_dist: float, seed: int = 0):
     raise RuntimeError("random_packing: could not place particles")
 
 
-__all__ = ["State", "NeighborList", "WCA", "NVE", "Langevin", "Simulation", "jittered_lattice", "random_packing"]
+class BatchMD:
+    """Velocity Verlet, WCA and neighbour-list upkeep for many States, one launch each."""
+
+    def __init__(self):
+        b = C.c_void_p()
+        _lib.check(_lib.load().esp_md_batch_create(C.byref(b)))
+        self._b = b
+        self.members: list = []
+
+    def _check(self, rc):
+        if rc != _lib.ESP_OK:
+            raise _lib.EspError(rc, _lib.load().esp_md_batch_last_error(self._b).decode())
+
+    def set(self, members, seeds=None) -> None:
+        """members: list of (State, NeighborList or None)."""
+        n = len(members)
+        systems = (_lib.EspSystem * max(n, 1))()
+        vel = (C.c_void_p * max(n, 1))()
+        outs = (_lib.EspOut * max(n, 1))()
+        nls = (C.c_void_p * max(n, 1))()
+        sd = (C.c_uint64 * max(n, 1))()
+        for i, (st, nl) in enumerate(members):
+            systems[i] = st.c_system()
+            vel[i] = st.vel.ptr
+            outs[i] = st.c_out()
+            nls[i] = nl._nl.value if (nl is not None and nl._nl is not None) else None
+            sd[i] = int(seeds[i]) if seeds is not None else 0
+        self._check(_lib.load().esp_md_batch_set(self._b, n, systems, vel, outs, nls, sd))
+        self.members = list(members)
+
+    def integrate_first(self, dt: float, stream=None) -> None:
+        self._check(_lib.load().esp_md_batch_integrate_first(self._b, dt, stream.handle if stream else None))
+
+    def integrate_second(self, dt: float, step: int, gamma: float = 0.0, kT: float = 0.0, stream=None) -> None:
+        self._check(_lib.load().esp_md_batch_integrate_second(self._b, dt, gamma, kT, int(step), stream.handle if stream else None))
+
+    def wca(self, sigma: float = 1.0, epsilon: float = 1.0, stream=None) -> None:
+        self._check(_lib.load().esp_md_batch_wca(self._b, sigma, epsilon, stream.handle if stream else None))
+
+    def update_nlists(self, stream=None) -> list:
+        """Rebuild the lists of members that moved more than buffer / 2; returns the indices
+        rebuilt (their States' list pointers are refreshed)."""
+        n = len(self.members)
+        rebuilt = C.c_uint(0)
+        which = (C.c_ubyte * max(n, 1))()
+        self._check(_lib.load().esp_md_batch_update_nlists(self._b, C.byref(rebuilt), which, stream.handle if stream else None))
+        out = [i for i in range(n) if which[i]]
+        lib = _lib.load()
+        for i in out:
+            st, nl = self.members[i]
+            a, b, c = C.c_void_p(), C.c_void_p(), C.c_void_p()
+            nmax = C.c_uint()
+            _lib.check(lib.esp_nlist_arrays(nl._nl, C.byref(a), C.byref(b), C.byref(c), C.byref(nmax)))
+            st.nlist_ptrs = (a.value, b.value, c.value)
+            nl.nmax = nmax.value
+        return out
+
+    def __del__(self):
+        try:
+            if self._b.value:
+                _lib.load().esp_md_batch_destroy(self._b)
+        except Exception:
+            pass
+
+
+__all__ = ["State", "NeighborList", "WCA", "NVE", "Langevin", "Simulation", "BatchMD", "jittered_lattice", "random_packing"]

@@ -12,8 +12,14 @@ asks for `steps` MD steps (velocity Verlet, WCA + ESP electrostatics).  The serv
   them evaluated by one batched launch per stage (replayed from a captured hipGraph while
   the set of resident requests is unchanged), so requests that arrive while others run join
   the next tick and their work overlaps on the device;
-* records per-step latency (tick wall time, every resident request advancing one step),
-  queueing delay, completed requests and particle-steps.
+* records per-step latency as each request sees it (wall time from its previous step, or its
+  admission, to the end of the tick that advanced it, host work included), queueing delay,
+  completed requests and particle-steps.
+
+Per tick the host issues a fixed number of launches whatever the number of resident
+requests: batched first half-kick and drift, one displacement check for all neighbour lists
+(only lists that need it are rebuilt), the batched ESP forces (hipGraph replay), batched WCA
+and the batched second half-kick.
 
 ESP parameters per request come from the autotuner's model at the force-error target
 (cached per size/density bucket).  The step is deterministic in mixed precision: a request
@@ -33,7 +39,7 @@ import numpy as np
 from . import _lib
 from .batch import Batch
 from .device import synchronize
-from .md import NeighborList, State, WCA
+from .md import BatchMD, NeighborList, State, WCA
 from .solver import Params, Solver
 from . import tune as _tune
 
@@ -86,6 +92,8 @@ class Request:
     final_positions: Optional[np.ndarray] = field(default=None, repr=False)
     final_forces: Optional[np.ndarray] = field(default=None, repr=False)
     prepared: Optional[tuple] = field(default=None, repr=False)
+    admitted_wall: float = 0.0
+    last_step_end: float = 0.0
 
     def setup(self) -> Tuple[np.ndarray, np.ndarray, float]:
         N = self.N - (self.N % 2)
@@ -121,6 +129,10 @@ class Server:
         self.mesh_range = mesh_range
         self.batch = Batch(use_graph=use_graph)       # every resident request, every tick
         self.batch_new = Batch(use_graph=False)       # initial forces of newly admitted requests
+        self.md = BatchMD()                           # integration, WCA and list upkeep of the resident set
+        self._member_ids: List[int] = []
+        self._esp_args = None
+        self.step_latency: List[float] = []
         self.wca = WCA(WCA_SIGMA, 1.0)
         self.pending: List[Tuple[float, int, Request]] = []
         self.active: List[Request] = []
@@ -136,6 +148,11 @@ class Server:
 
     # ---- request lifecycle ----------------------------------------------------------
     def submit(self, req: Request) -> None:
+        """Queue a request.  Its particle data (the client's input) is generated here, so a
+        stream submitted ahead of run() is served without client-side work in the loop."""
+        pos, q, L = req.setup()
+        params, nbar = self._params_for(len(pos), L)
+        req.prepared = (pos, q, L, params, footprint_bytes(len(pos), params, nbar))
         heapq.heappush(self.pending, (req.arrival, req.rid, req))
 
     def _params_for(self, N: int, L: float) -> Tuple[Params, float]:
@@ -143,7 +160,7 @@ class Server:
         key = (int(round(math.log(N) * 8)), int(round(math.log(density) * 16)))
         if key not in self._tune_cache:
             ch = _tune.tune(N, (L, L, L), self.target, self.precision, orders=self.tuner_orders,
-                            mesh_range=self.mesh_range)[0]
+                            mesh_range=self.mesh_range, batched=True)[0]
             self._tune_cache[key] = (ch.params, ch.c, ch.sigma)
         p, c, sigma = self._tune_cache[key]
         # same c and sigma, mesh spacing and cutoff rescaled to this box
@@ -151,9 +168,10 @@ class Server:
         h = L / n
         rc = sigma * c * h / math.pi
         if rc + SKIN > 0.5 * L:
-            ch = _tune.tune(N, (L, L, L), self.target, self.precision, orders=self.tuner_orders, mesh_range=self.mesh_range)[0]
+            ch = _tune.tune(N, (L, L, L), self.target, self.precision, orders=self.tuner_orders, mesh_range=self.mesh_range,
+                            batched=True)[0]
             return ch.params, ch.neighbours
-        params = Params(nx=n, ny=n, nz=n, order=p.order, kappa=c / rc, rcut=rc, precision=self.precision)
+        params = Params(nx=n, ny=n, nz=n, order=p.order, kappa=c / rc, rcut=rc, precision=self.precision, tile=p.tile)
         return params, density * 4.0 / 3.0 * math.pi * (rc + SKIN) ** 3
 
     def _admit(self, now: float) -> bool:
@@ -177,6 +195,7 @@ class Server:
             req.nlist.update(st, max(params.rcut, self.wca.r_cut), force_rebuild=True)
             req.solver = Solver(params)
             req.admitted_at = now
+            req.admitted_wall = time.perf_counter()
             self.active.append(req)
             self.resident_bytes += need
             self.peak_bytes = max(self.peak_bytes, self.resident_bytes)
@@ -204,9 +223,19 @@ class Server:
                 keep.append(r)
         self.active = keep
 
+    def _sync_members(self) -> None:
+        """Re-describe the resident set to the batched MD and ESP launches after it changed."""
+        ids = [r.rid for r in self.active]
+        if ids == self._member_ids:
+            return
+        self._member_ids = ids
+        self.md.set([(r.state, r.nlist) for r in self.active], seeds=[r.seed for r in self.active])
+        self._esp_args = self.batch.prepare([(r.solver, r.state) for r in self.active])
+
     def tick(self, now: float) -> None:
-        """One MD step of every resident request."""
-        lib = self._lib
+        """One MD step of every resident request: batched first half kick + drift, one
+        displacement check (rebuilding only the lists that need it), batched ESP forces
+        (captured graph), batched WCA, batched second half kick."""
         t0 = time.perf_counter()
         new = [r for r in self.active if r.steps_done == 0]
         if new:
@@ -217,19 +246,21 @@ class Server:
                 synchronize()
                 for r in new:
                     r.first_forces = r.state.force.numpy().copy()
-        for r in self.active:
-            st = r.state
-            box = np.ascontiguousarray(st.box[:3])
-            _lib.check(lib.esp_integrate_first(st.N, st.postype.ptr, st.vel.ptr, st.force.ptr, self.dt, _lib.dptr(box), None))
-        for r in self.active:
-            r.nlist.update(r.state, max(r.params.rcut, self.wca.r_cut))
-        self._forces(self.active, self.batch)
-        for r in self.active:
-            st = r.state
-            _lib.check(lib.esp_integrate_second(st.N, st.vel.ptr, st.force.ptr, self.dt, 0.0, 0.0, 0, r.steps_done + 1, None))
-            r.steps_done += 1
+        self._sync_members()
+        self.md.integrate_first(self.dt)
+        for i in self.md.update_nlists():
+            Batch.refresh(self._esp_args, i)
+        self.batch.enqueue_prepared(self._esp_args)
+        self.md.wca(WCA_SIGMA, 1.0)
+        self.md.integrate_second(self.dt, step=len(self.tick_ms) + 1)
         synchronize()
-        self.tick_ms.append((time.perf_counter() - t0) * 1e3)
+        end = time.perf_counter()
+        for r in self.active:
+            r.steps_done += 1
+            # time since this request's previous step finished (or since it was admitted)
+            self.step_latency.append((end - (r.last_step_end if r.last_step_end else r.admitted_wall)) * 1e3)
+            r.last_step_end = end
+        self.tick_ms.append((end - t0) * 1e3)
         self.tick_active.append(len(self.active))
         self.tick_particles.append(sum(r.state.N for r in self.active))
 
@@ -253,10 +284,9 @@ class Server:
         return self.stats(wall)
 
     def stats(self, wall: float) -> dict:
-        lat = []
-        for ms, k in zip(self.tick_ms, self.tick_active):
-            lat.extend([ms] * k)  # every resident request waited this long for its step
-        lat = np.asarray(lat) if lat else np.zeros(1)
+        # per-step latency as a request sees it: the wall time from its previous step (or its
+        # admission) to the end of the tick that advanced it, host work between ticks included
+        lat = np.asarray(self.step_latency) if self.step_latency else np.zeros(1)
         steps = int(sum(self.tick_active))
         psteps = int(sum(self.tick_particles))
         qdelay = [r.admitted_at - r.arrival for r in self.done if r.admitted_at is not None]

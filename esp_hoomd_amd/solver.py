@@ -34,6 +34,7 @@ class Params:
     window_degree: int = 0
     pair_lanes: int = 0
     k_cut: float = 0.0
+    tile: int = 0
 
     def to_c(self) -> _lib.EspParams:
         p = _lib.EspParams()
@@ -45,6 +46,7 @@ class Params:
         p.window_c, p.window_degree = float(self.window_c), int(self.window_degree)
         p.pair_lanes = int(self.pair_lanes)
         p.k_cut = float(self.k_cut)
+        p.tile = int(self.tile)
         return p
 
 

@@ -72,6 +72,8 @@ typedef struct esp_params {
     double k_cut;                  /* spherical reciprocal-space cutoff: the influence function is 0
                                       for |k| > k_cut.  0 = automatic (1.2 kappa for the PSWF split,
                                       none for the Gaussian split); < 0 = none */
+    unsigned int tile;             /* spreading/gathering tile edge in mesh cells: 2, 4, 8 or 16;
+                                      0 = automatic */
 } esp_params;
 
 /* One system: device pointers owned by the caller (HOOMD's ParticleData, NeighborList and
@@ -232,6 +234,25 @@ int esp_integrate_second(unsigned int N, double* vel, const double* force, doubl
                          uint64_t step, esp_stream_t stream);
 /* total kinetic energy and sum of per-particle energies (force.w); synchronises */
 int esp_kinetic_potential(unsigned int N, const double* vel, const double* force, double* ke, double* pe);
+
+/* Batched MD: the operations above for many independent systems, one launch each (no
+ * counterpart in the reference; used by the serving loop).  esp_md_batch_set records the
+ * members: systems[i] (N, postype, box), vel[i], outs[i] (force, virial, pitch), nls[i] (the
+ * system's neighbour list or NULL: no WCA, no rebuilds) and seeds[i] (Langevin stream, or
+ * NULL for 0).  esp_md_batch_update_nlists rebuilds the lists of members that moved more
+ * than r_buff / 2 (one displacement launch for the batch and one host synchronisation);
+ * which[i] (optional, host) tells whose list was rebuilt, so callers refresh the list
+ * pointers they pass elsewhere (esp_nlist_arrays). */
+typedef struct esp_md_batch esp_md_batch;
+int esp_md_batch_create(esp_md_batch** batch);
+int esp_md_batch_destroy(esp_md_batch* batch);
+const char* esp_md_batch_last_error(const esp_md_batch* batch);
+int esp_md_batch_set(esp_md_batch* batch, unsigned int n, const esp_system* systems, double* const* vel, const esp_out* outs,
+                     esp_nlist* const* nls, const uint64_t* seeds);
+int esp_md_batch_integrate_first(esp_md_batch* batch, double dt, esp_stream_t stream);
+int esp_md_batch_integrate_second(esp_md_batch* batch, double dt, double gamma, double kT, uint64_t step, esp_stream_t stream);
+int esp_md_batch_wca(esp_md_batch* batch, double sigma, double epsilon, esp_stream_t stream);
+int esp_md_batch_update_nlists(esp_md_batch* batch, unsigned int* rebuilt, unsigned char* which, esp_stream_t stream);
 
 /* ---- device utilities (plumbing for callers without a device allocator) ---------------- */
 int esp_device_count(int* n);
