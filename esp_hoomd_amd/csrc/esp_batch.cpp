@@ -58,7 +58,7 @@ void EspBatch::build_plan(bool fp64)
     using kernel::BatchItem;
     const unsigned int n = unsigned(m_desc.size());
     const size_t cplx = fp64 ? 16 : 8;
-    std::vector<BatchItem> pair, bin, scan, tiles, fx, fy, fz;
+    std::vector<BatchItem> pack, pair, bin, scan, tiles, fx, fy, fz;
     m_lds_stencil = m_lds_fx = m_lds_fy = m_lds_fz = 0;
     m_lds_pair = 0;
     m_any_mesh = false;
@@ -68,6 +68,8 @@ void EspBatch::build_plan(bool fp64)
         const kernel::BatchSys& d = m_desc[s];
         for (unsigned int c = 0; c < (d.pair.N + per_chunk - 1) / per_chunk; ++c)
             pair.push_back({s, c});
+        for (unsigned int c = 0; c < (d.pair.N + 255u) / 256u; ++c)
+            pack.push_back({s, c});
         if (d.pair.compact_tab)
             m_lds_pair = std::max(m_lds_pair, size_t(d.pair.n_compact) * (d.pair.compact_deg + 1) * (fp64 ? 8 : 4));
         if (!(d.flags & COMPUTE_MESH))
@@ -97,12 +99,13 @@ void EspBatch::build_plan(bool fp64)
             throw EspError(STATUS_UNSUPPORTED, "a batched stage needs more than 64 KiB of LDS");
 
     std::vector<BatchItem> all;
-    all.reserve(pair.size() + bin.size() + scan.size() + tiles.size() + fx.size() + fy.size() + fz.size());
+    all.reserve(pack.size() + pair.size() + bin.size() + scan.size() + tiles.size() + fx.size() + fy.size() + fz.size());
     auto place = [&](const std::vector<BatchItem>& v, size_t& off) {
         off = all.size();
         all.insert(all.end(), v.begin(), v.end());
     };
-    size_t o_pair, o_bin, o_scan, o_tiles, o_fx, o_fy, o_fz;
+    size_t o_pack, o_pair, o_bin, o_scan, o_tiles, o_fx, o_fy, o_fz;
+    place(pack, o_pack);
     place(pair, o_pair);
     place(bin, o_bin);
     place(scan, o_scan);
@@ -123,6 +126,7 @@ void EspBatch::build_plan(bool fp64)
             l.grid = 1;
         return l;
     };
+    m_l.pack = list(o_pack, pack.size(), m_max_grid);
     m_l.pair = list(o_pair, pair.size(), m_max_grid);
     m_l.bin = list(o_bin, bin.size(), m_max_grid);
     m_l.scatter = m_l.bin;
@@ -141,6 +145,7 @@ template<typename R> void EspBatch::enqueue_stages(hipStream_t st)
     // arrays) waits for it
     check_hip(hipEventRecord(m_fork, st), "hipEventRecord");
     check_hip(hipStreamWaitEvent(m_side, m_fork, 0), "hipStreamWaitEvent");
+    check_hip(kernel::batch_pack(d, m_l.pack, m_side), "batch_pack");
     check_hip(kernel::batch_pair<R>(d, m_l.pair, m_lds_pair, m_side), "batch_pair");
     check_hip(hipEventRecord(m_join, m_side), "hipEventRecord");
     if (m_any_mesh)

@@ -39,7 +39,7 @@ class EspBatch
     private:
     struct Lists
         {
-        kernel::BatchList pair, bin, scan, scatter, tiles, fft_x, fft_y, fft_z, fin;
+        kernel::BatchList pack, pair, bin, scan, scatter, tiles, fft_x, fft_y, fft_z, fin;
         };
     void build_plan(bool fp64);
     void enqueue(hipStream_t s);

@@ -92,7 +92,10 @@ struct PairArgs
     double* force;           // Scalar4[N], overwritten
     double* virial;          // [6][pitch], overwritten (may be null)
     size_t virial_pitch;
+    double* pq;              // scratch Scalar4[N] = (x, y, z, q), filled before the pair pass
     };
+//! Pack (x, y, z, q) of every particle into a.pq (one 32-byte record per neighbour read).
+hipError_t pack_positions_charges(const PairArgs& a, hipStream_t s);
 template<typename R> hipError_t pair_forces(const PairArgs& a, int lanes_per_particle, hipStream_t s);
 
 // ---- batched solves: many independent systems per launch ------------------------------
@@ -164,6 +167,7 @@ template<typename R> hipError_t batch_fft_z_conv(const BatchSys* d, BatchList l,
 template<typename R> hipError_t batch_fft_x_c2r(const BatchSys* d, BatchList l, size_t lds, hipStream_t s);
 hipError_t batch_finalize(const BatchSys* d, BatchList l, hipStream_t s);
 template<typename R> hipError_t batch_pair(const BatchSys* d, BatchList l, size_t lds, hipStream_t s);
+hipError_t batch_pack(const BatchSys* d, BatchList l, hipStream_t s);
 size_t tp_lds_bytes(const MeshGeom& g, int P, bool fp64);
 bool batch_order_supported(int P);
 bool stencil_tile_ok(int T, int P, bool fp64, bool batched);

@@ -4,12 +4,96 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <sstream>
+#include <tuple>
 
 #include "esp_kernels.h"
 
 namespace esp
 {
+
+// ---------------------------------------------------------------------------
+// Process-wide cache of the host tables.  In the reduced variable s = r / r_c the split
+// tables, the compact pair table and the influence quadrature depend on the bandwidth
+// c = kappa r_c only (the self and background constants scale as 1/r_c and r_c^2), and the
+// window on (P, D, c_w) only; many solvers of one batch or server share them, and building
+// them (prolate eigenproblem, quadratures, fits) costs tens of milliseconds.
+// ---------------------------------------------------------------------------
+namespace
+{
+struct ReducedSplit
+    {
+    std::vector<double> screen, dscreen, r_lo, dr_inv;
+    double self_rc = 0.0;     // self_coeff * r_c
+    double bg_rc2 = 0.0;      // background_coeff / r_c^2
+    };
+
+std::mutex g_cache_mutex;
+std::map<std::tuple<int, double, unsigned int>, std::shared_ptr<const ReducedSplit>> g_split_cache;
+std::map<std::tuple<int, double, unsigned int, unsigned int>, std::shared_ptr<const std::vector<double>>> g_compact_cache;
+std::map<std::tuple<unsigned int, unsigned int, double>, std::shared_ptr<const math::WindowTable>> g_window_cache;
+std::map<std::tuple<int, double, int>, std::shared_ptr<const std::pair<std::vector<double>, std::vector<double>>>> g_quad_cache;
+
+std::shared_ptr<const ReducedSplit> reduced_split(math::SplitKind kind, double c, unsigned int n_table)
+    {
+    const auto key = std::make_tuple(int(kind), c, n_table);
+    std::lock_guard<std::mutex> lock(g_cache_mutex);
+    auto it = g_split_cache.find(key);
+    if (it != g_split_cache.end())
+        return it->second;
+    auto t = std::make_shared<ReducedSplit>();
+    math::SplitFunction split(kind, c, 1.0);  // r_c = 1: reduced units
+    math::build_split_table(split, n_table, t->screen, t->dscreen, t->r_lo, t->dr_inv);
+    t->self_rc = split.self_coeff();
+    t->bg_rc2 = split.background_coeff();
+    g_split_cache.emplace(key, t);
+    return t;
+    }
+
+std::shared_ptr<const std::vector<double>> compact_split(math::SplitKind kind, double c, unsigned int nseg, unsigned int deg)
+    {
+    const auto key = std::make_tuple(int(kind), c, nseg, deg);
+    std::lock_guard<std::mutex> lock(g_cache_mutex);
+    auto it = g_compact_cache.find(key);
+    if (it != g_compact_cache.end())
+        return it->second;
+    math::SplitFunction split(kind, c, 1.0);
+    auto t = std::make_shared<std::vector<double>>();
+    math::build_compact_split(split, nseg, deg, *t);
+    g_compact_cache.emplace(key, t);
+    return t;
+    }
+
+std::shared_ptr<const math::WindowTable> cached_window(unsigned int P, unsigned int D, double cw)
+    {
+    const auto key = std::make_tuple(P, D, cw);
+    std::lock_guard<std::mutex> lock(g_cache_mutex);
+    auto it = g_window_cache.find(key);
+    if (it != g_window_cache.end())
+        return it->second;
+    auto w = std::make_shared<math::WindowTable>(math::build_window(P, D, cw));
+    g_window_cache.emplace(key, w);
+    return w;
+    }
+
+//! Quadrature (x in [0, 1], weights) with F(k) = sum w cos(k r_c x): depends on c and the node count.
+std::shared_ptr<const std::pair<std::vector<double>, std::vector<double>>> cached_quadrature(math::SplitKind kind, double c, int nq)
+    {
+    const auto key = std::make_tuple(int(kind), c, nq);
+    std::lock_guard<std::mutex> lock(g_cache_mutex);
+    auto it = g_quad_cache.find(key);
+    if (it != g_quad_cache.end())
+        return it->second;
+    math::SplitFunction split(kind, c, 1.0);
+    auto q = std::make_shared<std::pair<std::vector<double>, std::vector<double>>>();
+    split.quadrature(nq, q->first, q->second);
+    g_quad_cache.emplace(key, q);
+    return q;
+    }
+} // namespace
 
 void check_hip(hipError_t e, const char* what)
     {
@@ -252,11 +336,16 @@ void EspSolver::set_split_table(unsigned int n_segs, const double* screen, const
 void EspSolver::build_host_tables()
     {
     const math::SplitKind kind = m_p.split == 1 ? math::SplitKind::GAUSSIAN : math::SplitKind::PSWF;
-    math::SplitFunction split(kind, m_p.kappa, m_p.rcut);
+    const auto red = reduced_split(kind, m_p.kappa * m_p.rcut, m_p.n_table);
     if (!m_external_table)
-        math::build_split_table(split, m_p.n_table, m_screen, m_dscreen, m_r_lo, m_dr_inv);
-    m_self_coeff = split.self_coeff();
-    m_bg_coeff = split.background_coeff();
+        {
+        m_screen = red->screen;
+        m_dscreen = red->dscreen;
+        m_r_lo = red->r_lo;
+        m_dr_inv = red->dr_inv;
+        }
+    m_self_coeff = red->self_rc / m_p.rcut;
+    m_bg_coeff = red->bg_rc2 * m_p.rcut * m_p.rcut;
 
     // window: oversampling sigma = (pi / h) / K, K the effective band limit of F
     const unsigned int P = m_p.order;
@@ -271,7 +360,7 @@ void EspSolver::build_host_tables()
         const double sigma = math::PI / (h * K);
         cw = math::default_window_c(P, sigma);
         }
-    m_window = math::build_window(P, m_D, cw);
+    m_window = *cached_window(P, m_D, cw);
     m_tables_built = true;
     }
 
@@ -280,7 +369,7 @@ size_t EspSolver::memory_bytes() const
     size_t b = 0;
     const DeviceBuffer* bufs[] = {&m_split_tab, &m_wcoef, &m_tw[0], &m_tw[1], &m_tw[2], &m_G, &m_Gv, &m_qxd, &m_qwd, &m_w2[0], &m_w2[1],
                                   &m_w2[2], &m_mesh, &m_spec, &m_phi, &m_tile_count, &m_tile_start, &m_key, &m_slot, &m_rec,
-                                  &m_sidx, &m_qpart, &m_evpart, &m_qsum, &m_ev_internal};
+                                  &m_sidx, &m_qpart, &m_evpart, &m_qsum, &m_ev_internal, &m_pq, &m_compact};
     for (const DeviceBuffer* d : bufs)
         b += d->bytes();
     return b;
@@ -377,13 +466,11 @@ void EspSolver::initialize(const SystemView& sys, hipStream_t s)
     if (!m_external_table)
         {
         const math::SplitKind kind = m_p.split == 1 ? math::SplitKind::GAUSSIAN : math::SplitKind::PSWF;
-        math::SplitFunction split(kind, m_p.kappa, m_p.rcut);
-        std::vector<double> ct;
-        math::build_compact_split(split, COMPACT_SEGS, COMPACT_DEG, ct);
+        const auto ct = compact_split(kind, m_p.kappa * m_p.rcut, COMPACT_SEGS, COMPACT_DEG);
         if (fp64)
-            upload_as<double>(m_compact, ct);
+            upload_as<double>(m_compact, *ct);
         else
-            upload_as<float>(m_compact, ct);
+            upload_as<float>(m_compact, *ct);
         }
     if (fp64)
         upload_as<double>(m_wcoef, m_window.coef);
@@ -464,10 +551,11 @@ void EspSolver::build_influence(hipStream_t s)
     int nq = 0;
     if (kind == math::SplitKind::PSWF)
         {
-        math::SplitFunction split(kind, m_p.kappa, m_p.rcut);
         const double kmax = math::PI * std::sqrt(1.0 / (g.hx * g.hx) + 1.0 / (g.hy * g.hy) + 1.0 / (g.hz * g.hz));
         nq = math::SplitFunction::quadrature_nodes(m_p.kappa * m_p.rcut, kmax * m_p.rcut);
-        split.quadrature(nq, m_qx, m_qw);
+        const auto quad = cached_quadrature(kind, m_p.kappa * m_p.rcut, nq);
+        m_qx = quad->first;
+        m_qw = quad->second;
         upload_as<double>(m_qxd, m_qx);
         upload_as<double>(m_qwd, m_qw);
         }
@@ -519,6 +607,7 @@ kernel::PairArgs EspSolver::pair_args(const SystemView& sys, const OutputView& o
     a.force = out.force;
     a.virial = out.virial;
     a.virial_pitch = out.virial_pitch;
+    a.pq = m_pq.as<double>();
     return a;
     }
 
@@ -526,6 +615,7 @@ template<typename R> void EspSolver::run_pair(const SystemView& sys, const Outpu
     {
     const kernel::PairArgs a = pair_args(sys, out);
     int lanes = m_p.pair_lanes ? m_p.pair_lanes : 8;
+    check_hip(kernel::pack_positions_charges(a, s), "pack_positions_charges");
     check_hip(kernel::pair_forces<R>(a, lanes, s), "pair_forces");
     }
 
@@ -613,6 +703,8 @@ unsigned int EspSolver::prepare(const SystemView& sys, const OutputView& out, hi
         }
     const unsigned int n = sys.group_idx ? sys.group_size : sys.N;
     ensure_particles(std::max(n, 1u));
+    if (m_pq.bytes() < size_t(std::max(sys.N, 1u)) * 32)
+        m_pq.resize(size_t(std::max(sys.N, 1u)) * 32 + m_pq.bytes() / 2);
     return n;
     }
 

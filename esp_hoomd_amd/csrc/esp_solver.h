@@ -222,6 +222,7 @@ class EspSolver
     DeviceBuffer m_phi;         // potential mesh R
     DeviceBuffer m_tile_count, m_tile_start;
     DeviceBuffer m_key, m_slot, m_rec, m_sidx;
+    DeviceBuffer m_pq;          // packed (x, y, z, q) per particle for the pair pass
     DeviceBuffer m_qpart, m_evpart, m_qsum, m_ev_internal;
     unsigned int m_cap_particles = 0;
     };

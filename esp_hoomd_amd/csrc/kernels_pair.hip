@@ -93,14 +93,15 @@ __device__ __forceinline__ void pair_particle(unsigned int i, int sub, const ker
     R f[3] = {R(0), R(0), R(0)};
     R e = R(0);
     R v[6] = {R(0), R(0), R(0), R(0), R(0), R(0)};
-    const double4 pi = reinterpret_cast<const double4*>(a.postype)[i];
-    const double qi = a.charge[i];
+    // packed (x, y, z, q): one 32-byte record per particle
+    const double4* __restrict__ pq = reinterpret_cast<const double4*>(a.pq);
+    const double4 pi = pq[i];
+    const double qi = pi.w;
     if (a.nlist != nullptr && qi != 0.0 && a.rc >= 1.0e-12)
         {
-        const double4* __restrict__ pos = reinterpret_cast<const double4*>(a.postype);
-        auto add = [&](const double4& pj, double qj)
+        auto add = [&](const double4& pj)
             {
-            const R qiqj = R(qi * qj);
+            const R qiqj = R(qi * pj.w);
             const R dx = R(min_image(pi.x - pj.x, a.Lx, invLx));
             const R dy = R(min_image(pi.y - pj.y, a.Ly, invLy));
             const R dz = R(min_image(pi.z - pj.z, a.Lz, invLz));
@@ -135,22 +136,17 @@ __device__ __forceinline__ void pair_particle(unsigned int i, int sub, const ker
                 const bool two = k + G < nn;
                 const unsigned int j0 = a.nlist[head + k];
                 const unsigned int j1 = two ? a.nlist[head + k + G] : j0;
-                const double4 p0 = pos[j0];
-                const double4 p1 = pos[j1];
-                const double q0 = a.charge[j0];
-                const double q1 = a.charge[j1];
-                add(p0, q0);
+                const double4 p0 = pq[j0];
+                const double4 p1 = pq[j1];
+                add(p0);
                 if (two)
-                    add(p1, q1);
+                    add(p1);
                 }
             }
         else
             {
             for (unsigned int k = sub; k < nn; k += G)
-                {
-                const unsigned int j = a.nlist[head + k];
-                add(pos[j], a.charge[j]);
-                }
+                add(pq[a.nlist[head + k]]);
             }
         }
     if (a.n_ex != nullptr && qi != 0.0)
@@ -162,8 +158,8 @@ __device__ __forceinline__ void pair_particle(unsigned int i, int sub, const ker
             const unsigned int j = a.ex_list[size_t(i) * a.ex_pitch + k];
             if (j >= a.N)
                 continue;
-            const double4 pj = reinterpret_cast<const double4*>(a.postype)[j];
-            const R qiqj = R(qi * a.charge[j]);
+            const double4 pj = pq[j];
+            const R qiqj = R(qi * pj.w);
             const R dx = R(min_image(pi.x - pj.x, a.Lx, invLx));
             const R dy = R(min_image(pi.y - pj.y, a.Ly, invLy));
             const R dz = R(min_image(pi.z - pj.z, a.Lz, invLz));
@@ -228,10 +224,10 @@ __device__ __forceinline__ const R* stage_compact_table(const kernel::PairArgs& 
     return t;
     }
 
-//! Single-system pair kernel.  MAP = 1: XCD-aware contiguous ranges -- block b runs on XCD
-//! b % 8, so blocks are renumbered to give each XCD one contiguous eighth of the particles
-//! (its L2 then holds the neighbour positions of one region instead of every region);
-//! MAP = 0: plain grid stride.
+//! Single-system pair kernel.  Block b runs on XCD b % 8.  MAP = 2: each XCD takes one
+//! contiguous eighth of the particles and its blocks sweep it interleaved, so the XCD's L2
+//! holds the neighbour positions of one moving window; MAP = 1: the same eighths, but each
+//! block takes a contiguous sub-range; MAP = 0: plain grid stride.
 template<typename R, int G, bool COMPACT, int U, int MAP>
 __global__ void __launch_bounds__(256) k_pair(kernel::PairArgs a)
     {
@@ -244,7 +240,20 @@ __global__ void __launch_bounds__(256) k_pair(kernel::PairArgs a)
         }
     const int sub = int(threadIdx.x % G);
     const unsigned int per_pass = blockDim.x / G;
-    if constexpr (MAP == 1)
+    if constexpr (MAP == 2)
+        {
+        // each XCD owns one contiguous eighth of the particles, and its blocks sweep that
+        // range together (interleaved), so the XCD's L2 holds one window of neighbours
+        const unsigned int nb = gridDim.x;
+        const unsigned int nx = (nb % 8u == 0u) ? 8u : 1u;
+        const unsigned int x = blockIdx.x % nx, j = blockIdx.x / nx, per_x = nb / nx;
+        const unsigned int span = (a.N + nx - 1u) / nx;
+        const unsigned int p0 = x * span;
+        const unsigned int p1 = min(a.N, p0 + span);
+        for (unsigned int i = p0 + j * per_pass + threadIdx.x / G; i < p1; i += per_x * per_pass)
+            pair_particle<R, G, COMPACT, U>(i, sub, a, tab);
+        }
+    else if constexpr (MAP == 1)
         {
         const unsigned int nb = gridDim.x;
         const unsigned int lb = (nb % 8u == 0u) ? (blockIdx.x % 8u) * (nb / 8u) + blockIdx.x / 8u : blockIdx.x;
@@ -304,19 +313,46 @@ __global__ void __launch_bounds__(256) kb_pair(const kernel::BatchSys* __restric
         }
     }
 
+__device__ __forceinline__ void pack_one(unsigned int i, const kernel::PairArgs& a)
+    {
+    const double4 p = reinterpret_cast<const double4*>(a.postype)[i];
+    reinterpret_cast<double4*>(a.pq)[i] = make_double4(p.x, p.y, p.z, a.charge[i]);
+    }
+
+__global__ void __launch_bounds__(256) k_pack_pq(kernel::PairArgs a)
+    {
+    const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < a.N)
+        pack_one(i, a);
+    }
+
+//! Batched packing: items are (system, chunk of 256 particles).
+__global__ void __launch_bounds__(256) kb_pack_pq(const kernel::BatchSys* __restrict__ ds, kernel::BatchList l)
+    {
+    for (unsigned int it = blockIdx.x; it < l.n; it += gridDim.x)
+        {
+        const kernel::BatchItem w = l.items[it];
+        const kernel::PairArgs& a = ds[w.sys].pair;
+        const unsigned int i = w.local * 256u + threadIdx.x;
+        if (i < a.N)
+            pack_one(i, a);
+        }
+    }
+
     } // namespace dev
 
 namespace kernel
 {
-//! Pair-kernel variant (environment ESP_PAIR_VARIANT, default 3): bit 0 = XCD-aware
-//! contiguous ranges, bit 1 = two neighbours in flight per lane.
+//! Pair-kernel variant (environment ESP_PAIR_VARIANT, default 5): 0/1/4 one neighbour in
+//! flight per lane, 2/3/5 two; particle mapping 0/2 plain grid stride, 1/3 XCD-contiguous
+//! block ranges, 4/5 XCD-contiguous ranges swept by interleaved blocks.
 static int pair_variant()
     {
     static int v = -1;
     if (v < 0)
         {
         const char* e = getenv("ESP_PAIR_VARIANT");
-        v = e ? (atoi(e) & 3) : 3;
+        v = e ? atoi(e) : 5;
         }
     return v;
     }
@@ -328,7 +364,9 @@ template<typename R, int G, bool C> static void launch_pair_v(const PairArgs& a,
         case 0: hipLaunchKernelGGL((dev::k_pair<R, G, C, 1, 0>), dim3(nb), dim3(256), lds, s, a); break;
         case 1: hipLaunchKernelGGL((dev::k_pair<R, G, C, 1, 1>), dim3(nb), dim3(256), lds, s, a); break;
         case 2: hipLaunchKernelGGL((dev::k_pair<R, G, C, 2, 0>), dim3(nb), dim3(256), lds, s, a); break;
-        default: hipLaunchKernelGGL((dev::k_pair<R, G, C, 2, 1>), dim3(nb), dim3(256), lds, s, a); break;
+        case 3: hipLaunchKernelGGL((dev::k_pair<R, G, C, 2, 1>), dim3(nb), dim3(256), lds, s, a); break;
+        case 4: hipLaunchKernelGGL((dev::k_pair<R, G, C, 1, 2>), dim3(nb), dim3(256), lds, s, a); break;
+        default: hipLaunchKernelGGL((dev::k_pair<R, G, C, 2, 2>), dim3(nb), dim3(256), lds, s, a); break;
         }
     }
 
@@ -338,6 +376,22 @@ template<typename R, int G> static void launch_pair(const PairArgs& a, unsigned 
         launch_pair_v<R, G, true>(a, nb, size_t(a.n_compact) * (a.compact_deg + 1) * sizeof(R), s);
     else
         launch_pair_v<R, G, false>(a, nb, 0, s);
+    }
+
+hipError_t pack_positions_charges(const PairArgs& a, hipStream_t s)
+    {
+    if (a.N == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(dev::k_pack_pq, dim3((a.N + 255u) / 256u), dim3(256), 0, s, a);
+    return hipGetLastError();
+    }
+
+hipError_t batch_pack(const BatchSys* d, BatchList l, hipStream_t s)
+    {
+    if (l.n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(dev::kb_pack_pq, dim3(l.grid), dim3(256), 0, s, d, l);
+    return hipGetLastError();
     }
 
 template<typename R> hipError_t pair_forces(const PairArgs& a, int lanes_per_particle, hipStream_t s)
