@@ -92,7 +92,7 @@ def stage_kernel(stage, precision):
 
 
 def workload_key(N, params, precision):
-    return f"N{N}_mesh{params.nx}_P{params.order}_rc{params.rcut:.4g}_{precision}"
+    return f"N{N}_mesh{params.nx}_P{params.order}_{precision}"
 
 
 def measured_traffic(key, kernel):
@@ -230,7 +230,7 @@ def batched_leg(n_sys, steps, warmup, seed=6):
     from esp_hoomd_amd.device import Event, Stream, synchronize
     from esp_hoomd_amd.md import NeighborList, State
     from esp_hoomd_amd.serve import lattice_positions, neutral_charges
-    from esp_hoomd_amd.solver import Solver
+    from esp_hoomd_amd.solver import Params, Solver
 
     rng = np.random.default_rng(seed)
     Ns = (np.exp(rng.uniform(np.log(500), np.log(50_000), n_sys)).astype(int) // 2 * 2)
@@ -271,16 +271,23 @@ def batched_leg(n_sys, steps, warmup, seed=6):
     e1.record(stream)
     stream.synchronize()
     ms = e0.elapsed_ms(e1) / steps
-    # the same systems one after another through the single-system path
-    for s, st in pairs[:4]:
-        s.enqueue_state(st, stream=stream)
-    stream.synchronize()
-    e0.record(stream)
-    for s, st in pairs:
-        s.enqueue_state(st, stream=stream)
-    e1.record(stream)
-    stream.synchronize()
-    seq_ms = e0.elapsed_ms(e1)
+
+    def sequential(precision):
+        """The same systems one after another through the plain single-system path (same
+        ESP parameters, the library's automatic tile for single systems)."""
+        solos = [Solver(Params(**{**ch.params.__dict__, "tile": 0, "precision": precision})) for ch in choices]
+        for s, (_, st) in list(zip(solos, pairs))[:4]:
+            s.enqueue_state(st, stream=stream)
+        stream.synchronize()
+        e0.record(stream)
+        for s, (_, st) in zip(solos, pairs):
+            s.enqueue_state(st, stream=stream)
+        e1.record(stream)
+        stream.synchronize()
+        return e0.elapsed_ms(e1)
+
+    seq_mixed_ms = sequential(_lib.ESP_PREC_MIXED)
+    seq_fp64_ms = sequential(_lib.ESP_PREC_FP64)
     # measured errors on a sample of systems (batched forces against the tight reference)
     batch.enqueue(pairs, stream=stream)
     stream.synchronize()
@@ -299,8 +306,10 @@ def batched_leg(n_sys, steps, warmup, seed=6):
         "ms_per_batch_step": ms,
         "solves_per_s": n_sys / (ms * 1e-3),
         "particle_steps_per_s": total_N / (ms * 1e-3),
-        "sequential_ms": seq_ms,
-        "batching_speedup": seq_ms / ms,
+        "sequential_single_mixed_ms": seq_mixed_ms,
+        "sequential_single_fp64_ms": seq_fp64_ms,
+        "batching_speedup_vs_single_mixed": seq_mixed_ms / ms,
+        "speedup_vs_single_fp64": seq_fp64_ms / ms,
         "error_sample_max": float(max(errs)),
         "error_sample_median": float(np.median(errs)),
         "meshes": [meshes[0], meshes[-1]],
