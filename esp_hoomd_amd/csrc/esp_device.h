@@ -246,6 +246,20 @@ __device__ __forceinline__ void lds_add(int* p, float v)
     {
     __hip_atomic_fetch_add(p, __float2int_rn(v * float(FIXED_SCALE)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
+//! Scale applied to a value before it enters the LDS accumulator (fixed point for FP32).
+template<typename R> __device__ __forceinline__ constexpr R acc_scale()
+    {
+    return sizeof(R) == 4 ? R(FIXED_SCALE) : R(1);
+    }
+//! LDS accumulation of a value already multiplied by acc_scale<R>().
+__device__ __forceinline__ void lds_add_scaled(double* p, double v)
+    {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+__device__ __forceinline__ void lds_add_scaled(int* p, float v)
+    {
+    __hip_atomic_fetch_add(p, __float2int_rn(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
 __device__ __forceinline__ void glob_add(double* p, double v)
     {
     unsafeAtomicAdd(p, v);

@@ -114,6 +114,45 @@ __device__ __forceinline__ void tile_coords(const MeshGeom& g, unsigned int tile
     tx = int(tile) - tz * txy - ty * g.tx;
     }
 
+//! Walk the tile's (B^3)-cell LDS box with a block stride, calling fn(c, gx, gy, gz) with the
+//! box index c and the periodic mesh coordinates of the cell.  The start and the stride are
+//! split into (x, y, z) once and advanced with carries, so the loop has no integer division.
+template<typename Fn>
+__device__ __forceinline__ void for_box_cells(const MeshGeom& g, int tx, int ty, int tz, Fn fn)
+    {
+    const int B = g.B;
+    const int B2 = B * B;
+    const int B3 = B2 * B;
+    const int t0 = int(threadIdx.x), step = int(blockDim.x);
+    int lz = t0 / B2, ly = (t0 - lz * B2) / B, lx = t0 - lz * B2 - ly * B;
+    const int sz = step / B2, sy = (step - sz * B2) / B, sx = step - sz * B2 - sy * B;
+    const int x0 = tx * g.T, y0 = ty * g.T, z0 = tz * g.T;
+    for (int c = t0; c < B3; c += step)
+        {
+        int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+        while (gx >= g.nx)
+            gx -= g.nx;
+        while (gy >= g.ny)
+            gy -= g.ny;
+        while (gz >= g.nz)
+            gz -= g.nz;
+        fn(c, gx, gy, gz);
+        lx += sx;
+        ly += sy;
+        lz += sz;
+        if (lx >= B)
+            {
+            lx -= B;
+            ++ly;
+            }
+        if (ly >= B)
+            {
+            ly -= B;
+            ++lz;
+            }
+        }
+    }
+
 // ---------------------------------------------------------------------------
 // Spread
 // ---------------------------------------------------------------------------
@@ -256,16 +295,7 @@ __global__ void __launch_bounds__(256) k_gather(MeshGeom g,
 
     int tx, ty, tz;
     tile_coords(g, tile, tx, ty, tz);
-    for (int c = threadIdx.x; c < B3; c += blockDim.x)
-        {
-        const int lz = c / B2;
-        const int ly = (c - lz * B2) / B;
-        const int lx = c - lz * B2 - ly * B;
-        const int gx = (tx * g.T + lx) % g.nx;
-        const int gy = (ty * g.T + ly) % g.ny;
-        const int gz = (tz * g.T + lz) % g.nz;
-        box[c] = phi[(size_t(gz) * g.ny + gy) * g.nx + gx];
-        }
+    for_box_cells(g, tx, ty, tz, [&](int c, int gx, int gy, int gz) { box[c] = phi[(size_t(gz) * g.ny + gy) * g.nx + gx]; });
     if (threadIdx.x == 0)
         box[B3] = R(0);
     for (int c = threadIdx.x; c < P * (D + 1); c += blockDim.x)
@@ -422,6 +452,10 @@ __device__ __forceinline__ void spread_tile_tp(unsigned int tile,
         eval_taps<R, P>(wc, D, r.tx, wx, nullptr);
         eval_taps<R, P>(wc, D, r.tz, wz, nullptr);
         L* base = box + (oz * B + oy) * B + ox;
+        // fold the accumulator's fixed-point scale into the x weights once per particle
+#pragma unroll
+        for (int a = 0; a < P; ++a)
+            wx[a] *= acc_scale<R>();
         for (int b = 0; b < P; ++b)
             {
             R dwy;
@@ -433,7 +467,7 @@ __device__ __forceinline__ void spread_tile_tp(unsigned int tile,
                 const R v = qy * wz[c];
 #pragma unroll
                 for (int a = 0; a < P; ++a)
-                    lds_add(row + c * B2 + a, v * wx[a]);
+                    lds_add_scaled(row + c * B2 + a, v * wx[a]);
                 }
             }
         }
@@ -441,20 +475,13 @@ __device__ __forceinline__ void spread_tile_tp(unsigned int tile,
 
     int tx, ty, tz;
     tile_coords(g, tile, tx, ty, tz);
-    for (int c = threadIdx.x; c < B3; c += blockDim.x)
-        {
-        const L v = box[c];
-        if (v != L(0))
-            {
-            const int lz = c / B2;
-            const int ly = (c - lz * B2) / B;
-            const int lx = c - lz * B2 - ly * B;
-            const int gx = (tx * g.T + lx) % g.nx;
-            const int gy = (ty * g.T + ly) % g.ny;
-            const int gz = (tz * g.T + lz) % g.nz;
-            glob_add(&mesh[(size_t(gz) * g.ny + gy) * g.nx + gx], v);
-            }
-        }
+    for_box_cells(g, tx, ty, tz,
+                  [&](int c, int gx, int gy, int gz)
+                      {
+                      const L v = box[c];
+                      if (v != L(0))
+                          glob_add(&mesh[(size_t(gz) * g.ny + gy) * g.nx + gx], v);
+                      });
     }
 
 template<typename R, int P>
@@ -491,16 +518,7 @@ __device__ __forceinline__ void gather_tile_tp(unsigned int tile,
     R* wc = reinterpret_cast<R*>(smem + ((size_t(B3 + 1) * sizeof(R) + 15) & ~size_t(15)));
     int tx, ty, tz;
     tile_coords(g, tile, tx, ty, tz);
-    for (int c = threadIdx.x; c < B3; c += blockDim.x)
-        {
-        const int lz = c / B2;
-        const int ly = (c - lz * B2) / B;
-        const int lx = c - lz * B2 - ly * B;
-        const int gx = (tx * g.T + lx) % g.nx;
-        const int gy = (ty * g.T + ly) % g.ny;
-        const int gz = (tz * g.T + lz) % g.nz;
-        box[c] = phi[(size_t(gz) * g.ny + gy) * g.nx + gx];
-        }
+    for_box_cells(g, tx, ty, tz, [&](int c, int gx, int gy, int gz) { box[c] = phi[(size_t(gz) * g.ny + gy) * g.nx + gx]; });
     if (threadIdx.x == 0)
         box[B3] = R(0);
     for (int c = threadIdx.x; c < P * (D + 1); c += blockDim.x)
