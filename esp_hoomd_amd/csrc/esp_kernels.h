@@ -71,6 +71,8 @@ hipError_t finalize(int n_ev, const double* ev_part, int n_q, const double* qpar
                     double V, int want_ev, double* out, double* qsum, hipStream_t s);
 
 // ---- short-range pair correction (EvaluatorPairPSWF over a full neighbour list) ----
+//! Degree of the compact per-segment polynomial of S(s) the pair kernels evaluate.
+constexpr unsigned int PAIR_COMPACT_DEG = 11;
 struct PairArgs
     {
     unsigned int N;
@@ -96,7 +98,8 @@ struct PairArgs
     };
 //! Pack (x, y, z, q) of every particle into a.pq (one 32-byte record per neighbour read).
 hipError_t pack_positions_charges(const PairArgs& a, hipStream_t s);
-template<typename R> hipError_t pair_forces(const PairArgs& a, int lanes_per_particle, hipStream_t s);
+//! max_blocks caps the persistent grid (0 = the default of 2048 blocks).
+template<typename R> hipError_t pair_forces(const PairArgs& a, int lanes_per_particle, unsigned int max_blocks, hipStream_t s);
 
 // ---- batched solves: many independent systems per launch ------------------------------
 //! Everything one system contributes to a batched launch: its geometry and plans, the

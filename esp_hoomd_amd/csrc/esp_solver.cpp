@@ -232,6 +232,20 @@ void EspSolver::mark(int b, hipStream_t s)
         m_ev_cur = nullptr;
     }
 
+//! Environment ESP_PAIR_FORK (default 0): the pair branch is launched on the side stream at the
+//! start of mesh-chain stage k (1 binning, 3 scatter, 4 spread, 5 forward x FFT, ... 10 gather);
+//! 0 launches it before the chain.
+int EspSolver::fork_stage()
+    {
+    static int v = -1;
+    if (v < 0)
+        {
+        const char* e = getenv("ESP_PAIR_FORK");
+        v = e ? std::min(std::max(atoi(e), 0), 10) : 0;
+        }
+    return v;
+    }
+
 //! Record the start (end = false) or the end of the pair stage on the stream it runs on.
 void EspSolver::pair_mark(bool end, hipStream_t s)
     {
@@ -611,50 +625,80 @@ kernel::PairArgs EspSolver::pair_args(const SystemView& sys, const OutputView& o
     return a;
     }
 
-template<typename R> void EspSolver::run_pair(const SystemView& sys, const OutputView& out, hipStream_t s)
+template<typename R> void EspSolver::run_pair(const SystemView& sys, const OutputView& out, hipStream_t s, bool concurrent)
     {
     const kernel::PairArgs a = pair_args(sys, out);
     int lanes = m_p.pair_lanes ? m_p.pair_lanes : 8;
+    // Beside the FP32 mesh chain the pass is held to 2 blocks (8 waves) per CU, which leaves
+    // the spreading and FFT kernels room to run alongside it (measured at 1e6 particles:
+    // 1.34 ms per step against 1.54 ms with the full grid).  The FP64 chain, which needs
+    // more registers per wave, gains nothing from the cap.
+    const unsigned int cap = (concurrent && sizeof(R) == 4) ? 2u * cu_count() : 0u;
     check_hip(kernel::pack_positions_charges(a, s), "pack_positions_charges");
-    check_hip(kernel::pair_forces<R>(a, lanes, s), "pair_forces");
+    check_hip(kernel::pair_forces<R>(a, lanes, cap, s), "pair_forces");
+    }
+
+unsigned int cu_count()
+    {
+    static std::mutex mu;
+    static std::map<int, unsigned int> counts;
+    int dev = 0;
+    check_hip(hipGetDevice(&dev), "hipGetDevice");
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = counts.find(dev);
+    if (it != counts.end())
+        return it->second;
+    int n = 0;
+    check_hip(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev), "hipDeviceGetAttribute");
+    const unsigned int c = unsigned(std::max(n, 1));
+    counts.emplace(dev, c);
+    return c;
     }
 
 template<typename R>
-void EspSolver::run_mesh(const SystemView& sys, const OutputView& out, unsigned int n, hipStream_t s, bool want_ev, hipEvent_t join)
+void EspSolver::run_mesh(const SystemView& sys, const OutputView& out, unsigned int n, hipStream_t s, bool want_ev, hipEvent_t join,
+                         const std::function<void()>& fork)
     {
     const MeshGeom& g = m_geom;
+    // stage boundary k: profiling mark, then the pair branch forks off if this is its stage
+    auto mark_fork = [&](int k)
+    {
+        mark(k, s);
+        if (fork && k == fork_stage())
+            fork();
+    };
     const int P = int(m_p.order);
     const int D = int(m_D);
     const unsigned int nqb = (n + 255) / 256;
-    mark(1, s);
+    mark_fork(1);
     check_hip(kernel::bin_particles(n, sys.group_idx, sys.postype, sys.charge, g, P, m_tile_count.as<unsigned int>(),
                                     m_key.as<unsigned int>(), m_slot.as<unsigned int>(), m_qpart.as<double>(), s),
               "bin_particles");
-    mark(2, s);
+    mark_fork(2);
     check_hip(kernel::scan_tiles(m_n_tiles, m_tile_count.as<unsigned int>(), m_tile_start.as<unsigned int>(), s), "scan_tiles");
-    mark(3, s);
+    mark_fork(3);
     check_hip(kernel::scatter_particles<R>(n, sys.group_idx, sys.postype, sys.charge, g, P, m_key.as<unsigned int>(),
                                            m_slot.as<unsigned int>(), m_tile_start.as<unsigned int>(), m_rec.get(),
                                            m_sidx.as<SortedIdx>(), s),
               "scatter_particles");
-    mark(4, s);
+    mark_fork(4);
     check_hip(kernel::spread<R>(g, P, D, m_n_tiles, m_tile_start.as<unsigned int>(), m_rec.get(), m_sidx.as<SortedIdx>(), m_wcoef.get(),
                                 m_mesh.get(), s),
               "spread");
-    mark(5, s);
+    mark_fork(5);
     check_hip(kernel::fft_forward_x<R>(g, m_plan[0], m_tw[0].get(), m_mesh.get(), m_spec.get(), s), "fft_forward_x");
-    mark(6, s);
+    mark_fork(6);
     check_hip(kernel::fft_y<R>(g, m_plan[1], m_tw[1].get(), m_spec.get(), -1, s), "fft_y");
-    mark(7, s);
+    mark_fork(7);
     check_hip(kernel::fft_z_convolve<R>(g, m_plan[2], m_tw[2].get(), m_spec.get(), m_G.as<double>(), m_Gv.as<double>(), want_ev ? 1 : 0,
                                         m_evpart.as<double>(), s),
               "fft_z_convolve");
-    mark(8, s);
+    mark_fork(8);
     check_hip(kernel::fft_y<R>(g, m_plan[1], m_tw[1].get(), m_spec.get(), +1, s), "fft_y inverse");
-    mark(9, s);
+    mark_fork(9);
     const double V = g.Lx * g.Ly * g.Lz;
     check_hip(kernel::fft_inverse_x<R>(g, m_plan[0], m_tw[0].get(), m_spec.get(), 1.0 / V, m_phi.get(), s), "fft_inverse_x");
-    mark(10, s);
+    mark_fork(10);
     if (join)
         check_hip(hipStreamWaitEvent(s, join, 0), "hipStreamWaitEvent");
     check_hip(kernel::gather<R>(g, P, D, m_n_tiles, m_tile_start.as<unsigned int>(), m_rec.get(), m_sidx.as<SortedIdx>(), m_wcoef.get(),
@@ -714,33 +758,45 @@ void EspSolver::compute(const SystemView& sys, const OutputView& out, hipStream_
     const bool fp64 = m_p.precision == 0;
     const unsigned int flags = out.flags ? out.flags : COMPUTE_ALL;
     mark(0, s);
-    // the pair correction runs on a side stream, overlapping binning, spreading and the FFTs;
-    // the gather (which adds into the same force array) waits for it
+    // the pair correction runs on a side stream, forked at stage boundary fork_stage() of the
+    // mesh chain (0: before binning) so that it overlaps the rest of the chain; the gather
+    // (which adds into the same force array) waits for it
+    const bool mesh = (flags & COMPUTE_MESH) != 0;
+    const bool side = (flags & COMPUTE_PAIR) && mesh && m_overlap;
     hipEvent_t join = nullptr;
-    if (flags & COMPUTE_PAIR)
+    if (side && !m_side)
         {
+        check_hip(hipStreamCreateWithFlags(&m_side, hipStreamNonBlocking), "hipStreamCreate");
+        check_hip(hipEventCreateWithFlags(&m_ev_fork, hipEventDisableTiming), "hipEventCreate");
+        check_hip(hipEventCreateWithFlags(&m_ev_join, hipEventDisableTiming), "hipEventCreate");
+        }
+    if (side)
+        join = m_ev_join;
+    auto launch_pair = [&]()
+    {
         hipStream_t ps = s;
-        if ((flags & COMPUTE_MESH) && m_overlap)
+        if (side)
             {
-            if (!m_side)
-                {
-                check_hip(hipStreamCreateWithFlags(&m_side, hipStreamNonBlocking), "hipStreamCreate");
-                check_hip(hipEventCreateWithFlags(&m_ev_fork, hipEventDisableTiming), "hipEventCreate");
-                check_hip(hipEventCreateWithFlags(&m_ev_join, hipEventDisableTiming), "hipEventCreate");
-                }
             check_hip(hipEventRecord(m_ev_fork, s), "hipEventRecord");
             check_hip(hipStreamWaitEvent(m_side, m_ev_fork, 0), "hipStreamWaitEvent");
             ps = m_side;
-            join = m_ev_join;
             }
         pair_mark(false, ps);
         if (fp64)
-            run_pair<double>(sys, out, ps);
+            run_pair<double>(sys, out, ps, side);
         else
-            run_pair<float>(sys, out, ps);
+            run_pair<float>(sys, out, ps, side);
         pair_mark(true, ps);
-        if (join)
-            check_hip(hipEventRecord(join, ps), "hipEventRecord");
+        if (side)
+            check_hip(hipEventRecord(m_ev_join, ps), "hipEventRecord");
+    };
+    std::function<void()> late_fork;
+    if (flags & COMPUTE_PAIR)
+        {
+        if (side && fork_stage() > 0)
+            late_fork = launch_pair;
+        else
+            launch_pair();
         }
     else
         {
@@ -749,13 +805,13 @@ void EspSolver::compute(const SystemView& sys, const OutputView& out, hipStream_
             for (int c = 0; c < 6; ++c)
                 check_hip(hipMemsetAsync(out.virial + size_t(c) * out.virial_pitch, 0, size_t(sys.N) * 8, s), "hipMemsetAsync virial");
         }
-    if (flags & COMPUTE_MESH)
+    if (mesh)
         {
         const bool want_ev = (flags & COMPUTE_ENERGY_VIRIAL) != 0;
         if (fp64)
-            run_mesh<double>(sys, out, n, s, want_ev, join);
+            run_mesh<double>(sys, out, n, s, want_ev, join, late_fork);
         else
-            run_mesh<float>(sys, out, n, s, want_ev, join);
+            run_mesh<float>(sys, out, n, s, want_ev, join, late_fork);
         }
     else
         for (int b = 1; b <= 11; ++b)

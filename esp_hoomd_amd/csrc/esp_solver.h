@@ -10,6 +10,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -44,6 +45,8 @@ enum Status
     };
 
 void check_hip(hipError_t e, const char* what);
+//! Compute units of the current device (cached per device).
+unsigned int cu_count();
 
 //! RAII device allocation.
 class DeviceBuffer
@@ -189,8 +192,12 @@ class EspSolver
     void build_influence(hipStream_t s);
     void ensure_particles(unsigned int n);
     template<typename R>
-    void run_mesh(const SystemView& sys, const OutputView& out, unsigned int n, hipStream_t s, bool want_ev, hipEvent_t join);
-    template<typename R> void run_pair(const SystemView& sys, const OutputView& out, hipStream_t s);
+    void run_mesh(const SystemView& sys, const OutputView& out, unsigned int n, hipStream_t s, bool want_ev, hipEvent_t join,
+                  const std::function<void()>& fork);
+    //! Mesh-chain stage (1..10) at whose start the pair branch forks; 0 = before the chain.
+    static int fork_stage();
+    //! concurrent: the pass runs on the side stream beside the mesh chain.
+    template<typename R> void run_pair(const SystemView& sys, const OutputView& out, hipStream_t s, bool concurrent);
 
     SolverParams m_p;
     bool m_need_init = true;
@@ -210,7 +217,7 @@ class EspSolver
 
     // device
     static constexpr unsigned int COMPACT_SEGS = 64;
-    static constexpr unsigned int COMPACT_DEG = 11;
+    static constexpr unsigned int COMPACT_DEG = kernel::PAIR_COMPACT_DEG;
     DeviceBuffer m_split_tab;   // [n_segs][12] R
     DeviceBuffer m_compact;     // [COMPACT_SEGS][COMPACT_DEG + 1] R
     DeviceBuffer m_wcoef;       // [P][D+1] R

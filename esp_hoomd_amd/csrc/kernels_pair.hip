@@ -28,23 +28,62 @@ __device__ __forceinline__ double min_image(double d, double L, double invL)
     return d - L * rint(d * invL);
     }
 
-//! S and dS/ds from the compact table: segment = floor(s * NSEG), z = 2 t - 1.
+//! S and dS/ds from the compact table: segment = floor(s * NSEG), z = 2 t - 1.  The degree is
+//! fixed at compile time so the row is fetched with 16-byte LDS loads and Horner is unrolled.
 template<typename R>
-__device__ __forceinline__ void compact_split(const R* __restrict__ tab, int nseg, int deg, R s, R& S, R& dS)
+__device__ __forceinline__ void compact_split(const R* __restrict__ tab, int nseg, R s, R& S, R& dS)
     {
+    constexpr int NC = kernel::PAIR_COMPACT_DEG + 1;
+    constexpr int PER16 = 16 / int(sizeof(R));
+    static_assert(NC % PER16 == 0, "compact rows must be whole 16-byte words");
     const R u = s * R(nseg);
     int seg = int(u);
     seg = seg < 0 ? 0 : (seg >= nseg ? nseg - 1 : seg);
     const R z = R(2) * (u - R(seg)) - R(1);
-    const R* c = tab + seg * (deg + 1);
-    R v = c[deg], d = R(0);
-    for (int j = deg - 1; j >= 0; --j)
+    R c[NC];
+    const uint4* row = reinterpret_cast<const uint4*>(tab + seg * NC);
+#pragma unroll
+    for (int w = 0; w < NC / PER16; ++w)
+        {
+        const uint4 x = row[w];
+        const unsigned int bits[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+        for (int k = 0; k < PER16; ++k)
+            {
+            if constexpr (sizeof(R) == 4)
+                c[w * PER16 + k] = __uint_as_float(bits[k]);
+            else
+                c[w * PER16 + k] = __hiloint2double(int(bits[2 * k + 1]), int(bits[2 * k]));
+            }
+        }
+    R v = c[NC - 1], d = R(0);
+#pragma unroll
+    for (int j = NC - 2; j >= 0; --j)
         {
         d = d * z + v;
         v = v * z + c[j];
         }
     S = v;
     dS = d * R(2 * nseg);  // dS/ds = dS/dz * dz/ds
+    }
+
+//! r and 1/r from r^2: exact square root and division in FP64; the hardware reciprocal square
+//! root (1 ulp) in FP32.
+template<typename R>
+__device__ __forceinline__ void r_and_rinv(R rsq, R& r, R& rinv)
+    {
+    if constexpr (sizeof(R) == 4)
+        {
+        // v_rsq_f32 without the denormal-input scaling; a zero distance gives r = 0 (which the
+        // r < 1e-12 test then rejects), not 0 * inf
+        rinv = __builtin_amdgcn_rsqf(fmaxf(rsq, 1.0e-30f));
+        r = rsq * rinv;
+        }
+    else
+        {
+        r = sqrt(rsq);
+        rinv = R(1) / r;
+        }
     }
 
 //! L(r) and (-dL/dr) / r for 0 < r <= rc (zero outside), given rinv = 1 / r.
@@ -67,7 +106,7 @@ __device__ __forceinline__ void pair_terms(const R* __restrict__ tab,
             return;
             }
         R S, dS;
-        compact_split<R>(tab, int(a.n_compact), int(a.compact_deg), r * inv_rc, S, dS);
+        compact_split<R>(tab, int(a.n_compact), r * inv_rc, S, dS);
         const R inv4pir = R(0.079577471545947673) * rinv;
         L = S * inv4pir;
         ndr = (S * rinv - dS * inv_rc) * inv4pir * rinv;
@@ -97,6 +136,21 @@ __device__ __forceinline__ void pair_particle(unsigned int i, int sub, const ker
     const double4* __restrict__ pq = reinterpret_cast<const double4*>(a.pq);
     const double4 pi = pq[i];
     const double qi = pi.w;
+    // force += fdivr * d; energy and virial are accumulated doubled and halved at the end
+    auto accumulate = [&](R fdivr, R dE, R dx, R dy, R dz)
+    {
+        const R tx = fdivr * dx, ty = fdivr * dy, tz = fdivr * dz;
+        f[0] += tx;
+        f[1] += ty;
+        f[2] += tz;
+        e += dE;
+        v[0] += tx * dx;
+        v[1] += tx * dy;
+        v[2] += tx * dz;
+        v[3] += ty * dy;
+        v[4] += ty * dz;
+        v[5] += tz * dz;
+    };
     if (a.nlist != nullptr && qi != 0.0 && a.rc >= 1.0e-12)
         {
         auto add = [&](const double4& pj)
@@ -108,39 +162,37 @@ __device__ __forceinline__ void pair_particle(unsigned int i, int sub, const ker
             const R rsq = dx * dx + dy * dy + dz * dz;
             if (rsq >= rc2 || qiqj == R(0))
                 return;
-            const R r = sqrt(rsq);
-            const R rinv = R(1) / r;
+            R r, rinv;
+            r_and_rinv<R>(rsq, r, rinv);
             R Lr, ndr;
             pair_terms<R, COMPACT>(tab, a, r, rinv, rc, inv_rc, Lr, ndr);
-            const R fdivr = qiqj * ndr;
-            f[0] += fdivr * dx;
-            f[1] += fdivr * dy;
-            f[2] += fdivr * dz;
-            e += R(0.5) * qiqj * Lr;
-            const R hf = R(0.5) * fdivr;
-            v[0] += hf * dx * dx;
-            v[1] += hf * dx * dy;
-            v[2] += hf * dx * dz;
-            v[3] += hf * dy * dy;
-            v[4] += hf * dy * dz;
-            v[5] += hf * dz * dz;
+            accumulate(qiqj * ndr, qiqj * Lr, dx, dy, dz);
             };
         const unsigned int nn = a.n_neigh[i];
         const uint64_t head = a.head_list[i];
         if constexpr (U == 2)
             {
-            // two neighbours in flight per lane; they are accumulated in list order, so the
-            // sums are the same as one at a time
-            for (unsigned int k = sub; k < nn; k += 2 * G)
+            // two neighbours in flight per lane, and the list entries of the next pair are
+            // fetched before this pair is evaluated.  Entries past the end are read at the
+            // clamped index nn - 1 (in bounds) and not evaluated.  Neighbours are accumulated
+            // in list order, so the sums are the same as one at a time.
+            if (unsigned(sub) < nn)
                 {
-                const bool two = k + G < nn;
-                const unsigned int j0 = a.nlist[head + k];
-                const unsigned int j1 = two ? a.nlist[head + k + G] : j0;
-                const double4 p0 = pq[j0];
-                const double4 p1 = pq[j1];
-                add(p0);
-                if (two)
-                    add(p1);
+                const unsigned int last = nn - 1u;
+                const unsigned int* __restrict__ nl = a.nlist + head;
+                unsigned int j0 = nl[sub];
+                unsigned int j1 = nl[min(unsigned(sub) + unsigned(G), last)];
+                for (unsigned int k = sub; k < nn; k += 2 * G)
+                    {
+                    const double4 p0 = pq[j0];
+                    const double4 p1 = pq[j1];
+                    const unsigned int kn = k + 2 * G;
+                    j0 = nl[min(kn, last)];
+                    j1 = nl[min(kn + G, last)];
+                    add(p0);
+                    if (k + G < nn)
+                        add(p1);
+                    }
                 }
             }
         else
@@ -166,27 +218,19 @@ __device__ __forceinline__ void pair_particle(unsigned int i, int sub, const ker
             const R rsq = dx * dx + dy * dy + dz * dz;
             if (rsq < R(1.0e-20))
                 continue;
-            const R r = sqrt(rsq);
-            const R rinv = R(1) / r;
+            R r, rinv;
+            r_and_rinv<R>(rsq, r, rinv);
             R Lr = R(0), ndr = R(0);
             if (r <= rc)
                 pair_terms<R, COMPACT>(tab, a, r, rinv, rc, inv_rc, Lr, ndr);
             const R inv4pir = R(0.079577471545947673) * rinv;
-            const R dE = qiqj * (Lr - inv4pir);
-            const R fdivr = qiqj * (ndr - inv4pir * rinv * rinv);
-            f[0] += fdivr * dx;
-            f[1] += fdivr * dy;
-            f[2] += fdivr * dz;
-            e += R(0.5) * dE;
-            const R hf = R(0.5) * fdivr;
-            v[0] += hf * dx * dx;
-            v[1] += hf * dx * dy;
-            v[2] += hf * dx * dz;
-            v[3] += hf * dy * dy;
-            v[4] += hf * dy * dz;
-            v[5] += hf * dz * dz;
+            accumulate(qiqj * (ndr - inv4pir * rinv * rinv), qiqj * (Lr - inv4pir), dx, dy, dz);
             }
         }
+    e *= R(0.5);
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+        v[c] *= R(0.5);
     // reduce over the G lanes of this particle (every lane of the group takes part)
 #pragma unroll
     for (int c = 0; c < 3; ++c)
@@ -394,19 +438,22 @@ hipError_t batch_pack(const BatchSys* d, BatchList l, hipStream_t s)
     return hipGetLastError();
     }
 
-template<typename R> hipError_t pair_forces(const PairArgs& a, int lanes_per_particle, hipStream_t s)
+template<typename R> hipError_t pair_forces(const PairArgs& a, int lanes_per_particle, unsigned int max_blocks, hipStream_t s)
     {
     if (a.N == 0)
         return hipSuccess;
     const size_t threads = size_t(a.N) * lanes_per_particle;
-    // persistent grid: at most 8 blocks per CU of a 256-CU device
-    static int max_blocks = -1;
-    if (max_blocks < 0)
+    // persistent grid: by default at most 8 blocks per CU of a 256-CU device; the caller caps
+    // it lower when the pass shares the GPU with the mesh chain (environment
+    // ESP_PAIR_MAX_BLOCKS overrides both)
+    static int env_blocks = -2;
+    if (env_blocks == -2)
         {
         const char* e = getenv("ESP_PAIR_MAX_BLOCKS");
-        max_blocks = e ? std::max(8, atoi(e)) : 2048;
+        env_blocks = e ? std::max(8, atoi(e)) : -1;
         }
-    unsigned int nb = unsigned(std::min<size_t>((threads + 255) / 256, size_t(max_blocks)));
+    const size_t cap = env_blocks > 0 ? size_t(env_blocks) : (max_blocks ? size_t(max_blocks) : size_t(2048));
+    unsigned int nb = unsigned(std::min<size_t>((threads + 255) / 256, cap));
     if (nb >= 8)
         nb &= ~7u;  // a multiple of the 8 XCDs for the XCD-aware mapping
     switch (lanes_per_particle)
@@ -421,8 +468,8 @@ template<typename R> hipError_t pair_forces(const PairArgs& a, int lanes_per_par
         }
     return hipGetLastError();
     }
-template hipError_t pair_forces<double>(const PairArgs&, int, hipStream_t);
-template hipError_t pair_forces<float>(const PairArgs&, int, hipStream_t);
+template hipError_t pair_forces<double>(const PairArgs&, int, unsigned int, hipStream_t);
+template hipError_t pair_forces<float>(const PairArgs&, int, unsigned int, hipStream_t);
 
 template<typename R> hipError_t batch_pair(const BatchSys* d, BatchList l, size_t lds, hipStream_t s)
     {
