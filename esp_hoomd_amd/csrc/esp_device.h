@@ -238,13 +238,21 @@ template<> struct Acc<float>
     using glob_t = unsigned long long;
     };
 
+//! Nearest integer (ties up) in one instruction, v_cvt_rpi_i32_f32 = floor(v + 0.5); the
+//! accumulated values stay far inside the int32 range.
+__device__ __forceinline__ int round_to_int(float v)
+    {
+    int r;
+    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(v));
+    return r;
+    }
 __device__ __forceinline__ void lds_add(double* p, double v)
     {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
 __device__ __forceinline__ void lds_add(int* p, float v)
     {
-    __hip_atomic_fetch_add(p, __float2int_rn(v * float(FIXED_SCALE)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(p, round_to_int(v * float(FIXED_SCALE)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
 //! Scale applied to a value before it enters the LDS accumulator (fixed point for FP32).
 template<typename R> __device__ __forceinline__ constexpr R acc_scale()
@@ -258,7 +266,7 @@ __device__ __forceinline__ void lds_add_scaled(double* p, double v)
     }
 __device__ __forceinline__ void lds_add_scaled(int* p, float v)
     {
-    __hip_atomic_fetch_add(p, __float2int_rn(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(p, round_to_int(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
 __device__ __forceinline__ void glob_add(double* p, double v)
     {

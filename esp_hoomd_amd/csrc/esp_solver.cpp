@@ -635,7 +635,10 @@ template<typename R> void EspSolver::run_pair(const SystemView& sys, const Outpu
     // more registers per wave, gains nothing from the cap.
     const unsigned int cap = (concurrent && sizeof(R) == 4) ? 2u * cu_count() : 0u;
     check_hip(kernel::pack_positions_charges(a, s), "pack_positions_charges");
+    // the "pair" stage of the per-stage timing spans the pair kernel alone
+    pair_mark(false, s);
     check_hip(kernel::pair_forces<R>(a, lanes, cap, s), "pair_forces");
+    pair_mark(true, s);
     }
 
 unsigned int cu_count()
@@ -781,12 +784,10 @@ void EspSolver::compute(const SystemView& sys, const OutputView& out, hipStream_
             check_hip(hipStreamWaitEvent(m_side, m_ev_fork, 0), "hipStreamWaitEvent");
             ps = m_side;
             }
-        pair_mark(false, ps);
         if (fp64)
             run_pair<double>(sys, out, ps, side);
         else
             run_pair<float>(sys, out, ps, side);
-        pair_mark(true, ps);
         if (side)
             check_hip(hipEventRecord(m_ev_join, ps), "hipEventRecord");
     };

@@ -381,40 +381,38 @@ __global__ void __launch_bounds__(256) k_gather(MeshGeom g,
 // weights (and derivatives) in registers, evaluates the y weight per row and walks
 // the P^2 rows of P contiguous LDS cells.  No cross-lane traffic at all.
 // ---------------------------------------------------------------------------
-template<typename R, int P>
-__device__ __forceinline__ void eval_taps(const R* __restrict__ wc, int D, R t, R* w, R* dw)
+//! All P taps of the window at fractional offset t (z = 2 t - 1), and with DERIV their
+//! derivatives dw/dt: one Horner recurrence over the degree with the P taps side by side, so
+//! the loop over the (run-time) degree carries P independent FMAs per step.  The [P][D+1]
+//! coefficients are read with lane-uniform addresses.  Per tap the operations and their
+//! order are those of a single-tap Horner evaluation.
+template<typename R, int P, bool DERIV>
+__device__ __forceinline__ void eval_taps(const R* __restrict__ wc, int D, R t, R (&w)[P], R (&dw)[P])
     {
     const R z = R(2) * t - R(1);
+    const int stride = D + 1;
 #pragma unroll
     for (int k = 0; k < P; ++k)
         {
-        const R* c = wc + k * (D + 1);
-        R v = c[D], d = R(0);
-        for (int j = D - 1; j >= 0; --j)
-            {
-            if (dw)
-                d = d * z + v;
-            v = v * z + c[j];
-            }
-        w[k] = v;
-        if (dw)
-            dw[k] = R(2) * d;
+        w[k] = wc[k * stride + D];
+        dw[k] = R(0);
         }
-    }
-
-template<typename R>
-__device__ __forceinline__ R eval_tap(const R* __restrict__ wc, int D, int k, R t, R& dw)
-    {
-    const R z = R(2) * t - R(1);
-    const R* c = wc + k * (D + 1);
-    R v = c[D], d = R(0);
     for (int j = D - 1; j >= 0; --j)
         {
-        d = d * z + v;
-        v = v * z + c[j];
+#pragma unroll
+        for (int k = 0; k < P; ++k)
+            {
+            if constexpr (DERIV)
+                dw[k] = dw[k] * z + w[k];
+            w[k] = w[k] * z + wc[k * stride + j];
+            }
         }
-    dw = R(2) * d;
-    return v;
+    if constexpr (DERIV)
+        {
+#pragma unroll
+        for (int k = 0; k < P; ++k)
+            dw[k] = R(2) * dw[k];
+        }
     }
 
 template<typename R, int P>
@@ -436,11 +434,8 @@ __device__ __forceinline__ void spread_tile_tp(unsigned int tile,
     const int B2 = B * B;
     const int B3 = B2 * B;
     L* box = reinterpret_cast<L*>(smem);
-    R* wc = reinterpret_cast<R*>(smem + ((size_t(B3 + 1) * sizeof(L) + 15) & ~size_t(15)));
     for (int c = threadIdx.x; c <= B3; c += blockDim.x)
         box[c] = L(0);
-    for (int c = threadIdx.x; c < P * (D + 1); c += blockDim.x)
-        wc[c] = wcoef[c];
     __syncthreads();
 
     for (unsigned int p = s0 + threadIdx.x; p < s1; p += blockDim.x)
@@ -448,18 +443,19 @@ __device__ __forceinline__ void spread_tile_tp(unsigned int tile,
         const SortedRec<R> r = rec[p];
         const unsigned int loc = sidx[p].local;
         const int ox = loc & 0xff, oy = (loc >> 8) & 0xff, oz = loc >> 16;
-        R wx[P], wz[P];
-        eval_taps<R, P>(wc, D, r.tx, wx, nullptr);
-        eval_taps<R, P>(wc, D, r.tz, wz, nullptr);
+        R wx[P], wy[P], wz[P], unused[P];
+        eval_taps<R, P, false>(wcoef, D, r.tx, wx, unused);
+        eval_taps<R, P, false>(wcoef, D, r.ty, wy, unused);
+        eval_taps<R, P, false>(wcoef, D, r.tz, wz, unused);
         L* base = box + (oz * B + oy) * B + ox;
         // fold the accumulator's fixed-point scale into the x weights once per particle
 #pragma unroll
         for (int a = 0; a < P; ++a)
             wx[a] *= acc_scale<R>();
+#pragma unroll
         for (int b = 0; b < P; ++b)
             {
-            R dwy;
-            const R qy = r.q * eval_tap<R>(wc, D, b, r.ty, dwy);
+            const R qy = r.q * wy[b];
             L* row = base + b * B;
 #pragma unroll
             for (int c = 0; c < P; ++c)
@@ -515,14 +511,11 @@ __device__ __forceinline__ void gather_tile_tp(unsigned int tile,
     const int B2 = B * B;
     const int B3 = B2 * B;
     R* box = reinterpret_cast<R*>(smem);
-    R* wc = reinterpret_cast<R*>(smem + ((size_t(B3 + 1) * sizeof(R) + 15) & ~size_t(15)));
     int tx, ty, tz;
     tile_coords(g, tile, tx, ty, tz);
     for_box_cells(g, tx, ty, tz, [&](int c, int gx, int gy, int gz) { box[c] = phi[(size_t(gz) * g.ny + gy) * g.nx + gx]; });
     if (threadIdx.x == 0)
         box[B3] = R(0);
-    for (int c = threadIdx.x; c < P * (D + 1); c += blockDim.x)
-        wc[c] = wcoef[c];
     __syncthreads();
 
     const R inv_hx = R(1.0 / g.hx), inv_hy = R(1.0 / g.hy), inv_hz = R(1.0 / g.hz);
@@ -531,15 +524,15 @@ __device__ __forceinline__ void gather_tile_tp(unsigned int tile,
         const SortedRec<R> r = rec[p];
         const SortedIdx id = sidx[p];
         const int ox = id.local & 0xff, oy = (id.local >> 8) & 0xff, oz = id.local >> 16;
-        R wx[P], dwx[P], wz[P], dwz[P];
-        eval_taps<R, P>(wc, D, r.tx, wx, dwx);
-        eval_taps<R, P>(wc, D, r.tz, wz, dwz);
+        R wx[P], dwx[P], wy[P], dwy[P], wz[P], dwz[P];
+        eval_taps<R, P, true>(wcoef, D, r.tx, wx, dwx);
+        eval_taps<R, P, true>(wcoef, D, r.ty, wy, dwy);
+        eval_taps<R, P, true>(wcoef, D, r.tz, wz, dwz);
         const R* base = box + (oz * B + oy) * B + ox;
         R ex = R(0), ey = R(0), ez = R(0);
+#pragma unroll
         for (int b = 0; b < P; ++b)
             {
-            R dwy;
-            const R wy = eval_tap<R>(wc, D, b, r.ty, dwy);
             const R* row = base + b * B;
             R Sx = R(0), Sdx = R(0), Sdz = R(0);
 #pragma unroll
@@ -557,9 +550,9 @@ __device__ __forceinline__ void gather_tile_tp(unsigned int tile,
                 Sdx += wz[c] * sdx;
                 Sdz += dwz[c] * sx;
                 }
-            ex += wy * Sdx;
-            ey += dwy * Sx;
-            ez += wy * Sdz;
+            ex += wy[b] * Sdx;
+            ey += dwy[b] * Sx;
+            ez += wy[b] * Sdz;
             }
         const double q = double(r.q);
         double* f = reinterpret_cast<double*>(&force[id.idx]);
