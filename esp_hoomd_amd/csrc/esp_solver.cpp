@@ -417,6 +417,7 @@ void EspSolver::initialize(const SystemView& sys, hipStream_t s)
     g.ny = int(m_p.ny);
     g.nz = int(m_p.nz);
     g.nxh = g.nx / 2 + 1;
+    g.sxp = (g.nxh + 15) & ~15;
     // tile edge: the requested one, else the largest of {8, 4, 2} whose kernels fit their LDS
     // box, dropping to 4 on small meshes (fewer than 256 tiles of 8) for parallelism
     auto tiles_of = [&](int t) { return size_t((g.nx + t - 1) / t) * ((g.ny + t - 1) / t) * ((g.nz + t - 1) / t); };
@@ -510,7 +511,7 @@ void EspSolver::initialize(const SystemView& sys, hipStream_t s)
 
     // device buffers
     const size_t M = size_t(g.nx) * g.ny * g.nz;
-    const size_t Mh = size_t(g.nxh) * g.ny * g.nz;
+    const size_t Mh = size_t(g.sxp) * g.ny * g.nz;  // padded rows
     m_mesh.resize(M * 8, true);
     m_spec.resize(Mh * 2 * (fp64 ? 8 : 4));
     m_phi.resize(M * (fp64 ? 8 : 4));

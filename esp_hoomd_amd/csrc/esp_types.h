@@ -23,6 +23,8 @@ struct MeshGeom
     {
     int nx, ny, nz;     // mesh points per axis
     int nxh;            // nx / 2 + 1 (half spectrum along x)
+    int sxp;            // row pitch of the half spectrum and the influence tables: nxh rounded
+                        // up to 16 elements, so 16-column row segments are 128-byte aligned
     int T;              // spreading tile edge (cells)
     int tx, ty, tz;     // tiles per axis
     int B;              // LDS box edge T + P - 1

@@ -255,9 +255,9 @@ __device__ __forceinline__ void fft_x_r2c_block(int blk,
         const C2<R> x0 = {R(0.5) * (z.x + zc.x), R(0.5) * (z.y - zc.y)};
         const C2<R> x1 = {R(0.5) * (z.y + zc.y), R(-0.5) * (z.x - zc.x)};
         if (l0 < nlines)
-            spec[size_t(l0) * nxh + k] = x0;
+            spec[size_t(l0) * g.sxp + k] = x0;
         if (l1 < nlines)
-            spec[size_t(l1) * nxh + k] = x1;
+            spec[size_t(l1) * g.sxp + k] = x1;
         }
     }
 
@@ -302,9 +302,9 @@ __device__ __forceinline__ void fft_x_c2r_block(int blk,
         const int l1 = l0 + 1;
         C2<R> x0 = {R(0), R(0)}, x1 = {R(0), R(0)};
         if (l0 < nlines)
-            x0 = spec[size_t(l0) * nxh + k];
+            x0 = spec[size_t(l0) * g.sxp + k];
         if (l1 < nlines)
-            x1 = spec[size_t(l1) * nxh + k];
+            x1 = spec[size_t(l1) * g.sxp + k];
         // Z[k] = X0[k] + i X1[k]; Z[n-k] = conj(X0[k]) + i conj(X1[k])
         a[pr * ls + k] = C2<R> {x0.x - x1.y, x0.y + x1.x};
         if (k != 0 && k != nx - k)
@@ -366,7 +366,7 @@ __device__ __forceinline__ void fft_y_block(int blk,
         const int y = idx / ncol;
         const int xi = idx - y * ncol;
         if (xi < nc)
-            a[xi * ls + y] = spec[(size_t(z) * ny + y) * nxh + x0 + xi];
+            a[xi * ls + y] = spec[(size_t(z) * ny + y) * g.sxp + x0 + xi];
         }
     __syncthreads();
     C2<R>* res = lds_fft<R>(a, b, tw, plan, ls, nc, sign, threadIdx.x, blockDim.x);
@@ -375,7 +375,7 @@ __device__ __forceinline__ void fft_y_block(int blk,
         const int y = idx / ncol;
         const int xi = idx - y * ncol;
         if (xi < nc)
-            spec[(size_t(z) * ny + y) * nxh + x0 + xi] = res[xi * ls + y];
+            spec[(size_t(z) * ny + y) * g.sxp + x0 + xi] = res[xi * ls + y];
         }
     }
 
@@ -419,7 +419,7 @@ __device__ __forceinline__ void fft_z_conv_block(int blk,
         const int z = idx / ncol;
         const int xi = idx - z * ncol;
         if (xi < nc)
-            a[xi * ls + z] = spec[(size_t(z) * ny + y) * nxh + x0 + xi];
+            a[xi * ls + z] = spec[(size_t(z) * ny + y) * g.sxp + x0 + xi];
         }
     __syncthreads();
     C2<R>* res = lds_fft<R>(a, b, tw, plan, ls, nc, -1, threadIdx.x, blockDim.x);
@@ -435,7 +435,7 @@ __device__ __forceinline__ void fft_z_conv_block(int blk,
         if (xi >= nc)
             continue;
         const int x = x0 + xi;
-        const size_t m = (size_t(z) * ny + y) * nxh + x;
+        const size_t m = (size_t(z) * ny + y) * g.sxp + x;
         const double gk = G[m];
         C2<R> c = res[xi * ls + z];
         if (want_ev)
@@ -481,7 +481,7 @@ __device__ __forceinline__ void fft_z_conv_block(int blk,
         const int z = idx / ncol;
         const int xi = idx - z * ncol;
         if (xi < nc)
-            spec[(size_t(z) * ny + y) * nxh + x0 + xi] = res2[xi * ls + z];
+            spec[(size_t(z) * ny + y) * g.sxp + x0 + xi] = res2[xi * ls + z];
         }
     if (want_ev)
         {
@@ -547,6 +547,7 @@ __global__ void __launch_bounds__(256) k_build_influence(MeshGeom g,
     const int x = int(m % g.nxh);
     const int y = int((m / g.nxh) % g.ny);
     const int z = int(m / (size_t(g.nxh) * g.ny));
+    const size_t out = (size_t(z) * g.ny + y) * g.sxp + x;  // padded row layout
     const int ky_i = (y <= g.ny / 2) ? y : y - g.ny;
     const int kz_i = (z <= g.nz / 2) ? z : z - g.nz;
     const double two_pi = 6.283185307179586476925286766559;
@@ -554,8 +555,8 @@ __global__ void __launch_bounds__(256) k_build_influence(MeshGeom g,
     const double k2 = kx * kx + ky * ky + kz * kz;
     if (k2 == 0.0 || (k_cut > 0.0 && k2 > k_cut * k_cut))
         {
-        G[m] = 0.0;
-        Gv[m] = 0.0;
+        G[out] = 0.0;
+        Gv[out] = 0.0;
         return;
         }
     const double k = sqrt(k2);
@@ -576,8 +577,8 @@ __global__ void __launch_bounds__(256) k_build_influence(MeshGeom g,
             }
         }
     const double wp = w2x[x] * w2y[y] * w2z[z];
-    G[m] = F / (k2 * wp);
-    Gv[m] = (dF / (k2 * k) - 2.0 * F / (k2 * k2)) / wp;
+    G[out] = F / (k2 * wp);
+    Gv[out] = (dF / (k2 * k) - 2.0 * F / (k2 * k2)) / wp;
     }
 
 //! Final FP64 reduction: out[0] = E_rec + E_self + E_bg, out[1..6] = W_rec + background;
