@@ -15,11 +15,13 @@ band limit by the mesh (h = mesh spacing):
 * the default reciprocal-space cutoff |k| <= 1.2 kappa raises the method error by up to
   ~1.3x (measured on the same systems), so the first two terms carry a factor CUT_FACTOR.
 
-Cost model.  Per-step time on one MI355X, from the per-stage timings of the 1M-particle
-workload: the pair kernel costs `pair_ps` per full-list neighbour entry, binning costs
-`bin_ns` per particle, spread + gather cost `stencil_ps * P^3` per particle, and the FFT
-passes plus tile halos cost `cell_ps` per mesh cell; a step pays `overhead_us` of launch and
-reduction latency.  The tuner picks, over FFT-friendly meshes and the allowed orders, the
+Cost model.  Per-step time on one MI355X, fitted to mesh-only, pair-only and full steps of
+the 1M-particle workload at P = 8 and 10 (meshes 162^3-216^3, r_c 5.4-7.2): the pair pass
+alone costs `pair_ps` per full-list neighbour entry; the mesh chain costs `bin_ns` per
+particle for binning, `stencil_ps * P^3` per particle for spread + gather and `cell_ps` per
+mesh cell for the FFT passes and tile halos.  The pair pass runs beside the mesh chain, which
+hides a share `overlap` of the shorter of the two; a step also pays `overhead_us` of launch
+and reduction latency.  The fit is within about 10% on those steps.  The tuner picks, over FFT-friendly meshes and the allowed orders, the
 smallest r_c that meets the target (minimising sigma * c at fixed h) and then the cheapest
 candidate.  `tune_measured` times the best few candidates on the GPU instead of trusting
 the model, and `verify` measures the chosen configuration's error on a proxy system
@@ -92,17 +94,32 @@ def predicted_error(c: float, sigma: float, P: int, precision: int = _lib.ESP_PR
 @dataclass
 class CostModel:
     """Per-step cost coefficients on one MI355X (see the module docstring)."""
-    pair_ps: Tuple[float, float] = (10.0, 6.5)        # (FP64, mixed) per neighbour entry
-    bin_ns: Tuple[float, float] = (0.18, 0.15)        # per particle (bin + scan + scatter)
-    stencil_ps: Tuple[float, float] = (1.17, 0.62)    # per particle per P^3 (spread + gather)
-    cell_ps: Tuple[float, float] = (90.0, 70.0)       # per mesh cell (FFT passes + halos)
+    pair_ps: Tuple[float, float] = (6.1, 4.85)        # (FP64, mixed) per neighbour entry, pair pass alone
+    bin_ns: Tuple[float, float] = (0.15, 0.11)        # per particle (bin + scan + scatter)
+    stencil_ps: Tuple[float, float] = (0.74, 0.56)    # per particle per P^3 (spread + gather)
+    cell_ps: Tuple[float, float] = (94.0, 65.0)       # per mesh cell (FFT passes + tile halos)
+    overlap: Tuple[float, float] = (0.3, 0.55)        # share of the shorter branch hidden by the other
     overhead_us: float = 25.0
+
+    def branch_ms(self, N: int, n_cells: int, P: int, neighbours: float, precision: int) -> Tuple[float, float]:
+        """(mesh chain, pair pass) each run alone, in ms."""
+        k = 0 if precision == _lib.ESP_PREC_FP64 else 1
+        mesh = (N * P ** 3 * self.stencil_ps[k] + n_cells * self.cell_ps[k] + N * self.bin_ns[k] * 1e3) * 1e-9
+        pair = N * neighbours * self.pair_ps[k] * 1e-9
+        return mesh, pair
 
     def step_ms(self, N: int, n_cells: int, P: int, neighbours: float, precision: int) -> float:
         k = 0 if precision == _lib.ESP_PREC_FP64 else 1
-        ps = (N * neighbours * self.pair_ps[k] + N * P ** 3 * self.stencil_ps[k] + n_cells * self.cell_ps[k]
-              + N * self.bin_ns[k] * 1e3)
-        return ps * 1e-9 + self.overhead_us * 1e-3
+        mesh, pair = self.branch_ms(N, n_cells, P, neighbours, precision)
+        return mesh + pair - self.overlap[k] * min(mesh, pair) + self.overhead_us * 1e-3
+
+
+#: Systems solved inside a batch (many small systems, pair and mesh items of all systems sharing
+#: the GPU) are tuned with an additive model that gives the pair pass no overlap credit: on the
+#: 512-system batch of 500-50,000 particles its choices measured 13.0 ms per batch step against
+#: 15.4 ms for the single-system fit above.
+BATCHED_COST = CostModel(pair_ps=(10.0, 6.5), bin_ns=(0.18, 0.15), stencil_ps=(1.17, 0.62), cell_ps=(90.0, 70.0),
+                         overlap=(0.0, 0.0))
 
 
 @dataclass
@@ -186,7 +203,7 @@ def tune(N: int, box: Sequence[float], target: float = 1e-5, precision: int = _l
     """Candidate configurations meeting `target / safety` by the accuracy model, cheapest first
     by the cost model (the first `top` are returned).  `batched` picks the tile edge for
     solving inside a batch (batch_tile); otherwise the library's automatic tile is used."""
-    cost = cost or CostModel()
+    cost = cost or (BATCHED_COST if batched else CostModel())
     L = np.asarray(box[:3], float)
     rho = N / float(np.prod(L))
     goal = target / safety
