@@ -66,7 +66,7 @@ def kernel_bytes(stage, N, n_charged, n_pairs, M, Mh, R, want_ev=True):
     counted once; neighbour-position gathers excluded)."""
     rec = 4 * R + 8
     return {
-        "pair": N * (32 + 8 + 32 + 48 + 12) + 4 * n_pairs,
+        "pair": N * (4 * R + 8 + 32 + 48 + 12) + 4 * n_pairs,  # own record: 32 B FP64, 16 B mixed
         "bin": N * (32 + 8 + 8),
         "scan": 0,
         "scatter": N * (32 + 8 + 8) + n_charged * rec,

@@ -96,8 +96,9 @@ struct PairArgs
     size_t virial_pitch;
     double* pq;              // scratch Scalar4[N] = (x, y, z, q), filled before the pair pass
     };
-//! Pack (x, y, z, q) of every particle into a.pq (one 32-byte record per neighbour read).
-hipError_t pack_positions_charges(const PairArgs& a, hipStream_t s);
+//! Pack every particle's neighbour record into a.pq: (x, y, z, q) doubles for FP64, 32-bit
+//! fixed-point box fractions and a float charge (16 bytes) for FP32.
+template<typename R> hipError_t pack_positions_charges(const PairArgs& a, hipStream_t s);
 //! max_blocks caps the persistent grid (0 = the default of 2048 blocks).
 template<typename R> hipError_t pair_forces(const PairArgs& a, int lanes_per_particle, unsigned int max_blocks, hipStream_t s);
 
@@ -170,7 +171,7 @@ template<typename R> hipError_t batch_fft_z_conv(const BatchSys* d, BatchList l,
 template<typename R> hipError_t batch_fft_x_c2r(const BatchSys* d, BatchList l, size_t lds, hipStream_t s);
 hipError_t batch_finalize(const BatchSys* d, BatchList l, hipStream_t s);
 template<typename R> hipError_t batch_pair(const BatchSys* d, BatchList l, size_t lds, hipStream_t s);
-hipError_t batch_pack(const BatchSys* d, BatchList l, hipStream_t s);
+template<typename R> hipError_t batch_pack(const BatchSys* d, BatchList l, hipStream_t s);
 size_t tp_lds_bytes(const MeshGeom& g, int P, bool fp64);
 bool batch_order_supported(int P);
 bool stencil_tile_ok(int T, int P, bool fp64, bool batched);

@@ -635,7 +635,7 @@ template<typename R> void EspSolver::run_pair(const SystemView& sys, const Outpu
     // 1.34 ms per step against 1.54 ms with the full grid).  The FP64 chain, which needs
     // more registers per wave, gains nothing from the cap.
     const unsigned int cap = (concurrent && sizeof(R) == 4) ? 2u * cu_count() : 0u;
-    check_hip(kernel::pack_positions_charges(a, s), "pack_positions_charges");
+    check_hip(kernel::pack_positions_charges<R>(a, s), "pack_positions_charges");
     // the "pair" stage of the per-stage timing spans the pair kernel alone
     pair_mark(false, s);
     check_hip(kernel::pair_forces<R>(a, lanes, cap, s), "pair_forces");

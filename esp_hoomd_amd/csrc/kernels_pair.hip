@@ -28,6 +28,65 @@ __device__ __forceinline__ double min_image(double d, double L, double invL)
     return d - L * rint(d * invL);
     }
 
+// Neighbour records of the pair pass, written by the pack kernel.
+//  * FP64: (x, y, z, q) as doubles, 32 bytes; minimum image by rounding.
+//  * FP32 (mixed): x, y, z as 32-bit fixed-point fractions of the box edge and q as a float,
+//    16 bytes, so each neighbour costs one 16-byte load.  The difference of two fractions
+//    taken modulo 2^32 and read as a signed integer is the minimum-image separation in units
+//    of L / 2^32 (5e-8 at L = 215, far below FP32 resolution of the separation itself).
+template<typename R> struct PairRec;
+template<> struct PairRec<double>
+    {
+    using type = double4;
+    };
+template<> struct PairRec<float>
+    {
+    using type = uint4;
+    };
+
+//! Fixed-point fraction of the periodic box edge, x in box coordinates centred on the origin.
+__device__ __forceinline__ unsigned int box_fraction(double x, double invL)
+    {
+    double u = x * invL + 0.5;
+    u -= floor(u);
+    return static_cast<unsigned int>(static_cast<unsigned long long>(u * 4294967296.0));
+    }
+
+template<typename R> __device__ __forceinline__ double rec_charge(const typename PairRec<R>::type& p)
+    {
+    if constexpr (sizeof(R) == 8)
+        return p.w;
+    else
+        return double(__uint_as_float(p.w));
+    }
+
+//! Minimum-image separation r_i - r_j of two records and the charge product.
+template<typename R>
+__device__ __forceinline__ void rec_separation(const typename PairRec<R>::type& pi,
+                                               const typename PairRec<R>::type& pj,
+                                               const kernel::PairArgs& a,
+                                               R& dx,
+                                               R& dy,
+                                               R& dz,
+                                               R& qiqj)
+    {
+    if constexpr (sizeof(R) == 8)
+        {
+        dx = min_image(pi.x - pj.x, a.Lx, 1.0 / a.Lx);
+        dy = min_image(pi.y - pj.y, a.Ly, 1.0 / a.Ly);
+        dz = min_image(pi.z - pj.z, a.Lz, 1.0 / a.Lz);
+        qiqj = pi.w * pj.w;
+        }
+    else
+        {
+        constexpr double TWO_M32 = 2.3283064365386962890625e-10;  // 2^-32
+        dx = float(int(pi.x - pj.x)) * float(a.Lx * TWO_M32);
+        dy = float(int(pi.y - pj.y)) * float(a.Ly * TWO_M32);
+        dz = float(int(pi.z - pj.z)) * float(a.Lz * TWO_M32);
+        qiqj = __uint_as_float(pi.w) * __uint_as_float(pj.w);
+        }
+    }
+
 //! S and dS/ds from the compact table: segment = floor(s * NSEG), z = 2 t - 1.  The degree is
 //! fixed at compile time so the row is fetched with 16-byte LDS loads and Horner is unrolled.
 template<typename R>
@@ -125,17 +184,16 @@ __device__ __forceinline__ void pair_terms(const R* __restrict__ tab,
 template<typename R, int G, bool COMPACT, int U = 2>
 __device__ __forceinline__ void pair_particle(unsigned int i, int sub, const kernel::PairArgs& a, const R* __restrict__ tab)
     {
-    const double invLx = 1.0 / a.Lx, invLy = 1.0 / a.Ly, invLz = 1.0 / a.Lz;
     const R rc = R(a.rc);
     const R inv_rc = R(1.0 / a.rc);
     const R rc2 = rc * rc;
     R f[3] = {R(0), R(0), R(0)};
     R e = R(0);
     R v[6] = {R(0), R(0), R(0), R(0), R(0), R(0)};
-    // packed (x, y, z, q): one 32-byte record per particle
-    const double4* __restrict__ pq = reinterpret_cast<const double4*>(a.pq);
-    const double4 pi = pq[i];
-    const double qi = pi.w;
+    using Rec = typename PairRec<R>::type;
+    const Rec* __restrict__ pq = reinterpret_cast<const Rec*>(a.pq);
+    const Rec pi = pq[i];
+    const double qi = rec_charge<R>(pi);
     // force += fdivr * d; energy and virial are accumulated doubled and halved at the end
     auto accumulate = [&](R fdivr, R dE, R dx, R dy, R dz)
     {
@@ -153,12 +211,10 @@ __device__ __forceinline__ void pair_particle(unsigned int i, int sub, const ker
     };
     if (a.nlist != nullptr && qi != 0.0 && a.rc >= 1.0e-12)
         {
-        auto add = [&](const double4& pj)
+        auto add = [&](const Rec& pj)
             {
-            const R qiqj = R(qi * pj.w);
-            const R dx = R(min_image(pi.x - pj.x, a.Lx, invLx));
-            const R dy = R(min_image(pi.y - pj.y, a.Ly, invLy));
-            const R dz = R(min_image(pi.z - pj.z, a.Lz, invLz));
+            R dx, dy, dz, qiqj;
+            rec_separation<R>(pi, pj, a, dx, dy, dz, qiqj);
             const R rsq = dx * dx + dy * dy + dz * dz;
             if (rsq >= rc2 || qiqj == R(0))
                 return;
@@ -184,8 +240,8 @@ __device__ __forceinline__ void pair_particle(unsigned int i, int sub, const ker
                 unsigned int j1 = nl[min(unsigned(sub) + unsigned(G), last)];
                 for (unsigned int k = sub; k < nn; k += 2 * G)
                     {
-                    const double4 p0 = pq[j0];
-                    const double4 p1 = pq[j1];
+                    const Rec p0 = pq[j0];
+                    const Rec p1 = pq[j1];
                     const unsigned int kn = k + 2 * G;
                     j0 = nl[min(kn, last)];
                     j1 = nl[min(kn + G, last)];
@@ -210,11 +266,9 @@ __device__ __forceinline__ void pair_particle(unsigned int i, int sub, const ker
             const unsigned int j = a.ex_list[size_t(i) * a.ex_pitch + k];
             if (j >= a.N)
                 continue;
-            const double4 pj = pq[j];
-            const R qiqj = R(qi * pj.w);
-            const R dx = R(min_image(pi.x - pj.x, a.Lx, invLx));
-            const R dy = R(min_image(pi.y - pj.y, a.Ly, invLy));
-            const R dz = R(min_image(pi.z - pj.z, a.Lz, invLz));
+            const Rec pj = pq[j];
+            R dx, dy, dz, qiqj;
+            rec_separation<R>(pi, pj, a, dx, dy, dz, qiqj);
             const R rsq = dx * dx + dy * dy + dz * dz;
             if (rsq < R(1.0e-20))
                 continue;
@@ -357,21 +411,25 @@ __global__ void __launch_bounds__(256) kb_pair(const kernel::BatchSys* __restric
         }
     }
 
-__device__ __forceinline__ void pack_one(unsigned int i, const kernel::PairArgs& a)
+template<typename R> __device__ __forceinline__ void pack_one(unsigned int i, const kernel::PairArgs& a)
     {
     const double4 p = reinterpret_cast<const double4*>(a.postype)[i];
-    reinterpret_cast<double4*>(a.pq)[i] = make_double4(p.x, p.y, p.z, a.charge[i]);
+    if constexpr (sizeof(R) == 8)
+        reinterpret_cast<double4*>(a.pq)[i] = make_double4(p.x, p.y, p.z, a.charge[i]);
+    else
+        reinterpret_cast<uint4*>(a.pq)[i] = make_uint4(box_fraction(p.x, 1.0 / a.Lx), box_fraction(p.y, 1.0 / a.Ly),
+                                                      box_fraction(p.z, 1.0 / a.Lz), __float_as_uint(float(a.charge[i])));
     }
 
-__global__ void __launch_bounds__(256) k_pack_pq(kernel::PairArgs a)
+template<typename R> __global__ void __launch_bounds__(256) k_pack_pq(kernel::PairArgs a)
     {
     const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < a.N)
-        pack_one(i, a);
+        pack_one<R>(i, a);
     }
 
 //! Batched packing: items are (system, chunk of 256 particles).
-__global__ void __launch_bounds__(256) kb_pack_pq(const kernel::BatchSys* __restrict__ ds, kernel::BatchList l)
+template<typename R> __global__ void __launch_bounds__(256) kb_pack_pq(const kernel::BatchSys* __restrict__ ds, kernel::BatchList l)
     {
     for (unsigned int it = blockIdx.x; it < l.n; it += gridDim.x)
         {
@@ -379,7 +437,7 @@ __global__ void __launch_bounds__(256) kb_pack_pq(const kernel::BatchSys* __rest
         const kernel::PairArgs& a = ds[w.sys].pair;
         const unsigned int i = w.local * 256u + threadIdx.x;
         if (i < a.N)
-            pack_one(i, a);
+            pack_one<R>(i, a);
         }
     }
 
@@ -422,21 +480,25 @@ template<typename R, int G> static void launch_pair(const PairArgs& a, unsigned 
         launch_pair_v<R, G, false>(a, nb, 0, s);
     }
 
-hipError_t pack_positions_charges(const PairArgs& a, hipStream_t s)
+template<typename R> hipError_t pack_positions_charges(const PairArgs& a, hipStream_t s)
     {
     if (a.N == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(dev::k_pack_pq, dim3((a.N + 255u) / 256u), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(dev::k_pack_pq<R>, dim3((a.N + 255u) / 256u), dim3(256), 0, s, a);
     return hipGetLastError();
     }
+template hipError_t pack_positions_charges<double>(const PairArgs&, hipStream_t);
+template hipError_t pack_positions_charges<float>(const PairArgs&, hipStream_t);
 
-hipError_t batch_pack(const BatchSys* d, BatchList l, hipStream_t s)
+template<typename R> hipError_t batch_pack(const BatchSys* d, BatchList l, hipStream_t s)
     {
     if (l.n == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(dev::kb_pack_pq, dim3(l.grid), dim3(256), 0, s, d, l);
+    hipLaunchKernelGGL(dev::kb_pack_pq<R>, dim3(l.grid), dim3(256), 0, s, d, l);
     return hipGetLastError();
     }
+template hipError_t batch_pack<double>(const BatchSys*, BatchList, hipStream_t);
+template hipError_t batch_pack<float>(const BatchSys*, BatchList, hipStream_t);
 
 template<typename R> hipError_t pair_forces(const PairArgs& a, int lanes_per_particle, unsigned int max_blocks, hipStream_t s)
     {

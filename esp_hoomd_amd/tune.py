@@ -94,7 +94,7 @@ def predicted_error(c: float, sigma: float, P: int, precision: int = _lib.ESP_PR
 @dataclass
 class CostModel:
     """Per-step cost coefficients on one MI355X (see the module docstring)."""
-    pair_ps: Tuple[float, float] = (6.1, 4.85)        # (FP64, mixed) per neighbour entry, pair pass alone
+    pair_ps: Tuple[float, float] = (6.1, 3.9)         # (FP64, mixed) per neighbour entry, pair pass alone
     bin_ns: Tuple[float, float] = (0.15, 0.11)        # per particle (bin + scan + scatter)
     stencil_ps: Tuple[float, float] = (0.74, 0.56)    # per particle per P^3 (spread + gather)
     cell_ps: Tuple[float, float] = (94.0, 65.0)       # per mesh cell (FFT passes + tile halos)
