@@ -630,11 +630,12 @@ template<typename R> void EspSolver::run_pair(const SystemView& sys, const Outpu
     {
     const kernel::PairArgs a = pair_args(sys, out);
     int lanes = m_p.pair_lanes ? m_p.pair_lanes : 8;
-    // Beside the FP32 mesh chain the pass is held to 2 blocks (8 waves) per CU, which leaves
-    // the spreading and FFT kernels room to run alongside it (measured at 1e6 particles:
-    // 1.34 ms per step against 1.54 ms with the full grid).  The FP64 chain, which needs
-    // more registers per wave, gains nothing from the cap.
-    const unsigned int cap = (concurrent && sizeof(R) == 4) ? 2u * cu_count() : 0u;
+    // Beside the FP32 mesh chain the pass is held to 3 blocks (12 waves) per CU, which leaves
+    // the spreading and FFT kernels room to run alongside it (measured at 1e6 particles, mesh
+    // 160^3, r_c = 6.4: 0.99 ms per step against 1.13 ms at 2 blocks, 1.06 ms at 4 and
+    // 1.37 ms with the full grid).  The FP64 chain, which needs more registers per wave,
+    // gains nothing from the cap.
+    const unsigned int cap = (concurrent && sizeof(R) == 4) ? 3u * cu_count() : 0u;
     check_hip(kernel::pack_positions_charges<R>(a, s), "pack_positions_charges");
     // the "pair" stage of the per-stage timing spans the pair kernel alone
     pair_mark(false, s);
