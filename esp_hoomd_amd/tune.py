@@ -16,12 +16,13 @@ band limit by the mesh (h = mesh spacing):
   ~1.3x (measured on the same systems), so the first two terms carry a factor CUT_FACTOR.
 
 Cost model.  Per-step time on one MI355X, fitted to mesh-only, pair-only and full steps of
-the 1M-particle workload at P = 8 and 10 (meshes 162^3-216^3, r_c 5.4-7.2): the pair pass
-alone costs `pair_ps` per full-list neighbour entry; the mesh chain costs `bin_ns` per
-particle for binning, `stencil_ps * P^3` per particle for spread + gather and `cell_ps` per
-mesh cell for the FFT passes and tile halos.  The pair pass runs beside the mesh chain, which
-hides a share `overlap` of the shorter of the two; a step also pays `overhead_us` of launch
-and reduction latency.  The fit is within about 10% on those steps.  The tuner picks, over FFT-friendly meshes and the allowed orders, the
+the 1M-particle workload (P = 6-12, meshes 128^3-216^3, r_c 4-8): the pair pass alone costs
+`pair_us` plus `pair_ps` per full-list neighbour entry; the mesh chain costs `overhead_us`,
+`bin_ns` per particle for binning, `stencil_ps * P^3` per particle for spread + gather, and per
+mesh cell `cell_ps` for the FFT passes plus `halo_ps` times the tile-box volume ratio
+((T + P - 1) / T)^3 for the halos of the spreading tiles.  The pair pass runs beside the
+mesh chain, which hides min(hide_pair * pair, hide_mesh * mesh) of it.  The mesh-chain fit
+is within 7% (mixed) and 9% (FP64) RMS, the full step within about 5% near the optimum.  The tuner picks, over FFT-friendly meshes and the allowed orders, the
 smallest r_c that meets the target (minimising sigma * c at fixed h) and then the cheapest
 candidate.  `tune_measured` times the best few candidates on the GPU instead of trusting
 the model, and `verify` measures the chosen configuration's error on a proxy system
@@ -93,33 +94,41 @@ def predicted_error(c: float, sigma: float, P: int, precision: int = _lib.ESP_PR
 
 @dataclass
 class CostModel:
-    """Per-step cost coefficients on one MI355X (see the module docstring)."""
-    pair_ps: Tuple[float, float] = (6.1, 3.9)         # (FP64, mixed) per neighbour entry, pair pass alone
+    """Per-step cost coefficients on one MI355X (see the module docstring); pairs are
+    (FP64, mixed)."""
+    pair_ps: Tuple[float, float] = (5.3, 2.9)         # per neighbour entry, pair pass alone
+    pair_us: Tuple[float, float] = (90.0, 70.0)       # fixed cost of the pair pass
     bin_ns: Tuple[float, float] = (0.15, 0.11)        # per particle (bin + scan + scatter)
-    stencil_ps: Tuple[float, float] = (0.74, 0.56)    # per particle per P^3 (spread + gather)
-    cell_ps: Tuple[float, float] = (94.0, 65.0)       # per mesh cell (FFT passes + tile halos)
-    overlap: Tuple[float, float] = (0.3, 0.55)        # share of the shorter branch hidden by the other
-    overhead_us: float = 25.0
+    stencil_ps: Tuple[float, float] = (0.53, 0.26)    # per particle per P^3 (spread + gather)
+    cell_ps: Tuple[float, float] = (11.8, 31.7)       # per mesh cell (FFT passes)
+    halo_ps: Tuple[float, float] = (12.7, 7.2)        # per mesh cell per ((T + P - 1) / T)^3, T = 8
+    hide_pair: Tuple[float, float] = (0.25, 0.8)      # the mesh chain hides at most this share
+    hide_mesh: Tuple[float, float] = (0.15, 0.4)      # of the pair pass, and at most this share of itself
+    overhead_us: Tuple[float, float] = (120.0, 84.0)  # launch and reduction latency of the mesh chain
+    overlap: bool = True                              # False: branches add up (no hiding)
 
     def branch_ms(self, N: int, n_cells: int, P: int, neighbours: float, precision: int) -> Tuple[float, float]:
         """(mesh chain, pair pass) each run alone, in ms."""
         k = 0 if precision == _lib.ESP_PREC_FP64 else 1
-        mesh = (N * P ** 3 * self.stencil_ps[k] + n_cells * self.cell_ps[k] + N * self.bin_ns[k] * 1e3) * 1e-9
-        pair = N * neighbours * self.pair_ps[k] * 1e-9
+        halo = ((8 + P - 1) / 8.0) ** 3
+        mesh = (N * P ** 3 * self.stencil_ps[k] + n_cells * (self.cell_ps[k] + self.halo_ps[k] * halo)
+                + N * self.bin_ns[k] * 1e3) * 1e-9 + self.overhead_us[k] * 1e-3
+        pair = N * neighbours * self.pair_ps[k] * 1e-9 + self.pair_us[k] * 1e-3
         return mesh, pair
 
     def step_ms(self, N: int, n_cells: int, P: int, neighbours: float, precision: int) -> float:
         k = 0 if precision == _lib.ESP_PREC_FP64 else 1
         mesh, pair = self.branch_ms(N, n_cells, P, neighbours, precision)
-        return mesh + pair - self.overlap[k] * min(mesh, pair) + self.overhead_us * 1e-3
+        hidden = min(self.hide_pair[k] * pair, self.hide_mesh[k] * mesh) if self.overlap else 0.0
+        return mesh + pair - hidden
 
 
 #: Systems solved inside a batch (many small systems, pair and mesh items of all systems sharing
 #: the GPU) are tuned with an additive model that gives the pair pass no overlap credit: on the
 #: 512-system batch of 500-50,000 particles its choices measured 13.0 ms per batch step against
 #: 15.4 ms for the single-system fit above.
-BATCHED_COST = CostModel(pair_ps=(10.0, 6.5), bin_ns=(0.18, 0.15), stencil_ps=(1.17, 0.62), cell_ps=(90.0, 70.0),
-                         overlap=(0.0, 0.0))
+BATCHED_COST = CostModel(pair_ps=(10.0, 6.5), pair_us=(0.0, 0.0), bin_ns=(0.18, 0.15), stencil_ps=(1.17, 0.62),
+                         cell_ps=(90.0, 70.0), halo_ps=(0.0, 0.0), overhead_us=(25.0, 25.0), overlap=False)
 
 
 @dataclass
