@@ -159,9 +159,9 @@ template<typename R> void EspBatch::enqueue_stages(hipStream_t st)
     const kernel::BatchSys* d = m_desc_dev.as<kernel::BatchSys>();
     // the pair correction runs on a branch; the gather (which adds into the same force
     // arrays) waits for it
+    check_hip(kernel::batch_pack<R>(d, m_l.pack, st), "batch_pack");
     check_hip(hipEventRecord(m_fork, st), "hipEventRecord");
     check_hip(hipStreamWaitEvent(m_side, m_fork, 0), "hipStreamWaitEvent");
-    check_hip(kernel::batch_pack<R>(d, m_l.pack, m_side), "batch_pack");
     check_hip(kernel::batch_pair<R>(d, m_l.pair, m_lds_pair, m_side), "batch_pair");
     check_hip(hipEventRecord(m_join, m_side), "hipEventRecord");
     if (m_any_mesh)

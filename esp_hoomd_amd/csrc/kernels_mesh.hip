@@ -110,37 +110,73 @@ __global__ void __launch_bounds__(256) k_bin(unsigned int n,
         }
     }
 
-//! Exclusive scan of tile counts by one 1024-thread workgroup; clears the counts for the next step.
+//! Exclusive scan of tile counts by one 1024-thread workgroup; clears the counts for the next
+//! step.  Rounds of 16 consecutive counts per thread: the 16 loads of a thread are independent
+//! and in flight together, the thread sums are scanned with wave shuffles, then across the 16
+//! waves.
 __device__ __forceinline__ void scan_one(unsigned int n_tiles, unsigned int* __restrict__ count, unsigned int* __restrict__ start)
     {
-    __shared__ unsigned int part[1024];
-    const unsigned int tid = threadIdx.x;
-    const unsigned int chunk = (n_tiles + 1023u) / 1024u;
-    const unsigned int b0 = tid * chunk;
-    const unsigned int b1 = min(b0 + chunk, n_tiles);
-    unsigned int s = 0;
-    for (unsigned int i = b0; i < b1; ++i)
-        s += count[i];
-    part[tid] = s;
-    __syncthreads();
-    // Hillis-Steele inclusive scan over 1024 partials
-    for (unsigned int off = 1; off < 1024u; off <<= 1)
+    constexpr int CH = 16;
+    __shared__ unsigned int wsum[16];
+    const unsigned int tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    unsigned int carry = 0;
+    for (unsigned int base = 0; base < n_tiles; base += 1024u * CH)
         {
-        const unsigned int v = (tid >= off) ? part[tid - off] : 0u;
+        const unsigned int b0 = base + tid * CH;
+        unsigned int v[CH];
+        unsigned int s = 0;
+#pragma unroll
+        for (int k = 0; k < CH; ++k)
+            {
+            const unsigned int i = b0 + unsigned(k);
+            v[k] = i < n_tiles ? count[i] : 0u;
+            }
+#pragma unroll
+        for (int k = 0; k < CH; ++k)
+            s += v[k];
+        // inclusive scan of the thread sums within the wave
+        unsigned int x = s;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1)
+            {
+            const unsigned int y = __shfl_up(x, off);
+            if (int(lane) >= off)
+                x += y;
+            }
+        if (lane == 63u)
+            wsum[wave] = x;
         __syncthreads();
-        part[tid] += v;
+        if (wave == 0)
+            {
+            unsigned int w = lane < 16u ? wsum[lane] : 0u;
+#pragma unroll
+            for (int off = 1; off < 16; off <<= 1)
+                {
+                const unsigned int y = __shfl_up(w, off);
+                if (int(lane) >= off)
+                    w += y;
+                }
+            if (lane < 16u)
+                wsum[lane] = w;
+            }
+        __syncthreads();
+        unsigned int run = carry + (wave ? wsum[wave - 1] : 0u) + x - s;
+#pragma unroll
+        for (int k = 0; k < CH; ++k)
+            {
+            const unsigned int i = b0 + unsigned(k);
+            if (i < n_tiles)
+                {
+                start[i] = run;
+                count[i] = 0u;
+                }
+            run += v[k];
+            }
+        carry += wsum[15];
         __syncthreads();
         }
-    unsigned int run = part[tid] - s;
-    for (unsigned int i = b0; i < b1; ++i)
-        {
-        const unsigned int c = count[i];
-        start[i] = run;
-        run += c;
-        count[i] = 0u;
-        }
-    if (tid == 1023u)
-        start[n_tiles] = part[1023];
+    if (tid == 0)
+        start[n_tiles] = carry;
     }
 
 __global__ void __launch_bounds__(1024) k_scan(unsigned int n_tiles, unsigned int* __restrict__ count, unsigned int* __restrict__ start)
