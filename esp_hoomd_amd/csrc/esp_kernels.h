@@ -71,8 +71,14 @@ hipError_t finalize(int n_ev, const double* ev_part, int n_q, const double* qpar
                     double V, int want_ev, double* out, double* qsum, hipStream_t s);
 
 // ---- short-range pair correction (EvaluatorPairPSWF over a full neighbour list) ----
-//! Degree of the compact per-segment polynomial of S(s) the pair kernels evaluate.
-constexpr unsigned int PAIR_COMPACT_DEG = 11;
+//! Degree of the compact per-segment polynomial of S(s) the pair kernels evaluate: 11 in FP64;
+//! 7 in FP32, whose fit error over the 64 segments (below 1e-12 in S, 1e-9 in dS/ds relative
+//! to its scale) is far below FP32 resolution and which needs two 16-byte LDS loads and 14
+//! FMAs per neighbour instead of three and 22.
+template<typename R> constexpr unsigned int pair_compact_deg()
+    {
+    return sizeof(R) == 4 ? 7u : 11u;
+    }
 struct PairArgs
     {
     unsigned int N;

@@ -481,7 +481,7 @@ void EspSolver::initialize(const SystemView& sys, hipStream_t s)
     if (!m_external_table)
         {
         const math::SplitKind kind = m_p.split == 1 ? math::SplitKind::GAUSSIAN : math::SplitKind::PSWF;
-        const auto ct = compact_split(kind, m_p.kappa * m_p.rcut, COMPACT_SEGS, COMPACT_DEG);
+        const auto ct = compact_split(kind, m_p.kappa * m_p.rcut, COMPACT_SEGS, compact_deg());
         if (fp64)
             upload_as<double>(m_compact, *ct);
         else
@@ -617,7 +617,7 @@ kernel::PairArgs EspSolver::pair_args(const SystemView& sys, const OutputView& o
     a.n_segs = unsigned(m_r_lo.size());
     a.compact_tab = m_external_table ? nullptr : m_compact.get();
     a.n_compact = COMPACT_SEGS;
-    a.compact_deg = COMPACT_DEG;
+    a.compact_deg = compact_deg();
     a.rc = m_p.rcut;
     a.force = out.force;
     a.virial = out.virial;

@@ -217,9 +217,9 @@ class EspSolver
 
     // device
     static constexpr unsigned int COMPACT_SEGS = 64;
-    static constexpr unsigned int COMPACT_DEG = kernel::PAIR_COMPACT_DEG;
     DeviceBuffer m_split_tab;   // [n_segs][12] R
-    DeviceBuffer m_compact;     // [COMPACT_SEGS][COMPACT_DEG + 1] R
+    DeviceBuffer m_compact;     // [COMPACT_SEGS][compact_deg() + 1] R
+    unsigned int compact_deg() const { return m_p.precision == 0 ? kernel::pair_compact_deg<double>() : kernel::pair_compact_deg<float>(); }
     DeviceBuffer m_wcoef;       // [P][D+1] R
     DeviceBuffer m_tw[3];       // twiddles per axis, complex R
     DeviceBuffer m_G, m_Gv;     // half spectrum FP64

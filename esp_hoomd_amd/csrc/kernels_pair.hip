@@ -92,7 +92,7 @@ __device__ __forceinline__ void rec_separation(const typename PairRec<R>::type& 
 template<typename R>
 __device__ __forceinline__ void compact_split(const R* __restrict__ tab, int nseg, R s, R& S, R& dS)
     {
-    constexpr int NC = kernel::PAIR_COMPACT_DEG + 1;
+    constexpr int NC = int(kernel::pair_compact_deg<R>()) + 1;
     constexpr int PER16 = 16 / int(sizeof(R));
     static_assert(NC % PER16 == 0, "compact rows must be whole 16-byte words");
     const R u = s * R(nseg);
