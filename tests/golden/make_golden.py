@@ -5,7 +5,7 @@ text data (PSWF_SCREEN_COEFFS, PSWF_DSCREEN_COEFFS, PSWF_R_LO, PSWF_DR_INV; line
 keeps a sample of segments, and evaluates the lookup of EvaluatorPairPSWF.h:113-143
 (clamp s to [0, 1 - 1e-9], seg = min(floor(s * 4096), 4095), t = (r - r_lo) * dr_inv,
 Horner c4 -> c0, L = S / (4 pi r), -dL/dr = S / (4 pi r^2) - S' / (4 pi r)) at r_c = 1 with
-numpy.polyval.  Run in the build container only (the reference is not on the GPU box):
+numpy.polyval.  Run where the reference sources are present (they are not needed by the tests):
 
     python tests/golden/make_golden.py /root/reference/src/PSWF_Coeffs.h
 """
