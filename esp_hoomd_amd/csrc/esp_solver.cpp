@@ -232,20 +232,6 @@ void EspSolver::mark(int b, hipStream_t s)
         m_ev_cur = nullptr;
     }
 
-//! Environment ESP_PAIR_FORK (default 0): the pair branch is launched on the side stream at the
-//! start of mesh-chain stage k (1 binning, 3 scatter, 4 spread, 5 forward x FFT, ... 10 gather);
-//! 0 launches it before the chain.
-int EspSolver::fork_stage()
-    {
-    static int v = -1;
-    if (v < 0)
-        {
-        const char* e = getenv("ESP_PAIR_FORK");
-        v = e ? std::min(std::max(atoi(e), 0), 10) : 0;
-        }
-    return v;
-    }
-
 //! Record the start (end = false) or the end of the pair stage on the stream it runs on.
 void EspSolver::pair_mark(bool end, hipStream_t s)
     {
@@ -661,49 +647,41 @@ unsigned int cu_count()
     }
 
 template<typename R>
-void EspSolver::run_mesh(const SystemView& sys, const OutputView& out, unsigned int n, hipStream_t s, bool want_ev, hipEvent_t join,
-                         const std::function<void()>& fork)
+void EspSolver::run_mesh(const SystemView& sys, const OutputView& out, unsigned int n, hipStream_t s, bool want_ev, hipEvent_t join)
     {
     const MeshGeom& g = m_geom;
-    // stage boundary k: profiling mark, then the pair branch forks off if this is its stage
-    auto mark_fork = [&](int k)
-    {
-        mark(k, s);
-        if (fork && k == fork_stage())
-            fork();
-    };
     const int P = int(m_p.order);
     const int D = int(m_D);
     const unsigned int nqb = (n + 255) / 256;
-    mark_fork(1);
+    mark(1, s);
     check_hip(kernel::bin_particles(n, sys.group_idx, sys.postype, sys.charge, g, P, m_tile_count.as<unsigned int>(),
                                     m_key.as<unsigned int>(), m_slot.as<unsigned int>(), m_qpart.as<double>(), s),
               "bin_particles");
-    mark_fork(2);
+    mark(2, s);
     check_hip(kernel::scan_tiles(m_n_tiles, m_tile_count.as<unsigned int>(), m_tile_start.as<unsigned int>(), s), "scan_tiles");
-    mark_fork(3);
+    mark(3, s);
     check_hip(kernel::scatter_particles<R>(n, sys.group_idx, sys.postype, sys.charge, g, P, m_key.as<unsigned int>(),
                                            m_slot.as<unsigned int>(), m_tile_start.as<unsigned int>(), m_rec.get(),
                                            m_sidx.as<SortedIdx>(), s),
               "scatter_particles");
-    mark_fork(4);
+    mark(4, s);
     check_hip(kernel::spread<R>(g, P, D, m_n_tiles, m_tile_start.as<unsigned int>(), m_rec.get(), m_sidx.as<SortedIdx>(), m_wcoef.get(),
                                 m_mesh.get(), s),
               "spread");
-    mark_fork(5);
+    mark(5, s);
     check_hip(kernel::fft_forward_x<R>(g, m_plan[0], m_tw[0].get(), m_mesh.get(), m_spec.get(), s), "fft_forward_x");
-    mark_fork(6);
+    mark(6, s);
     check_hip(kernel::fft_y<R>(g, m_plan[1], m_tw[1].get(), m_spec.get(), -1, s), "fft_y");
-    mark_fork(7);
+    mark(7, s);
     check_hip(kernel::fft_z_convolve<R>(g, m_plan[2], m_tw[2].get(), m_spec.get(), m_G.as<double>(), m_Gv.as<double>(), want_ev ? 1 : 0,
                                         m_evpart.as<double>(), s),
               "fft_z_convolve");
-    mark_fork(8);
+    mark(8, s);
     check_hip(kernel::fft_y<R>(g, m_plan[1], m_tw[1].get(), m_spec.get(), +1, s), "fft_y inverse");
-    mark_fork(9);
+    mark(9, s);
     const double V = g.Lx * g.Ly * g.Lz;
     check_hip(kernel::fft_inverse_x<R>(g, m_plan[0], m_tw[0].get(), m_spec.get(), 1.0 / V, m_phi.get(), s), "fft_inverse_x");
-    mark_fork(10);
+    mark(10, s);
     if (join)
         check_hip(hipStreamWaitEvent(s, join, 0), "hipStreamWaitEvent");
     check_hip(kernel::gather<R>(g, P, D, m_n_tiles, m_tile_start.as<unsigned int>(), m_rec.get(), m_sidx.as<SortedIdx>(), m_wcoef.get(),
@@ -763,9 +741,9 @@ void EspSolver::compute(const SystemView& sys, const OutputView& out, hipStream_
     const bool fp64 = m_p.precision == 0;
     const unsigned int flags = out.flags ? out.flags : COMPUTE_ALL;
     mark(0, s);
-    // the pair correction runs on a side stream, forked at stage boundary fork_stage() of the
-    // mesh chain (0: before binning) so that it overlaps the rest of the chain; the gather
-    // (which adds into the same force array) waits for it
+    // the pair correction runs on a side stream, overlapping binning, spreading and the FFTs;
+    // the gather (which adds into the same force array) waits for it.  (Forking it later in
+    // the chain measured no faster.)
     const bool mesh = (flags & COMPUTE_MESH) != 0;
     const bool side = (flags & COMPUTE_PAIR) && mesh && m_overlap;
     hipEvent_t join = nullptr;
@@ -777,8 +755,8 @@ void EspSolver::compute(const SystemView& sys, const OutputView& out, hipStream_
         }
     if (side)
         join = m_ev_join;
-    auto launch_pair = [&]()
-    {
+    if (flags & COMPUTE_PAIR)
+        {
         hipStream_t ps = s;
         if (side)
             {
@@ -792,14 +770,6 @@ void EspSolver::compute(const SystemView& sys, const OutputView& out, hipStream_
             run_pair<float>(sys, out, ps, side);
         if (side)
             check_hip(hipEventRecord(m_ev_join, ps), "hipEventRecord");
-    };
-    std::function<void()> late_fork;
-    if (flags & COMPUTE_PAIR)
-        {
-        if (side && fork_stage() > 0)
-            late_fork = launch_pair;
-        else
-            launch_pair();
         }
     else
         {
@@ -812,9 +782,9 @@ void EspSolver::compute(const SystemView& sys, const OutputView& out, hipStream_
         {
         const bool want_ev = (flags & COMPUTE_ENERGY_VIRIAL) != 0;
         if (fp64)
-            run_mesh<double>(sys, out, n, s, want_ev, join, late_fork);
+            run_mesh<double>(sys, out, n, s, want_ev, join);
         else
-            run_mesh<float>(sys, out, n, s, want_ev, join, late_fork);
+            run_mesh<float>(sys, out, n, s, want_ev, join);
         }
     else
         for (int b = 1; b <= 11; ++b)

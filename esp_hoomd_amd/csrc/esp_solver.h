@@ -10,7 +10,6 @@
 
 #include <cstddef>
 #include <cstdint>
-#include <functional>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -192,10 +191,7 @@ class EspSolver
     void build_influence(hipStream_t s);
     void ensure_particles(unsigned int n);
     template<typename R>
-    void run_mesh(const SystemView& sys, const OutputView& out, unsigned int n, hipStream_t s, bool want_ev, hipEvent_t join,
-                  const std::function<void()>& fork);
-    //! Mesh-chain stage (1..10) at whose start the pair branch forks; 0 = before the chain.
-    static int fork_stage();
+    void run_mesh(const SystemView& sys, const OutputView& out, unsigned int n, hipStream_t s, bool want_ev, hipEvent_t join);
     //! concurrent: the pass runs on the side stream beside the mesh chain.
     template<typename R> void run_pair(const SystemView& sys, const OutputView& out, hipStream_t s, bool concurrent);
 

@@ -181,7 +181,7 @@ __device__ __forceinline__ void pair_terms(const R* __restrict__ tab,
 //! Pair correction of particle i by the G lanes of its group (`sub` = lane within the group);
 //! every lane of the group must call it.  Writes force (fx, fy, fz, energy) and, when
 //! requested, the six per-particle virial components.
-template<typename R, int G, bool COMPACT, int U = 2>
+template<typename R, int G, bool COMPACT>
 __device__ __forceinline__ void pair_particle(unsigned int i, int sub, const kernel::PairArgs& a, const R* __restrict__ tab)
     {
     const R rc = R(a.rc);
@@ -226,35 +226,27 @@ __device__ __forceinline__ void pair_particle(unsigned int i, int sub, const ker
             };
         const unsigned int nn = a.n_neigh[i];
         const uint64_t head = a.head_list[i];
-        if constexpr (U == 2)
+        // two neighbours in flight per lane, and the list entries of the next pair are fetched
+        // before this pair is evaluated.  Entries past the end are read at the clamped index
+        // nn - 1 (in bounds) and not evaluated.  Neighbours are accumulated in list order, so
+        // the sums are the same as one at a time.
+        if (unsigned(sub) < nn)
             {
-            // two neighbours in flight per lane, and the list entries of the next pair are
-            // fetched before this pair is evaluated.  Entries past the end are read at the
-            // clamped index nn - 1 (in bounds) and not evaluated.  Neighbours are accumulated
-            // in list order, so the sums are the same as one at a time.
-            if (unsigned(sub) < nn)
+            const unsigned int last = nn - 1u;
+            const unsigned int* __restrict__ nl = a.nlist + head;
+            unsigned int j0 = nl[sub];
+            unsigned int j1 = nl[min(unsigned(sub) + unsigned(G), last)];
+            for (unsigned int k = sub; k < nn; k += 2 * G)
                 {
-                const unsigned int last = nn - 1u;
-                const unsigned int* __restrict__ nl = a.nlist + head;
-                unsigned int j0 = nl[sub];
-                unsigned int j1 = nl[min(unsigned(sub) + unsigned(G), last)];
-                for (unsigned int k = sub; k < nn; k += 2 * G)
-                    {
-                    const Rec p0 = pq[j0];
-                    const Rec p1 = pq[j1];
-                    const unsigned int kn = k + 2 * G;
-                    j0 = nl[min(kn, last)];
-                    j1 = nl[min(kn + G, last)];
-                    add(p0);
-                    if (k + G < nn)
-                        add(p1);
-                    }
+                const Rec p0 = pq[j0];
+                const Rec p1 = pq[j1];
+                const unsigned int kn = k + 2 * G;
+                j0 = nl[min(kn, last)];
+                j1 = nl[min(kn + G, last)];
+                add(p0);
+                if (k + G < nn)
+                    add(p1);
                 }
-            }
-        else
-            {
-            for (unsigned int k = sub; k < nn; k += G)
-                add(pq[a.nlist[head + k]]);
             }
         }
     if (a.n_ex != nullptr && qi != 0.0)
@@ -322,11 +314,11 @@ __device__ __forceinline__ const R* stage_compact_table(const kernel::PairArgs& 
     return t;
     }
 
-//! Single-system pair kernel.  Block b runs on XCD b % 8.  MAP = 2: each XCD takes one
-//! contiguous eighth of the particles and its blocks sweep it interleaved, so the XCD's L2
-//! holds the neighbour positions of one moving window; MAP = 1: the same eighths, but each
-//! block takes a contiguous sub-range; MAP = 0: plain grid stride.
-template<typename R, int G, bool COMPACT, int U, int MAP>
+//! Single-system pair kernel, persistent grid.  Blocks are placed on the 8 XCDs round-robin
+//! (block b on XCD (b + k) mod 8 for a launch-dependent k), so the blocks with one value of
+//! b mod 8 share an XCD: they take one contiguous eighth of the particles and sweep it
+//! together (interleaved), so that XCD's L2 holds the neighbour records of one moving window.
+template<typename R, int G, bool COMPACT>
 __global__ void __launch_bounds__(256) k_pair(kernel::PairArgs a)
     {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -338,35 +330,14 @@ __global__ void __launch_bounds__(256) k_pair(kernel::PairArgs a)
         }
     const int sub = int(threadIdx.x % G);
     const unsigned int per_pass = blockDim.x / G;
-    if constexpr (MAP == 2)
-        {
-        // each XCD owns one contiguous eighth of the particles, and its blocks sweep that
-        // range together (interleaved), so the XCD's L2 holds one window of neighbours
-        const unsigned int nb = gridDim.x;
-        const unsigned int nx = (nb % 8u == 0u) ? 8u : 1u;
-        const unsigned int x = blockIdx.x % nx, j = blockIdx.x / nx, per_x = nb / nx;
-        const unsigned int span = (a.N + nx - 1u) / nx;
-        const unsigned int p0 = x * span;
-        const unsigned int p1 = min(a.N, p0 + span);
-        for (unsigned int i = p0 + j * per_pass + threadIdx.x / G; i < p1; i += per_x * per_pass)
-            pair_particle<R, G, COMPACT, U>(i, sub, a, tab);
-        }
-    else if constexpr (MAP == 1)
-        {
-        const unsigned int nb = gridDim.x;
-        const unsigned int lb = (nb % 8u == 0u) ? (blockIdx.x % 8u) * (nb / 8u) + blockIdx.x / 8u : blockIdx.x;
-        const unsigned int span = (a.N + nb - 1u) / nb;
-        const unsigned int p0 = lb * span;
-        const unsigned int p1 = min(a.N, p0 + span);
-        for (unsigned int i = p0 + threadIdx.x / G; i < p1; i += per_pass)
-            pair_particle<R, G, COMPACT, U>(i, sub, a, tab);
-        }
-    else
-        {
-        const unsigned int groups_per_grid = gridDim.x * per_pass;
-        for (unsigned int i = (blockIdx.x * blockDim.x + threadIdx.x) / G; i < a.N + 0u; i += groups_per_grid)
-            pair_particle<R, G, COMPACT, U>(i, sub, a, tab);
-        }
+    const unsigned int nb = gridDim.x;
+    const unsigned int nx = (nb % 8u == 0u) ? 8u : 1u;
+    const unsigned int x = blockIdx.x % nx, j = blockIdx.x / nx, per_x = nb / nx;
+    const unsigned int span = (a.N + nx - 1u) / nx;
+    const unsigned int p0 = x * span;
+    const unsigned int p1 = min(a.N, p0 + span);
+    for (unsigned int i = p0 + j * per_pass + threadIdx.x / G; i < p1; i += per_x * per_pass)
+        pair_particle<R, G, COMPACT>(i, sub, a, tab);
     }
 
 //! Batched pair correction: items are (system, chunk of 256 / G particles).  The compact
@@ -445,39 +416,12 @@ template<typename R> __global__ void __launch_bounds__(256) kb_pack_pq(const ker
 
 namespace kernel
 {
-//! Pair-kernel variant (environment ESP_PAIR_VARIANT, default 5): 0/1/4 one neighbour in
-//! flight per lane, 2/3/5 two; particle mapping 0/2 plain grid stride, 1/3 XCD-contiguous
-//! block ranges, 4/5 XCD-contiguous ranges swept by interleaved blocks.
-static int pair_variant()
-    {
-    static int v = -1;
-    if (v < 0)
-        {
-        const char* e = getenv("ESP_PAIR_VARIANT");
-        v = e ? atoi(e) : 5;
-        }
-    return v;
-    }
-
-template<typename R, int G, bool C> static void launch_pair_v(const PairArgs& a, unsigned int nb, size_t lds, hipStream_t s)
-    {
-    switch (pair_variant())
-        {
-        case 0: hipLaunchKernelGGL((dev::k_pair<R, G, C, 1, 0>), dim3(nb), dim3(256), lds, s, a); break;
-        case 1: hipLaunchKernelGGL((dev::k_pair<R, G, C, 1, 1>), dim3(nb), dim3(256), lds, s, a); break;
-        case 2: hipLaunchKernelGGL((dev::k_pair<R, G, C, 2, 0>), dim3(nb), dim3(256), lds, s, a); break;
-        case 3: hipLaunchKernelGGL((dev::k_pair<R, G, C, 2, 1>), dim3(nb), dim3(256), lds, s, a); break;
-        case 4: hipLaunchKernelGGL((dev::k_pair<R, G, C, 1, 2>), dim3(nb), dim3(256), lds, s, a); break;
-        default: hipLaunchKernelGGL((dev::k_pair<R, G, C, 2, 2>), dim3(nb), dim3(256), lds, s, a); break;
-        }
-    }
-
 template<typename R, int G> static void launch_pair(const PairArgs& a, unsigned int nb, hipStream_t s)
     {
     if (a.compact_tab != nullptr)
-        launch_pair_v<R, G, true>(a, nb, size_t(a.n_compact) * (a.compact_deg + 1) * sizeof(R), s);
+        hipLaunchKernelGGL((dev::k_pair<R, G, true>), dim3(nb), dim3(256), size_t(a.n_compact) * (a.compact_deg + 1) * sizeof(R), s, a);
     else
-        launch_pair_v<R, G, false>(a, nb, 0, s);
+        hipLaunchKernelGGL((dev::k_pair<R, G, false>), dim3(nb), dim3(256), 0, s, a);
     }
 
 template<typename R> hipError_t pack_positions_charges(const PairArgs& a, hipStream_t s)
