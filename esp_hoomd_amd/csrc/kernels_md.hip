@@ -101,43 +101,65 @@ __global__ void __launch_bounds__(256) k_nlist(unsigned int N, const double4* __
 
 //! WCA (LJ truncated at 2^(1/6) sigma and shifted) of particle i over the full list; adds to
 //! its force/virial.
-__device__ __forceinline__ void wca_one(unsigned int i, const double4* __restrict__ pos, double Lx, double Ly, double Lz,
-                                        const unsigned int* __restrict__ n_neigh, const unsigned int* __restrict__ nlist,
-                                        const uint64_t* __restrict__ head, double sigma, double epsilon, double4* __restrict__ force,
-                                        double* __restrict__ virial, size_t pitch)
+//! WCA force, energy and virial of particle i added to its outputs, by the G lanes of its group
+//! (`sub` = lane within the group) over the (longer) ESP neighbour list: each lane takes every
+//! G-th entry, so the group has G neighbour loads in flight (the list is mostly beyond the
+//! WCA cutoff, so the pass is bound by those loads).  Every lane of the group must call it;
+//! `active` (i < N) is the same for the whole group.
+constexpr int WCA_LANES = 8;
+template<int G>
+__device__ __forceinline__ void wca_group(unsigned int i, int sub, bool active, const double4* __restrict__ pos, double Lx, double Ly,
+                                          double Lz, const unsigned int* __restrict__ n_neigh, const unsigned int* __restrict__ nlist,
+                                          const uint64_t* __restrict__ head, double sigma, double epsilon, double4* __restrict__ force,
+                                          double* __restrict__ virial, size_t pitch)
     {
-    const double4 ri = pos[i];
-    const double invLx = 1.0 / Lx, invLy = 1.0 / Ly, invLz = 1.0 / Lz;
-    const double rc2 = 1.2599210498948732 * sigma * sigma;  // (2^(1/6) sigma)^2
-    const double s2 = sigma * sigma;
     double fx = 0, fy = 0, fz = 0, e = 0, v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
-    const unsigned int nn = n_neigh[i];
-    const uint64_t h = head[i];
-    for (unsigned int k = 0; k < nn; ++k)
+    if (active)
         {
-        const unsigned int j = nlist[h + k];
-        const double4 rj = pos[j];
-        const double dx = wrap_min_image(ri.x - rj.x, Lx, invLx);
-        const double dy = wrap_min_image(ri.y - rj.y, Ly, invLy);
-        const double dz = wrap_min_image(ri.z - rj.z, Lz, invLz);
-        const double r2 = dx * dx + dy * dy + dz * dz;
-        if (r2 >= rc2)
-            continue;
-        const double sr2 = s2 / r2;
-        const double sr6 = sr2 * sr2 * sr2;
-        const double fdivr = 24.0 * epsilon * sr6 * (2.0 * sr6 - 1.0) / r2;
-        fx += fdivr * dx;
-        fy += fdivr * dy;
-        fz += fdivr * dz;
-        e += 0.5 * (4.0 * epsilon * sr6 * (sr6 - 1.0) + epsilon);
-        const double hf = 0.5 * fdivr;
-        v0 += hf * dx * dx;
-        v1 += hf * dx * dy;
-        v2 += hf * dx * dz;
-        v3 += hf * dy * dy;
-        v4 += hf * dy * dz;
-        v5 += hf * dz * dz;
+        const double4 ri = pos[i];
+        const double invLx = 1.0 / Lx, invLy = 1.0 / Ly, invLz = 1.0 / Lz;
+        const double rc2 = 1.2599210498948732 * sigma * sigma;  // (2^(1/6) sigma)^2
+        const double s2 = sigma * sigma;
+        const unsigned int nn = n_neigh[i];
+        const uint64_t h = head[i];
+        for (unsigned int k = unsigned(sub); k < nn; k += unsigned(G))
+            {
+            const unsigned int j = nlist[h + k];
+            const double4 rj = pos[j];
+            const double dx = wrap_min_image(ri.x - rj.x, Lx, invLx);
+            const double dy = wrap_min_image(ri.y - rj.y, Ly, invLy);
+            const double dz = wrap_min_image(ri.z - rj.z, Lz, invLz);
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 >= rc2)
+                continue;
+            const double sr2 = s2 / r2;
+            const double sr6 = sr2 * sr2 * sr2;
+            const double fdivr = 24.0 * epsilon * sr6 * (2.0 * sr6 - 1.0) / r2;
+            fx += fdivr * dx;
+            fy += fdivr * dy;
+            fz += fdivr * dz;
+            e += 0.5 * (4.0 * epsilon * sr6 * (sr6 - 1.0) + epsilon);
+            const double hf = 0.5 * fdivr;
+            v0 += hf * dx * dx;
+            v1 += hf * dx * dy;
+            v2 += hf * dx * dz;
+            v3 += hf * dy * dy;
+            v4 += hf * dy * dz;
+            v5 += hf * dz * dz;
+            }
         }
+    fx = group_sum<G>(fx);
+    fy = group_sum<G>(fy);
+    fz = group_sum<G>(fz);
+    e = group_sum<G>(e);
+    v0 = group_sum<G>(v0);
+    v1 = group_sum<G>(v1);
+    v2 = group_sum<G>(v2);
+    v3 = group_sum<G>(v3);
+    v4 = group_sum<G>(v4);
+    v5 = group_sum<G>(v5);
+    if (!active || sub != 0)
+        return;
     double4 f = force[i];
     f.x += fx;
     f.y += fy;
@@ -160,11 +182,10 @@ __global__ void __launch_bounds__(256) k_wca(unsigned int N, const double4* __re
                                              const uint64_t* __restrict__ head, double sigma, double epsilon, double4* __restrict__ force,
                                              double* __restrict__ virial, size_t pitch)
     {
-    const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < N)
-        wca_one(i, pos, Lx, Ly, Lz, n_neigh, nlist, head, sigma, epsilon, force, virial, pitch);
+    const unsigned int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned int i = t / WCA_LANES;
+    wca_group<WCA_LANES>(i, int(t % WCA_LANES), i < N, pos, Lx, Ly, Lz, n_neigh, nlist, head, sigma, epsilon, force, virial, pitch);
     }
-
 //! Counter-based uniform random number in (-1, 1) from (seed, step, particle, component).
 __device__ __forceinline__ double uniform_pm1(uint64_t seed, uint64_t step, uint64_t i, uint32_t c)
     {
@@ -338,10 +359,16 @@ __global__ void __launch_bounds__(256) kb_wca(const md::MDSys* __restrict__ ds, 
     {
     const md::MDItem w = items[blockIdx.x];
     const md::MDSys& d = ds[w.sys];
-    const unsigned int i = w.chunk * 256u + threadIdx.x;
-    if (i < d.N && d.nlist != nullptr)
-        wca_one(i, reinterpret_cast<const double4*>(d.postype), d.L[0], d.L[1], d.L[2], d.n_neigh, d.nlist, d.head, sigma, epsilon,
-                reinterpret_cast<double4*>(d.force), d.virial, d.pitch);
+    if (d.nlist == nullptr)
+        return;
+    // the item's 256 particles in WCA_LANES passes of 256 / WCA_LANES particle groups
+    constexpr unsigned int PER_PASS = 256u / unsigned(WCA_LANES);
+    for (unsigned int pass = 0; pass < unsigned(WCA_LANES); ++pass)
+        {
+        const unsigned int i = w.chunk * 256u + pass * PER_PASS + threadIdx.x / unsigned(WCA_LANES);
+        wca_group<WCA_LANES>(i, int(threadIdx.x % WCA_LANES), i < d.N, reinterpret_cast<const double4*>(d.postype), d.L[0], d.L[1],
+                             d.L[2], d.n_neigh, d.nlist, d.head, sigma, epsilon, reinterpret_cast<double4*>(d.force), d.virial, d.pitch);
+        }
     }
 
 //! Per-system largest squared displacement since its last neighbour-list build (bits of a
@@ -414,7 +441,7 @@ hipError_t wca_forces(unsigned int N, const double* pos, double Lx, double Ly, d
     {
     if (N == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_wca, dim3(nblocks(N)), dim3(256), 0, s, N, reinterpret_cast<const double4*>(pos), Lx, Ly, Lz, n_neigh, nlist, head,
+    hipLaunchKernelGGL(k_wca, dim3(nblocks(N * unsigned(dev::WCA_LANES))), dim3(256), 0, s, N, reinterpret_cast<const double4*>(pos), Lx, Ly, Lz, n_neigh, nlist, head,
                        sigma, epsilon, reinterpret_cast<double4*>(force), virial, pitch);
     return hipGetLastError();
     }
