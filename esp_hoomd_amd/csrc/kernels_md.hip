@@ -49,20 +49,31 @@ __global__ void __launch_bounds__(256) k_cell_fill(unsigned int N, const unsigne
 //! Full neighbour list: for every particle, all others within r_list (minimum image), excluding
 //! the pairs in its exclusion list.  Rows of width nmax; overflow raises *overflow to the
 //! largest count seen so the host can grow nmax and rebuild.
-__global__ void __launch_bounds__(256) k_nlist(unsigned int N, const double4* __restrict__ pos, md::CellGeom cg,
-                                               const unsigned int* __restrict__ start, const unsigned int* __restrict__ members,
-                                               double r_list2, const unsigned int* __restrict__ n_ex,
-                                               const unsigned int* __restrict__ ex_list, unsigned int ex_pitch, unsigned int nmax,
-                                               unsigned int* __restrict__ n_neigh, unsigned int* __restrict__ nlist,
-                                               uint64_t* __restrict__ head, unsigned int* __restrict__ overflow)
+//! Full neighbour list of particle i (r < r_list, exclusions left out) by the G lanes of its
+//! group: the group walks the 27 neighbour cells together, each lane tests every G-th member
+//! of a cell, and the accepted candidates are compacted with a ballot into consecutive slots of
+//! the particle's row, so stores are contiguous and G candidate loads are in flight.  Every
+//! lane of the group must call it; `active` (i < N) is the same for the whole group.
+constexpr int NLIST_LANES = 8;
+template<int G>
+__device__ __forceinline__ void nlist_group(unsigned int i, int sub, bool active, const double4* __restrict__ pos, const md::CellGeom& cg,
+                                            const unsigned int* __restrict__ start, const unsigned int* __restrict__ members,
+                                            double r_list2, const unsigned int* __restrict__ n_ex,
+                                            const unsigned int* __restrict__ ex_list, unsigned int ex_pitch, unsigned int nmax,
+                                            unsigned int* __restrict__ n_neigh, unsigned int* __restrict__ nlist,
+                                            uint64_t* __restrict__ head, unsigned int* __restrict__ overflow)
     {
-    const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N)
+    if (!active)
         return;
     const double4 ri = pos[i];
     const int cx = cell_coord(ri.x, cg.Lx, cg.nx), cy = cell_coord(ri.y, cg.Ly, cg.ny), cz = cell_coord(ri.z, cg.Lz, cg.nz);
     const double invLx = 1.0 / cg.Lx, invLy = 1.0 / cg.Ly, invLz = 1.0 / cg.Lz;
     const unsigned int ne = n_ex ? n_ex[i] : 0u;
+    const unsigned int lane = __lane_id();
+    const unsigned int gshift = lane & ~unsigned(G - 1);         // first lane of the group
+    const unsigned long long gmask = (G == 64) ? ~0ull : ((1ull << G) - 1ull);
+    const unsigned long long below = (1ull << unsigned(sub)) - 1ull;
+    unsigned int* row = nlist + size_t(i) * nmax;
     unsigned int k = 0;
     // neighbour cells, each visited once even when an axis has fewer than 3 cells
     const int rx = cg.nx >= 3 ? 1 : (cg.nx == 2 ? 1 : 0), ry = cg.ny >= 3 ? 1 : (cg.ny == 2 ? 1 : 0), rz = cg.nz >= 3 ? 1 : (cg.nz == 2 ? 1 : 0);
@@ -72,35 +83,55 @@ __global__ void __launch_bounds__(256) k_nlist(unsigned int N, const double4* __
                 {
                 const int x = (cx + dx + cg.nx) % cg.nx, y = (cy + dy + cg.ny) % cg.ny, z = (cz + dz + cg.nz) % cg.nz;
                 const unsigned int c = unsigned((z * cg.ny + y) * cg.nx + x);
-                for (unsigned int m = start[c]; m < start[c + 1]; ++m)
+                const unsigned int m0 = start[c], m1 = start[c + 1];
+                for (unsigned int m = m0; m < m1; m += unsigned(G))
                     {
-                    const unsigned int j = members[m];
-                    if (j == i)
-                        continue;
-                    const double4 rj = pos[j];
-                    const double ddx = wrap_min_image(ri.x - rj.x, cg.Lx, invLx);
-                    const double ddy = wrap_min_image(ri.y - rj.y, cg.Ly, invLy);
-                    const double ddz = wrap_min_image(ri.z - rj.z, cg.Lz, invLz);
-                    if (ddx * ddx + ddy * ddy + ddz * ddz >= r_list2)
-                        continue;
-                    bool excluded = false;
-                    for (unsigned int e = 0; e < ne; ++e)
-                        excluded |= (ex_list[size_t(i) * ex_pitch + e] == j);
-                    if (excluded)
-                        continue;
-                    if (k < nmax)
-                        nlist[size_t(i) * nmax + k] = j;
-                    ++k;
+                    const unsigned int mm = m + unsigned(sub);
+                    unsigned int j = 0;
+                    bool take = false;
+                    if (mm < m1)
+                        {
+                        j = members[mm];
+                        if (j != i)
+                            {
+                            const double4 rj = pos[j];
+                            const double ddx = wrap_min_image(ri.x - rj.x, cg.Lx, invLx);
+                            const double ddy = wrap_min_image(ri.y - rj.y, cg.Ly, invLy);
+                            const double ddz = wrap_min_image(ri.z - rj.z, cg.Lz, invLz);
+                            take = ddx * ddx + ddy * ddy + ddz * ddz < r_list2;
+                            for (unsigned int e = 0; take && e < ne; ++e)
+                                take = ex_list[size_t(i) * ex_pitch + e] != j;
+                            }
+                        }
+                    const unsigned long long accepted = (__ballot(take) >> gshift) & gmask;
+                    const unsigned int slot = k + unsigned(__popcll(accepted & below));
+                    if (take && slot < nmax)
+                        row[slot] = j;
+                    k += unsigned(__popcll(accepted));
                     }
                 }
-    n_neigh[i] = k < nmax ? k : nmax;
-    head[i] = uint64_t(i) * nmax;
-    if (k > nmax)
-        atomicMax(overflow, k);
+    if (sub == 0)
+        {
+        n_neigh[i] = k < nmax ? k : nmax;
+        head[i] = uint64_t(i) * nmax;
+        if (k > nmax)
+            atomicMax(overflow, k);
+        }
     }
 
-//! WCA (LJ truncated at 2^(1/6) sigma and shifted) of particle i over the full list; adds to
-//! its force/virial.
+__global__ void __launch_bounds__(256) k_nlist(unsigned int N, const double4* __restrict__ pos, md::CellGeom cg,
+                                               const unsigned int* __restrict__ start, const unsigned int* __restrict__ members,
+                                               double r_list2, const unsigned int* __restrict__ n_ex,
+                                               const unsigned int* __restrict__ ex_list, unsigned int ex_pitch, unsigned int nmax,
+                                               unsigned int* __restrict__ n_neigh, unsigned int* __restrict__ nlist,
+                                               uint64_t* __restrict__ head, unsigned int* __restrict__ overflow)
+    {
+    const unsigned int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned int i = t / NLIST_LANES;
+    nlist_group<NLIST_LANES>(i, int(t % NLIST_LANES), i < N, pos, cg, start, members, r_list2, n_ex, ex_list, ex_pitch, nmax,
+                             n_neigh, nlist, head, overflow);
+    }
+
 //! WCA force, energy and virial of particle i added to its outputs, by the G lanes of its group
 //! (`sub` = lane within the group) over the (longer) ESP neighbour list: each lane takes every
 //! G-th entry, so the group has G neighbour loads in flight (the list is mostly beyond the
@@ -430,7 +461,7 @@ hipError_t build_nlist(unsigned int N, const double* pos, const CellGeom& cg, co
     {
     if (N == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_nlist, dim3(nblocks(N)), dim3(256), 0, s, N, reinterpret_cast<const double4*>(pos), cg, start, members,
+    hipLaunchKernelGGL(k_nlist, dim3(nblocks(N * unsigned(dev::NLIST_LANES))), dim3(256), 0, s, N, reinterpret_cast<const double4*>(pos), cg, start, members,
                        r_list * r_list, n_ex, ex_list, ex_pitch, nmax, n_neigh, nlist, head, overflow);
     return hipGetLastError();
     }
