@@ -16,6 +16,17 @@ struct CellGeom
     double Lx, Ly, Lz;
     };
 
+//! Optional inner list written by the same neighbour-list pass: the entries of the full list
+//! with r^2 < r2 (r2 = 0: none), in rows of width nmax (HOOMD layout, head[i] = i * nmax).
+struct InnerList
+    {
+    double r2;
+    unsigned int nmax;
+    unsigned int* n;
+    unsigned int* list;
+    uint64_t* head;
+    };
+
 //! One system of a batched MD launch (device pointers; positions, velocities and forces are
 //! Scalar4 arrays).
 struct MDSys
@@ -31,6 +42,10 @@ struct MDSys
     const unsigned int* nlist;
     const uint64_t* head;
     const double* last_pos; // positions at the last list build (null: no displacement check)
+    const unsigned int* n_inner;  // inner list (null: none); valid for pair cutoffs <= r_inner
+    const unsigned int* inner;
+    const uint64_t* head_inner;
+    double r_inner;
     uint64_t seed;          // Langevin stream of this system
     };
 
@@ -53,7 +68,8 @@ hipError_t cell_fill(unsigned int N, const unsigned int* cell, const unsigned in
                      hipStream_t s);
 hipError_t build_nlist(unsigned int N, const double* pos, const CellGeom& cg, const unsigned int* start, const unsigned int* members,
                        double r_list, const unsigned int* n_ex, const unsigned int* ex_list, unsigned int ex_pitch, unsigned int nmax,
-                       unsigned int* n_neigh, unsigned int* nlist, uint64_t* head, unsigned int* overflow, hipStream_t s);
+                       unsigned int* n_neigh, unsigned int* nlist, uint64_t* head, unsigned int* overflow, const InnerList& inner,
+                       hipStream_t s);
 hipError_t wca_forces(unsigned int N, const double* pos, double Lx, double Ly, double Lz, const unsigned int* n_neigh,
                       const unsigned int* nlist, const uint64_t* head, double sigma, double epsilon, double* force, double* virial,
                       size_t pitch, hipStream_t s);

@@ -18,13 +18,17 @@ struct esp_nlist
     {
     double r_cut = 0.0;
     double r_buff = 0.0;
+    double r_inner = 0.0;      // 0: no inner list
     unsigned int nmax = 32;
+    unsigned int nmax_in = 16;
+    bool inner_built = false;
     unsigned int N = 0;
     unsigned int n_cells = 0;
     double box[3] = {0, 0, 0};
     bool built = false;
     unsigned long long builds = 0;
     esp::DeviceBuffer n_neigh, nlist, head, cell, slot, count, start, members, overflow, last_pos, disp;
+    esp::DeviceBuffer n_inner, inner, head_inner;
     std::string last_error;
     };
 
@@ -97,11 +101,28 @@ void do_build(esp_nlist* nl, const esp_system* sys, hipStream_t s)
         nl->N = N;
         nl->n_cells = nc;
         }
+    const bool want_inner = nl->r_inner > 0.0 && nl->r_inner < nl->r_cut;
+    if (want_inner)
+        {
+        nl->n_inner.resize(size_t(std::max(N, 1u)) * 4);
+        nl->head_inner.resize(size_t(std::max(N, 1u)) * 8);
+        }
+    nl->inner_built = false;
     for (int attempt = 0; attempt < 4; ++attempt)
         {
         nl->nlist.resize(size_t(std::max(N, 1u)) * nl->nmax * 4);
+        esp::md::InnerList in {};
+        if (want_inner)
+            {
+            nl->inner.resize(size_t(std::max(N, 1u)) * nl->nmax_in * 4);
+            in.r2 = (nl->r_inner + nl->r_buff) * (nl->r_inner + nl->r_buff);
+            in.nmax = nl->nmax_in;
+            in.n = nl->n_inner.as<unsigned int>();
+            in.list = nl->inner.as<unsigned int>();
+            in.head = nl->head_inner.as<uint64_t>();
+            }
         check_hip(hipMemsetAsync(nl->count.get(), 0, size_t(nc + 1) * 4, s), "hipMemsetAsync");
-        check_hip(hipMemsetAsync(nl->overflow.get(), 0, 4, s), "hipMemsetAsync");
+        check_hip(hipMemsetAsync(nl->overflow.get(), 0, 8, s), "hipMemsetAsync");
         check_hip(esp::md::cell_count(N, sys->postype, cg, nl->count.as<unsigned int>(), nl->cell.as<unsigned int>(),
                                       nl->slot.as<unsigned int>(), s),
                   "cell_count");
@@ -111,13 +132,14 @@ void do_build(esp_nlist* nl, const esp_system* sys, hipStream_t s)
                   "cell_fill");
         check_hip(esp::md::build_nlist(N, sys->postype, cg, nl->start.as<unsigned int>(), nl->members.as<unsigned int>(), r_list,
                                        sys->n_ex, sys->ex_list, sys->ex_pitch, nl->nmax, nl->n_neigh.as<unsigned int>(),
-                                       nl->nlist.as<unsigned int>(), nl->head.as<uint64_t>(), nl->overflow.as<unsigned int>(), s),
+                                       nl->nlist.as<unsigned int>(), nl->head.as<uint64_t>(), nl->overflow.as<unsigned int>(), in, s),
                   "build_nlist");
-        unsigned int over = 0;
-        check_hip(hipMemcpyAsync(&over, nl->overflow.get(), 4, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+        unsigned int over[2] = {0, 0};
+        check_hip(hipMemcpyAsync(over, nl->overflow.get(), 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
         check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
-        if (over == 0)
+        if (over[0] == 0 && over[1] == 0)
             {
+            nl->inner_built = want_inner;
             check_hip(esp::md::copy4(N, sys->postype, nl->last_pos.as<double>(), s), "copy4");
             nl->box[0] = sys->box[0];
             nl->box[1] = sys->box[1];
@@ -126,7 +148,10 @@ void do_build(esp_nlist* nl, const esp_system* sys, hipStream_t s)
             nl->builds += 1;
             return;
             }
-        nl->nmax = ((over + over / 4) + 7) & ~7u;
+        if (over[0])
+            nl->nmax = ((over[0] + over[0] / 4) + 7) & ~7u;
+        if (over[1])
+            nl->nmax_in = ((over[1] + over[1] / 4) + 7) & ~7u;
         }
     throw esp::EspError(ESP_ERR_STATE, "neighbour list did not converge");
     }
@@ -209,6 +234,20 @@ void md_batch_refresh(esp_md_batch* b, unsigned int i)
         d.nlist = nullptr;
         d.head = nullptr;
         d.last_pos = nullptr;
+        }
+    if (nl && nl->built && nl->inner_built)
+        {
+        d.n_inner = nl->n_inner.as<unsigned int>();
+        d.inner = nl->inner.as<unsigned int>();
+        d.head_inner = nl->head_inner.as<uint64_t>();
+        d.r_inner = nl->r_inner;
+        }
+    else
+        {
+        d.n_inner = nullptr;
+        d.inner = nullptr;
+        d.head_inner = nullptr;
+        d.r_inner = 0.0;
         }
     }
 
@@ -431,6 +470,35 @@ int esp_nlist_arrays(esp_nlist* nl, const unsigned int** n_neigh, const unsigned
         *head_list = nl->head.as<uint64_t>();
     if (nmax)
         *nmax = nl->nmax;
+    return ESP_OK;
+    }
+
+int esp_nlist_set_inner(esp_nlist* nl, double r_inner)
+    {
+    if (!nl || !(r_inner >= 0.0))
+        return fail(nl, ESP_ERR_ARG, "esp_nlist_set_inner: r_inner must be >= 0");
+    if (r_inner != nl->r_inner)
+        {
+        nl->r_inner = r_inner;
+        nl->built = false;
+        nl->inner_built = false;
+        }
+    return ESP_OK;
+    }
+
+int esp_nlist_inner_arrays(esp_nlist* nl, const unsigned int** n_neigh, const unsigned int** nlist, const uint64_t** head_list,
+                           unsigned int* nmax)
+    {
+    if (!nl || !nl->built || !nl->inner_built)
+        return fail(nl, ESP_ERR_STATE, "esp_nlist_inner_arrays: no inner list built (esp_nlist_set_inner with 0 < r_inner < r_cut)");
+    if (n_neigh)
+        *n_neigh = nl->n_inner.as<unsigned int>();
+    if (nlist)
+        *nlist = nl->inner.as<unsigned int>();
+    if (head_list)
+        *head_list = nl->head_inner.as<uint64_t>();
+    if (nmax)
+        *nmax = nl->nmax_in;
     return ESP_OK;
     }
 

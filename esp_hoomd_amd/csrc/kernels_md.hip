@@ -52,8 +52,11 @@ __global__ void __launch_bounds__(256) k_cell_fill(unsigned int N, const unsigne
 //! Full neighbour list of particle i (r < r_list, exclusions left out) by the G lanes of its
 //! group: the group walks the 27 neighbour cells together, each lane tests every G-th member
 //! of a cell, and the accepted candidates are compacted with a ballot into consecutive slots of
-//! the particle's row, so stores are contiguous and G candidate loads are in flight.  Every
-//! lane of the group must call it; `active` (i < N) is the same for the whole group.
+//! the particle's row, so stores are contiguous and G candidate loads are in flight.  With
+//! r_inner2 > 0 the same pass also writes the inner list (the entries with r^2 < r_inner2) to
+//! rows of width nmax_in; overflow[0] / overflow[1] collect the largest full / inner counts
+//! that did not fit.  Every lane of the group must call it; `active` (i < N) is the same for
+//! the whole group.
 constexpr int NLIST_LANES = 8;
 template<int G>
 __device__ __forceinline__ void nlist_group(unsigned int i, int sub, bool active, const double4* __restrict__ pos, const md::CellGeom& cg,
@@ -61,7 +64,7 @@ __device__ __forceinline__ void nlist_group(unsigned int i, int sub, bool active
                                             double r_list2, const unsigned int* __restrict__ n_ex,
                                             const unsigned int* __restrict__ ex_list, unsigned int ex_pitch, unsigned int nmax,
                                             unsigned int* __restrict__ n_neigh, unsigned int* __restrict__ nlist,
-                                            uint64_t* __restrict__ head, unsigned int* __restrict__ overflow)
+                                            uint64_t* __restrict__ head, unsigned int* __restrict__ overflow, const md::InnerList& in)
     {
     if (!active)
         return;
@@ -74,7 +77,8 @@ __device__ __forceinline__ void nlist_group(unsigned int i, int sub, bool active
     const unsigned long long gmask = (G == 64) ? ~0ull : ((1ull << G) - 1ull);
     const unsigned long long below = (1ull << unsigned(sub)) - 1ull;
     unsigned int* row = nlist + size_t(i) * nmax;
-    unsigned int k = 0;
+    unsigned int* row_in = in.list + size_t(i) * in.nmax;
+    unsigned int k = 0, k_in = 0;
     // neighbour cells, each visited once even when an axis has fewer than 3 cells
     const int rx = cg.nx >= 3 ? 1 : (cg.nx == 2 ? 1 : 0), ry = cg.ny >= 3 ? 1 : (cg.ny == 2 ? 1 : 0), rz = cg.nz >= 3 ? 1 : (cg.nz == 2 ? 1 : 0);
     for (int dz = -rz; dz <= (cg.nz == 2 ? 0 : rz); ++dz)
@@ -89,6 +93,7 @@ __device__ __forceinline__ void nlist_group(unsigned int i, int sub, bool active
                     const unsigned int mm = m + unsigned(sub);
                     unsigned int j = 0;
                     bool take = false;
+                    double r2 = 0.0;
                     if (mm < m1)
                         {
                         j = members[mm];
@@ -98,7 +103,8 @@ __device__ __forceinline__ void nlist_group(unsigned int i, int sub, bool active
                             const double ddx = wrap_min_image(ri.x - rj.x, cg.Lx, invLx);
                             const double ddy = wrap_min_image(ri.y - rj.y, cg.Ly, invLy);
                             const double ddz = wrap_min_image(ri.z - rj.z, cg.Lz, invLz);
-                            take = ddx * ddx + ddy * ddy + ddz * ddz < r_list2;
+                            r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                            take = r2 < r_list2;
                             for (unsigned int e = 0; take && e < ne; ++e)
                                 take = ex_list[size_t(i) * ex_pitch + e] != j;
                             }
@@ -108,6 +114,15 @@ __device__ __forceinline__ void nlist_group(unsigned int i, int sub, bool active
                     if (take && slot < nmax)
                         row[slot] = j;
                     k += unsigned(__popcll(accepted));
+                    if (in.r2 > 0.0)
+                        {
+                        const bool inner = take && r2 < in.r2;
+                        const unsigned long long acc_in = (__ballot(inner) >> gshift) & gmask;
+                        const unsigned int slot_in = k_in + unsigned(__popcll(acc_in & below));
+                        if (inner && slot_in < in.nmax)
+                            row_in[slot_in] = j;
+                        k_in += unsigned(__popcll(acc_in));
+                        }
                     }
                 }
     if (sub == 0)
@@ -116,6 +131,13 @@ __device__ __forceinline__ void nlist_group(unsigned int i, int sub, bool active
         head[i] = uint64_t(i) * nmax;
         if (k > nmax)
             atomicMax(overflow, k);
+        if (in.r2 > 0.0)
+            {
+            in.n[i] = k_in < in.nmax ? k_in : in.nmax;
+            in.head[i] = uint64_t(i) * in.nmax;
+            if (k_in > in.nmax)
+                atomicMax(overflow + 1, k_in);
+            }
         }
     }
 
@@ -124,12 +146,12 @@ __global__ void __launch_bounds__(256) k_nlist(unsigned int N, const double4* __
                                                double r_list2, const unsigned int* __restrict__ n_ex,
                                                const unsigned int* __restrict__ ex_list, unsigned int ex_pitch, unsigned int nmax,
                                                unsigned int* __restrict__ n_neigh, unsigned int* __restrict__ nlist,
-                                               uint64_t* __restrict__ head, unsigned int* __restrict__ overflow)
+                                               uint64_t* __restrict__ head, unsigned int* __restrict__ overflow, md::InnerList in)
     {
     const unsigned int t = blockIdx.x * blockDim.x + threadIdx.x;
     const unsigned int i = t / NLIST_LANES;
     nlist_group<NLIST_LANES>(i, int(t % NLIST_LANES), i < N, pos, cg, start, members, r_list2, n_ex, ex_list, ex_pitch, nmax,
-                             n_neigh, nlist, head, overflow);
+                             n_neigh, nlist, head, overflow, in);
     }
 
 //! WCA force, energy and virial of particle i added to its outputs, by the G lanes of its group
@@ -298,6 +320,28 @@ __global__ void __launch_bounds__(256) k_vv_second(unsigned int N, double4* __re
         vv_second_one(i, vel, force, dt, gamma, kT, seed, step);
     }
 
+//! Raises *out (the bits of a non-negative double, monotone in unsigned comparison) to the
+//! workgroup's largest d2: wave maxima through LDS, then one atomic per workgroup, skipped when
+//! the value already there is at least as large (values only grow, so a stale read only costs
+//! an unneeded atomic).  Every thread of the workgroup must call it.
+__device__ __forceinline__ void block_max_to(double d2, unsigned long long* out)
+    {
+    __shared__ double wave_max[4];
+    for (int off = 32; off > 0; off >>= 1)
+        d2 = fmax(d2, __shfl_xor(d2, off, 64));
+    if ((threadIdx.x & 63) == 0)
+        wave_max[threadIdx.x >> 6] = d2;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        {
+        for (unsigned int w = 1; w < (blockDim.x >> 6); ++w)
+            d2 = fmax(d2, wave_max[w]);
+        const unsigned long long bits = static_cast<unsigned long long>(__double_as_longlong(d2));
+        if (bits > *static_cast<volatile unsigned long long*>(out))
+            atomicMax(out, bits);
+        }
+    }
+
 //! Largest squared minimum-image displacement since the last neighbour-list build, as the bits
 //! of a non-negative double (monotone in unsigned comparison).
 __global__ void __launch_bounds__(256) k_max_disp(unsigned int N, const double4* __restrict__ pos, const double4* __restrict__ last,
@@ -313,11 +357,7 @@ __global__ void __launch_bounds__(256) k_max_disp(unsigned int N, const double4*
         const double dz = wrap_min_image(a.z - b.z, Lz, 1.0 / Lz);
         d2 = dx * dx + dy * dy + dz * dz;
         }
-    // wave maximum, then one atomic per wave
-    for (int off = 32; off > 0; off >>= 1)
-        d2 = fmax(d2, __shfl_xor(d2, off, 64));
-    if ((threadIdx.x & 63) == 0)
-        atomicMax(out, static_cast<unsigned long long>(__double_as_longlong(d2)));
+    block_max_to(d2, out);
     }
 
 __global__ void __launch_bounds__(256) k_copy4(unsigned int N, const double4* __restrict__ src, double4* __restrict__ dst)
@@ -392,13 +432,19 @@ __global__ void __launch_bounds__(256) kb_wca(const md::MDSys* __restrict__ ds, 
     const md::MDSys& d = ds[w.sys];
     if (d.nlist == nullptr)
         return;
+    // the inner list when it covers the WCA cutoff 2^(1/6) sigma (it holds every pair the full
+    // list holds within r_inner + buffer, so it stays valid as long as the full list does)
+    const bool use_inner = d.inner != nullptr && 1.122462048309373 * sigma <= d.r_inner;
+    const unsigned int* n_neigh = use_inner ? d.n_inner : d.n_neigh;
+    const unsigned int* nlist = use_inner ? d.inner : d.nlist;
+    const uint64_t* head = use_inner ? d.head_inner : d.head;
     // the item's 256 particles in WCA_LANES passes of 256 / WCA_LANES particle groups
     constexpr unsigned int PER_PASS = 256u / unsigned(WCA_LANES);
     for (unsigned int pass = 0; pass < unsigned(WCA_LANES); ++pass)
         {
         const unsigned int i = w.chunk * 256u + pass * PER_PASS + threadIdx.x / unsigned(WCA_LANES);
         wca_group<WCA_LANES>(i, int(threadIdx.x % WCA_LANES), i < d.N, reinterpret_cast<const double4*>(d.postype), d.L[0], d.L[1],
-                             d.L[2], d.n_neigh, d.nlist, d.head, sigma, epsilon, reinterpret_cast<double4*>(d.force), d.virial, d.pitch);
+                             d.L[2], n_neigh, nlist, head, sigma, epsilon, reinterpret_cast<double4*>(d.force), d.virial, d.pitch);
         }
     }
 
@@ -420,10 +466,7 @@ __global__ void __launch_bounds__(256) kb_max_disp(const md::MDSys* __restrict__
         const double dz = wrap_min_image(a.z - b.z, d.L[2], 1.0 / d.L[2]);
         d2 = dx * dx + dy * dy + dz * dz;
         }
-    for (int off = 32; off > 0; off >>= 1)
-        d2 = fmax(d2, __shfl_xor(d2, off, 64));
-    if ((threadIdx.x & 63) == 0)
-        atomicMax(out + w.sys, static_cast<unsigned long long>(__double_as_longlong(d2)));
+    block_max_to(d2, out + w.sys);
     }
 
     } // namespace dev
@@ -457,12 +500,15 @@ hipError_t cell_fill(unsigned int N, const unsigned int* cell, const unsigned in
 
 hipError_t build_nlist(unsigned int N, const double* pos, const CellGeom& cg, const unsigned int* start, const unsigned int* members,
                        double r_list, const unsigned int* n_ex, const unsigned int* ex_list, unsigned int ex_pitch, unsigned int nmax,
-                       unsigned int* n_neigh, unsigned int* nlist, uint64_t* head, unsigned int* overflow, hipStream_t s)
+                       unsigned int* n_neigh, unsigned int* nlist, uint64_t* head, unsigned int* overflow, const InnerList& inner,
+                       hipStream_t s)
     {
     if (N == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_nlist, dim3(nblocks(N * unsigned(dev::NLIST_LANES))), dim3(256), 0, s, N, reinterpret_cast<const double4*>(pos), cg, start, members,
-                       r_list * r_list, n_ex, ex_list, ex_pitch, nmax, n_neigh, nlist, head, overflow);
+    if (inner.r2 > 0.0 && (!inner.n || !inner.list || !inner.head || inner.nmax == 0))
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_nlist, dim3(nblocks(N * unsigned(dev::NLIST_LANES))), dim3(256), 0, s, N, reinterpret_cast<const double4*>(pos), cg,
+                       start, members, r_list * r_list, n_ex, ex_list, ex_pitch, nmax, n_neigh, nlist, head, overflow, inner);
     return hipGetLastError();
     }
 

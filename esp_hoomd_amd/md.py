@@ -48,6 +48,7 @@ class State:
             self.ex_list = DeviceArray.from_numpy(ex_list)
             self.ex_pitch = pitch
         self.nlist_ptrs = None  # (n_neigh, nlist, head) device pointers of the current list
+        self.inner_ptrs = None  # (n_neigh, nlist, head, r_inner) of the current inner list, if any
 
     @property
     def volume(self) -> float:
@@ -100,12 +101,15 @@ class State:
 
 class NeighborList:
     """GPU cell-list neighbour list (full list, HOOMD layout) with a skin `buffer`; rebuilt when
-    a particle has moved more than buffer / 2 since the last build."""
+    a particle has moved more than buffer / 2 since the last build.  With `inner` > 0 each build
+    also keeps the pairs within inner + buffer in a second list, which short-ranged forces
+    (WCA) with a cutoff <= inner walk instead of the full list."""
 
-    def __init__(self, buffer: float = 0.4):
-        if buffer < 0:
-            raise ValueError("NeighborList: buffer must be >= 0")
+    def __init__(self, buffer: float = 0.4, inner: float = 0.0):
+        if buffer < 0 or inner < 0:
+            raise ValueError("NeighborList: buffer and inner must be >= 0")
         self.buffer = float(buffer)
+        self.inner = float(inner)
         self.r_cut = 0.0
         self._nl = None
 
@@ -118,6 +122,32 @@ class NeighborList:
             _lib.check(lib.esp_nlist_create(C.byref(nl), r_cut, self.buffer))
             self._nl = nl
             self.r_cut = r_cut
+            if self.inner > 0:
+                _lib.check(lib.esp_nlist_set_inner(self._nl, self.inner))
+
+    def set_inner(self, inner: float) -> None:
+        """Change the inner radius (0: none); the next update rebuilds."""
+        if inner < 0:
+            raise ValueError("NeighborList: inner must be >= 0")
+        if float(inner) != self.inner:
+            self.inner = float(inner)
+            if self._nl is not None:
+                _lib.check(_lib.load().esp_nlist_set_inner(self._nl, self.inner))
+
+    def _publish(self, state: "State") -> None:
+        """Point the state at the current list (and inner list) arrays."""
+        lib = _lib.load()
+        a, b, c = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        nmax = C.c_uint()
+        _lib.check(lib.esp_nlist_arrays(self._nl, C.byref(a), C.byref(b), C.byref(c), C.byref(nmax)))
+        state.nlist_ptrs = (a.value, b.value, c.value)
+        self.nmax = nmax.value
+        state.inner_ptrs = None
+        self.nmax_inner = 0
+        if 0 < self.inner < self.r_cut:
+            _lib.check(lib.esp_nlist_inner_arrays(self._nl, C.byref(a), C.byref(b), C.byref(c), C.byref(nmax)))
+            state.inner_ptrs = (a.value, b.value, c.value, self.inner)
+            self.nmax_inner = nmax.value
 
     def update(self, state: State, r_cut: float, force_rebuild: bool = False) -> bool:
         lib = _lib.load()
@@ -131,11 +161,7 @@ class NeighborList:
             rc = lib.esp_nlist_update(self._nl, C.byref(s), C.byref(rebuilt), None)
         if rc != _lib.ESP_OK:
             raise _lib.EspError(rc, lib.esp_nlist_last_error(self._nl).decode())
-        a, b, c = C.c_void_p(), C.c_void_p(), C.c_void_p()
-        nmax = C.c_uint()
-        _lib.check(lib.esp_nlist_arrays(self._nl, C.byref(a), C.byref(b), C.byref(c), C.byref(nmax)))
-        state.nlist_ptrs = (a.value, b.value, c.value)
-        self.nmax = nmax.value
+        self._publish(state)
         return bool(rebuilt.value)
 
     @property
@@ -144,16 +170,18 @@ class NeighborList:
         _lib.check(_lib.load().esp_nlist_stats(self._nl, C.byref(n)))
         return int(n.value)
 
-    def host_arrays(self, state: State):
-        """(n_neigh, [neighbour indices per particle]) copied to the host (for tests)."""
+    def host_arrays(self, state: State, inner: bool = False):
+        """(n_neigh, [neighbour indices per particle]) of the list (or the inner list) copied to
+        the host (for tests)."""
         import ctypes
+        ptrs, width = (state.inner_ptrs, self.nmax_inner) if inner else (state.nlist_ptrs, self.nmax)
+        if ptrs is None:
+            raise RuntimeError("NeighborList.host_arrays: no such list")
         n = np.empty(state.N, np.uint32)
-        _lib.check(_lib.load().esp_memcpy_dtoh(n.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(state.nlist_ptrs[0]),
-                                               n.nbytes))
-        flat = np.empty(state.N * self.nmax, np.uint32)
-        _lib.check(_lib.load().esp_memcpy_dtoh(flat.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(state.nlist_ptrs[1]),
-                                               flat.nbytes))
-        flat = flat.reshape(state.N, self.nmax)
+        _lib.check(_lib.load().esp_memcpy_dtoh(n.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(ptrs[0]), n.nbytes))
+        flat = np.empty(state.N * width, np.uint32)
+        _lib.check(_lib.load().esp_memcpy_dtoh(flat.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(ptrs[1]), flat.nbytes))
+        flat = flat.reshape(state.N, width)
         return n, [flat[i, : n[i]] for i in range(state.N)]
 
     def __del__(self):
@@ -177,6 +205,8 @@ class WCA:
 
     def add_to(self, state: State, stream=None) -> None:
         s = state.c_system()
+        if state.inner_ptrs is not None and self.r_cut <= state.inner_ptrs[3]:
+            s.n_neigh, s.nlist, s.head_list = state.inner_ptrs[:3]
         o = state.c_out()
         _lib.check(_lib.load().esp_wca_forces(C.byref(s), self.sigma, self.epsilon, C.byref(o),
                                               stream.handle if stream else None))
@@ -229,6 +259,9 @@ class Simulation:
     def compute_forces(self) -> None:
         st = self.state
         if self.forces:
+            # WCA walks an inner list when a longer-ranged force sets the list's cutoff
+            wca = [f.r_cut for f in self.forces if isinstance(f, WCA)]
+            self.nlist.set_inner(max(wca) if wca and max(wca) < self._r_cut() else 0.0)
             self.nlist.update(st, self._r_cut())
         # the ESP objects (Spectral mesh part, Local pair part) share one solver: one call with
         # the union of their flags overwrites the net force; other forces then add to it
@@ -353,14 +386,9 @@ class BatchMD:
         which = (C.c_ubyte * max(n, 1))()
         self._check(_lib.load().esp_md_batch_update_nlists(self._b, C.byref(rebuilt), which, stream.handle if stream else None))
         out = [i for i in range(n) if which[i]]
-        lib = _lib.load()
         for i in out:
             st, nl = self.members[i]
-            a, b, c = C.c_void_p(), C.c_void_p(), C.c_void_p()
-            nmax = C.c_uint()
-            _lib.check(lib.esp_nlist_arrays(nl._nl, C.byref(a), C.byref(b), C.byref(c), C.byref(nmax)))
-            st.nlist_ptrs = (a.value, b.value, c.value)
-            nl.nmax = nmax.value
+            nl._publish(st)
         return out
 
     def __del__(self):

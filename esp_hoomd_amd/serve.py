@@ -104,11 +104,12 @@ class Request:
 def footprint_bytes(N: int, params: Params, neighbours: float) -> int:
     """Device bytes a request holds while resident: particle arrays (position, charge,
     velocity, force, virial: 152 B), neighbour list (4 B per entry at 1.3x the mean count,
-    plus counts and heads), sorting scratch (~48 B), and the solver's meshes and tables."""
+    plus counts and heads), the WCA inner list (16 entries, count and head: 76 B), sorting
+    scratch (~48 B), and the solver's meshes and tables."""
     M = params.nx * params.ny * params.nz
     Mh = (params.nx // 2 + 1) * params.ny * params.nz
     R = 8 if params.precision == _lib.ESP_PREC_FP64 else 4
-    per_particle = 152 + 48 + 12 + 4 * 1.3 * neighbours
+    per_particle = 152 + 48 + 12 + 76 + 4 * 1.3 * neighbours
     mesh = M * 8 + 2 * Mh * R + M * R + 2 * Mh * 8
     return int(N * per_particle + mesh + (1 << 20))
 
@@ -191,7 +192,7 @@ class Server:
             st = State(pos, q, [L, L, L])
             st.thermalize(req.kT, seed=req.seed + 1)
             req.state = st
-            req.nlist = NeighborList(buffer=SKIN)
+            req.nlist = NeighborList(buffer=SKIN, inner=self.wca.r_cut)
             req.nlist.update(st, max(params.rcut, self.wca.r_cut), force_rebuild=True)
             req.solver = Solver(params)
             req.admitted_at = now

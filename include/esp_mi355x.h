@@ -224,6 +224,15 @@ int esp_nlist_update(esp_nlist* nl, const esp_system* sys, int* rebuilt, esp_str
 int esp_nlist_arrays(esp_nlist* nl, const unsigned int** n_neigh, const unsigned int** nlist, const uint64_t** head_list,
                      unsigned int* nmax);
 int esp_nlist_stats(esp_nlist* nl, unsigned long long* builds);
+/* Inner list (no counterpart in the reference, where a HOOMD list is built at the largest cutoff
+ * of all pair forces and each force walks all of it): with r_inner > 0 (and below r_cut) every
+ * build also keeps, in its own rows, the pairs within r_inner + r_buff, which is a valid list for
+ * any pair force with cutoff <= r_inner under the same rebuild rule (the batched WCA uses it);
+ * 0 turns it off.  Takes effect at the next build (the next update rebuilds). */
+int esp_nlist_set_inner(esp_nlist* nl, double r_inner);
+/* device arrays of the current inner list (HOOMD layout); ESP_ERR_STATE if there is none */
+int esp_nlist_inner_arrays(esp_nlist* nl, const unsigned int** n_neigh, const unsigned int** nlist, const uint64_t** head_list,
+                           unsigned int* nmax);
 /* WCA repulsion (LJ truncated at 2^(1/6) sigma, shifted) added to out->force / out->virial */
 int esp_wca_forces(const esp_system* sys, double sigma, double epsilon, esp_out* out, esp_stream_t stream);
 /* velocity Verlet: vel = (vx, vy, vz, mass) per particle; box = (Lx, Ly, Lz) */

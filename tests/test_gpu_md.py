@@ -15,12 +15,13 @@ def _kdtree_pairs(pos, L, r):
     return {tuple(p) for p in t.query_pairs(r, output_type="ndarray")}
 
 
-@pytest.mark.parametrize("N,L,r_cut,buf", [(500, 10.0, 2.5, 0.3), (300, 7.0, 3.0, 0.4), (64, 6.5, 1.2, 0.0)])
-def test_nlist_matches_kdtree(N, L, r_cut, buf):
+@pytest.mark.parametrize("N,L,r_cut,buf,inner", [(500, 10.0, 2.5, 0.3, 1.2), (300, 7.0, 3.0, 0.4, 0.0), (64, 6.5, 1.2, 0.0, 0.5),
+                                                 (2000, 10.0, 2.5, 0.4, 1.5)])
+def test_nlist_matches_kdtree(N, L, r_cut, buf, inner):
     pos, q = random_electrolyte(N, L, seed=N)
     excl = [(0, 1), (5, 9)]
     st = md.State(pos, q, (L, L, L), exclusions=excl)
-    nl = md.NeighborList(buffer=buf)
+    nl = md.NeighborList(buffer=buf, inner=inner)
     nl.update(st, r_cut, force_rebuild=True)
     n, lists = nl.host_arrays(st)
     got = {(min(i, int(j)), max(i, int(j))) for i in range(N) for j in lists[i]}
@@ -29,6 +30,43 @@ def test_nlist_matches_kdtree(N, L, r_cut, buf):
     assert all(i not in set(lists[i].tolist()) for i in range(N))
     # symmetric full list
     assert sum(len(x) for x in lists) == 2 * len(exp)
+    if inner == 0.0:
+        assert st.inner_ptrs is None
+        return
+    # the inner list: the same pass, the pairs within inner + buffer
+    n_in, lists_in = nl.host_arrays(st, inner=True)
+    got_in = {(min(i, int(j)), max(i, int(j))) for i in range(N) for j in lists_in[i]}
+    assert got_in == _kdtree_pairs(pos, L, inner + buf) - {(0, 1), (5, 9)}
+    assert sum(len(x) for x in lists_in) == 2 * len(got_in)
+
+
+def test_wca_over_inner_list_equals_full_list():
+    """WCA summed over the inner list (single-system and batched launches) against the same
+    force over the full list; only the summation order differs."""
+    L, N = 12.0, 800
+    pos = md.random_packing(N, L, 0.85, seed=5)
+    out = {}
+    for inner in (0.0, 1.2):
+        st = md.State(pos, np.zeros(N), (L, L, L))
+        nl = md.NeighborList(0.3, inner=inner)
+        nl.update(st, 3.0, force_rebuild=True)
+        assert (st.inner_ptrs is not None) == (inner > 0)
+        st.force.zero()
+        st.virial.zero()
+        md.WCA(1.0, 1.0).add_to(st)
+        single = (st.force.numpy().copy(), st.virial.numpy().copy())
+        st.force.zero()
+        st.virial.zero()
+        b = md.BatchMD()
+        b.set([(st, nl)])
+        b.wca(1.0, 1.0)
+        out[inner] = (single, (st.force.numpy().copy(), st.virial.numpy().copy()))
+    full = out[0.0][0]
+    assert np.abs(full[0][:, :3]).max() > 1.0          # the packing has overlapping WCA shells
+    for f, v in out[1.2]:
+        assert rel_rms(f, full[0]) < 1e-13
+        assert rel_rms(v, full[1]) < 1e-13
+    assert rel_rms(out[0.0][1][0], full[0]) < 1e-13
 
 
 def test_wca_matches_numpy():
