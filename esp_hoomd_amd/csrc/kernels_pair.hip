@@ -90,14 +90,14 @@ __device__ __forceinline__ void rec_separation(const typename PairRec<R>::type& 
 //! S and dS/ds from the compact table: segment = floor(s * NSEG), z = 2 t - 1.  The degree is
 //! fixed at compile time so the row is fetched with 16-byte LDS loads and Horner is unrolled.
 template<typename R>
-__device__ __forceinline__ void compact_split(const R* __restrict__ tab, int nseg, R s, R& S, R& dS)
+__device__ __forceinline__ void compact_split(const R* __restrict__ tab, int nseg, R s, R& S, R& dSdz)
     {
     constexpr int NC = int(kernel::pair_compact_deg<R>()) + 1;
     constexpr int PER16 = 16 / int(sizeof(R));
     static_assert(NC % PER16 == 0, "compact rows must be whole 16-byte words");
     const R u = s * R(nseg);
-    int seg = int(u);
-    seg = seg < 0 ? 0 : (seg >= nseg ? nseg - 1 : seg);
+    // s >= 0, so only the top end (s = 1, or s a rounding step above it) needs the clamp
+    const int seg = min(int(u), nseg - 1);
     const R z = R(2) * (u - R(seg)) - R(1);
     R c[NC];
     const uint4* row = reinterpret_cast<const uint4*>(tab + seg * NC);
@@ -115,15 +115,17 @@ __device__ __forceinline__ void compact_split(const R* __restrict__ tab, int nse
                 c[w * PER16 + k] = __hiloint2double(int(bits[2 * k + 1]), int(bits[2 * k]));
             }
         }
-    R v = c[NC - 1], d = R(0);
+    // value and derivative by one Horner pass (its first derivative step, 0 * z + c[NC-1], peeled)
+    R d = c[NC - 1];
+    R v = d * z + c[NC - 2];
 #pragma unroll
-    for (int j = NC - 2; j >= 0; --j)
+    for (int j = NC - 3; j >= 0; --j)
         {
         d = d * z + v;
         v = v * z + c[j];
         }
     S = v;
-    dS = d * R(2 * nseg);  // dS/ds = dS/dz * dz/ds
+    dSdz = d;  // dS/ds = dS/dz * 2 nseg
     }
 
 //! r and 1/r from r^2: exact square root and division in FP64; the hardware reciprocal square
@@ -133,9 +135,8 @@ __device__ __forceinline__ void r_and_rinv(R rsq, R& r, R& rinv)
     {
     if constexpr (sizeof(R) == 4)
         {
-        // v_rsq_f32 without the denormal-input scaling; a zero distance gives r = 0 (which the
-        // r < 1e-12 test then rejects), not 0 * inf
-        rinv = __builtin_amdgcn_rsqf(fmaxf(rsq, 1.0e-30f));
+        // v_rsq_f32 without the denormal-input scaling (callers pass rsq >= 1e-24)
+        rinv = __builtin_amdgcn_rsqf(rsq);
         r = rsq * rinv;
         }
     else
@@ -145,7 +146,10 @@ __device__ __forceinline__ void r_and_rinv(R rsq, R& r, R& rinv)
         }
     }
 
-//! L(r) and (-dL/dr) / r for 0 < r <= rc (zero outside), given rinv = 1 / r.
+//! L(r) and (-dL/dr) / r given rinv = 1 / r.  The compact form expects 1e-12 <= r and
+//! r^2 < rc^2 (r may round to an ulp above rc: the last segment's polynomial is then taken a
+//! rounding step past its end), which its callers test; the segment-table form returns zero
+//! outside (1e-12, rc].
 template<typename R, bool COMPACT>
 __device__ __forceinline__ void pair_terms(const R* __restrict__ tab,
                                            const kernel::PairArgs& a,
@@ -158,17 +162,12 @@ __device__ __forceinline__ void pair_terms(const R* __restrict__ tab,
     {
     if constexpr (COMPACT)
         {
-        if (r < R(1.0e-12) || r > rc)
-            {
-            L = R(0);
-            ndr = R(0);
-            return;
-            }
-        R S, dS;
-        compact_split<R>(tab, int(a.n_compact), r * inv_rc, S, dS);
+        R S, dSdz;
+        compact_split<R>(tab, int(a.n_compact), r * inv_rc, S, dSdz);
+        const R dz_dr = R(2 * int(a.n_compact)) * inv_rc;  // loop invariant
         const R inv4pir = R(0.079577471545947673) * rinv;
         L = S * inv4pir;
-        ndr = (S * rinv - dS * inv_rc) * inv4pir * rinv;
+        ndr = (S * rinv - dSdz * dz_dr) * inv4pir * rinv;
         }
     else
         {
@@ -216,7 +215,8 @@ __device__ __forceinline__ void pair_particle(unsigned int i, int sub, const ker
             R dx, dy, dz, qiqj;
             rec_separation<R>(pi, pj, a, dx, dy, dz, qiqj);
             const R rsq = dx * dx + dy * dy + dz * dz;
-            if (rsq >= rc2 || qiqj == R(0))
+            // outside the cutoff, uncharged, or coincident (r < 1e-12: no defined force)
+            if (!(rsq < rc2 && rsq >= R(1.0e-24)) || qiqj == R(0))
                 return;
             R r, rinv;
             r_and_rinv<R>(rsq, r, rinv);
@@ -267,7 +267,7 @@ __device__ __forceinline__ void pair_particle(unsigned int i, int sub, const ker
             R r, rinv;
             r_and_rinv<R>(rsq, r, rinv);
             R Lr = R(0), ndr = R(0);
-            if (r <= rc)
+            if (rsq < rc2)
                 pair_terms<R, COMPACT>(tab, a, r, rinv, rc, inv_rc, Lr, ndr);
             const R inv4pir = R(0.079577471545947673) * rinv;
             accumulate(qiqj * (ndr - inv4pir * rinv * rinv), qiqj * (Lr - inv4pir), dx, dy, dz);
