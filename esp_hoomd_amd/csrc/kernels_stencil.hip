@@ -415,7 +415,10 @@ __device__ __forceinline__ void eval_taps(const R* __restrict__ wc, int D, R t, 
         }
     }
 
-template<typename R, int P>
+//! LPP lanes per particle (1 or 2): with 2, lane `sub` of the pair adds rows b in
+//! [sub P/2, (sub+1) P/2), so a tile's particles fill twice as many lanes (the LDS-atomic
+//! stream of the tile is spread over more waves).  The fixed-point sums do not depend on it.
+template<typename R, int P, int LPP = 1>
 __device__ __forceinline__ void spread_tile_tp(unsigned int tile,
                                                const MeshGeom& g,
                                                int D,
@@ -438,8 +441,13 @@ __device__ __forceinline__ void spread_tile_tp(unsigned int tile,
         box[c] = L(0);
     __syncthreads();
 
-    for (unsigned int p = s0 + threadIdx.x; p < s1; p += blockDim.x)
+    static_assert(LPP == 1 || (LPP == 2 && P % 2 == 0), "1 or 2 lanes per particle (even P)");
+    constexpr int PB = P / LPP;  // rows per lane
+    const unsigned int n_items = (s1 - s0) * unsigned(LPP);
+    for (unsigned int t = threadIdx.x; t < n_items; t += blockDim.x)
         {
+        const unsigned int p = s0 + t / unsigned(LPP);
+        const int sub = int(t % unsigned(LPP));
         const SortedRec<R> r = rec[p];
         const unsigned int loc = sidx[p].local;
         const int ox = loc & 0xff, oy = (loc >> 8) & 0xff, oz = loc >> 16;
@@ -447,15 +455,20 @@ __device__ __forceinline__ void spread_tile_tp(unsigned int tile,
         eval_taps<R, P, false>(wcoef, D, r.tx, wx, unused);
         eval_taps<R, P, false>(wcoef, D, r.ty, wy, unused);
         eval_taps<R, P, false>(wcoef, D, r.tz, wz, unused);
-        L* base = box + (oz * B + oy) * B + ox;
+        // this lane's rows (a select, not a run-time register index)
+        R wyl[PB];
+#pragma unroll
+        for (int bb = 0; bb < PB; ++bb)
+            wyl[bb] = (LPP == 2 && sub) ? wy[PB + bb] : wy[bb];
+        L* base = box + (oz * B + oy + sub * PB) * B + ox;
         // fold the accumulator's fixed-point scale into the x weights once per particle
 #pragma unroll
         for (int a = 0; a < P; ++a)
             wx[a] *= acc_scale<R>();
 #pragma unroll
-        for (int b = 0; b < P; ++b)
+        for (int b = 0; b < PB; ++b)
             {
-            const R qy = r.q * wy[b];
+            const R qy = r.q * wyl[b];
             L* row = base + b * B;
 #pragma unroll
             for (int c = 0; c < P; ++c)
@@ -480,7 +493,7 @@ __device__ __forceinline__ void spread_tile_tp(unsigned int tile,
                       });
     }
 
-template<typename R, int P>
+template<typename R, int P, int LPP>
 __global__ void __launch_bounds__(256) k_spread_tp(MeshGeom g,
                                                    int D,
                                                    const unsigned int* __restrict__ tile_start,
@@ -489,10 +502,12 @@ __global__ void __launch_bounds__(256) k_spread_tp(MeshGeom g,
                                                    const R* __restrict__ wcoef,
                                                    typename Acc<R>::glob_t* __restrict__ mesh)
     {
-    spread_tile_tp<R, P>(blockIdx.x, g, D, tile_start, rec, sidx, wcoef, mesh);
+    spread_tile_tp<R, P, LPP>(blockIdx.x, g, D, tile_start, rec, sidx, wcoef, mesh);
     }
 
-template<typename R, int P>
+//! LPP lanes per particle (1 or 2): with 2, lane `sub` of the (adjacent) pair sums rows b in
+//! [sub P/2, (sub+1) P/2) and the pair adds its halves with one lane exchange.
+template<typename R, int P, int LPP = 1>
 __device__ __forceinline__ void gather_tile_tp(unsigned int tile,
                                                const MeshGeom& g,
                                                int D,
@@ -519,19 +534,32 @@ __device__ __forceinline__ void gather_tile_tp(unsigned int tile,
     __syncthreads();
 
     const R inv_hx = R(1.0 / g.hx), inv_hy = R(1.0 / g.hy), inv_hz = R(1.0 / g.hz);
-    for (unsigned int p = s0 + threadIdx.x; p < s1; p += blockDim.x)
+    static_assert(LPP == 1 || (LPP == 2 && P % 2 == 0), "1 or 2 lanes per particle (even P)");
+    constexpr int PB = P / LPP;  // rows per lane
+    const unsigned int n_items = (s1 - s0) * unsigned(LPP);
+    for (unsigned int t = threadIdx.x; t < n_items; t += blockDim.x)
         {
+        const unsigned int p = s0 + t / unsigned(LPP);
+        const int sub = int(t % unsigned(LPP));
         const SortedRec<R> r = rec[p];
         const SortedIdx id = sidx[p];
         const int ox = id.local & 0xff, oy = (id.local >> 8) & 0xff, oz = id.local >> 16;
-        R wx[P], dwx[P], wy[P], dwy[P], wz[P], dwz[P];
+        R wx[P], dwx[P], wyf[P], dwyf[P], wz[P], dwz[P];
         eval_taps<R, P, true>(wcoef, D, r.tx, wx, dwx);
-        eval_taps<R, P, true>(wcoef, D, r.ty, wy, dwy);
+        eval_taps<R, P, true>(wcoef, D, r.ty, wyf, dwyf);
         eval_taps<R, P, true>(wcoef, D, r.tz, wz, dwz);
-        const R* base = box + (oz * B + oy) * B + ox;
+        // this lane's rows (a select, not a run-time register index)
+        R wy[PB], dwy[PB];
+#pragma unroll
+        for (int bb = 0; bb < PB; ++bb)
+            {
+            wy[bb] = (LPP == 2 && sub) ? wyf[PB + bb] : wyf[bb];
+            dwy[bb] = (LPP == 2 && sub) ? dwyf[PB + bb] : dwyf[bb];
+            }
+        const R* base = box + (oz * B + oy + sub * PB) * B + ox;
         R ex = R(0), ey = R(0), ez = R(0);
 #pragma unroll
-        for (int b = 0; b < P; ++b)
+        for (int b = 0; b < PB; ++b)
             {
             const R* row = base + b * B;
             R Sx = R(0), Sdx = R(0), Sdz = R(0);
@@ -554,6 +582,15 @@ __device__ __forceinline__ void gather_tile_tp(unsigned int tile,
             ey += dwy[b] * Sx;
             ez += wy[b] * Sdz;
             }
+        if constexpr (LPP == 2)
+            {
+            // both lanes of a pair are active together (n_items is even, pairs are lane aligned)
+            ex += __shfl_xor(ex, 1, 64);
+            ey += __shfl_xor(ey, 1, 64);
+            ez += __shfl_xor(ez, 1, 64);
+            if (sub != 0)
+                continue;
+            }
         const double q = double(r.q);
         double* f = reinterpret_cast<double*>(&force[id.idx]);
         unsafeAtomicAdd(f + 0, q * double(ex * inv_hx));
@@ -562,7 +599,7 @@ __device__ __forceinline__ void gather_tile_tp(unsigned int tile,
         }
     }
 
-template<typename R, int P>
+template<typename R, int P, int LPP>
 __global__ void __launch_bounds__(256) k_gather_tp(MeshGeom g,
                                                    int D,
                                                    const unsigned int* __restrict__ tile_start,
@@ -572,7 +609,7 @@ __global__ void __launch_bounds__(256) k_gather_tp(MeshGeom g,
                                                    const R* __restrict__ phi,
                                                    double4* __restrict__ force)
     {
-    gather_tile_tp<R, P>(blockIdx.x, g, D, tile_start, rec, sidx, wcoef, phi, force);
+    gather_tile_tp<R, P, LPP>(blockIdx.x, g, D, tile_start, rec, sidx, wcoef, phi, force);
     }
 
 // ---------------------------------------------------------------------------
@@ -722,6 +759,13 @@ __global__ void __launch_bounds__(1024) k_spread_own(MeshGeom g,
 // Batched spreading and gathering: items are (system, tile); the stencil order is a
 // per-system runtime value dispatched to the compiled orders.
 // ---------------------------------------------------------------------------
+//! Lanes per particle of the two-lane thread-per-particle variant: 2 for even orders (half the
+//! rows each), 1 for odd ones.
+constexpr int tp_pair_lanes(int P)
+    {
+    return P % 2 == 0 ? 2 : 1;
+    }
+
 #define ESP_BATCH_P_CASES(X) X(4) X(6) X(8) X(10) X(12)
 
 template<typename R>
@@ -738,7 +782,7 @@ __global__ void __launch_bounds__(256) kb_spread(const kernel::BatchSys* __restr
             {
 #define ESP_CASE(PP)                                                                 \
     case PP:                                                                         \
-        spread_tile_tp<R, PP>(w.local, d.g, d.D, d.tile_start, rec, d.sidx, wc, mesh); \
+        spread_tile_tp<R, PP, 1>(w.local, d.g, d.D, d.tile_start, rec, d.sidx, wc, mesh); \
         break;
             ESP_BATCH_P_CASES(ESP_CASE)
 #undef ESP_CASE
@@ -763,7 +807,7 @@ __global__ void __launch_bounds__(256) kb_gather(const kernel::BatchSys* __restr
             {
 #define ESP_CASE(PP)                                                                          \
     case PP:                                                                                  \
-        gather_tile_tp<R, PP>(w.local, d.g, d.D, d.tile_start, rec, d.sidx, wc, phi, force); \
+        gather_tile_tp<R, PP, 1>(w.local, d.g, d.D, d.tile_start, rec, d.sidx, wc, phi, force); \
         break;
             ESP_BATCH_P_CASES(ESP_CASE)
 #undef ESP_CASE
@@ -805,10 +849,14 @@ static void launch_spread_own(const MeshGeom& g, int D, unsigned int n_tiles, co
     }
 
 //! Stencil kernel variant: 0 = wave segments per particle, 1 = thread per particle,
-//! 2 = column ownership (spread only).
-//! Defaults (measured on MI355X at N = 1e6, 128^3, P = 10): FP64 spreads by column ownership,
-//! FP32 by thread-per-particle LDS atomics; both gather thread-per-particle.  The environment
-//! variables ESP_SPREAD_VARIANT / ESP_GATHER_VARIANT override them.
+//! 2 = column ownership (spread only), 3 = thread per particle with two lanes per particle
+//! (even orders; half the rows each).
+//! Defaults (measured on MI355X at N = 1e6, P = 10): FP64 spreads by column ownership and
+//! gathers with two lanes per particle; FP32 spreads with two lanes (fixed-point sums: the same
+//! result as one lane) and gathers with one, as the batched kernels do (their 16^3 tiles
+//! already fill the workgroup), so a mixed-precision system's forces are bit-identical alone
+//! and in a batch.  The environment variables ESP_SPREAD_VARIANT / ESP_GATHER_VARIANT override
+//! them.
 static int stencil_variant(int which, bool fp64)
     {
     static int v[2] = {-2, -2};
@@ -819,7 +867,7 @@ static int stencil_variant(int which, bool fp64)
         }
     if (v[which] >= 0)
         return v[which];
-    return which == 0 ? (fp64 ? 2 : 1) : 1;
+    return which == 0 ? (fp64 ? 2 : 3) : (fp64 ? 3 : 1);
     }
 
 size_t tp_lds_bytes(const MeshGeom& g, int P, bool fp64);
@@ -845,8 +893,8 @@ bool stencil_tile_ok(int T, int P, bool fp64, bool batched)
         lds = own_lds_bytes(P, fp64);
         }
     else
-        lds = sv == 1 ? tp_lds_bytes(g, P, fp64) : spread_lds_bytes(g, P, 0, fp64);
-    lds = std::max(lds, gv == 1 ? tp_lds_bytes(g, P, fp64) : gather_lds_bytes(g, P, 0, fp64));
+        lds = (sv == 1 || sv == 3) ? tp_lds_bytes(g, P, fp64) : spread_lds_bytes(g, P, 0, fp64);
+    lds = std::max(lds, (gv == 1 || gv == 3) ? tp_lds_bytes(g, P, fp64) : gather_lds_bytes(g, P, 0, fp64));
     return lds <= 64 * 1024;
     }
 
@@ -883,7 +931,7 @@ hipError_t spread(const MeshGeom& g, int P, int D, unsigned int n_tiles, const u
                   const SortedIdx* sidx, const void* wcoef, void* mesh, hipStream_t s)
     {
     const int variant = stencil_variant(0, sizeof(R) == 8);
-    const bool tp = variant == 1;
+    const bool tp = variant == 1 || variant == 3;
     const size_t lds = tp ? tp_lds_bytes(g, P, sizeof(R) == 8) : spread_lds_bytes(g, P, D, sizeof(R) == 8);
     switch (P)
         {
@@ -891,8 +939,12 @@ hipError_t spread(const MeshGeom& g, int P, int D, unsigned int n_tiles, const u
     case PP:                                                                                                                 \
         if (variant == 2)                                                                                                    \
             launch_spread_own<R, PP>(g, D, n_tiles, start, rec, sidx, wcoef, mesh, s);                                      \
+        else if (variant == 3)                                                                                               \
+            hipLaunchKernelGGL((k_spread_tp<R, PP, tp_pair_lanes(PP)>), dim3(n_tiles), dim3(256), lds, s, g, D, start,                      \
+                               reinterpret_cast<const SortedRec<R>*>(rec), sidx, reinterpret_cast<const R*>(wcoef),          \
+                               reinterpret_cast<typename Acc<R>::glob_t*>(mesh));                                            \
         else if (tp)                                                                                                         \
-            hipLaunchKernelGGL((k_spread_tp<R, PP>), dim3(n_tiles), dim3(256), lds, s, g, D, start,                         \
+            hipLaunchKernelGGL((k_spread_tp<R, PP, 1>), dim3(n_tiles), dim3(256), lds, s, g, D, start,                      \
                                reinterpret_cast<const SortedRec<R>*>(rec), sidx, reinterpret_cast<const R*>(wcoef),          \
                                reinterpret_cast<typename Acc<R>::glob_t*>(mesh));                                            \
         else                                                                                                                 \
@@ -911,14 +963,19 @@ template<typename R>
 hipError_t gather(const MeshGeom& g, int P, int D, unsigned int n_tiles, const unsigned int* start, const void* rec,
                   const SortedIdx* sidx, const void* wcoef, const void* phi, double* force, hipStream_t s)
     {
-    const bool tp = stencil_variant(1, sizeof(R) == 8) == 1;
+    const int variant = stencil_variant(1, sizeof(R) == 8);
+    const bool tp = variant == 1 || variant == 3;
     const size_t lds = tp ? tp_lds_bytes(g, P, sizeof(R) == 8) : gather_lds_bytes(g, P, D, sizeof(R) == 8);
     switch (P)
         {
 #define ESP_CASE(PP)                                                                                                         \
     case PP:                                                                                                                 \
-        if (tp)                                                                                                              \
-            hipLaunchKernelGGL((k_gather_tp<R, PP>), dim3(n_tiles), dim3(256), lds, s, g, D, start,                         \
+        if (variant == 3)                                                                                                    \
+            hipLaunchKernelGGL((k_gather_tp<R, PP, tp_pair_lanes(PP)>), dim3(n_tiles), dim3(256), lds, s, g, D, start,                      \
+                               reinterpret_cast<const SortedRec<R>*>(rec), sidx, reinterpret_cast<const R*>(wcoef),          \
+                               reinterpret_cast<const R*>(phi), reinterpret_cast<double4*>(force));                          \
+        else if (tp)                                                                                                         \
+            hipLaunchKernelGGL((k_gather_tp<R, PP, 1>), dim3(n_tiles), dim3(256), lds, s, g, D, start,                      \
                                reinterpret_cast<const SortedRec<R>*>(rec), sidx, reinterpret_cast<const R*>(wcoef),          \
                                reinterpret_cast<const R*>(phi), reinterpret_cast<double4*>(force));                          \
         else                                                                                                                 \
