@@ -180,6 +180,7 @@ EspSolver::~EspSolver()
         (void)hipStreamDestroy(m_side);
         (void)hipEventDestroy(m_ev_fork);
         (void)hipEventDestroy(m_ev_join);
+        (void)hipEventDestroy(m_ev_zdone);
         }
     }
 
@@ -646,8 +647,11 @@ unsigned int cu_count()
     return c;
     }
 
+//! The mesh chain on stream s.  With a side stream (where the pair pass runs) the final FP64
+//! reduction also runs there, after the pair pass and the z pass, so it overlaps the inverse y
+//! and x passes; the gather waits for the side stream.
 template<typename R>
-void EspSolver::run_mesh(const SystemView& sys, const OutputView& out, unsigned int n, hipStream_t s, bool want_ev, hipEvent_t join)
+void EspSolver::run_mesh(const SystemView& sys, const OutputView& out, unsigned int n, hipStream_t s, bool want_ev, hipStream_t side)
     {
     const MeshGeom& g = m_geom;
     const int P = int(m_p.order);
@@ -676,22 +680,34 @@ void EspSolver::run_mesh(const SystemView& sys, const OutputView& out, unsigned 
     check_hip(kernel::fft_z_convolve<R>(g, m_plan[2], m_tw[2].get(), m_spec.get(), m_G.as<double>(), m_Gv.as<double>(), want_ev ? 1 : 0,
                                         m_evpart.as<double>(), s),
               "fft_z_convolve");
+    const double V = g.Lx * g.Ly * g.Lz;
+    double* evout = out.ext_energy_virial ? out.ext_energy_virial : m_ev_internal.as<double>();
+    auto finalize = [&](hipStream_t fs)
+    {
+        check_hip(kernel::finalize(int(kernel::fft_z_blocks(g, sizeof(R) == 8)), m_evpart.as<double>(), int(nqb), m_qpart.as<double>(),
+                                   0.5 / V, m_self_coeff, m_bg_coeff, V, want_ev ? 1 : 0, evout, m_qsum.as<double>(), fs),
+                  "finalize");
+    };
+    if (side)
+        {
+        check_hip(hipEventRecord(m_ev_zdone, s), "hipEventRecord");
+        check_hip(hipStreamWaitEvent(side, m_ev_zdone, 0), "hipStreamWaitEvent");
+        finalize(side);
+        check_hip(hipEventRecord(m_ev_join, side), "hipEventRecord");
+        }
     mark(8, s);
     check_hip(kernel::fft_y<R>(g, m_plan[1], m_tw[1].get(), m_spec.get(), +1, s), "fft_y inverse");
     mark(9, s);
-    const double V = g.Lx * g.Ly * g.Lz;
     check_hip(kernel::fft_inverse_x<R>(g, m_plan[0], m_tw[0].get(), m_spec.get(), 1.0 / V, m_phi.get(), s), "fft_inverse_x");
     mark(10, s);
-    if (join)
-        check_hip(hipStreamWaitEvent(s, join, 0), "hipStreamWaitEvent");
+    if (side)
+        check_hip(hipStreamWaitEvent(s, m_ev_join, 0), "hipStreamWaitEvent");
     check_hip(kernel::gather<R>(g, P, D, m_n_tiles, m_tile_start.as<unsigned int>(), m_rec.get(), m_sidx.as<SortedIdx>(), m_wcoef.get(),
                                 m_phi.get(), out.force, s),
               "gather");
     mark(11, s);
-    double* evout = out.ext_energy_virial ? out.ext_energy_virial : m_ev_internal.as<double>();
-    check_hip(kernel::finalize(int(kernel::fft_z_blocks(g, sizeof(R) == 8)), m_evpart.as<double>(), int(nqb), m_qpart.as<double>(),
-                               0.5 / V, m_self_coeff, m_bg_coeff, V, want_ev ? 1 : 0, evout, m_qsum.as<double>(), s),
-              "finalize");
+    if (!side)
+        finalize(s);
     }
 
 unsigned int EspSolver::prepare(const SystemView& sys, const OutputView& out, hipStream_t s)
@@ -746,15 +762,13 @@ void EspSolver::compute(const SystemView& sys, const OutputView& out, hipStream_
     // the chain measured no faster.)
     const bool mesh = (flags & COMPUTE_MESH) != 0;
     const bool side = (flags & COMPUTE_PAIR) && mesh && m_overlap;
-    hipEvent_t join = nullptr;
     if (side && !m_side)
         {
         check_hip(hipStreamCreateWithFlags(&m_side, hipStreamNonBlocking), "hipStreamCreate");
         check_hip(hipEventCreateWithFlags(&m_ev_fork, hipEventDisableTiming), "hipEventCreate");
         check_hip(hipEventCreateWithFlags(&m_ev_join, hipEventDisableTiming), "hipEventCreate");
+        check_hip(hipEventCreateWithFlags(&m_ev_zdone, hipEventDisableTiming), "hipEventCreate");
         }
-    if (side)
-        join = m_ev_join;
     if (flags & COMPUTE_PAIR)
         {
         hipStream_t ps = s;
@@ -768,8 +782,7 @@ void EspSolver::compute(const SystemView& sys, const OutputView& out, hipStream_
             run_pair<double>(sys, out, ps, side);
         else
             run_pair<float>(sys, out, ps, side);
-        if (side)
-            check_hip(hipEventRecord(m_ev_join, ps), "hipEventRecord");
+        // with the side stream, the mesh chain records the join after the final reduction
         }
     else
         {
@@ -782,9 +795,9 @@ void EspSolver::compute(const SystemView& sys, const OutputView& out, hipStream_
         {
         const bool want_ev = (flags & COMPUTE_ENERGY_VIRIAL) != 0;
         if (fp64)
-            run_mesh<double>(sys, out, n, s, want_ev, join);
+            run_mesh<double>(sys, out, n, s, want_ev, side ? m_side : nullptr);
         else
-            run_mesh<float>(sys, out, n, s, want_ev, join);
+            run_mesh<float>(sys, out, n, s, want_ev, side ? m_side : nullptr);
         }
     else
         for (int b = 1; b <= 11; ++b)

@@ -175,6 +175,7 @@ class EspSolver
     hipStream_t m_side = nullptr;
     hipEvent_t m_ev_fork = nullptr;
     hipEvent_t m_ev_join = nullptr;
+    hipEvent_t m_ev_zdone = nullptr;  // the z pass has written the energy/virial partials
     bool m_profile = false;
     std::vector<std::vector<hipEvent_t>> m_ev_pool;   // free event sets (N_STAGES + 1 each)
     std::vector<std::vector<hipEvent_t>> m_ev_pending; // recorded, not yet harvested
@@ -191,7 +192,7 @@ class EspSolver
     void build_influence(hipStream_t s);
     void ensure_particles(unsigned int n);
     template<typename R>
-    void run_mesh(const SystemView& sys, const OutputView& out, unsigned int n, hipStream_t s, bool want_ev, hipEvent_t join);
+    void run_mesh(const SystemView& sys, const OutputView& out, unsigned int n, hipStream_t s, bool want_ev, hipStream_t side);
     //! concurrent: the pass runs on the side stream beside the mesh chain.
     template<typename R> void run_pair(const SystemView& sys, const OutputView& out, hipStream_t s, bool concurrent);
 
