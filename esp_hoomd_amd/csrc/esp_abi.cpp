@@ -422,7 +422,9 @@ int esp_set_profiling(esp_ctx* ctx, int enable)
     {
     if (!ctx)
         return ESP_ERR_ARG;
-    ctx->solver.set_profiling(enable != 0);
+    if (enable < 0 || enable > 2)
+        return ESP_ERR_ARG;
+    ctx->solver.set_profiling(enable);
     return ESP_OK;
     }
 

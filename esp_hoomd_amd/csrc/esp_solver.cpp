@@ -191,9 +191,9 @@ const char* EspSolver::stage_name(int i)
     return (i >= 0 && i < N_STAGES) ? names[i] : "";
     }
 
-void EspSolver::set_profiling(bool on)
+void EspSolver::set_profiling(int level)
     {
-    m_profile = on;
+    m_profile = (level >= 0 && level <= 2) ? level : 1;
     }
 
 void EspSolver::reset_stage_times()
@@ -207,10 +207,13 @@ void EspSolver::reset_stage_times()
     }
 
 //! Record mesh-pipeline boundary b (0..N_STAGES) of the current call on stream s; stage i in
-//! 1..N_STAGES-1 spans boundaries i and i+1.  Boundary 0 opens a new event set.
+//! 1..N_STAGES-1 spans boundaries i and i+1.  Boundary 0 opens a new event set, N_STAGES closes
+//! it.  At level 2 no boundary is recorded (only the pair stage's own events, on its stream).
 void EspSolver::mark(int b, hipStream_t s)
     {
     if (!m_profile)
+        return;
+    if (m_profile == 2 && b > 0 && b < N_STAGES)
         return;
     if (b == 0)
         {
@@ -224,11 +227,13 @@ void EspSolver::mark(int b, hipStream_t s)
         m_ev_pending.push_back(std::move(m_ev_pool.back()));
         m_ev_pool.pop_back();
         m_ev_marked.push_back(false);
+        m_ev_full.push_back(m_profile == 1);
         m_ev_cur = &m_ev_pending.back();
         }
     if (!m_ev_cur)
         return;
-    check_hip(hipEventRecord((*m_ev_cur)[b], s), "hipEventRecord");
+    if (m_profile == 1)
+        check_hip(hipEventRecord((*m_ev_cur)[b], s), "hipEventRecord");
     if (b == N_STAGES)
         m_ev_cur = nullptr;
     }
@@ -248,7 +253,8 @@ void EspSolver::harvest()
     for (size_t c = 0; c < m_ev_pending.size(); ++c)
         {
         std::vector<hipEvent_t>& v = m_ev_pending[c];
-        check_hip(hipEventSynchronize(v[N_STAGES]), "hipEventSynchronize");
+        if (m_ev_full[c])
+            check_hip(hipEventSynchronize(v[N_STAGES]), "hipEventSynchronize");
         float ms = 0.0f;
         if (m_ev_marked[c])
             {
@@ -259,16 +265,18 @@ void EspSolver::harvest()
                 m_stage_calls[0] += 1;
                 }
             }
-        for (int i = 1; i < N_STAGES; ++i)
-            if (hipEventElapsedTime(&ms, v[i], v[i + 1]) == hipSuccess)
-                {
-                m_stage_ms[i] += ms;
-                m_stage_calls[i] += 1;
-                }
+        if (m_ev_full[c])
+            for (int i = 1; i < N_STAGES; ++i)
+                if (hipEventElapsedTime(&ms, v[i], v[i + 1]) == hipSuccess)
+                    {
+                    m_stage_ms[i] += ms;
+                    m_stage_calls[i] += 1;
+                    }
         m_ev_pool.push_back(std::move(v));
         }
     m_ev_pending.clear();
     m_ev_marked.clear();
+    m_ev_full.clear();
     }
 
 void EspSolver::stage_times(double* ms, unsigned int* calls)

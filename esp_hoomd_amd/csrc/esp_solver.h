@@ -161,7 +161,7 @@ class EspSolver
     //! Per-stage HIP-event timing (stage boundaries recorded on the compute stream).
     static constexpr int N_STAGES = 12;
     static const char* stage_name(int i);
-    void set_profiling(bool on);
+    void set_profiling(int level);
     //! Totals in ms and call counts per stage since the last reset (synchronises).
     void stage_times(double* ms, unsigned int* calls);
     void reset_stage_times();
@@ -176,11 +176,12 @@ class EspSolver
     hipEvent_t m_ev_fork = nullptr;
     hipEvent_t m_ev_join = nullptr;
     hipEvent_t m_ev_zdone = nullptr;  // the z pass has written the energy/virial partials
-    bool m_profile = false;
+    int m_profile = 0;  // 0 off, 1 every stage, 2 the pair stage only
     std::vector<std::vector<hipEvent_t>> m_ev_pool;   // free event sets (N_STAGES + 1 each)
     std::vector<std::vector<hipEvent_t>> m_ev_pending; // recorded, not yet harvested
     std::vector<hipEvent_t>* m_ev_cur = nullptr;
     std::vector<bool> m_ev_marked;
+    std::vector<bool> m_ev_full;  // the set holds every stage boundary
     double m_stage_ms[N_STAGES] = {};
     unsigned int m_stage_calls[N_STAGES] = {};
 

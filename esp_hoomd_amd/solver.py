@@ -175,8 +175,10 @@ class Solver:
         _lib.check(_lib.load().esp_device_synchronize())
         return {"force": system.force.numpy(), "virial": system.virial.numpy()[:, : system.N]}
 
-    def set_profiling(self, on: bool = True) -> None:
-        _lib.check(_lib.load().esp_set_profiling(self._ctx, 1 if on else 0), self._ctx)
+    def set_profiling(self, on=True) -> None:
+        """True / 1: HIP-event times of every stage; 2: of the pair stage only; False / 0: off."""
+        level = 1 if on is True else (0 if on is False else int(on))
+        _lib.check(_lib.load().esp_set_profiling(self._ctx, level), self._ctx)
 
     def reset_stage_times(self) -> None:
         _lib.check(_lib.load().esp_reset_stage_times(self._ctx), self._ctx)

@@ -189,6 +189,8 @@ int esp_get_window(esp_ctx* ctx, unsigned int* P, unsigned int* D, double* coef)
 /* Per-stage HIP-event timing on the compute stream (no counterpart in the reference, which
  * only brackets kernels with HOOMD Autotuner begin/end, ESPForceComputeGPU.cc:230-250). */
 #define ESP_N_STAGES 12
+/* enable: 0 off, 1 every stage (an event pair around each), 2 the pair stage only (events on
+ * the pair kernel's stream; the step's other kernels run unmarked) */
 int esp_set_profiling(esp_ctx* ctx, int enable);
 /* totals in ms and call counts for the ESP_N_STAGES stages since the last reset; synchronises */
 int esp_get_stage_times(esp_ctx* ctx, double* ms, unsigned int* calls);
