@@ -628,9 +628,10 @@ template<typename R> void EspSolver::run_pair(const SystemView& sys, const Outpu
     // Beside the FP32 mesh chain the pass is held to 3 blocks (12 waves) per CU, which leaves
     // the spreading and FFT kernels room to run alongside it (measured at 1e6 particles, mesh
     // 160^3, r_c = 6.4: 0.99 ms per step against 1.13 ms at 2 blocks, 1.06 ms at 4 and
-    // 1.37 ms with the full grid).  The FP64 chain, which needs more registers per wave,
-    // gains nothing from the cap.
-    const unsigned int cap = (concurrent && sizeof(R) == 4) ? 3u * cu_count() : 0u;
+    // 1.37 ms with the full grid).  The FP64 kernel (124 VGPRs) fits 4 blocks per CU; a grid of
+    // exactly that many keeps its blocks from queueing ahead of the mesh chain's (1e6
+    // particles, mesh 180^3: 1.84 ms per step against 1.86 ms at 8 blocks per CU, 1.87 at 3).
+    const unsigned int cap = concurrent ? (sizeof(R) == 4 ? 3u : 4u) * cu_count() : 0u;
     check_hip(kernel::pack_positions_charges<R>(a, s), "pack_positions_charges");
     // the "pair" stage of the per-stage timing spans the pair kernel alone
     pair_mark(false, s);
