@@ -180,7 +180,18 @@ __device__ __forceinline__ void pair_terms(const R* __restrict__ tab,
 //! Pair correction of particle i by the G lanes of its group (`sub` = lane within the group);
 //! every lane of the group must call it.  Writes force (fx, fy, fz, energy) and, when
 //! requested, the six per-particle virial components.
-template<typename R, int G, bool COMPACT>
+//! Neighbour records in flight per lane: the pass is bound by the latency of these gathers.
+//! FP32 (16-byte records, 61 VGPRs at two): eight, which took the pass from 0.48 to 0.40 ms
+//! alone and the 1e6-particle mixed step from 0.965 to 0.92 ms (sixteen: 1.13 ms beside the
+//! mesh chain).  FP64 (32-byte records, 124 VGPRs): two; four gained nothing.  The batched
+//! kernel keeps two (its many small systems have short lists: eight cost 4 % on the 512-system
+//! batch); the order of accumulation, and so the result, does not depend on it.
+template<typename R> constexpr int pair_inflight()
+    {
+    return sizeof(R) == 4 ? 8 : 2;
+    }
+
+template<typename R, int G, bool COMPACT, int NF = pair_inflight<R>()>
 __device__ __forceinline__ void pair_particle(unsigned int i, int sub, const kernel::PairArgs& a, const R* __restrict__ tab)
     {
     const R rc = R(a.rc);
@@ -226,26 +237,33 @@ __device__ __forceinline__ void pair_particle(unsigned int i, int sub, const ker
             };
         const unsigned int nn = a.n_neigh[i];
         const uint64_t head = a.head_list[i];
-        // two neighbours in flight per lane, and the list entries of the next pair are fetched
-        // before this pair is evaluated.  Entries past the end are read at the clamped index
+        // NF neighbours in flight per lane, and the list entries of the next NF are fetched
+        // before these are evaluated.  Entries past the end are read at the clamped index
         // nn - 1 (in bounds) and not evaluated.  Neighbours are accumulated in list order, so
         // the sums are the same as one at a time.
         if (unsigned(sub) < nn)
             {
             const unsigned int last = nn - 1u;
             const unsigned int* __restrict__ nl = a.nlist + head;
-            unsigned int j0 = nl[sub];
-            unsigned int j1 = nl[min(unsigned(sub) + unsigned(G), last)];
-            for (unsigned int k = sub; k < nn; k += 2 * G)
+            unsigned int jn[NF];
+#pragma unroll
+            for (int f = 0; f < NF; ++f)
+                jn[f] = nl[min(unsigned(sub) + unsigned(f * G), last)];
+            for (unsigned int k = sub; k < nn; k += NF * G)
                 {
-                const Rec p0 = pq[j0];
-                const Rec p1 = pq[j1];
-                const unsigned int kn = k + 2 * G;
-                j0 = nl[min(kn, last)];
-                j1 = nl[min(kn + G, last)];
-                add(p0);
-                if (k + G < nn)
-                    add(p1);
+                Rec pf[NF];
+#pragma unroll
+                for (int f = 0; f < NF; ++f)
+                    pf[f] = pq[jn[f]];
+                const unsigned int kn = k + NF * G;
+#pragma unroll
+                for (int f = 0; f < NF; ++f)
+                    jn[f] = nl[min(kn + unsigned(f * G), last)];
+                add(pf[0]);
+#pragma unroll
+                for (int f = 1; f < NF; ++f)
+                    if (k + unsigned(f * G) < nn)
+                        add(pf[f]);
                 }
             }
         }
@@ -318,7 +336,7 @@ __device__ __forceinline__ const R* stage_compact_table(const kernel::PairArgs& 
 //! (block b on XCD (b + k) mod 8 for a launch-dependent k), so the blocks with one value of
 //! b mod 8 share an XCD: they take one contiguous eighth of the particles and sweep it
 //! together (interleaved), so that XCD's L2 holds the neighbour records of one moving window.
-template<typename R, int G, bool COMPACT>
+template<typename R, int G, bool COMPACT, int NF>
 __global__ void __launch_bounds__(256) k_pair(kernel::PairArgs a)
     {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -337,7 +355,7 @@ __global__ void __launch_bounds__(256) k_pair(kernel::PairArgs a)
     const unsigned int p0 = x * span;
     const unsigned int p1 = min(a.N, p0 + span);
     for (unsigned int i = p0 + j * per_pass + threadIdx.x / G; i < p1; i += per_x * per_pass)
-        pair_particle<R, G, COMPACT>(i, sub, a, tab);
+        pair_particle<R, G, COMPACT, NF>(i, sub, a, tab);
     }
 
 //! Batched pair correction: items are (system, chunk of 256 / G particles).  The compact
@@ -376,9 +394,9 @@ __global__ void __launch_bounds__(256) kb_pair(const kernel::BatchSys* __restric
             continue;
             }
         if (compact)
-            pair_particle<R, G, true>(i, sub, a, reinterpret_cast<const R*>(smem));
+            pair_particle<R, G, true, 2>(i, sub, a, reinterpret_cast<const R*>(smem));
         else
-            pair_particle<R, G, false>(i, sub, a, reinterpret_cast<const R*>(a.split_tab));
+            pair_particle<R, G, false, 2>(i, sub, a, reinterpret_cast<const R*>(a.split_tab));
         }
     }
 
@@ -418,10 +436,12 @@ namespace kernel
 {
 template<typename R, int G> static void launch_pair(const PairArgs& a, unsigned int nb, hipStream_t s)
     {
+    constexpr int NF = dev::pair_inflight<R>();
     if (a.compact_tab != nullptr)
-        hipLaunchKernelGGL((dev::k_pair<R, G, true>), dim3(nb), dim3(256), size_t(a.n_compact) * (a.compact_deg + 1) * sizeof(R), s, a);
+        hipLaunchKernelGGL((dev::k_pair<R, G, true, NF>), dim3(nb), dim3(256), size_t(a.n_compact) * (a.compact_deg + 1) * sizeof(R), s,
+                           a);
     else
-        hipLaunchKernelGGL((dev::k_pair<R, G, false>), dim3(nb), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((dev::k_pair<R, G, false, NF>), dim3(nb), dim3(256), 0, s, a);
     }
 
 template<typename R> hipError_t pack_positions_charges(const PairArgs& a, hipStream_t s)
