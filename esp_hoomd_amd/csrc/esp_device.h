@@ -185,6 +185,27 @@ __device__ __forceinline__ void stockham_stage(const C2<R>* __restrict__ in,
         }
     }
 
+//! for (i = tid; i < n; i += nt) put(i, get(i)), with the get()s of NB consecutive iterations
+//! issued before their put()s, so NB global loads are in flight per thread.  (When put() stores
+//! through a pointer the compiler cannot prove distinct from get()'s source, a plain loop waits
+//! for every load before its store.)  Iterations run in the same order as the plain loop.
+template<int NB, typename T, typename Get, typename Put>
+__device__ __forceinline__ void staged_copy(int n, int tid, int nt, Get get, Put put)
+    {
+    for (int i0 = tid; i0 < n; i0 += NB * nt)
+        {
+        T v[NB];
+#pragma unroll
+        for (int e = 0; e < NB; ++e)
+            if (i0 + e * nt < n)
+                v[e] = get(i0 + e * nt);
+#pragma unroll
+        for (int e = 0; e < NB; ++e)
+            if (i0 + e * nt < n)
+                put(i0 + e * nt, v[e]);
+        }
+    }
+
 //! Full 1-D FFT of nlines lines in LDS (ping-pong between a and b).  Returns the buffer holding
 //! the result.  radices[0..nstages) multiply to N.
 template<typename R>

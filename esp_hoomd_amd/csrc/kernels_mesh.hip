@@ -263,19 +263,26 @@ __device__ __forceinline__ void fft_x_r2c_block(int blk,
     const int pair0 = blk * pairs_per_block;
     for (int i = threadIdx.x; i < nx; i += blockDim.x)
         tw[i] = tw_g[i];
-    for (int idx = threadIdx.x; idx < pairs_per_block * nx; idx += blockDim.x)
+    staged_copy<8, C2<R>>(
+        pairs_per_block * nx, threadIdx.x, blockDim.x,
+        [&](int idx)
         {
-        const int pr = idx / nx;
-        const int x = idx - pr * nx;
-        const int l0 = 2 * (pair0 + pr);
-        const int l1 = l0 + 1;
-        R v0 = R(0), v1 = R(0);
-        if (l0 < nlines)
-            v0 = R(mesh_load_clear(&mesh[size_t(l0) * nx + x]));
-        if (l1 < nlines)
-            v1 = R(mesh_load_clear(&mesh[size_t(l1) * nx + x]));
-        a[pr * ls + x] = C2<R> {v0, v1};
-        }
+            const int pr = idx / nx;
+            const int x = idx - pr * nx;
+            const int l0 = 2 * (pair0 + pr);
+            const int l1 = l0 + 1;
+            R v0 = R(0), v1 = R(0);
+            if (l0 < nlines)
+                v0 = R(mesh_load_clear(&mesh[size_t(l0) * nx + x]));
+            if (l1 < nlines)
+                v1 = R(mesh_load_clear(&mesh[size_t(l1) * nx + x]));
+            return C2<R> {v0, v1};
+        },
+        [&](int idx, const C2<R>& v)
+        {
+            const int pr = idx / nx;
+            a[pr * ls + idx - pr * nx] = v;
+        });
     __syncthreads();
     C2<R>* res = lds_fft<R>(a, b, tw, plan, ls, pairs_per_block, -1, threadIdx.x, blockDim.x);
     const int nxh = g.nxh;
@@ -330,22 +337,34 @@ __device__ __forceinline__ void fft_x_c2r_block(int blk,
     const int pair0 = blk * pairs_per_block;
     for (int i = threadIdx.x; i < nx; i += blockDim.x)
         tw[i] = tw_g[i];
-    for (int idx = threadIdx.x; idx < pairs_per_block * nxh; idx += blockDim.x)
+    struct Pair2
         {
-        const int pr = idx / nxh;
-        const int k = idx - pr * nxh;
-        const int l0 = 2 * (pair0 + pr);
-        const int l1 = l0 + 1;
-        C2<R> x0 = {R(0), R(0)}, x1 = {R(0), R(0)};
-        if (l0 < nlines)
-            x0 = spec[size_t(l0) * g.sxp + k];
-        if (l1 < nlines)
-            x1 = spec[size_t(l1) * g.sxp + k];
-        // Z[k] = X0[k] + i X1[k]; Z[n-k] = conj(X0[k]) + i conj(X1[k])
-        a[pr * ls + k] = C2<R> {x0.x - x1.y, x0.y + x1.x};
-        if (k != 0 && k != nx - k)
-            a[pr * ls + nx - k] = C2<R> {x0.x + x1.y, -x0.y + x1.x};
-        }
+        C2<R> x0, x1;
+        };
+    staged_copy<8, Pair2>(
+        pairs_per_block * nxh, threadIdx.x, blockDim.x,
+        [&](int idx)
+        {
+            const int pr = idx / nxh;
+            const int k = idx - pr * nxh;
+            const int l0 = 2 * (pair0 + pr);
+            const int l1 = l0 + 1;
+            Pair2 v = {{R(0), R(0)}, {R(0), R(0)}};
+            if (l0 < nlines)
+                v.x0 = spec[size_t(l0) * g.sxp + k];
+            if (l1 < nlines)
+                v.x1 = spec[size_t(l1) * g.sxp + k];
+            return v;
+        },
+        [&](int idx, const Pair2& v)
+        {
+            const int pr = idx / nxh;
+            const int k = idx - pr * nxh;
+            // Z[k] = X0[k] + i X1[k]; Z[n-k] = conj(X0[k]) + i conj(X1[k])
+            a[pr * ls + k] = C2<R> {v.x0.x - v.x1.y, v.x0.y + v.x1.x};
+            if (k != 0 && k != nx - k)
+                a[pr * ls + nx - k] = C2<R> {v.x0.x + v.x1.y, -v.x0.y + v.x1.x};
+        });
     __syncthreads();
     C2<R>* res = lds_fft<R>(a, b, tw, plan, ls, pairs_per_block, +1, threadIdx.x, blockDim.x);
     for (int idx = threadIdx.x; idx < pairs_per_block * nx; idx += blockDim.x)
@@ -397,13 +416,21 @@ __device__ __forceinline__ void fft_y_block(int blk,
     const int nc = min(ncol, nxh - x0);
     for (int i = threadIdx.x; i < ny; i += blockDim.x)
         tw[i] = tw_g[i];
-    for (int idx = threadIdx.x; idx < ny * ncol; idx += blockDim.x)
+    staged_copy<8, C2<R>>(
+        ny * ncol, threadIdx.x, blockDim.x,
+        [&](int idx)
         {
-        const int y = idx / ncol;
-        const int xi = idx - y * ncol;
-        if (xi < nc)
-            a[xi * ls + y] = spec[(size_t(z) * ny + y) * g.sxp + x0 + xi];
-        }
+            const int y = idx / ncol;
+            const int xi = idx - y * ncol;
+            return xi < nc ? spec[(size_t(z) * ny + y) * g.sxp + x0 + xi] : C2<R> {R(0), R(0)};
+        },
+        [&](int idx, const C2<R>& v)
+        {
+            const int y = idx / ncol;
+            const int xi = idx - y * ncol;
+            if (xi < nc)
+                a[xi * ls + y] = v;
+        });
     __syncthreads();
     C2<R>* res = lds_fft<R>(a, b, tw, plan, ls, nc, sign, threadIdx.x, blockDim.x);
     for (int idx = threadIdx.x; idx < ny * ncol; idx += blockDim.x)
@@ -450,13 +477,21 @@ __device__ __forceinline__ void fft_z_conv_block(int blk,
     const int nc = min(ncol, nxh - x0);
     for (int i = threadIdx.x; i < nz; i += blockDim.x)
         tw[i] = tw_g[i];
-    for (int idx = threadIdx.x; idx < nz * ncol; idx += blockDim.x)
+    staged_copy<8, C2<R>>(
+        nz * ncol, threadIdx.x, blockDim.x,
+        [&](int idx)
         {
-        const int z = idx / ncol;
-        const int xi = idx - z * ncol;
-        if (xi < nc)
-            a[xi * ls + z] = spec[(size_t(z) * ny + y) * g.sxp + x0 + xi];
-        }
+            const int z = idx / ncol;
+            const int xi = idx - z * ncol;
+            return xi < nc ? spec[(size_t(z) * ny + y) * g.sxp + x0 + xi] : C2<R> {R(0), R(0)};
+        },
+        [&](int idx, const C2<R>& v)
+        {
+            const int z = idx / ncol;
+            const int xi = idx - z * ncol;
+            if (xi < nc)
+                a[xi * ls + z] = v;
+        });
     __syncthreads();
     C2<R>* res = lds_fft<R>(a, b, tw, plan, ls, nc, -1, threadIdx.x, blockDim.x);
 
@@ -464,20 +499,39 @@ __device__ __forceinline__ void fft_z_conv_block(int blk,
     const double two_pi = 6.283185307179586476925286766559;
     const int ky_i = (y <= ny / 2) ? y : y - ny;
     const double ky = two_pi * double(ky_i) / g.Ly;
-    for (int idx = threadIdx.x; idx < nz * ncol; idx += blockDim.x)
+    struct GG
+        {
+        double gk, gv;
+        };
+    staged_copy<8, GG>(
+        nz * ncol, threadIdx.x, blockDim.x,
+        [&](int idx)
+        {
+            const int z = idx / ncol;
+            const int xi = idx - z * ncol;
+            GG v = {0.0, 0.0};
+            if (xi < nc)
+                {
+                const size_t m = (size_t(z) * ny + y) * g.sxp + x0 + xi;
+                v.gk = G[m];
+                if (want_ev)
+                    v.gv = Gv[m];
+                }
+            return v;
+        },
+        [&](int idx, const GG& gg)
         {
         const int z = idx / ncol;
         const int xi = idx - z * ncol;
         if (xi >= nc)
-            continue;
+            return;
         const int x = x0 + xi;
-        const size_t m = (size_t(z) * ny + y) * g.sxp + x;
-        const double gk = G[m];
+        const double gk = gg.gk;
         C2<R> c = res[xi * ls + z];
         if (want_ev)
             {
             const double s2 = double(c.x) * double(c.x) + double(c.y) * double(c.y);
-            const double gv = Gv[m];
+            const double gv = gg.gv;
             const int kz_i = (z <= nz / 2) ? z : z - nz;
             const double kx = two_pi * double(x) / g.Lx;
             const double kz = two_pi * double(kz_i) / g.Lz;
@@ -508,7 +562,7 @@ __device__ __forceinline__ void fft_z_conv_block(int blk,
         c.x *= gr;
         c.y *= gr;
         res[xi * ls + z] = c;
-        }
+        });
     __syncthreads();
     C2<R>* other = (res == a) ? b : a;
     C2<R>* res2 = lds_fft<R>(res, other, tw, plan, ls, nc, +1, threadIdx.x, blockDim.x);

@@ -153,6 +153,58 @@ __device__ __forceinline__ void for_box_cells(const MeshGeom& g, int tx, int ty,
         }
     }
 
+//! Fill the tile's LDS box from the mesh: box[c] = src[cell (gx, gy, gz)] for every box cell,
+//! walked as for_box_cells does, with the global loads of NB consecutive iterations issued
+//! before their LDS stores.  (A plain loop waits for each load before its store, since the
+//! compiler cannot move a load above a store that may alias it: one load in flight per thread.)
+template<int NB, typename L, typename T>
+__device__ __forceinline__ void load_box_cells(const MeshGeom& g, int tx, int ty, int tz, const T* __restrict__ src, L* box)
+    {
+    const int B = g.B;
+    const int B2 = B * B;
+    const int B3 = B2 * B;
+    const int t0 = int(threadIdx.x), step = int(blockDim.x);
+    int lz = t0 / B2, ly = (t0 - lz * B2) / B, lx = t0 - lz * B2 - ly * B;
+    const int sz = step / B2, sy = (step - sz * B2) / B, sx = step - sz * B2 - sy * B;
+    const int x0 = tx * g.T, y0 = ty * g.T, z0 = tz * g.T;
+    for (int c0 = t0; c0 < B3; c0 += NB * step)
+        {
+        T v[NB];
+#pragma unroll
+        for (int e = 0; e < NB; ++e)
+            {
+            if (c0 + e * step < B3)
+                {
+                int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+                while (gx >= g.nx)
+                    gx -= g.nx;
+                while (gy >= g.ny)
+                    gy -= g.ny;
+                while (gz >= g.nz)
+                    gz -= g.nz;
+                v[e] = src[(size_t(gz) * g.ny + gy) * g.nx + gx];
+                lx += sx;
+                ly += sy;
+                lz += sz;
+                if (lx >= B)
+                    {
+                    lx -= B;
+                    ++ly;
+                    }
+                if (ly >= B)
+                    {
+                    ly -= B;
+                    ++lz;
+                    }
+                }
+            }
+#pragma unroll
+        for (int e = 0; e < NB; ++e)
+            if (c0 + e * step < B3)
+                box[c0 + e * step] = L(v[e]);
+        }
+    }
+
 // ---------------------------------------------------------------------------
 // Spread
 // ---------------------------------------------------------------------------
@@ -295,7 +347,7 @@ __global__ void __launch_bounds__(256) k_gather(MeshGeom g,
 
     int tx, ty, tz;
     tile_coords(g, tile, tx, ty, tz);
-    for_box_cells(g, tx, ty, tz, [&](int c, int gx, int gy, int gz) { box[c] = phi[(size_t(gz) * g.ny + gy) * g.nx + gx]; });
+    load_box_cells<8>(g, tx, ty, tz, phi, box);
     if (threadIdx.x == 0)
         box[B3] = R(0);
     for (int c = threadIdx.x; c < P * (D + 1); c += blockDim.x)
@@ -528,7 +580,7 @@ __device__ __forceinline__ void gather_tile_tp(unsigned int tile,
     R* box = reinterpret_cast<R*>(smem);
     int tx, ty, tz;
     tile_coords(g, tile, tx, ty, tz);
-    for_box_cells(g, tx, ty, tz, [&](int c, int gx, int gy, int gz) { box[c] = phi[(size_t(gz) * g.ny + gy) * g.nx + gx]; });
+    load_box_cells<8>(g, tx, ty, tz, phi, box);
     if (threadIdx.x == 0)
         box[B3] = R(0);
     __syncthreads();
