@@ -701,6 +701,10 @@ __device__ __forceinline__ void spread_tile_own(unsigned int tile,
     R* Wxy = Wz + CH * BZ;                        // [CH][2][P] x and y weights (times q for x)
     R* wc = Wxy + CH * 2 * P;                     // [P][D+1]
     unsigned int* org = reinterpret_cast<unsigned int*>(wc + P * (MAX_WINDOW_DEG + 1));  // [CH] ox | oy << 8
+    // the chunk's records and packed origins, loaded once per particle (16-byte aligned)
+    const size_t rec_off = (size_t(reinterpret_cast<unsigned char*>(org + CH) - smem) + 15) & ~size_t(15);
+    SortedRec<R>* srec = reinterpret_cast<SortedRec<R>*>(smem + rec_off);  // [CH]
+    unsigned int* sloc = reinterpret_cast<unsigned int*>(srec + CH);       // [CH]
 
     for (int c = threadIdx.x; c < P * (D + 1); c += blockDim.x)
         wc[c] = wcoef[c];
@@ -717,14 +721,20 @@ __device__ __forceinline__ void spread_tile_own(unsigned int tile,
     for (unsigned int c0 = s0; c0 < s1; c0 += CH)
         {
         const int n = int(min(unsigned(CH), s1 - c0));
+        for (int p = threadIdx.x; p < n; p += blockDim.x)
+            {
+            srec[p] = rec[c0 + p];
+            sloc[p] = sidx[c0 + p].local;
+            }
+        __syncthreads();
         // stage: items (p, j) with j < BZ for z, BZ <= j < BZ + 2P for x and y
         constexpr int PER = BZ + 2 * P;
         for (int item = threadIdx.x; item < n * PER; item += blockDim.x)
             {
             const int p = item / PER;
             const int j = item - p * PER;
-            const SortedRec<R> r = rec[c0 + p];
-            const unsigned int loc = sidx[c0 + p].local;
+            const SortedRec<R> r = srec[p];
+            const unsigned int loc = sloc[p];
             int k;
             R t;
             R scale = R(1);
@@ -880,7 +890,9 @@ size_t own_lds_bytes(int P, bool fp64)
     const size_t el = fp64 ? 8 : 4;
     const int B = 8 + P - 1;  // largest box edge (T = 8)
     const size_t BZ = size_t((B + 3) & ~3);
-    return 64 * BZ * el + 64 * 2 * size_t(P) * el + size_t(P) * (MAX_WINDOW_DEG + 1) * el + 64 * 4;
+    const size_t base = 64 * BZ * el + 64 * 2 * size_t(P) * el + size_t(P) * (MAX_WINDOW_DEG + 1) * el + 64 * 4;
+    // + the chunk's records (16-byte aligned) and packed origins
+    return ((base + 15) & ~size_t(15)) + 64 * 4 * el + 64 * 4;
     }
 
 template<typename R, int P>
