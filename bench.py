@@ -376,7 +376,7 @@ def main(argv=None):
 
     if args.gpus != 1:
         print(f"bench.py: only single-GPU runs are supported (got --gpus {args.gpus}); "
-              "multi-GPU sharding is a design note, not part of this build", file=sys.stderr)
+              "the library runs one system or batch per GPU and has no multi-GPU sharding", file=sys.stderr)
         return 2
     if args.calibrate_copy > 0:
         return copy_calibration(args.calibrate_copy, args.steps)
