@@ -160,3 +160,58 @@ def test_langevin_thermalises():
         sim.run(20)
         temps.append(sim.temperature())
     assert abs(np.mean(temps) - 1.5) < 0.15
+
+
+def test_rebuild_exactly_when_a_particle_moves_half_the_skin():
+    """esp_nlist_update and the batched displacement check rebuild a list when (and only when) a
+    particle has moved at least buffer / 2 since the last build, by minimum image (a move across
+    the periodic boundary counts by its short distance)."""
+    N, L, r_cut, buf = 400, 9.0, 2.0, 0.4
+
+    def wrap(p, box):
+        # positions handed to the library lie in the box [-L/2, L/2), as the integrator keeps them
+        return p - box * np.round(p / box)
+
+    pos, q = random_electrolyte(N, L, seed=21)
+    pos[7] = (L / 2 - 0.05, 0.0, 0.0)  # next to the +x face
+    st = md.State(pos, q, (L, L, L))
+    nl = md.NeighborList(buffer=buf)
+    nl.update(st, r_cut, force_rebuild=True)
+    assert nl.builds == 1
+    moved = pos.copy()
+    moved[7, 0] += 0.15  # crosses the +x face: -L/2 + 0.1 after wrapping, 0.15 by minimum image
+    moved = wrap(moved, L)
+    assert moved[7, 0] < 0
+    st.set_positions(moved)
+    assert not nl.update(st, r_cut) and nl.builds == 1
+    moved[3, 1] += 0.19
+    moved = wrap(moved, L)
+    st.set_positions(moved)
+    assert not nl.update(st, r_cut) and nl.builds == 1
+    moved[3, 1] += 0.02  # 0.21 >= buffer / 2
+    moved = wrap(moved, L)
+    st.set_positions(moved)
+    assert nl.update(st, r_cut) and nl.builds == 2
+    n, lists = nl.host_arrays(st)
+    got = {(min(i, int(j)), max(i, int(j))) for i in range(N) for j in lists[i]}
+    assert got == _kdtree_pairs(moved, L, r_cut + buf)
+
+    # the batched check: only the system that moved far enough is rebuilt
+    members = []
+    for k in range(3):
+        p_k, q_k = random_electrolyte(300, 8.0, seed=40 + k)
+        s_k = md.State(p_k, q_k, (8.0, 8.0, 8.0))
+        n_k = md.NeighborList(buffer=buf, inner=1.12)
+        n_k.update(s_k, r_cut, force_rebuild=True)
+        members.append((s_k, n_k, p_k))
+    b = md.BatchMD()
+    b.set([(s_k, n_k) for s_k, n_k, _ in members])
+    assert b.update_nlists() == []
+    p1 = members[1][2].copy()
+    p1[5, 2] += 0.25
+    members[1][0].set_positions(wrap(p1, 8.0))
+    p2 = members[2][2].copy()
+    p2[9, 0] += 0.1
+    members[2][0].set_positions(wrap(p2, 8.0))
+    assert b.update_nlists() == [1]
+    assert [m[1].builds for m in members] == [1, 2, 1]
