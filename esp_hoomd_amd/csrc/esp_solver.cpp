@@ -318,8 +318,8 @@ void EspSolver::set_params(const SolverParams& p)
         throw EspError(STATUS_ARG, "pair_lanes must be 0 or a power of two <= 32");
     if (!std::isfinite(p.k_cut))
         throw EspError(STATUS_ARG, "k_cut must be finite");
-    if (p.tile != 0 && p.tile != 2 && p.tile != 4 && p.tile != 8 && p.tile != 16)
-        throw EspError(STATUS_ARG, "tile must be 0, 2, 4, 8 or 16");
+    if (p.tile != 0 && (p.tile < 2 || p.tile > 16))
+        throw EspError(STATUS_ARG, "tile must be 0 (automatic) or an edge from 2 to 16");
     // validate FFT factorisation eagerly
     make_fft_plan(int(p.nx));
     make_fft_plan(int(p.ny));

@@ -952,7 +952,7 @@ bool stencil_tile_ok(int T, int P, bool fp64, bool batched)
     size_t lds = 0;
     if (sv == 2)
         {
-        if (T > 8)
+        if (T != 2 && T != 4 && T != 8)  // the instantiated box edges
             return false;
         lds = own_lds_bytes(P, fp64);
         }

@@ -72,8 +72,8 @@ typedef struct esp_params {
     double k_cut;                  /* spherical reciprocal-space cutoff: the influence function is 0
                                       for |k| > k_cut.  0 = automatic (1.2 kappa for the PSWF split,
                                       none for the Gaussian split); < 0 = none */
-    unsigned int tile;             /* spreading/gathering tile edge in mesh cells: 2, 4, 8 or 16;
-                                      0 = automatic */
+    unsigned int tile;             /* spreading/gathering tile edge in mesh cells, 2 to 16 (FP64
+                                      column-ownership spreading: 2, 4 or 8); 0 = automatic */
 } esp_params;
 
 /* One system: device pointers owned by the caller (HOOMD's ParticleData, NeighborList and

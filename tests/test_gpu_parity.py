@@ -164,3 +164,25 @@ def test_reference_table_lookup_on_device_matches_golden():
     np.testing.assert_allclose(res["force"][0::2, 0], -nd, rtol=1e-13, atol=1e-12)
     for k, x in enumerate(looks):
         assert abs(res["force"][2 * k, 3] - 0.5 * x["L"]) <= 1e-9 * abs(x["L"]) + 1e-20
+
+
+@pytest.mark.parametrize("tile", [4, 10, 12])
+def test_mixed_result_is_independent_of_the_tile_edge(tile):
+    """Mixed precision accumulates the mesh in fixed point and gathers each particle from the
+    same potential values in the same order, so any tile edge gives the same bits."""
+    N, L = 600, 12.0
+    pos, q = random_electrolyte(N, L, seed=31)
+    _, _, ref = run_gpu(make_params(40, 8, 12.0, 3.5, precision=1), pos, q, (L, L, L))
+    print(f"tile {tile}", flush=True)
+    _, _, res = run_gpu(make_params(40, 8, 12.0, 3.5, precision=1, tile=tile), pos, q, (L, L, L))
+    assert np.array_equal(res["force"], ref["force"])
+    assert np.array_equal(res["virial"], ref["virial"])
+    assert np.array_equal(res["ext"], ref["ext"])
+
+
+def test_tile_edges_outside_the_kernels_are_refused():
+    pos, q = random_electrolyte(100, 8.0, seed=2)
+    with pytest.raises(Exception):
+        run_gpu(make_params(32, 8, 12.0, 3.0, precision=0, tile=12), pos, q, (8.0, 8.0, 8.0))  # FP64 column spread: 2, 4, 8
+    with pytest.raises(Exception):
+        run_gpu(make_params(32, 8, 12.0, 3.0, precision=1, tile=17), pos, q, (8.0, 8.0, 8.0))
