@@ -755,16 +755,77 @@ using kernel::BatchItem;
 using kernel::BatchList;
 using kernel::BatchSys;
 
+//! Tile key of group member i (0xffffffff for an uncharged particle, which is not binned);
+//! returns its charge.
+__device__ __forceinline__ double bin_key(unsigned int i,
+                                          const unsigned int* __restrict__ group,
+                                          const double4* __restrict__ postype,
+                                          const double* __restrict__ charge,
+                                          const MeshGeom& g,
+                                          int P,
+                                          unsigned int& k)
+    {
+    const unsigned int p = group ? group[i] : i;
+    const double4 r = postype[p];
+    const double q = charge[p];
+    k = 0xffffffffu;
+    if (q != 0.0)
+        {
+        double t;
+        const int ox = stencil_origin(r.x, g.Lx, g.nx, P, t);
+        const int oy = stencil_origin(r.y, g.Ly, g.ny, P, t);
+        const int oz = stencil_origin(r.z, g.Lz, g.nz, P, t);
+        k = unsigned((oz / g.T) * g.ty + oy / g.T) * unsigned(g.tx) + unsigned(ox / g.T);
+        }
+    return q;
+    }
+
+//! Batched binning.  A work item is 256 particles of one system; when the system has at most
+//! KB_BIN_LDS_TILES tiles (small meshes: every batch tile is 16^3, so a 16^3-64^3 mesh has 1-64
+//! tiles) the item counts its particles per tile with LDS atomics and takes one global atomic
+//! per occupied tile, instead of one per particle on a handful of hot counters.  Slots within a
+//! tile then differ from the single-system order, which changes no result: the mixed-precision
+//! mesh is a fixed-point sum and the gather works per particle.
+constexpr unsigned int KB_BIN_LDS_TILES = 1024;
 __global__ void __launch_bounds__(256) kb_bin(const BatchSys* __restrict__ ds, BatchList l)
     {
     __shared__ double sq[4], sq2[4];
+    __shared__ unsigned int lcount[KB_BIN_LDS_TILES];
     for (unsigned int it = blockIdx.x; it < l.n; it += gridDim.x)
         {
         const BatchItem w = l.items[it];
         const BatchSys& d = ds[w.sys];
         const unsigned int i = w.local * kernel::BATCH_CHUNK + threadIdx.x;
         double q = 0.0;
-        if (i < d.n)
+        if (d.n_tiles <= KB_BIN_LDS_TILES)
+            {
+            for (unsigned int t = threadIdx.x; t < d.n_tiles; t += blockDim.x)
+                lcount[t] = 0u;
+            __syncthreads();
+            unsigned int k = 0xffffffffu, rank = 0;
+            if (i < d.n)
+                {
+                q = bin_key(i, d.group, reinterpret_cast<const double4*>(d.postype), d.charge, d.g, d.P, k);
+                if (k != 0xffffffffu)
+                    rank = atomicAdd(&lcount[k], 1u);
+                }
+            __syncthreads();
+            // one global atomic per occupied tile: lcount becomes the item's base slot there
+            for (unsigned int t = threadIdx.x; t < d.n_tiles; t += blockDim.x)
+                {
+                const unsigned int c = lcount[t];
+                if (c)
+                    lcount[t] = atomicAdd(&d.tile_count[t], c);
+                }
+            __syncthreads();
+            if (i < d.n)
+                {
+                if (k != 0xffffffffu)
+                    d.slot[i] = lcount[k] + rank;
+                d.key[i] = k;
+                }
+            }
+        else if (i < d.n)
             q = bin_one(i, d.group, reinterpret_cast<const double4*>(d.postype), d.charge, d.g, d.P, d.tile_count, d.key, d.slot);
         const double a = wave_sum(q), b = wave_sum(q * q);
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
