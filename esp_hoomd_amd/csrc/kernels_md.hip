@@ -438,6 +438,15 @@ __global__ void __launch_bounds__(256) kb_wca(const md::MDSys* __restrict__ ds, 
     const unsigned int* n_neigh = use_inner ? d.n_inner : d.n_neigh;
     const unsigned int* nlist = use_inner ? d.inner : d.nlist;
     const uint64_t* head = use_inner ? d.head_inner : d.head;
+    if (use_inner)
+        {
+        // a few entries per particle: one lane each, one pass (the lane groups below are for
+        // the ~100-entry ESP list and would leave most lanes idle through eight passes)
+        const unsigned int i = w.chunk * 256u + threadIdx.x;
+        wca_group<1>(i, 0, i < d.N, reinterpret_cast<const double4*>(d.postype), d.L[0], d.L[1], d.L[2], n_neigh, nlist, head, sigma,
+                     epsilon, reinterpret_cast<double4*>(d.force), d.virial, d.pitch);
+        return;
+        }
     // the item's 256 particles in WCA_LANES passes of 256 / WCA_LANES particle groups
     constexpr unsigned int PER_PASS = 256u / unsigned(WCA_LANES);
     for (unsigned int pass = 0; pass < unsigned(WCA_LANES); ++pass)
