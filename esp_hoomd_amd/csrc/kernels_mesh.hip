@@ -781,7 +781,7 @@ __device__ __forceinline__ double bin_key(unsigned int i,
     }
 
 //! Batched binning.  A work item is 256 particles of one system; when the system has at most
-//! KB_BIN_LDS_TILES tiles (small meshes: every batch tile is 16^3, so a 16^3-64^3 mesh has 1-64
+//! KB_BIN_LDS_TILES tiles (small meshes: batch tiles are 12^3, so a 16^3-64^3 mesh has 8-216
 //! tiles) the item counts its particles per tile with LDS atomics and takes one global atomic
 //! per occupied tile, instead of one per particle on a handful of hot counters.  Slots within a
 //! tile then differ from the single-system order, which changes no result: the mixed-precision

@@ -917,7 +917,7 @@ static void launch_spread_own(const MeshGeom& g, int D, unsigned int n_tiles, co
 //! (even orders; half the rows each).
 //! Defaults (measured on MI355X at N = 1e6, P = 10): FP64 spreads by column ownership and
 //! gathers with two lanes per particle; FP32 spreads with two lanes (fixed-point sums: the same
-//! result as one lane) and gathers with one, as the batched kernels do (their 16^3 tiles
+//! result as one lane) and gathers with one, as the batched kernels do (their 12^3 tiles
 //! already fill the workgroup), so a mixed-precision system's forces are bit-identical alone
 //! and in a batch.  The environment variables ESP_SPREAD_VARIANT / ESP_GATHER_VARIANT override
 //! them.
