@@ -128,10 +128,10 @@ void EspBatch::build_plan(bool fp64)
         return l;
     };
     m_l.pack = list(o_pack, pack.size(), m_max_grid);
-    // The pair branch runs beside the mesh stages.  In FP32 its grid is held to 4 blocks per
-    // CU, which leaves the mesh kernels room (512 mixed systems of 500-50,000 particles:
-    // 12.9 ms per batch step against 14.0 ms with 2048 blocks and 16.7 ms with 2 per CU).
-    // Environment ESP_BATCH_PAIR_GRID overrides the cap.
+    // The pair branch runs beside the mesh stages.  In FP32 its grid is held to 2 blocks per
+    // CU, which leaves the mesh kernels room (512 mixed systems of 500-50,000 particles, 12^3
+    // tiles: 8.5-8.8 ms per batch step against 8.7 ms at 1.5 blocks per CU, 9.3 at 1, 9.3-9.7
+    // at 3-4 and 10.6 with 2048 blocks).  Environment ESP_BATCH_PAIR_GRID overrides the cap.
     static long env_pair_grid = -1;
     if (env_pair_grid < 0)
         {
@@ -142,7 +142,7 @@ void EspBatch::build_plan(bool fp64)
     if (env_pair_grid)
         pair_cap = unsigned(env_pair_grid);
     else if (!fp64)
-        pair_cap = pair_cap ? std::min(pair_cap, 4u * cu_count()) : 4u * cu_count();
+        pair_cap = pair_cap ? std::min(pair_cap, 2u * cu_count()) : 2u * cu_count();
     m_l.pair = list(o_pair, pair.size(), pair_cap);
     m_l.bin = list(o_bin, bin.size(), m_max_grid);
     m_l.scatter = m_l.bin;
