@@ -416,7 +416,7 @@ def main(argv=None):
     for name, prec in (("fp64", _lib.ESP_PREC_FP64), ("mixed", _lib.ESP_PREC_MIXED)):
         if name not in legs:
             continue
-        ch = tune.autotune(pos, q, box, TARGET, prec, measure_time=True, top=6, reference=ref)
+        ch = tune.autotune(pos, q, box, TARGET, prec, measure_time=True, top=8, reference=ref)
         r = run_single(ch.params, pos, q, L, args.steps, args.warmup, ref)
         tuned[name] = (ch, r)
         out_legs[name] = {"params": ch.describe(), **{k: r[k] for k in ("ms_per_step", "particle_steps_per_s", "error",

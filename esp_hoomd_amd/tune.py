@@ -96,7 +96,9 @@ def predicted_error(c: float, sigma: float, P: int, precision: int = _lib.ESP_PR
 class CostModel:
     """Per-step cost coefficients on one MI355X (see the module docstring); pairs are
     (FP64, mixed)."""
-    pair_ps: Tuple[float, float] = (5.3, 2.9)         # per neighbour entry, pair pass alone
+    pair_ps: Tuple[float, float] = (4.7, 2.47)        # per neighbour entry, pair pass alone (1e6 particles:
+                                                      # FP32 0.39 ms at 1.31e8 entries, 0.46 ms at 1.58e8;
+                                                      # FP64 0.535 ms at 9.5e7)
     pair_us: Tuple[float, float] = (90.0, 70.0)       # fixed cost of the pair pass
     bin_ns: Tuple[float, float] = (0.15, 0.11)        # per particle (bin + scan + scatter)
     stencil_ps: Tuple[float, float] = (0.53, 0.26)    # per particle per P^3 (spread + gather)
