@@ -77,7 +77,7 @@ __device__ __forceinline__ void nlist_group(unsigned int i, int sub, bool active
     const unsigned long long gmask = (G == 64) ? ~0ull : ((1ull << G) - 1ull);
     const unsigned long long below = (1ull << unsigned(sub)) - 1ull;
     unsigned int* row = nlist + size_t(i) * nmax;
-    unsigned int* row_in = in.list + size_t(i) * in.nmax;
+    unsigned int* row_in = in.r2 > 0.0 ? in.list + size_t(i) * in.nmax : nullptr;
     unsigned int k = 0, k_in = 0;
     // neighbour cells, each visited once even when an axis has fewer than 3 cells
     const int rx = cg.nx >= 3 ? 1 : (cg.nx == 2 ? 1 : 0), ry = cg.ny >= 3 ? 1 : (cg.ny == 2 ? 1 : 0), rz = cg.nz >= 3 ? 1 : (cg.nz == 2 ? 1 : 0);
