@@ -28,6 +28,9 @@
  * Gaussian-split special case against Ewald, finite differences and brute force).
  */
 #include <complex.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -172,17 +175,149 @@ int oracle_fft3d(int nx, int ny, int nz, double* data, int sign)
 }
 
 /* ------------------------------------------------------------------------- */
-/* Direct Ewald sum (Gaussian split), orthorhombic box centred on the origin */
+/* Cell binning for real-space sums (any cutoff, periodic images included)    */
 /* ------------------------------------------------------------------------- */
 static void min_image(double* d, const double* L)
 {
     for (int a = 0; a < 3; ++a) d[a] -= L[a] * nearbyint(d[a] / L[a]);
 }
 
+typedef struct {
+    int nc[3];       /* cells per axis */
+    int reach[3];    /* cell offsets to visit per axis: |offset| <= reach */
+    int* start;      /* [ncells + 1] */
+    int* idx;        /* particle indices ordered by cell (increasing index within a cell) */
+    double* xyz;     /* their coordinates wrapped into the primary box, in the same order */
+} cells_t;
+
+/* cell of coordinate x (box centred on the origin) along an axis of length L with nc cells */
+static int cell_of(double x, double L, int nc)
+{
+    double f = x / L + 0.5;
+    f -= floor(f);
+    int c = (int)(f * nc);
+    return c >= nc ? nc - 1 : (c < 0 ? 0 : c);
+}
+
+/* Bin N particles into cells of edge >= rcut / 3 (at most 256 per axis).  A source image within
+ * rcut of a target lies at most `reach` cells away along each axis, counting cells across the
+ * periodic boundary as images (cell c + s*nc is cell c shifted by s box lengths). */
+static void cells_build(cells_t* c, int N, const double* postype, const double* L, double rcut)
+{
+    size_t ncell = 1;
+    for (int a = 0; a < 3; ++a) {
+        int n = (int)floor(L[a] / (rcut / 3.0));
+        if (n < 1) n = 1;
+        if (n > 256) n = 256;
+        c->nc[a] = n;
+        c->reach[a] = (int)ceil(rcut / (L[a] / n));
+        ncell *= (size_t)n;
+    }
+    c->start = calloc(ncell + 1, sizeof(int));
+    c->idx = malloc(sizeof(int) * (size_t)(N > 0 ? N : 1));
+    int* cell = malloc(sizeof(int) * (size_t)(N > 0 ? N : 1));
+    for (int j = 0; j < N; ++j) {
+        const int cx = cell_of(postype[4 * j], L[0], c->nc[0]);
+        const int cy = cell_of(postype[4 * j + 1], L[1], c->nc[1]);
+        const int cz = cell_of(postype[4 * j + 2], L[2], c->nc[2]);
+        cell[j] = (cz * c->nc[1] + cy) * c->nc[0] + cx;
+        c->start[cell[j] + 1]++;
+    }
+    for (size_t k = 0; k < ncell; ++k) c->start[k + 1] += c->start[k];
+    int* fill = malloc(sizeof(int) * (ncell > 0 ? ncell : 1));
+    memcpy(fill, c->start, sizeof(int) * ncell);
+    for (int j = 0; j < N; ++j) c->idx[fill[cell[j]]++] = j;
+    c->xyz = malloc(sizeof(double) * 3 * (size_t)(N > 0 ? N : 1));
+    for (int k = 0; k < N; ++k)
+        for (int a = 0; a < 3; ++a) {
+            const double f = postype[4 * c->idx[k] + a] / L[a] + 0.5;
+            c->xyz[3 * k + a] = (f - floor(f) - 0.5) * L[a];
+        }
+    free(fill);
+    free(cell);
+}
+
+static void cells_free(cells_t* c)
+{
+    free(c->start);
+    free(c->idx);
+    free(c->xyz);
+}
+
+static int floor_div(int a, int n) { return (a >= 0) ? a / n : -((-a + n - 1) / n); }
+
+/* Call fn(ctx, j, d) for every source image j (excluding i itself at zero shift) with
+ * |d| < rcut, d = r_i - (r_j + shift).  Each (j, image) pair is visited once. */
+typedef void (*pair_fn)(void* ctx, int j, const double* d, double r2);
+static void cells_for_neighbours(const cells_t* c, const double* postype, const double* L, int i, const double* ri,
+                                 double rcut, pair_fn fn, void* ctx)
+{
+    const double rc2 = rcut * rcut;
+    const int ci[3] = {cell_of(ri[0], L[0], c->nc[0]), cell_of(ri[1], L[1], c->nc[1]), cell_of(ri[2], L[2], c->nc[2])};
+    for (int oz = -c->reach[2]; oz <= c->reach[2]; ++oz) {
+        const int vz = ci[2] + oz, wz = vz - floor_div(vz, c->nc[2]) * c->nc[2];
+        const double sz = floor_div(vz, c->nc[2]) * L[2];
+        for (int oy = -c->reach[1]; oy <= c->reach[1]; ++oy) {
+            const int vy = ci[1] + oy, wy = vy - floor_div(vy, c->nc[1]) * c->nc[1];
+            const double sy = floor_div(vy, c->nc[1]) * L[1];
+            for (int ox = -c->reach[0]; ox <= c->reach[0]; ++ox) {
+                const int vx = ci[0] + ox, wx = vx - floor_div(vx, c->nc[0]) * c->nc[0];
+                const double sx = floor_div(vx, c->nc[0]) * L[0];
+                const int zero_shift = (sx == 0.0 && sy == 0.0 && sz == 0.0);
+                const int cell = (wz * c->nc[1] + wy) * c->nc[0] + wx;
+                /* the source's coordinate is taken inside the primary box, as its cell was */
+                const double px = ri[0] - sx, py = ri[1] - sy, pz = ri[2] - sz;
+                for (int k = c->start[cell]; k < c->start[cell + 1]; ++k) {
+                    const double d[3] = {px - c->xyz[3 * k], py - c->xyz[3 * k + 1], pz - c->xyz[3 * k + 2]};
+                    const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                    if (r2 >= rc2) continue;
+                    const int j = c->idx[k];
+                    if (j == i && zero_shift) continue;
+                    fn(ctx, j, d, r2);
+                }
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Direct Ewald sum (Gaussian split), orthorhombic box centred on the origin */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+    const double* q;
+    double qi, alpha, two_a_spi;
+    int want_ev;
+    double f[3], E, W[6];
+} ewald_real_ctx;
+
+static void ewald_real_pair(void* vctx, int j, const double* d, double r2)
+{
+    ewald_real_ctx* c = (ewald_real_ctx*)vctx;
+    const double r = sqrt(r2);
+    const double qq = c->qi * c->q[j] * ORACLE_INV_4PI;
+    const double er = erfc(c->alpha * r);
+    const double fdivr = qq * (er / r + c->two_a_spi * exp(-c->alpha * c->alpha * r2)) / r2;
+    for (int k = 0; k < 3; ++k) c->f[k] += fdivr * d[k];
+    if (c->want_ev) {
+        c->E += 0.5 * qq * er / r;
+        c->W[0] += 0.5 * d[0] * d[0] * fdivr;
+        c->W[1] += 0.5 * d[0] * d[1] * fdivr;
+        c->W[2] += 0.5 * d[0] * d[2] * fdivr;
+        c->W[3] += 0.5 * d[1] * d[1] * fdivr;
+        c->W[4] += 0.5 * d[1] * d[2] * fdivr;
+        c->W[5] += 0.5 * d[2] * d[2] * fdivr;
+    }
+}
+
+/* one row of the reciprocal half space: fixed (b, c), a in [a_lo, a_hi], S stored from off */
+typedef struct { int b, c, a_lo, a_hi; long off; } krow_t;
+
 /*
  * Forces on the n_targets particles listed in targets (all if targets == NULL) from all N
  * particles; energy and virial of the whole system when energy != NULL (needs all targets).
- * Real space: all images within rcut (rcut may exceed L/2).  k space: |k| <= kcut.
+ * Real space: every periodic image within rcut (rcut may exceed L/2), found through a cell
+ * list.  k space: |k| <= kcut over the half space, the structure factor accumulated per
+ * thread from per-particle phase factors e^{-i k_a x} (one sincos per axis wavenumber).
  */
 int oracle_ewald(int N, const double* postype, const double* q, const double* Lbox, double alpha, double rcut,
                  double kcut, int n_targets, const int* targets, double* force, double* energy, double* virial)
@@ -192,121 +327,187 @@ int oracle_ewald(int N, const double* postype, const double* q, const double* Lb
     const int nt = targets ? n_targets : N;
     double E = 0.0, W[6] = {0, 0, 0, 0, 0, 0};
     const int want_ev = (energy != NULL) || (virial != NULL);
-    const int ix = (int)ceil(rcut / L[0]), iy = (int)ceil(rcut / L[1]), iz = (int)ceil(rcut / L[2]);
-    const double rc2 = rcut * rcut;
     const double two_a_spi = 2.0 * alpha / sqrt(ORACLE_PI);
 
     memset(force, 0, sizeof(double) * 3 * (size_t)nt);
     /* real space */
+    cells_t cl;
+    cells_build(&cl, N, postype, L, rcut);
 #pragma omp parallel for schedule(dynamic, 4) reduction(+ : E) reduction(+ : W[:6])
     for (int t = 0; t < nt; ++t) {
         const int i = targets ? targets[t] : t;
-        double fi[3] = {0, 0, 0};
-        for (int j = 0; j < N; ++j) {
-            double d0[3] = {postype[4 * i] - postype[4 * j], postype[4 * i + 1] - postype[4 * j + 1],
-                            postype[4 * i + 2] - postype[4 * j + 2]};
-            min_image(d0, L);
-            for (int a = -ix; a <= ix; ++a)
-                for (int b = -iy; b <= iy; ++b)
-                    for (int c = -iz; c <= iz; ++c) {
-                        if (i == j && a == 0 && b == 0 && c == 0) continue;
-                        const double d[3] = {d0[0] + a * L[0], d0[1] + b * L[1], d0[2] + c * L[2]};
-                        const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-                        if (r2 >= rc2) continue;
-                        const double r = sqrt(r2);
-                        const double qq = q[i] * q[j] * ORACLE_INV_4PI;
-                        const double er = erfc(alpha * r);
-                        const double fdivr = qq * (er / r + two_a_spi * exp(-alpha * alpha * r2)) / r2;
-                        for (int k = 0; k < 3; ++k) fi[k] += fdivr * d[k];
-                        if (want_ev) {
-                            E += 0.5 * qq * er / r;
-                            W[0] += 0.5 * d[0] * d[0] * fdivr;
-                            W[1] += 0.5 * d[0] * d[1] * fdivr;
-                            W[2] += 0.5 * d[0] * d[2] * fdivr;
-                            W[3] += 0.5 * d[1] * d[1] * fdivr;
-                            W[4] += 0.5 * d[1] * d[2] * fdivr;
-                            W[5] += 0.5 * d[2] * d[2] * fdivr;
-                        }
-                    }
+        ewald_real_ctx c;
+        memset(&c, 0, sizeof c);
+        c.q = q;
+        c.qi = q[i];
+        c.alpha = alpha;
+        c.two_a_spi = two_a_spi;
+        c.want_ev = want_ev;
+        double ri[3];
+        for (int a = 0; a < 3; ++a) {
+            double f = postype[4 * i + a] / L[a] + 0.5;
+            ri[a] = (f - floor(f) - 0.5) * L[a];
         }
-        for (int k = 0; k < 3; ++k) force[3 * t + k] = fi[k];
+        cells_for_neighbours(&cl, postype, L, i, ri, rcut, ewald_real_pair, &c);
+        for (int k = 0; k < 3; ++k) force[3 * t + k] = c.f[k];
+        E += c.E;
+        for (int k = 0; k < 6; ++k) W[k] += c.W[k];
     }
+    cells_free(&cl);
+
     /* reciprocal space over the half space k_z > 0 or (k_z == 0 and (k_y > 0 or (k_y == 0 and k_x > 0))) */
     const int mx = (int)floor(kcut * L[0] / (2 * ORACLE_PI)), my = (int)floor(kcut * L[1] / (2 * ORACLE_PI)),
               mz = (int)floor(kcut * L[2] / (2 * ORACLE_PI));
     const double kc2 = kcut * kcut;
-    /* collect k vectors */
-    long nk = 0;
+    const double ux = 2 * ORACLE_PI / L[0], uy = 2 * ORACLE_PI / L[1], uz = 2 * ORACLE_PI / L[2];
+    krow_t* rows = malloc(sizeof(krow_t) * (size_t)(mz + 1) * (2 * my + 1));
+    long nrow = 0, nk = 0;
     for (int c = 0; c <= mz; ++c)
-        for (int b = -my; b <= my; ++b)
-            for (int a = -mx; a <= mx; ++a) {
-                if (c == 0 && (b < 0 || (b == 0 && a <= 0))) continue;
-                const double kx = 2 * ORACLE_PI * a / L[0], ky = 2 * ORACLE_PI * b / L[1], kz = 2 * ORACLE_PI * c / L[2];
-                if (kx * kx + ky * ky + kz * kz > kc2) continue;
-                ++nk;
-            }
-    double* kv = malloc(sizeof(double) * 3 * (size_t)(nk > 0 ? nk : 1));
-    nk = 0;
-    for (int c = 0; c <= mz; ++c)
-        for (int b = -my; b <= my; ++b)
-            for (int a = -mx; a <= mx; ++a) {
-                if (c == 0 && (b < 0 || (b == 0 && a <= 0))) continue;
-                const double kx = 2 * ORACLE_PI * a / L[0], ky = 2 * ORACLE_PI * b / L[1], kz = 2 * ORACLE_PI * c / L[2];
-                if (kx * kx + ky * ky + kz * kz > kc2) continue;
-                kv[3 * nk] = kx;
-                kv[3 * nk + 1] = ky;
-                kv[3 * nk + 2] = kz;
-                ++nk;
-            }
-    double* Sre = calloc((size_t)(nk > 0 ? nk : 1), sizeof(double));
-    double* Sim = calloc((size_t)(nk > 0 ? nk : 1), sizeof(double));
-#pragma omp parallel for schedule(static)
-    for (long k = 0; k < nk; ++k) {
-        double sr = 0, si = 0;
-        for (int j = 0; j < N; ++j) {
-            const double ph = kv[3 * k] * postype[4 * j] + kv[3 * k + 1] * postype[4 * j + 1] + kv[3 * k + 2] * postype[4 * j + 2];
-            sr += q[j] * cos(ph);
-            si -= q[j] * sin(ph);
+        for (int b = -my; b <= my; ++b) {
+            if (c == 0 && b < 0) continue;
+            const double rest = kc2 - (b * uy) * (b * uy) - (c * uz) * (c * uz);
+            if (rest < 0) continue;
+            int am = (int)floor(sqrt(rest) / ux);
+            if (am > mx) am = mx;
+            while (am >= 0 && (am * ux) * (am * ux) > rest) --am;
+            if (am < 0) continue;
+            int a_lo = -am;
+            if (c == 0 && b == 0) a_lo = 1;
+            if (a_lo > am) continue;
+            rows[nrow].b = b;
+            rows[nrow].c = c;
+            rows[nrow].a_lo = a_lo;
+            rows[nrow].a_hi = am;
+            rows[nrow].off = nk;
+            nk += am - a_lo + 1;
+            ++nrow;
         }
-        Sre[k] = sr;
-        Sim[k] = si;
+    const size_t nkz = (size_t)(nk > 0 ? nk : 1);
+    double* Sre = calloc(nkz, sizeof(double));
+    double* Sim = calloc(nkz, sizeof(double));
+    double* gk = malloc(sizeof(double) * nkz);
+    for (long r = 0; r < nrow; ++r)
+        for (int a = rows[r].a_lo; a <= rows[r].a_hi; ++a) {
+            const double kx = a * ux, ky = rows[r].b * uy, kz = rows[r].c * uz;
+            const double k2 = kx * kx + ky * ky + kz * kz;
+            gk[rows[r].off + a - rows[r].a_lo] = exp(-k2 / (4 * alpha * alpha)) / k2;
+        }
+    const int wx = 2 * mx + 1, wy = 2 * my + 1;
+#pragma omp parallel
+    {
+        double* sr = calloc(nkz, sizeof(double));
+        double* si = calloc(nkz, sizeof(double));
+        double* ex = malloc(sizeof(double) * 2 * wx);
+        double* ey = malloc(sizeof(double) * 2 * wy);
+        double* ez = malloc(sizeof(double) * 2 * (mz + 1));
+#pragma omp for schedule(static)
+        for (int j = 0; j < N; ++j) {
+            /* e^{-i k x_j} per axis wavenumber */
+            for (int a = -mx; a <= mx; ++a) {
+                ex[2 * (a + mx)] = cos(a * ux * postype[4 * j]);
+                ex[2 * (a + mx) + 1] = -sin(a * ux * postype[4 * j]);
+            }
+            for (int b = -my; b <= my; ++b) {
+                ey[2 * (b + my)] = cos(b * uy * postype[4 * j + 1]);
+                ey[2 * (b + my) + 1] = -sin(b * uy * postype[4 * j + 1]);
+            }
+            for (int c = 0; c <= mz; ++c) {
+                ez[2 * c] = cos(c * uz * postype[4 * j + 2]);
+                ez[2 * c + 1] = -sin(c * uz * postype[4 * j + 2]);
+            }
+            for (long r = 0; r < nrow; ++r) {
+                const double* yb = &ey[2 * (rows[r].b + my)];
+                const double* zc = &ez[2 * rows[r].c];
+                const double tr = q[j] * (yb[0] * zc[0] - yb[1] * zc[1]);
+                const double ti = q[j] * (yb[0] * zc[1] + yb[1] * zc[0]);
+                double* Sr = sr + rows[r].off - rows[r].a_lo;
+                double* Si = si + rows[r].off - rows[r].a_lo;
+                for (int a = rows[r].a_lo; a <= rows[r].a_hi; ++a) {
+                    const double xr = ex[2 * (a + mx)], xi = ex[2 * (a + mx) + 1];
+                    Sr[a] += tr * xr - ti * xi;
+                    Si[a] += tr * xi + ti * xr;
+                }
+            }
+        }
+#pragma omp critical
+        for (size_t k = 0; k < (size_t)nk; ++k) {
+            Sre[k] += sr[k];
+            Sim[k] += si[k];
+        }
+        free(sr);
+        free(si);
+        free(ex);
+        free(ey);
+        free(ez);
     }
     double Ek = 0.0, Wk[6] = {0, 0, 0, 0, 0, 0};
     if (want_ev) {
-        for (long k = 0; k < nk; ++k) {
-            const double kx = kv[3 * k], ky = kv[3 * k + 1], kz = kv[3 * k + 2];
-            const double k2 = kx * kx + ky * ky + kz * kz;
-            const double g = exp(-k2 / (4 * alpha * alpha)) / k2;
-            const double s2 = Sre[k] * Sre[k] + Sim[k] * Sim[k];
-            /* half space: factor 2 */
-            const double e = g * s2 / V;
-            Ek += e;
-            const double fac = -2.0 / k2 * (1.0 + k2 / (4 * alpha * alpha));
-            Wk[0] += e * (1.0 + fac * kx * kx);
-            Wk[1] += e * (fac * kx * ky);
-            Wk[2] += e * (fac * kx * kz);
-            Wk[3] += e * (1.0 + fac * ky * ky);
-            Wk[4] += e * (fac * ky * kz);
-            Wk[5] += e * (1.0 + fac * kz * kz);
-        }
+        for (long r = 0; r < nrow; ++r)
+            for (int a = rows[r].a_lo; a <= rows[r].a_hi; ++a) {
+                const long k = rows[r].off + a - rows[r].a_lo;
+                const double kx = a * ux, ky = rows[r].b * uy, kz = rows[r].c * uz;
+                const double k2 = kx * kx + ky * ky + kz * kz;
+                const double s2 = Sre[k] * Sre[k] + Sim[k] * Sim[k];
+                /* half space: factor 2 */
+                const double e = gk[k] * s2 / V;
+                Ek += e;
+                const double fac = -2.0 / k2 * (1.0 + k2 / (4 * alpha * alpha));
+                Wk[0] += e * (1.0 + fac * kx * kx);
+                Wk[1] += e * (fac * kx * ky);
+                Wk[2] += e * (fac * kx * kz);
+                Wk[3] += e * (1.0 + fac * ky * ky);
+                Wk[4] += e * (fac * ky * kz);
+                Wk[5] += e * (1.0 + fac * kz * kz);
+            }
     }
-#pragma omp parallel for schedule(static)
-    for (int t = 0; t < nt; ++t) {
-        const int j = targets ? targets[t] : t;
-        double f[3] = {0, 0, 0};
-        for (long k = 0; k < nk; ++k) {
-            const double kx = kv[3 * k], ky = kv[3 * k + 1], kz = kv[3 * k + 2];
-            const double k2 = kx * kx + ky * ky + kz * kz;
-            const double g = exp(-k2 / (4 * alpha * alpha)) / k2;
-            const double ph = kx * postype[4 * j] + ky * postype[4 * j + 1] + kz * postype[4 * j + 2];
-            /* Im(e^{i k r_j} S(k)), times 2 for the half space */
-            const double im = sin(ph) * Sre[k] + cos(ph) * Sim[k];
-            const double s = 2.0 * q[j] * g / V * im;
-            f[0] += s * kx;
-            f[1] += s * ky;
-            f[2] += s * kz;
+#pragma omp parallel
+    {
+        double* ex = malloc(sizeof(double) * 2 * wx);
+        double* ey = malloc(sizeof(double) * 2 * wy);
+        double* ez = malloc(sizeof(double) * 2 * (mz + 1));
+#pragma omp for schedule(dynamic, 16)
+        for (int t = 0; t < nt; ++t) {
+            const int j = targets ? targets[t] : t;
+            /* e^{+i k x_j} per axis wavenumber */
+            for (int a = -mx; a <= mx; ++a) {
+                ex[2 * (a + mx)] = cos(a * ux * postype[4 * j]);
+                ex[2 * (a + mx) + 1] = sin(a * ux * postype[4 * j]);
+            }
+            for (int b = -my; b <= my; ++b) {
+                ey[2 * (b + my)] = cos(b * uy * postype[4 * j + 1]);
+                ey[2 * (b + my) + 1] = sin(b * uy * postype[4 * j + 1]);
+            }
+            for (int c = 0; c <= mz; ++c) {
+                ez[2 * c] = cos(c * uz * postype[4 * j + 2]);
+                ez[2 * c + 1] = sin(c * uz * postype[4 * j + 2]);
+            }
+            double f[3] = {0, 0, 0};
+            for (long r = 0; r < nrow; ++r) {
+                const double* yb = &ey[2 * (rows[r].b + my)];
+                const double* zc = &ez[2 * rows[r].c];
+                const double tr = yb[0] * zc[0] - yb[1] * zc[1];
+                const double ti = yb[0] * zc[1] + yb[1] * zc[0];
+                double gx = 0, gyz = 0;
+                for (int a = rows[r].a_lo; a <= rows[r].a_hi; ++a) {
+                    const long k = rows[r].off + a - rows[r].a_lo;
+                    const double xr = ex[2 * (a + mx)], xi = ex[2 * (a + mx) + 1];
+                    /* Im(e^{i k r_j} S(k)) weighted by G(k) */
+                    const double er = tr * xr - ti * xi, ei = tr * xi + ti * xr;
+                    const double im = gk[k] * (ei * Sre[k] + er * Sim[k]);
+                    gx += im * (a * ux);
+                    gyz += im;
+                }
+                f[0] += gx;
+                f[1] += gyz * (rows[r].b * uy);
+                f[2] += gyz * (rows[r].c * uz);
+            }
+            /* times 2 for the half space */
+            const double s = 2.0 * q[j] / V;
+            for (int k = 0; k < 3; ++k) force[3 * t + k] += s * f[k];
         }
-        for (int k = 0; k < 3; ++k) force[3 * t + k] += f[k];
+        free(ex);
+        free(ey);
+        free(ez);
     }
     if (want_ev) {
         double q2 = 0, qs = 0;
@@ -325,10 +526,21 @@ int oracle_ewald(int N, const double* postype, const double* q, const double* Lb
             virial[5] += Ebg;
         }
     }
-    free(kv);
+    free(rows);
+    free(gk);
     free(Sre);
     free(Sim);
     return 0;
+}
+
+/* OpenMP threads the oracle's parallel loops use */
+int oracle_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
 }
 
 /* ------------------------------------------------------------------------- */
@@ -480,6 +692,41 @@ static int cmp_double(const void* a, const void* b)
 {
     const double x = *(const double*)a, y = *(const double*)b;
     return (x > y) - (x < y);
+}
+
+/* per-target accumulator of the pair correction */
+typedef struct {
+    const oracle_esp_in* in;
+    int i;
+    double f[3], e, v[6];
+} esp_pair_ctx;
+
+static void pair_accumulate(esp_pair_ctx* c, const double* d, double fdivr, double dE)
+{
+    for (int a = 0; a < 3; ++a) c->f[a] += fdivr * d[a];
+    c->e += 0.5 * dE;
+    c->v[0] += 0.5 * d[0] * d[0] * fdivr;
+    c->v[1] += 0.5 * d[0] * d[1] * fdivr;
+    c->v[2] += 0.5 * d[0] * d[2] * fdivr;
+    c->v[3] += 0.5 * d[1] * d[1] * fdivr;
+    c->v[4] += 0.5 * d[1] * d[2] * fdivr;
+    c->v[5] += 0.5 * d[2] * d[2] * fdivr;
+}
+
+/* q_i q_j L(r) for a non-excluded pair within rc (EvaluatorPairPSWF.h:274-295) */
+static void esp_pair_term(void* vctx, int j, const double* d, double r2)
+{
+    esp_pair_ctx* c = (esp_pair_ctx*)vctx;
+    const oracle_esp_in* in = c->in;
+    const double qq = in->q[c->i] * in->q[j];
+    if (qq == 0.0) return;
+    if (in->n_ex)
+        for (unsigned int k = 0; k < in->n_ex[c->i]; ++k)
+            if (in->ex_list[(size_t)c->i * in->ex_pitch + k] == (unsigned)j) return;
+    const double r = sqrt(r2);
+    double Lr, nd;
+    lookup(in->screen, in->dscreen, in->r_lo, in->dr_inv, (unsigned)in->n_segs, in->rc, r, &Lr, &nd, NULL, NULL);
+    pair_accumulate(c, d, qq * nd / r, qq * Lr);
 }
 
 int oracle_esp(const oracle_esp_in* in, oracle_esp_out* out)
@@ -637,61 +884,49 @@ int oracle_esp(const oracle_esp_in* in, oracle_esp_out* out)
         out->force[4 * j + 2] = in->q[j] * ez / h[2];
     }
 
-    /* ---- computeRealSpaceCorrection: pair sum of q_i q_j L(r), all pairs within rc ---- */
-    /* exclusion lookup helper: pair (i,j) excluded? */
-#pragma omp parallel for schedule(dynamic, 8)
+    /* ---- computeRealSpaceCorrection: pair sum of q_i q_j L(r) over the pairs within rc (cell
+     * list; rc <= L/2, so each pair is its nearest image), then fixExclusions for the excluded
+     * pairs at any distance ---- */
+    cells_t cl;
+    cells_build(&cl, N, in->postype, L, in->rc);
+#pragma omp parallel for schedule(dynamic, 64)
     for (int i = 0; i < N; ++i) {
-        double f[3] = {0, 0, 0}, e = 0, v[6] = {0, 0, 0, 0, 0, 0};
-        for (int j = 0; j < N; ++j) {
-            if (j == i) continue;
-            int excluded = 0;
-            if (in->n_ex)
-                for (unsigned int k = 0; k < in->n_ex[i]; ++k)
-                    if (in->ex_list[(size_t)i * in->ex_pitch + k] == (unsigned)j) excluded = 1;
-            double d[3] = {in->postype[4 * i] - in->postype[4 * j], in->postype[4 * i + 1] - in->postype[4 * j + 1],
-                           in->postype[4 * i + 2] - in->postype[4 * j + 2]};
-            min_image(d, L);
-            const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-            const double r = sqrt(r2);
-            const double qq = in->q[i] * in->q[j];
-            if (!excluded) {
-                if (r2 >= in->rc * in->rc || qq == 0.0) continue;
-                double Lr, nd;
-                lookup(in->screen, in->dscreen, in->r_lo, in->dr_inv, (unsigned)in->n_segs, in->rc, r, &Lr, &nd, NULL, NULL);
-                const double fdivr = qq * nd / r;
-                for (int a = 0; a < 3; ++a) f[a] += fdivr * d[a];
-                e += 0.5 * qq * Lr;
-                v[0] += 0.5 * d[0] * d[0] * fdivr;
-                v[1] += 0.5 * d[0] * d[1] * fdivr;
-                v[2] += 0.5 * d[0] * d[2] * fdivr;
-                v[3] += 0.5 * d[1] * d[1] * fdivr;
-                v[4] += 0.5 * d[1] * d[2] * fdivr;
-                v[5] += 0.5 * d[2] * d[2] * fdivr;
-            } else {
+        esp_pair_ctx c;
+        memset(&c, 0, sizeof c);
+        c.in = in;
+        c.i = i;
+        double ri[3];
+        for (int a = 0; a < 3; ++a) {
+            double f = in->postype[4 * i + a] / L[a] + 0.5;
+            ri[a] = (f - floor(f) - 0.5) * L[a];
+        }
+        cells_for_neighbours(&cl, in->postype, L, i, ri, in->rc, esp_pair_term, &c);
+        if (in->n_ex)
+            for (unsigned int k = 0; k < in->n_ex[i]; ++k) {
                 /* fixExclusions (ESPForceComputeGPU.cu:441-458): remove the mesh's M(r) = 1/(4 pi r) - L(r)
                  * for the excluded pair, at any distance */
+                const unsigned int j = in->ex_list[(size_t)i * in->ex_pitch + k];
+                double d[3] = {in->postype[4 * i] - in->postype[4 * j], in->postype[4 * i + 1] - in->postype[4 * j + 1],
+                               in->postype[4 * i + 2] - in->postype[4 * j + 2]};
+                min_image(d, L);
+                const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
                 if (r2 < 1.0e-20) continue;
+                const double r = sqrt(r2);
+                const double qq = in->q[i] * in->q[j];
                 double Lr = 0, nd = 0;
                 if (r <= in->rc)
                     lookup(in->screen, in->dscreen, in->r_lo, in->dr_inv, (unsigned)in->n_segs, in->rc, r, &Lr, &nd, NULL, NULL);
                 const double dE = qq * (Lr - ORACLE_INV_4PI / r);
                 const double fdivr = qq * (nd - ORACLE_INV_4PI / r2) / r;
-                for (int a = 0; a < 3; ++a) f[a] += fdivr * d[a];
-                e += 0.5 * dE;
-                v[0] += 0.5 * d[0] * d[0] * fdivr;
-                v[1] += 0.5 * d[0] * d[1] * fdivr;
-                v[2] += 0.5 * d[0] * d[2] * fdivr;
-                v[3] += 0.5 * d[1] * d[1] * fdivr;
-                v[4] += 0.5 * d[1] * d[2] * fdivr;
-                v[5] += 0.5 * d[2] * d[2] * fdivr;
+                pair_accumulate(&c, d, fdivr, dE);
             }
-        }
-        out->force[4 * i + 0] += f[0];
-        out->force[4 * i + 1] += f[1];
-        out->force[4 * i + 2] += f[2];
-        out->force[4 * i + 3] += e;
-        for (int c = 0; c < 6; ++c) out->virial[(size_t)c * out->virial_pitch + i] += v[c];
+        out->force[4 * i + 0] += c.f[0];
+        out->force[4 * i + 1] += c.f[1];
+        out->force[4 * i + 2] += c.f[2];
+        out->force[4 * i + 3] += c.e;
+        for (int k = 0; k < 6; ++k) out->virial[(size_t)k * out->virial_pitch + i] += c.v[k];
     }
+    cells_free(&cl);
 
     /* ---- self and neutralising-background terms ---- */
     double q2 = 0, qs = 0;
