@@ -6,18 +6,28 @@ p50/p99 step latency beside it.  A step is one full ESP force evaluation (pair c
 the neighbour list, tile binning, PSWF spreading, R2C FFT, influence multiply with the FP64
 energy/virial reduction, C2R FFT, gather) of a system whose inputs are resident in HBM.
 
-Legs (each reports its measured force error against the tight FP64 reference of
-esp_hoomd_amd.accuracy, computed on the GPU for the same positions):
+The 1,000,000-particle electrolyte (density 0.1, +-1) starts from a jittered simple-cubic
+lattice and is equilibrated by 1,000 Langevin steps (T = 1, gamma = 1, seed 4) on the GPU before
+anything is measured.  Legs (each reports its measured force error against the tight FP64
+reference of esp_hoomd_amd.accuracy, computed on the GPU for the same positions; the test
+suite ties that reference to the direct Ewald sum at this size):
 
-* literal: the 1,000,000-particle electrolyte (density 0.1, jittered lattice, +-1) with the
-  parameters of the reference's GPU configuration (mesh 128^3, P = 10, r_c = 3.5) in FP64;
-* fp64 / mixed: the same system with parameters autotuned (and error-verified) for 1e-5, in
-  FP64 and in mixed precision -- the mixed leg is the headline `value`;
+* literal: the parameters of the reference's GPU configuration (mesh 128^3, P = 10, r_c = 3.5)
+  in FP64;
+* fp64 / mixed: parameters autotuned (and error-verified) for 1e-5, in FP64 and in mixed
+  precision -- the mixed leg is the headline `value`;
+* md: NVE MD steps with the headline configuration plus WCA (integration, neighbour-list
+  upkeep and rebuilds included), wall-clock timed;
+* shuffled: the headline configuration with the particles in a random order;
 * batched: 512 independent electrolytes, N log-uniform in [500, 50,000], +-1 with 10 % +-2,
   density 0.1, per-system parameters from the autotuner (meshes 16^3-64^3, P in {6, 8, 10}),
   mixed precision, one launch per stage replayed from a captured hipGraph;
 * served: a Poisson stream of MD requests (N log-uniform in [1e3, 2e5], density U[0.05, 0.2],
-  10-1000 steps) through the continuous-batching server for a fixed wall time.
+  10-1000 steps) through the continuous-batching server for a fixed wall time; every finished
+  request's force error is measured afterwards.
+
+`cpu_baseline` times the oracle (FP64 CPU ESP on OpenMP) on the same equilibrated system with
+the headline parameters.
 
 `roofline` names the dominant kernel (from a per-stage HIP-event pass outside the timed region)
 and divides its algorithmic bytes by its average launch time; for the pair kernel that time
@@ -44,26 +54,6 @@ sys.path.insert(0, str(ROOT))
 METRIC = "particle-steps/s at 1e-5 rel force err; batched solves/s + p50/p99 latency"
 HBM_PEAK_GBS = 8000.0
 TARGET = 1e-5
-
-
-def lattice_electrolyte(N, rho, seed_pos=2, seed_q=3, jitter=0.1):
-    """Seeded jittered simple-cubic lattice of N = m^3 sites at number density rho,
-    box centred on the origin, neutral +-1 charges in a random permutation."""
-    m = int(round(N ** (1.0 / 3.0)))
-    if m ** 3 != N:
-        raise ValueError("N must be a perfect cube")
-    L = (N / rho) ** (1.0 / 3.0)
-    a = L / m
-    g = (np.arange(m) + 0.5) * a - L / 2
-    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
-    pos = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
-    rng = np.random.default_rng(seed_pos)
-    pos += rng.uniform(-jitter * a, jitter * a, pos.shape)
-    q = np.where(np.arange(N) < N // 2, 1.0, -1.0)
-    if N % 2:
-        q[-1] = 0.0
-    q = np.random.default_rng(seed_q).permutation(q)
-    return pos, q, L
 
 
 def kernel_bytes(stage, N, n_charged, n_pairs, M, Mh, R, want_ev=True):
@@ -132,33 +122,51 @@ def copy_calibration(mib, steps):
     return 0
 
 
-def cpu_baseline(h, rc, P, kappa, n_mesh=48, reps=2):
-    """The oracle (FP64 CPU ESP, one thread) on a same-density sample with the same h, r_c, P."""
+def _cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(pos, q, L, p, reps=1):
+    """The oracle (FP64 CPU ESP: spreading, FFTs, influence, gather and a cell-list pair sum, its
+    parallel loops on OpenMP over the host cores) evaluating the same system with the same mesh,
+    P, r_c and kappa as the headline configuration."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle as O
     from esp_hoomd_amd import solver as S
 
-    L = n_mesh * h
-    m = int(round((0.1 * L ** 3) ** (1.0 / 3.0)))
-    N = m ** 3
-    pos, q, _ = lattice_electrolyte(N, N / L ** 3)
-    sigma = np.pi / (h * kappa)
-    D, cw = S.default_window(P, sigma)
-    w = S.build_window(P, D, cw)
-    scr, dscr, rlo, dri = S.build_split_table(0, kappa, rc, 4096)
-    o = O.OracleESP((L, L, L), (n_mesh,) * 3, P, w, scr, dscr, rlo, dri, rc, k_cut=1.2 * kappa)
-    o.compute(pos[:64], q[:64])  # warm the library
+    N = len(pos)
+    h = L / p.nx
+    sigma = np.pi / (h * p.kappa)
+    D, cw = S.default_window(p.order, sigma)
+    w = S.build_window(p.order, D, cw)
+    scr, dscr, rlo, dri = S.build_split_table(0, p.kappa, p.rcut, 4096)
+    o = O.OracleESP((L, L, L), (p.nx, p.ny, p.nz), p.order, w, scr, dscr, rlo, dri, p.rcut, k_cut=1.2 * p.kappa)
+    o.compute(pos[:64], q[:64])  # load the library
     t0 = time.perf_counter()
     for _ in range(reps):
         o.compute(pos, q)
     dt = (time.perf_counter() - t0) / reps
+    threads = O.num_threads()
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count()
     return {
         "value": N / dt,
         "unit": "particle-steps/s",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
-        "sample": f"oracle FP64 ESP (C, one thread, brute-force pair) on N={N} at density 0.1, mesh {n_mesh}^3 "
-                  f"(h={h:.4f}), P={P}, r_c={rc:.4g}, kappa={kappa:.4g}; mean of {reps} evaluations ({dt:.2f} s each)",
+        "nproc": affinity,
+        "cpu_model": _cpu_model(),
+        "sample": f"the oracle (C, FP64, OpenMP on {threads} threads; cell-list pair sum, serial spreading) "
+                  f"evaluating the whole N={N} system once per step with the headline parameters (mesh {p.nx}^3, "
+                  f"P={p.order}, r_c={p.rcut:.4g}, kappa={p.kappa:.4g}); mean of {reps} evaluation(s), {dt:.2f} s each",
     }
 
 
@@ -218,6 +226,33 @@ def run_single(params, pos, q, L, steps, warmup, ref, profile=True, skin=0.4):
             "n_pairs": _n_pairs(st)}
 
 
+def md_leg(params, pos, vel, q, L, steps, warmup, skin=0.4):
+    """NVE velocity-Verlet MD of the resident system with this ESP configuration plus WCA
+    (sigma = epsilon = 1), dt = 0.002: per step the first half kick and drift, the neighbour
+    list's displacement check (one host synchronisation) and its rebuilds when a particle has
+    moved half the skin, the ESP forces, WCA over the inner list and the second half kick.
+    Timed by wall clock over `steps` steps after `warmup`."""
+    from esp_hoomd_amd.device import synchronize
+    from esp_hoomd_amd.md import ESPForce, NeighborList, Simulation, State, WCA
+    from esp_hoomd_amd.solver import Solver
+
+    st = State(pos, q, [L, L, L], velocities=vel)
+    sim = Simulation(st, dt=0.002, nlist=NeighborList(skin))
+    sim.add_force(ESPForce(Solver(params)))
+    sim.add_force(WCA(1.0, 1.0))
+    sim.run(warmup)
+    ke0, pe0 = sim.energies()
+    b0 = sim.nlist.builds
+    synchronize()
+    t0 = time.perf_counter()
+    sim.run(steps)
+    dt = time.perf_counter() - t0
+    ke, pe = sim.energies()
+    return {"steps": steps, "ms_per_step": dt / steps * 1e3, "particle_steps_per_s": len(pos) * steps / dt,
+            "list_builds": sim.nlist.builds - b0, "energy_change_per_KE0": abs(ke + pe - ke0 - pe0) / ke0,
+            "temperature": sim.temperature()}
+
+
 def roofline(res, params, N, n_charged, precision):
     """Dominant kernel by the per-stage pass; its launch time from the timed region's events
     when it is the pair kernel (the one the timed region marks)."""
@@ -246,19 +281,16 @@ def roofline(res, params, N, n_charged, precision):
 
 def batched_leg(n_sys, steps, warmup, seed=6):
     """512 independent electrolytes (or n_sys), tuned per system, one batched launch per stage."""
-    from esp_hoomd_amd import _lib, accuracy, tune
+    from esp_hoomd_amd import _lib, accuracy, tune, workloads as W
     from esp_hoomd_amd.batch import Batch
     from esp_hoomd_amd.device import Event, Stream, synchronize
     from esp_hoomd_amd.md import NeighborList, State
-    from esp_hoomd_amd.serve import lattice_positions, neutral_charges
     from esp_hoomd_amd.solver import Params, Solver
 
     rng = np.random.default_rng(seed)
-    Ns = (np.exp(rng.uniform(np.log(500), np.log(50_000), n_sys)).astype(int) // 2 * 2)
-    systems = []
-    for i, N in enumerate(Ns):
-        pos, L = lattice_positions(int(N), 0.1, seed=seed * 1000 + i)
-        systems.append((pos, neutral_charges(int(N), 0.1, seed * 1000 + i), L))
+    Ns = W.batch_sizes(n_sys, seed)
+    rng.uniform(size=n_sys)  # the sizes' draws (keeps the error sample below as before)
+    systems = W.batch_systems(n_sys, seed)
     # the model is fitted on random electrolytes; rescale it to these lattice systems with
     # measured errors on three of them (smallest, median, largest), then tune every system
     order = np.argsort(Ns)
@@ -347,12 +379,23 @@ def served_leg(seconds, rate, seed=7, budget_gb=64):
     from esp_hoomd_amd.serve import Server, poisson_requests
     n = int(rate * seconds * 1.5) + 8
     reqs = poisson_requests(n, rate=rate, seed=seed)
-    srv = Server(hbm_budget=int(budget_gb * (1 << 30)), max_active=128, precision=_lib.ESP_PREC_MIXED, target=TARGET)
+    srv = Server(hbm_budget=int(budget_gb * (1 << 30)), max_active=128, precision=_lib.ESP_PREC_MIXED, target=TARGET,
+                 keep_final_positions=True)
+    t0 = time.perf_counter()
+    scales = srv.calibrate()
+    calib_s = time.perf_counter() - t0
     for r in reqs:
         srv.submit(r)
     st = srv.run(time_limit=seconds)
     st["offered_rate_per_s"] = rate
     st["hbm_budget_bytes"] = int(budget_gb * (1 << 30))
+    # every finished request's force error at its final positions, against the tight reference
+    t0 = time.perf_counter()
+    errs = np.asarray(srv.verify_results()) if srv.done else np.zeros(1)
+    st["force_error"] = {"verified_results": len(srv.done), "p50": float(np.median(errs)),
+                         "p99": float(np.percentile(errs, 99)), "max": float(errs.max()),
+                         "over_target": int((errs > TARGET).sum()), "verify_s": time.perf_counter() - t0}
+    st["calibration"] = {"method_scale_by_density": scales, "seconds": calib_s}
     return st
 
 
@@ -362,8 +405,11 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--N", type=int, default=1_000_000)
-    ap.add_argument("--legs", default="literal,fp64,mixed,batched,served",
-                    help="comma list of literal, fp64, mixed, batched, served")
+    ap.add_argument("--legs", default="literal,fp64,mixed,md,shuffled,batched,served",
+                    help="comma list of literal, fp64, mixed, md, shuffled, batched, served")
+    ap.add_argument("--equilibrate", type=int, default=1000,
+                    help="Langevin steps (T = 1, gamma = 1, seed 4) before anything is measured")
+    ap.add_argument("--md-steps", type=int, default=300, help="timed NVE steps of the md leg")
     ap.add_argument("--batch-systems", type=int, default=512)
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -381,15 +427,26 @@ def main(argv=None):
     if args.calibrate_copy > 0:
         return copy_calibration(args.calibrate_copy, args.steps)
 
-    from esp_hoomd_amd import _lib, accuracy, tune
+    from esp_hoomd_amd import _lib, accuracy, tune, workloads as W
+    from esp_hoomd_amd.md import State
     from esp_hoomd_amd.solver import Params
 
     legs = set(args.legs.split(","))
-    pos, q, L = lattice_electrolyte(args.N, 0.1)
+    t_all = time.perf_counter()
+    pos, q, L = W.lattice_electrolyte(args.N, 0.1)
     box = (L, L, L)
     n_charged = int(np.count_nonzero(q))
     literal = Params(nx=128, ny=128, nz=128, order=10, kappa=np.pi / (1.3 * L / 128), rcut=3.5,
                      precision=_lib.ESP_PREC_FP64)
+    vel = None
+    eq_s = 0.0
+    if args.equilibrate > 0:
+        st = State(pos, q, list(box))
+        t0 = time.perf_counter()
+        W.equilibrate(st, literal, steps=args.equilibrate)
+        eq_s = time.perf_counter() - t0
+        pos, vel = st.positions().copy(), st.velocities().copy()
+        del st
 
     if args.only:
         prec = _lib.ESP_PREC_FP64 if args.only in ("literal", "fp64") else _lib.ESP_PREC_MIXED
@@ -404,7 +461,6 @@ def main(argv=None):
                           "stages_ms": res["stages_ms"], "roofline": roofline(res, p, args.N, n_charged, pname)}))
         return 0
 
-    t_all = time.perf_counter()
     ref = accuracy.reference_forces(pos, q, box)
     out_legs = {}
     if "literal" in legs:
@@ -421,6 +477,15 @@ def main(argv=None):
         tuned[name] = (ch, r)
         out_legs[name] = {"params": ch.describe(), **{k: r[k] for k in ("ms_per_step", "particle_steps_per_s", "error",
                                                                        "stages_ms")}}
+    head = "mixed" if "mixed" in tuned else ("fp64" if "fp64" in tuned else None)
+    if head is not None and "md" in legs:
+        out_legs["md"] = {"precision": head, **md_leg(tuned[head][0].params, pos, vel, q, L, args.md_steps, 20)}
+    if head is not None and "shuffled" in legs:
+        # the same system with its particles in a random order (no spatial locality of the
+        # neighbour records; HOOMD itself keeps particles sorted along a space-filling curve)
+        perm = np.random.default_rng(11).permutation(len(pos))
+        r = run_single(tuned[head][0].params, pos[perm], q[perm], L, args.steps, args.warmup, ref[perm], profile=False)
+        out_legs["shuffled"] = {"precision": head, **{k: r[k] for k in ("ms_per_step", "particle_steps_per_s", "error")}}
     if "batched" in legs:
         out_legs["batched"] = batched_leg(args.batch_systems, max(args.steps // 5, 5), 3)
     if "served" in legs:
@@ -429,13 +494,14 @@ def main(argv=None):
         mean_work = (2e5 - 1e3) / math.log(200) * (1000 - 10) / math.log(100)
         out_legs["served"] = served_leg(args.serve_seconds, rate=0.7 * cap_ps / mean_work)
 
-    head = "mixed" if "mixed" in tuned else ("fp64" if "fp64" in tuned else None)
     if head is not None:
         ch, r = tuned[head]
         value, ms, params, dtype = r["particle_steps_per_s"], r["ms_per_step"], ch.params, head
     else:
         r = None
         value, ms, params, dtype = None, None, literal, "fp64"
+    state_desc = (f"equilibrated {args.equilibrate} Langevin steps (T=1, gamma=1, dt=0.002) from a jittered simple-cubic "
+                  f"lattice" if args.equilibrate > 0 else "jittered simple-cubic lattice")
     out = {
         "metric": METRIC,
         "value": value,
@@ -450,12 +516,14 @@ def main(argv=None):
         "dtype": "f32 (FP64 influence/reductions)" if dtype == "mixed" else "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"1 system: N={args.N} primitive-model electrolyte (q=+-1, density 0.1, jittered simple-cubic "
-                        f"lattice), L={L:.3f}, {head or 'literal'} precision, ESP parameters autotuned to a measured "
+            "workload": f"1 system: N={args.N} primitive-model electrolyte (q=+-1, density 0.1), {state_desc}, "
+                        f"L={L:.3f}; a step is one full ESP force evaluation (pair correction + mesh + energy/virial) "
+                        f"of resident inputs; {head or 'literal'} precision, ESP parameters autotuned to a measured "
                         f"relative RMS force error <= {TARGET:g} against the tight FP64 reference: "
                         f"mesh {params.nx}^3, P={params.order}, r_c={params.rcut:.4g}, kappa={params.kappa:.4g}",
             "force_error": r["error"] if r else None,
             "target_error": TARGET,
+            "equilibration_s": eq_s,
         },
         "legs": out_legs,
     }
@@ -469,9 +537,8 @@ def main(argv=None):
             "mixed_batched": out_legs.get("batched", {}).get("particle_steps_per_s", 0.0) / f64,
             "mixed_served": out_legs.get("served", {}).get("particle_steps_per_s", 0.0) / f64,
         }
-    if not args.no_cpu_baseline:
-        p = params
-        out["cpu_baseline"] = cpu_baseline(L / p.nx, p.rcut, p.order, p.kappa)
+    if not args.no_cpu_baseline and head is not None:
+        out["cpu_baseline"] = cpu_baseline(pos, q, L, params)
     else:
         out["cpu_baseline"] = None
     out["bench_wall_s"] = time.perf_counter() - t_all

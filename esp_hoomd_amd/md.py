@@ -212,6 +212,21 @@ class WCA:
                                               stream.handle if stream else None))
 
 
+class ESPForce:
+    """The whole ESP force of one Solver (pair correction, mesh part, energy and virial) as a
+    Simulation force: one esp_compute_forces call per step that overwrites the net force."""
+
+    def __init__(self, solver):
+        self.solver = solver
+        self.r_cut = float(solver.params.rcut)
+
+    def _esp_flags(self) -> int:
+        return _lib.ESP_COMPUTE_ALL
+
+    def _esp_solver(self):
+        return self.solver
+
+
 class NVE:
     """Velocity-Verlet integration."""
 
@@ -399,4 +414,4 @@ class BatchMD:
             pass
 
 
-__all__ = ["State", "NeighborList", "WCA", "NVE", "Langevin", "Simulation", "BatchMD", "jittered_lattice", "random_packing"]
+__all__ = ["State", "NeighborList", "WCA", "ESPForce", "NVE", "Langevin", "Simulation", "BatchMD", "jittered_lattice", "random_packing"]

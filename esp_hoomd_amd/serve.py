@@ -21,8 +21,10 @@ requests: batched first half-kick and drift, one displacement check for all neig
 (only lists that need it are rebuilt), the batched ESP forces (hipGraph replay), batched WCA
 and the batched second half-kick.
 
-ESP parameters per request come from the autotuner's model at the force-error target
-(cached per size/density bucket).  The step is deterministic in mixed precision: a request
+ESP parameters per request come from the autotuner's model at the force-error target (cached
+per size/density bucket), with the model's method scale measured on probe requests
+(`calibrate`); `verify_results` measures every finished request's force error against the
+tight FP64 reference.  The step is deterministic in mixed precision: a request
 computes the same trajectory alone or in any mix (the tests check this).
 """
 from __future__ import annotations
@@ -42,31 +44,10 @@ from .device import synchronize
 from .md import BatchMD, NeighborList, State, WCA
 from .solver import Params, Solver
 from . import tune as _tune
+from .workloads import lattice_positions, neutral_charges
 
 WCA_SIGMA = 1.0
 SKIN = 0.4
-
-
-def lattice_positions(N: int, density: float, seed: int, jitter: float = 0.1):
-    """N particles on randomly chosen sites of a jittered simple-cubic lattice (cubic box of
-    volume N / density, centred on the origin): no close contacts, so MD starts cleanly."""
-    L = (N / density) ** (1.0 / 3.0)
-    m = int(math.ceil(N ** (1.0 / 3.0)))
-    a = L / m
-    rng = np.random.default_rng(seed)
-    sites = rng.permutation(m ** 3)[:N]
-    idx = np.stack([sites % m, (sites // m) % m, sites // (m * m)], 1)
-    pos = (idx + 0.5) * a - L / 2 + rng.uniform(-jitter * a, jitter * a, (N, 3))
-    return pos, L
-
-
-def neutral_charges(N: int, q2_frac: float, seed: int) -> np.ndarray:
-    """+-1 charges with a fraction q2_frac of +-2, each valence class neutral (N even)."""
-    rng = np.random.default_rng(seed + 7919)
-    n2 = int(N * q2_frac) // 2 * 2
-    n1 = N - n2
-    q = np.concatenate([np.tile([2.0, -2.0], n2 // 2), np.tile([1.0, -1.0], n1 // 2), np.zeros(n1 % 2)])
-    return rng.permutation(q)
 
 
 @dataclass
@@ -91,6 +72,7 @@ class Request:
     first_forces: Optional[np.ndarray] = field(default=None, repr=False)
     final_positions: Optional[np.ndarray] = field(default=None, repr=False)
     final_forces: Optional[np.ndarray] = field(default=None, repr=False)
+    force_error: Optional[float] = None  # relative RMS electrostatic force error (verify_results)
     prepared: Optional[tuple] = field(default=None, repr=False)
     admitted_wall: float = 0.0
     last_step_end: float = 0.0
@@ -119,13 +101,17 @@ class Server:
 
     def __init__(self, hbm_budget: int = 32 << 30, max_active: int = 64, precision: int = _lib.ESP_PREC_MIXED,
                  target: float = 1e-5, dt: float = 0.002, use_graph: bool = True, keep_results: bool = False,
-                 tuner_orders: Sequence[int] = (6, 8, 10), mesh_range: Tuple[int, int] = (16, 128)):
+                 tuner_orders: Sequence[int] = (6, 8, 10), mesh_range: Tuple[int, int] = (16, 128),
+                 keep_final_positions: bool = False):
         self.hbm_budget = int(hbm_budget)
         self.max_active = int(max_active)
         self.precision = precision
         self.target = target
         self.dt = dt
         self.keep_results = keep_results
+        self.keep_final_positions = keep_final_positions or keep_results
+        # accuracy-model method scale per probe density (calibrate()); 1.0 until calibrated
+        self.method_scales: Dict[float, float] = {}
         self.tuner_orders = tuple(tuner_orders)
         self.mesh_range = mesh_range
         self.batch = Batch(use_graph=use_graph)       # every resident request, every tick
@@ -156,12 +142,61 @@ class Server:
         req.prepared = (pos, q, L, params, footprint_bytes(len(pos), params, nbar))
         heapq.heappush(self.pending, (req.arrival, req.rid, req))
 
+    # ---- accuracy ---------------------------------------------------------------------
+    def calibrate(self, densities: Sequence[float] = (0.05, 0.1, 0.2), N: int = 20_000, q2_frac: float = 0.1,
+                  seed: int = 99) -> Dict[float, float]:
+        """Measure the accuracy model's method scale on probe requests (the request generator's
+        systems at each density) with the autotuner, which compares forces against the tight
+        FP64 reference and rescales the model until the target is met.  The relative error of
+        these lattice-started systems depends mostly on density (their RMS force falls with the
+        lattice spacing), so a request uses the scale of the nearest probe density at or below
+        its own (the larger scale of the two neighbours when between)."""
+        for rho in densities:
+            pos, L = lattice_positions(int(N), float(rho), seed)
+            q = neutral_charges(int(N), q2_frac, seed)
+            ch = _tune.autotune(pos, q, (L, L, L), self.target, self.precision, orders=self.tuner_orders,
+                                mesh_range=self.mesh_range, batched=True)
+            self.method_scales[float(rho)] = float(ch.method_scale)
+        self._tune_cache.clear()
+        return dict(self.method_scales)
+
+    def _scale_for(self, density: float) -> float:
+        if not self.method_scales:
+            return 1.0
+        rhos = sorted(self.method_scales)
+        below = [r for r in rhos if r <= density]
+        above = [r for r in rhos if r >= density]
+        cands = []
+        if below:
+            cands.append(self.method_scales[below[-1]])
+        if above:
+            cands.append(self.method_scales[above[0]])
+        return max(cands)
+
+    def verify_results(self, reqs: Optional[Sequence[Request]] = None) -> List[float]:
+        """Relative RMS electrostatic force error of each finished request at its final
+        positions: its ESP configuration against the tight FP64 reference (on the GPU, outside
+        the serving loop).  Needs keep_final_positions (or keep_results).  Sets
+        request.force_error and returns the errors."""
+        from .accuracy import forces_with, reference_forces, rel_rms_error
+        errs = []
+        for r in (self.done if reqs is None else reqs):
+            if r.final_positions is None:
+                raise RuntimeError("Server.verify_results: final positions were not kept")
+            _, q, L = r.setup()
+            box = (L, L, L)
+            r.force_error = rel_rms_error(forces_with(r.params, r.final_positions, q, box),
+                                          reference_forces(r.final_positions, q, box))
+            errs.append(r.force_error)
+        return errs
+
     def _params_for(self, N: int, L: float) -> Tuple[Params, float]:
         density = N / L ** 3
-        key = (int(round(math.log(N) * 8)), int(round(math.log(density) * 16)))
+        scale = self._scale_for(density)
+        key = (int(round(math.log(N) * 8)), int(round(math.log(density) * 16)), scale)
         if key not in self._tune_cache:
             ch = _tune.tune(N, (L, L, L), self.target, self.precision, orders=self.tuner_orders,
-                            mesh_range=self.mesh_range, batched=True)[0]
+                            mesh_range=self.mesh_range, batched=True, method_scale=scale)[0]
             self._tune_cache[key] = (ch.params, ch.c, ch.sigma)
         p, c, sigma = self._tune_cache[key]
         # same c and sigma, mesh spacing and cutoff rescaled to this box
@@ -170,7 +205,7 @@ class Server:
         rc = sigma * c * h / math.pi
         if rc + SKIN > 0.5 * L:
             ch = _tune.tune(N, (L, L, L), self.target, self.precision, orders=self.tuner_orders, mesh_range=self.mesh_range,
-                            batched=True)[0]
+                            batched=True, method_scale=scale)[0]
             return ch.params, ch.neighbours
         params = Params(nx=n, ny=n, nz=n, order=p.order, kappa=c / rc, rcut=rc, precision=self.precision, tile=p.tile)
         return params, density * 4.0 / 3.0 * math.pi * (rc + SKIN) ** 3
@@ -214,8 +249,9 @@ class Server:
         for r in self.active:
             if r.steps_done >= r.steps:
                 r.finished_at = now
-                if self.keep_results:
+                if self.keep_final_positions:
                     r.final_positions = r.state.positions().copy()
+                if self.keep_results:
                     r.final_forces = r.state.force.numpy().copy()
                 self.resident_bytes -= r.footprint
                 r.state = r.solver = r.nlist = None

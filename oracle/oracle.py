@@ -57,8 +57,15 @@ def load():
     lib.oracle_window_eval.restype = None
     lib.oracle_split_F.argtypes = [C.POINTER(EspIn), C.c_double, _DP, _DP]
     lib.oracle_split_F.restype = None
+    lib.oracle_num_threads.argtypes = []
+    lib.oracle_num_threads.restype = C.c_int
     _lib = lib
     return lib
+
+
+def num_threads() -> int:
+    """OpenMP threads the oracle's parallel loops use."""
+    return int(load().oracle_num_threads())
 
 
 def _p(a):

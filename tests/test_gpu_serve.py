@@ -69,3 +69,21 @@ def test_poisson_stream_statistics():
     assert 0 < st["step_latency_ms_p50"] <= st["step_latency_ms_p99"]
     assert st["particle_steps_per_s"] > 0
     assert st["batch"]["replays"] >= 1
+
+
+def test_served_results_meet_the_target_with_a_request_near_200k():
+    """A stream with a request of 199,998 particles among smaller ones: the server calibrates its
+    accuracy model on probe requests, and every result's force error at its final positions,
+    against the tight FP64 reference, is within 1e-5."""
+    srv = Server(precision=_lib.ESP_PREC_MIXED, keep_final_positions=True)
+    scales = srv.calibrate()
+    reqs = [_req(0, 199_998, 12, density=0.08), _req(1, 1500, 10, density=0.2),
+            _req(2, 30_000, 20, density=0.05, arrival=0.005), _req(3, 6000, 15, density=0.12, arrival=0.01)]
+    for r in reqs:
+        srv.submit(r)
+    st = srv.run()
+    errs = srv.verify_results()
+    print(f"calibrated scales {scales}; served errors {[f'{e:.2e}' for e in errs]}; "
+          f"params {[(r.N, r.params.nx, r.params.order, round(r.params.rcut, 3)) for r in srv.done]}", flush=True)
+    assert st["requests_done"] == 4
+    assert max(errs) <= 1e-5
