@@ -83,6 +83,14 @@ class EspInfo(C.Structure):
     ]
 
 
+class EspPairParam(C.Structure):
+    """EvaluatorPairPSWF::param_type (include/EvaluatorPairPSWF.h)."""
+    _fields_ = [
+        ("kappa", C.c_double), ("alpha", C.c_double), ("rcut", C.c_double), ("V_cut", C.c_double),
+        ("table_host", C.c_void_p), ("table_device", C.c_void_p), ("n_segs", C.c_uint), ("pad", C.c_uint * 3),
+    ]
+
+
 class EspError(RuntimeError):
     """Raised for a non-zero status from the C ABI."""
 
@@ -133,6 +141,9 @@ def load() -> C.CDLL:
     _sig(lib, "esp_get_table_entry", C.c_int, [_VP, C.c_uint, C.POINTER(EspTableEntry)])
     _sig(lib, "esp_get_split_table", C.c_int, [_VP, C.POINTER(C.c_uint), _DP, _DP, _DP, _DP])
     _sig(lib, "esp_set_split_table", C.c_int, [_VP, C.c_uint, _DP, _DP, _DP, _DP])
+    _sig(lib, "esp_get_pair_table", C.c_int, [_VP, C.POINTER(_VP), C.POINTER(_VP), C.POINTER(C.c_uint)])
+    _sig(lib, "esp_potential_pair_forces", C.c_int, [C.POINTER(EspSystem), C.POINTER(EspPairParam), C.c_double, C.c_int,
+                                                     C.POINTER(EspOut), _VP])
     _sig(lib, "esp_get_window", C.c_int, [_VP, C.POINTER(C.c_uint), C.POINTER(C.c_uint), _DP])
     _sig(lib, "esp_set_profiling", C.c_int, [_VP, C.c_int])
     _sig(lib, "esp_get_stage_times", C.c_int, [_VP, _DP, C.POINTER(C.c_uint)])

@@ -392,6 +392,13 @@ int esp_get_split_table(esp_ctx* ctx, unsigned int* n_segs, double* screen, doub
                    });
     }
 
+int esp_get_pair_table(esp_ctx* ctx, const double** host_table, const double** device_table, unsigned int* n_segs)
+    {
+    if (!ctx || !host_table || !device_table || !n_segs)
+        return ESP_ERR_ARG;
+    return guarded(ctx, [&] { ctx->solver.pair_table(host_table, device_table, n_segs); });
+    }
+
 int esp_set_split_table(esp_ctx* ctx, unsigned int n_segs, const double* screen, const double* dscreen, const double* r_lo,
                         const double* dr_inv)
     {

@@ -373,12 +373,36 @@ void EspSolver::build_host_tables()
     m_tables_built = true;
     }
 
+void EspSolver::pair_table(const double** host, const double** device, unsigned int* n_segs)
+    {
+    if (m_r_lo.empty() || m_need_init)
+        build_host_tables();
+    const unsigned int n = unsigned(m_r_lo.size());
+    m_pair_rec.assign(size_t(n) * 12, 0.0);
+    for (unsigned int i = 0; i < n; ++i)
+        {
+        m_pair_rec[size_t(i) * 12 + 0] = m_r_lo[i];
+        m_pair_rec[size_t(i) * 12 + 1] = m_dr_inv[i];
+        for (int c = 0; c < 5; ++c)
+            {
+            m_pair_rec[size_t(i) * 12 + 2 + c] = m_screen[size_t(i) * 5 + c];
+            m_pair_rec[size_t(i) * 12 + 7 + c] = m_dscreen[size_t(i) * 5 + c];
+            }
+        }
+    m_pair_rec_dev.resize(m_pair_rec.size() * sizeof(double));
+    check_hip(hipMemcpy(m_pair_rec_dev.get(), m_pair_rec.data(), m_pair_rec.size() * sizeof(double), hipMemcpyHostToDevice),
+              "hipMemcpy pair table");
+    *host = m_pair_rec.data();
+    *device = m_pair_rec_dev.as<double>();
+    *n_segs = n;
+    }
+
 size_t EspSolver::memory_bytes() const
     {
     size_t b = 0;
     const DeviceBuffer* bufs[] = {&m_split_tab, &m_wcoef, &m_tw[0], &m_tw[1], &m_tw[2], &m_G, &m_Gv, &m_qxd, &m_qwd, &m_w2[0], &m_w2[1],
                                   &m_w2[2], &m_mesh, &m_spec, &m_phi, &m_tile_count, &m_tile_start, &m_key, &m_slot, &m_rec,
-                                  &m_sidx, &m_qpart, &m_evpart, &m_qsum, &m_ev_internal, &m_pq, &m_compact};
+                                  &m_sidx, &m_qpart, &m_evpart, &m_qsum, &m_ev_internal, &m_pq, &m_compact, &m_pair_rec_dev};
     for (const DeviceBuffer* d : bufs)
         b += d->bytes();
     return b;

@@ -144,6 +144,9 @@ class EspSolver
     void set_split_table(unsigned int n_segs, const double* screen, const double* dscreen, const double* r_lo, const double* dr_inv);
     //! Build host tables now (no device work); used by introspection before the first step.
     void build_host_tables();
+    //! The split table as interleaved FP64 records [r_lo, dr_inv, S c0..c4, dS c0..c4], host and
+    //! device copies (for the HOOMD-concept pair evaluator); rebuilt on every call.
+    void pair_table(const double** host, const double** device, unsigned int* n_segs);
     double self_coeff() const { return m_self_coeff; }
     //! Reciprocal-space cutoff in use: 1.2 kappa by default for the prolate split (beyond the
     //! band limit, where deconvolving the window would only amplify round-off), 0 = none.
@@ -228,6 +231,8 @@ class EspSolver
     DeviceBuffer m_tile_count, m_tile_start;
     DeviceBuffer m_key, m_slot, m_rec, m_sidx;
     DeviceBuffer m_pq;          // packed (x, y, z, q) per particle for the pair pass
+    std::vector<double> m_pair_rec;  // interleaved FP64 split table for the pair evaluator
+    DeviceBuffer m_pair_rec_dev;
     DeviceBuffer m_qpart, m_evpart, m_qsum, m_ev_internal;
     unsigned int m_cap_particles = 0;
     };

@@ -50,6 +50,16 @@ class Params:
         return p
 
 
+def potential_pair_forces(system: "DeviceSystem", param: _lib.EspPairParam, r_cut: float, energy_shift: bool = False):
+    """HOOMD's generic pair loop over EvaluatorPairPSWF (esp_potential_pair_forces); host copies
+    of force (N, 4) and virial (6, N)."""
+    s = system.c_system()
+    o = system.c_out(_lib.ESP_COMPUTE_PAIR)
+    _lib.check(_lib.load().esp_potential_pair_forces(C.byref(s), C.byref(param), float(r_cut), int(energy_shift), C.byref(o), None))
+    _lib.check(_lib.load().esp_device_synchronize())
+    return {"force": system.force.numpy(), "virial": system.virial.numpy()[:, : system.N]}
+
+
 class DeviceSystem:
     """Device-resident particle state in the HOOMD layout."""
 
@@ -214,6 +224,16 @@ class Solver:
         _lib.check(lib.esp_get_split_table(self._ctx, C.byref(nn), _lib.dptr(scr), _lib.dptr(dscr), _lib.dptr(rlo),
                                            _lib.dptr(dri)), self._ctx)
         return scr.reshape(n, 5), dscr.reshape(n, 5), rlo, dri
+
+    def pair_param(self, V_cut: float = 0.0) -> _lib.EspPairParam:
+        """EvaluatorPairPSWF::param_type for this context's split (the table addresses stay valid
+        until the parameters or table change)."""
+        h, d, n = C.c_void_p(), C.c_void_p(), C.c_uint()
+        _lib.check(_lib.load().esp_get_pair_table(self._ctx, C.byref(h), C.byref(d), C.byref(n)), self._ctx)
+        p = _lib.EspPairParam()
+        p.kappa, p.alpha, p.rcut, p.V_cut = self.params.kappa, self.params.alpha, self.params.rcut, V_cut
+        p.table_host, p.table_device, p.n_segs = h.value, d.value, n.value
+        return p
 
     def set_split_table(self, screen, dscreen, r_lo, dr_inv) -> None:
         screen = np.ascontiguousarray(screen, np.float64).reshape(-1)

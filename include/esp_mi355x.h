@@ -148,6 +148,30 @@ int esp_compute_forces(esp_ctx* ctx, uint64_t timestep, const esp_system* sys, e
 /* the PotentialPair<EvaluatorPairPSWF> path alone (module.cc:73,79): pair + exclusions */
 int esp_pair_forces(esp_ctx* ctx, const esp_system* sys, esp_out* out, esp_stream_t stream);
 
+/* ---- the pair evaluator in HOOMD's concept (include/EvaluatorPairPSWF.h) ------------------
+ * esp_pair_param mirrors EvaluatorPairPSWF::param_type (EvaluatorPairPSWF.h:180-225 of the
+ * reference, plus the split table the evaluator reads): kappa, alpha, rcut, V_cut, the table's
+ * host and device addresses and its segment count. */
+typedef struct esp_pair_param {
+    double kappa, alpha, rcut, V_cut;
+    const double* table_host;
+    const double* table_device;
+    unsigned int n_segs;
+    unsigned int pad[3];  /* 64 bytes, as the 16-byte aligned param_type */
+} esp_pair_param;
+/* The context's split table for the evaluator: n_segs records of 12 doubles (r_lo, dr_inv,
+ * S c0..c4, dS/ds c0..c4, reduced s = r / rc), one host and one device copy owned by the
+ * context (getTablePtr, ESPForceCompute.cc:129-132, returns the host address).  Valid until the
+ * context's parameters or table change or the context is destroyed; call again after
+ * esp_set_params / esp_set_split_table. */
+int esp_get_pair_table(esp_ctx* ctx, const double** host_table, const double** device_table, unsigned int* n_segs);
+/* PotentialPairGPU<EvaluatorPairPSWF>::computeForces (module.cc:79): HOOMD's generic pair loop,
+ * one thread per particle over the full neighbour list, the evaluator of EvaluatorPairPSWF.h for
+ * every pair within r_cut; force, per-particle energy (half of each pair) and virial (half of
+ * each pair) overwritten.  Excluded pairs must be absent from the list, as HOOMD builds it. */
+int esp_potential_pair_forces(const esp_system* sys, const esp_pair_param* param, double r_cut, int energy_shift, esp_out* out,
+                              esp_stream_t stream);
+
 /* ---- batched solves --------------------------------------------------------------------
  * Many independent systems, one launch per stage (no counterpart in the reference, which
  * evaluates one system per ESPForceCompute).  System i is evaluated with the parameters,
