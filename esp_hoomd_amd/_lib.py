@@ -31,6 +31,7 @@ ESP_COMPUTE_PAIR = 1
 ESP_COMPUTE_MESH = 2
 ESP_COMPUTE_ENERGY_VIRIAL = 4
 ESP_COMPUTE_ALL = 7
+ESP_NOALLOC_MAX_PARTICLES = 262144
 ESP_N_STAGES = 12
 
 
@@ -129,6 +130,7 @@ def load() -> C.CDLL:
     _sig(lib, "esp_last_error", C.c_char_p, [_VP])
     _sig(lib, "esp_compute_forces", C.c_int, [_VP, C.c_uint64, C.POINTER(EspSystem), C.POINTER(EspOut), _VP])
     _sig(lib, "esp_pair_forces", C.c_int, [_VP, C.POINTER(EspSystem), C.POINTER(EspOut), _VP])
+    _sig(lib, "esp_pair_forces_noalloc", C.c_int, [_VP, C.POINTER(EspSystem), C.POINTER(EspOut), _VP])
     _sig(lib, "esp_batch_create", C.c_int, [C.POINTER(_VP)])
     _sig(lib, "esp_batch_destroy", C.c_int, [_VP])
     _sig(lib, "esp_batch_last_error", C.c_char_p, [_VP])

@@ -6,6 +6,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -29,6 +30,37 @@ struct esp_batch
     std::vector<esp::SystemView> views;
     std::vector<esp::OutputView> outs;
     };
+
+// ---- scratch of the no-allocation pair entry point ------------------------------------------
+// One packed (x, y, z, q) record of 32 bytes per particle for up to ESP_NOALLOC_MAX_PARTICLES
+// particles: 8 MiB that every GPU the library runs on holds until the process ends.  Host code
+// never passes `scratch` itself to a kernel: its device address comes from hipGetSymbolAddress,
+// once per device, with that device current.
+static constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(ESP_NOALLOC_MAX_PARTICLES) * 32;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+static constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+//! Device address of `scratch` on the current device (looked up on the first call per device).
+static void* scratch_on_current_device()
+    {
+    int dev = 0;
+    esp::check_hip(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+        throw esp::EspError(ESP_ERR_UNSUPPORTED, "device ordinal beyond the no-allocation scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev])
+        {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw esp::EspError(ESP_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+        }
+    return g_scratch_ptr[dev];
+    }
 
 namespace
 {
@@ -227,6 +259,26 @@ int esp_pair_forces(esp_ctx* ctx, const esp_system* sys, esp_out* out, esp_strea
     esp::OutputView o = to_out(out);
     o.flags = esp::COMPUTE_PAIR;
     return guarded(ctx, [&] { ctx->solver.compute(to_view(sys), o, static_cast<hipStream_t>(stream)); });
+    }
+
+int esp_pair_forces_noalloc(esp_ctx* ctx, const esp_system* sys, esp_out* out, esp_stream_t stream)
+    {
+    if (!ctx)
+        return ESP_ERR_ARG;
+    if (!sys || !out)
+        {
+        ctx->last_error = "esp_pair_forces_noalloc: null system or output";
+        return ESP_ERR_ARG;
+        }
+    return guarded(ctx,
+                   [&]
+                   {
+                       if (sys->N > ESP_NOALLOC_MAX_PARTICLES)
+                           throw esp::EspError(ESP_ERR_RANGE, "esp_pair_forces_noalloc: N exceeds ESP_NOALLOC_MAX_PARTICLES");
+                       void* p = scratch_on_current_device();
+                       ctx->solver.pair_forces_into(to_view(sys), to_out(out), static_cast<hipStream_t>(stream), p,
+                                                    NOALLOC_SCRATCH_BYTES);
+                   });
     }
 
 int esp_batch_create(esp_batch** b)

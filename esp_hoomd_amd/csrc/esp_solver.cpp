@@ -838,6 +838,42 @@ void EspSolver::compute(const SystemView& sys, const OutputView& out, hipStream_
     mark(N_STAGES, s);
     }
 
+void EspSolver::pair_forces_into(const SystemView& sys, const OutputView& out, hipStream_t s, void* scratch, size_t scratch_bytes)
+    {
+    if (m_need_init || !m_split_tab.get())
+        throw EspError(STATUS_STATE, "the context is not initialised: one esp_compute_forces or esp_pair_forces call (which "
+                                     "allocates) must follow esp_create / esp_set_params before the no-allocation pair entry point");
+    if (sys.N == 0)
+        return;
+    if (!sys.postype || !sys.charge || !out.force)
+        throw EspError(STATUS_ARG, "postype, charge and force must be device pointers");
+    if (!sys.nlist || !sys.n_neigh || !sys.head_list)
+        throw EspError(STATUS_ARG, "the pair correction needs n_neigh, nlist and head_list");
+    if (out.virial && out.virial_pitch < sys.N)
+        throw EspError(STATUS_ARG, "virial_pitch must be >= N");
+    if (sys.n_ex && (!sys.ex_list || sys.ex_pitch == 0))
+        throw EspError(STATUS_ARG, "exclusions need ex_list and ex_pitch");
+    if (!(sys.box[0] > 0 && sys.box[1] > 0 && sys.box[2] > 0))
+        throw EspError(STATUS_ARG, "box lengths must be > 0");
+    if (sys.box[3] != 0.0 || sys.box[4] != 0.0 || sys.box[5] != 0.0)
+        throw EspError(STATUS_UNSUPPORTED, "triclinic boxes (non-zero tilt) are not supported");
+    if (size_t(sys.N) * 32 > scratch_bytes)
+        throw EspError(STATUS_RANGE, "N exceeds the particles the no-allocation scratch holds");
+    kernel::PairArgs a = pair_args(sys, out);
+    a.pq = static_cast<double*>(scratch);
+    const int lanes = m_p.pair_lanes ? m_p.pair_lanes : 8;
+    if (m_p.precision == 0)
+        {
+        check_hip(kernel::pack_positions_charges<double>(a, s), "pack_positions_charges");
+        check_hip(kernel::pair_forces<double>(a, lanes, 0u, s), "pair_forces");
+        }
+    else
+        {
+        check_hip(kernel::pack_positions_charges<float>(a, s), "pack_positions_charges");
+        check_hip(kernel::pair_forces<float>(a, lanes, 0u, s), "pair_forces");
+        }
+    }
+
 void EspSolver::describe(const SystemView& sys, const OutputView& out, hipStream_t s, kernel::BatchSys& d)
     {
     if (!kernel::batch_order_supported(int(m_p.order)))

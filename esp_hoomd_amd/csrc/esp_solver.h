@@ -129,6 +129,12 @@ class EspSolver
 
     //! Full ESP evaluation: pair correction + mesh; outputs overwritten.
     void compute(const SystemView& sys, const OutputView& out, hipStream_t s);
+    //! The pair correction alone (the kernels of compute with COMPUTE_PAIR) with its packed
+    //! neighbour records in caller-provided device memory of `scratch_bytes` bytes (32 per
+    //! particle); allocates nothing.  The context must already be initialised (a previous
+    //! compute after the last parameter change), else STATUS_STATE; more particles than the
+    //! scratch holds give STATUS_RANGE.
+    void pair_forces_into(const SystemView& sys, const OutputView& out, hipStream_t s, void* scratch, size_t scratch_bytes);
     //! Prepare this solver for a batched call on `sys` (initialisation, scratch sizing) and
     //! describe its part of the batch in `d`; the batch launches the kernels.
     void describe(const SystemView& sys, const OutputView& out, hipStream_t s, kernel::BatchSys& d);

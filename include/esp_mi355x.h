@@ -148,6 +148,18 @@ int esp_compute_forces(esp_ctx* ctx, uint64_t timestep, const esp_system* sys, e
 /* the PotentialPair<EvaluatorPairPSWF> path alone (module.cc:73,79): pair + exclusions */
 int esp_pair_forces(esp_ctx* ctx, const esp_system* sys, esp_out* out, esp_stream_t stream);
 
+/* No counterpart in the reference: esp_pair_forces for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  The packed neighbour records
+ * live in a static device array of the library (32 bytes per particle, ESP_NOALLOC_MAX_PARTICLES
+ * particles, 8 MiB held on every GPU the library runs on until the process ends); nothing is
+ * allocated.  Same kernels and results as esp_pair_forces.  The context must have been
+ * initialised by one allocating call (esp_compute_forces or esp_pair_forces) after esp_create /
+ * esp_set_params, else ESP_ERR_STATE; N above the maximum gives ESP_ERR_RANGE.  All calls on a
+ * GPU share its one scratch array: calls that overlap in time on the same GPU (different
+ * streams or threads) race. */
+#define ESP_NOALLOC_MAX_PARTICLES 262144u
+int esp_pair_forces_noalloc(esp_ctx* ctx, const esp_system* sys, esp_out* out, esp_stream_t stream);
+
 /* ---- the pair evaluator in HOOMD's concept (include/EvaluatorPairPSWF.h) ------------------
  * esp_pair_param mirrors EvaluatorPairPSWF::param_type (EvaluatorPairPSWF.h:180-225 of the
  * reference, plus the split table the evaluator reads): kappa, alpha, rcut, V_cut, the table's
