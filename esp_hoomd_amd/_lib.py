@@ -143,6 +143,7 @@ def load() -> C.CDLL:
     _sig(lib, "esp_get_table_entry", C.c_int, [_VP, C.c_uint, C.POINTER(EspTableEntry)])
     _sig(lib, "esp_get_split_table", C.c_int, [_VP, C.POINTER(C.c_uint), _DP, _DP, _DP, _DP])
     _sig(lib, "esp_set_split_table", C.c_int, [_VP, C.c_uint, _DP, _DP, _DP, _DP])
+    _sig(lib, "esp_get_fixed_point_scale", C.c_int, [_VP, C.POINTER(C.c_float), C.POINTER(C.c_int)])
     _sig(lib, "esp_get_pair_table", C.c_int, [_VP, C.POINTER(_VP), C.POINTER(_VP), C.POINTER(C.c_uint)])
     _sig(lib, "esp_potential_pair_forces", C.c_int, [C.POINTER(EspSystem), C.POINTER(EspPairParam), C.c_double, C.c_int,
                                                      C.POINTER(EspOut), _VP])

@@ -444,6 +444,13 @@ int esp_get_split_table(esp_ctx* ctx, unsigned int* n_segs, double* screen, doub
                    });
     }
 
+int esp_get_fixed_point_scale(esp_ctx* ctx, float* scale, int* lowered)
+    {
+    if (!ctx || !scale || !lowered)
+        return ESP_ERR_ARG;
+    return guarded(ctx, [&] { ctx->solver.fixed_point_scale(*scale, *lowered); });
+    }
+
 int esp_get_pair_table(esp_ctx* ctx, const double** host_table, const double** device_table, unsigned int* n_segs)
     {
     if (!ctx || !host_table || !device_table || !n_segs)

@@ -21,6 +21,17 @@ template<typename T> __device__ __forceinline__ T wave_sum(T v)
     return v;
     }
 
+template<typename T> __device__ __forceinline__ T wave_max(T v)
+    {
+#pragma unroll
+    for (int off = WAVE / 2; off > 0; off >>= 1)
+        {
+        const T o = __shfl_xor(v, off, WAVE);
+        v = o > v ? o : v;
+        }
+    return v;
+    }
+
 //! Sum over aligned groups of G lanes (G a power of two <= 64).
 template<int G, typename T> __device__ __forceinline__ T group_sum(T v)
     {
@@ -271,14 +282,11 @@ __device__ __forceinline__ void lds_add(double* p, double v)
     {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
-__device__ __forceinline__ void lds_add(int* p, float v)
+//! Scale applied to a value before it enters the LDS accumulator: the step's fixed-point scale
+//! (fx[0], written by the tile scan) for FP32, 1 for FP64.
+template<typename R> __device__ __forceinline__ R acc_scale(const float* fx)
     {
-    __hip_atomic_fetch_add(p, round_to_int(v * float(FIXED_SCALE)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    }
-//! Scale applied to a value before it enters the LDS accumulator (fixed point for FP32).
-template<typename R> __device__ __forceinline__ constexpr R acc_scale()
-    {
-    return sizeof(R) == 4 ? R(FIXED_SCALE) : R(1);
+    return sizeof(R) == 4 ? R(fx[0]) : R(1);
     }
 //! LDS accumulation of a value already multiplied by acc_scale<R>().
 __device__ __forceinline__ void lds_add_scaled(double* p, double v)

@@ -26,7 +26,10 @@ hipError_t bin_particles(unsigned int n,
                          unsigned int* slot,
                          double* qpart,
                          hipStream_t s);
-hipError_t scan_tiles(unsigned int n_tiles, unsigned int* count, unsigned int* start, hipStream_t s);
+//! Exclusive scan of the tile counts (which it clears); with fx (FP32 meshes) it also writes the
+//! step's fixed-point scale from the largest tile count and |q| (qpart: 3 per binning block).
+hipError_t scan_tiles(unsigned int n_tiles, unsigned int* count, unsigned int* start, int n_q, const double* qpart, float w3max,
+                      float* fx, hipStream_t s);
 template<typename R>
 hipError_t scatter_particles(unsigned int n,
                              const unsigned int* group,
@@ -46,7 +49,7 @@ size_t spread_lds_bytes(const MeshGeom& g, int P, int D, bool fp64);
 size_t gather_lds_bytes(const MeshGeom& g, int P, int D, bool fp64);
 template<typename R>
 hipError_t spread(const MeshGeom& g, int P, int D, unsigned int n_tiles, const unsigned int* start, const void* rec,
-                  const SortedIdx* sidx, const void* wcoef, void* mesh, hipStream_t s);
+                  const SortedIdx* sidx, const void* wcoef, void* mesh, const float* fx, hipStream_t s);
 template<typename R>
 hipError_t gather(const MeshGeom& g, int P, int D, unsigned int n_tiles, const unsigned int* start, const void* rec,
                   const SortedIdx* sidx, const void* wcoef, const void* phi, double* force, hipStream_t s);
@@ -56,7 +59,7 @@ int fft_x_pairs_per_block(int nx, bool fp64);
 int fft_cols_per_block(int n, bool fp64);
 unsigned int fft_z_blocks(const MeshGeom& g, bool fp64);
 template<typename R>
-hipError_t fft_forward_x(const MeshGeom& g, const FFTPlan& px, const void* tw, void* mesh, void* spec, hipStream_t s);
+hipError_t fft_forward_x(const MeshGeom& g, const FFTPlan& px, const void* tw, void* mesh, void* spec, const float* fx, hipStream_t s);
 template<typename R>
 hipError_t fft_inverse_x(const MeshGeom& g, const FFTPlan& px, const void* tw, const void* spec, double scale, void* phi, hipStream_t s);
 template<typename R> hipError_t fft_y(const MeshGeom& g, const FFTPlan& py, const void* tw, void* spec, int sign, hipStream_t s);
@@ -140,7 +143,9 @@ struct BatchSys
     unsigned int* slot;
     void* rec;
     SortedIdx* sidx;
-    double* qpart;
+    double* qpart;           // 3 per binning block: sum q, sum q^2, max |q|
+    float* fx;               // fixed-point scale, its inverse, lowered flag (FP32)
+    float w3max;             // largest |w_x w_y w_z| of the window
     double* evpart;
     double* qsum;
     double* ev_out;

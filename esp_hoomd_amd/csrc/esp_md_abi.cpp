@@ -126,7 +126,7 @@ void do_build(esp_nlist* nl, const esp_system* sys, hipStream_t s)
         check_hip(esp::md::cell_count(N, sys->postype, cg, nl->count.as<unsigned int>(), nl->cell.as<unsigned int>(),
                                       nl->slot.as<unsigned int>(), s),
                   "cell_count");
-        check_hip(esp::kernel::scan_tiles(nc, nl->count.as<unsigned int>(), nl->start.as<unsigned int>(), s), "scan");
+        check_hip(esp::kernel::scan_tiles(nc, nl->count.as<unsigned int>(), nl->start.as<unsigned int>(), 0, nullptr, 0.0f, nullptr, s), "scan");
         check_hip(esp::md::cell_fill(N, nl->cell.as<unsigned int>(), nl->slot.as<unsigned int>(), nl->start.as<unsigned int>(),
                                      nl->members.as<unsigned int>(), s),
                   "cell_fill");

@@ -370,6 +370,12 @@ void EspSolver::build_host_tables()
         cw = math::default_window_c(P, sigma);
         }
     m_window = *cached_window(P, m_D, cw);
+    // largest |w_x w_y w_z| (bounds the fixed-point contributions of one particle to one cell)
+    double wmax = 0.0;
+    for (unsigned int k = 0; k < P; ++k)
+        for (int i = 0; i <= 512; ++i)
+            wmax = std::max(wmax, std::fabs(m_window.eval(k, double(i) / 512.0)));
+    m_w3max = float(wmax * wmax * wmax * 1.01);
     m_tables_built = true;
     }
 
@@ -402,7 +408,7 @@ size_t EspSolver::memory_bytes() const
     size_t b = 0;
     const DeviceBuffer* bufs[] = {&m_split_tab, &m_wcoef, &m_tw[0], &m_tw[1], &m_tw[2], &m_G, &m_Gv, &m_qxd, &m_qwd, &m_w2[0], &m_w2[1],
                                   &m_w2[2], &m_mesh, &m_spec, &m_phi, &m_tile_count, &m_tile_start, &m_key, &m_slot, &m_rec,
-                                  &m_sidx, &m_qpart, &m_evpart, &m_qsum, &m_ev_internal, &m_pq, &m_compact, &m_pair_rec_dev};
+                                  &m_sidx, &m_qpart, &m_evpart, &m_qsum, &m_ev_internal, &m_pq, &m_compact, &m_pair_rec_dev, &m_fx};
     for (const DeviceBuffer* d : bufs)
         b += d->bytes();
     return b;
@@ -541,6 +547,7 @@ void EspSolver::initialize(const SystemView& sys, hipStream_t s)
     m_evpart.resize(size_t(kernel::fft_z_blocks(g, fp64)) * 8 * 8, true);
     m_qsum.resize(2 * 8, true);
     m_ev_internal.resize(8 * 8, true);
+    m_fx.resize(4 * sizeof(float), true);
     m_cap_particles = 0;
 
     build_influence(s);
@@ -613,7 +620,7 @@ void EspSolver::ensure_particles(unsigned int n)
     m_slot.resize(size_t(cap) * 4);
     m_rec.resize(size_t(cap) * 4 * (fp64 ? 8 : 4));
     m_sidx.resize(size_t(cap) * sizeof(SortedIdx));
-    m_qpart.resize(size_t((cap + 255) / 256 + 1) * 2 * 8);
+    m_qpart.resize(size_t((cap + 255) / 256 + 1) * 3 * 8);
     m_cap_particles = cap;
     }
 
@@ -695,7 +702,9 @@ void EspSolver::run_mesh(const SystemView& sys, const OutputView& out, unsigned 
                                     m_key.as<unsigned int>(), m_slot.as<unsigned int>(), m_qpart.as<double>(), s),
               "bin_particles");
     mark(2, s);
-    check_hip(kernel::scan_tiles(m_n_tiles, m_tile_count.as<unsigned int>(), m_tile_start.as<unsigned int>(), s), "scan_tiles");
+    check_hip(kernel::scan_tiles(m_n_tiles, m_tile_count.as<unsigned int>(), m_tile_start.as<unsigned int>(), int(nqb),
+                                 m_qpart.as<double>(), m_w3max, sizeof(R) == 4 ? m_fx.as<float>() : nullptr, s),
+              "scan_tiles");
     mark(3, s);
     check_hip(kernel::scatter_particles<R>(n, sys.group_idx, sys.postype, sys.charge, g, P, m_key.as<unsigned int>(),
                                            m_slot.as<unsigned int>(), m_tile_start.as<unsigned int>(), m_rec.get(),
@@ -703,10 +712,11 @@ void EspSolver::run_mesh(const SystemView& sys, const OutputView& out, unsigned 
               "scatter_particles");
     mark(4, s);
     check_hip(kernel::spread<R>(g, P, D, m_n_tiles, m_tile_start.as<unsigned int>(), m_rec.get(), m_sidx.as<SortedIdx>(), m_wcoef.get(),
-                                m_mesh.get(), s),
+                                m_mesh.get(), m_fx.as<float>(), s),
               "spread");
     mark(5, s);
-    check_hip(kernel::fft_forward_x<R>(g, m_plan[0], m_tw[0].get(), m_mesh.get(), m_spec.get(), s), "fft_forward_x");
+    check_hip(kernel::fft_forward_x<R>(g, m_plan[0], m_tw[0].get(), m_mesh.get(), m_spec.get(), sizeof(R) == 4 ? m_fx.as<float>() : nullptr, s),
+              "fft_forward_x");
     mark(6, s);
     check_hip(kernel::fft_y<R>(g, m_plan[1], m_tw[1].get(), m_spec.get(), -1, s), "fft_y");
     mark(7, s);
@@ -921,10 +931,24 @@ void EspSolver::describe(const SystemView& sys, const OutputView& out, hipStream
     d.rec = m_rec.get();
     d.sidx = m_sidx.as<SortedIdx>();
     d.qpart = m_qpart.as<double>();
+    d.fx = fp64 ? nullptr : m_fx.as<float>();
+    d.w3max = m_w3max;
     d.evpart = m_evpart.as<double>();
     d.qsum = m_qsum.as<double>();
     d.ev_out = out.ext_energy_virial ? out.ext_energy_virial : m_ev_internal.as<double>();
     d.pair = pair_args(sys, out);
+    }
+
+void EspSolver::fixed_point_scale(float& scale, int& lowered)
+    {
+    float h[4] = {0, 0, 0, 0};
+    if (m_fx.get() && m_p.precision != 0)
+        {
+        check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        check_hip(hipMemcpy(h, m_fx.get(), sizeof(h), hipMemcpyDeviceToHost), "hipMemcpy fx");
+        }
+    scale = h[0];
+    lowered = h[2] != 0.0f ? 1 : 0;
     }
 
 void EspSolver::charge_sums(double& q, double& q2)

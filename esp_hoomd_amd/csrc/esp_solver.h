@@ -163,6 +163,9 @@ class EspSolver
     void charge_sums(double& q, double& q2);
     //! Tile edge and tile count chosen for the mesh.
     const MeshGeom& geometry() const { return m_geom; }
+    //! Fixed-point scale of the last FP32 step and whether the tile bound lowered it
+    //! (synchronises; 0 in FP64 or before the first step).
+    void fixed_point_scale(float& scale, int& lowered);
     //! Device pointers of the influence tables (half spectrum), for tests.
     const double* device_green() const { return m_G.as<double>(); }
     size_t memory_bytes() const;
@@ -240,6 +243,8 @@ class EspSolver
     std::vector<double> m_pair_rec;  // interleaved FP64 split table for the pair evaluator
     DeviceBuffer m_pair_rec_dev;
     DeviceBuffer m_qpart, m_evpart, m_qsum, m_ev_internal;
+    DeviceBuffer m_fx;          // FP32 mesh: fixed-point scale, its inverse, lowered flag (float[4])
+    float m_w3max = 1.0f;       // largest |w_x w_y w_z| of the window
     unsigned int m_cap_particles = 0;
     };
 

@@ -44,8 +44,10 @@ struct SortedIdx
     uint32_t idx;     // particle index into postype / force
     };
 
-//! Fixed-point scale for the deterministic mixed-precision mesh accumulation.
+//! Fixed-point scale for the deterministic mixed-precision mesh accumulation, for charges of
+//! magnitude <= 1: the scale in use is 2^23 / 2^ceil(log2 max|q|), halved further while the
+//! largest tile's charge could overflow the int32 LDS accumulators (chosen on the device each
+//! step by the tile scan, fixed_point_scale).
 constexpr double FIXED_SCALE = 8388608.0;  // 2^23
-constexpr double FIXED_INV_SCALE = 1.0 / 8388608.0;
 
     } // namespace esp

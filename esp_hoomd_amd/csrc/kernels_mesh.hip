@@ -13,17 +13,44 @@ namespace esp
 namespace dev
 {
 
-__device__ __forceinline__ double mesh_load_clear(double* p)
+__device__ __forceinline__ double mesh_load_clear(double* p, double /*inv_scale*/)
     {
     const double v = *p;
     *p = 0.0;
     return v;
     }
-__device__ __forceinline__ float mesh_load_clear(unsigned long long* p)
+__device__ __forceinline__ float mesh_load_clear(unsigned long long* p, double inv_scale)
     {
     const long long v = static_cast<long long>(*p);
     *p = 0ull;
-    return static_cast<float>(static_cast<double>(v) * FIXED_INV_SCALE);
+    return static_cast<float>(static_cast<double>(v) * inv_scale);
+    }
+
+//! The step's fixed-point scale for the FP32 mesh accumulation: fx[0] = S, fx[1] = 1 / S,
+//! fx[2] = 1 when the tile bound lowered S.  S = 2^23 / 2^ceil(log2 max|q|) (clamped to
+//! [1, 2^40]) keeps the quantisation step relative to the charges (2^-23 of the largest), and
+//! is halved while the most populated tile could overflow an int32 LDS cell: a cell of a tile
+//! box sums at most `cmax` contributions, each |q w_x w_y w_z| S <= max|q| w3max S rounded.
+//! Without halving, S depends only on the charges, so every tiling gives the same sums.
+__device__ __forceinline__ void fixed_point_scale(unsigned int cmax, double qmax, float w3max, float* fx)
+    {
+    const double qm = qmax > 0.0 ? qmax : 1.0;
+    int e = 0;
+    const double m = frexp(qm, &e);  // qm = m 2^e, m in [0.5, 1)
+    const int ceil_log2 = (m == 0.5) ? e - 1 : e;
+    int k = 23 - ceil_log2;
+    k = k < 0 ? 0 : (k > 40 ? 40 : k);
+    double S = ldexp(1.0, k);
+    float lowered = 0.0f;
+    const double limit = 0.9 * 2147483648.0;
+    while (S > 1.0 && double(cmax) * (qm * double(w3max) * S + 0.5) >= limit)
+        {
+        S *= 0.5;
+        lowered = 1.0f;
+        }
+    fx[0] = float(S);
+    fx[1] = float(1.0 / S);
+    fx[2] = lowered;
     }
 
 //! Stencil origin o = ceil(g - P/2) (wrapped into [0, n)) and offset t = o - (g - P/2) in [0, 1)
@@ -87,26 +114,29 @@ __global__ void __launch_bounds__(256) k_bin(unsigned int n,
     double q = 0.0;
     if (i < n)
         q = bin_one(i, group, postype, charge, g, P, tile_count, key, slot);
-    // block partial sums of q and q^2 (deterministic order)
-    __shared__ double sq[8], sq2[8];
-    double a = wave_sum(q), b = wave_sum(q * q);
+    // block partial sums of q and q^2 (deterministic order) and the largest |q|
+    __shared__ double sq[8], sq2[8], sqm[8];
+    double a = wave_sum(q), b = wave_sum(q * q), c = wave_max(fabs(q));
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (lane == 0)
         {
         sq[wave] = a;
         sq2[wave] = b;
+        sqm[wave] = c;
         }
     __syncthreads();
     if (threadIdx.x == 0)
         {
-        double s = 0, s2 = 0;
+        double s = 0, s2 = 0, sm = 0;
         for (int w = 0; w < int(blockDim.x >> 6); ++w)
             {
             s += sq[w];
             s2 += sq2[w];
+            sm = fmax(sm, sqm[w]);
             }
-        qpart[2 * blockIdx.x] = s;
-        qpart[2 * blockIdx.x + 1] = s2;
+        qpart[3 * blockIdx.x] = s;
+        qpart[3 * blockIdx.x + 1] = s2;
+        qpart[3 * blockIdx.x + 2] = sm;
         }
     }
 
@@ -114,12 +144,16 @@ __global__ void __launch_bounds__(256) k_bin(unsigned int n,
 //! step.  Rounds of 16 consecutive counts per thread: the 16 loads of a thread are independent
 //! and in flight together, the thread sums are scanned with wave shuffles, then across the 16
 //! waves.
-__device__ __forceinline__ void scan_one(unsigned int n_tiles, unsigned int* __restrict__ count, unsigned int* __restrict__ start)
+__device__ __forceinline__ void scan_one(unsigned int n_tiles, unsigned int* __restrict__ count, unsigned int* __restrict__ start,
+                                         int n_q, const double* __restrict__ qpart, float w3max, float* __restrict__ fx)
     {
     constexpr int CH = 16;
     __shared__ unsigned int wsum[16];
+    __shared__ unsigned int wmax[16];
+    __shared__ double wq[16];
     const unsigned int tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
     unsigned int carry = 0;
+    unsigned int cmax = 0;
     for (unsigned int base = 0; base < n_tiles; base += 1024u * CH)
         {
         const unsigned int b0 = base + tid * CH;
@@ -133,7 +167,10 @@ __device__ __forceinline__ void scan_one(unsigned int n_tiles, unsigned int* __r
             }
 #pragma unroll
         for (int k = 0; k < CH; ++k)
+            {
             s += v[k];
+            cmax = max(cmax, v[k]);
+            }
         // inclusive scan of the thread sums within the wave
         unsigned int x = s;
 #pragma unroll
@@ -177,11 +214,36 @@ __device__ __forceinline__ void scan_one(unsigned int n_tiles, unsigned int* __r
         }
     if (tid == 0)
         start[n_tiles] = carry;
+    if (fx)
+        {
+        // the most populated tile and the largest |q| (from the binning partials)
+        double qm = 0.0;
+        for (int i = int(tid); i < n_q; i += int(blockDim.x))
+            qm = fmax(qm, qpart[3 * i + 2]);
+        cmax = wave_max(cmax);
+        qm = wave_max(qm);
+        if (lane == 0)
+            {
+            wmax[wave] = cmax;
+            wq[wave] = qm;
+            }
+        __syncthreads();
+        if (tid == 0)
+            {
+            for (unsigned int w = 1; w < blockDim.x / 64u; ++w)
+                {
+                cmax = max(cmax, wmax[w]);
+                qm = fmax(qm, wq[w]);
+                }
+            fixed_point_scale(cmax, qm, w3max, fx);
+            }
+        }
     }
 
-__global__ void __launch_bounds__(1024) k_scan(unsigned int n_tiles, unsigned int* __restrict__ count, unsigned int* __restrict__ start)
+__global__ void __launch_bounds__(1024) k_scan(unsigned int n_tiles, unsigned int* __restrict__ count, unsigned int* __restrict__ start,
+                                              int n_q, const double* __restrict__ qpart, float w3max, float* __restrict__ fx)
     {
-    scan_one(n_tiles, count, start);
+    scan_one(n_tiles, count, start, n_q, qpart, w3max, fx);
     }
 
 //! Write group member i's record into its tile-sorted slot.
@@ -251,11 +313,13 @@ __device__ __forceinline__ void fft_x_r2c_block(int blk,
                                                 int pairs_per_block,
                                                 const C2<R>* __restrict__ tw_g,
                                                 typename Acc<R>::glob_t* __restrict__ mesh,
-                                                C2<R>* __restrict__ spec)
+                                                C2<R>* __restrict__ spec,
+                                                const float* __restrict__ fx)
     {
     extern __shared__ __align__(16) unsigned char smem[];
     const int nx = g.nx;
     const int ls = nx + 1;
+    const double inv_scale = fx ? double(fx[1]) : 1.0;
     C2<R>* tw = reinterpret_cast<C2<R>*>(smem);
     C2<R>* a = tw + nx;
     C2<R>* b = a + pairs_per_block * ls;
@@ -273,9 +337,9 @@ __device__ __forceinline__ void fft_x_r2c_block(int blk,
             const int l1 = l0 + 1;
             R v0 = R(0), v1 = R(0);
             if (l0 < nlines)
-                v0 = R(mesh_load_clear(&mesh[size_t(l0) * nx + x]));
+                v0 = R(mesh_load_clear(&mesh[size_t(l0) * nx + x], inv_scale));
             if (l1 < nlines)
-                v1 = R(mesh_load_clear(&mesh[size_t(l1) * nx + x]));
+                v1 = R(mesh_load_clear(&mesh[size_t(l1) * nx + x], inv_scale));
             return C2<R> {v0, v1};
         },
         [&](int idx, const C2<R>& v)
@@ -310,9 +374,10 @@ __global__ void __launch_bounds__(256) k_fft_x_r2c(MeshGeom g,
                                                    int pairs_per_block,
                                                    const C2<R>* __restrict__ tw_g,
                                                    typename Acc<R>::glob_t* __restrict__ mesh,
-                                                   C2<R>* __restrict__ spec)
+                                                   C2<R>* __restrict__ spec,
+                                                   const float* __restrict__ fx)
     {
-    fft_x_r2c_block<R>(blockIdx.x, g, plan, pairs_per_block, tw_g, mesh, spec);
+    fft_x_r2c_block<R>(blockIdx.x, g, plan, pairs_per_block, tw_g, mesh, spec, fx);
     }
 
 //! C2R along x for pairs of half-spectrum lines; phi = scale * real(IFFT).
@@ -693,8 +758,8 @@ __device__ __forceinline__ void finalize_one(int n_ev,
                 acc[c] += ev_part[size_t(i) * 8 + c];
     for (int i = threadIdx.x; i < n_q; i += blockDim.x)
         {
-        acc[7] += qpart[2 * i];
-        acc[8] += qpart[2 * i + 1];
+        acc[7] += qpart[3 * i];
+        acc[8] += qpart[3 * i + 1];
         }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -789,7 +854,7 @@ __device__ __forceinline__ double bin_key(unsigned int i,
 constexpr unsigned int KB_BIN_LDS_TILES = 1024;
 __global__ void __launch_bounds__(256) kb_bin(const BatchSys* __restrict__ ds, BatchList l)
     {
-    __shared__ double sq[4], sq2[4];
+    __shared__ double sq[4], sq2[4], sqm[4];
     __shared__ unsigned int lcount[KB_BIN_LDS_TILES];
     for (unsigned int it = blockIdx.x; it < l.n; it += gridDim.x)
         {
@@ -827,25 +892,28 @@ __global__ void __launch_bounds__(256) kb_bin(const BatchSys* __restrict__ ds, B
             }
         else if (i < d.n)
             q = bin_one(i, d.group, reinterpret_cast<const double4*>(d.postype), d.charge, d.g, d.P, d.tile_count, d.key, d.slot);
-        const double a = wave_sum(q), b = wave_sum(q * q);
+        const double a = wave_sum(q), b = wave_sum(q * q), c = wave_max(fabs(q));
         const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
         if (lane == 0)
             {
             sq[wave] = a;
             sq2[wave] = b;
+            sqm[wave] = c;
             }
         __syncthreads();
         if (threadIdx.x == 0)
             {
             // same summation order as k_bin, so batched and single results agree bit for bit
-            double s = 0, s2 = 0;
+            double s = 0, s2 = 0, sm = 0;
             for (int k = 0; k < 4; ++k)
                 {
                 s += sq[k];
                 s2 += sq2[k];
+                sm = fmax(sm, sqm[k]);
                 }
-            d.qpart[2 * w.local] = s;
-            d.qpart[2 * w.local + 1] = s2;
+            d.qpart[3 * w.local] = s;
+            d.qpart[3 * w.local + 1] = s2;
+            d.qpart[3 * w.local + 2] = sm;
             }
         __syncthreads();
         }
@@ -856,7 +924,7 @@ __global__ void __launch_bounds__(1024) kb_scan(const BatchSys* __restrict__ ds,
     for (unsigned int it = blockIdx.x; it < l.n; it += gridDim.x)
         {
         const BatchSys& d = ds[l.items[it].sys];
-        scan_one(d.n_tiles, d.tile_count, d.tile_start);
+        scan_one(d.n_tiles, d.tile_count, d.tile_start, int(d.n_qblk), d.qpart, d.w3max, d.fx);
         __syncthreads();
         }
     }
@@ -883,7 +951,7 @@ __global__ void __launch_bounds__(256) kb_fft_x_r2c(const BatchSys* __restrict__
         const BatchItem w = l.items[it];
         const BatchSys& d = ds[w.sys];
         fft_x_r2c_block<R>(int(w.local), d.g, d.plan[0], d.ppb_x, reinterpret_cast<const C2<R>*>(d.tw[0]),
-                           reinterpret_cast<typename Acc<R>::glob_t*>(d.mesh), reinterpret_cast<C2<R>*>(d.spec));
+                           reinterpret_cast<typename Acc<R>::glob_t*>(d.mesh), reinterpret_cast<C2<R>*>(d.spec), d.fx);
         __syncthreads();
         }
     }
@@ -967,9 +1035,10 @@ hipError_t bin_particles(unsigned int n,
     return hipGetLastError();
     }
 
-hipError_t scan_tiles(unsigned int n_tiles, unsigned int* count, unsigned int* start, hipStream_t s)
+hipError_t scan_tiles(unsigned int n_tiles, unsigned int* count, unsigned int* start, int n_q, const double* qpart, float w3max,
+                      float* fx, hipStream_t s)
     {
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, n_tiles, count, start);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, n_tiles, count, start, n_q, qpart, w3max, fx);
     return hipGetLastError();
     }
 
@@ -1020,14 +1089,14 @@ int fft_cols_per_block(int n, bool fp64)
     }
 
 template<typename R>
-hipError_t fft_forward_x(const MeshGeom& g, const FFTPlan& px, const void* tw, void* mesh, void* spec, hipStream_t s)
+hipError_t fft_forward_x(const MeshGeom& g, const FFTPlan& px, const void* tw, void* mesh, void* spec, const float* fx, hipStream_t s)
     {
     const int ppb = fft_x_pairs_per_block(g.nx, sizeof(R) == 8);
     const int npairs = (g.ny * g.nz + 1) / 2;
     const int nb = (npairs + ppb - 1) / ppb;
     const size_t lds = (size_t(g.nx) + 2 * size_t(ppb) * (g.nx + 1)) * sizeof(C2<R>);
     hipLaunchKernelGGL(k_fft_x_r2c<R>, dim3(nb), dim3(256), lds, s, g, px, ppb, reinterpret_cast<const C2<R>*>(tw),
-                       reinterpret_cast<typename Acc<R>::glob_t*>(mesh), reinterpret_cast<C2<R>*>(spec));
+                       reinterpret_cast<typename Acc<R>::glob_t*>(mesh), reinterpret_cast<C2<R>*>(spec), fx);
     return hipGetLastError();
     }
 
@@ -1074,7 +1143,7 @@ hipError_t fft_z_convolve(const MeshGeom& g, const FFTPlan& pz, const void* tw, 
     }
 
 #define ESP_INST_FFT(R)                                                                                                          \
-    template hipError_t fft_forward_x<R>(const MeshGeom&, const FFTPlan&, const void*, void*, void*, hipStream_t);                \
+    template hipError_t fft_forward_x<R>(const MeshGeom&, const FFTPlan&, const void*, void*, void*, const float*, hipStream_t);                \
     template hipError_t fft_inverse_x<R>(const MeshGeom&, const FFTPlan&, const void*, const void*, double, void*, hipStream_t);  \
     template hipError_t fft_y<R>(const MeshGeom&, const FFTPlan&, const void*, void*, int, hipStream_t);                        \
     template hipError_t fft_z_convolve<R>(const MeshGeom&, const FFTPlan&, const void*, void*, const double*, const double*, int, \

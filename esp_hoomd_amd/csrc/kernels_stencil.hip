@@ -215,7 +215,8 @@ __global__ void __launch_bounds__(256) k_spread(MeshGeom g,
                                                 const SortedRec<R>* __restrict__ rec,
                                                 const SortedIdx* __restrict__ sidx,
                                                 const R* __restrict__ wcoef,
-                                                typename Acc<R>::glob_t* __restrict__ mesh)
+                                                typename Acc<R>::glob_t* __restrict__ mesh,
+                                                const float* __restrict__ fx)
     {
     using L = typename Acc<R>::lds_t;
     constexpr int CH = chunk_size<R>();
@@ -244,6 +245,7 @@ __global__ void __launch_bounds__(256) k_spread(MeshGeom g,
         box[c] = L(0);
     for (int c = threadIdx.x; c < P * (D + 1); c += blockDim.x)
         wc[c] = wcoef[c];
+    const R sc = acc_scale<R>(fx);
     __syncthreads();
 
     const int lane = threadIdx.x & (WAVE - 1);
@@ -278,11 +280,11 @@ __global__ void __launch_bounds__(256) k_spread(MeshGeom g,
                         {
                         const int b = item / P;
                         const int a = item - b * P;
-                        const R qw = q * wp[a] * wp[(P + 1) + b];
+                        const R qw = q * wp[a] * wp[(P + 1) + b] * sc;
                         L* dst = box + (oz * B + oy + b) * B + ox + a;
 #pragma unroll
                         for (int c = 0; c < P; ++c)
-                            lds_add(dst + c * B2, qw * wz[c]);
+                            lds_add_scaled(dst + c * B2, qw * wz[c]);
                         }
                     }
                 }
@@ -311,6 +313,18 @@ __global__ void __launch_bounds__(256) k_spread(MeshGeom g,
 // ---------------------------------------------------------------------------
 // Gather
 // ---------------------------------------------------------------------------
+//! Add a particle's mesh force to the pair pass's result: one 32-byte load and one store.  Every
+//! particle is gathered by exactly one thread, after the pair pass that wrote its record (or the
+//! clear of the output) has completed, so no atomic is needed.
+__device__ __forceinline__ void add_mesh_force(double4* __restrict__ f, double fx, double fy, double fz)
+    {
+    double4 v = *f;
+    v.x += fx;
+    v.y += fy;
+    v.z += fz;
+    *f = v;
+    }
+
 template<typename R, int P>
 __global__ void __launch_bounds__(256) k_gather(MeshGeom g,
                                                 int D,
@@ -418,10 +432,7 @@ __global__ void __launch_bounds__(256) k_gather(MeshGeom g,
             if (p < n && sl == 0)
                 {
                 const double q = double(qs[p]);
-                double* f = reinterpret_cast<double*>(&force[idxs[p]]);
-                unsafeAtomicAdd(f + 0, q * double(ex * inv_hx));
-                unsafeAtomicAdd(f + 1, q * double(ey * inv_hy));
-                unsafeAtomicAdd(f + 2, q * double(ez * inv_hz));
+                add_mesh_force(force + idxs[p], q * double(ex * inv_hx), q * double(ey * inv_hy), q * double(ez * inv_hz));
                 }
             }
         __syncthreads();
@@ -478,7 +489,8 @@ __device__ __forceinline__ void spread_tile_tp(unsigned int tile,
                                                const SortedRec<R>* __restrict__ rec,
                                                const SortedIdx* __restrict__ sidx,
                                                const R* __restrict__ wcoef,
-                                               typename Acc<R>::glob_t* __restrict__ mesh)
+                                               typename Acc<R>::glob_t* __restrict__ mesh,
+                                               const float* __restrict__ fx)
     {
     using L = typename Acc<R>::lds_t;
     const unsigned int s0 = tile_start[tile], s1 = tile_start[tile + 1];
@@ -495,6 +507,7 @@ __device__ __forceinline__ void spread_tile_tp(unsigned int tile,
 
     static_assert(LPP == 1 || (LPP == 2 && P % 2 == 0), "1 or 2 lanes per particle (even P)");
     constexpr int PB = P / LPP;  // rows per lane
+    const R sc = acc_scale<R>(fx);
     const unsigned int n_items = (s1 - s0) * unsigned(LPP);
     for (unsigned int t = threadIdx.x; t < n_items; t += blockDim.x)
         {
@@ -516,7 +529,7 @@ __device__ __forceinline__ void spread_tile_tp(unsigned int tile,
         // fold the accumulator's fixed-point scale into the x weights once per particle
 #pragma unroll
         for (int a = 0; a < P; ++a)
-            wx[a] *= acc_scale<R>();
+            wx[a] *= sc;
 #pragma unroll
         for (int b = 0; b < PB; ++b)
             {
@@ -552,9 +565,10 @@ __global__ void __launch_bounds__(256) k_spread_tp(MeshGeom g,
                                                    const SortedRec<R>* __restrict__ rec,
                                                    const SortedIdx* __restrict__ sidx,
                                                    const R* __restrict__ wcoef,
-                                                   typename Acc<R>::glob_t* __restrict__ mesh)
+                                                   typename Acc<R>::glob_t* __restrict__ mesh,
+                                                   const float* __restrict__ fx)
     {
-    spread_tile_tp<R, P, LPP>(blockIdx.x, g, D, tile_start, rec, sidx, wcoef, mesh);
+    spread_tile_tp<R, P, LPP>(blockIdx.x, g, D, tile_start, rec, sidx, wcoef, mesh, fx);
     }
 
 //! LPP lanes per particle (1 or 2): with 2, lane `sub` of the (adjacent) pair sums rows b in
@@ -644,10 +658,7 @@ __device__ __forceinline__ void gather_tile_tp(unsigned int tile,
                 continue;
             }
         const double q = double(r.q);
-        double* f = reinterpret_cast<double*>(&force[id.idx]);
-        unsafeAtomicAdd(f + 0, q * double(ex * inv_hx));
-        unsafeAtomicAdd(f + 1, q * double(ey * inv_hy));
-        unsafeAtomicAdd(f + 2, q * double(ez * inv_hz));
+        add_mesh_force(force + id.idx, q * double(ex * inv_hx), q * double(ey * inv_hy), q * double(ez * inv_hz));
         }
     }
 
@@ -671,13 +682,13 @@ __global__ void __launch_bounds__(256) k_gather_tp(MeshGeom g,
 // the stencil).  A lane whose column lies in a particle's xy footprint adds
 // B = T+P-1 products; no LDS atomics, fixed accumulation order within a tile.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void glob_add_real(double* p, double v)
+__device__ __forceinline__ void glob_add_real(double* p, double v, double /*scale*/)
     {
     unsafeAtomicAdd(p, v);
     }
-__device__ __forceinline__ void glob_add_real(unsigned long long* p, float v)
+__device__ __forceinline__ void glob_add_real(unsigned long long* p, float v, double scale)
     {
-    atomicAdd(p, static_cast<unsigned long long>(__double2ll_rn(double(v) * FIXED_SCALE)));
+    atomicAdd(p, static_cast<unsigned long long>(__double2ll_rn(double(v) * scale)));
     }
 
 template<typename R, int P, int T>
@@ -688,7 +699,8 @@ __device__ __forceinline__ void spread_tile_own(unsigned int tile,
                                                 const SortedRec<R>* __restrict__ rec,
                                                 const SortedIdx* __restrict__ sidx,
                                                 const R* __restrict__ wcoef,
-                                                typename Acc<R>::glob_t* __restrict__ mesh)
+                                                typename Acc<R>::glob_t* __restrict__ mesh,
+                                                const float* __restrict__ fx)
     {
     constexpr int B = T + P - 1;
     constexpr int BZ = (B + 3) & ~3;             // z weights padded to a multiple of 4
@@ -795,12 +807,13 @@ __device__ __forceinline__ void spread_tile_own(unsigned int tile,
         tile_coords(g, tile, tx, ty, tz);
         const int gx = (tx * T + cx) % g.nx;
         const int gy = (ty * T + cy) % g.ny;
+        const double scale = sizeof(R) == 4 ? double(fx[0]) : 1.0;
 #pragma unroll
         for (int z = 0; z < B; ++z)
             if (col[z] != R(0))
                 {
                 const int gz = (tz * T + z) % g.nz;
-                glob_add_real(&mesh[(size_t(gz) * g.ny + gy) * g.nx + gx], col[z]);
+                glob_add_real(&mesh[(size_t(gz) * g.ny + gy) * g.nx + gx], col[z], scale);
                 }
         }
     }
@@ -812,9 +825,10 @@ __global__ void __launch_bounds__(1024) k_spread_own(MeshGeom g,
                                                      const SortedRec<R>* __restrict__ rec,
                                                      const SortedIdx* __restrict__ sidx,
                                                      const R* __restrict__ wcoef,
-                                                     typename Acc<R>::glob_t* __restrict__ mesh)
+                                                     typename Acc<R>::glob_t* __restrict__ mesh,
+                                                     const float* __restrict__ fx)
     {
-    spread_tile_own<R, P, T>(blockIdx.x, g, D, tile_start, rec, sidx, wcoef, mesh);
+    spread_tile_own<R, P, T>(blockIdx.x, g, D, tile_start, rec, sidx, wcoef, mesh, fx);
     }
 
 // ---------------------------------------------------------------------------
@@ -844,7 +858,7 @@ __global__ void __launch_bounds__(256) kb_spread(const kernel::BatchSys* __restr
             {
 #define ESP_CASE(PP)                                                                 \
     case PP:                                                                         \
-        spread_tile_tp<R, PP, 1>(w.local, d.g, d.D, d.tile_start, rec, d.sidx, wc, mesh); \
+        spread_tile_tp<R, PP, 1>(w.local, d.g, d.D, d.tile_start, rec, d.sidx, wc, mesh, d.fx); \
         break;
             ESP_BATCH_P_CASES(ESP_CASE)
 #undef ESP_CASE
@@ -897,7 +911,7 @@ size_t own_lds_bytes(int P, bool fp64)
 
 template<typename R, int P>
 static void launch_spread_own(const MeshGeom& g, int D, unsigned int n_tiles, const unsigned int* start, const void* rec,
-                              const SortedIdx* sidx, const void* wcoef, void* mesh, hipStream_t s)
+                              const SortedIdx* sidx, const void* wcoef, void* mesh, const float* fx, hipStream_t s)
     {
     const size_t lds = own_lds_bytes(P, sizeof(R) == 8);
     const unsigned int threads = unsigned((g.B * g.B + 63) / 64 * 64);
@@ -905,11 +919,11 @@ static void launch_spread_own(const MeshGeom& g, int D, unsigned int n_tiles, co
     const auto* wc = reinterpret_cast<const R*>(wcoef);
     auto* m = reinterpret_cast<typename Acc<R>::glob_t*>(mesh);
     if (g.T == 8)
-        hipLaunchKernelGGL((k_spread_own<R, P, 8>), dim3(n_tiles), dim3(threads), lds, s, g, D, start, rr, sidx, wc, m);
+        hipLaunchKernelGGL((k_spread_own<R, P, 8>), dim3(n_tiles), dim3(threads), lds, s, g, D, start, rr, sidx, wc, m, fx);
     else if (g.T == 4)
-        hipLaunchKernelGGL((k_spread_own<R, P, 4>), dim3(n_tiles), dim3(threads), lds, s, g, D, start, rr, sidx, wc, m);
+        hipLaunchKernelGGL((k_spread_own<R, P, 4>), dim3(n_tiles), dim3(threads), lds, s, g, D, start, rr, sidx, wc, m, fx);
     else
-        hipLaunchKernelGGL((k_spread_own<R, P, 2>), dim3(n_tiles), dim3(threads), lds, s, g, D, start, rr, sidx, wc, m);
+        hipLaunchKernelGGL((k_spread_own<R, P, 2>), dim3(n_tiles), dim3(threads), lds, s, g, D, start, rr, sidx, wc, m, fx);
     }
 
 //! Stencil kernel variant: 0 = wave segments per particle, 1 = thread per particle,
@@ -992,7 +1006,7 @@ size_t gather_lds_bytes(const MeshGeom& g, int P, int D, bool fp64)
 
 template<typename R>
 hipError_t spread(const MeshGeom& g, int P, int D, unsigned int n_tiles, const unsigned int* start, const void* rec,
-                  const SortedIdx* sidx, const void* wcoef, void* mesh, hipStream_t s)
+                  const SortedIdx* sidx, const void* wcoef, void* mesh, const float* fx, hipStream_t s)
     {
     const int variant = stencil_variant(0, sizeof(R) == 8);
     const bool tp = variant == 1 || variant == 3;
@@ -1002,19 +1016,19 @@ hipError_t spread(const MeshGeom& g, int P, int D, unsigned int n_tiles, const u
 #define ESP_CASE(PP)                                                                                                         \
     case PP:                                                                                                                 \
         if (variant == 2)                                                                                                    \
-            launch_spread_own<R, PP>(g, D, n_tiles, start, rec, sidx, wcoef, mesh, s);                                      \
+            launch_spread_own<R, PP>(g, D, n_tiles, start, rec, sidx, wcoef, mesh, fx, s);                                  \
         else if (variant == 3)                                                                                               \
             hipLaunchKernelGGL((k_spread_tp<R, PP, tp_pair_lanes(PP)>), dim3(n_tiles), dim3(256), lds, s, g, D, start,                      \
                                reinterpret_cast<const SortedRec<R>*>(rec), sidx, reinterpret_cast<const R*>(wcoef),          \
-                               reinterpret_cast<typename Acc<R>::glob_t*>(mesh));                                            \
+                               reinterpret_cast<typename Acc<R>::glob_t*>(mesh), fx);                                        \
         else if (tp)                                                                                                         \
             hipLaunchKernelGGL((k_spread_tp<R, PP, 1>), dim3(n_tiles), dim3(256), lds, s, g, D, start,                      \
                                reinterpret_cast<const SortedRec<R>*>(rec), sidx, reinterpret_cast<const R*>(wcoef),          \
-                               reinterpret_cast<typename Acc<R>::glob_t*>(mesh));                                            \
+                               reinterpret_cast<typename Acc<R>::glob_t*>(mesh), fx);                                        \
         else                                                                                                                 \
             hipLaunchKernelGGL((k_spread<R, PP>), dim3(n_tiles), dim3(256), lds, s, g, D, start,                            \
                                reinterpret_cast<const SortedRec<R>*>(rec), sidx, reinterpret_cast<const R*>(wcoef),          \
-                               reinterpret_cast<typename Acc<R>::glob_t*>(mesh));                                            \
+                               reinterpret_cast<typename Acc<R>::glob_t*>(mesh), fx);                                        \
         break;
         ESP_P_CASES(ESP_CASE)
 #undef ESP_CASE
@@ -1055,9 +1069,9 @@ hipError_t gather(const MeshGeom& g, int P, int D, unsigned int n_tiles, const u
     }
 
 template hipError_t spread<double>(const MeshGeom&, int, int, unsigned int, const unsigned int*, const void*, const SortedIdx*,
-                                   const void*, void*, hipStream_t);
+                                   const void*, void*, const float*, hipStream_t);
 template hipError_t spread<float>(const MeshGeom&, int, int, unsigned int, const unsigned int*, const void*, const SortedIdx*,
-                                  const void*, void*, hipStream_t);
+                                  const void*, void*, const float*, hipStream_t);
 template hipError_t gather<double>(const MeshGeom&, int, int, unsigned int, const unsigned int*, const void*, const SortedIdx*,
                                    const void*, const void*, double*, hipStream_t);
 template hipError_t gather<float>(const MeshGeom&, int, int, unsigned int, const unsigned int*, const void*, const SortedIdx*,

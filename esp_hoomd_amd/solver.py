@@ -225,6 +225,12 @@ class Solver:
                                            _lib.dptr(dri)), self._ctx)
         return scr.reshape(n, 5), dscr.reshape(n, 5), rlo, dri
 
+    def fixed_point_scale(self):
+        """(scale, lowered) of the last mixed-precision step's fixed-point mesh accumulation."""
+        sc, low = C.c_float(), C.c_int()
+        _lib.check(_lib.load().esp_get_fixed_point_scale(self._ctx, C.byref(sc), C.byref(low)), self._ctx)
+        return float(sc.value), bool(low.value)
+
     def pair_param(self, V_cut: float = 0.0) -> _lib.EspPairParam:
         """EvaluatorPairPSWF::param_type for this context's split (the table addresses stay valid
         until the parameters or table change)."""

@@ -208,6 +208,11 @@ int esp_batch_stats(esp_batch* batch, unsigned long long* captures, unsigned lon
 int esp_get_info(esp_ctx* ctx, esp_info* info);
 /* getQSum / getQ2Sum (ESPForceCompute.h:113-114); synchronises with the device */
 int esp_get_charge_sums(esp_ctx* ctx, double* q_sum, double* q2_sum);
+/* Mixed precision: the fixed-point scale of the last step's mesh accumulation (2^23 / 2^ceil(log2
+ * max|q|), lower when the most populated tile could overflow its int32 accumulators, then
+ * *lowered = 1); 0 in FP64 or before the first step.  Synchronises.  No counterpart in the
+ * reference (FP64 only). */
+int esp_get_fixed_point_scale(esp_ctx* ctx, float* scale, int* lowered);
 /* getTableSize / getTableEntry (ESPForceCompute.cc:129-141, 336-372) */
 int esp_get_table_size(esp_ctx* ctx, unsigned int* n);
 int esp_get_table_entry(esp_ctx* ctx, unsigned int i, esp_table_entry* entry);

@@ -186,3 +186,41 @@ def test_tile_edges_outside_the_kernels_are_refused():
         run_gpu(make_params(32, 8, 12.0, 3.0, precision=0, tile=12), pos, q, (8.0, 8.0, 8.0))  # FP64 column spread: 2, 4, 8
     with pytest.raises(Exception):
         run_gpu(make_params(32, 8, 12.0, 3.0, precision=1, tile=17), pos, q, (8.0, 8.0, 8.0))
+
+
+def test_mixed_fixed_point_follows_small_charges():
+    """The fixed-point mesh scale follows the largest |q| (2^23 / 2^ceil(log2 max|q|)): a system of
+    +-0.01 charges keeps the relative accuracy of the same system with +-1 charges."""
+    N, L = 800, 12.0
+    pos, q = random_electrolyte(N, L, seed=17)
+    errs = {}
+    for scale in (1.0, 0.01):
+        p64 = make_params(32, 8, 12.0, 3.5, precision=0)
+        p32 = make_params(32, 8, 12.0, 3.5, precision=1)
+        _, _, ref = run_gpu(p64, pos, q * scale, (L, L, L))
+        solver, _, res = run_gpu(p32, pos, q * scale, (L, L, L))
+        errs[scale] = rel_rms(res["force"][:, :3], ref["force"][:, :3])
+        fx, lowered = solver.fixed_point_scale()
+        print(f"charges x{scale}: mixed vs FP64 {errs[scale]:.3e}, fixed-point scale 2^{np.log2(fx):.0f}", flush=True)
+        assert fx == (2.0 ** 23 if scale == 1.0 else 2.0 ** 29) and not lowered
+    assert errs[0.01] < 1e-6
+    assert errs[0.01] < 3 * errs[1.0]
+
+
+def test_mixed_fixed_point_guards_dense_high_charge_tiles():
+    """50,000 particles in a 10^3 box with 16-cell tiles put ~6,000 charges in a tile, whose int32
+    LDS cells the 2^23 scale would overflow: the scale is lowered and the result stays close to
+    FP64."""
+    N, L = 50_000, 10.0
+    rng = np.random.default_rng(5)
+    pos = rng.uniform(-L / 2, L / 2, (N, 3))
+    q = np.where(np.arange(N) % 2 == 0, 2.0, -2.0)
+    p64 = make_params(32, 4, 8.0, 1.0, precision=0)
+    p32 = make_params(32, 4, 8.0, 1.0, precision=1, tile=16)
+    _, _, ref = run_gpu(p64, pos, q, (L, L, L), skin=0.1)
+    solver, _, res = run_gpu(p32, pos, q, (L, L, L), skin=0.1)
+    fx, lowered = solver.fixed_point_scale()
+    err = rel_rms(res["force"][:, :3], ref["force"][:, :3])
+    print(f"dense tiles: scale 2^{np.log2(fx):.0f} lowered={lowered}, mixed vs FP64 {err:.3e}", flush=True)
+    assert lowered and fx < 2.0 ** 22
+    assert err < 1e-4
