@@ -412,6 +412,8 @@ def main(argv=None):
     ap.add_argument("--md-steps", type=int, default=300, help="timed NVE steps of the md leg")
     ap.add_argument("--batch-systems", type=int, default=512)
     ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--serve-rate", type=float, default=0.0,
+                    help="offered requests/s of the served leg (default: 70 %% of the batched capacity)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="MIB",
                     help="only run the record-copy kernel over MIB MiB, --steps times (counter calibration)")
@@ -492,7 +494,8 @@ def main(argv=None):
         # offered rate ~70 % of the batched capacity for the mean request (E[N] E[steps])
         cap_ps = out_legs.get("batched", {}).get("particle_steps_per_s", 3e8)
         mean_work = (2e5 - 1e3) / math.log(200) * (1000 - 10) / math.log(100)
-        out_legs["served"] = served_leg(args.serve_seconds, rate=0.7 * cap_ps / mean_work)
+        rate = args.serve_rate if args.serve_rate > 0 else 0.7 * cap_ps / mean_work
+        out_legs["served"] = served_leg(args.serve_seconds, rate=rate)
 
     if head is not None:
         ch, r = tuned[head]

@@ -29,7 +29,15 @@ struct esp_nlist
     unsigned long long builds = 0;
     esp::DeviceBuffer n_neigh, nlist, head, cell, slot, count, start, members, overflow, last_pos, disp;
     esp::DeviceBuffer n_inner, inner, head_inner;
+    unsigned int* h_over = nullptr;  // pinned: overflow counts of the last build (full, inner)
+    esp::md::CellGeom cg {};
+    bool want_inner = false;
     std::string last_error;
+    ~esp_nlist()
+        {
+        if (h_over)
+            (void)hipHostFree(h_over);
+        }
     };
 
 namespace
@@ -61,7 +69,8 @@ template<class Fn> int guard(esp_nlist* nl, Fn fn)
         }
     }
 
-void do_build(esp_nlist* nl, const esp_system* sys, hipStream_t s)
+//! Size the list's buffers for this system and fix the cell geometry (host work only).
+void build_prepare(esp_nlist* nl, const esp_system* sys)
     {
     using esp::check_hip;
     const unsigned int N = sys->N;
@@ -71,7 +80,7 @@ void do_build(esp_nlist* nl, const esp_system* sys, hipStream_t s)
     const double Lmin = std::min({sys->box[0], sys->box[1], sys->box[2]});
     if (!(r_list > 0.0) || r_list >= 0.5 * Lmin)
         throw esp::EspError(ESP_ERR_ARG, "r_cut + r_buff must be positive and below half the box length");
-    esp::md::CellGeom cg;
+    esp::md::CellGeom& cg = nl->cg;
     cg.Lx = sys->box[0];
     cg.Ly = sys->box[1];
     cg.Lz = sys->box[2];
@@ -101,57 +110,85 @@ void do_build(esp_nlist* nl, const esp_system* sys, hipStream_t s)
         nl->N = N;
         nl->n_cells = nc;
         }
-    const bool want_inner = nl->r_inner > 0.0 && nl->r_inner < nl->r_cut;
-    if (want_inner)
+    nl->want_inner = nl->r_inner > 0.0 && nl->r_inner < nl->r_cut;
+    if (nl->want_inner)
         {
         nl->n_inner.resize(size_t(std::max(N, 1u)) * 4);
         nl->head_inner.resize(size_t(std::max(N, 1u)) * 8);
         }
+    if (!nl->h_over)
+        check_hip(hipHostMalloc(reinterpret_cast<void**>(&nl->h_over), 2 * sizeof(unsigned int), hipHostMallocDefault), "hipHostMalloc");
     nl->inner_built = false;
+    }
+
+//! Enqueue one build attempt at the current capacities; its overflow counts land in h_over
+//! when the stream reaches them.
+void build_enqueue(esp_nlist* nl, const esp_system* sys, hipStream_t s)
+    {
+    using esp::check_hip;
+    const unsigned int N = sys->N;
+    const unsigned int nc = nl->n_cells;
+    const double r_list = nl->r_cut + nl->r_buff;
+    nl->nlist.resize(size_t(std::max(N, 1u)) * nl->nmax * 4);
+    esp::md::InnerList in {};
+    if (nl->want_inner)
+        {
+        nl->inner.resize(size_t(std::max(N, 1u)) * nl->nmax_in * 4);
+        in.r2 = (nl->r_inner + nl->r_buff) * (nl->r_inner + nl->r_buff);
+        in.nmax = nl->nmax_in;
+        in.n = nl->n_inner.as<unsigned int>();
+        in.list = nl->inner.as<unsigned int>();
+        in.head = nl->head_inner.as<uint64_t>();
+        }
+    check_hip(hipMemsetAsync(nl->count.get(), 0, size_t(nc + 1) * 4, s), "hipMemsetAsync");
+    check_hip(hipMemsetAsync(nl->overflow.get(), 0, 8, s), "hipMemsetAsync");
+    check_hip(esp::md::cell_count(N, sys->postype, nl->cg, nl->count.as<unsigned int>(), nl->cell.as<unsigned int>(),
+                                  nl->slot.as<unsigned int>(), s),
+              "cell_count");
+    check_hip(esp::kernel::scan_tiles(nc, nl->count.as<unsigned int>(), nl->start.as<unsigned int>(), 0, nullptr, 0.0f, nullptr, s), "scan");
+    check_hip(esp::md::cell_fill(N, nl->cell.as<unsigned int>(), nl->slot.as<unsigned int>(), nl->start.as<unsigned int>(),
+                                 nl->members.as<unsigned int>(), s),
+              "cell_fill");
+    check_hip(esp::md::build_nlist(N, sys->postype, nl->cg, nl->start.as<unsigned int>(), nl->members.as<unsigned int>(), r_list,
+                                   sys->n_ex, sys->ex_list, sys->ex_pitch, nl->nmax, nl->n_neigh.as<unsigned int>(),
+                                   nl->nlist.as<unsigned int>(), nl->head.as<uint64_t>(), nl->overflow.as<unsigned int>(), in, s),
+              "build_nlist");
+    check_hip(hipMemcpyAsync(nl->h_over, nl->overflow.get(), 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+    }
+
+//! After the stream has passed build_enqueue: accept the list (true) or raise the capacities
+//! the attempt overflowed (false: enqueue again).
+bool build_finish(esp_nlist* nl, const esp_system* sys, hipStream_t s)
+    {
+    const unsigned int over0 = nl->h_over[0], over1 = nl->h_over[1];
+    if (over0 == 0 && over1 == 0)
+        {
+        nl->inner_built = nl->want_inner;
+        esp::check_hip(esp::md::copy4(sys->N, sys->postype, nl->last_pos.as<double>(), s), "copy4");
+        nl->box[0] = sys->box[0];
+        nl->box[1] = sys->box[1];
+        nl->box[2] = sys->box[2];
+        nl->built = true;
+        nl->builds += 1;
+        return true;
+        }
+    if (over0)
+        nl->nmax = ((over0 + over0 / 4) + 7) & ~7u;
+    if (over1)
+        nl->nmax_in = ((over1 + over1 / 4) + 7) & ~7u;
+    return false;
+    }
+
+//! Build now: enqueue, wait, and retry with larger capacities while the list overflows.
+void do_build(esp_nlist* nl, const esp_system* sys, hipStream_t s)
+    {
+    build_prepare(nl, sys);
     for (int attempt = 0; attempt < 4; ++attempt)
         {
-        nl->nlist.resize(size_t(std::max(N, 1u)) * nl->nmax * 4);
-        esp::md::InnerList in {};
-        if (want_inner)
-            {
-            nl->inner.resize(size_t(std::max(N, 1u)) * nl->nmax_in * 4);
-            in.r2 = (nl->r_inner + nl->r_buff) * (nl->r_inner + nl->r_buff);
-            in.nmax = nl->nmax_in;
-            in.n = nl->n_inner.as<unsigned int>();
-            in.list = nl->inner.as<unsigned int>();
-            in.head = nl->head_inner.as<uint64_t>();
-            }
-        check_hip(hipMemsetAsync(nl->count.get(), 0, size_t(nc + 1) * 4, s), "hipMemsetAsync");
-        check_hip(hipMemsetAsync(nl->overflow.get(), 0, 8, s), "hipMemsetAsync");
-        check_hip(esp::md::cell_count(N, sys->postype, cg, nl->count.as<unsigned int>(), nl->cell.as<unsigned int>(),
-                                      nl->slot.as<unsigned int>(), s),
-                  "cell_count");
-        check_hip(esp::kernel::scan_tiles(nc, nl->count.as<unsigned int>(), nl->start.as<unsigned int>(), 0, nullptr, 0.0f, nullptr, s), "scan");
-        check_hip(esp::md::cell_fill(N, nl->cell.as<unsigned int>(), nl->slot.as<unsigned int>(), nl->start.as<unsigned int>(),
-                                     nl->members.as<unsigned int>(), s),
-                  "cell_fill");
-        check_hip(esp::md::build_nlist(N, sys->postype, cg, nl->start.as<unsigned int>(), nl->members.as<unsigned int>(), r_list,
-                                       sys->n_ex, sys->ex_list, sys->ex_pitch, nl->nmax, nl->n_neigh.as<unsigned int>(),
-                                       nl->nlist.as<unsigned int>(), nl->head.as<uint64_t>(), nl->overflow.as<unsigned int>(), in, s),
-                  "build_nlist");
-        unsigned int over[2] = {0, 0};
-        check_hip(hipMemcpyAsync(over, nl->overflow.get(), 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
-        check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
-        if (over[0] == 0 && over[1] == 0)
-            {
-            nl->inner_built = want_inner;
-            check_hip(esp::md::copy4(N, sys->postype, nl->last_pos.as<double>(), s), "copy4");
-            nl->box[0] = sys->box[0];
-            nl->box[1] = sys->box[1];
-            nl->box[2] = sys->box[2];
-            nl->built = true;
-            nl->builds += 1;
+        build_enqueue(nl, sys, s);
+        esp::check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
+        if (build_finish(nl, sys, s))
             return;
-            }
-        if (over[0])
-            nl->nmax = ((over[0] + over[0] / 4) + 7) & ~7u;
-        if (over[1])
-            nl->nmax_in = ((over[1] + over[1] / 4) + 7) & ~7u;
         }
     throw esp::EspError(ESP_ERR_STATE, "neighbour list did not converge");
     }
@@ -387,7 +424,9 @@ int esp_md_batch_update_nlists(esp_md_batch* b, unsigned int* rebuilt, unsigned 
                                                  "hipMemcpyAsync");
                                   esp::check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
                                   }
-                              bool changed = false;
+                              // the lists to rebuild are enqueued together and waited for once; a
+                              // list that overflowed its capacity is rebuilt on its own afterwards
+                              std::vector<unsigned int> todo;
                               for (unsigned int i = 0; i < n; ++i)
                                   {
                                   esp_nlist* nl = b->nls[i];
@@ -401,14 +440,28 @@ int esp_md_batch_update_nlists(esp_md_batch* b, unsigned int* rebuilt, unsigned 
                                              || sy.box[2] != nl->box[2] || d2 >= 0.25 * nl->r_buff * nl->r_buff;
                                       if (need)
                                           {
-                                          do_build(nl, &sy, s);
-                                          md_batch_refresh(b, i);
-                                          changed = true;
-                                          ++count;
+                                          build_prepare(nl, &sy);
+                                          build_enqueue(nl, &sy, s);
+                                          todo.push_back(i);
                                           }
                                       }
                                   if (which)
                                       which[i] = need ? 1 : 0;
+                                  }
+                              bool changed = false;
+                              if (!todo.empty())
+                                  {
+                                  esp::check_hip(hipStreamSynchronize(s), "hipStreamSynchronize");
+                                  for (unsigned int i : todo)
+                                      {
+                                      esp_nlist* nl = b->nls[i];
+                                      const esp_system& sy = b->sys[i];
+                                      if (!build_finish(nl, &sy, s))
+                                          do_build(nl, &sy, s);
+                                      md_batch_refresh(b, i);
+                                      changed = true;
+                                      ++count;
+                                      }
                                   }
                               if (changed)
                                   md_batch_upload(b);

@@ -8,24 +8,63 @@ import numpy as np
 from . import _lib
 
 
-class DeviceArray:
-    """A device allocation with a numpy dtype and shape."""
+class _BlockCache:
+    """Freed device blocks of pooled arrays kept for reuse, by size class (powers of two from 256
+    bytes), up to a byte limit: hipMalloc / hipFree of per-request arrays would otherwise sit on
+    the serving loop's critical path (hipFree waits for the device).  A pooled array must only be
+    dropped when no queued GPU work still uses it (hipFree's implicit wait is skipped)."""
 
-    def __init__(self, shape, dtype=np.float64, zero: bool = False):
+    LIMIT = 8 << 30
+
+    def __init__(self):
+        self.free_blocks = {}
+        self.held = 0
+
+    @staticmethod
+    def size_class(nbytes: int) -> int:
+        return max(256, 1 << (max(int(nbytes), 1) - 1).bit_length())
+
+    def take(self, nbytes: int):
+        blocks = self.free_blocks.get(self.size_class(nbytes))
+        if blocks:
+            p = blocks.pop()
+            self.held -= self.size_class(nbytes)
+            return p
+        return None
+
+    def give(self, ptr: C.c_void_p, cls: int) -> bool:
+        if self.held + cls > self.LIMIT:
+            return False
+        self.free_blocks.setdefault(cls, []).append(ptr)
+        self.held += cls
+        return True
+
+
+_cache = _BlockCache()
+
+
+class DeviceArray:
+    """A device allocation with a numpy dtype and shape.  `pooled` arrays take and return their
+    blocks through a bounded size-class cache (see _BlockCache)."""
+
+    def __init__(self, shape, dtype=np.float64, zero: bool = False, pooled: bool = False):
         lib = _lib.load()
         self.shape = tuple(int(s) for s in (shape if isinstance(shape, (tuple, list)) else (shape,)))
         self.dtype = np.dtype(dtype)
         self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
-        p = C.c_void_p()
-        _lib.check(lib.esp_device_malloc(C.byref(p), max(self.nbytes, 8)))
+        self._cls = _BlockCache.size_class(max(self.nbytes, 8)) if pooled else 0
+        p = _cache.take(self._cls) if pooled else None
+        if p is None:
+            p = C.c_void_p()
+            _lib.check(lib.esp_device_malloc(C.byref(p), self._cls if pooled else max(self.nbytes, 8)))
         self._ptr = p
         if zero:
             _lib.check(lib.esp_memset(self._ptr, 0, max(self.nbytes, 8)))
 
     @classmethod
-    def from_numpy(cls, a: np.ndarray) -> "DeviceArray":
+    def from_numpy(cls, a: np.ndarray, pooled: bool = False) -> "DeviceArray":
         a = np.ascontiguousarray(a)
-        d = cls(a.shape, a.dtype)
+        d = cls(a.shape, a.dtype, pooled=pooled)
         d.upload(a)
         return d
 
@@ -52,7 +91,8 @@ class DeviceArray:
 
     def free(self) -> None:
         if getattr(self, "_ptr", None) is not None and self._ptr.value:
-            _lib.load().esp_device_free(self._ptr)
+            if not (self._cls and _cache.give(self._ptr, self._cls)):
+                _lib.load().esp_device_free(self._ptr)
         self._ptr = C.c_void_p()
 
     def __del__(self):

@@ -19,7 +19,8 @@ class State:
     """Particle data in the HOOMD layout: postype (x, y, z, type), charge, velocity (vx, vy,
     vz, mass), net force (fx, fy, fz, energy) and per-particle virial (6 x pitch)."""
 
-    def __init__(self, positions, charges, box, velocities=None, masses=1.0, types=None, exclusions=None):
+    def __init__(self, positions, charges, box, velocities=None, masses=1.0, types=None, exclusions=None,
+                 zero_outputs: bool = True, pooled: bool = False):
         pos = np.asarray(positions, dtype=np.float64)
         self.N = len(pos)
         box = list(box)
@@ -28,17 +29,20 @@ class State:
         pt[:, :3] = pos
         if types is not None:
             pt[:, 3] = types
-        self.postype = DeviceArray.from_numpy(pt)
-        self.charge = DeviceArray.from_numpy(np.asarray(charges, dtype=np.float64))
+        # pooled: the arrays come from and return to the device block cache (device.DeviceArray)
+        self.postype = DeviceArray.from_numpy(pt, pooled=pooled)
+        self.charge = DeviceArray.from_numpy(np.asarray(charges, dtype=np.float64), pooled=pooled)
         v = np.zeros((self.N, 4))
         if velocities is not None:
             v[:, :3] = velocities
         v[:, 3] = masses
-        self.vel = DeviceArray.from_numpy(v)
-        self.force = DeviceArray((self.N, 4), np.float64, zero=True)
+        self.vel = DeviceArray.from_numpy(v, pooled=pooled)
+        # zero_outputs=False leaves force and virial unset (for callers whose first force
+        # evaluation overwrites them)
+        self.force = DeviceArray((self.N, 4), np.float64, zero=zero_outputs, pooled=pooled)
         self.virial_pitch = max(self.N, 1)
-        self.virial = DeviceArray((6, self.virial_pitch), np.float64, zero=True)
-        self.ext = DeviceArray((8,), np.float64, zero=True)
+        self.virial = DeviceArray((6, self.virial_pitch), np.float64, zero=zero_outputs, pooled=pooled)
+        self.ext = DeviceArray((8,), np.float64, zero=True, pooled=pooled)
         self.exclusions = list(exclusions) if exclusions else None
         self.n_ex = self.ex_list = None
         self.ex_pitch = 0
@@ -92,11 +96,9 @@ class State:
 
     def thermalize(self, kT: float, seed: int = 1) -> None:
         """Maxwell-Boltzmann velocities at kT with zero net momentum."""
-        m = self.vel.numpy()[:, 3]
-        rng = np.random.default_rng(seed)
-        v = rng.standard_normal((self.N, 3)) * np.sqrt(kT / m)[:, None]
-        v -= (m[:, None] * v).sum(0) / m.sum()
-        self.set_velocities(v)
+        a = self.vel.numpy()
+        a[:, :3] = maxwell_boltzmann(a[:, 3], kT, seed)
+        self.vel.upload(a)
 
 
 class NeighborList:
@@ -325,6 +327,15 @@ class Simulation:
         return 2.0 * ke / (3.0 * (self.state.N - 1)) if self.state.N > 1 else 0.0
 
 
+def maxwell_boltzmann(masses, kT: float, seed: int = 1) -> np.ndarray:
+    """(N, 3) Maxwell-Boltzmann velocities at kT (seeded) with zero net momentum."""
+    m = np.broadcast_to(np.asarray(masses, np.float64), (len(np.atleast_1d(masses)),))
+    rng = np.random.default_rng(seed)
+    v = rng.standard_normal((len(m), 3)) * np.sqrt(kT / m)[:, None]
+    v -= (m[:, None] * v).sum(0) / m.sum()
+    return v
+
+
 def jittered_lattice(N: int, rho: float, seed: int = 2, jitter: float = 0.1):
     """Seeded jittered simple-cubic lattice of N = m^3 sites at density rho (box centred on 0)."""
     m = round(N ** (1.0 / 3.0))
@@ -414,4 +425,5 @@ class BatchMD:
             pass
 
 
-__all__ = ["State", "NeighborList", "WCA", "ESPForce", "NVE", "Langevin", "Simulation", "BatchMD", "jittered_lattice", "random_packing"]
+__all__ = ["State", "NeighborList", "WCA", "ESPForce", "NVE", "Langevin", "Simulation", "BatchMD", "jittered_lattice",
+           "random_packing", "maxwell_boltzmann"]

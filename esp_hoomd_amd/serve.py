@@ -40,14 +40,18 @@ import numpy as np
 
 from . import _lib
 from .batch import Batch
-from .device import synchronize
-from .md import BatchMD, NeighborList, State, WCA
+from .device import Stream, synchronize
+from .md import BatchMD, NeighborList, State, WCA, maxwell_boltzmann
 from .solver import Params, Solver
 from . import tune as _tune
 from .workloads import lattice_positions, neutral_charges
 
 WCA_SIGMA = 1.0
 SKIN = 0.4
+#: margin of the per-request tuning below the force-error target: a result's error varies with
+#: its structure along the trajectory around the calibrated model (one in ~850 served results
+#: exceeded 1e-5 at a margin of 1.25)
+SAFETY = 1.4
 
 
 @dataclass
@@ -117,6 +121,11 @@ class Server:
         self.batch = Batch(use_graph=use_graph)       # every resident request, every tick
         self.batch_new = Batch(use_graph=False)       # initial forces of newly admitted requests
         self.md = BatchMD()                           # integration, WCA and list upkeep of the resident set
+        # the resident set's steps run on their own (non-blocking) stream; admissions (uploads,
+        # first neighbour list, first forces) run on the null stream meanwhile, so the host
+        # prepares new requests while the GPU advances the others
+        self.stream = Stream()
+        self._inflight = None  # (requests, launch time, particles) of the step on the stream
         self._member_ids: List[int] = []
         self._esp_args = None
         self.step_latency: List[float] = []
@@ -130,6 +139,11 @@ class Server:
         self.tick_active: List[int] = []
         self.tick_particles: List[int] = []
         self.waited_for_memory = 0
+        # wall seconds per phase of the loop (admission, first forces of new requests, member
+        # bookkeeping, list upkeep, ESP launch, the rest of the step up to its synchronisation)
+        self.admit_s: Dict[str, float] = {k: 0.0 for k in ("state", "nlist", "solver", "first_forces")}
+        self.phase_s: Dict[str, float] = {k: 0.0 for k in ("admit", "wait", "retire", "members", "nlists", "esp",
+                                                          "step_rest")}
         self._tune_cache: Dict[Tuple[int, int], Params] = {}
         self._lib = _lib.load()
 
@@ -196,7 +210,7 @@ class Server:
         key = (int(round(math.log(N) * 8)), int(round(math.log(density) * 16)), scale)
         if key not in self._tune_cache:
             ch = _tune.tune(N, (L, L, L), self.target, self.precision, orders=self.tuner_orders,
-                            mesh_range=self.mesh_range, batched=True, method_scale=scale)[0]
+                            mesh_range=self.mesh_range, batched=True, method_scale=scale, safety=SAFETY)[0]
             self._tune_cache[key] = (ch.params, ch.c, ch.sigma)
         p, c, sigma = self._tune_cache[key]
         # same c and sigma, mesh spacing and cutoff rescaled to this box
@@ -205,13 +219,14 @@ class Server:
         rc = sigma * c * h / math.pi
         if rc + SKIN > 0.5 * L:
             ch = _tune.tune(N, (L, L, L), self.target, self.precision, orders=self.tuner_orders, mesh_range=self.mesh_range,
-                            batched=True, method_scale=scale)[0]
+                            batched=True, method_scale=scale, safety=SAFETY)[0]
             return ch.params, ch.neighbours
         params = Params(nx=n, ny=n, nz=n, order=p.order, kappa=c / rc, rcut=rc, precision=self.precision, tile=p.tile)
         return params, density * 4.0 / 3.0 * math.pi * (rc + SKIN) ** 3
 
     def _admit(self, now: float) -> bool:
         changed = False
+        new: List[Request] = []
         while self.pending and self.pending[0][0] <= now and len(self.active) < self.max_active:
             req = self.pending[0][2]
             if req.prepared is None:
@@ -224,18 +239,37 @@ class Server:
                 break
             heapq.heappop(self.pending)
             req.params, req.footprint, req.prepared = params, need, None
-            st = State(pos, q, [L, L, L])
-            st.thermalize(req.kT, seed=req.seed + 1)
+            ta = time.perf_counter()
+            # velocities drawn on the host; force and virial are overwritten by the first forces
+            st = State(pos, q, [L, L, L], velocities=maxwell_boltzmann(np.ones(len(pos)), req.kT, seed=req.seed + 1),
+                       zero_outputs=False, pooled=True)
             req.state = st
+            tb = time.perf_counter()
             req.nlist = NeighborList(buffer=SKIN, inner=self.wca.r_cut)
             req.nlist.update(st, max(params.rcut, self.wca.r_cut), force_rebuild=True)
+            tc = time.perf_counter()
             req.solver = Solver(params)
+            self.admit_s["state"] += tb - ta
+            self.admit_s["nlist"] += tc - tb
+            self.admit_s["solver"] += time.perf_counter() - tc
             req.admitted_at = now
             req.admitted_wall = time.perf_counter()
             self.active.append(req)
             self.resident_bytes += need
             self.peak_bytes = max(self.peak_bytes, self.resident_bytes)
+            new.append(req)
             changed = True
+        if new:
+            # forces at the initial positions (the first half kick needs them), on the null
+            # stream beside the resident set's step; a separate batch, so the resident set's
+            # captured graph stays valid
+            tf = time.perf_counter()
+            self._forces(new, self.batch_new)
+            if self.keep_results:
+                for r in new:
+                    r.first_forces = r.state.force.numpy().copy()
+            _lib.check(self._lib.esp_stream_synchronize(None))
+            self.admit_s["first_forces"] += time.perf_counter() - tf
         return changed
 
     def _forces(self, reqs: List[Request], batch: Batch) -> None:
@@ -269,54 +303,78 @@ class Server:
         self.md.set([(r.state, r.nlist) for r in self.active], seeds=[r.seed for r in self.active])
         self._esp_args = self.batch.prepare([(r.solver, r.state) for r in self.active])
 
-    def tick(self, now: float) -> None:
-        """One MD step of every resident request: batched first half kick + drift, one
-        displacement check (rebuilding only the lists that need it), batched ESP forces
-        (captured graph), batched WCA, batched second half kick."""
-        t0 = time.perf_counter()
-        new = [r for r in self.active if r.steps_done == 0]
-        if new:
-            # forces at the initial positions of newly admitted requests (the first half kick
-            # needs them); a separate batch, so the resident set's captured graph stays valid
-            self._forces(new, self.batch_new)
-            if self.keep_results:
-                synchronize()
-                for r in new:
-                    r.first_forces = r.state.force.numpy().copy()
-        self._sync_members()
-        self.md.integrate_first(self.dt)
-        for i in self.md.update_nlists():
-            Batch.refresh(self._esp_args, i)
-        self.batch.enqueue_prepared(self._esp_args)
-        self.md.wca(WCA_SIGMA, 1.0)
-        self.md.integrate_second(self.dt, step=len(self.tick_ms) + 1)
-        synchronize()
+    def _complete(self) -> None:
+        """Wait for the step in flight and account it: each request's step latency runs from its
+        previous step's completion (or its admission) to now."""
+        if self._inflight is None:
+            return
+        reqs, t0, particles = self._inflight
+        tw = time.perf_counter()
+        self.stream.synchronize()
         end = time.perf_counter()
-        for r in self.active:
+        self.phase_s["wait"] += end - tw
+        for r in reqs:
             r.steps_done += 1
-            # time since this request's previous step finished (or since it was admitted)
             self.step_latency.append((end - (r.last_step_end if r.last_step_end else r.admitted_wall)) * 1e3)
             r.last_step_end = end
         self.tick_ms.append((end - t0) * 1e3)
-        self.tick_active.append(len(self.active))
-        self.tick_particles.append(sum(r.state.N for r in self.active))
+        self.tick_active.append(len(reqs))
+        self.tick_particles.append(particles)
+        self._inflight = None
+
+    def _launch(self) -> None:
+        """Enqueue one MD step of every resident request on the server's stream: batched first
+        half kick + drift, one displacement check (rebuilding only the lists that need it),
+        batched ESP forces (captured graph), batched WCA, batched second half kick.  Returns
+        once the step is enqueued; _complete waits for it."""
+        ph = self.phase_s
+        t0 = time.perf_counter()
+        self._sync_members()
+        t1 = time.perf_counter()
+        st = self.stream
+        self.md.integrate_first(self.dt, stream=st)
+        for i in self.md.update_nlists(stream=st):
+            Batch.refresh(self._esp_args, i)
+        t2 = time.perf_counter()
+        self.batch.enqueue_prepared(self._esp_args, stream=st)
+        t3 = time.perf_counter()
+        self.md.wca(WCA_SIGMA, 1.0, stream=st)
+        self.md.integrate_second(self.dt, step=len(self.tick_ms) + 1, stream=st)
+        t4 = time.perf_counter()
+        ph["members"] += t1 - t0
+        ph["nlists"] += t2 - t1
+        ph["esp"] += t3 - t2
+        ph["step_rest"] += t4 - t3
+        self._inflight = (list(self.active), t0, sum(r.state.N for r in self.active))
+
+    def tick(self, now: float) -> None:
+        """One MD step of every resident request, waited for."""
+        self._launch()
+        self._complete()
 
     def run(self, time_limit: float = 1e9) -> dict:
         """Serve until every submitted request is done (or time_limit seconds pass)."""
         start = time.perf_counter()
-        while self.pending or self.active:
+        while self.pending or self.active or self._inflight is not None:
             now = time.perf_counter() - start
             if now > time_limit:
                 break
+            # admissions overlap the step in flight
+            ta = time.perf_counter()
             self._admit(now)
+            self.phase_s["admit"] += time.perf_counter() - ta
+            self._complete()
+            tr = time.perf_counter()
+            self._retire(tr - start)
+            self.phase_s["retire"] += time.perf_counter() - tr
             if not self.active:
                 # idle until the next arrival
-                wait = self.pending[0][0] - now if self.pending else 0.0
+                wait = self.pending[0][0] - (time.perf_counter() - start) if self.pending else 0.0
                 if wait > 0:
                     time.sleep(min(wait, 0.01))
                 continue
-            self.tick(now)
-            self._retire(time.perf_counter() - start)
+            self._launch()
+        self._complete()
         wall = time.perf_counter() - start
         return self.stats(wall)
 
@@ -343,6 +401,8 @@ class Server:
             "peak_resident_bytes": self.peak_bytes,
             "admissions_deferred_for_memory": self.waited_for_memory,
             "batch": self.batch.stats(),
+            "phase_s": {k: round(v, 3) for k, v in self.phase_s.items()},
+            "admit_s": {k: round(v, 3) for k, v in self.admit_s.items()},
         }
 
 
