@@ -186,6 +186,9 @@ template<typename R> hipError_t batch_pack(const BatchSys* d, BatchList l, hipSt
 size_t tp_lds_bytes(const MeshGeom& g, int P, bool fp64);
 bool batch_order_supported(int P);
 bool stencil_tile_ok(int T, int P, bool fp64, bool batched);
+//! Tile edge the matrix-core FP32 spread needs at order P (17 - P) when that variant is
+//! selected, else 0.
+int mfma_spread_tile(int P, bool fp64);
 
     } // namespace kernel
     } // namespace esp

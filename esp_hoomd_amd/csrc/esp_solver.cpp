@@ -453,6 +453,8 @@ void EspSolver::initialize(const SystemView& sys, hipStream_t s)
         if (!kernel::stencil_tile_ok(T, int(P), fp64, false))
             throw EspError(STATUS_UNSUPPORTED, "the requested tile does not fit the stencil kernels' LDS at this order and precision");
         }
+    else if (kernel::mfma_spread_tile(int(P), fp64))
+        T = kernel::mfma_spread_tile(int(P), fp64);
     else
         {
         for (int t : {8, 4, 2})

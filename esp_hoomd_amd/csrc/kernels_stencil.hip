@@ -676,6 +676,110 @@ __global__ void __launch_bounds__(256) k_gather_tp(MeshGeom g,
     }
 
 // ---------------------------------------------------------------------------
+// Matrix-core spreading (FP32): the tile's box as a GEMM.  With a 16-cell box edge (tile edge
+// T = 17 - P), box[z][y][x] = sum_p wx_p[x] (q_p wy_p[y] wz_p[z]): per z-plane a 16 x 16 block
+// C = Wx^T (16 x K) * Y_z (K x 16) over the tile's K particles, one v_mfma_f32_16x16x4_f32 per
+// four particles and plane.  Each of the four waves owns four z-planes (four 16 x 16 fp32
+// accumulators); the particles' dense, box-aligned weight vectors (zero outside the stencil)
+// are staged in LDS in chunks.  Lane layout of the instruction (checked on gfx950): A[l % 16][l
+// / 16], B[l / 16][l % 16], D[4 (l / 16) + r][l % 16], r = 0..3.  The sums are FP32 in the
+// order of the tile's particles; the box enters the fixed-point mesh like the LDS-atomic
+// variants' boxes.
+// ---------------------------------------------------------------------------
+typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+
+template<int P>
+__global__ void __launch_bounds__(256) k_spread_mfma(MeshGeom g,
+                                                     int D,
+                                                     const unsigned int* __restrict__ tile_start,
+                                                     const SortedRec<float>* __restrict__ rec,
+                                                     const SortedIdx* __restrict__ sidx,
+                                                     const float* __restrict__ wcoef,
+                                                     unsigned long long* __restrict__ mesh,
+                                                     const float* __restrict__ fx)
+    {
+    constexpr int BX = 16;    // box edge
+    constexpr int CH = 64;    // particles staged per chunk
+    static_assert(P >= 4 && P <= 12, "tile edge 17 - P");
+    const unsigned int tile = blockIdx.x;
+    const unsigned int s0 = tile_start[tile], s1 = tile_start[tile + 1];
+    if (s0 == s1)
+        return;
+    __shared__ float wxd[CH][BX], wyd[CH][BX], qwzd[CH][BX];
+    __shared__ float wc[P * (MAX_WINDOW_DEG + 1)];
+    for (int c = threadIdx.x; c < P * (D + 1); c += blockDim.x)
+        wc[c] = wcoef[c];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int kk = lane >> 4, col = lane & 15;
+    mfma_f4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+        acc[t] = mfma_f4 {0.f, 0.f, 0.f, 0.f};
+
+    for (unsigned int c0 = s0; c0 < s1; c0 += CH)
+        {
+        const int n = int(min(unsigned(CH), s1 - c0));
+        for (int e = threadIdx.x; e < CH * BX; e += blockDim.x)
+            {
+            (&wxd[0][0])[e] = 0.f;
+            (&wyd[0][0])[e] = 0.f;
+            (&qwzd[0][0])[e] = 0.f;
+            }
+        __syncthreads();
+        if (int(threadIdx.x) < n)
+            {
+            const int p = int(threadIdx.x);
+            const SortedRec<float> r = rec[c0 + p];
+            const unsigned int loc = sidx[c0 + p].local;
+            const int ox = loc & 0xff, oy = (loc >> 8) & 0xff, oz = loc >> 16;
+            float wx[P], wy[P], wz[P], unused[P];
+            eval_taps<float, P, false>(wc, D, r.tx, wx, unused);
+            eval_taps<float, P, false>(wc, D, r.ty, wy, unused);
+            eval_taps<float, P, false>(wc, D, r.tz, wz, unused);
+#pragma unroll
+            for (int a = 0; a < P; ++a)
+                {
+                wxd[p][ox + a] = wx[a];
+                wyd[p][oy + a] = wy[a];
+                qwzd[p][oz + a] = r.q * wz[a];
+                }
+            }
+        __syncthreads();
+        for (int k0 = 0; k0 < n; k0 += 4)
+            {
+            const int p = k0 + kk;  // < CH: n <= CH and CH is a multiple of 4 (zero rows beyond n)
+            const float a = wxd[p][col];
+            const float wyv = wyd[p][col];
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, wyv * qwzd[p][4 * wave + t], acc[t], 0, 0, 0);
+            }
+        __syncthreads();
+        }
+
+    int tx, ty, tz;
+    tile_coords(g, tile, tx, ty, tz);
+    const double scale = double(fx[0]);
+    const int gy = (ty * g.T + col) % g.ny;
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+        {
+        const int gz = (tz * g.T + 4 * wave + t) % g.nz;
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+            {
+            const float v = acc[t][r];
+            if (v != 0.f)
+                {
+                const int gx = (tx * g.T + 4 * kk + r) % g.nx;
+                atomicAdd(&mesh[(size_t(gz) * g.ny + gy) * g.nx + gx],
+                          static_cast<unsigned long long>(__double2ll_rn(double(v) * scale)));
+                }
+            }
+        }
+    }
+
+// ---------------------------------------------------------------------------
 // Column-ownership spreading: lane (cx, cy) owns the z column of the tile's
 // (T+P-1)^3 box in registers.  Particles are broadcast through LDS: x and y
 // weights, and the z weights already shifted to box coordinates (zero outside
@@ -928,7 +1032,8 @@ static void launch_spread_own(const MeshGeom& g, int D, unsigned int n_tiles, co
 
 //! Stencil kernel variant: 0 = wave segments per particle, 1 = thread per particle,
 //! 2 = column ownership (spread only), 3 = thread per particle with two lanes per particle
-//! (even orders; half the rows each).
+//! (even orders; half the rows each), 4 = matrix cores (FP32 spread with a 16-cell box, tile
+//! edge 17 - P; other precisions and the gather use their defaults).
 //! Defaults (measured on MI355X at N = 1e6, P = 10): FP64 spreads by column ownership and
 //! gathers with two lanes per particle; FP32 spreads with two lanes (fixed-point sums: the same
 //! result as one lane) and gathers with one, as the batched kernels do (their 12^3 tiles
@@ -943,9 +1048,14 @@ static int stencil_variant(int which, bool fp64)
         const char* e = getenv(which == 0 ? "ESP_SPREAD_VARIANT" : "ESP_GATHER_VARIANT");
         v[which] = e ? atoi(e) : -1;
         }
-    if (v[which] >= 0)
+    if (v[which] >= 0 && !(v[which] == 4 && (fp64 || which != 0)))
         return v[which];
     return which == 0 ? (fp64 ? 2 : 3) : (fp64 ? 3 : 1);
+    }
+
+int mfma_spread_tile(int P, bool fp64)
+    {
+    return (!fp64 && stencil_variant(0, false) == 4 && P >= 4 && P <= 12) ? 17 - P : 0;
     }
 
 size_t tp_lds_bytes(const MeshGeom& g, int P, bool fp64);
@@ -969,6 +1079,12 @@ bool stencil_tile_ok(int T, int P, bool fp64, bool batched)
         if (T != 2 && T != 4 && T != 8)  // the instantiated box edges
             return false;
         lds = own_lds_bytes(P, fp64);
+        }
+    else if (sv == 4)
+        {
+        if (T != 17 - P)  // 16-cell box
+            return false;
+        lds = 3 * 64 * 16 * 4 + size_t(P) * (MAX_WINDOW_DEG + 1) * 4;
         }
     else
         lds = (sv == 1 || sv == 3) ? tp_lds_bytes(g, P, fp64) : spread_lds_bytes(g, P, 0, fp64);
@@ -1015,7 +1131,11 @@ hipError_t spread(const MeshGeom& g, int P, int D, unsigned int n_tiles, const u
         {
 #define ESP_CASE(PP)                                                                                                         \
     case PP:                                                                                                                 \
-        if (variant == 2)                                                                                                    \
+        if (variant == 4 && sizeof(R) == 4 && PP >= 4 && PP <= 12)                                                          \
+            hipLaunchKernelGGL((k_spread_mfma<(PP >= 4 && PP <= 12) ? PP : 4>), dim3(n_tiles), dim3(256), 0, s, g, D, start,  \
+                               reinterpret_cast<const SortedRec<float>*>(rec), sidx, reinterpret_cast<const float*>(wcoef), \
+                               reinterpret_cast<unsigned long long*>(mesh), fx);                                           \
+        else if (variant == 2)                                                                                               \
             launch_spread_own<R, PP>(g, D, n_tiles, start, rec, sidx, wcoef, mesh, fx, s);                                  \
         else if (variant == 3)                                                                                               \
             hipLaunchKernelGGL((k_spread_tp<R, PP, tp_pair_lanes(PP)>), dim3(n_tiles), dim3(256), lds, s, g, D, start,                      \
