@@ -80,7 +80,7 @@ def stage_kernel(stage, precision):
     """Device kernel that one stage launches (as rocprofv3 names it, template arguments dropped)."""
     return {
         "pair": "k_pair", "bin": "k_bin", "scan": "k_scan", "scatter": "k_scatter",
-        "spread": "k_spread_own" if precision == "fp64" else "k_spread_tp",
+        "spread": "k_spread_tp",
         "fft_x_r2c": "k_fft_x_r2c", "fft_y": "k_fft_y", "fft_z_convolve": "k_fft_z_conv",
         "ifft_y": "k_fft_y", "ifft_x_c2r": "k_fft_x_c2r", "gather": "k_gather_tp", "finalize": "k_finalize",
     }[stage]

@@ -256,13 +256,17 @@ __device__ C2<R>* lds_fft(C2<R>* a,
 // ---------------------------------------------------------------------------
 // Mesh accumulation
 // ---------------------------------------------------------------------------
-//! Accumulator types: FP64 mode accumulates the mesh in FP64; the mixed mode accumulates it
-//! in fixed point (int32 in LDS, int64 in HBM), which is exact and order independent.
+//! Accumulator types.  Both precisions accumulate the mesh in fixed point, which is exact and
+//! independent of the order in which particles arrive: the mixed mode rounds FP32 taps into
+//! int32 LDS cells, the FP64 mode rounds FP64 taps into int64 cells, and both meshes in HBM are
+//! int64.  The step's scale (the tile scan's fx[0]) keeps every tap and every cell sum inside
+//! its integer type, so the spread mesh -- and everything computed from it -- is bit-identical
+//! from run to run and for any particle order or tiling of the same positions.
 template<typename R> struct Acc;
 template<> struct Acc<double>
     {
-    using lds_t = double;
-    using glob_t = double;
+    using lds_t = long long;
+    using glob_t = unsigned long long;
     };
 template<> struct Acc<float>
     {
@@ -278,32 +282,34 @@ __device__ __forceinline__ int round_to_int(float v)
     asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(v));
     return r;
     }
-__device__ __forceinline__ void lds_add(double* p, double v)
+//! 1.5 * 2^52 and its bit pattern: for |v| < 2^51, v + FIX_MAGIC lies in [2^52, 2^53), where
+//! the spacing of doubles is 1, so the sum's bits are FIX_MAGIC_BITS + rint(v) (round to
+//! nearest even) -- an int64 conversion in one FP64 add (or fma, which rounds the exact
+//! product) and one integer subtract.
+constexpr double FIX_MAGIC = 6755399441055744.0;
+constexpr long long FIX_MAGIC_BITS = 0x4338000000000000LL;
+__device__ __forceinline__ long long round_to_ll(double v)
     {
-    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    return __double_as_longlong(v + FIX_MAGIC) - FIX_MAGIC_BITS;
     }
-//! Scale applied to a value before it enters the LDS accumulator: the step's fixed-point scale
-//! (fx[0], written by the tile scan) for FP32, 1 for FP64.
+//! Scale applied to a value before it enters the fixed-point accumulator: the step's scale
+//! fx[0], written by the tile scan.
 template<typename R> __device__ __forceinline__ R acc_scale(const float* fx)
     {
-    return sizeof(R) == 4 ? R(fx[0]) : R(1);
+    return R(fx[0]);
     }
 //! LDS accumulation of a value already multiplied by acc_scale<R>().
-__device__ __forceinline__ void lds_add_scaled(double* p, double v)
+__device__ __forceinline__ void lds_add_scaled(long long* p, double v)
     {
-    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(p, round_to_ll(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
 __device__ __forceinline__ void lds_add_scaled(int* p, float v)
     {
     __hip_atomic_fetch_add(p, round_to_int(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
-__device__ __forceinline__ void glob_add(double* p, double v)
+__device__ __forceinline__ void glob_add(unsigned long long* p, long long v)
     {
-    unsafeAtomicAdd(p, v);
-    }
-__device__ __forceinline__ void glob_add(unsigned long long* p, int v)
-    {
-    atomicAdd(p, static_cast<unsigned long long>(static_cast<long long>(v)));
+    atomicAdd(p, static_cast<unsigned long long>(v));
     }
 // ---------------------------------------------------------------------------
 // Split-table lookup (EvaluatorPairPSWF.h:113-143, reduced s = r / rc).

@@ -26,9 +26,9 @@ hipError_t bin_particles(unsigned int n,
                          unsigned int* slot,
                          double* qpart,
                          hipStream_t s);
-//! Exclusive scan of the tile counts (which it clears); with fx (FP32 meshes) it also writes the
-//! step's fixed-point scale from the largest tile count and |q| (qpart: 3 per binning block).
-hipError_t scan_tiles(unsigned int n_tiles, unsigned int* count, unsigned int* start, int n_q, const double* qpart, float w3max,
+//! Exclusive scan of the tile counts (which it clears); with fx it also writes the step's
+//! fixed-point scale from the largest tile count and |q| (qpart: 3 per binning block).
+hipError_t scan_tiles(unsigned int n_tiles, unsigned int* count, unsigned int* start, int n_q, const double* qpart, FixedPointRule rule,
                       float* fx, hipStream_t s);
 template<typename R>
 hipError_t scatter_particles(unsigned int n,
@@ -144,8 +144,8 @@ struct BatchSys
     void* rec;
     SortedIdx* sidx;
     double* qpart;           // 3 per binning block: sum q, sum q^2, max |q|
-    float* fx;               // fixed-point scale, its inverse, lowered flag (FP32)
-    float w3max;             // largest |w_x w_y w_z| of the window
+    float* fx;               // fixed-point scale, its inverse, lowered flag
+    FixedPointRule fxr;      // how the scan chooses the scale
     double* evpart;
     double* qsum;
     double* ev_out;

@@ -705,7 +705,7 @@ void EspSolver::run_mesh(const SystemView& sys, const OutputView& out, unsigned 
               "bin_particles");
     mark(2, s);
     check_hip(kernel::scan_tiles(m_n_tiles, m_tile_count.as<unsigned int>(), m_tile_start.as<unsigned int>(), int(nqb),
-                                 m_qpart.as<double>(), m_w3max, sizeof(R) == 4 ? m_fx.as<float>() : nullptr, s),
+                                 m_qpart.as<double>(), fixed_point_rule(), m_fx.as<float>(), s),
               "scan_tiles");
     mark(3, s);
     check_hip(kernel::scatter_particles<R>(n, sys.group_idx, sys.postype, sys.charge, g, P, m_key.as<unsigned int>(),
@@ -717,7 +717,7 @@ void EspSolver::run_mesh(const SystemView& sys, const OutputView& out, unsigned 
                                 m_mesh.get(), m_fx.as<float>(), s),
               "spread");
     mark(5, s);
-    check_hip(kernel::fft_forward_x<R>(g, m_plan[0], m_tw[0].get(), m_mesh.get(), m_spec.get(), sizeof(R) == 4 ? m_fx.as<float>() : nullptr, s),
+    check_hip(kernel::fft_forward_x<R>(g, m_plan[0], m_tw[0].get(), m_mesh.get(), m_spec.get(), m_fx.as<float>(), s),
               "fft_forward_x");
     mark(6, s);
     check_hip(kernel::fft_y<R>(g, m_plan[1], m_tw[1].get(), m_spec.get(), -1, s), "fft_y");
@@ -933,8 +933,8 @@ void EspSolver::describe(const SystemView& sys, const OutputView& out, hipStream
     d.rec = m_rec.get();
     d.sidx = m_sidx.as<SortedIdx>();
     d.qpart = m_qpart.as<double>();
-    d.fx = fp64 ? nullptr : m_fx.as<float>();
-    d.w3max = m_w3max;
+    d.fx = m_fx.as<float>();
+    d.fxr = fixed_point_rule();
     d.evpart = m_evpart.as<double>();
     d.qsum = m_qsum.as<double>();
     d.ev_out = out.ext_energy_virial ? out.ext_energy_virial : m_ev_internal.as<double>();
@@ -944,7 +944,7 @@ void EspSolver::describe(const SystemView& sys, const OutputView& out, hipStream
 void EspSolver::fixed_point_scale(float& scale, int& lowered)
     {
     float h[4] = {0, 0, 0, 0};
-    if (m_fx.get() && m_p.precision != 0)
+    if (m_fx.get())
         {
         check_hip(hipDeviceSynchronize(), "hipDeviceSynchronize");
         check_hip(hipMemcpy(h, m_fx.get(), sizeof(h), hipMemcpyDeviceToHost), "hipMemcpy fx");

@@ -163,8 +163,8 @@ class EspSolver
     void charge_sums(double& q, double& q2);
     //! Tile edge and tile count chosen for the mesh.
     const MeshGeom& geometry() const { return m_geom; }
-    //! Fixed-point scale of the last FP32 step and whether the tile bound lowered it
-    //! (synchronises; 0 in FP64 or before the first step).
+    //! Fixed-point scale of the last step's mesh and whether a population bound lowered it
+    //! (synchronises; 0 before the first step).
     void fixed_point_scale(float& scale, int& lowered);
     //! Device pointers of the influence tables (half spectrum), for tests.
     const double* device_green() const { return m_G.as<double>(); }
@@ -243,8 +243,14 @@ class EspSolver
     std::vector<double> m_pair_rec;  // interleaved FP64 split table for the pair evaluator
     DeviceBuffer m_pair_rec_dev;
     DeviceBuffer m_qpart, m_evpart, m_qsum, m_ev_internal;
-    DeviceBuffer m_fx;          // FP32 mesh: fixed-point scale, its inverse, lowered flag (float[4])
+    DeviceBuffer m_fx;          // the mesh's fixed-point scale, its inverse, lowered flag (float[4])
     float m_w3max = 1.0f;       // largest |w_x w_y w_z| of the window
+    //! The scan's rule for the fixed-point scale: int32 LDS cells for FP32 taps, int64 cells
+    //! (over the tile boxes that overlap a cell) for FP64 taps.
+    FixedPointRule fixed_point_rule() const
+        {
+        return FixedPointRule {m_w3max, tile_box_cover(m_geom.T, int(m_p.order)), m_p.precision == 0 ? 1 : 0};
+        }
     unsigned int m_cap_particles = 0;
     };
 

@@ -44,10 +44,21 @@ struct SortedIdx
     uint32_t idx;     // particle index into postype / force
     };
 
-//! Fixed-point scale for the deterministic mixed-precision mesh accumulation, for charges of
-//! magnitude <= 1: the scale in use is 2^23 / 2^ceil(log2 max|q|), halved further while the
-//! largest tile's charge could overflow the int32 LDS accumulators (chosen on the device each
-//! step by the tile scan, fixed_point_scale).
-constexpr double FIXED_SCALE = 8388608.0;  // 2^23
+//! How the tile scan chooses the mesh's fixed-point scale each step (fixed_point_scale in
+//! kernels_mesh.hip): 2^23 / 2^ceil(log2 max|q|) for FP32 taps into int32 LDS cells, 2^50 /
+//! 2^ceil(log2 max|q|) for FP64 taps into int64 cells, halved while the most populated tile
+//! could overflow the accumulators.
+struct FixedPointRule
+    {
+    float w3max;  // largest |w_x w_y w_z| of the window
+    float cover;  // tile boxes that overlap one mesh cell: (1 + ceil((P - 1) / T))^3
+    int wide;     // 0: FP32 taps into int32 LDS cells; 1: FP64 taps into int64 cells
+    };
+//! Tile boxes (edge T + P - 1, one per tile of edge T) that contain a given mesh cell.
+inline float tile_box_cover(int T, int P)
+    {
+    const int per_axis = 1 + (P - 1 + T - 1) / T;
+    return float(per_axis * per_axis * per_axis);
+    }
 
     } // namespace esp

@@ -37,6 +37,7 @@ __global__ void __launch_bounds__(256) k_cell_count(unsigned int N, const double
     slot[i] = atomicAdd(&count[c], 1u);
     }
 
+//! Cell members in arrival order (the slots come from atomics, so the order varies by run).
 __global__ void __launch_bounds__(256) k_cell_fill(unsigned int N, const unsigned int* __restrict__ cell, const unsigned int* __restrict__ slot,
                                                    const unsigned int* __restrict__ start, unsigned int* __restrict__ members)
     {
@@ -44,6 +45,24 @@ __global__ void __launch_bounds__(256) k_cell_fill(unsigned int N, const unsigne
     if (i >= N)
         return;
     members[start[cell[i]] + slot[i]] = i;
+    }
+
+//! Cell members in ascending particle index: particle i's rank in its cell is the number of
+//! members below i.  The neighbour rows then list candidates in an order fixed by the
+//! positions alone, so pair sums over them are reproducible bit for bit.
+__global__ void __launch_bounds__(256) k_cell_order(unsigned int N, const unsigned int* __restrict__ cell,
+                                                    const unsigned int* __restrict__ start, const unsigned int* __restrict__ unsorted,
+                                                    unsigned int* __restrict__ members)
+    {
+    const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N)
+        return;
+    const unsigned int c = cell[i];
+    const unsigned int m0 = start[c], m1 = start[c + 1];
+    unsigned int rank = 0;
+    for (unsigned int m = m0; m < m1; ++m)
+        rank += unsorted[m] < i ? 1u : 0u;
+    members[m0 + rank] = i;
     }
 
 //! Full neighbour list: for every particle, all others within r_list (minimum image), excluding
@@ -498,12 +517,13 @@ hipError_t cell_count(unsigned int N, const double* pos, const CellGeom& cg, uns
     return hipGetLastError();
     }
 
-hipError_t cell_fill(unsigned int N, const unsigned int* cell, const unsigned int* slot, const unsigned int* start, unsigned int* members,
-                     hipStream_t s)
+hipError_t cell_fill(unsigned int N, const unsigned int* cell, const unsigned int* slot, const unsigned int* start, unsigned int* unsorted,
+                     unsigned int* members, hipStream_t s)
     {
     if (N == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_cell_fill, dim3(nblocks(N)), dim3(256), 0, s, N, cell, slot, start, members);
+    hipLaunchKernelGGL(k_cell_fill, dim3(nblocks(N)), dim3(256), 0, s, N, cell, slot, start, unsorted);
+    hipLaunchKernelGGL(k_cell_order, dim3(nblocks(N)), dim3(256), 0, s, N, cell, start, unsorted, members);
     return hipGetLastError();
     }
 

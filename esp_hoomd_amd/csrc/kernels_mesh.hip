@@ -13,40 +13,62 @@ namespace esp
 namespace dev
 {
 
-__device__ __forceinline__ double mesh_load_clear(double* p, double /*inv_scale*/)
-    {
-    const double v = *p;
-    *p = 0.0;
-    return v;
-    }
-__device__ __forceinline__ float mesh_load_clear(unsigned long long* p, double inv_scale)
+//! Read a fixed-point mesh cell as a real (times 1 / S) and clear it for the next step.
+template<typename R> __device__ __forceinline__ R mesh_load_clear(unsigned long long* p, double inv_scale)
     {
     const long long v = static_cast<long long>(*p);
     *p = 0ull;
-    return static_cast<float>(static_cast<double>(v) * inv_scale);
+    return static_cast<R>(static_cast<double>(v) * inv_scale);
     }
 
-//! The step's fixed-point scale for the FP32 mesh accumulation: fx[0] = S, fx[1] = 1 / S,
-//! fx[2] = 1 when the tile bound lowered S.  S = 2^23 / 2^ceil(log2 max|q|) (clamped to
-//! [1, 2^40]) keeps the quantisation step relative to the charges (2^-23 of the largest), and
-//! is halved while the most populated tile could overflow an int32 LDS cell: a cell of a tile
-//! box sums at most `cmax` contributions, each |q w_x w_y w_z| S <= max|q| w3max S rounded.
+//! The step's fixed-point scale S for the mesh accumulation: fx[0] = S, fx[1] = 1 / S, fx[2] = 1
+//! when a population bound lowered S.  S is a power of two, so both are exact in FP32.
+//!  * FP32 taps into int32 LDS cells (rule.wide = 0): S = 2^23 / 2^ceil(log2 max|q|) (clamped to
+//!    [1, 2^40]) keeps the quantisation step relative to the charges (2^-23 of the largest), and
+//!    is halved while the most populated tile could overflow an int32 LDS cell: a cell of a
+//!    tile box sums at most `cmax` contributions, each |q w_x w_y w_z| S <= max|q| w3max S
+//!    rounded.
+//!  * FP64 taps into int64 cells (rule.wide = 1): S = 2^50 / 2^ceil(log2 max|q|) (clamped to
+//!    [2^-100, 2^100]), a step of 2^-50 of the largest charge (FP64 keeps 2^-52 of a value);
+//!    halved while a tap could reach 2^51 (the limit of round_to_ll) or while the cells of the
+//!    `cover` tile boxes that overlap one mesh cell, `cmax` contributions each, could overflow
+//!    int64.
 //! Without halving, S depends only on the charges, so every tiling gives the same sums.
-__device__ __forceinline__ void fixed_point_scale(unsigned int cmax, double qmax, float w3max, float* fx)
+__device__ __forceinline__ void fixed_point_scale(unsigned int cmax, double qmax, const FixedPointRule& rule, float* fx)
     {
     const double qm = qmax > 0.0 ? qmax : 1.0;
     int e = 0;
     const double m = frexp(qm, &e);  // qm = m 2^e, m in [0.5, 1)
     const int ceil_log2 = (m == 0.5) ? e - 1 : e;
-    int k = 23 - ceil_log2;
-    k = k < 0 ? 0 : (k > 40 ? 40 : k);
-    double S = ldexp(1.0, k);
+    const double tap = qm * double(rule.w3max);
     float lowered = 0.0f;
-    const double limit = 0.9 * 2147483648.0;
-    while (S > 1.0 && double(cmax) * (qm * double(w3max) * S + 0.5) >= limit)
+    double S;
+    if (rule.wide)
         {
-        S *= 0.5;
-        lowered = 1.0f;
+        int k = 50 - ceil_log2;
+        k = k < -100 ? -100 : (k > 100 ? 100 : k);
+        S = ldexp(1.0, k);
+        const double tap_limit = 2251799813685248.0;                  // 2^51
+        const double sum_limit = 0.9 * 9223372036854775808.0;         // 0.9 * 2^63
+        const double n = double(rule.cover) * double(cmax);
+        while (k > -120 && (tap * S >= tap_limit || n * (tap * S + 0.5) >= sum_limit))
+            {
+            S *= 0.5;
+            --k;
+            lowered = 1.0f;
+            }
+        }
+    else
+        {
+        int k = 23 - ceil_log2;
+        k = k < 0 ? 0 : (k > 40 ? 40 : k);
+        S = ldexp(1.0, k);
+        const double limit = 0.9 * 2147483648.0;
+        while (S > 1.0 && double(cmax) * (tap * S + 0.5) >= limit)
+            {
+            S *= 0.5;
+            lowered = 1.0f;
+            }
         }
     fx[0] = float(S);
     fx[1] = float(1.0 / S);
@@ -145,7 +167,7 @@ __global__ void __launch_bounds__(256) k_bin(unsigned int n,
 //! and in flight together, the thread sums are scanned with wave shuffles, then across the 16
 //! waves.
 __device__ __forceinline__ void scan_one(unsigned int n_tiles, unsigned int* __restrict__ count, unsigned int* __restrict__ start,
-                                         int n_q, const double* __restrict__ qpart, float w3max, float* __restrict__ fx)
+                                         int n_q, const double* __restrict__ qpart, FixedPointRule rule, float* __restrict__ fx)
     {
     constexpr int CH = 16;
     __shared__ unsigned int wsum[16];
@@ -235,15 +257,15 @@ __device__ __forceinline__ void scan_one(unsigned int n_tiles, unsigned int* __r
                 cmax = max(cmax, wmax[w]);
                 qm = fmax(qm, wq[w]);
                 }
-            fixed_point_scale(cmax, qm, w3max, fx);
+            fixed_point_scale(cmax, qm, rule, fx);
             }
         }
     }
 
 __global__ void __launch_bounds__(1024) k_scan(unsigned int n_tiles, unsigned int* __restrict__ count, unsigned int* __restrict__ start,
-                                              int n_q, const double* __restrict__ qpart, float w3max, float* __restrict__ fx)
+                                              int n_q, const double* __restrict__ qpart, FixedPointRule rule, float* __restrict__ fx)
     {
-    scan_one(n_tiles, count, start, n_q, qpart, w3max, fx);
+    scan_one(n_tiles, count, start, n_q, qpart, rule, fx);
     }
 
 //! Write group member i's record into its tile-sorted slot.
@@ -319,7 +341,7 @@ __device__ __forceinline__ void fft_x_r2c_block(int blk,
     extern __shared__ __align__(16) unsigned char smem[];
     const int nx = g.nx;
     const int ls = nx + 1;
-    const double inv_scale = fx ? double(fx[1]) : 1.0;
+    const double inv_scale = double(fx[1]);
     C2<R>* tw = reinterpret_cast<C2<R>*>(smem);
     C2<R>* a = tw + nx;
     C2<R>* b = a + pairs_per_block * ls;
@@ -337,9 +359,9 @@ __device__ __forceinline__ void fft_x_r2c_block(int blk,
             const int l1 = l0 + 1;
             R v0 = R(0), v1 = R(0);
             if (l0 < nlines)
-                v0 = R(mesh_load_clear(&mesh[size_t(l0) * nx + x], inv_scale));
+                v0 = mesh_load_clear<R>(&mesh[size_t(l0) * nx + x], inv_scale);
             if (l1 < nlines)
-                v1 = R(mesh_load_clear(&mesh[size_t(l1) * nx + x], inv_scale));
+                v1 = mesh_load_clear<R>(&mesh[size_t(l1) * nx + x], inv_scale);
             return C2<R> {v0, v1};
         },
         [&](int idx, const C2<R>& v)
@@ -924,7 +946,7 @@ __global__ void __launch_bounds__(1024) kb_scan(const BatchSys* __restrict__ ds,
     for (unsigned int it = blockIdx.x; it < l.n; it += gridDim.x)
         {
         const BatchSys& d = ds[l.items[it].sys];
-        scan_one(d.n_tiles, d.tile_count, d.tile_start, int(d.n_qblk), d.qpart, d.w3max, d.fx);
+        scan_one(d.n_tiles, d.tile_count, d.tile_start, int(d.n_qblk), d.qpart, d.fxr, d.fx);
         __syncthreads();
         }
     }
@@ -1035,10 +1057,10 @@ hipError_t bin_particles(unsigned int n,
     return hipGetLastError();
     }
 
-hipError_t scan_tiles(unsigned int n_tiles, unsigned int* count, unsigned int* start, int n_q, const double* qpart, float w3max,
+hipError_t scan_tiles(unsigned int n_tiles, unsigned int* count, unsigned int* start, int n_q, const double* qpart, FixedPointRule rule,
                       float* fx, hipStream_t s)
     {
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, n_tiles, count, start, n_q, qpart, w3max, fx);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, n_tiles, count, start, n_q, qpart, rule, fx);
     return hipGetLastError();
     }
 

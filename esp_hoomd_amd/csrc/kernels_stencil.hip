@@ -7,10 +7,12 @@
 // a lane owns stencil columns (a, b) -- pairs of x offsets in the gather, so one ds_read2 feeds
 // two columns -- and runs the P-long z loop with the z weights held in registers.
 //
-// Spreading accumulates in the LDS box with LDS atomics and adds the box to the mesh in HBM
-// (FP64 atomics, or int64 fixed point in the mixed mode).  The gather reduces its per-column
-// partial sums with cross-lane shuffles inside each G-lane segment and adds the particle's
-// force with no-return global atomics, so no wave waits on a global load per particle.
+// Spreading accumulates in the LDS box (LDS atomics, or registers in the column-ownership form)
+// and adds the box to the mesh in HBM with integer atomics: both precisions accumulate in fixed
+// point (int64 cells for FP64 taps, int32 cells in LDS for FP32 taps; int64 in HBM), so the
+// mesh does not depend on the particle order.  The gather reduces its per-column partial sums
+// across the lanes that share a particle and adds the particle's mesh force to its output
+// record with one 32-byte load and store (one thread per particle).
 //
 // Reference counterparts: gpu_esp_assign_particles_kernel (ESPForceComputeGPU.cu:206-272) and
 // gpu_esp_interpolate_forces_kernel (ESPForceComputeGPU.cu:292-373).
@@ -19,6 +21,7 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <type_traits>
 
 namespace esp
 {
@@ -784,12 +787,11 @@ __global__ void __launch_bounds__(256) k_spread_mfma(MeshGeom g,
 // (T+P-1)^3 box in registers.  Particles are broadcast through LDS: x and y
 // weights, and the z weights already shifted to box coordinates (zero outside
 // the stencil).  A lane whose column lies in a particle's xy footprint adds
-// B = T+P-1 products; no LDS atomics, fixed accumulation order within a tile.
+// B = T+P-1 products; no LDS atomics.  FP64: every product is rounded to the fixed-point grid
+// as it is added (one fma with FIX_MAGIC, summed as int64 bit patterns), so the column sums are
+// exact integers whatever the order of the tile's particles.  FP32: the column sums are FP32 in
+// the tile's particle order and enter the int64 mesh rounded.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void glob_add_real(double* p, double v, double /*scale*/)
-    {
-    unsafeAtomicAdd(p, v);
-    }
 __device__ __forceinline__ void glob_add_real(unsigned long long* p, float v, double scale)
     {
     atomicAdd(p, static_cast<unsigned long long>(__double2ll_rn(double(v) * scale)));
@@ -828,10 +830,14 @@ __device__ __forceinline__ void spread_tile_own(unsigned int tile,
     const int cx = int(threadIdx.x) % B;
     const int cy = int(threadIdx.x) / B;
     const bool owner = int(threadIdx.x) < B * B;
-    R col[B];
+    constexpr bool FIX = sizeof(R) == 8;         // FP64: int64 bit-pattern sums (see above)
+    using ColT = typename std::conditional<FIX, unsigned long long, R>::type;
+    ColT col[B];
 #pragma unroll
     for (int z = 0; z < B; ++z)
-        col[z] = R(0);
+        col[z] = ColT(0);
+    unsigned int n_add = 0;                      // FP64: products added per z entry
+    const R qscale = FIX ? R(fx[0]) : R(1);
     __syncthreads();
 
     for (unsigned int c0 = s0; c0 < s1; c0 += CH)
@@ -863,7 +869,7 @@ __device__ __forceinline__ void spread_tile_own(unsigned int tile,
                 {
                 k = j - BZ;
                 t = r.tx;
-                scale = r.q;
+                scale = r.q * qscale;
                 }
             else
                 {
@@ -897,9 +903,19 @@ __device__ __forceinline__ void spread_tile_own(unsigned int tile,
                 {
                 const R w = Wxy[p * 2 * P + dx] * Wxy[p * 2 * P + P + dy];
                 const R* wz = Wz + p * BZ;
+                if constexpr (FIX)
+                    {
 #pragma unroll
-                for (int z = 0; z < B; ++z)
-                    col[z] += w * wz[z];
+                    for (int z = 0; z < B; ++z)
+                        col[z] += static_cast<unsigned long long>(__double_as_longlong(fma(w, wz[z], FIX_MAGIC)));
+                    ++n_add;
+                    }
+                else
+                    {
+#pragma unroll
+                    for (int z = 0; z < B; ++z)
+                        col[z] += w * wz[z];
+                    }
                 }
             }
         __syncthreads();
@@ -911,14 +927,29 @@ __device__ __forceinline__ void spread_tile_own(unsigned int tile,
         tile_coords(g, tile, tx, ty, tz);
         const int gx = (tx * T + cx) % g.nx;
         const int gy = (ty * T + cy) % g.ny;
-        const double scale = sizeof(R) == 4 ? double(fx[0]) : 1.0;
+        if constexpr (FIX)
+            {
+            // remove the magic constant of every product added (modulo 2^64, like the sums)
+            const unsigned long long bias = static_cast<unsigned long long>(n_add) * static_cast<unsigned long long>(FIX_MAGIC_BITS);
 #pragma unroll
-        for (int z = 0; z < B; ++z)
-            if (col[z] != R(0))
+            for (int z = 0; z < B; ++z)
                 {
-                const int gz = (tz * T + z) % g.nz;
-                glob_add_real(&mesh[(size_t(gz) * g.ny + gy) * g.nx + gx], col[z], scale);
+                const unsigned long long v = col[z] - bias;
+                if (v != 0ull)
+                    atomicAdd(&mesh[(size_t((tz * T + z) % g.nz) * g.ny + gy) * g.nx + gx], v);
                 }
+            }
+        else
+            {
+            const double scale = double(fx[0]);
+#pragma unroll
+            for (int z = 0; z < B; ++z)
+                if (col[z] != R(0))
+                    {
+                    const int gz = (tz * T + z) % g.nz;
+                    glob_add_real(&mesh[(size_t(gz) * g.ny + gy) * g.nx + gx], col[z], scale);
+                    }
+            }
         }
     }
 
@@ -1034,12 +1065,13 @@ static void launch_spread_own(const MeshGeom& g, int D, unsigned int n_tiles, co
 //! 2 = column ownership (spread only), 3 = thread per particle with two lanes per particle
 //! (even orders; half the rows each), 4 = matrix cores (FP32 spread with a 16-cell box, tile
 //! edge 17 - P; other precisions and the gather use their defaults).
-//! Defaults (measured on MI355X at N = 1e6, P = 10): FP64 spreads by column ownership and
-//! gathers with two lanes per particle; FP32 spreads with two lanes (fixed-point sums: the same
-//! result as one lane) and gathers with one, as the batched kernels do (their 12^3 tiles
-//! already fill the workgroup), so a mixed-precision system's forces are bit-identical alone
-//! and in a batch.  The environment variables ESP_SPREAD_VARIANT / ESP_GATHER_VARIANT override
-//! them.
+//! Defaults (measured on MI355X at N = 1e6, P = 10): both precisions spread with two lanes per
+//! particle (fixed-point sums: the same result as one lane, so as the batched kernels; FP64
+//! 0.49 ms against 0.61 ms for column ownership with its per-product int64 rounding at 180^3)
+//! and gather with one, as the batched kernels do (their 12^3 tiles already fill the
+//! workgroup; FP64 0.433 ms against 0.427 ms with two lanes), so a system's forces are
+//! bit-identical alone and in a batch in both precisions.  The environment variables
+//! ESP_SPREAD_VARIANT / ESP_GATHER_VARIANT override them.
 static int stencil_variant(int which, bool fp64)
     {
     static int v[2] = {-2, -2};
@@ -1050,7 +1082,8 @@ static int stencil_variant(int which, bool fp64)
         }
     if (v[which] >= 0 && !(v[which] == 4 && (fp64 || which != 0)))
         return v[which];
-    return which == 0 ? (fp64 ? 2 : 3) : (fp64 ? 3 : 1);
+    (void)fp64;
+    return which == 0 ? 3 : 1;
     }
 
 int mfma_spread_tile(int P, bool fp64)

@@ -226,7 +226,7 @@ class Solver:
         return scr.reshape(n, 5), dscr.reshape(n, 5), rlo, dri
 
     def fixed_point_scale(self):
-        """(scale, lowered) of the last mixed-precision step's fixed-point mesh accumulation."""
+        """(scale, lowered) of the last step's fixed-point mesh accumulation (both precisions)."""
         sc, low = C.c_float(), C.c_int()
         _lib.check(_lib.load().esp_get_fixed_point_scale(self._ctx, C.byref(sc), C.byref(low)), self._ctx)
         return float(sc.value), bool(low.value)

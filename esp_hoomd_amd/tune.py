@@ -194,15 +194,14 @@ def best_split(P: int, target: float, precision: int, c_max: float = 32.0, scale
 
 
 def batch_tile(P: int, precision: int) -> int:
-    """Tile edge for a system solved inside a batch: in mixed precision 12, else 8, whichever
-    first has a thread-per-particle LDS box ((T + P - 1)^3 cells plus the window table) within
-    64 KiB; in FP64 8 (the same solver also runs alone, where FP64 spreads by column ownership,
-    compiled for edges 2, 4 and 8).  Batches get their parallelism from many systems, so tiles
-    bigger than the single-system 8 (more particles per workgroup, less halo) win there.  On
-    MI355X, for 512 systems of 500-50,000 particles in mixed precision: 9.6 ms per step at 12,
-    10.0 at 16, 10.0 at 10, 12.0 at 8."""
+    """Tile edge for a system solved inside a batch: 12, else 8, whichever first has a
+    thread-per-particle LDS box ((T + P - 1)^3 cells plus the window table) within 64 KiB
+    (FP64 cells take 8 bytes, so FP64 at P = 10 gets 8).  Batches get their parallelism from
+    many systems, so tiles bigger than the single-system 8 (more particles per workgroup, less
+    halo) win there.  On MI355X, for 512 systems of 500-50,000 particles in mixed precision:
+    9.6 ms per step at 12, 10.0 at 16, 10.0 at 10, 12.0 at 8."""
     el = 8 if precision == _lib.ESP_PREC_FP64 else 4
-    for T in ((8,) if precision == _lib.ESP_PREC_FP64 else (12, 8)):
+    for T in (12, 8):
         B = T + P - 1
         if ((B ** 3 + 1) * el + 15) // 16 * 16 + P * 16 * el <= 64 * 1024:
             return T

@@ -36,7 +36,9 @@ extern "C" {
 #define ESP_ERR_UNSUPPORTED (-6)
 
 /* precision modes */
-#define ESP_PREC_FP64 0   /* every stage in FP64 */
+#define ESP_PREC_FP64 0   /* every stage in FP64; the mesh sums FP64 stencil products in int64
+                             fixed point, so results are bit-reproducible (run to run, any
+                             particle order or tiling, alone or in a batch) */
 #define ESP_PREC_MIXED 1  /* FP32 spread/FFT/gather/pair, fixed-point mesh accumulation,
                              FP64 influence function, self/k=0 terms and reductions */
 
@@ -208,10 +210,11 @@ int esp_batch_stats(esp_batch* batch, unsigned long long* captures, unsigned lon
 int esp_get_info(esp_ctx* ctx, esp_info* info);
 /* getQSum / getQ2Sum (ESPForceCompute.h:113-114); synchronises with the device */
 int esp_get_charge_sums(esp_ctx* ctx, double* q_sum, double* q2_sum);
-/* Mixed precision: the fixed-point scale of the last step's mesh accumulation (2^23 / 2^ceil(log2
- * max|q|), lower when the most populated tile could overflow its int32 accumulators, then
- * *lowered = 1); 0 in FP64 or before the first step.  Synchronises.  No counterpart in the
- * reference (FP64 only). */
+/* The fixed-point scale of the last step's mesh accumulation: mixed precision 2^23 / 2^ceil(log2
+ * max|q|), FP64 2^50 / 2^ceil(log2 max|q|); lower when the most populated tile could overflow
+ * its accumulators (int32 LDS cells; int64 cells summed over the overlapping tile boxes), then
+ * *lowered = 1; 0 before the first step.  Synchronises.  No counterpart in the reference,
+ * which sums the mesh with FP64 atomics. */
 int esp_get_fixed_point_scale(esp_ctx* ctx, float* scale, int* lowered);
 /* getTableSize / getTableEntry (ESPForceCompute.cc:129-141, 336-372) */
 int esp_get_table_size(esp_ctx* ctx, unsigned int* n);

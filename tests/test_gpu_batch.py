@@ -1,9 +1,8 @@
 """Batched solves: many systems of uneven size, mesh and stencil order in one launch per stage.
 
-Each system's batched result must equal what its own context computes alone (bit-exact in
-mixed precision, whose mesh accumulation is fixed point and whose per-particle sums have a
-fixed order; 1e-12 relative in FP64, where the LDS accumulation order may vary) and match
-the oracle (FP64: 1e-10 relative RMS).  The captured hipGraph is replayed several times with
+Each system's batched result must equal what its own context computes alone, bit for bit in
+both precisions (the mesh accumulates in fixed point, and every per-particle sum has a fixed
+order), and match the oracle (FP64: 1e-10 relative RMS).  The captured hipGraph is replayed several times with
 new positions and host work in between, and every replay is checked against eager results.
 """
 import numpy as np
@@ -67,7 +66,7 @@ def test_batch_equals_each_system_alone(precision):
     res = batch.compute([(s, d) for s, d, *_ in systems])
     for k, (a, r) in enumerate(zip(alone, res)):
         print(f"system {k}", flush=True)
-        _check_equal(r, a, exact=(precision == 1))
+        _check_equal(r, a, exact=True)
 
 
 def test_batch_fp64_matches_oracle():

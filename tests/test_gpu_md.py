@@ -30,6 +30,15 @@ def test_nlist_matches_kdtree(N, L, r_cut, buf, inner):
     assert all(i not in set(lists[i].tolist()) for i in range(N))
     # symmetric full list
     assert sum(len(x) for x in lists) == 2 * len(exp)
+    # candidates are listed cell by cell, each cell in ascending index (not in the order the
+    # binning atomics happened to run): at most 27 ascending runs per row, and a second list
+    # built from the same positions is identical
+    assert max(1 + int(np.sum(np.diff(x.astype(np.int64)) < 0)) for x in lists if len(x)) <= 27
+    nl2 = md.NeighborList(buffer=buf, inner=inner)
+    st2 = md.State(pos, q, (L, L, L), exclusions=excl)
+    nl2.update(st2, r_cut, force_rebuild=True)
+    n2, lists2 = nl2.host_arrays(st2)
+    assert np.array_equal(n, n2) and all(np.array_equal(a, b) for a, b in zip(lists, lists2))
     if inner == 0.0:
         assert st.inner_ptrs is None
         return

@@ -1,7 +1,8 @@
 """GPU parity: the HIP path against the CPU oracle on identical seeded inputs.
 
 FP64 mode: forces, per-particle energies and virials, external energy and virial within
-1e-10 relative RMS.  Mixed mode: within 2e-6 of the FP64 oracle at the same parameters.
+1e-10 relative RMS, and bit-for-bit reproducible (the mesh accumulates in int64 fixed point).
+Mixed mode: within 2e-6 of the FP64 oracle at the same parameters.
 """
 import numpy as np
 import pytest
@@ -183,7 +184,7 @@ def test_mixed_result_is_independent_of_the_tile_edge(tile):
 def test_tile_edges_outside_the_kernels_are_refused():
     pos, q = random_electrolyte(100, 8.0, seed=2)
     with pytest.raises(Exception):
-        run_gpu(make_params(32, 8, 12.0, 3.0, precision=0, tile=12), pos, q, (8.0, 8.0, 8.0))  # FP64 column spread: 2, 4, 8
+        run_gpu(make_params(32, 8, 12.0, 3.0, precision=0, tile=16), pos, q, (8.0, 8.0, 8.0))  # FP64 box 23^3 > 64 KiB
     with pytest.raises(Exception):
         run_gpu(make_params(32, 8, 12.0, 3.0, precision=1, tile=17), pos, q, (8.0, 8.0, 8.0))
 
@@ -224,3 +225,75 @@ def test_mixed_fixed_point_guards_dense_high_charge_tiles():
     print(f"dense tiles: scale 2^{np.log2(fx):.0f} lowered={lowered}, mixed vs FP64 {err:.3e}", flush=True)
     assert lowered and fx < 2.0 ** 22
     assert err < 1e-4
+
+
+def _permuted_system(pos, q, box, r_list, perm):
+    """The same system with particle k = old particle perm[k]; every neighbour row keeps its
+    order (mapped to the new indices), as a caller that relabels particles would pass it."""
+    from esp_hoomd_amd.nlist import build_full_nlist
+    from esp_hoomd_amd.solver import DeviceSystem
+    n, lst, head = build_full_nlist(pos, box, r_list)
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(len(perm))
+    n_new = n[perm]
+    rows = [inv[lst[int(head[i]):int(head[i]) + int(n[i])]] for i in perm]
+    head_new = np.concatenate([[0], np.cumsum(n_new)[:-1]]).astype(np.uint64)
+    sysd = DeviceSystem(pos[perm], q[perm], box)
+    sysd.set_nlist(n_new, np.concatenate(rows).astype(np.uint32), head_new)
+    return sysd, inv
+
+
+def test_fp64_is_bitwise_reproducible_and_independent_of_particle_order():
+    """The FP64 mesh is summed in int64 fixed point and every particle's force, energy and
+    virial is reduced by one thread in its neighbour-row order, so the same input gives the same
+    bits on every run, and relabelling the particles (rows kept in order) permutes the outputs
+    exactly -- the external energy and virial included."""
+    from esp_hoomd_amd.solver import Solver
+    N, L = 3000, 16.0
+    pos, q = random_electrolyte(N, L, seed=41)
+    params = make_params(40, 8, 12.0, 3.5, precision=0)
+    solver, _, a = run_gpu(params, pos, q, (L, L, L), skin=0.0)
+    _, _, b = run_gpu(params, pos, q, (L, L, L), skin=0.0)
+    for k in ("force", "virial", "ext"):
+        assert np.array_equal(a[k], b[k]), k
+    fx, lowered = solver.fixed_point_scale()
+    print(f"FP64 fixed-point scale 2^{np.log2(fx):.0f}, lowered={lowered}", flush=True)
+    assert fx == 2.0 ** 50 and not lowered
+    perm = np.random.default_rng(42).permutation(N)
+    sysd, inv = _permuted_system(pos, q, (L, L, L), 3.5, perm)
+    c = Solver(params).compute(sysd)
+    assert np.array_equal(c["force"][inv], a["force"])
+    assert np.array_equal(c["virial"][:, inv], a["virial"])
+    assert np.array_equal(c["ext"], a["ext"])
+
+
+@pytest.mark.parametrize("tile", [2, 4, 12])
+def test_fp64_result_is_independent_of_the_tile_edge(tile):
+    """Each particle's stencil products are the same whatever tile holds it, and their int64 sums
+    do not depend on how tiles split them, so FP64 results match the default tiling bit for bit."""
+    N, L = 600, 12.0
+    pos, q = random_electrolyte(N, L, seed=31)
+    _, _, ref = run_gpu(make_params(40, 8, 12.0, 3.5, precision=0, tile=8), pos, q, (L, L, L))
+    print(f"tile {tile}", flush=True)
+    _, _, res = run_gpu(make_params(40, 8, 12.0, 3.5, precision=0, tile=tile), pos, q, (L, L, L))
+    assert np.array_equal(res["force"], ref["force"])
+    assert np.array_equal(res["ext"], ref["ext"])
+
+
+def test_fp64_fixed_point_guards_dense_high_charge_tiles():
+    """60,000 charges of +-2 packed into a 2^3 cube put at least 7,500 into one tile; over the
+    tile boxes that overlap a mesh cell the int64 cells could overflow at the 2^49 scale of
+    |q| = 2, so the scan lowers the scale, and the result keeps FP64 accuracy against the
+    oracle."""
+    N, L = 60_000, 10.0
+    rng = np.random.default_rng(5)
+    pos = rng.uniform(0.0, 2.0, (N, 3))
+    q = np.where(np.arange(N) % 2 == 0, 2.0, -2.0)
+    p64 = make_params(32, 4, 3.0, 0.15, precision=0)
+    solver, _, res = run_gpu(p64, pos, q, (L, L, L), skin=0.0)
+    fx, lowered = solver.fixed_point_scale()
+    print(f"FP64 dense tiles: scale 2^{np.log2(fx):.0f} lowered={lowered}", flush=True)
+    assert lowered and fx < 2.0 ** 49
+    ref = oracle_for(solver, (L, L, L), p64).compute(pos, q)
+    assert rel_rms(res["force"][:, :3], ref["force"][:, :3]) < FP64_TOL
+    assert rel_rms(res["ext"][1:7], ref["ext_virial"]) < FP64_TOL
