@@ -66,6 +66,47 @@ def build(verbose: bool = False) -> Path:
     return LIB
 
 
+# Host-only AddressSanitizer: each -fsanitize option goes to the host compilation (device code
+# is built exactly as in the library).
+ASAN_HOST = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fno-omit-frame-pointer"]
+ASAN_DRIVER_SRC = PKG.parent / "tests" / "asan" / "abi_driver.cpp"
+ASAN_DRIVER = PKG.parent / "tests" / "asan" / "abi_driver"
+
+
+def build_asan(verbose: bool = False) -> Path:
+    """Link tests/asan/abi_driver: the C-ABI driver over the library's host sources (csrc/*.cpp)
+    compiled with host AddressSanitizer and its kernel objects (csrc/*.hip) as built by
+    build().  The executable carries the ASan runtime itself, so it needs no preloading."""
+    adir = BUILD / "asan"
+    adir.mkdir(parents=True, exist_ok=True)
+    kernels = [BUILD / (src.stem + "_hip.o") for src in sorted(CSRC.glob("*.hip"))]
+    if not all(o.exists() for o in kernels):
+        build()
+
+    def compile_host(src: Path) -> Path:
+        obj = adir / (src.stem + "_asan.o")
+        cmd = [HIPCC, *COMMON, *ASAN_HOST, "-c", str(src), "-o", str(obj)]
+        if src.parent == CSRC:
+            cmd.insert(1, "-xhip")
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"hipcc (host ASan) failed for {src.name}:\n{res.stderr}")
+        return obj
+
+    srcs = sorted(CSRC.glob("*.cpp")) + [ASAN_DRIVER_SRC]
+    with _cf.ThreadPoolExecutor(max_workers=min(len(srcs), 8)) as ex:
+        objs = list(ex.map(compile_host, srcs))
+    tmp = ASAN_DRIVER.with_suffix(".tmp")
+    cmd = [HIPCC, f"--offload-arch={ARCH}", *ASAN_HOST, "-o", str(tmp), *map(str, objs), *map(str, kernels)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"ASan driver link failed:\n{res.stderr}")
+    os.replace(tmp, ASAN_DRIVER)
+    if verbose:
+        print(f"built {ASAN_DRIVER}")
+    return ASAN_DRIVER
+
+
 def build_oracle(verbose: bool = False) -> Path:
     """Compile the CPU oracle (test infrastructure) with its Makefile."""
     odir = PKG.parent / "oracle"
@@ -81,3 +122,5 @@ if __name__ == "__main__":
     build(verbose=True)
     if "--oracle" in sys.argv:
         build_oracle(verbose=True)
+    if "--asan" in sys.argv:
+        build_asan(verbose=True)

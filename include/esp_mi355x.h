@@ -74,8 +74,9 @@ typedef struct esp_params {
     double k_cut;                  /* spherical reciprocal-space cutoff: the influence function is 0
                                       for |k| > k_cut.  0 = automatic (1.2 kappa for the PSWF split,
                                       none for the Gaussian split); < 0 = none */
-    unsigned int tile;             /* spreading/gathering tile edge in mesh cells, 2 to 16 (FP64
-                                      column-ownership spreading: 2, 4 or 8); 0 = automatic */
+    unsigned int tile;             /* spreading/gathering tile edge in mesh cells, 2 to 16, whose
+                                      (tile + order - 1)^3 LDS box fits 64 KiB (8-byte cells in
+                                      FP64, 4-byte in mixed); 0 = automatic */
 } esp_params;
 
 /* One system: device pointers owned by the caller (HOOMD's ParticleData, NeighborList and
@@ -248,6 +249,7 @@ int esp_build_split_table(int split, double kappa, double rcut, unsigned int n_s
 int esp_split_eval(int split, double kappa, double rcut, int n, const double* s, double* S, double* dSds);
 int esp_split_fourier(int split, double kappa, double rcut, int n, const double* k, double* F, double* dF);
 int esp_split_constants(int split, double kappa, double rcut, double* self_coeff, double* background_coeff);
+/* prolate psi_0^c at n points: psi[i] = psi_0^c(x[i]), integral[i] = int_0^x[i] psi_0^c(t) dt */
 int esp_prolate_eval(double c, int n, const double* x, double* psi, double* integral);
 int esp_build_window(unsigned int P, unsigned int D, double window_c, double* coef);
 int esp_window_fourier(unsigned int P, unsigned int D, const double* coef, int n, const double* theta, double* what);
