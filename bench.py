@@ -56,6 +56,11 @@ HBM_PEAK_GBS = 8000.0
 TARGET = 1e-5
 
 
+def progress(msg):
+    """One line per finished piece of work on stderr (the JSON result goes to stdout)."""
+    print(f"bench.py: {msg}", file=sys.stderr, flush=True)
+
+
 def kernel_bytes(stage, N, n_charged, n_pairs, M, Mh, R, want_ev=True):
     """Algorithmic HBM bytes of one launch of each stage (reads + writes of its operands,
     counted once; neighbour-position gathers excluded)."""
@@ -447,6 +452,7 @@ def main(argv=None):
         t0 = time.perf_counter()
         W.equilibrate(st, literal, steps=args.equilibrate)
         eq_s = time.perf_counter() - t0
+        progress(f"equilibrated {args.equilibrate} steps in {eq_s:.1f} s")
         pos, vel = st.positions().copy(), st.velocities().copy()
         del st
 
@@ -470,6 +476,7 @@ def main(argv=None):
         out_legs["literal"] = {"params": {"mesh": 128, "order": 10, "rcut": 3.5, "kappa": literal.kappa,
                                           "c": literal.kappa * 3.5}, **{k: r[k] for k in ("ms_per_step",
                                           "particle_steps_per_s", "error", "stages_ms")}}
+        progress(f"literal leg {r['ms_per_step']:.3f} ms/step")
     tuned = {}
     for name, prec in (("fp64", _lib.ESP_PREC_FP64), ("mixed", _lib.ESP_PREC_MIXED)):
         if name not in legs:
@@ -479,23 +486,28 @@ def main(argv=None):
         tuned[name] = (ch, r)
         out_legs[name] = {"params": ch.describe(), **{k: r[k] for k in ("ms_per_step", "particle_steps_per_s", "error",
                                                                        "stages_ms")}}
+        progress(f"{name} leg {r['ms_per_step']:.3f} ms/step, error {r['error']:.3g}, {ch.describe()}")
     head = "mixed" if "mixed" in tuned else ("fp64" if "fp64" in tuned else None)
     if head is not None and "md" in legs:
         out_legs["md"] = {"precision": head, **md_leg(tuned[head][0].params, pos, vel, q, L, args.md_steps, 20)}
+        progress("md leg done")
     if head is not None and "shuffled" in legs:
         # the same system with its particles in a random order (no spatial locality of the
         # neighbour records; HOOMD itself keeps particles sorted along a space-filling curve)
         perm = np.random.default_rng(11).permutation(len(pos))
         r = run_single(tuned[head][0].params, pos[perm], q[perm], L, args.steps, args.warmup, ref[perm], profile=False)
         out_legs["shuffled"] = {"precision": head, **{k: r[k] for k in ("ms_per_step", "particle_steps_per_s", "error")}}
+        progress(f"shuffled leg {r['ms_per_step']:.3f} ms/step")
     if "batched" in legs:
         out_legs["batched"] = batched_leg(args.batch_systems, max(args.steps // 5, 5), 3)
+        progress("batched leg done")
     if "served" in legs:
         # offered rate ~70 % of the batched capacity for the mean request (E[N] E[steps])
         cap_ps = out_legs.get("batched", {}).get("particle_steps_per_s", 3e8)
         mean_work = (2e5 - 1e3) / math.log(200) * (1000 - 10) / math.log(100)
         rate = args.serve_rate if args.serve_rate > 0 else 0.7 * cap_ps / mean_work
         out_legs["served"] = served_leg(args.serve_seconds, rate=rate)
+        progress("served leg done")
 
     if head is not None:
         ch, r = tuned[head]

@@ -22,15 +22,22 @@ the 1M-particle workload (P = 6-12, meshes 128^3-216^3, r_c 4-8): the pair pass 
 mesh cell `cell_ps` for the FFT passes plus `halo_ps` times the tile-box volume ratio
 ((T + P - 1) / T)^3 for the halos of the spreading tiles.  The pair pass runs beside the
 mesh chain, which hides min(hide_pair * pair, hide_mesh * mesh) of it.  The mesh-chain fit
-is within 7% (mixed) and 9% (FP64) RMS, the full step within about 5% near the optimum.  The tuner picks, over FFT-friendly meshes and the allowed orders, the
-smallest r_c that meets the target (minimising sigma * c at fixed h) and then the cheapest
-candidate.  `tune_measured` times the best few candidates on the GPU instead of trusting
-the model, and `verify` measures the chosen configuration's error on a proxy system
-against the tight FP64 reference.
+is within 7% (mixed) and 9% (FP64) RMS, the full step within about 5% near the optimum.  A
+simulation also rebuilds its neighbour list (out to r_c + skin) every `rebuild_every` steps
+on average: the cell-list build costs `nlist_us` plus `nlist_ps` per list entry (k_nlist at
+1e6 particles: 3.0 ms for 1.78e8 entries), and its share per step is added to the step (the
+1e6-particle NVE run at T = 1, dt = 0.002, skin 0.4 rebuilds every ~19 steps; 0 leaves the
+list out, for force evaluations on a fixed list).  The tuner picks, over FFT-friendly meshes
+and the allowed orders, the smallest r_c that meets the target (minimising sigma * c at fixed
+h) and then the cheapest candidate.  `tune_measured` times the best few candidates on the GPU
+(the force evaluation and one list build) instead of trusting the model, and `verify`
+measures the chosen configuration's error on a proxy system against the tight FP64
+reference.
 """
 from __future__ import annotations
 
 import math
+import time
 from dataclasses import dataclass
 from typing import List, Optional, Sequence, Tuple
 
@@ -108,6 +115,13 @@ class CostModel:
     hide_mesh: Tuple[float, float] = (0.15, 0.4)      # of the pair pass, and at most this share of itself
     overhead_us: Tuple[float, float] = (120.0, 84.0)  # launch and reduction latency of the mesh chain
     overlap: bool = True                              # False: branches add up (no hiding)
+    nlist_ps: float = 16.9                            # neighbour-list build per list entry (FP64 positions)
+    nlist_us: float = 60.0                            # fixed cost of a build (cell count, scan, fill, order)
+    rebuild_every: float = 19.0                       # steps per list build; 0 = no list upkeep
+
+    def nlist_ms(self, N: int, neighbours: float) -> float:
+        """One neighbour-list build out to r_c + skin (`neighbours` entries per particle), in ms."""
+        return N * neighbours * self.nlist_ps * 1e-9 + self.nlist_us * 1e-3
 
     def branch_ms(self, N: int, n_cells: int, P: int, neighbours: float, precision: int) -> Tuple[float, float]:
         """(mesh chain, pair pass) each run alone, in ms."""
@@ -118,11 +132,19 @@ class CostModel:
         pair = N * neighbours * self.pair_ps[k] * 1e-9 + self.pair_us[k] * 1e-3
         return mesh, pair
 
-    def step_ms(self, N: int, n_cells: int, P: int, neighbours: float, precision: int) -> float:
+    def force_ms(self, N: int, n_cells: int, P: int, neighbours: float, precision: int) -> float:
+        """One force evaluation (pair pass beside the mesh chain), in ms."""
         k = 0 if precision == _lib.ESP_PREC_FP64 else 1
         mesh, pair = self.branch_ms(N, n_cells, P, neighbours, precision)
         hidden = min(self.hide_pair[k] * pair, self.hide_mesh[k] * mesh) if self.overlap else 0.0
         return mesh + pair - hidden
+
+    def step_ms(self, N: int, n_cells: int, P: int, neighbours: float, precision: int) -> float:
+        """A simulation step: the force evaluation plus the amortised neighbour-list build."""
+        ms = self.force_ms(N, n_cells, P, neighbours, precision)
+        if self.rebuild_every > 0:
+            ms += self.nlist_ms(N, neighbours) / self.rebuild_every
+        return ms
 
 
 #: Systems solved inside a batch (many small systems, pair and mesh items of all systems sharing
@@ -130,7 +152,8 @@ class CostModel:
 #: 512-system batch of 500-50,000 particles its choices measured 13.0 ms per batch step against
 #: 15.4 ms for the single-system fit above.
 BATCHED_COST = CostModel(pair_ps=(10.0, 6.5), pair_us=(0.0, 0.0), bin_ns=(0.18, 0.15), stencil_ps=(1.17, 0.62),
-                         cell_ps=(90.0, 70.0), halo_ps=(0.0, 0.0), overhead_us=(25.0, 25.0), overlap=False)
+                         cell_ps=(90.0, 70.0), halo_ps=(0.0, 0.0), overhead_us=(25.0, 25.0), overlap=False,
+                         nlist_us=0.0)
 
 
 @dataclass
@@ -141,16 +164,27 @@ class Choice:
     predicted_error: float
     predicted_ms: float
     neighbours: float
-    measured_ms: Optional[float] = None
+    measured_ms: Optional[float] = None         # force evaluation, timed on the GPU
     measured_error: Optional[float] = None
     method_scale: float = 1.0
+    measured_list_ms: Optional[float] = None    # one neighbour-list build, timed on the GPU
+    rebuild_every: float = 0.0                  # steps per list build the ranking assumed
+
+    def ranked_ms(self) -> float:
+        """Measured per-step cost used for ranking: force evaluation plus the amortised list
+        build."""
+        ms = self.measured_ms if self.measured_ms is not None else self.predicted_ms
+        if self.measured_list_ms is not None and self.rebuild_every > 0:
+            ms += self.measured_list_ms / self.rebuild_every
+        return ms
 
     def describe(self) -> dict:
         p = self.params
         return {"mesh": [int(p.nx), int(p.ny), int(p.nz)], "order": int(p.order), "rcut": round(float(p.rcut), 4),
                 "kappa": round(float(p.kappa), 5),
                 "c": round(self.c, 3), "sigma": round(self.sigma, 3), "predicted_error": self.predicted_error,
-                "predicted_ms": self.predicted_ms, "measured_ms": self.measured_ms, "measured_error": self.measured_error}
+                "predicted_ms": self.predicted_ms, "measured_ms": self.measured_ms, "measured_error": self.measured_error,
+                "measured_list_ms": self.measured_list_ms, "rebuild_every": self.rebuild_every}
 
 
 def fft_sizes(lo: int, hi: int) -> List[int]:
@@ -239,7 +273,8 @@ def tune(N: int, box: Sequence[float], target: float = 1e-5, precision: int = _l
             rc = float(rc)
             p = Params(nx=mesh[0], ny=mesh[1], nz=mesh[2], order=P, kappa=c / rc, rcut=rc, precision=precision,
                        tile=batch_tile(P, precision) if batched else 0)
-            cands.append(Choice(p, c, sigma, predicted_error(c, sigma, P, precision, method_scale), ms, nbar))
+            cands.append(Choice(p, c, sigma, predicted_error(c, sigma, P, precision, method_scale), ms, nbar,
+                                rebuild_every=cost.rebuild_every))
     if not cands:
         raise ValueError(f"no ESP configuration reaches {target:g} for N={N}, box={tuple(L)} within the mesh range")
     cands.sort(key=lambda ch: ch.predicted_ms)
@@ -248,13 +283,19 @@ def tune(N: int, box: Sequence[float], target: float = 1e-5, precision: int = _l
 
 def measure_step_ms(choice: Choice, positions: np.ndarray, charges: np.ndarray, box: Sequence[float],
                     steps: int = 10, skin: float = 0.4) -> float:
-    """Average time of one ESP step for this choice on the GPU (HIP events)."""
-    from .device import Event, Stream
+    """Average time of one ESP force evaluation for this choice on the GPU (HIP events); also
+    times one neighbour-list build into choice.measured_list_ms."""
+    from .device import Event, Stream, synchronize
     from .md import NeighborList, State
     from .solver import Solver
     st = State(positions, charges, list(box[:3]))
     nl = NeighborList(buffer=skin)
+    nl.update(st, choice.params.rcut, force_rebuild=True)  # sizes the list
+    synchronize()
+    t0 = time.perf_counter()
     nl.update(st, choice.params.rcut, force_rebuild=True)
+    synchronize()
+    choice.measured_list_ms = (time.perf_counter() - t0) * 1e3
     solver = Solver(choice.params)
     stream = Stream()
     for _ in range(2):
@@ -274,7 +315,7 @@ def tune_measured(positions: np.ndarray, charges: np.ndarray, box: Sequence[floa
     cands = tune(len(positions), box, target, precision, top=top, **kw)
     for ch in cands:
         ch.measured_ms = measure_step_ms(ch, positions, charges, box)
-    cands.sort(key=lambda ch: ch.measured_ms)
+    cands.sort(key=lambda ch: ch.ranked_ms())
     return cands
 
 
@@ -299,7 +340,7 @@ def autotune(positions: np.ndarray, charges: np.ndarray, box: Sequence[float], t
         if measure_time:
             for ch in cands:
                 ch.measured_ms = measure_step_ms(ch, positions, charges, box)
-            cands.sort(key=lambda ch: ch.measured_ms)
+            cands.sort(key=lambda ch: ch.ranked_ms())
         best = cands[0]
         best.measured_error = rel_rms_error(forces_with(best.params, positions, charges, box), ref)
         best.method_scale = scale

@@ -75,3 +75,23 @@ def test_reference_params_are_tight():
     assert tune.predicted_error(20.0, sigma, 12, _lib.ESP_PREC_FP64) < 4e-9
     small = reference_params((10.0, 11.0, 12.0))
     assert small.rcut <= 4.5 and min(small.nx, small.ny, small.nz) >= 24
+
+
+def test_list_upkeep_enters_the_step_and_shortens_the_cutoff():
+    """The amortised neighbour-list build is part of a simulation step: step = force + build /
+    rebuild_every, and charging it never makes the tuner pick a longer cutoff (the list grows
+    as (r_c + skin)^3; the mesh pays for a shorter one)."""
+    N, L = 1_000_000, (1_000_000 / 0.1) ** (1 / 3)
+    with_list = tune.CostModel()
+    no_list = tune.CostModel(rebuild_every=0.0)
+    f = no_list.step_ms(N, 144 ** 3, 10, 170.0, _lib.ESP_PREC_MIXED)
+    assert f == pytest.approx(with_list.force_ms(N, 144 ** 3, 10, 170.0, _lib.ESP_PREC_MIXED))
+    s = with_list.step_ms(N, 144 ** 3, 10, 170.0, _lib.ESP_PREC_MIXED)
+    assert s == pytest.approx(f + with_list.nlist_ms(N, 170.0) / with_list.rebuild_every)
+    # 1e6 particles, 178 entries: about 3 ms per build, as measured
+    assert with_list.nlist_ms(N, 178.0) == pytest.approx(3.07, rel=0.05)
+    for prec in (_lib.ESP_PREC_FP64, _lib.ESP_PREC_MIXED):
+        a = tune.tune(N, (L, L, L), 1e-5, prec, cost=with_list)[0]
+        b = tune.tune(N, (L, L, L), 1e-5, prec, cost=no_list)[0]
+        assert a.params.rcut <= b.params.rcut + 1e-12
+        print(prec, a.describe(), b.describe())
