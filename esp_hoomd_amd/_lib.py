@@ -149,6 +149,8 @@ def load() -> C.CDLL:
                                                      C.POINTER(EspOut), _VP])
     _sig(lib, "esp_get_window", C.c_int, [_VP, C.POINTER(C.c_uint), C.POINTER(C.c_uint), _DP])
     _sig(lib, "esp_set_profiling", C.c_int, [_VP, C.c_int])
+    _sig(lib, "esp_set_graph", C.c_int, [_VP, C.c_int])
+    _sig(lib, "esp_get_graph_stats", C.c_int, [_VP, C.POINTER(C.c_ulonglong), C.POINTER(C.c_ulonglong)])
     _sig(lib, "esp_get_stage_times", C.c_int, [_VP, _DP, C.POINTER(C.c_uint)])
     _sig(lib, "esp_reset_stage_times", C.c_int, [_VP])
     _sig(lib, "esp_stage_name", C.c_char_p, [C.c_int])

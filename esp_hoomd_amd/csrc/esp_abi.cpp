@@ -484,6 +484,21 @@ int esp_get_window(esp_ctx* ctx, unsigned int* P, unsigned int* D, double* coef)
     }
 
 // ---- tracing ------------------------------------------------------------------------------
+int esp_set_graph(esp_ctx* ctx, int enable)
+    {
+    if (!ctx || enable < 0 || enable > 1)
+        return ESP_ERR_ARG;
+    return guarded(ctx, [&] { ctx->solver.set_graph(enable != 0); });
+    }
+
+int esp_get_graph_stats(esp_ctx* ctx, unsigned long long* captures, unsigned long long* replays)
+    {
+    if (!ctx || !captures || !replays)
+        return ESP_ERR_ARG;
+    ctx->solver.graph_stats(*captures, *replays);
+    return ESP_OK;
+    }
+
 int esp_set_profiling(esp_ctx* ctx, int enable)
     {
     if (!ctx)

@@ -168,6 +168,9 @@ EspSolver::EspSolver() = default;
 
 EspSolver::~EspSolver()
     {
+    release_graph();
+    if (m_gstream)
+        (void)hipStreamDestroy(m_gstream);
     for (auto& v : m_ev_pool)
         for (hipEvent_t e : v)
             (void)hipEventDestroy(e);
@@ -193,6 +196,7 @@ const char* EspSolver::stage_name(int i)
 
 void EspSolver::set_profiling(int level)
     {
+    ++m_gen;
     m_profile = (level >= 0 && level <= 2) ? level : 1;
     }
 
@@ -291,6 +295,7 @@ void EspSolver::stage_times(double* ms, unsigned int* calls)
 
 void EspSolver::set_params(const SolverParams& p)
     {
+    ++m_gen;
     if (p.nx < 2 || p.ny < 2 || p.nz < 2)
         throw EspError(STATUS_ARG, "mesh resolution must be >= 2 along each axis");
     if (p.nx > 1024 || p.ny > 1024 || p.nz > 1024)
@@ -332,6 +337,7 @@ void EspSolver::set_params(const SolverParams& p)
 
 void EspSolver::set_split_table(unsigned int n_segs, const double* screen, const double* dscreen, const double* r_lo, const double* dr_inv)
     {
+    ++m_gen;
     if (n_segs < 1 || !screen || !dscreen || !r_lo || !dr_inv)
         throw EspError(STATUS_ARG, "set_split_table: invalid table");
     m_screen.assign(screen, screen + size_t(n_segs) * 5);
@@ -554,6 +560,7 @@ void EspSolver::initialize(const SystemView& sys, hipStream_t s)
 
     build_influence(s);
     m_need_init = false;
+    ++m_gen;
     }
 
 double EspSolver::effective_k_cut() const
@@ -618,6 +625,7 @@ void EspSolver::ensure_particles(unsigned int n)
         return;
     const unsigned int cap = std::max(n, m_cap_particles + m_cap_particles / 2);
     const bool fp64 = m_p.precision == 0;
+    ++m_gen;
     m_key.resize(size_t(cap) * 4);
     m_slot.resize(size_t(cap) * 4);
     m_rec.resize(size_t(cap) * 4 * (fp64 ? 8 : 4));
@@ -787,18 +795,101 @@ unsigned int EspSolver::prepare(const SystemView& sys, const OutputView& out, hi
             m_geom.hy = m_geom.Ly / m_geom.ny;
             m_geom.hz = m_geom.Lz / m_geom.nz;
             build_influence(s);
+            ++m_gen;
             }
         }
     const unsigned int n = sys.group_idx ? sys.group_size : sys.N;
     ensure_particles(std::max(n, 1u));
     if (m_pq.bytes() < size_t(std::max(sys.N, 1u)) * 32)
+        {
         m_pq.resize(size_t(std::max(sys.N, 1u)) * 32 + m_pq.bytes() / 2);
+        ++m_gen;
+        }
     return n;
+    }
+
+void EspSolver::set_graph(bool on)
+    {
+    m_graph_on = on;
+    if (!on)
+        release_graph();
+    }
+
+void EspSolver::release_graph()
+    {
+    if (m_gexec)
+        (void)hipGraphExecDestroy(m_gexec);
+    m_gexec = nullptr;
+    m_gkey.clear();
+    }
+
+//! Everything a captured step depends on besides device memory contents.
+std::vector<uint64_t> EspSolver::graph_key(const SystemView& sys, const OutputView& out, hipStream_t s) const
+    {
+    auto u = [](const void* p) { return uint64_t(reinterpret_cast<uintptr_t>(p)); };
+    std::vector<uint64_t> k = {m_gen, uint64_t(m_overlap), u(s), sys.N, sys.group_size, u(sys.postype), u(sys.charge),
+                               u(sys.group_idx), u(sys.n_neigh), u(sys.nlist), u(sys.head_list), u(sys.n_ex), u(sys.ex_list),
+                               sys.ex_pitch, u(out.force), u(out.virial), uint64_t(out.virial_pitch), u(out.ext_energy_virial),
+                               out.flags};
+    for (int a = 0; a < 6; ++a)
+        {
+        uint64_t b;
+        std::memcpy(&b, &sys.box[a], 8);
+        k.push_back(b);
+        }
+    return k;
     }
 
 void EspSolver::compute(const SystemView& sys, const OutputView& out, hipStream_t s)
     {
     const unsigned int n = prepare(sys, out, s);
+    const unsigned int flags = out.flags ? out.flags : COMPUTE_ALL;
+    // graphs hold no memset nodes: the mesh-only path (which clears the outputs) stays eager
+    if (!m_graph_on || m_profile || !(flags & COMPUTE_PAIR))
+        {
+        enqueue_step(sys, out, n, s);
+        return;
+        }
+    std::vector<uint64_t> key = graph_key(sys, out, s);
+    if (!m_gexec || key != m_gkey)
+        {
+        release_graph();
+        if (!m_gstream)
+            check_hip(hipStreamCreateWithFlags(&m_gstream, hipStreamNonBlocking), "hipStreamCreate");
+        // side-stream objects are created before the capture begins
+        if (m_overlap && (flags & COMPUTE_MESH) && !m_side)
+            {
+            check_hip(hipStreamCreateWithFlags(&m_side, hipStreamNonBlocking), "hipStreamCreate");
+            check_hip(hipEventCreateWithFlags(&m_ev_fork, hipEventDisableTiming), "hipEventCreate");
+            check_hip(hipEventCreateWithFlags(&m_ev_join, hipEventDisableTiming), "hipEventCreate");
+            check_hip(hipEventCreateWithFlags(&m_ev_zdone, hipEventDisableTiming), "hipEventCreate");
+            }
+        check_hip(hipStreamBeginCapture(m_gstream, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
+        hipGraph_t g = nullptr;
+        try
+            {
+            enqueue_step(sys, out, n, m_gstream);
+            }
+        catch (...)
+            {
+            (void)hipStreamEndCapture(m_gstream, &g);
+            if (g)
+                (void)hipGraphDestroy(g);
+            throw;
+            }
+        check_hip(hipStreamEndCapture(m_gstream, &g), "hipStreamEndCapture");
+        const hipError_t e = hipGraphInstantiate(&m_gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        check_hip(e, "hipGraphInstantiate");
+        m_gkey = std::move(key);
+        ++m_gcaptures;
+        }
+    check_hip(hipGraphLaunch(m_gexec, s), "hipGraphLaunch");
+    ++m_greplays;
+    }
+
+void EspSolver::enqueue_step(const SystemView& sys, const OutputView& out, unsigned int n, hipStream_t s)
+    {
     const bool fp64 = m_p.precision == 0;
     const unsigned int flags = out.flags ? out.flags : COMPUTE_ALL;
     mark(0, s);

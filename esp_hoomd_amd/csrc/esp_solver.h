@@ -125,7 +125,24 @@ class EspSolver
 
     void set_params(const SolverParams& p);
     const SolverParams& params() const { return m_p; }
-    void invalidate() { m_need_init = true; }
+    void invalidate()
+        {
+        m_need_init = true;
+        ++m_gen;
+        }
+
+    //! Step graphs: compute() captures its whole step (the pair branch on the side stream and
+    //! the mesh chain) into a hipGraph once and replays it while the call's arguments (system
+    //! and output pointers, sizes, box, flags, stream) and the context's buffers stay the same;
+    //! any change captures again.  The replayed kernels read the same device memory, so new
+    //! positions or charges in the same arrays are picked up.  Off by default; stage profiling
+    //! uses the eager path.
+    void set_graph(bool on);
+    void graph_stats(unsigned long long& captures, unsigned long long& replays) const
+        {
+        captures = m_gcaptures;
+        replays = m_greplays;
+        }
 
     //! Full ESP evaluation: pair correction + mesh; outputs overwritten.
     void compute(const SystemView& sys, const OutputView& out, hipStream_t s);
@@ -179,6 +196,17 @@ class EspSolver
     void reset_stage_times();
 
     private:
+    //! The step after prepare(): pair branch (side stream) and mesh chain on s.
+    void enqueue_step(const SystemView& sys, const OutputView& out, unsigned int n, hipStream_t s);
+    std::vector<uint64_t> graph_key(const SystemView& sys, const OutputView& out, hipStream_t s) const;
+    void release_graph();
+    // step-graph replay (set_graph)
+    bool m_graph_on = false;
+    hipStream_t m_gstream = nullptr;    // capture stream
+    hipGraphExec_t m_gexec = nullptr;
+    std::vector<uint64_t> m_gkey;
+    uint64_t m_gen = 0;                 // bumped whenever a buffer, table or parameter changes
+    unsigned long long m_gcaptures = 0, m_greplays = 0;
     void mark(int boundary, hipStream_t s);
     void pair_mark(bool end, hipStream_t s);
     void harvest();

@@ -185,6 +185,17 @@ class Solver:
         _lib.check(_lib.load().esp_device_synchronize())
         return {"force": system.force.numpy(), "virial": system.virial.numpy()[:, : system.N]}
 
+    def set_graph(self, on: bool = True) -> None:
+        """Replay the whole step as a captured hipGraph while the call's arguments stay the same
+        (see esp_set_graph)."""
+        _lib.check(_lib.load().esp_set_graph(self._ctx, 1 if on else 0), self._ctx)
+
+    def graph_stats(self):
+        """(captures, replays) of the step graph."""
+        c, r = C.c_ulonglong(), C.c_ulonglong()
+        _lib.check(_lib.load().esp_get_graph_stats(self._ctx, C.byref(c), C.byref(r)), self._ctx)
+        return int(c.value), int(r.value)
+
     def set_profiling(self, on=True) -> None:
         """True / 1: HIP-event times of every stage; 2: of the pair stage only; False / 0: off."""
         level = 1 if on is True else (0 if on is False else int(on))

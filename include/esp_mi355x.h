@@ -148,6 +148,15 @@ const char* esp_last_error(const esp_ctx* ctx);
  * (EvaluatorPairPSWF over the neighbour list, EvaluatorPairPSWF.h:274-295) + exclusion
  * correction (ESPForceComputeGPU.cu:391-471) + mesh part (.cu:206-373) + energy/virial. */
 int esp_compute_forces(esp_ctx* ctx, uint64_t timestep, const esp_system* sys, esp_out* out, esp_stream_t stream);
+/* Step graphs (no counterpart in the reference, which launches every kernel per step): with
+ * enable = 1, esp_compute_forces captures its whole step (pair branch and mesh chain) into a
+ * hipGraph on the first call and replays it on later calls whose arguments (system and output
+ * pointers, sizes, box, flags, stream) and parameters are unchanged, cutting the per-kernel
+ * launch gaps; any change captures again.  New contents of the same arrays (positions,
+ * charges, neighbour lists) are picked up.  Mesh-only calls (flags without ESP_COMPUTE_PAIR)
+ * and profiled calls stay eager.  Default 0. */
+int esp_set_graph(esp_ctx* ctx, int enable);
+int esp_get_graph_stats(esp_ctx* ctx, unsigned long long* captures, unsigned long long* replays);
 /* the PotentialPair<EvaluatorPairPSWF> path alone (module.cc:73,79): pair + exclusions */
 int esp_pair_forces(esp_ctx* ctx, const esp_system* sys, esp_out* out, esp_stream_t stream);
 
