@@ -303,7 +303,8 @@ def batched_leg(n_sys, steps, warmup, seed=6):
     scale = 1.0
     for i in pilots:
         pos, q, L = systems[i]
-        ch = tune.autotune(pos, q, (L, L, L), TARGET, _lib.ESP_PREC_MIXED, orders=(6, 8, 10), mesh_range=(16, 64))
+        ch = tune.autotune(pos, q, (L, L, L), TARGET, _lib.ESP_PREC_MIXED, orders=(6, 8, 10), mesh_range=(16, 64),
+                           cost=tune.CostModel(rebuild_every=0.0))
         scale = max(scale, ch.method_scale)
     t0 = time.perf_counter()
     pairs, states, choices = [], [], []

@@ -64,13 +64,15 @@ hipError_t batch_max_disp(const MDSys* d, const MDItem* items, unsigned int n_it
 
 hipError_t cell_count(unsigned int N, const double* pos, const CellGeom& cg, unsigned int* count, unsigned int* cell, unsigned int* slot,
                       hipStream_t s);
-//! Cell member lists in ascending particle index (`unsorted`: N entries of scratch).
-hipError_t cell_fill(unsigned int N, const unsigned int* cell, const unsigned int* slot, const unsigned int* start, unsigned int* unsorted,
-                     unsigned int* members, hipStream_t s);
-hipError_t build_nlist(unsigned int N, const double* pos, const CellGeom& cg, const unsigned int* start, const unsigned int* members,
-                       double r_list, const unsigned int* n_ex, const unsigned int* ex_list, unsigned int ex_pitch, unsigned int nmax,
-                       unsigned int* n_neigh, unsigned int* nlist, uint64_t* head, unsigned int* overflow, const InnerList& inner,
-                       hipStream_t s);
+//! Cell member lists in ascending particle index (`unsorted`: N entries of scratch) and the
+//! members' positions wrapped into the box in the same order (`spos`: N double4).
+hipError_t cell_fill(unsigned int N, const double* pos, const CellGeom& cg, const unsigned int* cell, const unsigned int* slot,
+                     const unsigned int* start, unsigned int* unsorted, unsigned int* members, double* spos, hipStream_t s);
+//! Full (and optional inner) neighbour lists from the cell-sorted members and positions.
+hipError_t build_nlist(unsigned int N, const double* spos, const CellGeom& cg, const unsigned int* cell, const unsigned int* start,
+                       const unsigned int* members, double r_list, const unsigned int* n_ex, const unsigned int* ex_list,
+                       unsigned int ex_pitch, unsigned int nmax, unsigned int* n_neigh, unsigned int* nlist, uint64_t* head,
+                       unsigned int* overflow, const InnerList& inner, hipStream_t s);
 hipError_t wca_forces(unsigned int N, const double* pos, double Lx, double Ly, double Lz, const unsigned int* n_neigh,
                       const unsigned int* nlist, const uint64_t* head, double sigma, double epsilon, double* force, double* virial,
                       size_t pitch, hipStream_t s);

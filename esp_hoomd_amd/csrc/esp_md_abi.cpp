@@ -27,7 +27,7 @@ struct esp_nlist
     double box[3] = {0, 0, 0};
     bool built = false;
     unsigned long long builds = 0;
-    esp::DeviceBuffer n_neigh, nlist, head, cell, slot, count, start, unsorted, members, overflow, last_pos, disp;
+    esp::DeviceBuffer n_neigh, nlist, head, cell, slot, count, start, unsorted, members, spos, overflow, last_pos, disp;
     esp::DeviceBuffer n_inner, inner, head_inner;
     unsigned int* h_over = nullptr;  // pinned: overflow counts of the last build (full, inner)
     esp::md::CellGeom cg {};
@@ -103,6 +103,7 @@ void build_prepare(esp_nlist* nl, const esp_system* sys)
         nl->slot.resize(size_t(std::max(N, 1u)) * 4);
         nl->members.resize(size_t(std::max(N, 1u)) * 4);
         nl->unsorted.resize(size_t(std::max(N, 1u)) * 4);
+        nl->spos.resize(size_t(std::max(N, 1u)) * 32);
         nl->last_pos.resize(size_t(std::max(N, 1u)) * 32);
         nl->count.resize(size_t(nc + 1) * 4, true);
         nl->start.resize(size_t(nc + 1) * 4, true);
@@ -149,10 +150,12 @@ void build_enqueue(esp_nlist* nl, const esp_system* sys, hipStream_t s)
     check_hip(esp::kernel::scan_tiles(nc, nl->count.as<unsigned int>(), nl->start.as<unsigned int>(), 0, nullptr,
                                       esp::FixedPointRule {}, nullptr, s),
               "scan");
-    check_hip(esp::md::cell_fill(N, nl->cell.as<unsigned int>(), nl->slot.as<unsigned int>(), nl->start.as<unsigned int>(),
-                                 nl->unsorted.as<unsigned int>(), nl->members.as<unsigned int>(), s),
+    check_hip(esp::md::cell_fill(N, sys->postype, nl->cg, nl->cell.as<unsigned int>(), nl->slot.as<unsigned int>(),
+                                 nl->start.as<unsigned int>(), nl->unsorted.as<unsigned int>(), nl->members.as<unsigned int>(),
+                                 nl->spos.as<double>(), s),
               "cell_fill");
-    check_hip(esp::md::build_nlist(N, sys->postype, nl->cg, nl->start.as<unsigned int>(), nl->members.as<unsigned int>(), r_list,
+    check_hip(esp::md::build_nlist(N, nl->spos.as<double>(), nl->cg, nl->cell.as<unsigned int>(), nl->start.as<unsigned int>(),
+                                   nl->members.as<unsigned int>(), r_list,
                                    sys->n_ex, sys->ex_list, sys->ex_pitch, nl->nmax, nl->n_neigh.as<unsigned int>(),
                                    nl->nlist.as<unsigned int>(), nl->head.as<uint64_t>(), nl->overflow.as<unsigned int>(), in, s),
               "build_nlist");

@@ -47,12 +47,23 @@ __global__ void __launch_bounds__(256) k_cell_fill(unsigned int N, const unsigne
     members[start[cell[i]] + slot[i]] = i;
     }
 
+//! Wrap a coordinate into [-L/2, L/2) the way cell_coord does.
+__device__ __forceinline__ double wrap_into_box(double x, double L)
+    {
+    double f = x / L + 0.5;
+    f -= floor(f);
+    return (f - 0.5) * L;
+    }
+
 //! Cell members in ascending particle index: particle i's rank in its cell is the number of
 //! members below i.  The neighbour rows then list candidates in an order fixed by the
-//! positions alone, so pair sums over them are reproducible bit for bit.
-__global__ void __launch_bounds__(256) k_cell_order(unsigned int N, const unsigned int* __restrict__ cell,
-                                                    const unsigned int* __restrict__ start, const unsigned int* __restrict__ unsorted,
-                                                    unsigned int* __restrict__ members)
+//! positions alone, so pair sums over them are reproducible bit for bit.  Each member's
+//! position, wrapped into the box, is stored in the same slot (`spos`), so the list pass
+//! reads every cell's candidates as one contiguous run.
+__global__ void __launch_bounds__(256) k_cell_order(unsigned int N, const double4* __restrict__ pos, md::CellGeom cg,
+                                                    const unsigned int* __restrict__ cell, const unsigned int* __restrict__ start,
+                                                    const unsigned int* __restrict__ unsorted, unsigned int* __restrict__ members,
+                                                    double4* __restrict__ spos)
     {
     const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N)
@@ -63,22 +74,27 @@ __global__ void __launch_bounds__(256) k_cell_order(unsigned int N, const unsign
     for (unsigned int m = m0; m < m1; ++m)
         rank += unsorted[m] < i ? 1u : 0u;
     members[m0 + rank] = i;
+    const double4 r = pos[i];
+    spos[m0 + rank] = make_double4(wrap_into_box(r.x, cg.Lx), wrap_into_box(r.y, cg.Ly), wrap_into_box(r.z, cg.Lz), r.w);
     }
 
-//! Full neighbour list: for every particle, all others within r_list (minimum image), excluding
-//! the pairs in its exclusion list.  Rows of width nmax; overflow raises *overflow to the
-//! largest count seen so the host can grow nmax and rebuild.
-//! Full neighbour list of particle i (r < r_list, exclusions left out) by the G lanes of its
-//! group: the group walks the 27 neighbour cells together, each lane tests every G-th member
-//! of a cell, and the accepted candidates are compacted with a ballot into consecutive slots of
-//! the particle's row, so stores are contiguous and G candidate loads are in flight.  With
+//! Full neighbour list of one particle (r < r_list, exclusions left out) by the G lanes of its
+//! group.  Groups take particles in cell-sorted order (slot m of the member list), so the
+//! groups of a wave share their 27 neighbour cells and the candidates' positions -- read from
+//! the cell-sorted copy `spos`, one contiguous run per cell -- stay in cache.  Each lane tests
+//! every G-th member of a cell, and the accepted candidates are compacted with a ballot into
+//! consecutive slots of the particle's row (row i, HOOMD layout), in cell-walk order and, within
+//! a cell, in ascending index.  Along an axis with at least 3 cells the periodic image of a
+//! neighbour cell is fixed by the walk (a shift of -L or +L where the cell index wraps), so no
+//! minimum-image rounding is needed per candidate; axes with fewer cells use rint.  With
 //! r_inner2 > 0 the same pass also writes the inner list (the entries with r^2 < r_inner2) to
 //! rows of width nmax_in; overflow[0] / overflow[1] collect the largest full / inner counts
-//! that did not fit.  Every lane of the group must call it; `active` (i < N) is the same for
+//! that did not fit.  Every lane of the group must call it; `active` (m < N) is the same for
 //! the whole group.
 constexpr int NLIST_LANES = 8;
 template<int G>
-__device__ __forceinline__ void nlist_group(unsigned int i, int sub, bool active, const double4* __restrict__ pos, const md::CellGeom& cg,
+__device__ __forceinline__ void nlist_group(unsigned int m, int sub, bool active, const double4* __restrict__ spos,
+                                            const md::CellGeom& cg, const unsigned int* __restrict__ cell,
                                             const unsigned int* __restrict__ start, const unsigned int* __restrict__ members,
                                             double r_list2, const unsigned int* __restrict__ n_ex,
                                             const unsigned int* __restrict__ ex_list, unsigned int ex_pitch, unsigned int nmax,
@@ -87,9 +103,12 @@ __device__ __forceinline__ void nlist_group(unsigned int i, int sub, bool active
     {
     if (!active)
         return;
-    const double4 ri = pos[i];
-    const int cx = cell_coord(ri.x, cg.Lx, cg.nx), cy = cell_coord(ri.y, cg.Ly, cg.ny), cz = cell_coord(ri.z, cg.Lz, cg.nz);
+    const unsigned int i = members[m];
+    const double4 ri = spos[m];
+    const unsigned int ci = cell[i];
+    const int cx = int(ci % unsigned(cg.nx)), cy = int((ci / unsigned(cg.nx)) % unsigned(cg.ny)), cz = int(ci / unsigned(cg.nx * cg.ny));
     const double invLx = 1.0 / cg.Lx, invLy = 1.0 / cg.Ly, invLz = 1.0 / cg.Lz;
+    const bool shx = cg.nx >= 3, shy = cg.ny >= 3, shz = cg.nz >= 3;
     const unsigned int ne = n_ex ? n_ex[i] : 0u;
     const unsigned int lane = __lane_id();
     const unsigned int gshift = lane & ~unsigned(G - 1);         // first lane of the group
@@ -99,31 +118,41 @@ __device__ __forceinline__ void nlist_group(unsigned int i, int sub, bool active
     unsigned int* row_in = in.r2 > 0.0 ? in.list + size_t(i) * in.nmax : nullptr;
     unsigned int k = 0, k_in = 0;
     // neighbour cells, each visited once even when an axis has fewer than 3 cells
-    const int rx = cg.nx >= 3 ? 1 : (cg.nx == 2 ? 1 : 0), ry = cg.ny >= 3 ? 1 : (cg.ny == 2 ? 1 : 0), rz = cg.nz >= 3 ? 1 : (cg.nz == 2 ? 1 : 0);
+    const int rx = cg.nx >= 2 ? 1 : 0, ry = cg.ny >= 2 ? 1 : 0, rz = cg.nz >= 2 ? 1 : 0;
     for (int dz = -rz; dz <= (cg.nz == 2 ? 0 : rz); ++dz)
         for (int dy = -ry; dy <= (cg.ny == 2 ? 0 : ry); ++dy)
             for (int dx = -rx; dx <= (cg.nx == 2 ? 0 : rx); ++dx)
                 {
-                const int x = (cx + dx + cg.nx) % cg.nx, y = (cy + dy + cg.ny) % cg.ny, z = (cz + dz + cg.nz) % cg.nz;
+                const int ux = cx + dx, uy = cy + dy, uz = cz + dz;
+                const int x = (ux + cg.nx) % cg.nx, y = (uy + cg.ny) % cg.ny, z = (uz + cg.nz) % cg.nz;
+                // image of the neighbour cell next to particle i: r_j + s
+                const double sx = shx ? (ux < 0 ? -cg.Lx : (ux >= cg.nx ? cg.Lx : 0.0)) : 0.0;
+                const double sy = shy ? (uy < 0 ? -cg.Ly : (uy >= cg.ny ? cg.Ly : 0.0)) : 0.0;
+                const double sz = shz ? (uz < 0 ? -cg.Lz : (uz >= cg.nz ? cg.Lz : 0.0)) : 0.0;
+                const double bx = ri.x - sx, by = ri.y - sy, bz = ri.z - sz;
                 const unsigned int c = unsigned((z * cg.ny + y) * cg.nx + x);
                 const unsigned int m0 = start[c], m1 = start[c + 1];
-                for (unsigned int m = m0; m < m1; m += unsigned(G))
+                for (unsigned int mm0 = m0; mm0 < m1; mm0 += unsigned(G))
                     {
-                    const unsigned int mm = m + unsigned(sub);
+                    const unsigned int mm = mm0 + unsigned(sub);
                     unsigned int j = 0;
                     bool take = false;
                     double r2 = 0.0;
-                    if (mm < m1)
+                    if (mm < m1 && mm != m)
                         {
-                        j = members[mm];
-                        if (j != i)
+                        const double4 rj = spos[mm];
+                        double ddx = bx - rj.x, ddy = by - rj.y, ddz = bz - rj.z;
+                        if (!shx)
+                            ddx -= cg.Lx * rint(ddx * invLx);
+                        if (!shy)
+                            ddy -= cg.Ly * rint(ddy * invLy);
+                        if (!shz)
+                            ddz -= cg.Lz * rint(ddz * invLz);
+                        r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                        take = r2 < r_list2;
+                        if (take)
                             {
-                            const double4 rj = pos[j];
-                            const double ddx = wrap_min_image(ri.x - rj.x, cg.Lx, invLx);
-                            const double ddy = wrap_min_image(ri.y - rj.y, cg.Ly, invLy);
-                            const double ddz = wrap_min_image(ri.z - rj.z, cg.Lz, invLz);
-                            r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                            take = r2 < r_list2;
+                            j = members[mm];
                             for (unsigned int e = 0; take && e < ne; ++e)
                                 take = ex_list[size_t(i) * ex_pitch + e] != j;
                             }
@@ -160,16 +189,16 @@ __device__ __forceinline__ void nlist_group(unsigned int i, int sub, bool active
         }
     }
 
-__global__ void __launch_bounds__(256) k_nlist(unsigned int N, const double4* __restrict__ pos, md::CellGeom cg,
-                                               const unsigned int* __restrict__ start, const unsigned int* __restrict__ members,
-                                               double r_list2, const unsigned int* __restrict__ n_ex,
+__global__ void __launch_bounds__(256) k_nlist(unsigned int N, const double4* __restrict__ spos, md::CellGeom cg,
+                                               const unsigned int* __restrict__ cell, const unsigned int* __restrict__ start,
+                                               const unsigned int* __restrict__ members, double r_list2, const unsigned int* __restrict__ n_ex,
                                                const unsigned int* __restrict__ ex_list, unsigned int ex_pitch, unsigned int nmax,
                                                unsigned int* __restrict__ n_neigh, unsigned int* __restrict__ nlist,
                                                uint64_t* __restrict__ head, unsigned int* __restrict__ overflow, md::InnerList in)
     {
     const unsigned int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const unsigned int i = t / NLIST_LANES;
-    nlist_group<NLIST_LANES>(i, int(t % NLIST_LANES), i < N, pos, cg, start, members, r_list2, n_ex, ex_list, ex_pitch, nmax,
+    const unsigned int m = t / NLIST_LANES;
+    nlist_group<NLIST_LANES>(m, int(t % NLIST_LANES), m < N, spos, cg, cell, start, members, r_list2, n_ex, ex_list, ex_pitch, nmax,
                              n_neigh, nlist, head, overflow, in);
     }
 
@@ -517,27 +546,28 @@ hipError_t cell_count(unsigned int N, const double* pos, const CellGeom& cg, uns
     return hipGetLastError();
     }
 
-hipError_t cell_fill(unsigned int N, const unsigned int* cell, const unsigned int* slot, const unsigned int* start, unsigned int* unsorted,
-                     unsigned int* members, hipStream_t s)
+hipError_t cell_fill(unsigned int N, const double* pos, const CellGeom& cg, const unsigned int* cell, const unsigned int* slot,
+                     const unsigned int* start, unsigned int* unsorted, unsigned int* members, double* spos, hipStream_t s)
     {
     if (N == 0)
         return hipSuccess;
     hipLaunchKernelGGL(k_cell_fill, dim3(nblocks(N)), dim3(256), 0, s, N, cell, slot, start, unsorted);
-    hipLaunchKernelGGL(k_cell_order, dim3(nblocks(N)), dim3(256), 0, s, N, cell, start, unsorted, members);
+    hipLaunchKernelGGL(k_cell_order, dim3(nblocks(N)), dim3(256), 0, s, N, reinterpret_cast<const double4*>(pos), cg, cell, start,
+                       unsorted, members, reinterpret_cast<double4*>(spos));
     return hipGetLastError();
     }
 
-hipError_t build_nlist(unsigned int N, const double* pos, const CellGeom& cg, const unsigned int* start, const unsigned int* members,
-                       double r_list, const unsigned int* n_ex, const unsigned int* ex_list, unsigned int ex_pitch, unsigned int nmax,
-                       unsigned int* n_neigh, unsigned int* nlist, uint64_t* head, unsigned int* overflow, const InnerList& inner,
-                       hipStream_t s)
+hipError_t build_nlist(unsigned int N, const double* spos, const CellGeom& cg, const unsigned int* cell, const unsigned int* start,
+                       const unsigned int* members, double r_list, const unsigned int* n_ex, const unsigned int* ex_list,
+                       unsigned int ex_pitch, unsigned int nmax, unsigned int* n_neigh, unsigned int* nlist, uint64_t* head,
+                       unsigned int* overflow, const InnerList& inner, hipStream_t s)
     {
     if (N == 0)
         return hipSuccess;
     if (inner.r2 > 0.0 && (!inner.n || !inner.list || !inner.head || inner.nmax == 0))
         return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_nlist, dim3(nblocks(N * unsigned(dev::NLIST_LANES))), dim3(256), 0, s, N, reinterpret_cast<const double4*>(pos), cg,
-                       start, members, r_list * r_list, n_ex, ex_list, ex_pitch, nmax, n_neigh, nlist, head, overflow, inner);
+    hipLaunchKernelGGL(k_nlist, dim3(nblocks(N * unsigned(dev::NLIST_LANES))), dim3(256), 0, s, N, reinterpret_cast<const double4*>(spos),
+                       cg, cell, start, members, r_list * r_list, n_ex, ex_list, ex_pitch, nmax, n_neigh, nlist, head, overflow, inner);
     return hipGetLastError();
     }
 

@@ -150,10 +150,10 @@ class CostModel:
 #: Systems solved inside a batch (many small systems, pair and mesh items of all systems sharing
 #: the GPU) are tuned with an additive model that gives the pair pass no overlap credit: on the
 #: 512-system batch of 500-50,000 particles its choices measured 13.0 ms per batch step against
-#: 15.4 ms for the single-system fit above.
+#: 15.4 ms for the single-system fit above.  It prices the batched solve alone (no list upkeep).
 BATCHED_COST = CostModel(pair_ps=(10.0, 6.5), pair_us=(0.0, 0.0), bin_ns=(0.18, 0.15), stencil_ps=(1.17, 0.62),
                          cell_ps=(90.0, 70.0), halo_ps=(0.0, 0.0), overhead_us=(25.0, 25.0), overlap=False,
-                         nlist_us=0.0)
+                         rebuild_every=0.0)
 
 
 @dataclass

@@ -224,7 +224,8 @@ def test_batch_sample_mixed_against_ewald_and_fp64_against_oracle():
     scale = 1.0
     for k in (0, len(systems) // 2, len(systems) - 1):
         pos, q, L = systems[k]
-        ch = tune.autotune(pos, q, (L, L, L), 1e-5, MIXED, orders=(6, 8, 10), mesh_range=(16, 64))
+        ch = tune.autotune(pos, q, (L, L, L), 1e-5, MIXED, orders=(6, 8, 10), mesh_range=(16, 64),
+                           cost=tune.CostModel(rebuild_every=0.0))
         scale = max(scale, ch.method_scale)
     chosen = [tune.tune(len(pos), (L, L, L), 1e-5, MIXED, orders=(6, 8, 10), mesh_range=(16, 64), method_scale=scale,
                         safety=1.1, batched=True)[0] for pos, q, L in systems]
