@@ -228,7 +228,7 @@ def run_single(params, pos, q, L, steps, warmup, ref, profile=True, skin=0.4):
     pair_timed = timed.get("pair", (0.0, 0))
     return {"ms_per_step": ms, "particle_steps_per_s": len(pos) / (ms * 1e-3), "error": err,
             "stages_ms": avg, "pair_ms_timed": pair_timed[0] / pair_timed[1] if pair_timed[1] else 0.0,
-            "n_pairs": _n_pairs(st)}
+            "n_pairs": _n_pairs(st), "tile": int(solver.info().tile)}
 
 
 def md_leg(params, pos, vel, q, L, steps, warmup, skin=0.4):
@@ -258,6 +258,48 @@ def md_leg(params, pos, vel, q, L, steps, warmup, skin=0.4):
             "temperature": sim.temperature()}
 
 
+# LDS peak: 32 banks x 4 bytes per clock per CU (the CDNA LDS width), 256 CUs at 2.4 GHz
+LDS_PEAK_TBS = 128 * 256 * 2.4e9 / 1e12
+
+
+def measured_lds(key, kernel):
+    """(lds_util, conflict) of `kernel` for workload `key` from the rocprofv3 counter summary
+    profiles/esp_step_N1M_lds_valu_counters.txt (lines 'LDSUTIL <key> <kernel> <util> <conflict>')."""
+    f = ROOT / "profiles" / "esp_step_N1M_lds_valu_counters.txt"
+    if not f.exists():
+        return None, None
+    for line in f.read_text().splitlines():
+        parts = line.split()
+        if len(parts) == 5 and parts[0] == "LDSUTIL" and parts[1] == key and parts[2] == kernel:
+            return float(parts[3]), float(parts[4])
+    return None, None
+
+
+def lds_roofline(res, params, N, n_charged, precision):
+    """LDS side of the spreading and gathering kernels, which are bound by LDS, not HBM: the
+    algorithmic LDS bytes of one launch -- spread: clearing, accumulating into (one atomic per
+    stencil tap: n_charged P^3) and reading out the tile boxes; gather: filling the boxes from
+    the potential mesh and reading the P^3 taps of every particle -- over the stage's average
+    duration, against the LDS peak; beside it the counter-measured LDS utilisation and bank
+    conflicts of the same kernels (rocprofv3, committed summary)."""
+    P, T = params.order, res.get("tile", 8) or 8
+    el = 8 if precision == "fp64" else 4
+    B = T + P - 1
+    tiles = math.ceil(params.nx / T) * math.ceil(params.ny / T) * math.ceil(params.nz / T)
+    box = tiles * B ** 3 * el
+    bytes_ = {"spread": n_charged * P ** 3 * el + 2 * box, "gather": n_charged * P ** 3 * el + box}
+    key = workload_key(N, params, precision)
+    out = {"peak": LDS_PEAK_TBS, "unit": "TB/s", "peak_basis": "128 B/clock/CU x 256 CUs x 2.4 GHz"}
+    for stage in ("spread", "gather"):
+        t = res["stages_ms"].get(stage, 0.0)
+        achieved = bytes_[stage] / (t * 1e-3) / 1e12 if t > 0 else 0.0
+        util, conflict = measured_lds(key, stage_kernel(stage, precision))
+        out[stage] = {"kernel": stage_kernel(stage, precision), "algorithmic_bytes": bytes_[stage], "avg_launch_ms": t,
+                      "achieved": achieved, "frac": achieved / LDS_PEAK_TBS, "measured_lds_util": util,
+                      "measured_bank_conflict": conflict}
+    return out
+
+
 def roofline(res, params, N, n_charged, precision):
     """Dominant kernel by the per-stage pass; its launch time from the timed region's events
     when it is the pair kernel (the one the timed region marks)."""
@@ -281,6 +323,7 @@ def roofline(res, params, N, n_charged, precision):
         "traffic": measured_traffic(workload_key(N, params, precision), kern),
         "algorithmic_bytes": b,
         "avg_launch_ms": avg[dom],
+        "lds": lds_roofline(res, params, N, n_charged, precision),
     }
 
 
