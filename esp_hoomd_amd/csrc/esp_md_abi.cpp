@@ -112,7 +112,19 @@ void build_prepare(esp_nlist* nl, const esp_system* sys)
         nl->N = N;
         nl->n_cells = nc;
         }
+    // Row capacities from the mean density (1.5x the mean count plus 32, in multiples of 8), so
+    // a first build rarely overflows and has to run again; they only ever grow.
+    const double rho = double(N) / (cg.Lx * cg.Ly * cg.Lz);
+    const double sphere = 4.18879020478639 * r_list * r_list * r_list;
+    const unsigned int est = (unsigned(1.5 * rho * sphere) + 32u + 7u) & ~7u;
+    nl->nmax = std::max(nl->nmax, std::min(est, std::max(N, 8u)));
     nl->want_inner = nl->r_inner > 0.0 && nl->r_inner < nl->r_cut;
+    if (nl->want_inner)
+        {
+        const double ri = nl->r_inner + nl->r_buff;
+        const unsigned int est_in = (unsigned(1.5 * rho * 4.18879020478639 * ri * ri * ri) + 16u + 7u) & ~7u;
+        nl->nmax_in = std::max(nl->nmax_in, std::min(est_in, std::max(N, 8u)));
+        }
     if (nl->want_inner)
         {
         nl->n_inner.resize(size_t(std::max(N, 1u)) * 4);
