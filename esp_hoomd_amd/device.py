@@ -2,6 +2,7 @@
 from __future__ import annotations
 
 import ctypes as C
+import threading
 
 import numpy as np
 
@@ -19,25 +20,28 @@ class _BlockCache:
     def __init__(self):
         self.free_blocks = {}
         self.held = 0
+        self.lock = threading.Lock()  # the serving loop frees while its admission thread allocates
 
     @staticmethod
     def size_class(nbytes: int) -> int:
         return max(256, 1 << (max(int(nbytes), 1) - 1).bit_length())
 
     def take(self, nbytes: int):
-        blocks = self.free_blocks.get(self.size_class(nbytes))
-        if blocks:
-            p = blocks.pop()
-            self.held -= self.size_class(nbytes)
-            return p
+        with self.lock:
+            blocks = self.free_blocks.get(self.size_class(nbytes))
+            if blocks:
+                p = blocks.pop()
+                self.held -= self.size_class(nbytes)
+                return p
         return None
 
     def give(self, ptr: C.c_void_p, cls: int) -> bool:
-        if self.held + cls > self.LIMIT:
-            return False
-        self.free_blocks.setdefault(cls, []).append(ptr)
-        self.held += cls
-        return True
+        with self.lock:
+            if self.held + cls > self.LIMIT:
+                return False
+            self.free_blocks.setdefault(cls, []).append(ptr)
+            self.held += cls
+            return True
 
 
 _cache = _BlockCache()

@@ -151,16 +151,19 @@ class NeighborList:
             state.inner_ptrs = (a.value, b.value, c.value, self.inner)
             self.nmax_inner = nmax.value
 
-    def update(self, state: State, r_cut: float, force_rebuild: bool = False) -> bool:
+    def update(self, state: State, r_cut: float, force_rebuild: bool = False, stream=None) -> bool:
+        """Rebuild the list if needed (or always, with force_rebuild) on `stream` (None: the null
+        stream); returns whether it was rebuilt.  Synchronises with the stream."""
         lib = _lib.load()
         self._ensure(r_cut)
         s = state.c_system()
         rebuilt = C.c_int(0)
+        h = stream.handle if stream is not None else None
         if force_rebuild:
-            rc = lib.esp_nlist_build(self._nl, C.byref(s), None)
+            rc = lib.esp_nlist_build(self._nl, C.byref(s), h)
             rebuilt.value = 1
         else:
-            rc = lib.esp_nlist_update(self._nl, C.byref(s), C.byref(rebuilt), None)
+            rc = lib.esp_nlist_update(self._nl, C.byref(s), C.byref(rebuilt), h)
         if rc != _lib.ESP_OK:
             raise _lib.EspError(rc, lib.esp_nlist_last_error(self._nl).decode())
         self._publish(state)
