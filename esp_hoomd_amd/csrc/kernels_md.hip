@@ -5,6 +5,8 @@
 #include "esp_device.h"
 #include "esp_md.h"
 
+#include <algorithm>
+
 namespace esp
 {
 namespace dev
@@ -278,14 +280,30 @@ __device__ __forceinline__ void wca_group(unsigned int i, int sub, bool active, 
         }
     }
 
+//! WCA over a neighbour list of any length, one wave per 64 particles: when the wave's rows are
+//! short (at most 16 entries, as in an inner list) each lane takes one particle in one pass;
+//! otherwise the wave takes its 64 particles in WCA_LANES passes of groups of WCA_LANES lanes.
 __global__ void __launch_bounds__(256) k_wca(unsigned int N, const double4* __restrict__ pos, double Lx, double Ly, double Lz,
                                              const unsigned int* __restrict__ n_neigh, const unsigned int* __restrict__ nlist,
                                              const uint64_t* __restrict__ head, double sigma, double epsilon, double4* __restrict__ force,
                                              double* __restrict__ virial, size_t pitch)
     {
-    const unsigned int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const unsigned int i = t / WCA_LANES;
-    wca_group<WCA_LANES>(i, int(t % WCA_LANES), i < N, pos, Lx, Ly, Lz, n_neigh, nlist, head, sigma, epsilon, force, virial, pitch);
+    const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned int lane = threadIdx.x & 63u;
+    const unsigned int longest = wave_max(i < N ? n_neigh[i] : 0u);
+    if (longest <= 16u)
+        {
+        wca_group<1>(i, 0, i < N, pos, Lx, Ly, Lz, n_neigh, nlist, head, sigma, epsilon, force, virial, pitch);
+        return;
+        }
+    const unsigned int first = i - lane;
+    constexpr unsigned int PER_PASS = 64u / unsigned(WCA_LANES);
+    for (unsigned int pass = 0; pass < unsigned(WCA_LANES); ++pass)
+        {
+        const unsigned int ip = first + pass * PER_PASS + lane / unsigned(WCA_LANES);
+        wca_group<WCA_LANES>(ip, int(lane % unsigned(WCA_LANES)), ip < N, pos, Lx, Ly, Lz, n_neigh, nlist, head, sigma, epsilon, force,
+                             virial, pitch);
+        }
     }
 //! Counter-based uniform random number in (-1, 1) from (seed, step, particle, component).
 __device__ __forceinline__ double uniform_pm1(uint64_t seed, uint64_t step, uint64_t i, uint32_t c)
@@ -392,18 +410,22 @@ __device__ __forceinline__ void block_max_to(double d2, unsigned long long* out)
 
 //! Largest squared minimum-image displacement since the last neighbour-list build, as the bits
 //! of a non-negative double (monotone in unsigned comparison).
+//! A grid of at most MAX_DISP_BLOCKS workgroups strides over the particles, so at most that many
+//! atomics meet on the one result word (one per 256 particles serialised there for ~40 us at
+//! 1e6 particles).
+constexpr unsigned int MAX_DISP_BLOCKS = 512;
 __global__ void __launch_bounds__(256) k_max_disp(unsigned int N, const double4* __restrict__ pos, const double4* __restrict__ last,
                                                   double Lx, double Ly, double Lz, unsigned long long* __restrict__ out)
     {
-    const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const double invLx = 1.0 / Lx, invLy = 1.0 / Ly, invLz = 1.0 / Lz;
     double d2 = 0.0;
-    if (i < N)
+    for (unsigned int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
         {
         const double4 a = pos[i], b = last[i];
-        const double dx = wrap_min_image(a.x - b.x, Lx, 1.0 / Lx);
-        const double dy = wrap_min_image(a.y - b.y, Ly, 1.0 / Ly);
-        const double dz = wrap_min_image(a.z - b.z, Lz, 1.0 / Lz);
-        d2 = dx * dx + dy * dy + dz * dz;
+        const double dx = wrap_min_image(a.x - b.x, Lx, invLx);
+        const double dy = wrap_min_image(a.y - b.y, Ly, invLy);
+        const double dz = wrap_min_image(a.z - b.z, Lz, invLz);
+        d2 = fmax(d2, dx * dx + dy * dy + dz * dz);
         }
     block_max_to(d2, out);
     }
@@ -577,7 +599,7 @@ hipError_t wca_forces(unsigned int N, const double* pos, double Lx, double Ly, d
     {
     if (N == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_wca, dim3(nblocks(N * unsigned(dev::WCA_LANES))), dim3(256), 0, s, N, reinterpret_cast<const double4*>(pos), Lx, Ly, Lz, n_neigh, nlist, head,
+    hipLaunchKernelGGL(k_wca, dim3(nblocks(N)), dim3(256), 0, s, N, reinterpret_cast<const double4*>(pos), Lx, Ly, Lz, n_neigh, nlist, head,
                        sigma, epsilon, reinterpret_cast<double4*>(force), virial, pitch);
     return hipGetLastError();
     }
@@ -606,7 +628,7 @@ hipError_t max_displacement(unsigned int N, const double* pos, const double* las
     {
     if (N == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(k_max_disp, dim3(nblocks(N)), dim3(256), 0, s, N, reinterpret_cast<const double4*>(pos),
+    hipLaunchKernelGGL(k_max_disp, dim3(std::min(nblocks(N), dev::MAX_DISP_BLOCKS)), dim3(256), 0, s, N, reinterpret_cast<const double4*>(pos),
                        reinterpret_cast<const double4*>(last), Lx, Ly, Lz, out);
     return hipGetLastError();
     }
