@@ -68,8 +68,10 @@ hipError_t fft_z_convolve(const MeshGeom& g, const FFTPlan& pz, const void* tw, 
                           int want_ev, double* ev_part, hipStream_t s);
 
 // ---- influence function and reductions ------------------------------------
-hipError_t build_influence(const MeshGeom& g, int split_kind, double kappa, double rc, double k_cut, int nq, const double* qx, const double* qw,
-                           const double* w2x, const double* w2y, const double* w2z, double* G, double* Gv, hipStream_t s);
+//! G = F(|k|) exp(-alpha |k|) / (k^2 |W|^2) (alpha = 0: no screening) and its virial companion Gv
+hipError_t build_influence(const MeshGeom& g, int split_kind, double kappa, double rc, double k_cut, double alpha, int nq,
+                           const double* qx, const double* qw, const double* w2x, const double* w2y, const double* w2z, double* G,
+                           double* Gv, hipStream_t s);
 hipError_t finalize(int n_ev, const double* ev_part, int n_q, const double* qpart, double inv_2V, double self_coeff, double bg_coeff,
                     double V, int want_ev, double* out, double* qsum, hipStream_t s);
 

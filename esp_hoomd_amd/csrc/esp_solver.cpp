@@ -306,8 +306,8 @@ void EspSolver::set_params(const SolverParams& p)
         throw EspError(STATUS_ARG, "rcut must be > 0");
     if (!(p.kappa > 0.0))
         throw EspError(STATUS_ARG, "kappa must be > 0");
-    if (p.alpha != 0.0)
-        throw EspError(STATUS_UNSUPPORTED, "Debye screening alpha != 0 is not supported");
+    if (!(p.alpha >= 0.0) || !std::isfinite(p.alpha))
+        throw EspError(STATUS_ARG, "alpha must be finite and >= 0");
     if (p.n_table < 16 || p.n_table > (1u << 20))
         throw EspError(STATUS_ARG, "n_table must be in [16, 2^20]");
     if (p.precision != 0 && p.precision != 1)
@@ -614,7 +614,7 @@ void EspSolver::build_influence(hipStream_t s)
         m_qxd.resize(8);
         m_qwd.resize(8);
         }
-    check_hip(kernel::build_influence(g, m_p.split, m_p.kappa, m_p.rcut, effective_k_cut(), nq, m_qxd.as<double>(), m_qwd.as<double>(), m_w2[0].as<double>(),
+    check_hip(kernel::build_influence(g, m_p.split, m_p.kappa, m_p.rcut, effective_k_cut(), m_p.alpha, nq, m_qxd.as<double>(), m_qwd.as<double>(), m_w2[0].as<double>(),
                                       m_w2[1].as<double>(), m_w2[2].as<double>(), m_G.as<double>(), m_Gv.as<double>(), s),
               "build_influence");
     }

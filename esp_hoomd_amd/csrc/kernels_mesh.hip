@@ -708,6 +708,7 @@ __global__ void __launch_bounds__(256) k_build_influence(MeshGeom g,
                                                          double kappa,
                                                          double rc,
                                                          double k_cut,
+                                                         double alpha,
                                                          int nq,
                                                          const double* __restrict__ qx,
                                                          const double* __restrict__ qw,
@@ -752,6 +753,14 @@ __global__ void __launch_bounds__(256) k_build_influence(MeshGeom g,
             F += qw[i] * c;
             dF -= qw[i] * rc * qx[i] * s;
             }
+        }
+    if (alpha > 0.0)
+        {
+        // screening factor exp(-alpha |k|) of the long-range part (ESPForceComputeGPU.cu:199);
+        // d/dk of F exp(-alpha k) enters the virial
+        const double scr = exp(-alpha * k);
+        dF = (dF - alpha * F) * scr;
+        F *= scr;
         }
     const double wp = w2x[x] * w2y[y] * w2z[z];
     G[out] = F / (k2 * wp);
@@ -1174,14 +1183,14 @@ ESP_INST_FFT(double)
 ESP_INST_FFT(float)
 #undef ESP_INST_FFT
 
-hipError_t build_influence(const MeshGeom& g, int split_kind, double kappa, double rc, double k_cut, int nq, const double* qx,
-                           const double* qw,
-                           const double* w2x, const double* w2y, const double* w2z, double* G, double* Gv, hipStream_t s)
+hipError_t build_influence(const MeshGeom& g, int split_kind, double kappa, double rc, double k_cut, double alpha, int nq,
+                           const double* qx, const double* qw, const double* w2x, const double* w2y, const double* w2z, double* G,
+                           double* Gv, hipStream_t s)
     {
     const size_t total = size_t(g.nz) * g.ny * g.nxh;
     const unsigned int nb = unsigned((total + 255) / 256);
-    hipLaunchKernelGGL(k_build_influence, dim3(nb), dim3(256), 0, s, g, split_kind, kappa, rc, k_cut, nq, qx, qw, w2x, w2y, w2z, G,
-                       Gv);
+    hipLaunchKernelGGL(k_build_influence, dim3(nb), dim3(256), 0, s, g, split_kind, kappa, rc, k_cut, alpha, nq, qx, qw, w2x, w2y, w2z,
+                       G, Gv);
     return hipGetLastError();
     }
 

@@ -8,9 +8,11 @@ and the ``Coulomb`` / ``Pair`` aliases.  The objects attach to this package's de
 ``md.Simulation`` (HOOMD-blue is not required); all computation runs in the HIP library.
 
 Documented differences from the reference: ``MAX_ORDER`` is 16 (the reference's Python layer
-caps at 8, esp.py:87, while its C++ allows 12, ESPForceCompute.h:63); ``alpha`` must be 0
-(Debye screening is not implemented); ``kappa`` is the band limit of the long-range part in
-1/length, i.e. the prolate bandwidth is c = kappa * r_cut.
+caps at 8, esp.py:87, while its C++ allows 12, ESPForceCompute.h:63); ``kappa`` is the band
+limit of the long-range part in 1/length, i.e. the prolate bandwidth is c = kappa * r_cut.
+``alpha`` (Debye screening) multiplies the long-range influence function by exp(-alpha |k|), as
+the reference's GPU influence build does (ESPForceComputeGPU.cu:199); the pair part, self and
+background terms do not depend on it, and energy and virial carry the factor consistently.
 """
 from __future__ import annotations
 

@@ -64,7 +64,8 @@ typedef struct esp_params {
     unsigned int order;            /* stencil order P, 2..16 */
     double kappa;                  /* splitting parameter [1/length] */
     double rcut;                   /* real-space cutoff r_c */
-    double alpha;                  /* Debye screening; must be 0 */
+    double alpha;                  /* Debye screening >= 0: the influence function is multiplied by
+                                      exp(-alpha |k|) (ESPForceComputeGPU.cu:199); 0 = none */
     unsigned int n_table;          /* split-table segments (PSWF_N_SEGS = 4096 default) */
     int precision;                 /* ESP_PREC_* */
     int split;                     /* ESP_SPLIT_* */

@@ -567,6 +567,7 @@ typedef struct {
     const unsigned int* ex_list;
     unsigned int ex_pitch;
     double k_cut;          /* spherical reciprocal-space cutoff: G = 0 for |k| > k_cut (0 = none) */
+    double alpha;          /* screening: G(k) carries exp(-alpha |k|) (ESPForceComputeGPU.cu:199); 0 = none */
 } oracle_esp_in;
 
 typedef struct {
@@ -827,8 +828,14 @@ int oracle_esp(const oracle_esp_in* in, oracle_esp_out* out)
                     size_t mid = (lo + hi) / 2;
                     if (uniq[mid] < k2) lo = mid + 1; else hi = mid;
                 }
-                const double F = uF[lo], dF = udF[lo];
+                double F = uF[lo], dF = udF[lo];
                 const double kk = sqrt(k2);
+                if (in->alpha > 0.0) {
+                    /* screened long-range part F(k) exp(-alpha k) (ESPForceComputeGPU.cu:199) and its k-derivative */
+                    const double scr = exp(-in->alpha * kk);
+                    dF = (dF - in->alpha * F) * scr;
+                    F *= scr;
+                }
                 const double wprod = w2[0][x] * w2[1][y] * w2[2][z];
                 const double G = F / (k2 * wprod);
                 /* d/dk (F/k^2) / k  / wprod */

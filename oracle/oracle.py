@@ -24,7 +24,7 @@ class EspIn(C.Structure):
         ("P", C.c_int), ("D", C.c_int), ("wcoef", _DP), ("n_segs", C.c_int), ("screen", _DP), ("dscreen", _DP),
         ("r_lo", _DP), ("dr_inv", _DP), ("rc", C.c_double), ("split_kind", C.c_int), ("kappa", C.c_double),
         ("n_ex", C.POINTER(C.c_uint)), ("ex_list", C.POINTER(C.c_uint)), ("ex_pitch", C.c_uint),
-        ("k_cut", C.c_double),
+        ("k_cut", C.c_double), ("alpha", C.c_double),
     ]
 
 
@@ -130,7 +130,7 @@ def ewald_auto(pos, q, box, tol=1e-15, targets=None, energy=True, rcut=None):
 class OracleESP:
     """Holds the tables of one ESP configuration and evaluates it on the CPU."""
 
-    def __init__(self, box, mesh, P, wcoef, screen, dscreen, r_lo, dr_inv, rc, split_kind=0, kappa=0.0, k_cut=0.0):
+    def __init__(self, box, mesh, P, wcoef, screen, dscreen, r_lo, dr_inv, rc, split_kind=0, kappa=0.0, k_cut=0.0, alpha=0.0):
         self.L = np.asarray(box[:3], np.float64)
         self.mesh = tuple(int(m) for m in mesh)
         self.P = int(P)
@@ -144,6 +144,7 @@ class OracleESP:
         self.split_kind = int(split_kind)
         self.kappa = float(kappa)
         self.k_cut = float(k_cut)
+        self.alpha = float(alpha)
 
     def _in(self, pt, q, n_ex=None, ex_list=None, ex_pitch=0):
         s = EspIn()
@@ -161,6 +162,7 @@ class OracleESP:
         s.split_kind = self.split_kind
         s.kappa = self.kappa
         s.k_cut = self.k_cut
+        s.alpha = self.alpha
         if n_ex is not None:
             s.n_ex = n_ex.ctypes.data_as(C.POINTER(C.c_uint))
             s.ex_list = ex_list.ctypes.data_as(C.POINTER(C.c_uint))

@@ -17,7 +17,7 @@ def oracle_for(solver: Solver, box, params: Params):
     scr, dscr, rlo, dri = solver.split_table()
     Pw, D, w = solver.window()
     return O.OracleESP(box, (params.nx, params.ny, params.nz), params.order, w, scr, dscr, rlo, dri, params.rcut,
-                       split_kind=params.split, kappa=params.kappa, k_cut=solver.info().k_cut)
+                       split_kind=params.split, kappa=params.kappa, k_cut=solver.info().k_cut, alpha=params.alpha)
 
 
 def run_gpu(params: Params, pos, q, box, exclusions=None, group=None, skin=0.3):

@@ -35,7 +35,8 @@ def test_version():
     (dict(order=17), _lib.ESP_ERR_ARG),
     (dict(rcut=0.0), _lib.ESP_ERR_ARG),
     (dict(kappa=-1.0), _lib.ESP_ERR_ARG),
-    (dict(alpha=0.5), _lib.ESP_ERR_UNSUPPORTED),
+    (dict(alpha=-0.5), _lib.ESP_ERR_ARG),
+    (dict(alpha=float("nan")), _lib.ESP_ERR_ARG),
     (dict(nx=14), _lib.ESP_ERR_UNSUPPORTED),
     (dict(nx=1), _lib.ESP_ERR_ARG),
     (dict(n_table=8), _lib.ESP_ERR_ARG),

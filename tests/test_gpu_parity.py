@@ -72,6 +72,29 @@ def test_fp64_group_and_zero_charges():
     assert abs(qs - q.sum()) < 1e-12 and abs(q2s - (q * q).sum()) < 1e-10
 
 
+@pytest.mark.parametrize("precision,tol", [(0, FP64_TOL), (1, 3e-6)])
+def test_screening_alpha_matches_oracle(precision, tol):
+    """Debye screening alpha > 0 (the influence function carries exp(-alpha |k|),
+    ESPForceComputeGPU.cu:199) in a rectangular box with a net charge: forces, energies and
+    virials against the oracle at the same alpha, and the screening changes the result."""
+    pos, q = random_electrolyte(200, 9.0, seed=17)
+    q[3] = 2.0
+    box = (9.0, 9.5, 10.0)
+    params = make_params((30, 32, 32), 8, 12.0, 3.0, precision=precision, alpha=0.35)
+    solver, _, res = run_gpu(params, pos, q, box)
+    ref = oracle_for(solver, box, params).compute(pos, q)
+    err = rel_rms(res["force"][:, :3], ref["force"][:, :3])
+    print(f"alpha=0.35 precision={precision} force err {err:.2e}", flush=True)
+    if precision == 0:
+        _compare(res, ref, tol)
+    else:
+        assert err < tol
+        assert abs(res["energy"] - ref["energy"]) < 1e-5 * abs(ref["energy"])
+    params0 = make_params((30, 32, 32), 8, 12.0, 3.0, precision=precision)
+    _, _, res0 = run_gpu(params0, pos, q, box)
+    assert rel_rms(res0["force"][:, :3], res["force"][:, :3]) > 1e-3
+
+
 def test_fp64_non_neutral_background():
     pos, q = random_electrolyte(90, 8.0, seed=11)
     q[0] = 2.0  # net charge

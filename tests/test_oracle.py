@@ -15,13 +15,13 @@ from esp_hoomd_amd import solver as S
 GOLDEN = Path(__file__).resolve().parent / "golden" / "pswf_ref_lookup.json"
 
 
-def esp_tables(kappa, rc, P, L, n, split=0, D=None, cw=None):
+def esp_tables(kappa, rc, P, L, n, split=0, D=None, cw=None, alpha=0.0):
     h = L / n
     K = kappa if split == 0 else 2 * kappa * 3.7
     Dd, cwd = S.default_window(P, np.pi / (h * K))
     w = S.build_window(P, D or Dd, cw or cwd)
     scr, dscr, rlo, dri = S.build_split_table(split, kappa, rc, 4096)
-    return O.OracleESP((L, L, L), (n, n, n), P, w, scr, dscr, rlo, dri, rc, split_kind=split, kappa=kappa)
+    return O.OracleESP((L, L, L), (n, n, n), P, w, scr, dscr, rlo, dri, rc, split_kind=split, kappa=kappa, alpha=alpha)
 
 
 @pytest.mark.parametrize("shape", [(8, 12, 10), (16, 16, 16), (5, 7, 3), (6, 9, 25)])
@@ -158,9 +158,10 @@ def test_oracle_pswf_accuracy_against_ewald(c, n, P, tol):
     assert rel_rms(r["total_virial"], we) < 10 * tol
 
 
-def test_oracle_forces_are_energy_gradient():
+@pytest.mark.parametrize("alpha", [0.0, 0.6])
+def test_oracle_forces_are_energy_gradient(alpha):
     pos, q = random_electrolyte(24, 6.0, seed=12, min_dist=0.7)
-    o = esp_tables(10 / 2.5, 2.5, 6, 6.0, 16)
+    o = esp_tables(10 / 2.5, 2.5, 6, 6.0, 16, alpha=alpha)
     r = o.compute(pos, q)
     eps = 1e-5
     for j in (0, 5, 17):
@@ -173,10 +174,11 @@ def test_oracle_forces_are_energy_gradient():
             assert abs(fd - r["force"][j, a]) < 1e-6 * max(1.0, abs(fd))
 
 
-def test_oracle_virial_is_finite_strain_derivative():
+@pytest.mark.parametrize("alpha", [0.0, 0.6])
+def test_oracle_virial_is_finite_strain_derivative(alpha):
     pos, q = random_electrolyte(30, 6.0, seed=13, min_dist=0.7)
     L = 6.0
-    base = esp_tables(10 / 2.5, 2.5, 6, L, 16)
+    base = esp_tables(10 / 2.5, 2.5, 6, L, 16, alpha=alpha)
     r = base.compute(pos, q)
     eps = 1e-6
     for a, comp in ((0, 0), (1, 3), (2, 5)):
@@ -184,7 +186,8 @@ def test_oracle_virial_is_finite_strain_derivative():
         for sgn in (+1, -1):
             s = np.ones(3)
             s[a] += sgn * eps
-            o = O.OracleESP(L * s, (16, 16, 16), 6, base.wcoef, base.screen, base.dscreen, base.r_lo, base.dr_inv, 2.5)
+            o = O.OracleESP(L * s, (16, 16, 16), 6, base.wcoef, base.screen, base.dscreen, base.r_lo, base.dr_inv, 2.5,
+                            alpha=alpha)
             es.append(o.compute(pos * s, q)["energy"])
         fd = -(es[0] - es[1]) / (2 * eps)
         assert abs(fd - r["total_virial"][comp]) < 1e-6 * abs(r["energy"])
@@ -232,3 +235,47 @@ def test_oracle_reciprocal_cutoff_limits():
     assert abs(tiny["ext_energy"] - expect) < 1e-12 * abs(expect)
     # the pair part is untouched by the cutoff
     np.testing.assert_array_equal(tiny["force"][:, 3], full["force"][:, 3])
+
+
+def _direct_reciprocal(pos, q, L, kappa, alpha, kmax):
+    """Reciprocal-space Gaussian-split energy and forces by the direct structure-factor sum
+    E = (1/2V) sum_{k != 0} g(k) |S(k)|^2, g = exp(-k^2 / 4 kappa^2) exp(-alpha k) / k^2,
+    F_j = (q_j / V) sum_k g(k) k Im(conj(S(k)) exp(i k.r_j))."""
+    n = int(np.ceil(kmax * L / (2 * np.pi)))
+    r = np.arange(-n, n + 1)
+    kv = 2 * np.pi / L * np.stack(np.meshgrid(r, r, r, indexing="ij"), -1).reshape(-1, 3)
+    k = np.linalg.norm(kv, axis=1)
+    keep = (k > 0) & (k <= kmax)
+    kv, k = kv[keep], k[keep]
+    g = np.exp(-k * k / (4 * kappa * kappa)) * np.exp(-alpha * k) / (k * k)
+    ph = np.exp(1j * pos @ kv.T)                # [N, K]
+    Sk = (q[:, None] * ph).sum(axis=0)
+    V = L ** 3
+    E = 0.5 / V * np.sum(g * np.abs(Sk) ** 2)
+    F = (q[:, None] / V) * (np.imag(np.conj(Sk)[None, :] * ph) * g[None, :]) @ kv
+    return E, F
+
+
+def test_oracle_screening_matches_direct_structure_factor_sum():
+    """The screening factor exp(-alpha |k|) of the long-range part (ESPForceComputeGPU.cu:199):
+    with the Gaussian split on a fine mesh, the change it makes to the oracle's energy and forces
+    equals the change it makes to the direct structure-factor sum (the real-space, self and
+    background terms do not depend on alpha and cancel)."""
+    L, kappa = 7.0, 1.1
+    pos, q = random_electrolyte(40, L, seed=31, min_dist=0.8)
+    kmax = 2 * kappa * 6.5
+    e_ref = {}
+    for alpha in (0.0, 0.45):
+        e_ref[alpha] = _direct_reciprocal(pos, q, L, kappa, alpha, kmax)
+    res = {}
+    for alpha in (0.0, 0.45):
+        o = esp_tables(kappa, 3.4, 12, L, 48, split=1, alpha=alpha)
+        res[alpha] = o.compute(pos, q)
+    dE_ref = e_ref[0.45][0] - e_ref[0.0][0]
+    dF_ref = e_ref[0.45][1] - e_ref[0.0][1]
+    dE = res[0.45]["energy"] - res[0.0]["energy"]
+    dF = res[0.45]["force"][:, :3] - res[0.0]["force"][:, :3]
+    assert abs(dE - dE_ref) < 1e-10 * abs(dE_ref)
+    assert rel_rms(dF, dF_ref) < 1e-9
+    # the pair part does not see alpha
+    np.testing.assert_array_equal(res[0.45]["force"][:, 3], res[0.0]["force"][:, 3])
