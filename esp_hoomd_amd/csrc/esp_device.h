@@ -13,6 +13,49 @@ namespace dev
 
 constexpr int WAVE = 64;
 
+//! Unwrapped fractional coordinates u = B^-1 r + 1/2 of a position r in a box centred on the
+//! origin, B = [a1 a2 a3] with a1 = (Lx, 0, 0), a2 = (xy Ly, Ly, 0), a3 = (xz Lz, yz Lz, Lz)
+//! (HOOMD's BoxDim::makeFraction).  An orthorhombic box takes the plain x / L + 1/2.
+struct Frac3
+    {
+    double x, y, z;
+    };
+//! The tilted forms are written with explicit fused multiply-adds, so that every kernel that
+//! inlines them (single-system and batched) rounds them the same way.
+__device__ __forceinline__ Frac3 box_frac(double x, double y, double z, const MeshGeom& g)
+    {
+    if (!g.tilted)
+        return {x / g.Lx + 0.5, y / g.Ly + 0.5, z / g.Lz + 0.5};
+    const double cxz = fma(-g.xy, g.yz, g.xz);  // xz - xy yz
+    return {fma(-cxz, z, fma(-g.xy, y, x)) / g.Lx + 0.5, fma(-g.yz, z, y) / g.Ly + 0.5, z / g.Lz + 0.5};
+    }
+
+//! Wavevector k = 2 pi B^-T m of the mesh mode m, given the per-axis wavenumbers
+//! k0 = 2 pi m_a / L_a: k_x = k0_x, k_y = k0_y - xy k0_x, k_z = k0_z - yz k0_y + (xy yz - xz) k0_x.
+__device__ __forceinline__ void mesh_wavevector(const MeshGeom& g, double k0x, double k0y, double k0z, double& kx, double& ky,
+                                                double& kz)
+    {
+    kx = k0x;
+    ky = k0y;
+    kz = k0z;
+    if (g.tilted)
+        {
+        ky = fma(-g.xy, k0x, k0y);
+        kz = fma(fma(g.xy, g.yz, -g.xz), k0x, fma(-g.yz, k0y, k0z));
+        }
+    }
+
+//! Cartesian gradient from the gradients along the mesh axes scaled by 1 / h_a
+//! (d/dr = B^-T d/du): g_y -= xy g_x, g_z += (xy yz - xz) g_x - yz g_y.
+__device__ __forceinline__ void mesh_gradient_cartesian(const MeshGeom& g, double& gx, double& gy, double& gz)
+    {
+    if (g.tilted)
+        {
+        gz = fma(fma(g.xy, g.yz, -g.xz), gx, fma(-g.yz, gy, gz));
+        gy = fma(-g.xy, gx, gy);
+        }
+    }
+
 template<typename T> __device__ __forceinline__ T wave_sum(T v)
     {
 #pragma unroll

@@ -90,6 +90,8 @@ struct PairArgs
     const double* postype;
     const double* charge;
     double Lx, Ly, Lz;
+    double xy, xz, yz;       // HOOMD tilt factors (0 for an orthorhombic box)
+    int tilted;              // non-zero when any tilt factor is non-zero
     const unsigned int* n_neigh;
     const unsigned int* nlist;
     const uint64_t* head_list;

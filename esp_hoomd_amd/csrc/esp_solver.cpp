@@ -368,8 +368,11 @@ void EspSolver::build_host_tables()
     double cw = m_p.window_c;
     if (cw <= 0.0)
         {
-        // mesh spacing is only known with the box; use the finest axis of the last box if any
-        const double Lmax = std::max({m_box[0] / m_p.nx, m_box[1] / m_p.ny, m_box[2] / m_p.nz});
+        // mesh spacing is only known with the box; use the coarsest axis of the last box if any
+        // (the length of lattice vector a over its mesh points: a tilted axis is longer than L)
+        const double a1 = m_box[0], a2 = m_box[1] * std::sqrt(1.0 + m_box[3] * m_box[3]),
+                     a3 = m_box[2] * std::sqrt(1.0 + m_box[4] * m_box[4] + m_box[5] * m_box[5]);
+        const double Lmax = std::max({a1 / m_p.nx, a2 / m_p.ny, a3 / m_p.nz});
         const double h = Lmax > 0 ? Lmax : 1.0;
         const double K = (kind == math::SplitKind::PSWF) ? m_p.kappa : 2.0 * m_p.kappa * 3.7;
         const double sigma = math::PI / (h * K);
@@ -427,15 +430,37 @@ template<typename R> static void upload_as(DeviceBuffer& buf, const std::vector<
     check_hip(hipMemcpy(buf.get(), tmp.data(), tmp.size() * sizeof(R), hipMemcpyHostToDevice), "hipMemcpy tables");
     }
 
+//! Box lengths > 0 and finite tilt factors (HOOMD's BoxDim: Lx, Ly, Lz, xy, xz, yz).
+static void check_box(const double* box)
+    {
+    if (!(box[0] > 0 && box[1] > 0 && box[2] > 0) || !std::isfinite(box[0]) || !std::isfinite(box[1]) || !std::isfinite(box[2]))
+        throw EspError(STATUS_ARG, "box lengths must be finite and > 0");
+    if (!std::isfinite(box[3]) || !std::isfinite(box[4]) || !std::isfinite(box[5]))
+        throw EspError(STATUS_ARG, "box tilt factors must be finite");
+    }
+
+void EspSolver::set_box_geometry(const double* box)
+    {
+    for (int a = 0; a < 6; ++a)
+        m_box[a] = box[a];
+    MeshGeom& g = m_geom;
+    g.Lx = m_box[0];
+    g.Ly = m_box[1];
+    g.Lz = m_box[2];
+    g.hx = g.Lx / g.nx;
+    g.hy = g.Ly / g.ny;
+    g.hz = g.Lz / g.nz;
+    g.xy = m_box[3];
+    g.xz = m_box[4];
+    g.yz = m_box[5];
+    g.tilted = (g.xy != 0.0 || g.xz != 0.0 || g.yz != 0.0) ? 1 : 0;
+    }
+
 void EspSolver::initialize(const SystemView& sys, hipStream_t s)
     {
-    if (sys.box[3] != 0.0 || sys.box[4] != 0.0 || sys.box[5] != 0.0)
-        throw EspError(STATUS_UNSUPPORTED, "triclinic boxes (non-zero tilt) are not supported");
-    if (!(sys.box[0] > 0 && sys.box[1] > 0 && sys.box[2] > 0))
-        throw EspError(STATUS_ARG, "box lengths must be > 0");
-    m_box[0] = sys.box[0];
-    m_box[1] = sys.box[1];
-    m_box[2] = sys.box[2];
+    check_box(sys.box);
+    for (int a = 0; a < 6; ++a)
+        m_box[a] = sys.box[a];
     const bool fp64 = m_p.precision == 0;
     const unsigned int P = m_p.order;
 
@@ -479,12 +504,7 @@ void EspSolver::initialize(const SystemView& sys, hipStream_t s)
     g.ty = (g.ny + T - 1) / T;
     g.tz = (g.nz + T - 1) / T;
     g.B = T + int(P) - 1;
-    g.Lx = m_box[0];
-    g.Ly = m_box[1];
-    g.Lz = m_box[2];
-    g.hx = g.Lx / g.nx;
-    g.hy = g.Ly / g.ny;
-    g.hz = g.Lz / g.nz;
+    set_box_geometry(sys.box);
     m_n_tiles = unsigned(g.tx) * g.ty * g.tz;
     if (m_n_tiles > (1u << 24))
         throw EspError(STATUS_ARG, "too many spreading tiles");
@@ -601,7 +621,15 @@ void EspSolver::build_influence(hipStream_t s)
     int nq = 0;
     if (kind == math::SplitKind::PSWF)
         {
-        const double kmax = math::PI * std::sqrt(1.0 / (g.hx * g.hx) + 1.0 / (g.hy * g.hy) + 1.0 / (g.hz * g.hz));
+        // largest |k| of the mesh: the corner modes (+-nx/2, +-ny/2, +-nz/2) of k = 2 pi B^-T m
+        double kmax = 0.0;
+        for (int sy = -1; sy <= 1; sy += 2)
+            for (int sz = -1; sz <= 1; sz += 2)
+                {
+                const double k0x = math::PI / g.hx, k0y = sy * math::PI / g.hy, k0z = sz * math::PI / g.hz;
+                const double kx = k0x, ky = k0y - g.xy * k0x, kz = k0z - g.yz * k0y + (g.xy * g.yz - g.xz) * k0x;
+                kmax = std::max(kmax, std::sqrt(kx * kx + ky * ky + kz * kz));
+                }
         nq = math::SplitFunction::quadrature_nodes(m_p.kappa * m_p.rcut, kmax * m_p.rcut);
         const auto quad = cached_quadrature(kind, m_p.kappa * m_p.rcut, nq);
         m_qx = quad->first;
@@ -643,6 +671,10 @@ kernel::PairArgs EspSolver::pair_args(const SystemView& sys, const OutputView& o
     a.Lx = sys.box[0];
     a.Ly = sys.box[1];
     a.Lz = sys.box[2];
+    a.xy = sys.box[3];
+    a.xz = sys.box[4];
+    a.yz = sys.box[5];
+    a.tilted = (a.xy != 0.0 || a.xz != 0.0 || a.yz != 0.0) ? 1 : 0;
     a.n_neigh = sys.n_neigh;
     a.nlist = sys.nlist;
     a.head_list = sys.head_list;
@@ -777,23 +809,17 @@ unsigned int EspSolver::prepare(const SystemView& sys, const OutputView& out, hi
         throw EspError(STATUS_ARG, "exclusions need ex_list and ex_pitch");
     if (sys.group_idx && sys.group_size > sys.N)
         throw EspError(STATUS_ARG, "group_size exceeds N");
-    if (m_need_init || sys.box[0] != m_box[0] || sys.box[1] != m_box[1] || sys.box[2] != m_box[2])
+    bool box_changed = false;
+    for (int a = 0; a < 6; ++a)
+        box_changed = box_changed || sys.box[a] != m_box[a];
+    if (m_need_init || box_changed)
         {
         if (m_need_init)
             initialize(sys, s);
         else
             {
-            if (sys.box[3] != 0.0 || sys.box[4] != 0.0 || sys.box[5] != 0.0)
-                throw EspError(STATUS_UNSUPPORTED, "triclinic boxes (non-zero tilt) are not supported");
-            m_box[0] = sys.box[0];
-            m_box[1] = sys.box[1];
-            m_box[2] = sys.box[2];
-            m_geom.Lx = m_box[0];
-            m_geom.Ly = m_box[1];
-            m_geom.Lz = m_box[2];
-            m_geom.hx = m_geom.Lx / m_geom.nx;
-            m_geom.hy = m_geom.Ly / m_geom.ny;
-            m_geom.hz = m_geom.Lz / m_geom.nz;
+            check_box(sys.box);
+            set_box_geometry(sys.box);
             build_influence(s);
             ++m_gen;
             }
@@ -956,10 +982,7 @@ void EspSolver::pair_forces_into(const SystemView& sys, const OutputView& out, h
         throw EspError(STATUS_ARG, "virial_pitch must be >= N");
     if (sys.n_ex && (!sys.ex_list || sys.ex_pitch == 0))
         throw EspError(STATUS_ARG, "exclusions need ex_list and ex_pitch");
-    if (!(sys.box[0] > 0 && sys.box[1] > 0 && sys.box[2] > 0))
-        throw EspError(STATUS_ARG, "box lengths must be > 0");
-    if (sys.box[3] != 0.0 || sys.box[4] != 0.0 || sys.box[5] != 0.0)
-        throw EspError(STATUS_UNSUPPORTED, "triclinic boxes (non-zero tilt) are not supported");
+    check_box(sys.box);
     if (size_t(sys.N) * 32 > scratch_bytes)
         throw EspError(STATUS_RANGE, "N exceeds the particles the no-allocation scratch holds");
     kernel::PairArgs a = pair_args(sys, out);

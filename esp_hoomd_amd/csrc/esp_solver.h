@@ -241,7 +241,8 @@ class EspSolver
     bool m_need_init = true;
     bool m_tables_built = false;
     bool m_external_table = false;
-    double m_box[3] = {0, 0, 0};
+    double m_box[6] = {0, 0, 0, 0, 0, 0};  // Lx, Ly, Lz, xy, xz, yz of the tables' box
+    void set_box_geometry(const double* box);
     MeshGeom m_geom {};
     FFTPlan m_plan[3] {};
     unsigned int m_n_tiles = 0;

@@ -28,8 +28,11 @@ struct MeshGeom
     int T;              // spreading tile edge (cells)
     int tx, ty, tz;     // tiles per axis
     int B;              // LDS box edge T + P - 1
-    double Lx, Ly, Lz;  // orthorhombic box lengths (box centred on the origin)
-    double hx, hy, hz;  // mesh spacing
+    double Lx, Ly, Lz;  // box lengths (box centred on the origin)
+    double hx, hy, hz;  // L / n per axis (the mesh axes follow the lattice vectors)
+    double xy, xz, yz;  // HOOMD tilt factors: lattice vectors a1 = (Lx, 0, 0),
+                        // a2 = (xy Ly, Ly, 0), a3 = (xz Lz, yz Lz, Lz)
+    int tilted;         // non-zero when any tilt factor is non-zero
     };
 
 //! Particle record in tile-sorted order (one per charged group member).

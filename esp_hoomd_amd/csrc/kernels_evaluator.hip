@@ -10,6 +10,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstddef>
 
 namespace esp
@@ -23,15 +24,27 @@ static_assert(sizeof(Eval::param_type) == sizeof(esp_pair_param), "esp_pair_para
 static_assert(offsetof(Eval::param_type, table_device) == offsetof(esp_pair_param, table_device), "param layout");
 static_assert(offsetof(Eval::param_type, n_segs) == offsetof(esp_pair_param, n_segs), "param layout");
 
-__device__ __forceinline__ double min_image(double d, double L, double inv_L)
+//! HOOMD's BoxDim::minImage for a box centred on the origin with tilt factors xy, xz, yz:
+//! lattice vector a3 = (xz Lz, yz Lz, Lz) by rint(d_z / Lz), then a2 = (xy Ly, Ly, 0) by
+//! rint(d_y / Ly), then a1 = (Lx, 0, 0) by rint(d_x / Lx).
+__device__ __forceinline__ void min_image(double& dx, double& dy, double& dz, double Lx, double Ly, double Lz, double xy, double xz,
+                                          double yz)
     {
-    return d - L * rint(d * inv_L);
+    const double iz = rint(dz * (1.0 / Lz));
+    dz -= Lz * iz;
+    dy -= yz * Lz * iz;
+    dx -= xz * Lz * iz;
+    const double iy = rint(dy * (1.0 / Ly));
+    dy -= Ly * iy;
+    dx -= xy * Ly * iy;
+    dx -= Lx * rint(dx * (1.0 / Lx));
     }
 
 __global__ void __launch_bounds__(256) k_potential_pair(unsigned int N, const double4* __restrict__ postype,
                                                         const double* __restrict__ charge, const unsigned int* __restrict__ n_neigh,
                                                         const unsigned int* __restrict__ nlist, const uint64_t* __restrict__ head,
-                                                        double Lx, double Ly, double Lz, Eval::param_type param, double rcutsq,
+                                                        double Lx, double Ly, double Lz, double xy, double xz, double yz,
+                                                        Eval::param_type param, double rcutsq,
                                                         int energy_shift, double4* __restrict__ force, double* __restrict__ virial,
                                                         size_t pitch)
     {
@@ -40,7 +53,6 @@ __global__ void __launch_bounds__(256) k_potential_pair(unsigned int N, const do
         return;
     const double4 pi = postype[i];
     const double qi = charge[i];
-    const double ix = 1.0 / Lx, iy = 1.0 / Ly, iz = 1.0 / Lz;
     double fx = 0, fy = 0, fz = 0, e = 0;
     double v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
     const unsigned int nn = n_neigh[i];
@@ -49,9 +61,8 @@ __global__ void __launch_bounds__(256) k_potential_pair(unsigned int N, const do
         {
         const unsigned int j = nlist[h + k];
         const double4 pj = postype[j];
-        const double dx = min_image(pi.x - pj.x, Lx, ix);
-        const double dy = min_image(pi.y - pj.y, Ly, iy);
-        const double dz = min_image(pi.z - pj.z, Lz, iz);
+        double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        min_image(dx, dy, dz, Lx, Ly, Lz, xy, xz, yz);
         const double rsq = dx * dx + dy * dy + dz * dz;
         Eval ev(rsq, rcutsq, param);
         if (Eval::needsCharge())
@@ -96,7 +107,8 @@ extern "C" int esp_potential_pair_forces(const esp_system* sys, const esp_pair_p
         return ESP_OK;
     if (!sys->postype || !sys->charge || !sys->n_neigh || !sys->nlist || !sys->head_list)
         return ESP_ERR_ARG;
-    if (!(sys->box[0] > 0 && sys->box[1] > 0 && sys->box[2] > 0) || sys->box[3] != 0.0 || sys->box[4] != 0.0 || sys->box[5] != 0.0)
+    if (!(sys->box[0] > 0 && sys->box[1] > 0 && sys->box[2] > 0) || !std::isfinite(sys->box[3]) || !std::isfinite(sys->box[4])
+        || !std::isfinite(sys->box[5]))
         return ESP_ERR_ARG;
     if (out->virial && out->virial_pitch < sys->N)
         return ESP_ERR_ARG;
@@ -108,7 +120,7 @@ extern "C" int esp_potential_pair_forces(const esp_system* sys, const esp_pair_p
     const unsigned int blocks = (sys->N + 255u) / 256u;
     hipLaunchKernelGGL(esp::kernel::k_potential_pair, dim3(blocks), dim3(256), 0, static_cast<hipStream_t>(stream), sys->N,
                        reinterpret_cast<const double4*>(sys->postype), sys->charge, sys->n_neigh, sys->nlist, sys->head_list, sys->box[0],
-                       sys->box[1], sys->box[2], p, r_cut * r_cut, energy_shift, reinterpret_cast<double4*>(out->force), out->virial,
+                       sys->box[1], sys->box[2], sys->box[3], sys->box[4], sys->box[5], p, r_cut * r_cut, energy_shift, reinterpret_cast<double4*>(out->force), out->virial,
                        out->virial_pitch);
     return hipGetLastError() == hipSuccess ? ESP_OK : ESP_ERR_HIP;
     }

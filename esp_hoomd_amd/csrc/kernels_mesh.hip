@@ -76,10 +76,10 @@ __device__ __forceinline__ void fixed_point_scale(unsigned int cmax, double qmax
     }
 
 //! Stencil origin o = ceil(g - P/2) (wrapped into [0, n)) and offset t = o - (g - P/2) in [0, 1)
-//! for one axis; g = makeFraction(x) * n for a box centred on the origin.
-__device__ __forceinline__ int stencil_origin(double x, double L, int n, int P, double& t)
+//! for one axis; g = makeFraction(r) * n, from the unwrapped fraction u of box_frac.
+__device__ __forceinline__ int stencil_origin(double u, int n, int P, double& t)
     {
-    double frac = x / L + 0.5;
+    double frac = u;
     frac -= floor(frac);
     const double gg = frac * double(n) - 0.5 * double(P);
     const double o = ceil(gg);
@@ -111,9 +111,10 @@ __device__ __forceinline__ double bin_one(unsigned int i,
     if (q != 0.0)
         {
         double t;
-        const int ox = stencil_origin(r.x, g.Lx, g.nx, P, t);
-        const int oy = stencil_origin(r.y, g.Ly, g.ny, P, t);
-        const int oz = stencil_origin(r.z, g.Lz, g.nz, P, t);
+        const Frac3 u = box_frac(r.x, r.y, r.z, g);
+        const int ox = stencil_origin(u.x, g.nx, P, t);
+        const int oy = stencil_origin(u.y, g.ny, P, t);
+        const int oz = stencil_origin(u.z, g.nz, P, t);
         k = unsigned((oz / g.T) * g.ty + oy / g.T) * unsigned(g.tx) + unsigned(ox / g.T);
         slot[i] = atomicAdd(&tile_count[k], 1u);
         }
@@ -288,9 +289,10 @@ __device__ __forceinline__ void scatter_one(unsigned int i,
     const unsigned int p = group ? group[i] : i;
     const double4 r = postype[p];
     double tx, ty, tz;
-    const int ox = stencil_origin(r.x, g.Lx, g.nx, P, tx);
-    const int oy = stencil_origin(r.y, g.Ly, g.ny, P, ty);
-    const int oz = stencil_origin(r.z, g.Lz, g.nz, P, tz);
+    const Frac3 u = box_frac(r.x, r.y, r.z, g);
+    const int ox = stencil_origin(u.x, g.nx, P, tx);
+    const int oy = stencil_origin(u.y, g.ny, P, ty);
+    const int oz = stencil_origin(u.z, g.nz, P, tz);
     const unsigned int pos = start[k] + slot[i];
     SortedRec<R> o;
     o.tx = R(tx);
@@ -585,7 +587,7 @@ __device__ __forceinline__ void fft_z_conv_block(int blk,
     double e = 0, v0 = 0, v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
     const double two_pi = 6.283185307179586476925286766559;
     const int ky_i = (y <= ny / 2) ? y : y - ny;
-    const double ky = two_pi * double(ky_i) / g.Ly;
+    const double ky0 = two_pi * double(ky_i) / g.Ly;
     struct GG
         {
         double gk, gv;
@@ -620,8 +622,10 @@ __device__ __forceinline__ void fft_z_conv_block(int blk,
             const double s2 = double(c.x) * double(c.x) + double(c.y) * double(c.y);
             const double gv = gg.gv;
             const int kz_i = (z <= nz / 2) ? z : z - nz;
-            const double kx = two_pi * double(x) / g.Lx;
-            const double kz = two_pi * double(kz_i) / g.Lz;
+            const double kx0 = two_pi * double(x) / g.Lx;
+            const double kz0 = two_pi * double(kz_i) / g.Lz;
+            double kx, ky, kz;
+            mesh_wavevector(g, kx0, ky0, kz0, kx, ky, kz);
             e += s2 * gk;
             v0 += s2 * (gk + gv * kx * kx);
             v1 += s2 * (gv * kx * ky);
@@ -633,9 +637,8 @@ __device__ __forceinline__ void fft_z_conv_block(int blk,
                 {
                 // the conjugate mode (-k) outside the half spectrum; a Nyquist index keeps
                 // its +N/2 wavenumber (k ordering i <= N/2 -> +i, ESPForceComputeGPU.cu:173-175)
-                const double kxp = -kx;
-                const double kyp = (2 * y == ny) ? ky : -ky;
-                const double kzp = (2 * z == nz) ? kz : -kz;
+                double kxp, kyp, kzp;
+                mesh_wavevector(g, -kx0, (2 * y == ny) ? ky0 : -ky0, (2 * z == nz) ? kz0 : -kz0, kxp, kyp, kzp);
                 e += s2 * gk;
                 v0 += s2 * (gk + gv * kxp * kxp);
                 v1 += s2 * (gv * kxp * kyp);
@@ -729,7 +732,8 @@ __global__ void __launch_bounds__(256) k_build_influence(MeshGeom g,
     const int ky_i = (y <= g.ny / 2) ? y : y - g.ny;
     const int kz_i = (z <= g.nz / 2) ? z : z - g.nz;
     const double two_pi = 6.283185307179586476925286766559;
-    const double kx = two_pi * x / g.Lx, ky = two_pi * ky_i / g.Ly, kz = two_pi * kz_i / g.Lz;
+    double kx, ky, kz;
+    mesh_wavevector(g, two_pi * x / g.Lx, two_pi * ky_i / g.Ly, two_pi * kz_i / g.Lz, kx, ky, kz);
     const double k2 = kx * kx + ky * ky + kz * kz;
     if (k2 == 0.0 || (k_cut > 0.0 && k2 > k_cut * k_cut))
         {
@@ -868,9 +872,10 @@ __device__ __forceinline__ double bin_key(unsigned int i,
     if (q != 0.0)
         {
         double t;
-        const int ox = stencil_origin(r.x, g.Lx, g.nx, P, t);
-        const int oy = stencil_origin(r.y, g.Ly, g.ny, P, t);
-        const int oz = stencil_origin(r.z, g.Lz, g.nz, P, t);
+        const Frac3 u = box_frac(r.x, r.y, r.z, g);
+        const int ox = stencil_origin(u.x, g.nx, P, t);
+        const int oy = stencil_origin(u.y, g.ny, P, t);
+        const int oz = stencil_origin(u.z, g.nz, P, t);
         k = unsigned((oz / g.T) * g.ty + oy / g.T) * unsigned(g.tx) + unsigned(ox / g.T);
         }
     return q;

@@ -28,6 +28,22 @@ __device__ __forceinline__ double min_image(double d, double L, double invL)
     return d - L * rint(d * invL);
     }
 
+//! Minimum image of d = r_i - r_j in a tilted box, HOOMD's BoxDim::minImage order: lattice
+//! vector a3 by rint(d_z / Lz), then a2 by rint(d_y / Ly), then a1 by rint(d_x / Lx).  For
+//! |d| below half the smallest perpendicular width of the box this is the nearest image.
+//! Explicit fused multiply-adds: single-system and batched kernels round alike.
+__device__ __forceinline__ void min_image_tilted(double& dx, double& dy, double& dz, const kernel::PairArgs& a)
+    {
+    const double iz = rint(dz * (1.0 / a.Lz));
+    dz = fma(-a.Lz, iz, dz);
+    dy = fma(-(a.yz * a.Lz), iz, dy);
+    dx = fma(-(a.xz * a.Lz), iz, dx);
+    const double iy = rint(dy * (1.0 / a.Ly));
+    dy = fma(-a.Ly, iy, dy);
+    dx = fma(-(a.xy * a.Ly), iy, dx);
+    dx = fma(-a.Lx, rint(dx * (1.0 / a.Lx)), dx);
+    }
+
 // Neighbour records of the pair pass, written by the pack kernel.
 //  * FP64: (x, y, z, q) as doubles, 32 bytes; minimum image by rounding.
 //  * FP32 (mixed): x, y, z as 32-bit fixed-point fractions of the box edge and q as a float,
@@ -72,17 +88,34 @@ __device__ __forceinline__ void rec_separation(const typename PairRec<R>::type& 
     {
     if constexpr (sizeof(R) == 8)
         {
-        dx = min_image(pi.x - pj.x, a.Lx, 1.0 / a.Lx);
-        dy = min_image(pi.y - pj.y, a.Ly, 1.0 / a.Ly);
-        dz = min_image(pi.z - pj.z, a.Lz, 1.0 / a.Lz);
+        if (a.tilted)
+            {
+            dx = pi.x - pj.x;
+            dy = pi.y - pj.y;
+            dz = pi.z - pj.z;
+            min_image_tilted(dx, dy, dz, a);
+            }
+        else
+            {
+            dx = min_image(pi.x - pj.x, a.Lx, 1.0 / a.Lx);
+            dy = min_image(pi.y - pj.y, a.Ly, 1.0 / a.Ly);
+            dz = min_image(pi.z - pj.z, a.Lz, 1.0 / a.Lz);
+            }
         qiqj = pi.w * pj.w;
         }
     else
         {
         constexpr double TWO_M32 = 2.3283064365386962890625e-10;  // 2^-32
-        dx = float(int(pi.x - pj.x)) * float(a.Lx * TWO_M32);
-        dy = float(int(pi.y - pj.y)) * float(a.Ly * TWO_M32);
-        dz = float(int(pi.z - pj.z)) * float(a.Lz * TWO_M32);
+        const float ux = float(int(pi.x - pj.x)), uy = float(int(pi.y - pj.y)), uz = float(int(pi.z - pj.z));
+        dx = ux * float(a.Lx * TWO_M32);
+        dy = uy * float(a.Ly * TWO_M32);
+        dz = uz * float(a.Lz * TWO_M32);
+        if (a.tilted)
+            {
+            // d = B (u_i - u_j): the wrapped fraction differences through the lattice vectors
+            dx = __fmaf_rn(uz, float(a.xz * a.Lz * TWO_M32), __fmaf_rn(uy, float(a.xy * a.Ly * TWO_M32), dx));
+            dy = __fmaf_rn(uz, float(a.yz * a.Lz * TWO_M32), dy);
+            }
         qiqj = __uint_as_float(pi.w) * __uint_as_float(pj.w);
         }
     }
@@ -405,9 +438,18 @@ template<typename R> __device__ __forceinline__ void pack_one(unsigned int i, co
     const double4 p = reinterpret_cast<const double4*>(a.postype)[i];
     if constexpr (sizeof(R) == 8)
         reinterpret_cast<double4*>(a.pq)[i] = make_double4(p.x, p.y, p.z, a.charge[i]);
-    else
+    else if (!a.tilted)
         reinterpret_cast<uint4*>(a.pq)[i] = make_uint4(box_fraction(p.x, 1.0 / a.Lx), box_fraction(p.y, 1.0 / a.Ly),
                                                       box_fraction(p.z, 1.0 / a.Lz), __float_as_uint(float(a.charge[i])));
+    else
+        {
+        // fractions along the lattice vectors (HOOMD's makeFraction), B^-1 r
+        const double fx = fma(-fma(-a.xy, a.yz, a.xz), p.z, fma(-a.xy, p.y, p.x)) * (1.0 / a.Lx);
+        const double fy = fma(-a.yz, p.z, p.y) * (1.0 / a.Ly);
+        const double fz = p.z * (1.0 / a.Lz);
+        reinterpret_cast<uint4*>(a.pq)[i] = make_uint4(box_fraction(fx, 1.0), box_fraction(fy, 1.0), box_fraction(fz, 1.0),
+                                                      __float_as_uint(float(a.charge[i])));
+        }
     }
 
 template<typename R> __global__ void __launch_bounds__(256) k_pack_pq(kernel::PairArgs a)

@@ -439,7 +439,9 @@ __global__ void __launch_bounds__(256) k_gather(MeshGeom g,
             if (p < n && sl == 0)
                 {
                 const double q = double(qs[p]);
-                add_mesh_force(force + idxs[p], q * double(ex * inv_hx), q * double(ey * inv_hy), q * double(ez * inv_hz));
+                double gx = double(ex * inv_hx), gy = double(ey * inv_hy), gz = double(ez * inv_hz);
+                mesh_gradient_cartesian(g, gx, gy, gz);
+                add_mesh_force(force + idxs[p], q * gx, q * gy, q * gz);
                 }
             }
         __syncthreads();
@@ -737,7 +739,9 @@ __device__ __forceinline__ void gather_tile_tp(unsigned int tile,
                 continue;
             }
         const double q = double(r.q);
-        add_mesh_force(force + id.idx, q * double(ex * inv_hx), q * double(ey * inv_hy), q * double(ez * inv_hz));
+        double gx = double(ex * inv_hx), gy = double(ey * inv_hy), gz = double(ez * inv_hz);
+        mesh_gradient_cartesian(g, gx, gy, gz);
+        add_mesh_force(force + id.idx, q * gx, q * gy, q * gz);
         }
     }
 

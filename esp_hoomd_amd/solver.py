@@ -93,7 +93,7 @@ class DeviceSystem:
             self.ex_list = DeviceArray.from_numpy(ex_list)
             self.ex_pitch = pitch
         if r_list is not None:
-            self.set_nlist(*build_full_nlist(pos, self.box[:3], r_list, self.exclusions))
+            self.set_nlist(*build_full_nlist(pos, self.box, r_list, self.exclusions))
 
     def set_nlist(self, n_neigh, nlist, head):
         self.n_neigh = DeviceArray.from_numpy(np.asarray(n_neigh, np.uint32))

@@ -88,7 +88,11 @@ typedef struct esp_system {
     const double* charge;           /* Scalar[N] */
     unsigned int group_size;        /* members of the mesh group (ignored if group_idx NULL) */
     const unsigned int* group_idx;  /* group member indices, or NULL for all N */
-    double box[6];                  /* Lx, Ly, Lz, xy, xz, yz (tilts must be 0) */
+    double box[6];                  /* HOOMD BoxDim: Lx, Ly, Lz and tilt factors xy, xz, yz (lattice vectors
+                                       (Lx,0,0), (xy Ly,Ly,0), (xz Lz,yz Lz,Lz)); spreading and gathering use
+                                       makeFraction (ESPForceComputeGPU.cu:228, 320), pairs HOOMD's minImage,
+                                       so in a tilted box rcut must stay below half the box's smallest
+                                       perpendicular width */
     const unsigned int* n_neigh;    /* full neighbour list, or NULL for no pair part */
     const unsigned int* nlist;
     const uint64_t* head_list;

@@ -177,9 +177,53 @@ int oracle_fft3d(int nx, int ny, int nz, double* data, int sign)
 /* ------------------------------------------------------------------------- */
 /* Cell binning for real-space sums (any cutoff, periodic images included)    */
 /* ------------------------------------------------------------------------- */
-static void min_image(double* d, const double* L)
+/* General (triclinic) box of HOOMD's BoxDim, centred on the origin: lattice vectors
+ * a1 = (Lx, 0, 0), a2 = (xy Ly, Ly, 0), a3 = (xz Lz, yz Lz, Lz). */
+typedef struct {
+    double L[3];
+    double xy, xz, yz;
+} obox_t;
+
+/* fractions u = B^-1 r + 1/2 (BoxDim::makeFraction), not wrapped */
+static void box_fraction3(const obox_t* b, const double* r, double* u)
 {
-    for (int a = 0; a < 3; ++a) d[a] -= L[a] * nearbyint(d[a] / L[a]);
+    u[0] = (r[0] - b->xy * r[1] - (b->xz - b->xy * b->yz) * r[2]) / b->L[0] + 0.5;
+    u[1] = (r[1] - b->yz * r[2]) / b->L[1] + 0.5;
+    u[2] = r[2] / b->L[2] + 0.5;
+}
+
+/* BoxDim::minImage: a3 by rint(d_z / Lz), then a2 by rint(d_y / Ly), then a1 by rint(d_x / Lx);
+ * the nearest image whenever |d| is below half the smallest perpendicular width of the box */
+static void min_image_box(double* d, const obox_t* b)
+{
+    const double iz = nearbyint(d[2] / b->L[2]);
+    d[2] -= b->L[2] * iz;
+    d[1] -= b->yz * b->L[2] * iz;
+    d[0] -= b->xz * b->L[2] * iz;
+    const double iy = nearbyint(d[1] / b->L[1]);
+    d[1] -= b->L[1] * iy;
+    d[0] -= b->xy * b->L[1] * iy;
+    d[0] -= b->L[0] * nearbyint(d[0] / b->L[0]);
+}
+
+/* k = 2 pi B^-T m for integer mode m */
+static void box_wavevector(const obox_t* b, int mx, int my, int mz, double* k)
+{
+    const double k0x = 2 * ORACLE_PI * mx / b->L[0], k0y = 2 * ORACLE_PI * my / b->L[1], k0z = 2 * ORACLE_PI * mz / b->L[2];
+    k[0] = k0x;
+    k[1] = k0y - b->xy * k0x;
+    k[2] = k0z - b->yz * k0y + (b->xy * b->yz - b->xz) * k0x;
+}
+
+/* half the smallest perpendicular width of the box: 1 / (2 max |b_a|), b_a the rows of B^-1 */
+static double box_half_width(const obox_t* b)
+{
+    const double n1 = sqrt(1.0 + b->xy * b->xy + (b->xy * b->yz - b->xz) * (b->xy * b->yz - b->xz)) / b->L[0];
+    const double n2 = sqrt(1.0 + b->yz * b->yz) / b->L[1];
+    const double n3 = 1.0 / b->L[2];
+    double m = n1 > n2 ? n1 : n2;
+    m = m > n3 ? m : n3;
+    return 0.5 / m;
 }
 
 typedef struct {
@@ -533,6 +577,164 @@ int oracle_ewald(int N, const double* postype, const double* q, const double* Lb
     return 0;
 }
 
+/*
+ * Direct Ewald sum in a general (triclinic) box: real space over the nearest image of every
+ * pair (rcut must be below half the smallest perpendicular width of the box), k space over
+ * k = 2 pi B^-T m with |k| <= kcut on the half space of m, the structure factor from per-axis
+ * phase factors e^{-2 pi i m_a u_a} of the fractions u = B^-1 r.  Forces on all particles,
+ * energy and virial.  An independent check of the mesh method in tilted boxes.
+ */
+int oracle_ewald_box(int N, const double* postype, const double* q, const double* box6, double alpha, double rcut, double kcut,
+                     double* force, double* energy, double* virial)
+{
+    obox_t b = {{box6[0], box6[1], box6[2]}, box6[3], box6[4], box6[5]};
+    if (rcut >= box_half_width(&b)) return -1;
+    const double V = b.L[0] * b.L[1] * b.L[2];
+    const double two_a_spi = 2.0 * alpha / sqrt(ORACLE_PI);
+    const double rc2 = rcut * rcut;
+    double E = 0.0, W[6] = {0, 0, 0, 0, 0, 0};
+    memset(force, 0, sizeof(double) * 3 * (size_t)N);
+#pragma omp parallel for schedule(dynamic, 8) reduction(+ : E) reduction(+ : W[:6])
+    for (int i = 0; i < N; ++i) {
+        double f[3] = {0, 0, 0};
+        for (int j = 0; j < N; ++j) {
+            if (j == i) continue;
+            double d[3] = {postype[4 * i] - postype[4 * j], postype[4 * i + 1] - postype[4 * j + 1],
+                           postype[4 * i + 2] - postype[4 * j + 2]};
+            min_image_box(d, &b);
+            const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            if (r2 >= rc2) continue;
+            const double r = sqrt(r2);
+            const double qq = q[i] * q[j] * ORACLE_INV_4PI;
+            const double er = erfc(alpha * r);
+            const double fdivr = qq * (er / r + two_a_spi * exp(-alpha * alpha * r2)) / r2;
+            for (int a = 0; a < 3; ++a) f[a] += fdivr * d[a];
+            E += 0.5 * qq * er / r;
+            W[0] += 0.5 * d[0] * d[0] * fdivr;
+            W[1] += 0.5 * d[0] * d[1] * fdivr;
+            W[2] += 0.5 * d[0] * d[2] * fdivr;
+            W[3] += 0.5 * d[1] * d[1] * fdivr;
+            W[4] += 0.5 * d[1] * d[2] * fdivr;
+            W[5] += 0.5 * d[2] * d[2] * fdivr;
+        }
+        for (int a = 0; a < 3; ++a) force[3 * i + a] = f[a];
+    }
+    /* |m_a| <= |a_a| kcut / (2 pi): m_a = a_a . k / (2 pi) */
+    const double alen[3] = {b.L[0], b.L[1] * sqrt(1.0 + b.xy * b.xy), b.L[2] * sqrt(1.0 + b.xz * b.xz + b.yz * b.yz)};
+    int mm[3];
+    for (int a = 0; a < 3; ++a) mm[a] = (int)floor(alen[a] * kcut / (2 * ORACLE_PI));
+    const int wx = 2 * mm[0] + 1, wy = 2 * mm[1] + 1, wz = mm[2] + 1;
+    /* mode list on the half space m_z > 0, or m_z = 0 and (m_y > 0 or (m_y = 0 and m_x > 0)) */
+    long nk = 0;
+    int* modes = malloc(sizeof(int) * 3 * (size_t)wx * wy * wz);
+    double* kv = malloc(sizeof(double) * 3 * (size_t)wx * wy * wz);
+    for (int c = 0; c <= mm[2]; ++c)
+        for (int bb = -mm[1]; bb <= mm[1]; ++bb)
+            for (int a = -mm[0]; a <= mm[0]; ++a) {
+                if (c == 0 && (bb < 0 || (bb == 0 && a <= 0))) continue;
+                double k[3];
+                box_wavevector(&b, a, bb, c, k);
+                if (k[0] * k[0] + k[1] * k[1] + k[2] * k[2] > kcut * kcut) continue;
+                modes[3 * nk] = a;
+                modes[3 * nk + 1] = bb;
+                modes[3 * nk + 2] = c;
+                for (int t = 0; t < 3; ++t) kv[3 * nk + t] = k[t];
+                ++nk;
+            }
+    const size_t nkz = (size_t)(nk > 0 ? nk : 1);
+    double* Sre = calloc(nkz, sizeof(double));
+    double* Sim = calloc(nkz, sizeof(double));
+    double* ph = malloc(sizeof(double) * 2 * (size_t)(wx + wy + wz) * (size_t)(N > 0 ? N : 1));
+    /* e^{-2 pi i m_a u_a} per particle and axis wavenumber */
+    for (int j = 0; j < N; ++j) {
+        double u[3];
+        box_fraction3(&b, &postype[4 * j], u);
+        double* pj = ph + 2 * (size_t)(wx + wy + wz) * j;
+        for (int a = -mm[0]; a <= mm[0]; ++a) {
+            pj[2 * (a + mm[0])] = cos(2 * ORACLE_PI * a * u[0]);
+            pj[2 * (a + mm[0]) + 1] = -sin(2 * ORACLE_PI * a * u[0]);
+        }
+        for (int bb = -mm[1]; bb <= mm[1]; ++bb) {
+            pj[2 * (wx + bb + mm[1])] = cos(2 * ORACLE_PI * bb * u[1]);
+            pj[2 * (wx + bb + mm[1]) + 1] = -sin(2 * ORACLE_PI * bb * u[1]);
+        }
+        for (int c = 0; c <= mm[2]; ++c) {
+            pj[2 * (wx + wy + c)] = cos(2 * ORACLE_PI * c * u[2]);
+            pj[2 * (wx + wy + c) + 1] = -sin(2 * ORACLE_PI * c * u[2]);
+        }
+    }
+#pragma omp parallel for schedule(static)
+    for (long k = 0; k < nk; ++k) {
+        double sr = 0, si = 0;
+        for (int j = 0; j < N; ++j) {
+            const double* pj = ph + 2 * (size_t)(wx + wy + wz) * j;
+            const double* ex = &pj[2 * (modes[3 * k] + mm[0])];
+            const double* ey = &pj[2 * (wx + modes[3 * k + 1] + mm[1])];
+            const double* ez = &pj[2 * (wx + wy + modes[3 * k + 2])];
+            const double tr = ey[0] * ez[0] - ey[1] * ez[1], ti = ey[0] * ez[1] + ey[1] * ez[0];
+            sr += q[j] * (tr * ex[0] - ti * ex[1]);
+            si += q[j] * (tr * ex[1] + ti * ex[0]);
+        }
+        Sre[k] = sr;
+        Sim[k] = si;
+    }
+    double Ek = 0.0, Wk[6] = {0, 0, 0, 0, 0, 0};
+    double* gk = malloc(sizeof(double) * nkz);
+    for (long k = 0; k < nk; ++k) {
+        const double* kk = &kv[3 * k];
+        const double k2 = kk[0] * kk[0] + kk[1] * kk[1] + kk[2] * kk[2];
+        gk[k] = exp(-k2 / (4 * alpha * alpha)) / k2;
+        /* half space: factor 2 of 1 / (2V) */
+        const double e = gk[k] * (Sre[k] * Sre[k] + Sim[k] * Sim[k]) / V;
+        Ek += e;
+        const double fac = -2.0 / k2 * (1.0 + k2 / (4 * alpha * alpha));
+        Wk[0] += e * (1.0 + fac * kk[0] * kk[0]);
+        Wk[1] += e * (fac * kk[0] * kk[1]);
+        Wk[2] += e * (fac * kk[0] * kk[2]);
+        Wk[3] += e * (1.0 + fac * kk[1] * kk[1]);
+        Wk[4] += e * (fac * kk[1] * kk[2]);
+        Wk[5] += e * (1.0 + fac * kk[2] * kk[2]);
+    }
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < N; ++j) {
+        const double* pj = ph + 2 * (size_t)(wx + wy + wz) * j;
+        double f[3] = {0, 0, 0};
+        for (long k = 0; k < nk; ++k) {
+            const double* ex = &pj[2 * (modes[3 * k] + mm[0])];
+            const double* ey = &pj[2 * (wx + modes[3 * k + 1] + mm[1])];
+            const double* ez = &pj[2 * (wx + wy + modes[3 * k + 2])];
+            /* e^{+i k r_j} = conj of the stored phases */
+            const double tr = ey[0] * ez[0] - ey[1] * ez[1], ti = ey[0] * ez[1] + ey[1] * ez[0];
+            const double er = tr * ex[0] - ti * ex[1], ei = -(tr * ex[1] + ti * ex[0]);
+            const double im = gk[k] * (ei * Sre[k] + er * Sim[k]);
+            for (int a = 0; a < 3; ++a) f[a] += im * kv[3 * k + a];
+        }
+        const double sc = 2.0 * q[j] / V;
+        for (int a = 0; a < 3; ++a) force[3 * j + a] += sc * f[a];
+    }
+    double q2 = 0, qs = 0;
+    for (int j = 0; j < N; ++j) {
+        q2 += q[j] * q[j];
+        qs += q[j];
+    }
+    const double Eself = -alpha / sqrt(ORACLE_PI) * ORACLE_INV_4PI * q2;
+    const double Ebg = -qs * qs / (8.0 * V * alpha * alpha);
+    if (energy) *energy = E + Ek + Eself + Ebg;
+    if (virial) {
+        for (int k = 0; k < 6; ++k) virial[k] = W[k] + Wk[k];
+        virial[0] += Ebg;
+        virial[3] += Ebg;
+        virial[5] += Ebg;
+    }
+    free(modes);
+    free(kv);
+    free(Sre);
+    free(Sim);
+    free(ph);
+    free(gk);
+    return 0;
+}
+
 /* OpenMP threads the oracle's parallel loops use */
 int oracle_num_threads(void)
 {
@@ -568,6 +770,7 @@ typedef struct {
     unsigned int ex_pitch;
     double k_cut;          /* spherical reciprocal-space cutoff: G = 0 for |k| > k_cut (0 = none) */
     double alpha;          /* screening: G(k) carries exp(-alpha |k|) (ESPForceComputeGPU.cu:199); 0 = none */
+    double tilt[3];        /* HOOMD tilt factors xy, xz, yz (0 = orthorhombic) */
 } oracle_esp_in;
 
 typedef struct {
@@ -739,14 +942,20 @@ int oracle_esp(const oracle_esp_in* in, oracle_esp_out* out)
     const double V = L[0] * L[1] * L[2];
     const double h[3] = {L[0] / nx, L[1] / ny, L[2] / nz};
     const int nm[3] = {nx, ny, nz};
+    const obox_t box = {{L[0], L[1], L[2]}, in->tilt[0], in->tilt[1], in->tilt[2]};
+    const int tilted = (box.xy != 0.0 || box.xz != 0.0 || box.yz != 0.0);
+    /* the pair sum of a tilted box takes the nearest image of every pair */
+    if (tilted && in->rc >= box_half_width(&box)) return -1;
 
     /* ---- assignParticles: spread q_j W(m - g_j) ---- */
     int* i0 = malloc(sizeof(int) * 3 * (size_t)N);
     double* wts = malloc(sizeof(double) * 3 * (size_t)N * P);
     double* dwts = malloc(sizeof(double) * 3 * (size_t)N * P);
-    for (int j = 0; j < N; ++j)
+    for (int j = 0; j < N; ++j) {
+        double uj[3];
+        box_fraction3(&box, &in->postype[4 * j], uj); /* makeFraction for a box centred at 0 */
         for (int a = 0; a < 3; ++a) {
-            double frac = in->postype[4 * j + a] / L[a] + 0.5; /* makeFraction for a box centred at 0 */
+            double frac = uj[a];
             frac -= floor(frac);
             const double g = frac * nm[a];
             const double o = ceil(g - 0.5 * P);
@@ -760,6 +969,7 @@ int oracle_esp(const oracle_esp_in* in, oracle_esp_out* out)
                 dwts[((size_t)j * 3 + a) * P + k] = dw;
             }
         }
+    }
     double complex* mesh = calloc(M, sizeof(double complex));
     for (int j = 0; j < N; ++j) {
         const double* wx = &wts[((size_t)j * 3 + 0) * P];
@@ -797,8 +1007,9 @@ int oracle_esp(const oracle_esp_in* in, oracle_esp_out* out)
         for (int y = 0; y < ny; ++y)
             for (int x = 0; x < nx; ++x) {
                 const int kx = (x <= nx / 2) ? x : x - nx, ky = (y <= ny / 2) ? y : y - ny, kz = (z <= nz / 2) ? z : z - nz;
-                const double a = 2 * ORACLE_PI * kx / L[0], b = 2 * ORACLE_PI * ky / L[1], c = 2 * ORACLE_PI * kz / L[2];
-                k2s[((size_t)z * ny + y) * nx + x] = a * a + b * b + c * c;
+                double kv[3];
+                box_wavevector(&box, kx, ky, kz, kv);
+                k2s[((size_t)z * ny + y) * nx + x] = kv[0] * kv[0] + kv[1] * kv[1] + kv[2] * kv[2];
             }
     double* uniq = malloc(sizeof(double) * M);
     memcpy(uniq, k2s, sizeof(double) * M);
@@ -843,7 +1054,8 @@ int oracle_esp(const oracle_esp_in* in, oracle_esp_out* out)
                 if (out->green) out->green[m] = G;
                 const double s2 = creal(mesh[m]) * creal(mesh[m]) + cimag(mesh[m]) * cimag(mesh[m]);
                 const int kx = (x <= nx / 2) ? x : x - nx, ky = (y <= ny / 2) ? y : y - ny, kz = (z <= nz / 2) ? z : z - nz;
-                const double k3[3] = {2 * ORACLE_PI * kx / L[0], 2 * ORACLE_PI * ky / L[1], 2 * ORACLE_PI * kz / L[2]};
+                double k3[3];
+                box_wavevector(&box, kx, ky, kz, k3);
                 const double e = 0.5 / V * s2;
                 Erec += e * G;
                 Wrec[0] += e * (G + Gv * k3[0] * k3[0]);
@@ -886,28 +1098,42 @@ int oracle_esp(const oracle_esp_in* in, oracle_esp_out* out)
                 }
             }
         }
-        out->force[4 * j + 0] = in->q[j] * ex / h[0];
-        out->force[4 * j + 1] = in->q[j] * ey / h[1];
-        out->force[4 * j + 2] = in->q[j] * ez / h[2];
+        /* gradient along the mesh axes -> Cartesian: d/dr = B^-T d/du */
+        const double gx = ex / h[0], gy = ey / h[1], gz = ez / h[2];
+        out->force[4 * j + 0] = in->q[j] * gx;
+        out->force[4 * j + 1] = in->q[j] * (gy - box.xy * gx);
+        out->force[4 * j + 2] = in->q[j] * (gz - box.yz * gy + (box.xy * box.yz - box.xz) * gx);
     }
 
     /* ---- computeRealSpaceCorrection: pair sum of q_i q_j L(r) over the pairs within rc (cell
      * list; rc <= L/2, so each pair is its nearest image), then fixExclusions for the excluded
      * pairs at any distance ---- */
     cells_t cl;
-    cells_build(&cl, N, in->postype, L, in->rc);
+    if (!tilted) cells_build(&cl, N, in->postype, L, in->rc);
 #pragma omp parallel for schedule(dynamic, 64)
     for (int i = 0; i < N; ++i) {
         esp_pair_ctx c;
         memset(&c, 0, sizeof c);
         c.in = in;
         c.i = i;
-        double ri[3];
-        for (int a = 0; a < 3; ++a) {
-            double f = in->postype[4 * i + a] / L[a] + 0.5;
-            ri[a] = (f - floor(f) - 0.5) * L[a];
+        if (!tilted) {
+            double ri[3];
+            for (int a = 0; a < 3; ++a) {
+                double f = in->postype[4 * i + a] / L[a] + 0.5;
+                ri[a] = (f - floor(f) - 0.5) * L[a];
+            }
+            cells_for_neighbours(&cl, in->postype, L, i, ri, in->rc, esp_pair_term, &c);
+        } else {
+            /* tilted box: every pair at its nearest image (rc below half the smallest width) */
+            for (int j = 0; j < N; ++j) {
+                if (j == i) continue;
+                double d[3] = {in->postype[4 * i] - in->postype[4 * j], in->postype[4 * i + 1] - in->postype[4 * j + 1],
+                               in->postype[4 * i + 2] - in->postype[4 * j + 2]};
+                min_image_box(d, &box);
+                const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+                if (r2 < in->rc * in->rc) esp_pair_term(&c, j, d, r2);
+            }
         }
-        cells_for_neighbours(&cl, in->postype, L, i, ri, in->rc, esp_pair_term, &c);
         if (in->n_ex)
             for (unsigned int k = 0; k < in->n_ex[i]; ++k) {
                 /* fixExclusions (ESPForceComputeGPU.cu:441-458): remove the mesh's M(r) = 1/(4 pi r) - L(r)
@@ -915,7 +1141,7 @@ int oracle_esp(const oracle_esp_in* in, oracle_esp_out* out)
                 const unsigned int j = in->ex_list[(size_t)i * in->ex_pitch + k];
                 double d[3] = {in->postype[4 * i] - in->postype[4 * j], in->postype[4 * i + 1] - in->postype[4 * j + 1],
                                in->postype[4 * i + 2] - in->postype[4 * j + 2]};
-                min_image(d, L);
+                min_image_box(d, &box);
                 const double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
                 if (r2 < 1.0e-20) continue;
                 const double r = sqrt(r2);
@@ -933,7 +1159,7 @@ int oracle_esp(const oracle_esp_in* in, oracle_esp_out* out)
         out->force[4 * i + 3] += c.e;
         for (int k = 0; k < 6; ++k) out->virial[(size_t)k * out->virial_pitch + i] += c.v[k];
     }
-    cells_free(&cl);
+    if (!tilted) cells_free(&cl);
 
     /* ---- self and neutralising-background terms ---- */
     double q2 = 0, qs = 0;

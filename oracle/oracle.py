@@ -24,7 +24,7 @@ class EspIn(C.Structure):
         ("P", C.c_int), ("D", C.c_int), ("wcoef", _DP), ("n_segs", C.c_int), ("screen", _DP), ("dscreen", _DP),
         ("r_lo", _DP), ("dr_inv", _DP), ("rc", C.c_double), ("split_kind", C.c_int), ("kappa", C.c_double),
         ("n_ex", C.POINTER(C.c_uint)), ("ex_list", C.POINTER(C.c_uint)), ("ex_pitch", C.c_uint),
-        ("k_cut", C.c_double), ("alpha", C.c_double),
+        ("k_cut", C.c_double), ("alpha", C.c_double), ("tilt", C.c_double * 3),
     ]
 
 
@@ -57,6 +57,8 @@ def load():
     lib.oracle_window_eval.restype = None
     lib.oracle_split_F.argtypes = [C.POINTER(EspIn), C.c_double, _DP, _DP]
     lib.oracle_split_F.restype = None
+    lib.oracle_ewald_box.argtypes = [C.c_int, _DP, _DP, _DP, C.c_double, C.c_double, C.c_double, _DP, _DP, _DP]
+    lib.oracle_ewald_box.restype = C.c_int
     lib.oracle_num_threads.argtypes = []
     lib.oracle_num_threads.restype = C.c_int
     _lib = lib
@@ -116,6 +118,41 @@ def ewald(pos, q, box, alpha, rcut, kcut, targets=None, energy=True):
     return f, (e[0] if energy else None), (w if energy else None)
 
 
+def box_half_width(box):
+    """Half the smallest perpendicular width of a HOOMD box (Lx, Ly, Lz, xy, xz, yz)."""
+    Lx, Ly, Lz, xy, xz, yz = (list(box) + [0.0, 0.0, 0.0])[:6]
+    n1 = np.sqrt(1.0 + xy * xy + (xy * yz - xz) ** 2) / Lx
+    n2 = np.sqrt(1.0 + yz * yz) / Ly
+    return 0.5 / max(n1, n2, 1.0 / Lz)
+
+
+def ewald_box(pos, q, box, alpha, rcut, kcut):
+    """FP64 Ewald sum in a general (triclinic) HOOMD box (Lx, Ly, Lz, xy, xz, yz): nearest-image
+    real space (rcut below half the smallest width), k = 2 pi B^-T m.  Returns (forces, energy,
+    virial)."""
+    pos = np.asarray(pos, np.float64)
+    N = len(pos)
+    pt = np.zeros((N, 4))
+    pt[:, :3] = pos
+    q = np.ascontiguousarray(q, np.float64)
+    b = np.ascontiguousarray((list(box) + [0.0, 0.0, 0.0])[:6], np.float64)
+    f = np.zeros((N, 3))
+    e = np.zeros(1)
+    w = np.zeros(6)
+    if load().oracle_ewald_box(N, _p(pt), _p(q), _p(b), alpha, rcut, kcut, _p(f), _p(e), _p(w)) != 0:
+        raise ValueError("rcut must be below half the smallest perpendicular width of the box")
+    return f, e[0], w
+
+
+def ewald_box_auto(pos, q, box, tol=1e-15, rcut=None):
+    """ewald_box with parameters chosen for ~tol relative accuracy."""
+    if rcut is None:
+        rcut = 0.99 * box_half_width(box)
+    s = np.sqrt(-np.log(tol))
+    alpha = s / rcut
+    return ewald_box(pos, q, box, alpha, rcut, 2.0 * alpha * s)
+
+
 def ewald_auto(pos, q, box, tol=1e-15, targets=None, energy=True, rcut=None):
     """Ewald with parameters chosen for ~tol relative accuracy."""
     L = np.asarray(box[:3], np.float64)
@@ -132,6 +169,7 @@ class OracleESP:
 
     def __init__(self, box, mesh, P, wcoef, screen, dscreen, r_lo, dr_inv, rc, split_kind=0, kappa=0.0, k_cut=0.0, alpha=0.0):
         self.L = np.asarray(box[:3], np.float64)
+        self.tilt = np.asarray((list(box) + [0.0, 0.0, 0.0])[3:6], np.float64)
         self.mesh = tuple(int(m) for m in mesh)
         self.P = int(P)
         self.wcoef = np.ascontiguousarray(wcoef, np.float64)
@@ -154,6 +192,7 @@ class OracleESP:
         for a in range(3):
             s.L[a] = self.L[a]
             s.n[a] = self.mesh[a]
+            s.tilt[a] = self.tilt[a]
         s.P, s.D = self.P, self.D
         s.wcoef = _p(self.wcoef)
         s.n_segs = len(self.r_lo)
@@ -200,7 +239,8 @@ class OracleESP:
         if want_mesh:
             mesh, phi, green = np.zeros(M), np.zeros(M), np.zeros(M)
             out.mesh, out.phi, out.green = _p(mesh), _p(phi), _p(green)
-        load().oracle_esp(C.byref(sin), C.byref(out))
+        if load().oracle_esp(C.byref(sin), C.byref(out)) != 0:
+            raise ValueError("in a tilted box rc must be below half the smallest perpendicular width")
         res = {"force": force, "virial": vir, "ext_energy": out.ext_energy, "ext_virial": np.array(out.ext_virial[:])}
         if want_mesh:
             shp = (self.mesh[2], self.mesh[1], self.mesh[0])

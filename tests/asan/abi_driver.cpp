@@ -110,8 +110,8 @@ static void context_host_paths()
     esp_params bad = make_params(64, 1, 2.0, 3.0, ESP_PREC_FP64);
     CHECK(esp_create(&ctx, &bad) != ESP_OK && ctx == nullptr);
     bad = make_params(64, 8, 2.0, 3.0, ESP_PREC_FP64);
-    bad.alpha = 0.5;
-    CHECK(esp_create(&ctx, &bad) == ESP_ERR_UNSUPPORTED);
+    bad.alpha = -0.5;
+    CHECK(esp_create(&ctx, &bad) == ESP_ERR_ARG);
     CHECK(std::strlen(esp_last_error(nullptr)) > 0);
 
     esp_params p = make_params(64, 8, 3.0, 4.0, ESP_PREC_FP64);
@@ -242,6 +242,29 @@ static void gpu_paths()
     int lowered = 1;
     OK(esp_get_fixed_point_scale(ctx, &fx, &lowered));
     CHECK(fx > 0 && lowered == 0);
+
+    // tilted box with screening, mesh part only: repeatable bits, finite forces
+    {
+        esp_params pt = make_params(40, 8, 3.0, 3.5, ESP_PREC_FP64);
+        pt.alpha = 0.3;
+        esp_ctx* ctt = nullptr;
+        OK(esp_create(&ctt, &pt));
+        esp_system st = sys;
+        st.box[3] = 0.25;
+        st.box[4] = -0.1;
+        st.box[5] = 0.2;
+        esp_out ot = out;
+        ot.flags = ESP_COMPUTE_MESH | ESP_COMPUTE_ENERGY_VIRIAL;
+        OK(esp_compute_forces(ctt, 0, &st, &ot, nullptr));
+        std::vector<double> t1(4 * N), t2(4 * N);
+        OK(esp_memcpy_dtoh(t1.data(), d_f.p, t1.size() * 8));
+        OK(esp_compute_forces(ctt, 1, &st, &ot, nullptr));
+        OK(esp_memcpy_dtoh(t2.data(), d_f.p, t2.size() * 8));
+        CHECK(all_finite(t1) && std::memcmp(t1.data(), t2.data(), t1.size() * 8) == 0);
+        st.box[4] = std::nan("");
+        CHECK(esp_compute_forces(ctt, 2, &st, &ot, nullptr) == ESP_ERR_ARG);
+        OK(esp_destroy(ctt));
+    }
 
     // batch of this context and a second FP64 one (another mesh and order): each member equals
     // its own evaluation bit for bit; the outputs are cleared first

@@ -54,6 +54,41 @@ def random_electrolyte(N, L, seed=0, min_dist=0.0, q2_frac=0.0):
     return pos, q
 
 
+def box_matrix(box):
+    """Columns a1, a2, a3 of a HOOMD box (Lx, Ly, Lz, xy, xz, yz)."""
+    Lx, Ly, Lz, xy, xz, yz = (list(box) + [0.0, 0.0, 0.0])[:6]
+    return np.array([[Lx, xy * Ly, xz * Lz], [0.0, Ly, yz * Lz], [0.0, 0.0, Lz]])
+
+
+def box_from_matrix(B):
+    """(Lx, Ly, Lz, xy, xz, yz) of an upper-triangular lattice matrix."""
+    Lx, Ly, Lz = B[0, 0], B[1, 1], B[2, 2]
+    return (Lx, Ly, Lz, B[0, 1] / Ly, B[0, 2] / Lz, B[1, 2] / Lz)
+
+
+def random_electrolyte_box(N, box, seed=0, min_dist=0.0):
+    """Neutral random +-1 charges uniform in a (tilted) HOOMD box centred on 0."""
+    from esp_hoomd_amd.nlist import min_image_tilted
+    rng = np.random.default_rng(seed)
+    B = box_matrix(box)
+    L = np.asarray(box[:3], float)
+    tilt = np.asarray((list(box) + [0.0, 0.0, 0.0])[3:6], float)
+    u = rng.uniform(0.0, 1.0, (N, 3))
+    for _ in range(500):
+        pos = (u - 0.5) @ B.T
+        if min_dist <= 0:
+            break
+        d = min_image_tilted(pos[:, None, :] - pos[None, :, :], L, tilt)
+        r2 = (d * d).sum(-1) + np.eye(N) * 1e9
+        bad = np.unique(np.nonzero(r2 < min_dist * min_dist)[1])
+        if len(bad) == 0:
+            break
+        u[bad] = rng.uniform(0.0, 1.0, (len(bad), 3))
+    q = np.where(np.arange(N) % 2 == 0, 1.0, -1.0)
+    rng.shuffle(q)
+    return pos, q
+
+
 def rel_rms(a, b):
     a = np.asarray(a)
     b = np.asarray(b)
