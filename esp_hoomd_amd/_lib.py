@@ -182,10 +182,19 @@ def load() -> C.CDLL:
     _sig(lib, "esp_md_batch_last_error", C.c_char_p, [_VP])
     _sig(lib, "esp_md_batch_set", C.c_int, [_VP, C.c_uint, C.POINTER(EspSystem), C.POINTER(_VP), C.POINTER(EspOut), C.POINTER(_VP),
                                           C.POINTER(C.c_uint64)])
+    _sig(lib, "esp_md_batch_set_async", C.c_int, [_VP, C.c_uint, C.POINTER(EspSystem), C.POINTER(_VP), C.POINTER(EspOut),
+                                                C.POINTER(_VP), C.POINTER(C.c_uint64), _VP])
+    _sig(lib, "esp_md_batch_set_hold", C.c_int, [_VP, C.POINTER(C.c_ubyte), _VP])
     _sig(lib, "esp_md_batch_integrate_first", C.c_int, [_VP, C.c_double, _VP])
     _sig(lib, "esp_md_batch_integrate_second", C.c_int, [_VP, C.c_double, C.c_double, C.c_double, C.c_uint64, _VP])
     _sig(lib, "esp_md_batch_wca", C.c_int, [_VP, C.c_double, C.c_double, _VP])
     _sig(lib, "esp_md_batch_update_nlists", C.c_int, [_VP, C.POINTER(C.c_uint), C.POINTER(C.c_ubyte), _VP])
+    _sig(lib, "esp_nlist_set_capacity", C.c_int, [_VP, C.c_uint, C.c_uint])
+    _sig(lib, "esp_nlist_set_cutoff", C.c_int, [_VP, C.c_double])
+    _sig(lib, "esp_md_batch_predict", C.c_int, [_VP, C.c_double, _VP])
+    _sig(lib, "esp_md_batch_read_predicted", C.c_int, [_VP, C.POINTER(C.c_ubyte)])
+    _sig(lib, "esp_md_batch_rebuild_async", C.c_int, [_VP, C.POINTER(C.c_ubyte), C.POINTER(C.c_ubyte), _VP])
+    _sig(lib, "esp_md_batch_finish_rebuilds", C.c_int, [_VP, C.POINTER(C.c_ubyte), C.POINTER(C.c_uint)])
     _sig(lib, "esp_device_count", C.c_int, [C.POINTER(C.c_int)])
     _sig(lib, "esp_set_device", C.c_int, [C.c_int])
     _sig(lib, "esp_device_malloc", C.c_int, [C.POINTER(_VP), C.c_size_t])
@@ -193,6 +202,9 @@ def load() -> C.CDLL:
     _sig(lib, "esp_memcpy_htod", C.c_int, [_VP, _VP, C.c_size_t])
     _sig(lib, "esp_memcpy_dtoh", C.c_int, [_VP, _VP, C.c_size_t])
     _sig(lib, "esp_memcpy_dtod", C.c_int, [_VP, _VP, C.c_size_t, _VP])
+    _sig(lib, "esp_memcpy_htod_async", C.c_int, [_VP, _VP, C.c_size_t, _VP])
+    _sig(lib, "esp_host_malloc_pinned", C.c_int, [C.POINTER(_VP), C.c_size_t])
+    _sig(lib, "esp_host_free_pinned", C.c_int, [_VP])
     _sig(lib, "esp_memset", C.c_int, [_VP, C.c_int, C.c_size_t])
     _sig(lib, "esp_copy_records", C.c_int, [_VP, _VP, C.c_uint, _VP])
     _sig(lib, "esp_stream_create", C.c_int, [C.POINTER(_VP)])

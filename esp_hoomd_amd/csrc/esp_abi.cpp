@@ -681,6 +681,24 @@ int esp_memcpy_dtoh(void* dst, const void* src, size_t bytes)
     return hip_status(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
     }
 
+int esp_memcpy_htod_async(void* dst, const void* src, size_t bytes, esp_stream_t stream)
+    {
+    return hip_status(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, static_cast<hipStream_t>(stream)));
+    }
+
+int esp_host_malloc_pinned(void** ptr, size_t bytes)
+    {
+    if (!ptr)
+        return ESP_ERR_ARG;
+    *ptr = nullptr;
+    return hip_status(hipHostMalloc(ptr, bytes ? bytes : 1, hipHostMallocDefault));
+    }
+
+int esp_host_free_pinned(void* ptr)
+    {
+    return ptr ? hip_status(hipHostFree(ptr)) : ESP_OK;
+    }
+
 int esp_memcpy_dtod(void* dst, const void* src, size_t bytes, esp_stream_t stream)
     {
     return hip_status(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, static_cast<hipStream_t>(stream)));

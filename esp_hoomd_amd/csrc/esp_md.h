@@ -56,11 +56,14 @@ struct MDItem
     unsigned int chunk;
     };
 
-hipError_t batch_vv_first(const MDSys* d, const MDItem* items, unsigned int n_items, double dt, hipStream_t s);
+//! hold (device, one byte per system, or null): systems with hold[i] != 0 are skipped
+hipError_t batch_vv_first(const MDSys* d, const MDItem* items, unsigned int n_items, double dt, const unsigned char* hold, hipStream_t s);
 hipError_t batch_vv_second(const MDSys* d, const MDItem* items, unsigned int n_items, double dt, double gamma, double kT, uint64_t step,
-                           hipStream_t s);
+                           const unsigned char* hold, hipStream_t s);
 hipError_t batch_wca(const MDSys* d, const MDItem* items, unsigned int n_items, double sigma, double epsilon, hipStream_t s);
 hipError_t batch_max_disp(const MDSys* d, const MDItem* items, unsigned int n_items, unsigned long long* out, hipStream_t s);
+hipError_t batch_predict_disp(const MDSys* d, const MDItem* items, unsigned int n_items, double dt, unsigned long long* out,
+                              hipStream_t s);
 
 hipError_t cell_count(unsigned int N, const double* pos, const CellGeom& cg, unsigned int* count, unsigned int* cell, unsigned int* slot,
                       hipStream_t s);

@@ -32,6 +32,7 @@ struct esp_nlist
     unsigned int* h_over = nullptr;  // pinned: overflow counts of the last build (full, inner)
     esp::md::CellGeom cg {};
     bool want_inner = false;
+    bool fixed_capacity = false;     // row capacities set by esp_nlist_set_capacity (not from the density)
     std::string last_error;
     ~esp_nlist()
         {
@@ -117,9 +118,10 @@ void build_prepare(esp_nlist* nl, const esp_system* sys)
     const double rho = double(N) / (cg.Lx * cg.Ly * cg.Lz);
     const double sphere = 4.18879020478639 * r_list * r_list * r_list;
     const unsigned int est = (unsigned(1.5 * rho * sphere) + 32u + 7u) & ~7u;
-    nl->nmax = std::max(nl->nmax, std::min(est, std::max(N, 8u)));
+    if (!nl->fixed_capacity)
+        nl->nmax = std::max(nl->nmax, std::min(est, std::max(N, 8u)));
     nl->want_inner = nl->r_inner > 0.0 && nl->r_inner < nl->r_cut;
-    if (nl->want_inner)
+    if (nl->want_inner && !nl->fixed_capacity)
         {
         const double ri = nl->r_inner + nl->r_buff;
         const unsigned int est_in = (unsigned(1.5 * rho * 4.18879020478639 * ri * ri * ri) + 16u + 7u) & ~7u;
@@ -239,9 +241,28 @@ struct esp_md_batch
     std::vector<esp_nlist*> nls;
     std::vector<esp::md::MDSys> desc;
     std::vector<esp::md::MDItem> items;
+    esp::DeviceBuffer d_disp_pred;  // predicted displacements of the last esp_md_batch_predict
     esp::DeviceBuffer d_desc, d_items, d_disp;
     std::vector<unsigned long long> h_disp;
+    unsigned long long* h_pred = nullptr;  // pinned copy of d_disp_pred
+    unsigned int pred_cap = 0;
+    std::vector<unsigned int> pending;      // members whose asynchronous rebuild awaits its check
+    void* h_stage = nullptr;                // pinned staging of descriptor and item uploads
+    size_t stage_cap = 0;
+    esp::DeviceBuffer d_hold;               // per member: 1 = not integrated this step
+    unsigned char* h_hold = nullptr;        // pinned staging of d_hold
+    unsigned int hold_cap = 0;
+    bool any_hold = false;
     std::string last_error;
+    ~esp_md_batch()
+        {
+        if (h_pred)
+            (void)hipHostFree(h_pred);
+        if (h_stage)
+            (void)hipHostFree(h_stage);
+        if (h_hold)
+            (void)hipHostFree(h_hold);
+        }
     };
 
 namespace
@@ -334,14 +355,32 @@ const char* esp_md_batch_last_error(const esp_md_batch* b)
     return b ? b->last_error.c_str() : g_md_error.c_str();
     }
 
+static int md_batch_set_impl(esp_md_batch* b, unsigned int n, const esp_system* systems, double* const* vel, const esp_out* outs,
+                             esp_nlist* const* nls, const uint64_t* seeds, bool async, hipStream_t stream);
+
 int esp_md_batch_set(esp_md_batch* b, unsigned int n, const esp_system* systems, double* const* vel, const esp_out* outs,
                      esp_nlist* const* nls, const uint64_t* seeds)
+    {
+    return md_batch_set_impl(b, n, systems, vel, outs, nls, seeds, false, nullptr);
+    }
+
+int esp_md_batch_set_async(esp_md_batch* b, unsigned int n, const esp_system* systems, double* const* vel, const esp_out* outs,
+                           esp_nlist* const* nls, const uint64_t* seeds, esp_stream_t stream)
+    {
+    return md_batch_set_impl(b, n, systems, vel, outs, nls, seeds, true, static_cast<hipStream_t>(stream));
+    }
+
+static int md_batch_set_impl(esp_md_batch* b, unsigned int n, const esp_system* systems, double* const* vel, const esp_out* outs,
+                             esp_nlist* const* nls, const uint64_t* seeds, bool async, hipStream_t stream)
     {
     if (!b || (n && (!systems || !vel || !outs)))
         return md_batch_fail(b, ESP_ERR_ARG, "esp_md_batch_set: null argument");
     return md_batch_guard(b,
                           [&]
                           {
+                              if (!b->pending.empty())
+                                  throw esp::EspError(ESP_ERR_STATE, "esp_md_batch_set: asynchronous rebuilds are not finished");
+                              b->any_hold = false;
                               b->sys.assign(systems, systems + n);
                               b->nls.assign(n, nullptr);
                               b->desc.assign(n, esp::md::MDSys {});
@@ -373,12 +412,72 @@ int esp_md_batch_set(esp_md_batch* b, unsigned int n, const esp_system* systems,
                               b->d_items.resize(std::max<size_t>(b->items.size(), 1) * sizeof(esp::md::MDItem));
                               b->d_disp.resize(std::max<size_t>(n, 1) * 8);
                               b->h_disp.assign(n, 0ull);
-                              if (n)
-                                  md_batch_upload(b);
-                              if (!b->items.empty())
-                                  esp::check_hip(hipMemcpy(b->d_items.get(), b->items.data(), b->items.size() * sizeof(esp::md::MDItem),
-                                                           hipMemcpyHostToDevice),
-                                                 "hipMemcpy items");
+                              const size_t dbytes = size_t(n) * sizeof(esp::md::MDSys);
+                              const size_t ibytes = b->items.size() * sizeof(esp::md::MDItem);
+                              if (!async)
+                                  {
+                                  if (n)
+                                      md_batch_upload(b);
+                                  if (ibytes)
+                                      esp::check_hip(hipMemcpy(b->d_items.get(), b->items.data(), ibytes, hipMemcpyHostToDevice),
+                                                     "hipMemcpy items");
+                                  return;
+                                  }
+                              // through pinned staging on the caller's stream: the caller synchronises
+                              // with that stream before the next set of this batch
+                              if (b->stage_cap < dbytes + ibytes)
+                                  {
+                                  if (b->h_stage)
+                                      esp::check_hip(hipHostFree(b->h_stage), "hipHostFree");
+                                  b->h_stage = nullptr;
+                                  b->stage_cap = 0;
+                                  esp::check_hip(hipHostMalloc(&b->h_stage, (dbytes + ibytes) * 2, hipHostMallocDefault), "hipHostMalloc");
+                                  b->stage_cap = (dbytes + ibytes) * 2;
+                                  }
+                              unsigned char* st = static_cast<unsigned char*>(b->h_stage);
+                              if (dbytes)
+                                  {
+                                  std::memcpy(st, b->desc.data(), dbytes);
+                                  esp::check_hip(hipMemcpyAsync(b->d_desc.get(), st, dbytes, hipMemcpyHostToDevice, stream),
+                                                 "hipMemcpyAsync descriptors");
+                                  }
+                              if (ibytes)
+                                  {
+                                  std::memcpy(st + dbytes, b->items.data(), ibytes);
+                                  esp::check_hip(hipMemcpyAsync(b->d_items.get(), st + dbytes, ibytes, hipMemcpyHostToDevice, stream),
+                                                 "hipMemcpyAsync items");
+                                  }
+                          });
+    }
+
+int esp_md_batch_set_hold(esp_md_batch* b, const unsigned char* hold, esp_stream_t stream)
+    {
+    if (!b)
+        return ESP_ERR_ARG;
+    return md_batch_guard(b,
+                          [&]
+                          {
+                              const unsigned int n = unsigned(b->desc.size());
+                              bool any = false;
+                              for (unsigned int i = 0; hold && i < n; ++i)
+                                  any = any || hold[i] != 0;
+                              b->any_hold = any;
+                              if (!any)
+                                  return;
+                              if (b->hold_cap < n)
+                                  {
+                                  if (b->h_hold)
+                                      esp::check_hip(hipHostFree(b->h_hold), "hipHostFree");
+                                  b->h_hold = nullptr;
+                                  b->hold_cap = 0;
+                                  esp::check_hip(hipHostMalloc(reinterpret_cast<void**>(&b->h_hold), size_t(n) * 2, hipHostMallocDefault),
+                                                 "hipHostMalloc");
+                                  b->hold_cap = n * 2;
+                                  }
+                              b->d_hold.resize(std::max<size_t>(n, 1));
+                              std::memcpy(b->h_hold, hold, n);
+                              esp::check_hip(hipMemcpyAsync(b->d_hold.get(), b->h_hold, n, hipMemcpyHostToDevice, static_cast<hipStream_t>(stream)),
+                                             "hipMemcpyAsync hold");
                           });
     }
 
@@ -390,7 +489,8 @@ int esp_md_batch_integrate_first(esp_md_batch* b, double dt, esp_stream_t stream
                           [&]
                           {
                               esp::check_hip(esp::md::batch_vv_first(b->d_desc.as<esp::md::MDSys>(), b->d_items.as<esp::md::MDItem>(),
-                                                                     unsigned(b->items.size()), dt, static_cast<hipStream_t>(stream)),
+                                                                     unsigned(b->items.size()), dt, b->any_hold ? b->d_hold.as<unsigned char>() : nullptr,
+                                                                     static_cast<hipStream_t>(stream)),
                                              "batch_vv_first");
                           });
     }
@@ -404,6 +504,7 @@ int esp_md_batch_integrate_second(esp_md_batch* b, double dt, double gamma, doub
                           {
                               esp::check_hip(esp::md::batch_vv_second(b->d_desc.as<esp::md::MDSys>(), b->d_items.as<esp::md::MDItem>(),
                                                                       unsigned(b->items.size()), dt, gamma, kT, step,
+                                                                      b->any_hold ? b->d_hold.as<unsigned char>() : nullptr,
                                                                       static_cast<hipStream_t>(stream)),
                                              "batch_vv_second");
                           });
@@ -419,6 +520,154 @@ int esp_md_batch_wca(esp_md_batch* b, double sigma, double epsilon, esp_stream_t
                               esp::check_hip(esp::md::batch_wca(b->d_desc.as<esp::md::MDSys>(), b->d_items.as<esp::md::MDItem>(),
                                                                 unsigned(b->items.size()), sigma, epsilon, static_cast<hipStream_t>(stream)),
                                              "batch_wca");
+                          });
+    }
+
+int esp_md_batch_predict(esp_md_batch* b, double dt, esp_stream_t stream)
+    {
+    if (!b)
+        return ESP_ERR_ARG;
+    return md_batch_guard(b,
+                          [&]
+                          {
+                              hipStream_t s = static_cast<hipStream_t>(stream);
+                              const unsigned int n = unsigned(b->desc.size());
+                              if (n == 0)
+                                  return;
+                              if (b->pred_cap < n)
+                                  {
+                                  if (b->h_pred)
+                                      esp::check_hip(hipHostFree(b->h_pred), "hipHostFree");
+                                  b->h_pred = nullptr;
+                                  esp::check_hip(hipHostMalloc(reinterpret_cast<void**>(&b->h_pred), size_t(n) * 8, hipHostMallocDefault),
+                                                 "hipHostMalloc");
+                                  b->pred_cap = n;
+                                  }
+                              b->d_disp_pred.resize(size_t(n) * 8);
+                              esp::check_hip(hipMemsetAsync(b->d_disp_pred.get(), 0, size_t(n) * 8, s), "hipMemsetAsync");
+                              esp::check_hip(esp::md::batch_predict_disp(b->d_desc.as<esp::md::MDSys>(), b->d_items.as<esp::md::MDItem>(),
+                                                                         unsigned(b->items.size()), dt,
+                                                                         b->d_disp_pred.as<unsigned long long>(), s),
+                                             "batch_predict_disp");
+                              esp::check_hip(hipMemcpyAsync(b->h_pred, b->d_disp_pred.get(), size_t(n) * 8, hipMemcpyDeviceToHost, s),
+                                             "hipMemcpyAsync");
+                          });
+    }
+
+//! Whether member i's list must be rebuilt once its positions have squared displacement d2 since
+//! the last build (the criterion of esp_md_batch_update_nlists).
+static bool needs_rebuild(const esp_md_batch* b, unsigned int i, double d2)
+    {
+    const esp_nlist* nl = b->nls[i];
+    if (!nl)
+        return false;
+    const esp_system& sy = b->sys[i];
+    return !nl->built || sy.N != nl->N || sy.box[0] != nl->box[0] || sy.box[1] != nl->box[1] || sy.box[2] != nl->box[2]
+           || d2 >= 0.25 * nl->r_buff * nl->r_buff;
+    }
+
+int esp_md_batch_read_predicted(esp_md_batch* b, unsigned char* need)
+    {
+    if (!b || !need)
+        return ESP_ERR_ARG;
+    return md_batch_guard(b,
+                          [&]
+                          {
+                              const unsigned int n = unsigned(b->desc.size());
+                              if (n > b->pred_cap)
+                                  throw esp::EspError(ESP_ERR_STATE, "esp_md_batch_read_predicted: no prediction for this member set");
+                              for (unsigned int i = 0; i < n; ++i)
+                                  {
+                                  double d2;
+                                  std::memcpy(&d2, &b->h_pred[i], 8);
+                                  need[i] = needs_rebuild(b, i, d2) ? 1 : 0;
+                                  }
+                          });
+    }
+
+int esp_md_batch_rebuild_async(esp_md_batch* b, const unsigned char* which, unsigned char* moved, esp_stream_t stream)
+    {
+    if (!b || !which)
+        return ESP_ERR_ARG;
+    return md_batch_guard(b,
+                          [&]
+                          {
+                              hipStream_t s = static_cast<hipStream_t>(stream);
+                              const unsigned int n = unsigned(b->desc.size());
+                              if (!b->pending.empty())
+                                  throw esp::EspError(ESP_ERR_STATE, "esp_md_batch_rebuild_async: the previous rebuilds were not finished");
+                              bool changed = false;
+                              for (unsigned int i = 0; i < n; ++i)
+                                  {
+                                  if (moved)
+                                      moved[i] = 0;
+                                  esp_nlist* nl = b->nls[i];
+                                  if (!which[i] || !nl)
+                                      continue;
+                                  const esp_system& sy = b->sys[i];
+                                  const void* before[5] = {nl->n_neigh.get(), nl->nlist.get(), nl->head.get(), nl->inner.get(),
+                                                           nl->last_pos.get()};
+                                  build_prepare(nl, &sy);
+                                  build_enqueue(nl, &sy, s);
+                                  // positions at this build (the drift of the step has run): the
+                                  // next predicted check measures from them
+                                  esp::check_hip(esp::md::copy4(sy.N, sy.postype, nl->last_pos.as<double>(), s), "copy4");
+                                  // taken as built; esp_md_batch_finish_rebuilds reports an overflow
+                                  nl->inner_built = nl->want_inner;
+                                  nl->built = true;
+                                  nl->box[0] = sy.box[0];
+                                  nl->box[1] = sy.box[1];
+                                  nl->box[2] = sy.box[2];
+                                  nl->builds += 1;
+                                  b->pending.push_back(i);
+                                  const void* after[5] = {nl->n_neigh.get(), nl->nlist.get(), nl->head.get(), nl->inner.get(),
+                                                          nl->last_pos.get()};
+                                  const bool mv = std::memcmp(before, after, sizeof(before)) != 0;
+                                  if (moved)
+                                      moved[i] = mv ? 1 : 0;
+                                  esp::md::MDSys old = b->desc[i];
+                                  md_batch_refresh(b, i);
+                                  changed = changed || std::memcmp(&old, &b->desc[i], sizeof(old)) != 0;
+                                  }
+                              if (changed)
+                                  esp::check_hip(hipMemcpyAsync(b->d_desc.get(), b->desc.data(), b->desc.size() * sizeof(esp::md::MDSys),
+                                                                hipMemcpyHostToDevice, s),
+                                                 "hipMemcpyAsync descriptors");
+                          });
+    }
+
+int esp_md_batch_finish_rebuilds(esp_md_batch* b, unsigned char* overflowed, unsigned int* n_overflowed)
+    {
+    if (!b)
+        return ESP_ERR_ARG;
+    return md_batch_guard(b,
+                          [&]
+                          {
+                              const unsigned int n = unsigned(b->desc.size());
+                              unsigned int count = 0;
+                              if (overflowed)
+                                  std::memset(overflowed, 0, n);
+                              for (unsigned int i : b->pending)
+                                  {
+                                  esp_nlist* nl = b->nls[i];
+                                  const unsigned int over0 = nl->h_over[0], over1 = nl->h_over[1];
+                                  if (over0 == 0 && over1 == 0)
+                                      continue;
+                                  // the rows were too short: raise the capacities, mark the list
+                                  // unbuilt (the caller repeats the member's step with a fresh build)
+                                  if (over0)
+                                      nl->nmax = ((over0 + over0 / 4) + 7) & ~7u;
+                                  if (over1)
+                                      nl->nmax_in = ((over1 + over1 / 4) + 7) & ~7u;
+                                  nl->built = false;
+                                  nl->inner_built = false;
+                                  if (overflowed && i < n)
+                                      overflowed[i] = 1;
+                                  ++count;
+                                  }
+                              b->pending.clear();
+                              if (n_overflowed)
+                                  *n_overflowed = count;
                           });
     }
 
@@ -554,6 +803,34 @@ int esp_nlist_set_inner(esp_nlist* nl, double r_inner)
         nl->built = false;
         nl->inner_built = false;
         }
+    return ESP_OK;
+    }
+
+int esp_nlist_set_cutoff(esp_nlist* nl, double r_cut)
+    {
+    if (!nl || !(r_cut > 0.0))
+        return fail(nl, ESP_ERR_ARG, "esp_nlist_set_cutoff: r_cut must be > 0");
+    if (r_cut != nl->r_cut)
+        {
+        nl->r_cut = r_cut;
+        nl->built = false;
+        nl->inner_built = false;
+        }
+    return ESP_OK;
+    }
+
+int esp_nlist_set_capacity(esp_nlist* nl, unsigned int nmax, unsigned int nmax_inner)
+    {
+    if (!nl)
+        return fail(nl, ESP_ERR_ARG, "esp_nlist_set_capacity: null list");
+    if (nmax == 0)
+        {
+        nl->fixed_capacity = false;
+        return ESP_OK;
+        }
+    nl->fixed_capacity = true;
+    nl->nmax = (nmax + 7u) & ~7u;
+    nl->nmax_in = std::max(8u, (nmax_inner + 7u) & ~7u);
     return ESP_OK;
     }
 

@@ -324,24 +324,41 @@ __device__ __forceinline__ double uniform_pm1(uint64_t seed, uint64_t step, uint
 
 //! Velocity-Verlet first half for particle i: v += f/(2m) dt; x += v dt (wrapped into the box).
 //! vel = (vx, vy, vz, mass).
+//! First velocity-Verlet half: half kick v += (dt / 2m) f, drift r += dt v, wrap into the box.
+//! Written with explicit fused multiply-adds, so the drift kernel and the predicted-displacement
+//! check (kb_predict_disp) compute the same new position bit for bit.
+__device__ __forceinline__ void vv_first_values(double4& r, double4& v, const double4& f, double dt, double Lx, double Ly, double Lz)
+    {
+    const double h = 0.5 * dt / v.w;
+    v.x = fma(h, f.x, v.x);
+    v.y = fma(h, f.y, v.y);
+    v.z = fma(h, f.z, v.z);
+    r.x = fma(v.x, dt, r.x);
+    r.y = fma(v.y, dt, r.y);
+    r.z = fma(v.z, dt, r.z);
+    r.x = fma(-Lx, rint(r.x / Lx), r.x);
+    r.y = fma(-Ly, rint(r.y / Ly), r.y);
+    r.z = fma(-Lz, rint(r.z / Lz), r.z);
+    }
+
 __device__ __forceinline__ void vv_first_one(unsigned int i, double4* __restrict__ pos, double4* __restrict__ vel,
                                              const double4* __restrict__ force, double dt, double Lx, double Ly, double Lz)
     {
     double4 v = vel[i];
-    const double4 f = force[i];
-    const double h = 0.5 * dt / v.w;
-    v.x += h * f.x;
-    v.y += h * f.y;
-    v.z += h * f.z;
     double4 r = pos[i];
-    r.x += v.x * dt;
-    r.y += v.y * dt;
-    r.z += v.z * dt;
-    r.x -= Lx * rint(r.x / Lx);
-    r.y -= Ly * rint(r.y / Ly);
-    r.z -= Lz * rint(r.z / Lz);
+    vv_first_values(r, v, force[i], dt, Lx, Ly, Lz);
     pos[i] = r;
     vel[i] = v;
+    }
+
+//! Squared minimum-image distance between a position and its position at the last list build
+//! (explicit fused multiply-adds: the same bits in every kernel that uses it).
+__device__ __forceinline__ double displacement2(const double4& a, const double4& b, const double* L)
+    {
+    const double dx = fma(-L[0], rint((a.x - b.x) * (1.0 / L[0])), a.x - b.x);
+    const double dy = fma(-L[1], rint((a.y - b.y) * (1.0 / L[1])), a.y - b.y);
+    const double dz = fma(-L[2], rint((a.z - b.z) * (1.0 / L[2])), a.z - b.z);
+    return fma(dx, dx, fma(dy, dy, dz * dz));
     }
 
 __global__ void __launch_bounds__(256) k_vv_first(unsigned int N, double4* __restrict__ pos, double4* __restrict__ vel,
@@ -475,9 +492,12 @@ __global__ void __launch_bounds__(256) k_energies(unsigned int N, const double4*
 // Batched MD: one launch advances many independent systems.  One workgroup per item
 // (system, chunk of 256 particles); each thread runs the single-system per-particle code.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) kb_vv_first(const md::MDSys* __restrict__ ds, const md::MDItem* __restrict__ items, double dt)
+__global__ void __launch_bounds__(256) kb_vv_first(const md::MDSys* __restrict__ ds, const md::MDItem* __restrict__ items, double dt,
+                                                   const unsigned char* __restrict__ hold)
     {
     const md::MDItem w = items[blockIdx.x];
+    if (hold != nullptr && hold[w.sys])
+        return;  // a held system is not integrated this step (its forces are being computed)
     const md::MDSys& d = ds[w.sys];
     const unsigned int i = w.chunk * 256u + threadIdx.x;
     if (i < d.N)
@@ -486,9 +506,11 @@ __global__ void __launch_bounds__(256) kb_vv_first(const md::MDSys* __restrict__
     }
 
 __global__ void __launch_bounds__(256) kb_vv_second(const md::MDSys* __restrict__ ds, const md::MDItem* __restrict__ items, double dt,
-                                                    double gamma, double kT, uint64_t step)
+                                                    double gamma, double kT, uint64_t step, const unsigned char* __restrict__ hold)
     {
     const md::MDItem w = items[blockIdx.x];
+    if (hold != nullptr && hold[w.sys])
+        return;
     const md::MDSys& d = ds[w.sys];
     const unsigned int i = w.chunk * 256u + threadIdx.x;
     if (i < d.N)
@@ -537,13 +559,28 @@ __global__ void __launch_bounds__(256) kb_max_disp(const md::MDSys* __restrict__
     const unsigned int i = w.chunk * 256u + threadIdx.x;
     double d2 = 0.0;
     if (i < d.N && d.last_pos != nullptr)
+        d2 = displacement2(reinterpret_cast<const double4*>(d.postype)[i], reinterpret_cast<const double4*>(d.last_pos)[i], d.L);
+    block_max_to(d2, out + w.sys);
+    }
+
+//! Predicted displacement check: per system, the largest squared distance between the position
+//! the next first half step (kb_vv_first at the same dt) will produce -- computed by the same
+//! arithmetic -- and the position at the last list build.  Run at the end of a step, it tells
+//! the host which lists the next step must rebuild after its drift, with no wait inside that
+//! step; the decision is the one kb_max_disp would give after the drift.
+__global__ void __launch_bounds__(256) kb_predict_disp(const md::MDSys* __restrict__ ds, const md::MDItem* __restrict__ items,
+                                                       double dt, unsigned long long* __restrict__ out)
+    {
+    const md::MDItem w = items[blockIdx.x];
+    const md::MDSys& d = ds[w.sys];
+    const unsigned int i = w.chunk * 256u + threadIdx.x;
+    double d2 = 0.0;
+    if (i < d.N && d.last_pos != nullptr)
         {
-        const double4 a = reinterpret_cast<const double4*>(d.postype)[i];
-        const double4 b = reinterpret_cast<const double4*>(d.last_pos)[i];
-        const double dx = wrap_min_image(a.x - b.x, d.L[0], 1.0 / d.L[0]);
-        const double dy = wrap_min_image(a.y - b.y, d.L[1], 1.0 / d.L[1]);
-        const double dz = wrap_min_image(a.z - b.z, d.L[2], 1.0 / d.L[2]);
-        d2 = dx * dx + dy * dy + dz * dz;
+        double4 r = reinterpret_cast<const double4*>(d.postype)[i];
+        double4 v = reinterpret_cast<const double4*>(d.vel)[i];
+        vv_first_values(r, v, reinterpret_cast<const double4*>(d.force)[i], dt, d.L[0], d.L[1], d.L[2]);
+        d2 = displacement2(r, reinterpret_cast<const double4*>(d.last_pos)[i], d.L);
         }
     block_max_to(d2, out + w.sys);
     }
@@ -650,20 +687,21 @@ hipError_t energies(unsigned int N, const double* vel, const double* force, doub
     return hipGetLastError();
     }
 
-hipError_t batch_vv_first(const MDSys* d, const MDItem* items, unsigned int n_items, double dt, hipStream_t s)
+hipError_t batch_vv_first(const MDSys* d, const MDItem* items, unsigned int n_items, double dt, const unsigned char* hold,
+                          hipStream_t s)
     {
     if (n_items == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(kb_vv_first, dim3(n_items), dim3(256), 0, s, d, items, dt);
+    hipLaunchKernelGGL(kb_vv_first, dim3(n_items), dim3(256), 0, s, d, items, dt, hold);
     return hipGetLastError();
     }
 
 hipError_t batch_vv_second(const MDSys* d, const MDItem* items, unsigned int n_items, double dt, double gamma, double kT, uint64_t step,
-                           hipStream_t s)
+                           const unsigned char* hold, hipStream_t s)
     {
     if (n_items == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(kb_vv_second, dim3(n_items), dim3(256), 0, s, d, items, dt, gamma, kT, step);
+    hipLaunchKernelGGL(kb_vv_second, dim3(n_items), dim3(256), 0, s, d, items, dt, gamma, kT, step, hold);
     return hipGetLastError();
     }
 
@@ -680,6 +718,15 @@ hipError_t batch_max_disp(const MDSys* d, const MDItem* items, unsigned int n_it
     if (n_items == 0)
         return hipSuccess;
     hipLaunchKernelGGL(kb_max_disp, dim3(n_items), dim3(256), 0, s, d, items, out);
+    return hipGetLastError();
+    }
+
+hipError_t batch_predict_disp(const MDSys* d, const MDItem* items, unsigned int n_items, double dt, unsigned long long* out,
+                              hipStream_t s)
+    {
+    if (n_items == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(kb_predict_disp, dim3(n_items), dim3(256), 0, s, d, items, dt, out);
     return hipGetLastError();
     }
 

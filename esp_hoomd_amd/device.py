@@ -47,6 +47,47 @@ class _BlockCache:
 _cache = _BlockCache()
 
 
+class PinnedBlock:
+    """A pinned (page-locked) host buffer of a size class, from a PinnedPool."""
+
+    def __init__(self, ptr: C.c_void_p, cls: int):
+        self.ptr = ptr
+        self.cls = cls
+
+
+class PinnedPool:
+    """Pinned host staging buffers for asynchronous uploads, reused by size class.  A block is
+    taken for one upload and released once the stream has passed that copy (hipHostMalloc, so
+    the copy engine reads it while the host moves on)."""
+
+    def __init__(self):
+        self.free_blocks = {}
+
+    def take(self, nbytes: int) -> PinnedBlock:
+        cls = _BlockCache.size_class(nbytes)
+        blocks = self.free_blocks.get(cls)
+        if blocks:
+            return blocks.pop()
+        p = C.c_void_p()
+        _lib.check(_lib.load().esp_host_malloc_pinned(C.byref(p), cls))
+        return PinnedBlock(p, cls)
+
+    def release(self, block: PinnedBlock) -> None:
+        self.free_blocks.setdefault(block.cls, []).append(block)
+
+    def close(self) -> None:
+        for blocks in self.free_blocks.values():
+            for b in blocks:
+                _lib.load().esp_host_free_pinned(b.ptr)
+        self.free_blocks = {}
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class DeviceArray:
     """A device allocation with a numpy dtype and shape.  `pooled` arrays take and return their
     blocks through a bounded size-class cache (see _BlockCache)."""
@@ -82,6 +123,18 @@ class DeviceArray:
             raise ValueError(f"size mismatch: {a.nbytes} != {self.nbytes}")
         if self.nbytes:
             _lib.check(_lib.load().esp_memcpy_htod(self._ptr, a.ctypes.data_as(C.c_void_p), self.nbytes))
+
+    def upload_async(self, a: np.ndarray, stream: "Stream", pool: PinnedPool) -> PinnedBlock:
+        """Copy `a` into a pinned staging block and enqueue its upload on `stream`; returns the
+        block, which the caller releases to the pool once the stream has passed the copy."""
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if a.nbytes != self.nbytes:
+            raise ValueError(f"size mismatch: {a.nbytes} != {self.nbytes}")
+        blk = pool.take(max(self.nbytes, 8))
+        if self.nbytes:
+            C.memmove(blk.ptr, a.ctypes.data, self.nbytes)
+            _lib.check(_lib.load().esp_memcpy_htod_async(self._ptr, blk.ptr, self.nbytes, stream.handle))
+        return blk
 
     def numpy(self) -> np.ndarray:
         out = np.empty(self.shape, self.dtype)

@@ -20,7 +20,7 @@ class State:
     vz, mass), net force (fx, fy, fz, energy) and per-particle virial (6 x pitch)."""
 
     def __init__(self, positions, charges, box, velocities=None, masses=1.0, types=None, exclusions=None,
-                 zero_outputs: bool = True, pooled: bool = False):
+                 zero_outputs: bool = True, pooled: bool = False, stream=None, staging=None):
         pos = np.asarray(positions, dtype=np.float64)
         self.N = len(pos)
         box = list(box)
@@ -29,14 +29,26 @@ class State:
         pt[:, :3] = pos
         if types is not None:
             pt[:, 3] = types
-        # pooled: the arrays come from and return to the device block cache (device.DeviceArray)
-        self.postype = DeviceArray.from_numpy(pt, pooled=pooled)
-        self.charge = DeviceArray.from_numpy(np.asarray(charges, dtype=np.float64), pooled=pooled)
         v = np.zeros((self.N, 4))
         if velocities is not None:
             v[:, :3] = velocities
         v[:, 3] = masses
-        self.vel = DeviceArray.from_numpy(v, pooled=pooled)
+        q = np.asarray(charges, dtype=np.float64)
+        # pooled: the arrays come from and return to the device block cache (device.DeviceArray)
+        # stream + staging (a device.PinnedPool): uploads are enqueued on the stream from pinned
+        # blocks, held in self.staging until release_staging() (after the stream passed them)
+        self.staging = []
+        if stream is not None and staging is not None:
+            self.postype = DeviceArray(pt.shape, np.float64, pooled=pooled)
+            self.charge = DeviceArray(q.shape, np.float64, pooled=pooled)
+            self.vel = DeviceArray(v.shape, np.float64, pooled=pooled)
+            self._staging_pool = staging
+            for d, a in ((self.postype, pt), (self.charge, q), (self.vel, v)):
+                self.staging.append(d.upload_async(a, stream, staging))
+        else:
+            self.postype = DeviceArray.from_numpy(pt, pooled=pooled)
+            self.charge = DeviceArray.from_numpy(q, pooled=pooled)
+            self.vel = DeviceArray.from_numpy(v, pooled=pooled)
         # zero_outputs=False leaves force and virial unset (for callers whose first force
         # evaluation overwrites them)
         self.force = DeviceArray((self.N, 4), np.float64, zero=zero_outputs, pooled=pooled)
@@ -53,6 +65,12 @@ class State:
             self.ex_pitch = pitch
         self.nlist_ptrs = None  # (n_neigh, nlist, head) device pointers of the current list
         self.inner_ptrs = None  # (n_neigh, nlist, head, r_inner) of the current inner list, if any
+
+    def release_staging(self) -> None:
+        """Return the pinned upload blocks (call once the upload stream has passed them)."""
+        for blk in self.staging:
+            self._staging_pool.release(blk)
+        self.staging = []
 
     @property
     def volume(self) -> float:
@@ -150,6 +168,22 @@ class NeighborList:
             _lib.check(lib.esp_nlist_inner_arrays(self._nl, C.byref(a), C.byref(b), C.byref(c), C.byref(nmax)))
             state.inner_ptrs = (a.value, b.value, c.value, self.inner)
             self.nmax_inner = nmax.value
+
+    def reset(self, r_cut: float) -> None:
+        """Make the list one for pair cutoff r_cut (rebuilt at the next update), keeping its
+        arrays for reuse."""
+        if self._nl is None:
+            self._ensure(r_cut)
+            return
+        _lib.check(_lib.load().esp_nlist_set_cutoff(self._nl, float(r_cut)))
+        self.r_cut = float(r_cut)
+
+    def set_capacity(self, nmax: int, nmax_inner: int = 0) -> None:
+        """Row capacities of the next builds (0: sized from the density); an overflowing build
+        raises them."""
+        if self._nl is None:
+            raise RuntimeError("NeighborList.set_capacity: no list yet (call update first)")
+        _lib.check(_lib.load().esp_nlist_set_capacity(self._nl, int(nmax), int(nmax_inner)))
 
     def update(self, state: State, r_cut: float, force_rebuild: bool = False, stream=None) -> bool:
         """Rebuild the list if needed (or always, with force_rebuild) on `stream` (None: the null
@@ -381,8 +415,9 @@ class BatchMD:
         if rc != _lib.ESP_OK:
             raise _lib.EspError(rc, _lib.load().esp_md_batch_last_error(self._b).decode())
 
-    def set(self, members, seeds=None) -> None:
-        """members: list of (State, NeighborList or None)."""
+    def set(self, members, seeds=None, stream=None) -> None:
+        """members: list of (State, NeighborList or None).  With a stream the descriptor uploads
+        are enqueued on it without waiting (synchronise with it before the next set)."""
         n = len(members)
         systems = (_lib.EspSystem * max(n, 1))()
         vel = (C.c_void_p * max(n, 1))()
@@ -395,8 +430,17 @@ class BatchMD:
             outs[i] = st.c_out()
             nls[i] = nl._nl.value if (nl is not None and nl._nl is not None) else None
             sd[i] = int(seeds[i]) if seeds is not None else 0
-        self._check(_lib.load().esp_md_batch_set(self._b, n, systems, vel, outs, nls, sd))
+        if stream is not None:
+            self._check(_lib.load().esp_md_batch_set_async(self._b, n, systems, vel, outs, nls, sd, stream.handle))
+        else:
+            self._check(_lib.load().esp_md_batch_set(self._b, n, systems, vel, outs, nls, sd))
         self.members = list(members)
+
+    def set_hold(self, hold, stream=None) -> None:
+        """Members flagged in `hold` are not integrated by the next integrate calls."""
+        n = len(self.members)
+        h = (C.c_ubyte * max(n, 1))(*[1 if x else 0 for x in hold])
+        self._check(_lib.load().esp_md_batch_set_hold(self._b, h, stream.handle if stream else None))
 
     def integrate_first(self, dt: float, stream=None) -> None:
         self._check(_lib.load().esp_md_batch_integrate_first(self._b, dt, stream.handle if stream else None))
@@ -419,6 +463,42 @@ class BatchMD:
             st, nl = self.members[i]
             nl._publish(st)
         return out
+
+    # ---- list upkeep without a wait inside the step -------------------------------------
+    def predict(self, dt: float, stream=None) -> None:
+        """Enqueue the predicted displacement check for the next first half step at dt (its
+        result reaches the host with the stream; read it after synchronising)."""
+        self._check(_lib.load().esp_md_batch_predict(self._b, float(dt), stream.handle if stream else None))
+
+    def read_predicted(self) -> list:
+        """Members whose list the next step must rebuild (after synchronising with the step that
+        ran predict())."""
+        n = len(self.members)
+        need = (C.c_ubyte * max(n, 1))()
+        self._check(_lib.load().esp_md_batch_read_predicted(self._b, need))
+        return [bool(need[i]) for i in range(n)]
+
+    def rebuild_async(self, which, stream=None) -> list:
+        """Enqueue the rebuilds of the members flagged in `which` (no wait); returns the members
+        whose list arrays moved (their States are re-pointed here)."""
+        n = len(self.members)
+        w = (C.c_ubyte * max(n, 1))(*[1 if x else 0 for x in which])
+        moved = (C.c_ubyte * max(n, 1))()
+        self._check(_lib.load().esp_md_batch_rebuild_async(self._b, w, moved, stream.handle if stream else None))
+        out = [i for i in range(n) if moved[i]]
+        for i in out:
+            st, nl = self.members[i]
+            nl._publish(st)
+        return out
+
+    def finish_rebuilds(self) -> list:
+        """After synchronising with the step that ran rebuild_async: the members whose rebuilt
+        list overflowed its rows (their step must be repeated)."""
+        n = len(self.members)
+        over = (C.c_ubyte * max(n, 1))()
+        cnt = C.c_uint(0)
+        self._check(_lib.load().esp_md_batch_finish_rebuilds(self._b, over, C.byref(cnt)))
+        return [i for i in range(n) if over[i]]
 
     def __del__(self):
         try:

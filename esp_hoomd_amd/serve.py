@@ -17,9 +17,19 @@ asks for `steps` MD steps (velocity Verlet, WCA + ESP electrostatics).  The serv
   completed requests and particle-steps.
 
 Per tick the host issues a fixed number of launches whatever the number of resident
-requests: batched first half-kick and drift, one displacement check for all neighbour lists
-(only lists that need it are rebuilt), the batched ESP forces (hipGraph replay), batched WCA
-and the batched second half-kick.
+requests: batched first half-kick and drift, the rebuilds of the neighbour lists that need one,
+the batched ESP forces (hipGraph replay), batched WCA, the batched second half-kick and a
+predicted displacement check.  The tick has no wait inside it: which lists a tick rebuilds is
+known before it starts, from the previous tick's check of the position its drift will produce
+(the same arithmetic as the drift, so the decisions are the ones a check after the drift would
+give).  A rebuilt list that overflowed its rows is found when the tick completes; that request's
+step is then repeated alone from a copy of its state taken before the tick (same bits).
+
+An admitted request's uploads are enqueued behind the tick in flight from pinned staging, and
+it joins the next tick held: that tick builds its first list and computes its forces at the
+initial positions with everyone else's, without integrating it.  Retired requests hand their
+solver and list objects to new ones, so admissions reuse device arrays instead of allocating.
+Between two ticks the host waits once, for the tick to complete.
 
 ESP parameters per request come from the autotuner's model at the force-error target (cached
 per size/density bucket), with the model's method scale measured on probe requests
@@ -40,7 +50,7 @@ import numpy as np
 
 from . import _lib
 from .batch import Batch
-from .device import Stream, synchronize
+from .device import PinnedPool, Stream, synchronize
 from .md import BatchMD, NeighborList, State, WCA, maxwell_boltzmann
 from .solver import Params, Solver
 from . import tune as _tune
@@ -78,6 +88,8 @@ class Request:
     final_forces: Optional[np.ndarray] = field(default=None, repr=False)
     force_error: Optional[float] = None  # relative RMS electrostatic force error (verify_results)
     prepared: Optional[tuple] = field(default=None, repr=False)
+    snapshot: Optional[tuple] = field(default=None, repr=False)  # state before a tick that rebuilds its list
+    first_tick: bool = False                     # admitted: the next tick computes its first forces
     admitted_wall: float = 0.0
     last_step_end: float = 0.0
 
@@ -106,7 +118,7 @@ class Server:
     def __init__(self, hbm_budget: int = 32 << 30, max_active: int = 64, precision: int = _lib.ESP_PREC_MIXED,
                  target: float = 1e-5, dt: float = 0.002, use_graph: bool = True, keep_results: bool = False,
                  tuner_orders: Sequence[int] = (6, 8, 10), mesh_range: Tuple[int, int] = (16, 128),
-                 keep_final_positions: bool = False):
+                 keep_final_positions: bool = False, predict_lists: bool = True, reuse_contexts: bool = True):
         self.hbm_budget = int(hbm_budget)
         self.max_active = int(max_active)
         self.precision = precision
@@ -119,13 +131,24 @@ class Server:
         self.tuner_orders = tuple(tuner_orders)
         self.mesh_range = mesh_range
         self.batch = Batch(use_graph=use_graph)       # every resident request, every tick
-        self.batch_new = Batch(use_graph=False)       # initial forces of newly admitted requests
         self.md = BatchMD()                           # integration, WCA and list upkeep of the resident set
-        # the resident set's steps run on their own (non-blocking) stream; admissions (uploads,
-        # first neighbour list, first forces) run on the null stream meanwhile, so the host
-        # prepares new requests while the GPU advances the others
+        self._pinned = PinnedPool()                   # staging of the admitted requests' uploads
+        self._held_uploaded = False                   # the batched MD holds some member this tick
+        # solvers and lists of retired requests, handed to new ones (device memory reused; at
+        # most max_active held beside the resident footprint)
+        self.reuse_contexts = bool(reuse_contexts)
+        self._ctx_pool: List[tuple] = []
+        self.contexts_reused = 0
+        # per request id: whether its next step must rebuild its list (predicted displacement)
+        self._need: Dict[int, bool] = {}
+        self.redone_steps = 0
+        # False: check the displacements after each drift and wait for it inside the tick (the
+        # decisions are the same; kept to show that)
+        self.predict_lists = bool(predict_lists)
+        # the resident set's steps and the admitted requests' uploads run on the server's own
+        # (non-blocking) stream; the host prepares new requests while the GPU advances the others
         self.stream = Stream()
-        self._inflight = None  # (requests, launch time, particles) of the step on the stream
+        self._inflight = None  # (requests, launch time, step number, held flags) of the tick on the stream
         self._member_ids: List[int] = []
         self._esp_args = None
         self.step_latency: List[float] = []
@@ -139,9 +162,9 @@ class Server:
         self.tick_active: List[int] = []
         self.tick_particles: List[int] = []
         self.waited_for_memory = 0
-        # wall seconds per phase of the loop (admission, first forces of new requests, member
+        # wall seconds per phase of the loop (admission, member
         # bookkeeping, list upkeep, ESP launch, the rest of the step up to its synchronisation)
-        self.admit_s: Dict[str, float] = {k: 0.0 for k in ("state", "nlist", "solver", "first_forces")}
+        self.admit_s: Dict[str, float] = {k: 0.0 for k in ("state", "nlist", "solver")}
         self.phase_s: Dict[str, float] = {k: 0.0 for k in ("admit", "wait", "retire", "members", "nlists", "esp",
                                                           "step_rest")}
         self._tune_cache: Dict[Tuple[int, int], Params] = {}
@@ -149,11 +172,10 @@ class Server:
 
     # ---- request lifecycle ----------------------------------------------------------
     def submit(self, req: Request) -> None:
-        """Queue a request.  Its particle data (the client's input) is generated here, so a
-        stream submitted ahead of run() is served without client-side work in the loop."""
-        pos, q, L = req.setup()
-        params, nbar = self._params_for(len(pos), L)
-        req.prepared = (pos, q, L, params, footprint_bytes(len(pos), params, nbar))
+        """Queue a request.  Its particle data and initial velocities (the client's input) are
+        generated here, so a stream submitted ahead of run() is served without client-side work
+        in the loop."""
+        req.prepared = self._prepare(req)
         heapq.heappush(self.pending, (req.arrival, req.rid, req))
 
     # ---- accuracy ---------------------------------------------------------------------
@@ -224,59 +246,63 @@ class Server:
         params = Params(nx=n, ny=n, nz=n, order=p.order, kappa=c / rc, rcut=rc, precision=self.precision, tile=p.tile)
         return params, density * 4.0 / 3.0 * math.pi * (rc + SKIN) ** 3
 
+    def _prepare(self, req: Request) -> tuple:
+        """The request's particles, parameters, footprint and initial velocities (host work, done
+        at submission when the request is known ahead)."""
+        pos, q, L = req.setup()
+        params, nbar = self._params_for(len(pos), L)
+        vel = maxwell_boltzmann(np.ones(len(pos)), req.kT, seed=req.seed + 1)
+        return (pos, q, L, params, footprint_bytes(len(pos), params, nbar), vel)
+
     def _admit(self, now: float) -> bool:
+        """Admit arrived requests that fit.  Their uploads are enqueued on the server's stream
+        behind the step in flight (from pinned staging; nothing here waits for the GPU) and they
+        join the next tick held: that tick builds their first neighbour list and computes their
+        forces at the initial positions with everyone else's, without integrating them, and
+        predicts their first drift; their MD steps start with the tick after."""
         changed = False
-        new: List[Request] = []
         while self.pending and self.pending[0][0] <= now and len(self.active) < self.max_active:
             req = self.pending[0][2]
             if req.prepared is None:
-                pos, q, L = req.setup()
-                params, nbar = self._params_for(len(pos), L)
-                req.prepared = (pos, q, L, params, footprint_bytes(len(pos), params, nbar))
-            pos, q, L, params, need = req.prepared
+                req.prepared = self._prepare(req)
+            pos, q, L, params, need, vel = req.prepared
             if self.resident_bytes + need > self.hbm_budget and self.active:
                 self.waited_for_memory += 1
                 break
             heapq.heappop(self.pending)
             req.params, req.footprint, req.prepared = params, need, None
             ta = time.perf_counter()
-            # velocities drawn on the host; force and virial are overwritten by the first forces
-            st = State(pos, q, [L, L, L], velocities=maxwell_boltzmann(np.ones(len(pos)), req.kT, seed=req.seed + 1),
-                       zero_outputs=False, pooled=True)
-            req.state = st
+            # force and virial are overwritten by the first forces
+            req.state = State(pos, q, [L, L, L], velocities=vel, zero_outputs=False, pooled=True, stream=self.stream,
+                              staging=self._pinned)
             tb = time.perf_counter()
-            req.nlist = NeighborList(buffer=SKIN, inner=self.wca.r_cut)
-            req.nlist.update(st, max(params.rcut, self.wca.r_cut), force_rebuild=True)
+            # a retired request's solver and list, when one is free: their device arrays are
+            # reused (set_params / reset rebuild everything that depends on the parameters)
+            pooled = self._ctx_pool.pop() if self._ctx_pool else None
+            if pooled is not None:
+                req.solver, req.nlist = pooled
+                req.nlist.reset(max(params.rcut, self.wca.r_cut))
+                self.contexts_reused += 1
+            else:
+                req.nlist = NeighborList(buffer=SKIN, inner=self.wca.r_cut)
+                req.nlist._ensure(max(params.rcut, self.wca.r_cut))
             tc = time.perf_counter()
-            req.solver = Solver(params)
+            if pooled is not None:
+                req.solver.set_params(params)
+            else:
+                req.solver = Solver(params)
             self.admit_s["state"] += tb - ta
             self.admit_s["nlist"] += tc - tb
             self.admit_s["solver"] += time.perf_counter() - tc
             req.admitted_at = now
             req.admitted_wall = time.perf_counter()
+            req.first_tick = True
+            self._need[req.rid] = True  # no list yet
             self.active.append(req)
             self.resident_bytes += need
             self.peak_bytes = max(self.peak_bytes, self.resident_bytes)
-            new.append(req)
             changed = True
-        if new:
-            # forces at the initial positions (the first half kick needs them), on the null
-            # stream beside the resident set's step; a separate batch, so the resident set's
-            # captured graph stays valid
-            tf = time.perf_counter()
-            self._forces(new, self.batch_new)
-            if self.keep_results:
-                for r in new:
-                    r.first_forces = r.state.force.numpy().copy()
-            _lib.check(self._lib.esp_stream_synchronize(None))
-            self.admit_s["first_forces"] += time.perf_counter() - tf
         return changed
-
-    def _forces(self, reqs: List[Request], batch: Batch) -> None:
-        """Net force of the given requests: batched ESP (overwrites), then WCA (adds)."""
-        batch.enqueue([(r.solver, r.state) for r in reqs])
-        for r in reqs:
-            self.wca.add_to(r.state)
 
     def _retire(self, now: float) -> None:
         keep = []
@@ -288,7 +314,10 @@ class Server:
                 if self.keep_results:
                     r.final_forces = r.state.force.numpy().copy()
                 self.resident_bytes -= r.footprint
-                r.state = r.solver = r.nlist = None
+                if self.reuse_contexts and len(self._ctx_pool) < self.max_active:
+                    self._ctx_pool.append((r.solver, r.nlist))
+                r.state = r.solver = r.nlist = r.snapshot = None
+                self._need.pop(r.rid, None)
                 self.done.append(r)
             else:
                 keep.append(r)
@@ -300,52 +329,125 @@ class Server:
         if ids == self._member_ids:
             return
         self._member_ids = ids
-        self.md.set([(r.state, r.nlist) for r in self.active], seeds=[r.seed for r in self.active])
+        self.md.set([(r.state, r.nlist) for r in self.active], seeds=[r.seed for r in self.active], stream=self.stream)
         self._esp_args = self.batch.prepare([(r.solver, r.state) for r in self.active])
 
     def _complete(self) -> None:
         """Wait for the step in flight and account it: each request's step latency runs from its
-        previous step's completion (or its admission) to now."""
+        previous step's completion (or its admission) to now.  Reads the tick's predicted
+        displacement checks and repeats the step of any request whose rebuilt list overflowed.
+        Requests held in the tick (admitted: their first forces) took no MD step in it."""
         if self._inflight is None:
             return
-        reqs, t0, particles = self._inflight
+        reqs, t0, step_no, held = self._inflight
         tw = time.perf_counter()
         self.stream.synchronize()
+        self.phase_s["wait"] += time.perf_counter() - tw
+        if self.predict_lists:
+            overflowed = self.md.finish_rebuilds()
+            for r, need in zip(reqs, self.md.read_predicted()):
+                self._need[r.rid] = need
+            for i in overflowed:
+                self._redo_step(reqs[i], step_no, held[i])
         end = time.perf_counter()
-        self.phase_s["wait"] += end - tw
-        for r in reqs:
+        particles = 0
+        stepped = 0
+        for r, h in zip(reqs, held):
+            if h:
+                r.first_tick = False
+                if self.keep_results:
+                    r.first_forces = r.state.force.numpy().copy()
+                r.state.release_staging()
+                continue
             r.steps_done += 1
+            stepped += 1
+            particles += r.state.N
             self.step_latency.append((end - (r.last_step_end if r.last_step_end else r.admitted_wall)) * 1e3)
             r.last_step_end = end
         self.tick_ms.append((end - t0) * 1e3)
-        self.tick_active.append(len(reqs))
+        self.tick_active.append(stepped)
         self.tick_particles.append(particles)
         self._inflight = None
 
+    def _snapshot(self, r: Request) -> None:
+        """Copy a request's positions, velocities and forces (the state a step starts from) on
+        the server's stream, for repeating its step if the list it rebuilds overflows."""
+        from .device import DeviceArray
+        st = r.state
+        if r.snapshot is None:
+            r.snapshot = tuple(DeviceArray(a.shape, a.dtype, pooled=True) for a in (st.postype, st.vel, st.force))
+        for dst, src in zip(r.snapshot, (st.postype, st.vel, st.force)):
+            _lib.check(self._lib.esp_memcpy_dtod(dst._ptr, src._ptr, src.nbytes, self.stream.handle))
+
+    def _redo_step(self, r: Request, step_no: int, held: bool) -> None:
+        """Repeat request r's last tick alone: restore the state it started from, drift (unless
+        it was held), build its list with the raised capacity (waiting for it), then forces,
+        WCA, second half kick (unless held) and the displacement prediction -- the launches of a
+        tick for one member, so the result is the same bits the tick would have given."""
+        st, h = r.state, self.stream.handle
+        for src, dst in zip(r.snapshot, (st.postype, st.vel, st.force)):
+            _lib.check(self._lib.esp_memcpy_dtod(dst._ptr, src._ptr, src.nbytes, h))
+        solo = BatchMD()
+        solo.set([(st, r.nlist)], seeds=[r.seed])
+        if not held:
+            solo.integrate_first(self.dt, stream=self.stream)
+        r.nlist.update(st, max(r.params.rcut, self.wca.r_cut), force_rebuild=True, stream=self.stream)
+        solo.set([(st, r.nlist)], seeds=[r.seed])  # the list arrays may have moved
+        Batch(use_graph=False).enqueue([(r.solver, st)], stream=self.stream)
+        solo.wca(WCA_SIGMA, 1.0, stream=self.stream)
+        if not held:
+            solo.integrate_second(self.dt, step=step_no, stream=self.stream)
+        solo.predict(self.dt, stream=self.stream)
+        self.stream.synchronize()
+        self._need[r.rid] = solo.read_predicted()[0]
+        self._member_ids = []  # re-describe the resident set (list pointers may have changed)
+        self.redone_steps += 1
+
     def _launch(self) -> None:
         """Enqueue one MD step of every resident request on the server's stream: batched first
-        half kick + drift, one displacement check (rebuilding only the lists that need it),
-        batched ESP forces (captured graph), batched WCA, batched second half kick.  Returns
-        once the step is enqueued; _complete waits for it."""
+        half kick + drift, the list rebuilds the previous tick's predicted check asked for,
+        batched ESP forces (captured graph), batched WCA, batched second half kick and the
+        predicted check for the next tick.  Nothing in it waits for the GPU; _complete waits
+        for it."""
         ph = self.phase_s
         t0 = time.perf_counter()
         self._sync_members()
         t1 = time.perf_counter()
         st = self.stream
-        self.md.integrate_first(self.dt, stream=st)
-        for i in self.md.update_nlists(stream=st):
+        held = [r.first_tick for r in self.active]
+        if any(held) or self._held_uploaded:
+            self.md.set_hold(held, stream=st)
+            self._held_uploaded = any(held)
+        if self.predict_lists:
+            need = [self._need.get(r.rid, True) for r in self.active]
+            for r, nd in zip(self.active, need):
+                if nd:
+                    self._snapshot(r)
+            self.md.integrate_first(self.dt, stream=st)
+            moved = set(self.md.rebuild_async(need, stream=st))
+        else:
+            self.md.integrate_first(self.dt, stream=st)
+            moved = set(self.md.update_nlists(stream=st))
+        for i, r in enumerate(self.active):
+            if held[i] and i not in moved:
+                r.nlist._publish(r.state)  # a reused list keeps its arrays: point the new State at them
+                moved.add(i)
+        for i in moved:
             Batch.refresh(self._esp_args, i)
         t2 = time.perf_counter()
         self.batch.enqueue_prepared(self._esp_args, stream=st)
         t3 = time.perf_counter()
+        step_no = len(self.tick_ms) + 1
         self.md.wca(WCA_SIGMA, 1.0, stream=st)
-        self.md.integrate_second(self.dt, step=len(self.tick_ms) + 1, stream=st)
+        self.md.integrate_second(self.dt, step=step_no, stream=st)
+        if self.predict_lists:
+            self.md.predict(self.dt, stream=st)
         t4 = time.perf_counter()
         ph["members"] += t1 - t0
         ph["nlists"] += t2 - t1
         ph["esp"] += t3 - t2
         ph["step_rest"] += t4 - t3
-        self._inflight = (list(self.active), t0, sum(r.state.N for r in self.active))
+        self._inflight = (list(self.active), t0, step_no, held)
 
     def tick(self, now: float) -> None:
         """One MD step of every resident request, waited for."""
@@ -401,6 +503,8 @@ class Server:
             "peak_resident_bytes": self.peak_bytes,
             "admissions_deferred_for_memory": self.waited_for_memory,
             "batch": self.batch.stats(),
+            "steps_repeated_for_list_overflow": self.redone_steps,
+            "contexts_reused": self.contexts_reused,
             "phase_s": {k: round(v, 3) for k, v in self.phase_s.items()},
             "admit_s": {k: round(v, 3) for k, v in self.admit_s.items()},
         }

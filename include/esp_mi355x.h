@@ -292,6 +292,15 @@ int esp_nlist_stats(esp_nlist* nl, unsigned long long* builds);
  * any pair force with cutoff <= r_inner under the same rebuild rule (the batched WCA uses it);
  * 0 turns it off.  Takes effect at the next build (the next update rebuilds). */
 int esp_nlist_set_inner(esp_nlist* nl, double r_inner);
+/* Change the list's cutoff (the next update rebuilds); its arrays are kept and reused by the
+ * builds that fit them (no counterpart in the reference: lets a server hand a retired request's
+ * list to a new one without reallocating) */
+int esp_nlist_set_cutoff(esp_nlist* nl, double r_cut);
+/* Row capacities (neighbours per particle) of the next builds, full and inner list; nmax = 0
+ * returns to the automatic sizing from the density (1.5x the mean count plus 32).  A build that
+ * overflows its rows raises them and is repeated (esp_nlist_build / _update) or reported
+ * (esp_md_batch_finish_rebuilds).  No counterpart in the reference. */
+int esp_nlist_set_capacity(esp_nlist* nl, unsigned int nmax, unsigned int nmax_inner);
 /* device arrays of the current inner list (HOOMD layout); ESP_ERR_STATE if there is none */
 int esp_nlist_inner_arrays(esp_nlist* nl, const unsigned int** n_neigh, const unsigned int** nlist, const uint64_t** head_list,
                            unsigned int* nmax);
@@ -320,10 +329,35 @@ int esp_md_batch_destroy(esp_md_batch* batch);
 const char* esp_md_batch_last_error(const esp_md_batch* batch);
 int esp_md_batch_set(esp_md_batch* batch, unsigned int n, const esp_system* systems, double* const* vel, const esp_out* outs,
                      esp_nlist* const* nls, const uint64_t* seeds);
+/* esp_md_batch_set with the descriptor uploads enqueued on `stream` from pinned staging (no
+ * wait); the caller synchronises with the stream before the next set of the same batch */
+int esp_md_batch_set_async(esp_md_batch* batch, unsigned int n, const esp_system* systems, double* const* vel, const esp_out* outs,
+                           esp_nlist* const* nls, const uint64_t* seeds, esp_stream_t stream);
+/* Members with hold[i] != 0 are skipped by the next integrate calls until the next set or
+ * set_hold (a member whose first forces are computed in this step); the flags are uploaded on
+ * `stream` (synchronise with it before the next set_hold); hold = NULL or all zero clears */
+int esp_md_batch_set_hold(esp_md_batch* batch, const unsigned char* hold, esp_stream_t stream);
 int esp_md_batch_integrate_first(esp_md_batch* batch, double dt, esp_stream_t stream);
 int esp_md_batch_integrate_second(esp_md_batch* batch, double dt, double gamma, double kT, uint64_t step, esp_stream_t stream);
 int esp_md_batch_wca(esp_md_batch* batch, double sigma, double epsilon, esp_stream_t stream);
 int esp_md_batch_update_nlists(esp_md_batch* batch, unsigned int* rebuilt, unsigned char* which, esp_stream_t stream);
+/* List upkeep without a wait inside the step (no counterpart in the reference).  At the end of
+ * a step, esp_md_batch_predict(dt) enqueues, per member, the largest squared distance between
+ * the position the next esp_md_batch_integrate_first(dt) will produce (same arithmetic, same
+ * bits) and the position at the last build, copied into pinned host memory.  Once the caller
+ * has synchronised with that step, esp_md_batch_read_predicted sets need[i] = 1 for the
+ * members whose list the next step must rebuild (the esp_md_batch_update_nlists criterion,
+ * so the decisions are identical).  In the next step, after integrate_first,
+ * esp_md_batch_rebuild_async(which = need) enqueues those builds and records their positions
+ * without waiting; moved[i] (optional) = 1 where the list arrays were reallocated (callers
+ * refresh the pointers they pass elsewhere).  After synchronising with that step,
+ * esp_md_batch_finish_rebuilds reports in overflowed[i] the lists whose rows were too short:
+ * their capacities are raised and the list marked unbuilt, the member's forces of that step are
+ * invalid and the caller repeats its step (e.g. from a copy of its state taken before it). */
+int esp_md_batch_predict(esp_md_batch* batch, double dt, esp_stream_t stream);
+int esp_md_batch_read_predicted(esp_md_batch* batch, unsigned char* need);
+int esp_md_batch_rebuild_async(esp_md_batch* batch, const unsigned char* which, unsigned char* moved, esp_stream_t stream);
+int esp_md_batch_finish_rebuilds(esp_md_batch* batch, unsigned char* overflowed, unsigned int* n_overflowed);
 
 /* ---- device utilities (plumbing for callers without a device allocator) ---------------- */
 int esp_device_count(int* n);
@@ -333,6 +367,11 @@ int esp_device_free(void* ptr);
 int esp_memcpy_htod(void* dst, const void* src, size_t bytes);
 int esp_memcpy_dtoh(void* dst, const void* src, size_t bytes);
 int esp_memcpy_dtod(void* dst, const void* src, size_t bytes, esp_stream_t stream);
+/* host-to-device copy enqueued on a stream: from pinned memory (esp_host_malloc_pinned) the call
+ * returns at once, and the source must stay unchanged until the stream has passed the copy */
+int esp_memcpy_htod_async(void* dst, const void* src, size_t bytes, esp_stream_t stream);
+int esp_host_malloc_pinned(void** ptr, size_t bytes);
+int esp_host_free_pinned(void* ptr);
 int esp_memset(void* dst, int value, size_t bytes);
 /* copy n 32-byte records with one vector load and one vector store per record (a plain
    streaming kernel of known traffic: 64 bytes of HBM per record) */

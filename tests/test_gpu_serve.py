@@ -87,3 +87,79 @@ def test_served_results_meet_the_target_with_a_request_near_200k():
           f"params {[(r.N, r.params.nx, r.params.order, round(r.params.rcut, 3)) for r in srv.done]}", flush=True)
     assert st["requests_done"] == 4
     assert max(errs) <= 1e-5
+
+
+def _lists_built(srv):
+    return sum(r.nlist.builds for r in srv.active) if srv.active else None
+
+
+@pytest.mark.parametrize("precision", [_lib.ESP_PREC_MIXED, _lib.ESP_PREC_FP64])
+def test_predicted_list_rebuilds_equal_checked_rebuilds(precision):
+    """The tick decides its list rebuilds from the previous tick's prediction of the position
+    its drift will produce; the trajectories equal, bit for bit, those of ticks that check the
+    displacements after the drift and wait for the result (long enough for several rebuilds)."""
+    out = {}
+    for predict in (True, False):
+        srv = Server(precision=precision, keep_results=True, predict_lists=predict)
+        reqs = [_req(0, 2500, 90, density=0.2), _req(1, 4000, 70, density=0.1, arrival=0.002),
+                _req(2, 1500, 80, density=0.15)]
+        for r in reqs:
+            srv.submit(r)
+        srv.run()
+        out[predict] = {r.rid: (r.final_positions.copy(), r.final_forces.copy()) for r in srv.done}
+    for rid in out[True]:
+        assert np.array_equal(out[True][rid][0], out[False][rid][0])
+        assert np.array_equal(out[True][rid][1], out[False][rid][1])
+
+
+def test_overflowing_list_rebuild_repeats_the_step():
+    """A request whose list rows are set far too short: its first rebuild inside a tick
+    overflows, the tick's step for that request is repeated alone from the state copied before
+    the tick with grown rows, and its trajectory equals the one of a server that never
+    overflowed, bit for bit."""
+    class Shrinking(Server):
+        shrunk = False
+
+        def _admit(self, now):
+            changed = super()._admit(now)
+            for r in self.active:
+                if r.rid == 0 and not self.shrunk:
+                    r.nlist.set_capacity(8, 8)
+                    self.shrunk = True
+            return changed
+
+    ref = Server(keep_results=True)
+    a = _req(0, 3000, 60, density=0.2)
+    ref.submit(a)
+    ref.submit(_req(1, 2000, 40))
+    ref.run()
+    srv = Shrinking(keep_results=True)
+    a2 = _req(0, 3000, 60, density=0.2)
+    srv.submit(a2)
+    srv.submit(_req(1, 2000, 40))
+    st = srv.run()
+    print(f"steps repeated: {st['steps_repeated_for_list_overflow']}", flush=True)
+    assert st["steps_repeated_for_list_overflow"] >= 1
+    assert np.array_equal(a2.final_positions, a.final_positions)
+    assert np.array_equal(a2.final_forces, a.final_forces)
+
+
+@pytest.mark.parametrize("precision", [_lib.ESP_PREC_MIXED, _lib.ESP_PREC_FP64])
+def test_reused_solver_and_list_give_the_same_trajectory(precision):
+    """A request admitted after another one retired takes over its solver and neighbour list
+    (device arrays reused, parameters and mesh different); its trajectory equals, bit for bit,
+    the one it has with fresh ones."""
+    fresh = Server(precision=precision, keep_results=True, reuse_contexts=False)
+    a = _req(0, 2600, 45, density=0.12)
+    fresh.submit(a)
+    fresh.run()
+    srv = Server(precision=precision, keep_results=True)
+    srv.submit(_req(1, 5000, 20, density=0.2))       # larger, different mesh and cutoff
+    srv.run()
+    a2 = _req(0, 2600, 45, density=0.12)
+    srv.submit(a2)
+    st = srv.run()
+    assert st["contexts_reused"] >= 1
+    assert np.array_equal(a2.first_forces, a.first_forces)
+    assert np.array_equal(a2.final_positions, a.final_positions)
+    assert np.array_equal(a2.final_forces, a.final_forces)
