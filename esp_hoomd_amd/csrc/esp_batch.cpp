@@ -63,10 +63,12 @@ void EspBatch::build_plan(bool fp64)
     m_lds_stencil = m_lds_fx = m_lds_fy = m_lds_fz = 0;
     m_lds_pair = 0;
     m_any_mesh = false;
+    m_any_tilted = false;
     const unsigned int per_chunk = 256u / unsigned(kernel::BATCH_PAIR_LANES);
     for (unsigned int s = 0; s < n; ++s)
         {
         const kernel::BatchSys& d = m_desc[s];
+        m_any_tilted = m_any_tilted || d.pair.tilted;
         for (unsigned int c = 0; c < (d.pair.N + per_chunk - 1) / per_chunk; ++c)
             pair.push_back({s, c});
         for (unsigned int c = 0; c < (d.pair.N + 255u) / 256u; ++c)
@@ -162,7 +164,7 @@ template<typename R> void EspBatch::enqueue_stages(hipStream_t st)
     check_hip(kernel::batch_pack<R>(d, m_l.pack, st), "batch_pack");
     check_hip(hipEventRecord(m_fork, st), "hipEventRecord");
     check_hip(hipStreamWaitEvent(m_side, m_fork, 0), "hipStreamWaitEvent");
-    check_hip(kernel::batch_pair<R>(d, m_l.pair, m_lds_pair, m_side), "batch_pair");
+    check_hip(kernel::batch_pair<R>(d, m_l.pair, m_lds_pair, m_any_tilted, m_side), "batch_pair");
     check_hip(hipEventRecord(m_join, m_side), "hipEventRecord");
     if (m_any_mesh)
         {
@@ -255,13 +257,14 @@ void EspBatch::compute(unsigned int n, EspSolver* const* solvers, const SystemVi
 
     // shape key: everything that fixes the work lists and launch sizes
     std::vector<uint64_t> shape;
-    shape.reserve(size_t(n) * 12 + 2);
+    shape.reserve(size_t(n) * 13 + 2);
     shape.push_back(n);
     shape.push_back(fp64 ? 1 : 0);
     for (const kernel::BatchSys& d : m_desc)
         {
         const uint64_t v[] = {d.n, d.pair.N, d.n_tiles, uint64_t(d.g.nx), uint64_t(d.g.ny), uint64_t(d.g.nz), uint64_t(d.P),
-                              uint64_t(d.ppb_x), uint64_t(d.ncol_y), uint64_t(d.ncol_z), d.flags, d.pair.compact_tab ? 1u : 0u};
+                              uint64_t(d.ppb_x), uint64_t(d.ncol_y), uint64_t(d.ncol_z), d.flags, d.pair.compact_tab ? 1u : 0u,
+                              uint64_t(d.pair.tilted)};
         shape.insert(shape.end(), std::begin(v), std::end(v));
         }
     const bool reshape = shape != m_shape;

@@ -60,6 +60,7 @@ class EspBatch
     Lists m_l {};
     size_t m_lds_stencil = 0, m_lds_fx = 0, m_lds_fy = 0, m_lds_fz = 0, m_lds_pair = 0;
     bool m_any_mesh = false;
+    bool m_any_tilted = false;  // some system of the plan has a tilted box
 
     hipStream_t m_cap = nullptr;   // capture stream
     hipStream_t m_side = nullptr;  // pair branch

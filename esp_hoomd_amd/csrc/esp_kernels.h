@@ -185,7 +185,7 @@ template<typename R> hipError_t batch_fft_y(const BatchSys* d, BatchList l, int 
 template<typename R> hipError_t batch_fft_z_conv(const BatchSys* d, BatchList l, size_t lds, hipStream_t s);
 template<typename R> hipError_t batch_fft_x_c2r(const BatchSys* d, BatchList l, size_t lds, hipStream_t s);
 hipError_t batch_finalize(const BatchSys* d, BatchList l, hipStream_t s);
-template<typename R> hipError_t batch_pair(const BatchSys* d, BatchList l, size_t lds, hipStream_t s);
+template<typename R> hipError_t batch_pair(const BatchSys* d, BatchList l, size_t lds, bool any_tilted, hipStream_t s);
 template<typename R> hipError_t batch_pack(const BatchSys* d, BatchList l, hipStream_t s);
 size_t tp_lds_bytes(const MeshGeom& g, int P, bool fp64);
 bool batch_order_supported(int P);

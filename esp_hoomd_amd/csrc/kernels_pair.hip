@@ -76,8 +76,9 @@ template<typename R> __device__ __forceinline__ double rec_charge(const typename
         return double(__uint_as_float(p.w));
     }
 
-//! Minimum-image separation r_i - r_j of two records and the charge product.
-template<typename R>
+//! Minimum-image separation r_i - r_j of two records and the charge product.  TILT selects the
+//! tilted-box form at compile time, so the orthorhombic kernels carry none of its registers.
+template<typename R, bool TILT>
 __device__ __forceinline__ void rec_separation(const typename PairRec<R>::type& pi,
                                                const typename PairRec<R>::type& pj,
                                                const kernel::PairArgs& a,
@@ -88,7 +89,7 @@ __device__ __forceinline__ void rec_separation(const typename PairRec<R>::type& 
     {
     if constexpr (sizeof(R) == 8)
         {
-        if (a.tilted)
+        if constexpr (TILT)
             {
             dx = pi.x - pj.x;
             dy = pi.y - pj.y;
@@ -110,7 +111,7 @@ __device__ __forceinline__ void rec_separation(const typename PairRec<R>::type& 
         dx = ux * float(a.Lx * TWO_M32);
         dy = uy * float(a.Ly * TWO_M32);
         dz = uz * float(a.Lz * TWO_M32);
-        if (a.tilted)
+        if constexpr (TILT)
             {
             // d = B (u_i - u_j): the wrapped fraction differences through the lattice vectors
             dx = __fmaf_rn(uz, float(a.xz * a.Lz * TWO_M32), __fmaf_rn(uy, float(a.xy * a.Ly * TWO_M32), dx));
@@ -224,7 +225,7 @@ template<typename R> constexpr int pair_inflight()
     return sizeof(R) == 4 ? 8 : 2;
     }
 
-template<typename R, int G, bool COMPACT, int NF = pair_inflight<R>()>
+template<typename R, int G, bool COMPACT, int NF = pair_inflight<R>(), bool TILT = false>
 __device__ __forceinline__ void pair_particle(unsigned int i, int sub, const kernel::PairArgs& a, const R* __restrict__ tab)
     {
     const R rc = R(a.rc);
@@ -257,7 +258,7 @@ __device__ __forceinline__ void pair_particle(unsigned int i, int sub, const ker
         auto add = [&](const Rec& pj)
             {
             R dx, dy, dz, qiqj;
-            rec_separation<R>(pi, pj, a, dx, dy, dz, qiqj);
+            rec_separation<R, TILT>(pi, pj, a, dx, dy, dz, qiqj);
             const R rsq = dx * dx + dy * dy + dz * dz;
             // outside the cutoff, uncharged, or coincident (r < 1e-12: no defined force)
             if (!(rsq < rc2 && rsq >= R(1.0e-24)) || qiqj == R(0))
@@ -311,7 +312,7 @@ __device__ __forceinline__ void pair_particle(unsigned int i, int sub, const ker
                 continue;
             const Rec pj = pq[j];
             R dx, dy, dz, qiqj;
-            rec_separation<R>(pi, pj, a, dx, dy, dz, qiqj);
+            rec_separation<R, TILT>(pi, pj, a, dx, dy, dz, qiqj);
             const R rsq = dx * dx + dy * dy + dz * dz;
             if (rsq < R(1.0e-20))
                 continue;
@@ -369,7 +370,7 @@ __device__ __forceinline__ const R* stage_compact_table(const kernel::PairArgs& 
 //! (block b on XCD (b + k) mod 8 for a launch-dependent k), so the blocks with one value of
 //! b mod 8 share an XCD: they take one contiguous eighth of the particles and sweep it
 //! together (interleaved), so that XCD's L2 holds the neighbour records of one moving window.
-template<typename R, int G, bool COMPACT, int NF>
+template<typename R, int G, bool COMPACT, int NF, bool TILT>
 __global__ void __launch_bounds__(256) k_pair(kernel::PairArgs a)
     {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -388,13 +389,13 @@ __global__ void __launch_bounds__(256) k_pair(kernel::PairArgs a)
     const unsigned int p0 = x * span;
     const unsigned int p1 = min(a.N, p0 + span);
     for (unsigned int i = p0 + j * per_pass + threadIdx.x / G; i < p1; i += per_x * per_pass)
-        pair_particle<R, G, COMPACT, NF>(i, sub, a, tab);
+        pair_particle<R, G, COMPACT, NF, TILT>(i, sub, a, tab);
     }
 
 //! Batched pair correction: items are (system, chunk of 256 / G particles).  The compact
 //! table is restaged whenever the block moves to another system.  Systems whose flags
 //! exclude the pair part get zero force and virial (the outputs are always overwritten).
-template<typename R, int G>
+template<typename R, int G, bool TILT>
 __global__ void __launch_bounds__(256) kb_pair(const kernel::BatchSys* __restrict__ ds, kernel::BatchList l)
     {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -426,10 +427,18 @@ __global__ void __launch_bounds__(256) kb_pair(const kernel::BatchSys* __restric
                 }
             continue;
             }
-        if (compact)
-            pair_particle<R, G, true, 2>(i, sub, a, reinterpret_cast<const R*>(smem));
+        // the tilted form only for tilted systems: an orthorhombic one keeps the bits it has alone
+        if (TILT && a.tilted)
+            {
+            if (compact)
+                pair_particle<R, G, true, 2, true>(i, sub, a, reinterpret_cast<const R*>(smem));
+            else
+                pair_particle<R, G, false, 2, true>(i, sub, a, reinterpret_cast<const R*>(a.split_tab));
+            }
+        else if (compact)
+            pair_particle<R, G, true, 2, false>(i, sub, a, reinterpret_cast<const R*>(smem));
         else
-            pair_particle<R, G, false, 2>(i, sub, a, reinterpret_cast<const R*>(a.split_tab));
+            pair_particle<R, G, false, 2, false>(i, sub, a, reinterpret_cast<const R*>(a.split_tab));
         }
     }
 
@@ -479,11 +488,18 @@ namespace kernel
 template<typename R, int G> static void launch_pair(const PairArgs& a, unsigned int nb, hipStream_t s)
     {
     constexpr int NF = dev::pair_inflight<R>();
+    const size_t lds = size_t(a.n_compact) * (a.compact_deg + 1) * sizeof(R);
     if (a.compact_tab != nullptr)
-        hipLaunchKernelGGL((dev::k_pair<R, G, true, NF>), dim3(nb), dim3(256), size_t(a.n_compact) * (a.compact_deg + 1) * sizeof(R), s,
-                           a);
+        {
+        if (a.tilted)
+            hipLaunchKernelGGL((dev::k_pair<R, G, true, NF, true>), dim3(nb), dim3(256), lds, s, a);
+        else
+            hipLaunchKernelGGL((dev::k_pair<R, G, true, NF, false>), dim3(nb), dim3(256), lds, s, a);
+        }
+    else if (a.tilted)
+        hipLaunchKernelGGL((dev::k_pair<R, G, false, NF, true>), dim3(nb), dim3(256), 0, s, a);
     else
-        hipLaunchKernelGGL((dev::k_pair<R, G, false, NF>), dim3(nb), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((dev::k_pair<R, G, false, NF, false>), dim3(nb), dim3(256), 0, s, a);
     }
 
 template<typename R> hipError_t pack_positions_charges(const PairArgs& a, hipStream_t s)
@@ -539,14 +555,19 @@ template<typename R> hipError_t pair_forces(const PairArgs& a, int lanes_per_par
 template hipError_t pair_forces<double>(const PairArgs&, int, unsigned int, hipStream_t);
 template hipError_t pair_forces<float>(const PairArgs&, int, unsigned int, hipStream_t);
 
-template<typename R> hipError_t batch_pair(const BatchSys* d, BatchList l, size_t lds, hipStream_t s)
+template<typename R> hipError_t batch_pair(const BatchSys* d, BatchList l, size_t lds, bool any_tilted, hipStream_t s)
     {
     if (l.n == 0)
         return hipSuccess;
-    hipLaunchKernelGGL((dev::kb_pair<R, BATCH_PAIR_LANES>), dim3(l.grid), dim3(256), lds, s, d, l);
+    // the tilted-box instantiation only when some system of the batch is tilted (it handles
+    // orthorhombic systems too, as the special case of zero tilt)
+    if (any_tilted)
+        hipLaunchKernelGGL((dev::kb_pair<R, BATCH_PAIR_LANES, true>), dim3(l.grid), dim3(256), lds, s, d, l);
+    else
+        hipLaunchKernelGGL((dev::kb_pair<R, BATCH_PAIR_LANES, false>), dim3(l.grid), dim3(256), lds, s, d, l);
     return hipGetLastError();
     }
-template hipError_t batch_pair<double>(const BatchSys*, BatchList, size_t, hipStream_t);
-template hipError_t batch_pair<float>(const BatchSys*, BatchList, size_t, hipStream_t);
+template hipError_t batch_pair<double>(const BatchSys*, BatchList, size_t, bool, hipStream_t);
+template hipError_t batch_pair<float>(const BatchSys*, BatchList, size_t, bool, hipStream_t);
     } // namespace kernel
     } // namespace esp
