@@ -23,8 +23,10 @@ suite ties that reference to the direct Ewald sum at this size):
   density 0.1, per-system parameters from the autotuner (meshes 16^3-64^3, P in {6, 8, 10}),
   mixed precision, one launch per stage replayed from a captured hipGraph;
 * served: a Poisson stream of MD requests (N log-uniform in [1e3, 2e5], density U[0.05, 0.2],
-  10-1000 steps) through the continuous-batching server for a fixed wall time; every finished
-  request's force error is measured afterwards.
+  10-1000 steps) through the continuous-batching server: first offered faster than it can serve
+  (10 s, its capacity in particle-steps/s), then offered at 70 % of that capacity for a fixed
+  wall time (latencies, queue delays); every finished request's force error is measured
+  afterwards.
 
 `cpu_baseline` times the oracle (FP64 CPU ESP on OpenMP) on the same equilibrated system with
 the headline parameters.
@@ -423,27 +425,50 @@ def batched_leg(n_sys, steps, warmup, seed=6):
     }
 
 
-def served_leg(seconds, rate, seed=7, budget_gb=64):
+MEAN_REQUEST_WORK = (2e5 - 1e3) / math.log(200) * (1000 - 10) / math.log(100)  # E[N] E[steps] of the stream
+
+
+def _serve_run(scales, seconds, rate, seed, budget_gb, verify):
     from esp_hoomd_amd import _lib
     from esp_hoomd_amd.serve import Server, poisson_requests
-    n = int(rate * seconds * 1.5) + 8
-    reqs = poisson_requests(n, rate=rate, seed=seed)
+    reqs = poisson_requests(int(rate * seconds * 1.5) + 8, rate=rate, seed=seed)
     srv = Server(hbm_budget=int(budget_gb * (1 << 30)), max_active=128, precision=_lib.ESP_PREC_MIXED, target=TARGET,
-                 keep_final_positions=True)
-    t0 = time.perf_counter()
-    scales = srv.calibrate()
-    calib_s = time.perf_counter() - t0
+                 keep_final_positions=verify)
+    srv.method_scales = dict(scales)
     for r in reqs:
         srv.submit(r)
     st = srv.run(time_limit=seconds)
     st["offered_rate_per_s"] = rate
-    st["hbm_budget_bytes"] = int(budget_gb * (1 << 30))
-    # every finished request's force error at its final positions, against the tight reference
+    if verify:
+        # every finished request's force error at its final positions, against the tight reference
+        t0 = time.perf_counter()
+        errs = np.asarray(srv.verify_results()) if srv.done else np.zeros(1)
+        st["force_error"] = {"verified_results": len(srv.done), "p50": float(np.median(errs)),
+                             "p99": float(np.percentile(errs, 99)), "max": float(errs.max()),
+                             "over_target": int((errs > TARGET).sum()), "verify_s": time.perf_counter() - t0}
+    return st
+
+
+def served_leg(seconds, rate=None, overload_rate=100.0, capacity_seconds=10.0, seed=7, budget_gb=64):
+    """Two phases on the same request generator: the server's capacity (a stream offered faster
+    than it can serve, `capacity_seconds`: the particle-steps/s it sustains), then the served
+    stream at `rate` -- by default 70 % of the measured capacity -- for `seconds`, with
+    latencies, queue delays and every finished result's force error."""
+    from esp_hoomd_amd import _lib
+    from esp_hoomd_amd.serve import Server
     t0 = time.perf_counter()
-    errs = np.asarray(srv.verify_results()) if srv.done else np.zeros(1)
-    st["force_error"] = {"verified_results": len(srv.done), "p50": float(np.median(errs)),
-                         "p99": float(np.percentile(errs, 99)), "max": float(errs.max()),
-                         "over_target": int((errs > TARGET).sum()), "verify_s": time.perf_counter() - t0}
+    scales = Server(precision=_lib.ESP_PREC_MIXED, target=TARGET).calibrate()
+    calib_s = time.perf_counter() - t0
+    cap = _serve_run(scales, capacity_seconds, overload_rate, seed + 1, budget_gb, verify=False)
+    capacity = {"particle_steps_per_s": cap["particle_steps_per_s"], "solves_per_s": cap["solves_per_s"],
+                "requests_per_s": cap["requests_per_s"], "offered_rate_per_s": overload_rate,
+                "seconds": capacity_seconds, "mean_active": cap["mean_active"],
+                "queue_delay_s_p50": cap["queue_delay_s_p50"]}
+    if rate is None:
+        rate = 0.7 * cap["particle_steps_per_s"] / MEAN_REQUEST_WORK
+    st = _serve_run(scales, seconds, rate, seed, budget_gb, verify=True)
+    st["capacity"] = capacity
+    st["hbm_budget_bytes"] = int(budget_gb * (1 << 30))
     st["calibration"] = {"method_scale_by_density": scales, "seconds": calib_s}
     return st
 
@@ -462,7 +487,7 @@ def main(argv=None):
     ap.add_argument("--batch-systems", type=int, default=512)
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--serve-rate", type=float, default=0.0,
-                    help="offered requests/s of the served leg (default: 70 %% of the batched capacity)")
+                    help="offered requests/s of the served leg (default: 70 %% of the server's measured capacity)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", type=float, default=0.0, metavar="MIB",
                     help="only run the record-copy kernel over MIB MiB, --steps times (counter calibration)")
@@ -546,11 +571,8 @@ def main(argv=None):
         out_legs["batched"] = batched_leg(args.batch_systems, max(args.steps // 5, 5), 3)
         progress("batched leg done")
     if "served" in legs:
-        # offered rate ~70 % of the batched capacity for the mean request (E[N] E[steps])
-        cap_ps = out_legs.get("batched", {}).get("particle_steps_per_s", 3e8)
-        mean_work = (2e5 - 1e3) / math.log(200) * (1000 - 10) / math.log(100)
-        rate = args.serve_rate if args.serve_rate > 0 else 0.7 * cap_ps / mean_work
-        out_legs["served"] = served_leg(args.serve_seconds, rate=rate)
+        # the server's capacity first, then the stream offered at 70 % of it (or --serve-rate)
+        out_legs["served"] = served_leg(args.serve_seconds, rate=args.serve_rate if args.serve_rate > 0 else None)
         progress("served leg done")
 
     if head is not None:
@@ -594,7 +616,8 @@ def main(argv=None):
         out["speedup_vs_fp64_single"] = {
             "mixed_single": tuned["mixed"][1]["particle_steps_per_s"] / f64,
             "mixed_batched": out_legs.get("batched", {}).get("particle_steps_per_s", 0.0) / f64,
-            "mixed_served": out_legs.get("served", {}).get("particle_steps_per_s", 0.0) / f64,
+            "mixed_served": out_legs.get("served", {}).get("capacity", {}).get("particle_steps_per_s", 0.0) / f64,
+            "mixed_served_at_offered_load": out_legs.get("served", {}).get("particle_steps_per_s", 0.0) / f64,
         }
     if not args.no_cpu_baseline and head is not None:
         out["cpu_baseline"] = cpu_baseline(pos, q, L, params)
