@@ -483,7 +483,7 @@ def main(argv=None):
                     help="comma list of literal, fp64, mixed, md, shuffled, batched, served")
     ap.add_argument("--equilibrate", type=int, default=1000,
                     help="Langevin steps (T = 1, gamma = 1, seed 4) before anything is measured")
-    ap.add_argument("--md-steps", type=int, default=300, help="timed NVE steps of the md leg")
+    ap.add_argument("--md-steps", type=int, default=1000, help="timed NVE steps of the md leg (the 1M-particle configuration's 1,000)")
     ap.add_argument("--batch-systems", type=int, default=512)
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--serve-rate", type=float, default=0.0,

@@ -87,6 +87,7 @@ class Request:
     final_positions: Optional[np.ndarray] = field(default=None, repr=False)
     final_forces: Optional[np.ndarray] = field(default=None, repr=False)
     force_error: Optional[float] = None  # relative RMS electrostatic force error (verify_results)
+    error: Optional[str] = None          # why the request failed (set up, admission); it takes no steps
     prepared: Optional[tuple] = field(default=None, repr=False)
     snapshot: Optional[tuple] = field(default=None, repr=False)  # state before a tick that rebuilds its list
     first_tick: bool = False                     # admitted: the next tick computes its first forces
@@ -133,6 +134,7 @@ class Server:
         self.batch = Batch(use_graph=use_graph)       # every resident request, every tick
         self.md = BatchMD()                           # integration, WCA and list upkeep of the resident set
         self._pinned = PinnedPool()                   # staging of the admitted requests' uploads
+        self._orphans: List[State] = []               # states of failed admissions, freed after a wait
         self._held_uploaded = False                   # the batched MD holds some member this tick
         # solvers and lists of retired requests, handed to new ones (device memory reused; at
         # most max_active held beside the resident footprint)
@@ -156,6 +158,7 @@ class Server:
         self.pending: List[Tuple[float, int, Request]] = []
         self.active: List[Request] = []
         self.done: List[Request] = []
+        self.failed: List[Request] = []
         self.resident_bytes = 0
         self.peak_bytes = 0
         self.tick_ms: List[float] = []
@@ -175,8 +178,21 @@ class Server:
         """Queue a request.  Its particle data and initial velocities (the client's input) are
         generated here, so a stream submitted ahead of run() is served without client-side work
         in the loop."""
-        req.prepared = self._prepare(req)
+        try:
+            req.prepared = self._prepare(req)
+        except (ValueError, ArithmeticError, _lib.EspError) as e:
+            self._fail(req, f"set-up: {e}")
+            return
+        if req.prepared[4] > self.hbm_budget:
+            self._fail(req, f"footprint {req.prepared[4]} bytes exceeds the HBM budget of {self.hbm_budget} bytes")
+            req.prepared = None
+            return
         heapq.heappush(self.pending, (req.arrival, req.rid, req))
+
+    def _fail(self, req: Request, msg: str) -> None:
+        """A request that cannot be served: recorded with its reason; the others are unaffected."""
+        req.error = msg
+        self.failed.append(req)
 
     # ---- accuracy ---------------------------------------------------------------------
     def calibrate(self, densities: Sequence[float] = (0.05, 0.1, 0.2), N: int = 20_000, q2_frac: float = 0.1,
@@ -249,6 +265,8 @@ class Server:
     def _prepare(self, req: Request) -> tuple:
         """The request's particles, parameters, footprint and initial velocities (host work, done
         at submission when the request is known ahead)."""
+        if not (req.N >= 2 and req.density > 0 and math.isfinite(req.density) and req.steps >= 1):
+            raise ValueError(f"request {req.rid}: needs N >= 2, a positive density and steps >= 1")
         pos, q, L = req.setup()
         params, nbar = self._params_for(len(pos), L)
         vel = maxwell_boltzmann(np.ones(len(pos)), req.kT, seed=req.seed + 1)
@@ -272,25 +290,33 @@ class Server:
             heapq.heappop(self.pending)
             req.params, req.footprint, req.prepared = params, need, None
             ta = time.perf_counter()
-            # force and virial are overwritten by the first forces
-            req.state = State(pos, q, [L, L, L], velocities=vel, zero_outputs=False, pooled=True, stream=self.stream,
-                              staging=self._pinned)
-            tb = time.perf_counter()
-            # a retired request's solver and list, when one is free: their device arrays are
-            # reused (set_params / reset rebuild everything that depends on the parameters)
-            pooled = self._ctx_pool.pop() if self._ctx_pool else None
-            if pooled is not None:
-                req.solver, req.nlist = pooled
-                req.nlist.reset(max(params.rcut, self.wca.r_cut))
-                self.contexts_reused += 1
-            else:
-                req.nlist = NeighborList(buffer=SKIN, inner=self.wca.r_cut)
-                req.nlist._ensure(max(params.rcut, self.wca.r_cut))
-            tc = time.perf_counter()
-            if pooled is not None:
-                req.solver.set_params(params)
-            else:
-                req.solver = Solver(params)
+            try:
+                # force and virial are overwritten by the first forces
+                req.state = State(pos, q, [L, L, L], velocities=vel, zero_outputs=False, pooled=True, stream=self.stream,
+                                  staging=self._pinned)
+                tb = time.perf_counter()
+                # a retired request's solver and list, when one is free: their device arrays are
+                # reused (set_params / reset rebuild everything that depends on the parameters)
+                pooled = self._ctx_pool.pop() if self._ctx_pool else None
+                if pooled is not None:
+                    req.solver, req.nlist = pooled
+                    req.nlist.reset(max(params.rcut, self.wca.r_cut))
+                    self.contexts_reused += 1
+                else:
+                    req.nlist = NeighborList(buffer=SKIN, inner=self.wca.r_cut)
+                    req.nlist._ensure(max(params.rcut, self.wca.r_cut))
+                tc = time.perf_counter()
+                if pooled is not None:
+                    req.solver.set_params(params)
+                else:
+                    req.solver = Solver(params)
+            except (ValueError, _lib.EspError) as e:
+                # this request alone fails; its uploads (if any) drain with the stream
+                self._fail(req, f"admission: {e}")
+                if req.state is not None:
+                    self._orphans.append(req.state)
+                req.state = req.solver = req.nlist = None
+                continue
             self.admit_s["state"] += tb - ta
             self.admit_s["nlist"] += tc - tb
             self.admit_s["solver"] += time.perf_counter() - tc
@@ -368,6 +394,9 @@ class Server:
         self.tick_active.append(stepped)
         self.tick_particles.append(particles)
         self._inflight = None
+        for st in self._orphans:
+            st.release_staging()
+        self._orphans = []
 
     def _snapshot(self, r: Request) -> None:
         """Copy a request's positions, velocities and forces (the state a step starts from) on
@@ -490,6 +519,7 @@ class Server:
         return {
             "wall_s": wall,
             "requests_done": len(self.done),
+            "requests_failed": len(self.failed),
             "requests_per_s": len(self.done) / wall if wall > 0 else 0.0,
             "solves": steps,
             "solves_per_s": steps / wall if wall > 0 else 0.0,

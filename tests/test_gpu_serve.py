@@ -163,3 +163,29 @@ def test_reused_solver_and_list_give_the_same_trajectory(precision):
     assert np.array_equal(a2.first_forces, a.first_forces)
     assert np.array_equal(a2.final_positions, a.final_positions)
     assert np.array_equal(a2.final_forces, a.final_forces)
+
+
+def test_failing_requests_do_not_disturb_the_others():
+    """A request with an invalid specification and one whose footprint exceeds the whole HBM
+    budget fail with their reason; the request served beside them computes the same trajectory
+    as alone."""
+    ref = Server(keep_results=True)
+    a = _req(0, 2000, 30)
+    ref.submit(a)
+    ref.run()
+    srv0 = Server()
+    p, nbar = srv0._params_for(2000, (2000 / 0.1) ** (1 / 3))
+    budget = 4 * footprint_bytes(2000, p, nbar)
+    srv = Server(keep_results=True, hbm_budget=budget)
+    bad = _req(1, 2000, 30, density=-0.1)
+    big = _req(2, 150_000, 10)
+    a2 = _req(0, 2000, 30)
+    for r in (bad, big, a2):
+        srv.submit(r)
+    st = srv.run()
+    print(f"failures: {bad.error!r}; {big.error!r}", flush=True)
+    assert st["requests_failed"] == 2 and st["requests_done"] == 1
+    assert bad.error and "density" in bad.error
+    assert big.error and "budget" in big.error
+    assert np.array_equal(a2.final_positions, a.final_positions)
+    assert np.array_equal(a2.final_forces, a.final_forces)
