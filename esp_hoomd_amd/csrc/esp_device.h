@@ -189,8 +189,24 @@ template<typename R> __device__ __forceinline__ void bfly5(C2<R>* v, int sign)
     v[3] = csub(a2, b2);
     }
 
+//! Quotient by a divisor fixed for a loop, as one v_mul_hi_u32 instead of the ~20 instructions
+//! of a general 32-bit division.  With m = ceil(2^32 / d) = (2^32 + e) / d, 0 <= e < d, and
+//! n = q d + r: n m / 2^32 = q + (r + n e / 2^32) / d, whose floor is q while n e < 2^32, which
+//! n d <= 2^32 guarantees.  The FFT passes divide indices below 2^13 by divisors below 2^12.
+//! d = 1 (m would be 2^32) returns n.
+struct FastDiv
+    {
+    unsigned int d, m;
+    __device__ __forceinline__ explicit FastDiv(int div) : d(unsigned(div)), m(div > 1 ? 0xffffffffu / unsigned(div) + 1u : 0u) {}
+    __device__ __forceinline__ int quot(int n) const
+        {
+        return d == 1u ? n : int(__umulhi(unsigned(n), m));
+        }
+    };
+
 //! One Stockham stage over nlines lines of length N stored with line stride ls in LDS.
-//! tw[m] = exp(-2 pi i m / N); the inverse uses its conjugate.
+//! tw[m] = exp(-2 pi i m / N); the inverse uses its conjugate.  The twiddle index
+//! k r (N / (Ns RAD)) <= (Ns - 1)(RAD - 1) N / (Ns RAD) stays below N, so it needs no wrap.
 template<typename R, int RAD>
 __device__ __forceinline__ void stockham_stage(const C2<R>* __restrict__ in,
                                                C2<R>* __restrict__ out,
@@ -205,21 +221,23 @@ __device__ __forceinline__ void stockham_stage(const C2<R>* __restrict__ in,
     {
     const int nb = N / RAD;
     const int twstep = N / (Ns * RAD);
+    const FastDiv by_nb(nb), by_ns(Ns);
     for (int idx = tid; idx < nlines * nb; idx += nthreads)
         {
-        const int line = idx / nb;
+        const int line = by_nb.quot(idx);
         const int j = idx - line * nb;
         const C2<R>* src = in + line * ls;
         C2<R>* dst = out + line * ls;
         C2<R> v[RAD];
-        const int k = j % Ns;
+        const int jq = by_ns.quot(j);
+        const int k = j - jq * Ns;
 #pragma unroll
         for (int r = 0; r < RAD; ++r)
             v[r] = src[j + r * nb];
 #pragma unroll
         for (int r = 1; r < RAD; ++r)
             {
-            C2<R> w = tw[(k * r * twstep) % N];
+            C2<R> w = tw[k * r * twstep];
             if (sign > 0)
                 w.y = -w.y;
             v[r] = cmul(v[r], w);
@@ -232,7 +250,7 @@ __device__ __forceinline__ void stockham_stage(const C2<R>* __restrict__ in,
             bfly4(v, sign);
         else
             bfly5(v, sign);
-        const int base = (j / Ns) * Ns * RAD + k;
+        const int base = jq * Ns * RAD + k;
 #pragma unroll
         for (int r = 0; r < RAD; ++r)
             dst[base + r * Ns] = v[r];

@@ -344,6 +344,7 @@ __device__ __forceinline__ void fft_x_r2c_block(int blk,
     const int nx = g.nx;
     const int ls = nx + 1;
     const double inv_scale = double(fx[1]);
+    const FastDiv by_nx(nx), by_nxh(g.nxh);
     C2<R>* tw = reinterpret_cast<C2<R>*>(smem);
     C2<R>* a = tw + nx;
     C2<R>* b = a + pairs_per_block * ls;
@@ -355,7 +356,7 @@ __device__ __forceinline__ void fft_x_r2c_block(int blk,
         pairs_per_block * nx, threadIdx.x, blockDim.x,
         [&](int idx)
         {
-            const int pr = idx / nx;
+            const int pr = by_nx.quot(idx);
             const int x = idx - pr * nx;
             const int l0 = 2 * (pair0 + pr);
             const int l1 = l0 + 1;
@@ -368,7 +369,7 @@ __device__ __forceinline__ void fft_x_r2c_block(int blk,
         },
         [&](int idx, const C2<R>& v)
         {
-            const int pr = idx / nx;
+            const int pr = by_nx.quot(idx);
             a[pr * ls + idx - pr * nx] = v;
         });
     __syncthreads();
@@ -376,7 +377,7 @@ __device__ __forceinline__ void fft_x_r2c_block(int blk,
     const int nxh = g.nxh;
     for (int idx = threadIdx.x; idx < pairs_per_block * nxh; idx += blockDim.x)
         {
-        const int pr = idx / nxh;
+        const int pr = by_nxh.quot(idx);
         const int k = idx - pr * nxh;
         const int l0 = 2 * (pair0 + pr);
         const int l1 = l0 + 1;
@@ -419,6 +420,7 @@ __device__ __forceinline__ void fft_x_c2r_block(int blk,
     const int nx = g.nx;
     const int nxh = g.nxh;
     const int ls = nx + 1;
+    const FastDiv by_nx(nx), by_nxh(nxh);
     C2<R>* tw = reinterpret_cast<C2<R>*>(smem);
     C2<R>* a = tw + nx;
     C2<R>* b = a + pairs_per_block * ls;
@@ -434,7 +436,7 @@ __device__ __forceinline__ void fft_x_c2r_block(int blk,
         pairs_per_block * nxh, threadIdx.x, blockDim.x,
         [&](int idx)
         {
-            const int pr = idx / nxh;
+            const int pr = by_nxh.quot(idx);
             const int k = idx - pr * nxh;
             const int l0 = 2 * (pair0 + pr);
             const int l1 = l0 + 1;
@@ -447,7 +449,7 @@ __device__ __forceinline__ void fft_x_c2r_block(int blk,
         },
         [&](int idx, const Pair2& v)
         {
-            const int pr = idx / nxh;
+            const int pr = by_nxh.quot(idx);
             const int k = idx - pr * nxh;
             // Z[k] = X0[k] + i X1[k]; Z[n-k] = conj(X0[k]) + i conj(X1[k])
             a[pr * ls + k] = C2<R> {v.x0.x - v.x1.y, v.x0.y + v.x1.x};
@@ -458,7 +460,7 @@ __device__ __forceinline__ void fft_x_c2r_block(int blk,
     C2<R>* res = lds_fft<R>(a, b, tw, plan, ls, pairs_per_block, +1, threadIdx.x, blockDim.x);
     for (int idx = threadIdx.x; idx < pairs_per_block * nx; idx += blockDim.x)
         {
-        const int pr = idx / nx;
+        const int pr = by_nx.quot(idx);
         const int x = idx - pr * nx;
         const int l0 = 2 * (pair0 + pr);
         const int l1 = l0 + 1;
@@ -503,19 +505,20 @@ __device__ __forceinline__ void fft_y_block(int blk,
     const int z = blk / nbx;
     const int x0 = (blk - z * nbx) * ncol;
     const int nc = min(ncol, nxh - x0);
+    const FastDiv by_ncol(ncol);
     for (int i = threadIdx.x; i < ny; i += blockDim.x)
         tw[i] = tw_g[i];
     staged_copy<8, C2<R>>(
         ny * ncol, threadIdx.x, blockDim.x,
         [&](int idx)
         {
-            const int y = idx / ncol;
+            const int y = by_ncol.quot(idx);
             const int xi = idx - y * ncol;
             return xi < nc ? spec[(size_t(z) * ny + y) * g.sxp + x0 + xi] : C2<R> {R(0), R(0)};
         },
         [&](int idx, const C2<R>& v)
         {
-            const int y = idx / ncol;
+            const int y = by_ncol.quot(idx);
             const int xi = idx - y * ncol;
             if (xi < nc)
                 a[xi * ls + y] = v;
@@ -524,7 +527,7 @@ __device__ __forceinline__ void fft_y_block(int blk,
     C2<R>* res = lds_fft<R>(a, b, tw, plan, ls, nc, sign, threadIdx.x, blockDim.x);
     for (int idx = threadIdx.x; idx < ny * ncol; idx += blockDim.x)
         {
-        const int y = idx / ncol;
+        const int y = by_ncol.quot(idx);
         const int xi = idx - y * ncol;
         if (xi < nc)
             spec[(size_t(z) * ny + y) * g.sxp + x0 + xi] = res[xi * ls + y];
@@ -564,19 +567,20 @@ __device__ __forceinline__ void fft_z_conv_block(int blk,
     const int y = blk / nbx;
     const int x0 = (blk - y * nbx) * ncol;
     const int nc = min(ncol, nxh - x0);
+    const FastDiv by_ncol(ncol);
     for (int i = threadIdx.x; i < nz; i += blockDim.x)
         tw[i] = tw_g[i];
     staged_copy<8, C2<R>>(
         nz * ncol, threadIdx.x, blockDim.x,
         [&](int idx)
         {
-            const int z = idx / ncol;
+            const int z = by_ncol.quot(idx);
             const int xi = idx - z * ncol;
             return xi < nc ? spec[(size_t(z) * ny + y) * g.sxp + x0 + xi] : C2<R> {R(0), R(0)};
         },
         [&](int idx, const C2<R>& v)
         {
-            const int z = idx / ncol;
+            const int z = by_ncol.quot(idx);
             const int xi = idx - z * ncol;
             if (xi < nc)
                 a[xi * ls + z] = v;
@@ -596,7 +600,7 @@ __device__ __forceinline__ void fft_z_conv_block(int blk,
         nz * ncol, threadIdx.x, blockDim.x,
         [&](int idx)
         {
-            const int z = idx / ncol;
+            const int z = by_ncol.quot(idx);
             const int xi = idx - z * ncol;
             GG v = {0.0, 0.0};
             if (xi < nc)
@@ -610,7 +614,7 @@ __device__ __forceinline__ void fft_z_conv_block(int blk,
         },
         [&](int idx, const GG& gg)
         {
-        const int z = idx / ncol;
+        const int z = by_ncol.quot(idx);
         const int xi = idx - z * ncol;
         if (xi >= nc)
             return;
@@ -658,7 +662,7 @@ __device__ __forceinline__ void fft_z_conv_block(int blk,
     C2<R>* res2 = lds_fft<R>(res, other, tw, plan, ls, nc, +1, threadIdx.x, blockDim.x);
     for (int idx = threadIdx.x; idx < nz * ncol; idx += blockDim.x)
         {
-        const int z = idx / ncol;
+        const int z = by_ncol.quot(idx);
         const int xi = idx - z * ncol;
         if (xi < nc)
             spec[(size_t(z) * ny + y) * g.sxp + x0 + xi] = res2[xi * ls + z];
