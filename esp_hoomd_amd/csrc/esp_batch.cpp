@@ -15,6 +15,17 @@ size_t fft_lds(int n, int cols, size_t complex_bytes)
     {
     return (size_t(n) + 2 * size_t(cols) * (size_t(n) + 1)) * complex_bytes;
     }
+
+//! The fused xy-plane FFT passes for small meshes (on unless ESP_BATCH_FUSED_XY=0; both forms
+//! give the same bits).
+bool fused_xy_enabled()
+    {
+    static const bool on = [] {
+        const char* e = getenv("ESP_BATCH_FUSED_XY");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+    }
 } // namespace
 
 EspBatch::EspBatch() = default;
@@ -59,8 +70,8 @@ void EspBatch::build_plan(bool fp64)
     using kernel::BatchItem;
     const unsigned int n = unsigned(m_desc.size());
     const size_t cplx = fp64 ? 16 : 8;
-    std::vector<BatchItem> pack, pair, bin, scan, tiles, fx, fy, fz;
-    m_lds_stencil = m_lds_fx = m_lds_fy = m_lds_fz = 0;
+    std::vector<BatchItem> pack, pair, bin, scan, tiles, fx, fy, fz, fxy;
+    m_lds_stencil = m_lds_fx = m_lds_fy = m_lds_fz = m_lds_fxy = 0;
     m_lds_pair = 0;
     m_any_mesh = false;
     m_any_tilted = false;
@@ -84,30 +95,41 @@ void EspBatch::build_plan(bool fp64)
         for (unsigned int t = 0; t < d.n_tiles; ++t)
             tiles.push_back({s, t});
         const MeshGeom& g = d.g;
-        const unsigned int npairs = unsigned(g.ny * g.nz + 1) / 2;
-        for (unsigned int b = 0; b < (npairs + d.ppb_x - 1) / unsigned(d.ppb_x); ++b)
-            fx.push_back({s, b});
-        const unsigned int nby = unsigned((g.nxh + d.ncol_y - 1) / d.ncol_y) * unsigned(g.nz);
-        for (unsigned int b = 0; b < nby; ++b)
-            fy.push_back({s, b});
+        if (fused_xy_enabled() && kernel::fft_xy_fused(g, fp64))
+            {
+            // small planes: the x and y passes fused, one workgroup per plane
+            for (int z = 0; z < g.nz; ++z)
+                fxy.push_back({s, unsigned(z)});
+            m_lds_fxy = std::max(m_lds_fxy, kernel::fft_xy_lds(g, fp64));
+            }
+        else
+            {
+            const unsigned int npairs = unsigned(g.ny * g.nz + 1) / 2;
+            for (unsigned int b = 0; b < (npairs + d.ppb_x - 1) / unsigned(d.ppb_x); ++b)
+                fx.push_back({s, b});
+            const unsigned int nby = unsigned((g.nxh + d.ncol_y - 1) / d.ncol_y) * unsigned(g.nz);
+            for (unsigned int b = 0; b < nby; ++b)
+                fy.push_back({s, b});
+            m_lds_fx = std::max(m_lds_fx, fft_lds(g.nx, d.ppb_x, cplx));
+            m_lds_fy = std::max(m_lds_fy, fft_lds(g.ny, d.ncol_y, cplx));
+            }
         for (unsigned int b = 0; b < d.n_zblk; ++b)
             fz.push_back({s, b});
         m_lds_stencil = std::max(m_lds_stencil, kernel::tp_lds_bytes(g, d.P, fp64));
-        m_lds_fx = std::max(m_lds_fx, fft_lds(g.nx, d.ppb_x, cplx));
-        m_lds_fy = std::max(m_lds_fy, fft_lds(g.ny, d.ncol_y, cplx));
         m_lds_fz = std::max(m_lds_fz, fft_lds(g.nz, d.ncol_z, cplx));
         }
-    for (size_t lds : {m_lds_stencil, m_lds_fx, m_lds_fy, m_lds_fz, m_lds_pair})
+    for (size_t lds : {m_lds_stencil, m_lds_fx, m_lds_fy, m_lds_fz, m_lds_fxy, m_lds_pair})
         if (lds > 64 * 1024)
             throw EspError(STATUS_UNSUPPORTED, "a batched stage needs more than 64 KiB of LDS");
 
     std::vector<BatchItem> all;
-    all.reserve(pack.size() + pair.size() + bin.size() + scan.size() + tiles.size() + fx.size() + fy.size() + fz.size());
+    all.reserve(pack.size() + pair.size() + bin.size() + scan.size() + tiles.size() + fx.size() + fy.size() + fz.size() +
+                fxy.size());
     auto place = [&](const std::vector<BatchItem>& v, size_t& off) {
         off = all.size();
         all.insert(all.end(), v.begin(), v.end());
     };
-    size_t o_pack, o_pair, o_bin, o_scan, o_tiles, o_fx, o_fy, o_fz;
+    size_t o_pack, o_pair, o_bin, o_scan, o_tiles, o_fx, o_fy, o_fz, o_fxy;
     place(pack, o_pack);
     place(pair, o_pair);
     place(bin, o_bin);
@@ -116,6 +138,7 @@ void EspBatch::build_plan(bool fp64)
     place(fx, o_fx);
     place(fy, o_fy);
     place(fz, o_fz);
+    place(fxy, o_fxy);
     m_items_dev.resize(std::max<size_t>(all.size(), 1) * sizeof(BatchItem));
     if (!all.empty())
         check_hip(hipMemcpy(m_items_dev.get(), all.data(), all.size() * sizeof(BatchItem), hipMemcpyHostToDevice), "hipMemcpy items");
@@ -154,6 +177,7 @@ void EspBatch::build_plan(bool fp64)
     m_l.fft_x = list(o_fx, fx.size(), m_max_grid);
     m_l.fft_y = list(o_fy, fy.size(), m_max_grid);
     m_l.fft_z = list(o_fz, fz.size(), m_max_grid);
+    m_l.fft_xy = list(o_fxy, fxy.size(), m_max_grid);
     }
 
 template<typename R> void EspBatch::enqueue_stages(hipStream_t st)
@@ -172,11 +196,13 @@ template<typename R> void EspBatch::enqueue_stages(hipStream_t st)
         check_hip(kernel::batch_scan(d, m_l.scan, st), "batch_scan");
         check_hip(kernel::batch_scatter<R>(d, m_l.scatter, st), "batch_scatter");
         check_hip(kernel::batch_spread<R>(d, m_l.tiles, m_lds_stencil, st), "batch_spread");
+        check_hip(kernel::batch_fft_xy_forward<R>(d, m_l.fft_xy, m_lds_fxy, st), "batch_fft_xy_forward");
         check_hip(kernel::batch_fft_x_r2c<R>(d, m_l.fft_x, m_lds_fx, st), "batch_fft_x_r2c");
         check_hip(kernel::batch_fft_y<R>(d, m_l.fft_y, -1, m_lds_fy, st), "batch_fft_y");
         check_hip(kernel::batch_fft_z_conv<R>(d, m_l.fft_z, m_lds_fz, st), "batch_fft_z_conv");
         check_hip(kernel::batch_fft_y<R>(d, m_l.fft_y, +1, m_lds_fy, st), "batch_fft_y inverse");
         check_hip(kernel::batch_fft_x_c2r<R>(d, m_l.fft_x, m_lds_fx, st), "batch_fft_x_c2r");
+        check_hip(kernel::batch_fft_xy_inverse<R>(d, m_l.fft_xy, m_lds_fxy, st), "batch_fft_xy_inverse");
         }
     check_hip(hipStreamWaitEvent(st, m_join, 0), "hipStreamWaitEvent");
     if (m_any_mesh)

@@ -39,7 +39,7 @@ class EspBatch
     private:
     struct Lists
         {
-        kernel::BatchList pack, pair, bin, scan, scatter, tiles, fft_x, fft_y, fft_z, fin;
+        kernel::BatchList pack, pair, bin, scan, scatter, tiles, fft_x, fft_y, fft_z, fft_xy, fin;
         };
     void build_plan(bool fp64);
     void enqueue(hipStream_t s);
@@ -58,7 +58,7 @@ class EspBatch
     DeviceBuffer m_desc_dev;
     DeviceBuffer m_items_dev;
     Lists m_l {};
-    size_t m_lds_stencil = 0, m_lds_fx = 0, m_lds_fy = 0, m_lds_fz = 0, m_lds_pair = 0;
+    size_t m_lds_stencil = 0, m_lds_fx = 0, m_lds_fy = 0, m_lds_fz = 0, m_lds_fxy = 0, m_lds_pair = 0;
     bool m_any_mesh = false;
     bool m_any_tilted = false;  // some system of the plan has a tilted box
 

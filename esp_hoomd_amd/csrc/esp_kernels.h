@@ -184,6 +184,12 @@ template<typename R> hipError_t batch_fft_x_r2c(const BatchSys* d, BatchList l, 
 template<typename R> hipError_t batch_fft_y(const BatchSys* d, BatchList l, int sign, size_t lds, hipStream_t s);
 template<typename R> hipError_t batch_fft_z_conv(const BatchSys* d, BatchList l, size_t lds, hipStream_t s);
 template<typename R> hipError_t batch_fft_x_c2r(const BatchSys* d, BatchList l, size_t lds, hipStream_t s);
+//! Fused x + y passes, one workgroup per xy plane (items (system, z)), for meshes with even ny
+//! whose plane fits the workgroup's LDS (fft_xy_fused); bit-identical to the separate passes.
+template<typename R> hipError_t batch_fft_xy_forward(const BatchSys* d, BatchList l, size_t lds, hipStream_t s);
+template<typename R> hipError_t batch_fft_xy_inverse(const BatchSys* d, BatchList l, size_t lds, hipStream_t s);
+size_t fft_xy_lds(const MeshGeom& g, bool fp64);
+bool fft_xy_fused(const MeshGeom& g, bool fp64);
 hipError_t batch_finalize(const BatchSys* d, BatchList l, hipStream_t s);
 template<typename R> hipError_t batch_pair(const BatchSys* d, BatchList l, size_t lds, bool any_tilted, hipStream_t s);
 template<typename R> hipError_t batch_pack(const BatchSys* d, BatchList l, hipStream_t s);

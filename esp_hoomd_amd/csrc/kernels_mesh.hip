@@ -1035,6 +1035,174 @@ __global__ void __launch_bounds__(256) kb_fft_x_c2r(const BatchSys* __restrict__
         }
     }
 
+//! Whole xy planes of a small mesh in one workgroup: the x pass (R2C of line pairs, or C2R) and
+//! the y pass run back to back in LDS, so the half spectrum makes one round trip to HBM per
+//! direction instead of two.  Each 1-D transform is the one the separate passes compute (same
+//! line pairs (y, y + 1) of the plane -- ny even, so no pair crosses a plane -- the same
+//! Stockham stages and twiddles), so the results are bit-identical to fft_x_r2c_block followed
+//! by fft_y_block (and fft_y_block(+1) followed by fft_x_c2r_block).
+//! LDS: the two twiddle tables and two buffers of max(ny/2 (nx + 1), nxh (ny + 1)) points.
+template<typename R>
+__device__ __forceinline__ void fft_xy_forward_plane(int z,
+                                                     const MeshGeom& g,
+                                                     const FFTPlan& px,
+                                                     const FFTPlan& py,
+                                                     const C2<R>* __restrict__ twx_g,
+                                                     const C2<R>* __restrict__ twy_g,
+                                                     typename Acc<R>::glob_t* __restrict__ mesh,
+                                                     C2<R>* __restrict__ spec,
+                                                     const float* __restrict__ fx)
+    {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int nx = g.nx, ny = g.ny, nxh = g.nxh;
+    const int np = ny / 2;
+    const int lsx = nx + 1, lsy = ny + 1;
+    const int buf = max(np * lsx, nxh * lsy);
+    const double inv_scale = double(fx[1]);
+    const FastDiv by_nx(nx), by_nxh(nxh);
+    C2<R>* twx = reinterpret_cast<C2<R>*>(smem);
+    C2<R>* twy = twx + nx;
+    C2<R>* a = twy + ny;
+    C2<R>* b = a + buf;
+    for (int i = threadIdx.x; i < nx; i += blockDim.x)
+        twx[i] = twx_g[i];
+    for (int i = threadIdx.x; i < ny; i += blockDim.x)
+        twy[i] = twy_g[i];
+    typename Acc<R>::glob_t* plane = mesh + size_t(z) * ny * nx;
+    staged_copy<8, C2<R>>(
+        np * nx, threadIdx.x, blockDim.x,
+        [&](int idx)
+        {
+            const int pr = by_nx.quot(idx);
+            const int x = idx - pr * nx;
+            const R v0 = mesh_load_clear<R>(&plane[size_t(2 * pr) * nx + x], inv_scale);
+            const R v1 = mesh_load_clear<R>(&plane[size_t(2 * pr + 1) * nx + x], inv_scale);
+            return C2<R> {v0, v1};
+        },
+        [&](int idx, const C2<R>& v)
+        {
+            const int pr = by_nx.quot(idx);
+            a[pr * lsx + idx - pr * nx] = v;
+        });
+    __syncthreads();
+    C2<R>* res = lds_fft<R>(a, b, twx, px, lsx, np, -1, threadIdx.x, blockDim.x);
+    // unpack the pairs into y lines (one per wavenumber k) of the other buffer
+    C2<R>* col = (res == a) ? b : a;
+    for (int idx = threadIdx.x; idx < np * nxh; idx += blockDim.x)
+        {
+        const int pr = by_nxh.quot(idx);
+        const int k = idx - pr * nxh;
+        const C2<R> zk = res[pr * lsx + k];
+        const C2<R> zc = res[pr * lsx + (k == 0 ? 0 : nx - k)];
+        col[k * lsy + 2 * pr] = C2<R> {R(0.5) * (zk.x + zc.x), R(0.5) * (zk.y - zc.y)};
+        col[k * lsy + 2 * pr + 1] = C2<R> {R(0.5) * (zk.y + zc.y), R(-0.5) * (zk.x - zc.x)};
+        }
+    __syncthreads();
+    C2<R>* other = (col == a) ? b : a;
+    C2<R>* out = lds_fft<R>(col, other, twy, py, lsy, nxh, -1, threadIdx.x, blockDim.x);
+    C2<R>* sp = spec + size_t(z) * ny * g.sxp;
+    for (int idx = threadIdx.x; idx < ny * nxh; idx += blockDim.x)
+        {
+        const int y = by_nxh.quot(idx);
+        const int k = idx - y * nxh;
+        sp[size_t(y) * g.sxp + k] = out[k * lsy + y];
+        }
+    }
+
+template<typename R>
+__device__ __forceinline__ void fft_xy_inverse_plane(int z,
+                                                     const MeshGeom& g,
+                                                     const FFTPlan& px,
+                                                     const FFTPlan& py,
+                                                     const C2<R>* __restrict__ twx_g,
+                                                     const C2<R>* __restrict__ twy_g,
+                                                     const C2<R>* __restrict__ spec,
+                                                     R scale,
+                                                     R* __restrict__ phi)
+    {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int nx = g.nx, ny = g.ny, nxh = g.nxh;
+    const int np = ny / 2;
+    const int lsx = nx + 1, lsy = ny + 1;
+    const int buf = max(np * lsx, nxh * lsy);
+    const FastDiv by_nx(nx), by_nxh(nxh);
+    C2<R>* twx = reinterpret_cast<C2<R>*>(smem);
+    C2<R>* twy = twx + nx;
+    C2<R>* a = twy + ny;
+    C2<R>* b = a + buf;
+    for (int i = threadIdx.x; i < nx; i += blockDim.x)
+        twx[i] = twx_g[i];
+    for (int i = threadIdx.x; i < ny; i += blockDim.x)
+        twy[i] = twy_g[i];
+    const C2<R>* sp = spec + size_t(z) * ny * g.sxp;
+    staged_copy<8, C2<R>>(
+        ny * nxh, threadIdx.x, blockDim.x,
+        [&](int idx)
+        {
+            const int y = by_nxh.quot(idx);
+            return sp[size_t(y) * g.sxp + idx - y * nxh];
+        },
+        [&](int idx, const C2<R>& v)
+        {
+            const int y = by_nxh.quot(idx);
+            a[(idx - y * nxh) * lsy + y] = v;
+        });
+    __syncthreads();
+    C2<R>* res = lds_fft<R>(a, b, twy, py, lsy, nxh, +1, threadIdx.x, blockDim.x);
+    // Z[k] = X0[k] + i X1[k]; Z[n-k] = conj(X0[k]) + i conj(X1[k]) for the line pair (2 pr, 2 pr + 1)
+    C2<R>* lines = (res == a) ? b : a;
+    for (int idx = threadIdx.x; idx < np * nxh; idx += blockDim.x)
+        {
+        const int pr = by_nxh.quot(idx);
+        const int k = idx - pr * nxh;
+        const C2<R> x0 = res[k * lsy + 2 * pr];
+        const C2<R> x1 = res[k * lsy + 2 * pr + 1];
+        lines[pr * lsx + k] = C2<R> {x0.x - x1.y, x0.y + x1.x};
+        if (k != 0 && k != nx - k)
+            lines[pr * lsx + nx - k] = C2<R> {x0.x + x1.y, -x0.y + x1.x};
+        }
+    __syncthreads();
+    C2<R>* other = (lines == a) ? b : a;
+    C2<R>* out = lds_fft<R>(lines, other, twx, px, lsx, np, +1, threadIdx.x, blockDim.x);
+    R* ph = phi + size_t(z) * ny * nx;
+    for (int idx = threadIdx.x; idx < np * nx; idx += blockDim.x)
+        {
+        const int pr = by_nx.quot(idx);
+        const int x = idx - pr * nx;
+        const C2<R> v = out[pr * lsx + x];
+        ph[size_t(2 * pr) * nx + x] = v.x * scale;
+        ph[size_t(2 * pr + 1) * nx + x] = v.y * scale;
+        }
+    }
+
+template<typename R>
+__global__ void __launch_bounds__(256) kb_fft_xy_forward(const BatchSys* __restrict__ ds, BatchList l)
+    {
+    for (unsigned int it = blockIdx.x; it < l.n; it += gridDim.x)
+        {
+        const BatchItem w = l.items[it];
+        const BatchSys& d = ds[w.sys];
+        fft_xy_forward_plane<R>(int(w.local), d.g, d.plan[0], d.plan[1], reinterpret_cast<const C2<R>*>(d.tw[0]),
+                                reinterpret_cast<const C2<R>*>(d.tw[1]), reinterpret_cast<typename Acc<R>::glob_t*>(d.mesh),
+                                reinterpret_cast<C2<R>*>(d.spec), d.fx);
+        __syncthreads();
+        }
+    }
+
+template<typename R>
+__global__ void __launch_bounds__(256) kb_fft_xy_inverse(const BatchSys* __restrict__ ds, BatchList l)
+    {
+    for (unsigned int it = blockIdx.x; it < l.n; it += gridDim.x)
+        {
+        const BatchItem w = l.items[it];
+        const BatchSys& d = ds[w.sys];
+        fft_xy_inverse_plane<R>(int(w.local), d.g, d.plan[0], d.plan[1], reinterpret_cast<const C2<R>*>(d.tw[0]),
+                                reinterpret_cast<const C2<R>*>(d.tw[1]), reinterpret_cast<const C2<R>*>(d.spec), R(1.0 / d.V),
+                                reinterpret_cast<R*>(d.phi));
+        __syncthreads();
+        }
+    }
+
 __global__ void __launch_bounds__(256) kb_finalize(const BatchSys* __restrict__ ds, BatchList l)
     {
     for (unsigned int it = blockIdx.x; it < l.n; it += gridDim.x)
@@ -1260,6 +1428,33 @@ template<typename R> hipError_t batch_fft_z_conv(const BatchSys* d, BatchList l,
     return hipGetLastError();
     }
 
+size_t fft_xy_lds(const MeshGeom& g, bool fp64)
+    {
+    const size_t buf = std::max(size_t(g.ny / 2) * size_t(g.nx + 1), size_t(g.nxh) * size_t(g.ny + 1));
+    return (size_t(g.nx) + size_t(g.ny) + 2 * buf) * (fp64 ? 16 : 8);
+    }
+
+bool fft_xy_fused(const MeshGeom& g, bool fp64)
+    {
+    return g.ny % 2 == 0 && fft_xy_lds(g, fp64) <= 64 * 1024;
+    }
+
+template<typename R> hipError_t batch_fft_xy_forward(const BatchSys* d, BatchList l, size_t lds, hipStream_t s)
+    {
+    if (l.n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(kb_fft_xy_forward<R>, dim3(l.grid), dim3(256), lds, s, d, l);
+    return hipGetLastError();
+    }
+
+template<typename R> hipError_t batch_fft_xy_inverse(const BatchSys* d, BatchList l, size_t lds, hipStream_t s)
+    {
+    if (l.n == 0)
+        return hipSuccess;
+    hipLaunchKernelGGL(kb_fft_xy_inverse<R>, dim3(l.grid), dim3(256), lds, s, d, l);
+    return hipGetLastError();
+    }
+
 template<typename R> hipError_t batch_fft_x_c2r(const BatchSys* d, BatchList l, size_t lds, hipStream_t s)
     {
     if (l.n == 0)
@@ -1281,7 +1476,9 @@ hipError_t batch_finalize(const BatchSys* d, BatchList l, hipStream_t s)
     template hipError_t batch_fft_x_r2c<R>(const BatchSys*, BatchList, size_t, hipStream_t);  \
     template hipError_t batch_fft_y<R>(const BatchSys*, BatchList, int, size_t, hipStream_t); \
     template hipError_t batch_fft_z_conv<R>(const BatchSys*, BatchList, size_t, hipStream_t); \
-    template hipError_t batch_fft_x_c2r<R>(const BatchSys*, BatchList, size_t, hipStream_t);
+    template hipError_t batch_fft_x_c2r<R>(const BatchSys*, BatchList, size_t, hipStream_t);  \
+    template hipError_t batch_fft_xy_forward<R>(const BatchSys*, BatchList, size_t, hipStream_t); \
+    template hipError_t batch_fft_xy_inverse<R>(const BatchSys*, BatchList, size_t, hipStream_t);
 ESP_INST_BATCH(double)
 ESP_INST_BATCH(float)
 #undef ESP_INST_BATCH
