@@ -104,7 +104,7 @@ void EspBatch::build_plan(bool fp64)
             }
         else
             {
-            const unsigned int npairs = unsigned(g.ny * g.nz + 1) / 2;
+            const unsigned int npairs = unsigned((g.ny + 1) / 2) * unsigned(g.nz);
             for (unsigned int b = 0; b < (npairs + d.ppb_x - 1) / unsigned(d.ppb_x); ++b)
                 fx.push_back({s, b});
             const unsigned int nby = unsigned((g.nxh + d.ncol_y - 1) / d.ncol_y) * unsigned(g.nz);

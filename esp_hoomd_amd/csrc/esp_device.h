@@ -116,9 +116,26 @@ template<typename R> struct C2
     R x, y;
     };
 
+//! a b + c with one rounding.  The FFT arithmetic is written with these explicitly, so the
+//! compiler has no product-sum left to contract: a transform rounds the same way in every
+//! kernel it is inlined into, whatever the kernel knows of its sign or sizes (the batched,
+//! fused and single-system passes give bit-identical spectra).
+template<typename R> __device__ __forceinline__ R fmar(R a, R b, R c)
+    {
+    if constexpr (sizeof(R) == 4)
+        return __fmaf_rn(a, b, c);
+    else
+        return __fma_rn(a, b, c);
+    }
+
+//! a b and a conj(b)
 template<typename R> __device__ __forceinline__ C2<R> cmul(C2<R> a, C2<R> b)
     {
-    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+    return {fmar(a.x, b.x, -(a.y * b.y)), fmar(a.x, b.y, a.y * b.x)};
+    }
+template<typename R> __device__ __forceinline__ C2<R> cmul_conj(C2<R> a, C2<R> b)
+    {
+    return {fmar(a.x, b.x, a.y * b.y), fmar(a.y, b.x, -(a.x * b.y))};
     }
 template<typename R> __device__ __forceinline__ C2<R> cadd(C2<R> a, C2<R> b)
     {
@@ -161,7 +178,7 @@ template<typename R> __device__ __forceinline__ void bfly3(C2<R>* v, int sign)
     const R s1 = R(sign) * R(0.86602540378443864676);
     const C2<R> t = cadd(v[1], v[2]);
     const C2<R> d = csub(v[1], v[2]);
-    const C2<R> m = {v[0].x + c1 * t.x, v[0].y + c1 * t.y};
+    const C2<R> m = {fmar(c1, t.x, v[0].x), fmar(c1, t.y, v[0].y)};
     // i * s1 * d
     const C2<R> j = {-s1 * d.y, s1 * d.x};
     v[0] = cadd(v[0], t);
@@ -177,11 +194,11 @@ template<typename R> __device__ __forceinline__ void bfly5(C2<R>* v, int sign)
     const R s2 = R(sign) * R(0.58778525229247312917);
     const C2<R> t1 = cadd(v[1], v[4]), d1 = csub(v[1], v[4]);
     const C2<R> t2 = cadd(v[2], v[3]), d2 = csub(v[2], v[3]);
-    const C2<R> a1 = {v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y};
-    const C2<R> a2 = {v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y};
+    const C2<R> a1 = {fmar(c2, t2.x, fmar(c1, t1.x, v[0].x)), fmar(c2, t2.y, fmar(c1, t1.y, v[0].y))};
+    const C2<R> a2 = {fmar(c1, t2.x, fmar(c2, t1.x, v[0].x)), fmar(c1, t2.y, fmar(c2, t1.y, v[0].y))};
     // i * (s1 d1 + s2 d2), i * (s2 d1 - s1 d2)
-    const C2<R> b1 = {-(s1 * d1.y + s2 * d2.y), s1 * d1.x + s2 * d2.x};
-    const C2<R> b2 = {-(s2 * d1.y - s1 * d2.y), s2 * d1.x - s1 * d2.x};
+    const C2<R> b1 = {-fmar(s1, d1.y, s2 * d2.y), fmar(s1, d1.x, s2 * d2.x)};
+    const C2<R> b2 = {-fmar(s2, d1.y, -(s1 * d2.y)), fmar(s2, d1.x, -(s1 * d2.x))};
     v[0] = {v[0].x + t1.x + t2.x, v[0].y + t1.y + t2.y};
     v[1] = cadd(a1, b1);
     v[4] = csub(a1, b1);
@@ -192,7 +209,8 @@ template<typename R> __device__ __forceinline__ void bfly5(C2<R>* v, int sign)
 //! Quotient by a divisor fixed for a loop, as one v_mul_hi_u32 instead of the ~20 instructions
 //! of a general 32-bit division.  With m = ceil(2^32 / d) = (2^32 + e) / d, 0 <= e < d, and
 //! n = q d + r: n m / 2^32 = q + (r + n e / 2^32) / d, whose floor is q while n e < 2^32, which
-//! n d <= 2^32 guarantees.  The FFT passes divide indices below 2^13 by divisors below 2^12.
+//! n d <= 2^32 guarantees.  The FFT passes divide indices below 2^20 by divisors of at most
+//! 1024 (mesh edges are at most 1024).
 //! d = 1 (m would be 2^32) returns n.
 struct FastDiv
     {
@@ -237,10 +255,8 @@ __device__ __forceinline__ void stockham_stage(const C2<R>* __restrict__ in,
 #pragma unroll
         for (int r = 1; r < RAD; ++r)
             {
-            C2<R> w = tw[k * r * twstep];
-            if (sign > 0)
-                w.y = -w.y;
-            v[r] = cmul(v[r], w);
+            const C2<R> w = tw[k * r * twstep];
+            v[r] = sign > 0 ? cmul_conj(v[r], w) : cmul(v[r], w);
             }
         if constexpr (RAD == 2)
             bfly2(v);

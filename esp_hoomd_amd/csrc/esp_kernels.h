@@ -56,7 +56,7 @@ hipError_t gather(const MeshGeom& g, int P, int D, unsigned int n_tiles, const u
 
 // ---- FFT passes -------------------------------------------------------------
 int fft_x_pairs_per_block(int nx, bool fp64);
-int fft_cols_per_block(int n, bool fp64);
+int fft_cols_per_block(int n, int nxh, bool fp64);
 unsigned int fft_z_blocks(const MeshGeom& g, bool fp64);
 template<typename R>
 hipError_t fft_forward_x(const MeshGeom& g, const FFTPlan& px, const void* tw, void* mesh, void* spec, const float* fx, hipStream_t s);
@@ -184,8 +184,8 @@ template<typename R> hipError_t batch_fft_x_r2c(const BatchSys* d, BatchList l, 
 template<typename R> hipError_t batch_fft_y(const BatchSys* d, BatchList l, int sign, size_t lds, hipStream_t s);
 template<typename R> hipError_t batch_fft_z_conv(const BatchSys* d, BatchList l, size_t lds, hipStream_t s);
 template<typename R> hipError_t batch_fft_x_c2r(const BatchSys* d, BatchList l, size_t lds, hipStream_t s);
-//! Fused x + y passes, one workgroup per xy plane (items (system, z)), for meshes with even ny
-//! whose plane fits the workgroup's LDS (fft_xy_fused); bit-identical to the separate passes.
+//! Fused x + y passes, one workgroup per xy plane (items (system, z)), for meshes whose plane
+//! fits the workgroup's LDS (fft_xy_fused); bit-identical to the separate passes.
 template<typename R> hipError_t batch_fft_xy_forward(const BatchSys* d, BatchList l, size_t lds, hipStream_t s);
 template<typename R> hipError_t batch_fft_xy_inverse(const BatchSys* d, BatchList l, size_t lds, hipStream_t s);
 size_t fft_xy_lds(const MeshGeom& g, bool fp64);

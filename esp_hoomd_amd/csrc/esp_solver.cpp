@@ -1020,8 +1020,8 @@ void EspSolver::describe(const SystemView& sys, const OutputView& out, hipStream
     d.flags = out.flags ? out.flags : COMPUTE_ALL;
     d.want_ev = (d.flags & COMPUTE_ENERGY_VIRIAL) ? 1 : 0;
     d.ppb_x = kernel::fft_x_pairs_per_block(g.nx, fp64);
-    d.ncol_y = kernel::fft_cols_per_block(g.ny, fp64);
-    d.ncol_z = kernel::fft_cols_per_block(g.nz, fp64);
+    d.ncol_y = kernel::fft_cols_per_block(g.ny, g.nxh, fp64);
+    d.ncol_z = kernel::fft_cols_per_block(g.nz, g.nxh, fp64);
     d.n_qblk = (n + kernel::BATCH_CHUNK - 1) / kernel::BATCH_CHUNK;
     d.n_zblk = kernel::fft_z_blocks(g, fp64);
     d.V = g.Lx * g.Ly * g.Lz;

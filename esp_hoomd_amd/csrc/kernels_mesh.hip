@@ -329,6 +329,29 @@ __global__ void __launch_bounds__(256) k_scatter(unsigned int n,
 // half spectrum along x (nxh = nx/2 + 1 columns).
 // ---------------------------------------------------------------------------
 
+//! Line pairs of the x passes stay inside one xy plane: pair p is plane z = p / pp,
+//! pp = ceil(ny / 2), lines y = 2 (p - z pp) and y + 1 -- the second absent (l1 = -1) when ny
+//! is odd and y + 1 = ny; a pair past the last plane has l0 = l1 = -1.  The fused xy-plane
+//! passes pair the same lines, so both forms compute the same transforms.
+struct XPairs
+    {
+    int ny, pp, npairs;
+    FastDiv by_pp;
+    __device__ __forceinline__ explicit XPairs(const MeshGeom& g) : ny(g.ny), pp((g.ny + 1) / 2), npairs(((g.ny + 1) / 2) * g.nz), by_pp((g.ny + 1) / 2) {}
+    __device__ __forceinline__ void lines(int p, int& l0, int& l1) const
+        {
+        if (p >= npairs)
+            {
+            l0 = l1 = -1;
+            return;
+            }
+        const int z = by_pp.quot(p);
+        const int y = 2 * (p - z * pp);
+        l0 = z * ny + y;
+        l1 = (y + 1 < ny) ? l0 + 1 : -1;
+        }
+    };
+
 //! R2C along x for pairs of real lines packed into one complex FFT; reads and clears the mesh.
 template<typename R>
 __device__ __forceinline__ void fft_x_r2c_block(int blk,
@@ -348,7 +371,7 @@ __device__ __forceinline__ void fft_x_r2c_block(int blk,
     C2<R>* tw = reinterpret_cast<C2<R>*>(smem);
     C2<R>* a = tw + nx;
     C2<R>* b = a + pairs_per_block * ls;
-    const int nlines = g.ny * g.nz;
+    const XPairs xp(g);
     const int pair0 = blk * pairs_per_block;
     for (int i = threadIdx.x; i < nx; i += blockDim.x)
         tw[i] = tw_g[i];
@@ -358,12 +381,12 @@ __device__ __forceinline__ void fft_x_r2c_block(int blk,
         {
             const int pr = by_nx.quot(idx);
             const int x = idx - pr * nx;
-            const int l0 = 2 * (pair0 + pr);
-            const int l1 = l0 + 1;
+            int l0, l1;
+            xp.lines(pair0 + pr, l0, l1);
             R v0 = R(0), v1 = R(0);
-            if (l0 < nlines)
+            if (l0 >= 0)
                 v0 = mesh_load_clear<R>(&mesh[size_t(l0) * nx + x], inv_scale);
-            if (l1 < nlines)
+            if (l1 >= 0)
                 v1 = mesh_load_clear<R>(&mesh[size_t(l1) * nx + x], inv_scale);
             return C2<R> {v0, v1};
         },
@@ -379,16 +402,16 @@ __device__ __forceinline__ void fft_x_r2c_block(int blk,
         {
         const int pr = by_nxh.quot(idx);
         const int k = idx - pr * nxh;
-        const int l0 = 2 * (pair0 + pr);
-        const int l1 = l0 + 1;
+        int l0, l1;
+        xp.lines(pair0 + pr, l0, l1);
         const C2<R> z = res[pr * ls + k];
         const C2<R> zc = res[pr * ls + (k == 0 ? 0 : nx - k)];
         // X0 = (Z[k] + conj(Z[-k])) / 2, X1 = (Z[k] - conj(Z[-k])) / (2 i)
         const C2<R> x0 = {R(0.5) * (z.x + zc.x), R(0.5) * (z.y - zc.y)};
         const C2<R> x1 = {R(0.5) * (z.y + zc.y), R(-0.5) * (z.x - zc.x)};
-        if (l0 < nlines)
+        if (l0 >= 0)
             spec[size_t(l0) * g.sxp + k] = x0;
-        if (l1 < nlines)
+        if (l1 >= 0)
             spec[size_t(l1) * g.sxp + k] = x1;
         }
     }
@@ -424,7 +447,7 @@ __device__ __forceinline__ void fft_x_c2r_block(int blk,
     C2<R>* tw = reinterpret_cast<C2<R>*>(smem);
     C2<R>* a = tw + nx;
     C2<R>* b = a + pairs_per_block * ls;
-    const int nlines = g.ny * g.nz;
+    const XPairs xp(g);
     const int pair0 = blk * pairs_per_block;
     for (int i = threadIdx.x; i < nx; i += blockDim.x)
         tw[i] = tw_g[i];
@@ -438,12 +461,12 @@ __device__ __forceinline__ void fft_x_c2r_block(int blk,
         {
             const int pr = by_nxh.quot(idx);
             const int k = idx - pr * nxh;
-            const int l0 = 2 * (pair0 + pr);
-            const int l1 = l0 + 1;
+            int l0, l1;
+            xp.lines(pair0 + pr, l0, l1);
             Pair2 v = {{R(0), R(0)}, {R(0), R(0)}};
-            if (l0 < nlines)
+            if (l0 >= 0)
                 v.x0 = spec[size_t(l0) * g.sxp + k];
-            if (l1 < nlines)
+            if (l1 >= 0)
                 v.x1 = spec[size_t(l1) * g.sxp + k];
             return v;
         },
@@ -462,12 +485,12 @@ __device__ __forceinline__ void fft_x_c2r_block(int blk,
         {
         const int pr = by_nx.quot(idx);
         const int x = idx - pr * nx;
-        const int l0 = 2 * (pair0 + pr);
-        const int l1 = l0 + 1;
+        int l0, l1;
+        xp.lines(pair0 + pr, l0, l1);
         const C2<R> z = res[pr * ls + x];
-        if (l0 < nlines)
+        if (l0 >= 0)
             phi[size_t(l0) * nx + x] = z.x * scale;
-        if (l1 < nlines)
+        if (l1 >= 0)
             phi[size_t(l1) * nx + x] = z.y * scale;
         }
     }
@@ -1037,11 +1060,11 @@ __global__ void __launch_bounds__(256) kb_fft_x_c2r(const BatchSys* __restrict__
 
 //! Whole xy planes of a small mesh in one workgroup: the x pass (R2C of line pairs, or C2R) and
 //! the y pass run back to back in LDS, so the half spectrum makes one round trip to HBM per
-//! direction instead of two.  Each 1-D transform is the one the separate passes compute (same
-//! line pairs (y, y + 1) of the plane -- ny even, so no pair crosses a plane -- the same
-//! Stockham stages and twiddles), so the results are bit-identical to fft_x_r2c_block followed
-//! by fft_y_block (and fft_y_block(+1) followed by fft_x_c2r_block).
-//! LDS: the two twiddle tables and two buffers of max(ny/2 (nx + 1), nxh (ny + 1)) points.
+//! direction instead of two.  Each 1-D transform is the one the separate passes compute (the
+//! same line pairs (XPairs), Stockham stages and twiddles), so the results are bit-identical
+//! to fft_x_r2c_block followed by fft_y_block (and fft_y_block(+1) followed by
+//! fft_x_c2r_block).  LDS: the two twiddle tables and two buffers of
+//! max(ceil(ny/2) (nx + 1), nxh (ny + 1)) points.
 template<typename R>
 __device__ __forceinline__ void fft_xy_forward_plane(int z,
                                                      const MeshGeom& g,
@@ -1055,7 +1078,7 @@ __device__ __forceinline__ void fft_xy_forward_plane(int z,
     {
     extern __shared__ __align__(16) unsigned char smem[];
     const int nx = g.nx, ny = g.ny, nxh = g.nxh;
-    const int np = ny / 2;
+    const int np = (ny + 1) / 2;
     const int lsx = nx + 1, lsy = ny + 1;
     const int buf = max(np * lsx, nxh * lsy);
     const double inv_scale = double(fx[1]);
@@ -1076,7 +1099,7 @@ __device__ __forceinline__ void fft_xy_forward_plane(int z,
             const int pr = by_nx.quot(idx);
             const int x = idx - pr * nx;
             const R v0 = mesh_load_clear<R>(&plane[size_t(2 * pr) * nx + x], inv_scale);
-            const R v1 = mesh_load_clear<R>(&plane[size_t(2 * pr + 1) * nx + x], inv_scale);
+            const R v1 = (2 * pr + 1 < ny) ? mesh_load_clear<R>(&plane[size_t(2 * pr + 1) * nx + x], inv_scale) : R(0);
             return C2<R> {v0, v1};
         },
         [&](int idx, const C2<R>& v)
@@ -1095,7 +1118,8 @@ __device__ __forceinline__ void fft_xy_forward_plane(int z,
         const C2<R> zk = res[pr * lsx + k];
         const C2<R> zc = res[pr * lsx + (k == 0 ? 0 : nx - k)];
         col[k * lsy + 2 * pr] = C2<R> {R(0.5) * (zk.x + zc.x), R(0.5) * (zk.y - zc.y)};
-        col[k * lsy + 2 * pr + 1] = C2<R> {R(0.5) * (zk.y + zc.y), R(-0.5) * (zk.x - zc.x)};
+        if (2 * pr + 1 < ny)
+            col[k * lsy + 2 * pr + 1] = C2<R> {R(0.5) * (zk.y + zc.y), R(-0.5) * (zk.x - zc.x)};
         }
     __syncthreads();
     C2<R>* other = (col == a) ? b : a;
@@ -1122,7 +1146,7 @@ __device__ __forceinline__ void fft_xy_inverse_plane(int z,
     {
     extern __shared__ __align__(16) unsigned char smem[];
     const int nx = g.nx, ny = g.ny, nxh = g.nxh;
-    const int np = ny / 2;
+    const int np = (ny + 1) / 2;
     const int lsx = nx + 1, lsy = ny + 1;
     const int buf = max(np * lsx, nxh * lsy);
     const FastDiv by_nx(nx), by_nxh(nxh);
@@ -1156,7 +1180,7 @@ __device__ __forceinline__ void fft_xy_inverse_plane(int z,
         const int pr = by_nxh.quot(idx);
         const int k = idx - pr * nxh;
         const C2<R> x0 = res[k * lsy + 2 * pr];
-        const C2<R> x1 = res[k * lsy + 2 * pr + 1];
+        const C2<R> x1 = (2 * pr + 1 < ny) ? res[k * lsy + 2 * pr + 1] : C2<R> {R(0), R(0)};
         lines[pr * lsx + k] = C2<R> {x0.x - x1.y, x0.y + x1.x};
         if (k != 0 && k != nx - k)
             lines[pr * lsx + nx - k] = C2<R> {x0.x + x1.y, -x0.y + x1.x};
@@ -1171,7 +1195,8 @@ __device__ __forceinline__ void fft_xy_inverse_plane(int z,
         const int x = idx - pr * nx;
         const C2<R> v = out[pr * lsx + x];
         ph[size_t(2 * pr) * nx + x] = v.x * scale;
-        ph[size_t(2 * pr + 1) * nx + x] = v.y * scale;
+        if (2 * pr + 1 < ny)
+            ph[size_t(2 * pr + 1) * nx + x] = v.y * scale;
         }
     }
 
@@ -1287,20 +1312,25 @@ int fft_x_pairs_per_block(int nx, bool fp64)
     return pairs;
     }
 
-int fft_cols_per_block(int n, bool fp64)
+//! Columns per workgroup of the y and z passes (lines of length n over the nxh columns of the
+//! half spectrum): at most 16 within 64 KiB of LDS, then evened out over the blocks a row of
+//! nxh columns needs (nxh = 33 takes three blocks of 11, not 16 + 16 + 1), so no block runs its
+//! copy loops over mostly absent columns.
+int fft_cols_per_block(int n, int nxh, bool fp64)
     {
     const size_t el = fp64 ? 16 : 8;
     int cols = 16;
     while (cols > 1 && (size_t(n) + 2 * size_t(cols) * (n + 1)) * el > 64 * 1024)
         cols >>= 1;
-    return cols;
+    const int blocks = (nxh + cols - 1) / cols;
+    return (nxh + blocks - 1) / blocks;
     }
 
 template<typename R>
 hipError_t fft_forward_x(const MeshGeom& g, const FFTPlan& px, const void* tw, void* mesh, void* spec, const float* fx, hipStream_t s)
     {
     const int ppb = fft_x_pairs_per_block(g.nx, sizeof(R) == 8);
-    const int npairs = (g.ny * g.nz + 1) / 2;
+    const int npairs = ((g.ny + 1) / 2) * g.nz;
     const int nb = (npairs + ppb - 1) / ppb;
     const size_t lds = (size_t(g.nx) + 2 * size_t(ppb) * (g.nx + 1)) * sizeof(C2<R>);
     hipLaunchKernelGGL(k_fft_x_r2c<R>, dim3(nb), dim3(256), lds, s, g, px, ppb, reinterpret_cast<const C2<R>*>(tw),
@@ -1312,7 +1342,7 @@ template<typename R>
 hipError_t fft_inverse_x(const MeshGeom& g, const FFTPlan& px, const void* tw, const void* spec, double scale, void* phi, hipStream_t s)
     {
     const int ppb = fft_x_pairs_per_block(g.nx, sizeof(R) == 8);
-    const int npairs = (g.ny * g.nz + 1) / 2;
+    const int npairs = ((g.ny + 1) / 2) * g.nz;
     const int nb = (npairs + ppb - 1) / ppb;
     const size_t lds = (size_t(g.nx) + 2 * size_t(ppb) * (g.nx + 1)) * sizeof(C2<R>);
     hipLaunchKernelGGL(k_fft_x_c2r<R>, dim3(nb), dim3(256), lds, s, g, px, ppb, reinterpret_cast<const C2<R>*>(tw),
@@ -1323,7 +1353,7 @@ hipError_t fft_inverse_x(const MeshGeom& g, const FFTPlan& px, const void* tw, c
 template<typename R>
 hipError_t fft_y(const MeshGeom& g, const FFTPlan& py, const void* tw, void* spec, int sign, hipStream_t s)
     {
-    const int ncol = fft_cols_per_block(g.ny, sizeof(R) == 8);
+    const int ncol = fft_cols_per_block(g.ny, g.nxh, sizeof(R) == 8);
     const int nbx = (g.nxh + ncol - 1) / ncol;
     const size_t lds = (size_t(g.ny) + 2 * size_t(ncol) * (g.ny + 1)) * sizeof(C2<R>);
     hipLaunchKernelGGL(k_fft_y<R>, dim3(nbx * g.nz), dim3(256), lds, s, g, py, ncol, reinterpret_cast<const C2<R>*>(tw),
@@ -1333,7 +1363,7 @@ hipError_t fft_y(const MeshGeom& g, const FFTPlan& py, const void* tw, void* spe
 
 unsigned int fft_z_blocks(const MeshGeom& g, bool fp64)
     {
-    const int ncol = fft_cols_per_block(g.nz, fp64);
+    const int ncol = fft_cols_per_block(g.nz, g.nxh, fp64);
     const int nbx = (g.nxh + ncol - 1) / ncol;
     return unsigned(nbx * g.ny);
     }
@@ -1342,7 +1372,7 @@ template<typename R>
 hipError_t fft_z_convolve(const MeshGeom& g, const FFTPlan& pz, const void* tw, void* spec, const double* G, const double* Gv,
                           int want_ev, double* ev_part, hipStream_t s)
     {
-    const int ncol = fft_cols_per_block(g.nz, sizeof(R) == 8);
+    const int ncol = fft_cols_per_block(g.nz, g.nxh, sizeof(R) == 8);
     const int nbx = (g.nxh + ncol - 1) / ncol;
     const size_t lds = (size_t(g.nz) + 2 * size_t(ncol) * (g.nz + 1)) * sizeof(C2<R>);
     hipLaunchKernelGGL(k_fft_z_conv<R>, dim3(nbx * g.ny), dim3(256), lds, s, g, pz, ncol, reinterpret_cast<const C2<R>*>(tw),
@@ -1430,13 +1460,13 @@ template<typename R> hipError_t batch_fft_z_conv(const BatchSys* d, BatchList l,
 
 size_t fft_xy_lds(const MeshGeom& g, bool fp64)
     {
-    const size_t buf = std::max(size_t(g.ny / 2) * size_t(g.nx + 1), size_t(g.nxh) * size_t(g.ny + 1));
+    const size_t buf = std::max(size_t((g.ny + 1) / 2) * size_t(g.nx + 1), size_t(g.nxh) * size_t(g.ny + 1));
     return (size_t(g.nx) + size_t(g.ny) + 2 * buf) * (fp64 ? 16 : 8);
     }
 
 bool fft_xy_fused(const MeshGeom& g, bool fp64)
     {
-    return g.ny % 2 == 0 && fft_xy_lds(g, fp64) <= 64 * 1024;
+    return fft_xy_lds(g, fp64) <= 64 * 1024;
     }
 
 template<typename R> hipError_t batch_fft_xy_forward(const BatchSys* d, BatchList l, size_t lds, hipStream_t s)

@@ -25,10 +25,12 @@ CASES = [
     (500, 12.0, 32, 12, 14.0, 3.0),
     (333, 11.0, 36, 10, 12.0, 3.2),
     (1, 5.0, 16, 4, 6.0, 2.0),
-    # odd ny: this member takes the separate x and y passes while the others have their xy
-    # planes fused (one workgroup per plane); a 64^3 FP64 plane does not fit the LDS either
+    # xy planes are transformed in one workgroup when they fit its LDS: odd ny (a half-empty
+    # line pair per plane), and two members that take the separate x and y passes instead --
+    # a 64^3 FP64 plane, and a 120 x 75 plane in either precision (odd ny there too)
     (150, 9.0, (20, 25, 18), 8, 10.0, 3.0),
     (40, 7.0, 64, 4, 6.0, 2.5),
+    (60, 8.0, (120, 75, 16), 6, 10.0, 2.5),
 ]
 
 
