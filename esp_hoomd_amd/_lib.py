@@ -130,6 +130,7 @@ def load() -> C.CDLL:
     _sig(lib, "esp_last_error", C.c_char_p, [_VP])
     _sig(lib, "esp_compute_forces", C.c_int, [_VP, C.c_uint64, C.POINTER(EspSystem), C.POINTER(EspOut), _VP])
     _sig(lib, "esp_pair_forces", C.c_int, [_VP, C.POINTER(EspSystem), C.POINTER(EspOut), _VP])
+    _sig(lib, "esp_prepare", C.c_int, [_VP, C.POINTER(EspSystem), C.POINTER(EspOut), _VP])
     _sig(lib, "esp_pair_forces_noalloc", C.c_int, [_VP, C.POINTER(EspSystem), C.POINTER(EspOut), _VP])
     _sig(lib, "esp_batch_create", C.c_int, [C.POINTER(_VP)])
     _sig(lib, "esp_batch_destroy", C.c_int, [_VP])

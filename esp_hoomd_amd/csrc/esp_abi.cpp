@@ -247,6 +247,18 @@ int esp_compute_forces(esp_ctx* ctx, uint64_t timestep, const esp_system* sys, e
     return guarded(ctx, [&] { ctx->solver.compute(to_view(sys), to_out(out), static_cast<hipStream_t>(stream)); });
     }
 
+int esp_prepare(esp_ctx* ctx, const esp_system* sys, const esp_out* out, esp_stream_t stream)
+    {
+    if (!ctx)
+        return ESP_ERR_ARG;
+    if (!sys || !out)
+        {
+        ctx->last_error = "esp_prepare: null system or output";
+        return ESP_ERR_ARG;
+        }
+    return guarded(ctx, [&] { ctx->solver.setup(to_view(sys), to_out(out), static_cast<hipStream_t>(stream)); });
+    }
+
 int esp_pair_forces(esp_ctx* ctx, const esp_system* sys, esp_out* out, esp_stream_t stream)
     {
     if (!ctx)

@@ -135,6 +135,13 @@ void DeviceBuffer::resize(size_t bytes, bool zero)
         check_hip(hipMemset(m_ptr, 0, m_bytes), "hipMemset");
     }
 
+void DeviceBuffer::resize_zeroed_async(size_t bytes, hipStream_t s)
+    {
+    resize(bytes, false);
+    if (m_ptr)
+        check_hip(hipMemsetAsync(m_ptr, 0, m_bytes, s), "hipMemsetAsync");
+    }
+
 FFTPlan make_fft_plan(int n)
     {
     FFTPlan p {};
@@ -169,6 +176,15 @@ EspSolver::EspSolver() = default;
 EspSolver::~EspSolver()
     {
     release_graph();
+    if (m_stage_ev)
+        {
+        (void)hipEventSynchronize(m_stage_ev);
+        (void)hipEventDestroy(m_stage_ev);
+        }
+    if (m_stage)
+        (void)hipHostFree(m_stage);
+    for (unsigned char* p : m_stage_retired)
+        (void)hipHostFree(p);
     if (m_gstream)
         (void)hipStreamDestroy(m_gstream);
     for (auto& v : m_ev_pool)
@@ -423,11 +439,51 @@ size_t EspSolver::memory_bytes() const
     return b;
     }
 
-template<typename R> static void upload_as(DeviceBuffer& buf, const std::vector<double>& v)
+void EspSolver::begin_uploads()
     {
-    std::vector<R> tmp(v.begin(), v.end());
-    buf.resize(tmp.size() * sizeof(R));
-    check_hip(hipMemcpy(buf.get(), tmp.data(), tmp.size() * sizeof(R), hipMemcpyHostToDevice), "hipMemcpy tables");
+    if (m_stage_ev && m_stage_recorded)
+        check_hip(hipEventSynchronize(m_stage_ev), "hipEventSynchronize");
+    for (unsigned char* p : m_stage_retired)
+        check_hip(hipHostFree(p), "hipHostFree");
+    m_stage_retired.clear();
+    m_stage_recorded = false;
+    m_stage_used = 0;
+    }
+
+void EspSolver::end_uploads(hipStream_t s)
+    {
+    if (!m_stage_ev)
+        check_hip(hipEventCreateWithFlags(&m_stage_ev, hipEventDisableTiming), "hipEventCreate");
+    check_hip(hipEventRecord(m_stage_ev, s), "hipEventRecord");
+    m_stage_recorded = true;
+    m_stage_stream = s;
+    }
+
+//! Convert to R and copy into `buf` through the staging arena (stream-ordered on s).
+template<typename R> void EspSolver::upload_as(DeviceBuffer& buf, const std::vector<double>& v, hipStream_t s)
+    {
+    const size_t bytes = v.size() * sizeof(R);
+    buf.resize(bytes);
+    if (bytes == 0)
+        return;
+    size_t off = (m_stage_used + 255) & ~size_t(255);
+    if (off + bytes > m_stage_cap)
+        {
+        // this set-up's earlier copies may still read the old block: it is freed at the next
+        // set-up, after they are done
+        if (m_stage)
+            m_stage_retired.push_back(m_stage);
+        m_stage = nullptr;
+        const size_t cap = std::max<size_t>(2 * bytes, size_t(2) << 20);
+        check_hip(hipHostMalloc(reinterpret_cast<void**>(&m_stage), cap, hipHostMallocDefault), "hipHostMalloc");
+        m_stage_cap = cap;
+        off = 0;
+        }
+    R* dst = reinterpret_cast<R*>(m_stage + off);
+    for (size_t i = 0; i < v.size(); ++i)
+        dst[i] = R(v[i]);
+    m_stage_used = off + bytes;
+    check_hip(hipMemcpyAsync(buf.get(), dst, bytes, hipMemcpyHostToDevice, s), "hipMemcpyAsync tables");
     }
 
 //! Box lengths > 0 and finite tilt factors (HOOMD's BoxDim: Lx, Ly, Lz, xy, xz, yz).
@@ -459,6 +515,7 @@ void EspSolver::set_box_geometry(const double* box)
 void EspSolver::initialize(const SystemView& sys, hipStream_t s)
     {
     check_box(sys.box);
+    begin_uploads();
     for (int a = 0; a < 6; ++a)
         m_box[a] = sys.box[a];
     const bool fp64 = m_p.precision == 0;
@@ -526,9 +583,9 @@ void EspSolver::initialize(const SystemView& sys, hipStream_t s)
             }
         }
     if (fp64)
-        upload_as<double>(m_split_tab, rec);
+        upload_as<double>(m_split_tab, rec, s);
     else
-        upload_as<float>(m_split_tab, rec);
+        upload_as<float>(m_split_tab, rec, s);
     // compact LDS-resident table of S for the pair kernel, unless an external
     // reference-format table was supplied (then the reference lookup is used verbatim)
     if (!m_external_table)
@@ -536,14 +593,14 @@ void EspSolver::initialize(const SystemView& sys, hipStream_t s)
         const math::SplitKind kind = m_p.split == 1 ? math::SplitKind::GAUSSIAN : math::SplitKind::PSWF;
         const auto ct = compact_split(kind, m_p.kappa * m_p.rcut, COMPACT_SEGS, compact_deg());
         if (fp64)
-            upload_as<double>(m_compact, *ct);
+            upload_as<double>(m_compact, *ct, s);
         else
-            upload_as<float>(m_compact, *ct);
+            upload_as<float>(m_compact, *ct, s);
         }
     if (fp64)
-        upload_as<double>(m_wcoef, m_window.coef);
+        upload_as<double>(m_wcoef, m_window.coef, s);
     else
-        upload_as<float>(m_wcoef, m_window.coef);
+        upload_as<float>(m_wcoef, m_window.coef, s);
 
     // twiddles exp(-2 pi i m / n)
     const int dims[3] = {g.nx, g.ny, g.nz};
@@ -557,28 +614,29 @@ void EspSolver::initialize(const SystemView& sys, hipStream_t s)
             tw[2 * m + 1] = std::sin(ang);
             }
         if (fp64)
-            upload_as<double>(m_tw[a], tw);
+            upload_as<double>(m_tw[a], tw, s);
         else
-            upload_as<float>(m_tw[a], tw);
+            upload_as<float>(m_tw[a], tw, s);
         }
 
     // device buffers
     const size_t M = size_t(g.nx) * g.ny * g.nz;
     const size_t Mh = size_t(g.sxp) * g.ny * g.nz;  // padded rows
-    m_mesh.resize(M * 8, true);
+    m_mesh.resize_zeroed_async(M * 8, s);
     m_spec.resize(Mh * 2 * (fp64 ? 8 : 4));
     m_phi.resize(M * (fp64 ? 8 : 4));
     m_G.resize(Mh * 8);
     m_Gv.resize(Mh * 8);
-    m_tile_count.resize((size_t(m_n_tiles) + 1) * 4, true);
-    m_tile_start.resize((size_t(m_n_tiles) + 1) * 4, true);
-    m_evpart.resize(size_t(kernel::fft_z_blocks(g, fp64)) * 8 * 8, true);
-    m_qsum.resize(2 * 8, true);
-    m_ev_internal.resize(8 * 8, true);
-    m_fx.resize(4 * sizeof(float), true);
+    m_tile_count.resize_zeroed_async((size_t(m_n_tiles) + 1) * 4, s);
+    m_tile_start.resize_zeroed_async((size_t(m_n_tiles) + 1) * 4, s);
+    m_evpart.resize_zeroed_async(size_t(kernel::fft_z_blocks(g, fp64)) * 8 * 8, s);
+    m_qsum.resize_zeroed_async(2 * 8, s);
+    m_ev_internal.resize_zeroed_async(8 * 8, s);
+    m_fx.resize_zeroed_async(4 * sizeof(float), s);
     m_cap_particles = 0;
 
     build_influence(s);
+    end_uploads(s);
     m_need_init = false;
     ++m_gen;
     }
@@ -615,9 +673,9 @@ void EspSolver::build_influence(hipStream_t s)
         const double f = m_window.fourier(2.0 * math::PI * k / g.nz);
         w2z[z] = f * f;
         }
-    upload_as<double>(m_w2[0], w2x);
-    upload_as<double>(m_w2[1], w2y);
-    upload_as<double>(m_w2[2], w2z);
+    upload_as<double>(m_w2[0], w2x, s);
+    upload_as<double>(m_w2[1], w2y, s);
+    upload_as<double>(m_w2[2], w2z, s);
     int nq = 0;
     if (kind == math::SplitKind::PSWF)
         {
@@ -634,8 +692,8 @@ void EspSolver::build_influence(hipStream_t s)
         const auto quad = cached_quadrature(kind, m_p.kappa * m_p.rcut, nq);
         m_qx = quad->first;
         m_qw = quad->second;
-        upload_as<double>(m_qxd, m_qx);
-        upload_as<double>(m_qwd, m_qw);
+        upload_as<double>(m_qxd, m_qx, s);
+        upload_as<double>(m_qwd, m_qw, s);
         }
     else
         {
@@ -819,10 +877,17 @@ unsigned int EspSolver::prepare(const SystemView& sys, const OutputView& out, hi
         else
             {
             check_box(sys.box);
+            begin_uploads();
             set_box_geometry(sys.box);
             build_influence(s);
+            end_uploads(s);
             ++m_gen;
             }
+        }
+    else if (m_stage_recorded && s != m_stage_stream)
+        {
+        // the set-up ran on another stream: its copies and clears come first
+        check_hip(hipStreamWaitEvent(s, m_stage_ev, 0), "hipStreamWaitEvent");
         }
     const unsigned int n = sys.group_idx ? sys.group_size : sys.N;
     ensure_particles(std::max(n, 1u));

@@ -56,6 +56,9 @@ class DeviceBuffer
     DeviceBuffer(const DeviceBuffer&) = delete;
     DeviceBuffer& operator=(const DeviceBuffer&) = delete;
     void resize(size_t bytes, bool zero = false);  //!< reallocates only when growing
+    //! The same, zeroing with hipMemsetAsync on `s` (the blocking hipMemset waits behind
+    //! whatever the device's queues hold).
+    void resize_zeroed_async(size_t bytes, hipStream_t s);
     void* get() const { return m_ptr; }
     template<typename T> T* as() const { return static_cast<T*>(m_ptr); }
     size_t bytes() const { return m_bytes; }
@@ -155,6 +158,9 @@ class EspSolver
     //! Prepare this solver for a batched call on `sys` (initialisation, scratch sizing) and
     //! describe its part of the batch in `d`; the batch launches the kernels.
     void describe(const SystemView& sys, const OutputView& out, hipStream_t s, kernel::BatchSys& d);
+    //! The set-up the first computation on `sys` would do (validation, tables, influence
+    //! function on `s`, scratch sizing), without computing.
+    void setup(const SystemView& sys, const OutputView& out, hipStream_t s) { (void)prepare(sys, out, s); }
     bool fp64() const { return m_p.precision == 0; }
 
     //! Host-side tables (built at initialisation).
@@ -236,6 +242,20 @@ class EspSolver
     void run_mesh(const SystemView& sys, const OutputView& out, unsigned int n, hipStream_t s, bool want_ev, hipStream_t side);
     //! concurrent: the pass runs on the side stream beside the mesh chain.
     template<typename R> void run_pair(const SystemView& sys, const OutputView& out, hipStream_t s, bool concurrent);
+
+    //! Set-up uploads (tables, twiddles, window, quadrature) go through a pinned staging arena
+    //! with hipMemcpyAsync on the set-up's stream, so a set-up does not wait for work already
+    //! queued on the device (a server prepares admissions while other requests' steps run).
+    //! begin_uploads waits for the previous set-up's copies before the arena is reused.
+    void begin_uploads();
+    void end_uploads(hipStream_t s);
+    template<typename R> void upload_as(DeviceBuffer& buf, const std::vector<double>& v, hipStream_t s);
+    unsigned char* m_stage = nullptr;
+    size_t m_stage_cap = 0, m_stage_used = 0;
+    hipEvent_t m_stage_ev = nullptr;
+    hipStream_t m_stage_stream = nullptr;
+    bool m_stage_recorded = false;
+    std::vector<unsigned char*> m_stage_retired;  // outgrown blocks, freed by the next set-up
 
     SolverParams m_p;
     bool m_need_init = true;

@@ -167,7 +167,7 @@ class Server:
         self.waited_for_memory = 0
         # wall seconds per phase of the loop (admission, member
         # bookkeeping, list upkeep, ESP launch, the rest of the step up to its synchronisation)
-        self.admit_s: Dict[str, float] = {k: 0.0 for k in ("state", "nlist", "solver")}
+        self.admit_s: Dict[str, float] = {k: 0.0 for k in ("state", "nlist", "solver", "prepare")}
         self.phase_s: Dict[str, float] = {k: 0.0 for k in ("admit", "wait", "retire", "members", "nlists", "esp",
                                                           "step_rest")}
         self._tune_cache: Dict[Tuple[int, int], Params] = {}
@@ -297,7 +297,7 @@ class Server:
                 tb = time.perf_counter()
                 # a retired request's solver and list, when one is free: their device arrays are
                 # reused (set_params / reset rebuild everything that depends on the parameters)
-                pooled = self._ctx_pool.pop() if self._ctx_pool else None
+                pooled = self._take_context(len(pos), params)
                 if pooled is not None:
                     req.solver, req.nlist = pooled
                     req.nlist.reset(max(params.rcut, self.wca.r_cut))
@@ -310,6 +310,10 @@ class Server:
                     req.solver.set_params(params)
                 else:
                     req.solver = Solver(params)
+                td = time.perf_counter()
+                # tables, window and influence function now, while the step in flight runs,
+                # not in the next tick's batched call (about 1.3 ms of host time per request)
+                req.solver.prepare(req.state, stream=self.stream)
             except (ValueError, _lib.EspError) as e:
                 # this request alone fails; its uploads (if any) drain with the stream
                 self._fail(req, f"admission: {e}")
@@ -319,7 +323,8 @@ class Server:
                 continue
             self.admit_s["state"] += tb - ta
             self.admit_s["nlist"] += tc - tb
-            self.admit_s["solver"] += time.perf_counter() - tc
+            self.admit_s["solver"] += td - tc
+            self.admit_s["prepare"] += time.perf_counter() - td
             req.admitted_at = now
             req.admitted_wall = time.perf_counter()
             req.first_tick = True
@@ -329,6 +334,35 @@ class Server:
             self.peak_bytes = max(self.peak_bytes, self.resident_bytes)
             changed = True
         return changed
+
+    @staticmethod
+    def _cells(params: Params) -> int:
+        return int(params.nx) * int(params.ny) * int(params.nz)
+
+    def _take_context(self, N: int, params: Params):
+        """The pooled (solver, list) with the smallest arrays that already hold a system of N
+        particles on this mesh, or None.  A context too small for the request is left in the
+        pool: growing its arrays would free the old ones, and hipFree waits for the whole GPU
+        (the step in flight), which admissions are meant to overlap."""
+        cells = self._cells(params)
+        best = None
+        for k, (_, _, n_cap, c_cap) in enumerate(self._ctx_pool):
+            if n_cap >= N and c_cap >= cells and (best is None or (c_cap, n_cap) < self._ctx_pool[best][2:]):
+                best = k
+        if best is None:
+            return None
+        solver, nlist, _, _ = self._ctx_pool.pop(best)
+        return solver, nlist
+
+    def _pool_context(self, r: Request) -> None:
+        # keep the largest arrays seen by this context (they never shrink)
+        n_cap = max(r.state.N, getattr(r.solver, "_pool_n", 0))
+        c_cap = max(self._cells(r.params), getattr(r.solver, "_pool_cells", 0))
+        r.solver._pool_n, r.solver._pool_cells = n_cap, c_cap
+        self._ctx_pool.append((r.solver, r.nlist, n_cap, c_cap))
+        if len(self._ctx_pool) > 2 * self.max_active:
+            # drop the smallest (its arrays are freed now, rarely)
+            self._ctx_pool.remove(min(self._ctx_pool, key=lambda e: (e[3], e[2])))
 
     def _retire(self, now: float) -> None:
         keep = []
@@ -340,8 +374,8 @@ class Server:
                 if self.keep_results:
                     r.final_forces = r.state.force.numpy().copy()
                 self.resident_bytes -= r.footprint
-                if self.reuse_contexts and len(self._ctx_pool) < self.max_active:
-                    self._ctx_pool.append((r.solver, r.nlist))
+                if self.reuse_contexts:
+                    self._pool_context(r)
                 r.state = r.solver = r.nlist = r.snapshot = None
                 self._need.pop(r.rid, None)
                 self.done.append(r)

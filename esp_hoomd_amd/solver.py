@@ -168,6 +168,15 @@ class Solver:
         _lib.check(_lib.load().esp_compute_forces(self._ctx, timestep, C.byref(s), C.byref(o),
                                                   stream.handle if stream is not None else None), self._ctx)
 
+    def prepare(self, system, stream=None) -> None:
+        """Do the set-up the first computation on `system` (a DeviceSystem or an md.State)
+        would do -- tables, window, influence function (enqueued on `stream`), scratch --
+        without computing, so that it can happen off a step's critical path."""
+        s = system.c_system()
+        o = system.c_out(0)
+        _lib.check(_lib.load().esp_prepare(self._ctx, C.byref(s), C.byref(o), stream.handle if stream is not None else None),
+                   self._ctx)
+
     def compute(self, system: DeviceSystem, timestep: int = 0, flags: int = 0):
         """Run one evaluation and return host copies: force (N,4), virial (6,N), ext (7,)."""
         self.enqueue(system, timestep, flags)

@@ -162,6 +162,14 @@ int esp_compute_forces(esp_ctx* ctx, uint64_t timestep, const esp_system* sys, e
  * and profiled calls stay eager.  Default 0. */
 int esp_set_graph(esp_ctx* ctx, int enable);
 int esp_get_graph_stats(esp_ctx* ctx, unsigned long long* captures, unsigned long long* replays);
+/* The set-up esp_compute_forces does lazily on its first call after esp_create,
+ * esp_set_params or a box change (ESPForceCompute::computeForces initialises when
+ * m_need_initialize is set, ESPForceCompute.cc:98-107): arguments validated, tables and
+ * window built, the influence function enqueued on `stream`, scratch sized -- nothing
+ * computed.  For callers that want that work off a step's critical path, e.g. a server
+ * admitting a request while other requests' steps run.  Errors are those of
+ * esp_compute_forces. */
+int esp_prepare(esp_ctx* ctx, const esp_system* sys, const esp_out* out, esp_stream_t stream);
 /* the PotentialPair<EvaluatorPairPSWF> path alone (module.cc:73,79): pair + exclusions */
 int esp_pair_forces(esp_ctx* ctx, const esp_system* sys, esp_out* out, esp_stream_t stream);
 

@@ -68,6 +68,8 @@ def test_null_arguments():
     assert lib.esp_compute_forces(None, 0, None, None, None) == _lib.ESP_ERR_ARG
     s = Solver(Params(kappa=3.0, rcut=3.0))
     assert lib.esp_compute_forces(s.ctx, 0, None, None, None) == _lib.ESP_ERR_ARG
+    assert lib.esp_prepare(None, None, None, None) == _lib.ESP_ERR_ARG
+    assert lib.esp_prepare(s.ctx, None, None, None) == _lib.ESP_ERR_ARG
 
 
 def test_table_entries_and_range_error():

@@ -75,6 +75,24 @@ def test_batch_equals_each_system_alone(precision):
         _check_equal(r, a, exact=True)
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_prepared_solvers_give_the_same_bits(precision):
+    # set-up done ahead (esp_prepare) leaves nothing different: alone and batched results
+    # match solvers that set themselves up in their first computation
+    fresh = _systems(precision, seed0=3, cases=CASES[:3])
+    early = _systems(precision, seed0=3, cases=CASES[:3])
+    for s, d, *_ in early:
+        s.prepare(d)
+    for (s0, d0, *_), (s1, d1, *_) in zip(fresh, early):
+        _check_equal(s1.compute(d1), s0.compute(d0), exact=True)
+    early = _systems(precision, seed0=3, cases=CASES[:3])
+    for s, d, *_ in early:
+        s.prepare(d)
+    res = Batch().compute([(s, d) for s, d, *_ in early])
+    for (s0, d0, *_), r in zip(fresh, res):
+        _check_equal(r, s0.compute(d0), exact=True)
+
+
 def test_batch_fp64_matches_oracle():
     systems = _systems(0, seed0=5)
     res = Batch().compute([(s, d) for s, d, *_ in systems])

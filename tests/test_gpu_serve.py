@@ -154,7 +154,8 @@ def test_reused_solver_and_list_give_the_same_trajectory(precision):
     fresh.submit(a)
     fresh.run()
     srv = Server(precision=precision, keep_results=True)
-    srv.submit(_req(1, 5000, 20, density=0.2))       # larger, different mesh and cutoff
+    # a larger system on a larger mesh (a context too small for a request is not reused)
+    srv.submit(_req(1, 5000, 20, density=0.1))
     srv.run()
     a2 = _req(0, 2600, 45, density=0.12)
     srv.submit(a2)
