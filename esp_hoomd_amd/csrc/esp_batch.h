@@ -41,7 +41,7 @@ class EspBatch
         {
         kernel::BatchList pack, pair, bin, scan, scatter, tiles, fft_x, fft_y, fft_z, fft_xy, fin;
         };
-    void build_plan(bool fp64);
+    void build_plan(bool fp64, hipStream_t s);
     void enqueue(hipStream_t s);
     template<typename R> void enqueue_stages(hipStream_t s);
     void release_graph();
@@ -54,6 +54,8 @@ class EspBatch
     std::vector<uint64_t> m_shape;            // shape key of the current plan
     kernel::BatchSys* m_staging = nullptr;    // pinned upload staging
     size_t m_staging_cap = 0;
+    kernel::BatchItem* m_items_staging = nullptr;  // pinned staging of the work lists
+    size_t m_items_staging_cap = 0;
     hipEvent_t m_upload_done = nullptr;
     DeviceBuffer m_desc_dev;
     DeviceBuffer m_items_dev;
