@@ -237,14 +237,18 @@ def md_leg(params, pos, vel, q, L, steps, warmup, skin=0.4):
     """NVE velocity-Verlet MD of the resident system with this ESP configuration plus WCA
     (sigma = epsilon = 1), dt = 0.002: per step the first half kick and drift, the neighbour
     list's displacement check (one host synchronisation) and its rebuilds when a particle has
-    moved half the skin, the ESP forces, WCA over the inner list and the second half kick.
+    moved half the skin, the ESP forces, WCA over the inner list and the second half kick; every
+    100 steps the particles are re-sorted into cell order (and the list rebuilt).
     Timed by wall clock over `steps` steps after `warmup`."""
     from esp_hoomd_amd.device import synchronize
     from esp_hoomd_amd.md import ESPForce, NeighborList, Simulation, State, WCA
     from esp_hoomd_amd.solver import Solver
 
     st = State(pos, q, [L, L, L], velocities=vel)
-    sim = Simulation(st, dt=0.002, nlist=NeighborList(skin))
+    # particles put back in cell order every 100 steps (HOOMD's SFCPack keeps its particles
+    # sorted the same way; without it they diffuse away from their lattice order and the pair,
+    # binning and spreading passes lose their memory locality step by step)
+    sim = Simulation(st, dt=0.002, nlist=NeighborList(skin), sort_every=100)
     sim.add_force(ESPForce(Solver(params)))
     sim.add_force(WCA(1.0, 1.0))
     sim.run(warmup)
@@ -256,7 +260,7 @@ def md_leg(params, pos, vel, q, L, steps, warmup, skin=0.4):
     dt = time.perf_counter() - t0
     ke, pe = sim.energies()
     return {"steps": steps, "ms_per_step": dt / steps * 1e3, "particle_steps_per_s": len(pos) * steps / dt,
-            "list_builds": sim.nlist.builds - b0, "energy_change_per_KE0": abs(ke + pe - ke0 - pe0) / ke0,
+            "list_builds": sim.nlist.builds - b0, "sorts": sim.sorts, "energy_change_per_KE0": abs(ke + pe - ke0 - pe0) / ke0,
             "temperature": sim.temperature()}
 
 

@@ -171,6 +171,8 @@ def load() -> C.CDLL:
     _sig(lib, "esp_nlist_update", C.c_int, [_VP, C.POINTER(EspSystem), C.POINTER(C.c_int), _VP])
     _sig(lib, "esp_nlist_arrays", C.c_int, [_VP, C.POINTER(_VP), C.POINTER(_VP), C.POINTER(_VP), C.POINTER(C.c_uint)])
     _sig(lib, "esp_nlist_stats", C.c_int, [_VP, C.POINTER(C.c_ulonglong)])
+    _sig(lib, "esp_nlist_cell_order", C.c_int, [_VP, C.POINTER(C.c_void_p), C.POINTER(C.c_uint)])
+    _sig(lib, "esp_gather_rows", C.c_int, [C.c_uint, C.c_uint, _VP, _VP, _VP, _VP])
     _sig(lib, "esp_nlist_set_inner", C.c_int, [_VP, C.c_double])
     _sig(lib, "esp_nlist_inner_arrays", C.c_int, [_VP, C.POINTER(_VP), C.POINTER(_VP), C.POINTER(_VP), C.POINTER(C.c_uint)])
     _sig(lib, "esp_wca_forces", C.c_int, [C.POINTER(EspSystem), C.c_double, C.c_double, C.POINTER(EspOut), _VP])

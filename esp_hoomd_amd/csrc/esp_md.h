@@ -62,6 +62,8 @@ hipError_t batch_vv_second(const MDSys* d, const MDItem* items, unsigned int n_i
                            const unsigned char* hold, hipStream_t s);
 hipError_t batch_wca(const MDSys* d, const MDItem* items, unsigned int n_items, double sigma, double epsilon, hipStream_t s);
 hipError_t batch_max_disp(const MDSys* d, const MDItem* items, unsigned int n_items, unsigned long long* out, hipStream_t s);
+//! dst row i = src row order[i] for n rows of `words` 4-byte words.
+hipError_t gather_rows(unsigned int n, unsigned int words, const unsigned int* order, const void* src, void* dst, hipStream_t s);
 hipError_t batch_predict_disp(const MDSys* d, const MDItem* items, unsigned int n_items, double dt, unsigned long long* out,
                               hipStream_t s);
 

@@ -858,6 +858,28 @@ int esp_nlist_stats(esp_nlist* nl, unsigned long long* builds)
     return ESP_OK;
     }
 
+int esp_nlist_cell_order(esp_nlist* nl, const unsigned int** members, unsigned int* n)
+    {
+    if (!nl || !members || !n)
+        return ESP_ERR_ARG;
+    if (nl->builds == 0 || !nl->members.get())
+        return fail(nl, ESP_ERR_STATE, "esp_nlist_cell_order: no list has been built");
+    *members = nl->members.as<unsigned int>();
+    *n = nl->N;
+    return ESP_OK;
+    }
+
+int esp_gather_rows(unsigned int n, unsigned int row_bytes, const unsigned int* order, const void* src, void* dst,
+                    esp_stream_t stream)
+    {
+    if (n == 0)
+        return ESP_OK;
+    if (!order || !src || !dst || src == dst || row_bytes == 0 || row_bytes % 4 != 0)
+        return fail(nullptr, ESP_ERR_ARG, "esp_gather_rows: needs order, distinct src and dst, and row_bytes a multiple of 4");
+    const hipError_t e = esp::md::gather_rows(n, row_bytes / 4, order, src, dst, static_cast<hipStream_t>(stream));
+    return e == hipSuccess ? ESP_OK : fail(nullptr, ESP_ERR_HIP, hipGetErrorString(e));
+    }
+
 int esp_wca_forces(const esp_system* sys, double sigma, double epsilon, esp_out* out, esp_stream_t stream)
     {
     if (!sys || !out || !out->force || !sys->nlist || !sys->n_neigh || !sys->head_list)

@@ -611,6 +611,22 @@ __global__ void __launch_bounds__(256) kb_predict_disp(const md::MDSys* __restri
     });
     }
 
+//! dst row i = src row order[i], rows of `words` 4-byte words (a particle permutation); rows
+//! whose index is out of range are left as they are.
+__global__ void __launch_bounds__(256) k_gather_rows(unsigned int n, unsigned int words, const unsigned int* __restrict__ order,
+                                                     const unsigned int* __restrict__ src, unsigned int* __restrict__ dst)
+    {
+    const size_t total = size_t(n) * words;
+    for (size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < total; idx += size_t(gridDim.x) * blockDim.x)
+        {
+        const unsigned int r = unsigned(idx / words);
+        const unsigned int w = unsigned(idx - size_t(r) * words);
+        const unsigned int j = order[r];
+        if (j < n)
+            dst[idx] = src[size_t(j) * words + w];
+        }
+    }
+
     } // namespace dev
 
 namespace md
@@ -744,6 +760,17 @@ hipError_t batch_max_disp(const MDSys* d, const MDItem* items, unsigned int n_it
     if (n_items == 0)
         return hipSuccess;
     hipLaunchKernelGGL(kb_max_disp, dim3((n_items + DISP_ITEMS - 1) / DISP_ITEMS), dim3(256), 0, s, d, items, n_items, out);
+    return hipGetLastError();
+    }
+
+hipError_t gather_rows(unsigned int n, unsigned int words, const unsigned int* order, const void* src, void* dst, hipStream_t s)
+    {
+    const size_t total = size_t(n) * words;
+    if (total == 0)
+        return hipSuccess;
+    const unsigned int nb = unsigned(std::min<size_t>((total + 255) / 256, 8192));
+    hipLaunchKernelGGL(k_gather_rows, dim3(nb), dim3(256), 0, s, n, words, order, static_cast<const unsigned int*>(src),
+                       static_cast<unsigned int*>(dst));
     return hipGetLastError();
     }
 

@@ -97,7 +97,41 @@ class State:
         return o
 
     def positions(self) -> np.ndarray:
+        """Positions in the arrays' current order (see permute)."""
         return self.postype.numpy()[:, :3]
+
+    def in_input_order(self, a: np.ndarray) -> np.ndarray:
+        """Rows of a per-particle host array in the current order, put back in the order the
+        particles were given in (the identity until permute is called)."""
+        if getattr(self, "tag", None) is None:
+            return a
+        out = np.empty_like(a)
+        out[self.tag.numpy().astype(np.int64)] = a
+        return out
+
+    def permute(self, order_ptr: int, stream=None) -> None:
+        """Reorder the particles: row i of positions, velocities, net forces and charges becomes
+        row order[i] (a device permutation of [0, N), e.g. NeighborList.cell_order()), so that
+        particles close in space are close in memory again (the role of HOOMD's SFCPack sorter).
+        `tag` follows, mapping rows back to the input order.  Neighbour lists must be rebuilt
+        afterwards; per-particle virials are not carried (recomputed with the next forces)."""
+        if self.exclusions:
+            raise ValueError("State.permute: exclusion lists index particles and are not remapped")
+        lib = _lib.load()
+        if getattr(self, "tag", None) is None:
+            self.tag = DeviceArray.from_numpy(np.arange(self.N, dtype=np.uint32))
+        spare = getattr(self, "_spare", None) or {}
+        h = stream.handle if stream is not None else None
+        for name in ("postype", "vel", "force", "charge", "tag"):
+            src = getattr(self, name)
+            dst = spare.get(name)
+            if dst is None or dst.shape != src.shape or dst.dtype != src.dtype:
+                dst = DeviceArray(src.shape, src.dtype)
+            row = src.nbytes // max(self.N, 1)
+            _lib.check(lib.esp_gather_rows(self.N, row, C.c_void_p(order_ptr), src._ptr, dst._ptr, h))
+            spare[name] = src
+            setattr(self, name, dst)
+        self._spare = spare
 
     def velocities(self) -> np.ndarray:
         return self.vel.numpy()[:, :3]
@@ -203,6 +237,14 @@ class NeighborList:
         self._publish(state)
         return bool(rebuilt.value)
 
+    def cell_order(self) -> int:
+        """Device pointer to the particle indices in cell order of the last build."""
+        p, n = C.c_void_p(), C.c_uint()
+        rc = _lib.load().esp_nlist_cell_order(self._nl, C.byref(p), C.byref(n))
+        if rc != _lib.ESP_OK:
+            raise _lib.EspError(rc, _lib.load().esp_nlist_last_error(self._nl).decode())
+        return p.value
+
     @property
     def builds(self) -> int:
         n = C.c_ulonglong()
@@ -287,9 +329,18 @@ class Simulation:
     """Minimal MD driver: forces (ESP Spectral/Local, WCA) on a shared neighbour list and one
     integration method, all device resident."""
 
-    def __init__(self, state: State, dt: float, method=None, nlist: NeighborList | None = None):
+    def __init__(self, state: State, dt: float, method=None, nlist: NeighborList | None = None, sort_every: int = 0):
+        """sort_every > 0: every that many steps the particles are put back in cell order
+        (State.permute with the list's cell order) and the list is rebuilt, as HOOMD's SFCPack
+        updater keeps its particles sorted; neighbours in space stay neighbours in memory, which
+        the pair, binning and spreading passes rely on."""
         if dt <= 0:
             raise ValueError("Simulation: dt must be > 0")
+        if sort_every and state.exclusions:
+            raise ValueError("Simulation: sort_every needs a state without exclusions")
+        self.sort_every = int(sort_every)
+        self._force_rebuild = False
+        self.sorts = 0
         self.state = state
         self.dt = float(dt)
         self.method = method if method is not None else NVE()
@@ -316,7 +367,8 @@ class Simulation:
             # WCA walks an inner list when a longer-ranged force sets the list's cutoff
             wca = [f.r_cut for f in self.forces if isinstance(f, WCA)]
             self.nlist.set_inner(max(wca) if wca and max(wca) < self._r_cut() else 0.0)
-            self.nlist.update(st, self._r_cut())
+            self.nlist.update(st, self._r_cut(), force_rebuild=self._force_rebuild)
+            self._force_rebuild = False
         # the ESP objects (Spectral mesh part, Local pair part) share one solver: one call with
         # the union of their flags overwrites the net force; other forces then add to it
         flags = 0
@@ -343,6 +395,10 @@ class Simulation:
             self.compute_forces()
         box = np.ascontiguousarray(st.box[:3])
         for _ in range(int(steps)):
+            if self.sort_every and self.timestep % self.sort_every == 0 and self.nlist._nl is not None and self.nlist.builds > 0:
+                st.permute(self.nlist.cell_order())
+                self._force_rebuild = True
+                self.sorts += 1
             _lib.check(lib.esp_integrate_first(st.N, st.postype.ptr, st.vel.ptr, st.force.ptr, self.dt, _lib.dptr(box), None))
             self.timestep += 1
             self.compute_forces()

@@ -294,6 +294,16 @@ int esp_nlist_update(esp_nlist* nl, const esp_system* sys, int* rebuilt, esp_str
 int esp_nlist_arrays(esp_nlist* nl, const unsigned int** n_neigh, const unsigned int** nlist, const uint64_t** head_list,
                      unsigned int* nmax);
 int esp_nlist_stats(esp_nlist* nl, unsigned long long* builds);
+/* The particle indices in cell order of the last build (members of cell 0, then cell 1, ...;
+ * ascending within a cell): a permutation of [0, N) for esp_gather_rows.  HOOMD keeps its
+ * particles in space-filling-curve order with its SFCPack updater; a caller without one sorts
+ * with this (no counterpart in the reference).  ESP_ERR_STATE before the first build. */
+int esp_nlist_cell_order(esp_nlist* nl, const unsigned int** members, unsigned int* n);
+/* dst row i = src row order[i] for n rows of row_bytes bytes (a multiple of 4) on `stream`:
+ * permutes per-particle arrays into the order above (no counterpart in the reference).  Rows
+ * whose order entry is >= n are left unwritten. */
+int esp_gather_rows(unsigned int n, unsigned int row_bytes, const unsigned int* order, const void* src, void* dst,
+                    esp_stream_t stream);
 /* Inner list (no counterpart in the reference, where a HOOMD list is built at the largest cutoff
  * of all pair forces and each force walks all of it): with r_inner > 0 (and below r_cut) every
  * build also keeps, in its own rows, the pairs within r_inner + r_buff, which is a valid list for

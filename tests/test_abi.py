@@ -69,6 +69,9 @@ def test_null_arguments():
     s = Solver(Params(kappa=3.0, rcut=3.0))
     assert lib.esp_compute_forces(s.ctx, 0, None, None, None) == _lib.ESP_ERR_ARG
     assert lib.esp_prepare(None, None, None, None) == _lib.ESP_ERR_ARG
+    assert lib.esp_nlist_cell_order(None, None, None) == _lib.ESP_ERR_ARG
+    assert lib.esp_gather_rows(4, 8, None, None, None, None) == _lib.ESP_ERR_ARG
+    assert lib.esp_gather_rows(0, 8, None, None, None, None) == _lib.ESP_OK
     assert lib.esp_prepare(s.ctx, None, None, None) == _lib.ESP_ERR_ARG
 
 
