@@ -222,6 +222,63 @@ static void gpu_paths()
     OK(esp_memcpy_dtoh(f2.data(), d_f.p, f2.size() * 8));
     CHECK(std::memcmp(f1.data(), f2.data(), f1.size() * 8) == 0);
     CHECK(all_finite(f1));
+    {
+    // set-up ahead of the first computation (stream-ordered uploads from pinned staging, on
+    // another stream than the computation) changes nothing; so does a second set-up after new
+    // parameters, which reuses the staging
+    esp_ctx* pre = nullptr;
+    OK(esp_create(&pre, &p));
+    esp_stream_t st = nullptr;
+    OK(esp_stream_create(&st));
+    OK(esp_prepare(pre, &sys, &out, st));
+    OK(esp_compute_forces(pre, 0, &sys, &out, nullptr));
+    std::vector<double> f3(4 * N);
+    OK(esp_stream_synchronize(nullptr));
+    OK(esp_memcpy_dtoh(f3.data(), d_f.p, f3.size() * 8));
+    CHECK(std::memcmp(f1.data(), f3.data(), f1.size() * 8) == 0);
+    esp_params p_other = make_params(32, 6, 3.0, 3.5, ESP_PREC_FP64);
+    OK(esp_set_params(pre, &p_other));
+    OK(esp_prepare(pre, &sys, &out, st));
+    OK(esp_set_params(pre, &p));
+    OK(esp_prepare(pre, &sys, &out, st));
+    OK(esp_compute_forces(pre, 0, &sys, &out, st));
+    OK(esp_stream_synchronize(st));
+    OK(esp_memcpy_dtoh(f3.data(), d_f.p, f3.size() * 8));
+    CHECK(std::memcmp(f1.data(), f3.data(), f1.size() * 8) == 0);
+    CHECK(esp_prepare(pre, nullptr, &out, st) == ESP_ERR_ARG);
+    OK(esp_destroy(pre));
+    OK(esp_stream_destroy(st));
+    }
+    {
+    // cell order of the list's last build is a permutation; gathering rows by it and back
+    // (by its inverse) restores the positions exactly
+    const unsigned int* order = nullptr;
+    unsigned int n_order = 0;
+    OK(esp_nlist_cell_order(nl, &order, &n_order));
+    CHECK(n_order == N);
+    std::vector<unsigned int> h_order(N);
+    OK(esp_memcpy_dtoh(h_order.data(), order, N * 4));
+    std::vector<unsigned int> seen(N, 0), inverse(N, 0);
+    for (unsigned int i = 0; i < N; ++i)
+        if (h_order[i] < N)
+            {
+            ++seen[h_order[i]];
+            inverse[h_order[i]] = i;
+            }
+    bool perm = true;
+    for (unsigned int i = 0; i < N; ++i)
+        perm = perm && seen[i] == 1;
+    CHECK(perm);
+    Dev d_sorted(pos.size() * 8), d_back(pos.size() * 8), d_inv(N * 4);
+    OK(esp_memcpy_htod(d_inv.p, inverse.data(), N * 4));
+    OK(esp_gather_rows(N, 32, order, d_pos.p, d_sorted.p, nullptr));
+    OK(esp_gather_rows(N, 32, d_inv.as<unsigned int>(), d_sorted.p, d_back.p, nullptr));
+    std::vector<double> back(pos.size());
+    OK(esp_memcpy_dtoh(back.data(), d_back.p, back.size() * 8));
+    CHECK(std::memcmp(back.data(), pos.data(), pos.size() * 8) == 0);
+    CHECK(esp_gather_rows(N, 6, order, d_pos.p, d_sorted.p, nullptr) == ESP_ERR_ARG);
+    CHECK(esp_gather_rows(N, 32, order, d_pos.p, d_pos.p, nullptr) == ESP_ERR_ARG);
+    }
     double fs[3] = {0, 0, 0}, fa = 0;
     for (unsigned int i = 0; i < N; ++i)
         for (int a = 0; a < 3; ++a)
