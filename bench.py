@@ -24,7 +24,8 @@ suite ties that reference to the direct Ewald sum at this size):
   mixed precision, one launch per stage replayed from a captured hipGraph;
 * served: a Poisson stream of MD requests (N log-uniform in [1e3, 2e5], density U[0.05, 0.2],
   10-1000 steps) through the continuous-batching server: first offered faster than it can serve
-  (10 s, its capacity in particle-steps/s), then offered at 70 % of that capacity for a fixed
+  (12 s; its capacity in particle-steps/s over the ticks after the first 3 s, in which it fills
+  up from empty), then offered at 70 % of that capacity for a fixed
   wall time (latencies, queue delays); every finished request's force error is measured
   afterwards.
 
@@ -432,7 +433,7 @@ def batched_leg(n_sys, steps, warmup, seed=6):
 MEAN_REQUEST_WORK = (2e5 - 1e3) / math.log(200) * (1000 - 10) / math.log(100)  # E[N] E[steps] of the stream
 
 
-def _serve_run(scales, seconds, rate, seed, budget_gb, verify):
+def _serve_run(scales, seconds, rate, seed, budget_gb, verify, steady_from=None):
     from esp_hoomd_amd import _lib
     from esp_hoomd_amd.serve import Server, poisson_requests
     reqs = poisson_requests(int(rate * seconds * 1.5) + 8, rate=rate, seed=seed)
@@ -443,6 +444,8 @@ def _serve_run(scales, seconds, rate, seed, budget_gb, verify):
         srv.submit(r)
     st = srv.run(time_limit=seconds)
     st["offered_rate_per_s"] = rate
+    if steady_from is not None:
+        st["steady"] = srv.rate_after(steady_from, seconds)
     if verify:
         # every finished request's force error at its final positions, against the tight reference
         t0 = time.perf_counter()
@@ -453,20 +456,24 @@ def _serve_run(scales, seconds, rate, seed, budget_gb, verify):
     return st
 
 
-def served_leg(seconds, rate=None, overload_rate=100.0, capacity_seconds=10.0, seed=7, budget_gb=64):
+def served_leg(seconds, rate=None, overload_rate=100.0, capacity_seconds=12.0, ramp_seconds=3.0, seed=7, budget_gb=64):
     """Two phases on the same request generator: the server's capacity (a stream offered faster
-    than it can serve, `capacity_seconds`: the particle-steps/s it sustains), then the served
-    stream at `rate` -- by default 70 % of the measured capacity -- for `seconds`, with
+    than it can serve for `capacity_seconds`: the particle-steps/s it sustains once full, i.e.
+    over the ticks after the first `ramp_seconds`, in which it fills up from empty), then the
+    served stream at `rate` -- by default 70 % of the measured capacity -- for `seconds`, with
     latencies, queue delays and every finished result's force error."""
     from esp_hoomd_amd import _lib
     from esp_hoomd_amd.serve import Server
     t0 = time.perf_counter()
     scales = Server(precision=_lib.ESP_PREC_MIXED, target=TARGET).calibrate()
     calib_s = time.perf_counter() - t0
-    cap = _serve_run(scales, capacity_seconds, overload_rate, seed + 1, budget_gb, verify=False)
-    capacity = {"particle_steps_per_s": cap["particle_steps_per_s"], "solves_per_s": cap["solves_per_s"],
+    cap = _serve_run(scales, capacity_seconds, overload_rate, seed + 1, budget_gb, verify=False, steady_from=ramp_seconds)
+    steady = cap["steady"]
+    capacity = {"particle_steps_per_s": steady["particle_steps_per_s"], "solves_per_s": steady["solves_per_s"],
+                "mean_active": steady["mean_active"], "ticks": steady["ticks"],
+                "seconds": capacity_seconds, "ramp_excluded_s": ramp_seconds,
+                "whole_run_particle_steps_per_s": cap["particle_steps_per_s"],
                 "requests_per_s": cap["requests_per_s"], "offered_rate_per_s": overload_rate,
-                "seconds": capacity_seconds, "mean_active": cap["mean_active"],
                 "queue_delay_s_p50": cap["queue_delay_s_p50"]}
     if rate is None:
         rate = 0.7 * cap["particle_steps_per_s"] / MEAN_REQUEST_WORK

@@ -164,6 +164,8 @@ class Server:
         self.tick_ms: List[float] = []
         self.tick_active: List[int] = []
         self.tick_particles: List[int] = []
+        self.tick_end: List[float] = []  # completion time of each tick, seconds since run() began
+        self._run_start = None
         self.waited_for_memory = 0
         # wall seconds per phase of the loop (admission, member
         # bookkeeping, list upkeep, ESP launch, the rest of the step up to its synchronisation)
@@ -427,6 +429,7 @@ class Server:
         self.tick_ms.append((end - t0) * 1e3)
         self.tick_active.append(stepped)
         self.tick_particles.append(particles)
+        self.tick_end.append(end - (self._run_start if self._run_start is not None else t0))
         self._inflight = None
         for st in self._orphans:
             st.release_staging()
@@ -520,6 +523,8 @@ class Server:
     def run(self, time_limit: float = 1e9) -> dict:
         """Serve until every submitted request is done (or time_limit seconds pass)."""
         start = time.perf_counter()
+        if self._run_start is None:
+            self._run_start = start
         while self.pending or self.active or self._inflight is not None:
             now = time.perf_counter() - start
             if now > time_limit:
@@ -542,6 +547,17 @@ class Server:
         self._complete()
         wall = time.perf_counter() - start
         return self.stats(wall)
+
+    def rate_after(self, t0: float, t1: float) -> dict:
+        """Sustained rates over the ticks that completed between t0 and t1 seconds after run()
+        began (a server filling up from empty is excluded by a t0 past its ramp)."""
+        sel = [k for k, t in enumerate(self.tick_end) if t0 < t <= t1]
+        span = (self.tick_end[sel[-1]] - self.tick_end[sel[0] - 1]) if sel and sel[0] > 0 else 0.0
+        if not sel or span <= 0:
+            return {"particle_steps_per_s": 0.0, "solves_per_s": 0.0, "mean_active": 0.0, "ticks": 0}
+        return {"particle_steps_per_s": sum(self.tick_particles[k] for k in sel) / span,
+                "solves_per_s": sum(self.tick_active[k] for k in sel) / span,
+                "mean_active": float(np.mean([self.tick_active[k] for k in sel])), "ticks": len(sel)}
 
     def stats(self, wall: float) -> dict:
         # per-step latency as a request sees it: the wall time from its previous step (or its
