@@ -164,7 +164,7 @@ class Server:
         self.tick_ms: List[float] = []
         self.tick_active: List[int] = []
         self.tick_particles: List[int] = []
-        self.tick_end: List[float] = []  # completion time of each tick, seconds since run() began
+        self.tick_end: List[float] = []  # completion time of each tick, seconds since the first run() began
         self._run_start = None
         self.waited_for_memory = 0
         # wall seconds per phase of the loop (admission, member
@@ -549,8 +549,8 @@ class Server:
         return self.stats(wall)
 
     def rate_after(self, t0: float, t1: float) -> dict:
-        """Sustained rates over the ticks that completed between t0 and t1 seconds after run()
-        began (a server filling up from empty is excluded by a t0 past its ramp)."""
+        """Sustained rates over the ticks that completed between t0 and t1 seconds after the
+        first run() began (a server filling up from empty is excluded by a t0 past its ramp)."""
         sel = [k for k, t in enumerate(self.tick_end) if t0 < t <= t1]
         span = (self.tick_end[sel[-1]] - self.tick_end[sel[0] - 1]) if sel and sel[0] > 0 else 0.0
         if not sel or span <= 0:
