@@ -14,6 +14,7 @@ struct CellGeom
     {
     int nx, ny, nz;
     double Lx, Ly, Lz;
+    int reach = 1;  //!< neighbour cells searched on each side: 1 (cells >= r_list) or 2 (cells >= r_list / 2)
     };
 
 //! Optional inner list written by the same neighbour-list pass: the entries of the full list

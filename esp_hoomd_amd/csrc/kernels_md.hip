@@ -119,62 +119,91 @@ __device__ __forceinline__ void nlist_group(unsigned int m, int sub, bool active
     unsigned int* row = nlist + size_t(i) * nmax;
     unsigned int* row_in = in.r2 > 0.0 ? in.list + size_t(i) * in.nmax : nullptr;
     unsigned int k = 0, k_in = 0;
-    // neighbour cells, each visited once even when an axis has fewer than 3 cells
-    const int rx = cg.nx >= 2 ? 1 : 0, ry = cg.ny >= 2 ? 1 : 0, rz = cg.nz >= 2 ? 1 : 0;
-    for (int dz = -rz; dz <= (cg.nz == 2 ? 0 : rz); ++dz)
-        for (int dy = -ry; dy <= (cg.ny == 2 ? 0 : ry); ++dy)
-            for (int dx = -rx; dx <= (cg.nx == 2 ? 0 : rx); ++dx)
+    // candidates mm in [m0, m1) (cell-sorted, one image shift (bx, by, bz) = r_i - s for all of
+    // them), compacted into the rows with one ballot per G candidates
+    auto scan_run = [&](unsigned int m0, unsigned int m1, double bx, double by, double bz)
+    {
+        for (unsigned int mm0 = m0; mm0 < m1; mm0 += unsigned(G))
+            {
+            const unsigned int mm = mm0 + unsigned(sub);
+            unsigned int j = 0;
+            bool take = false;
+            double r2 = 0.0;
+            if (mm < m1 && mm != m)
                 {
-                const int ux = cx + dx, uy = cy + dy, uz = cz + dz;
-                const int x = (ux + cg.nx) % cg.nx, y = (uy + cg.ny) % cg.ny, z = (uz + cg.nz) % cg.nz;
-                // image of the neighbour cell next to particle i: r_j + s
-                const double sx = shx ? (ux < 0 ? -cg.Lx : (ux >= cg.nx ? cg.Lx : 0.0)) : 0.0;
-                const double sy = shy ? (uy < 0 ? -cg.Ly : (uy >= cg.ny ? cg.Ly : 0.0)) : 0.0;
-                const double sz = shz ? (uz < 0 ? -cg.Lz : (uz >= cg.nz ? cg.Lz : 0.0)) : 0.0;
-                const double bx = ri.x - sx, by = ri.y - sy, bz = ri.z - sz;
-                const unsigned int c = unsigned((z * cg.ny + y) * cg.nx + x);
-                const unsigned int m0 = start[c], m1 = start[c + 1];
-                for (unsigned int mm0 = m0; mm0 < m1; mm0 += unsigned(G))
+                const double4 rj = spos[mm];
+                double ddx = bx - rj.x, ddy = by - rj.y, ddz = bz - rj.z;
+                if (!shx)
+                    ddx -= cg.Lx * rint(ddx * invLx);
+                if (!shy)
+                    ddy -= cg.Ly * rint(ddy * invLy);
+                if (!shz)
+                    ddz -= cg.Lz * rint(ddz * invLz);
+                r2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                take = r2 < r_list2;
+                if (take)
                     {
-                    const unsigned int mm = mm0 + unsigned(sub);
-                    unsigned int j = 0;
-                    bool take = false;
-                    double r2 = 0.0;
-                    if (mm < m1 && mm != m)
-                        {
-                        const double4 rj = spos[mm];
-                        double ddx = bx - rj.x, ddy = by - rj.y, ddz = bz - rj.z;
-                        if (!shx)
-                            ddx -= cg.Lx * rint(ddx * invLx);
-                        if (!shy)
-                            ddy -= cg.Ly * rint(ddy * invLy);
-                        if (!shz)
-                            ddz -= cg.Lz * rint(ddz * invLz);
-                        r2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                        take = r2 < r_list2;
-                        if (take)
-                            {
-                            j = members[mm];
-                            for (unsigned int e = 0; take && e < ne; ++e)
-                                take = ex_list[size_t(i) * ex_pitch + e] != j;
-                            }
-                        }
-                    const unsigned long long accepted = (__ballot(take) >> gshift) & gmask;
-                    const unsigned int slot = k + unsigned(__popcll(accepted & below));
-                    if (take && slot < nmax)
-                        row[slot] = j;
-                    k += unsigned(__popcll(accepted));
-                    if (in.r2 > 0.0)
-                        {
-                        const bool inner = take && r2 < in.r2;
-                        const unsigned long long acc_in = (__ballot(inner) >> gshift) & gmask;
-                        const unsigned int slot_in = k_in + unsigned(__popcll(acc_in & below));
-                        if (inner && slot_in < in.nmax)
-                            row_in[slot_in] = j;
-                        k_in += unsigned(__popcll(acc_in));
-                        }
+                    j = members[mm];
+                    for (unsigned int e = 0; take && e < ne; ++e)
+                        take = ex_list[size_t(i) * ex_pitch + e] != j;
                     }
                 }
+            const unsigned long long accepted = (__ballot(take) >> gshift) & gmask;
+            const unsigned int slot = k + unsigned(__popcll(accepted & below));
+            if (take && slot < nmax)
+                row[slot] = j;
+            k += unsigned(__popcll(accepted));
+            if (in.r2 > 0.0)
+                {
+                const bool inner = take && r2 < in.r2;
+                const unsigned long long acc_in = (__ballot(inner) >> gshift) & gmask;
+                const unsigned int slot_in = k_in + unsigned(__popcll(acc_in & below));
+                if (inner && slot_in < in.nmax)
+                    row_in[slot_in] = j;
+                k_in += unsigned(__popcll(acc_in));
+                }
+            }
+    };
+    if (cg.reach == 2)
+        {
+        // half-size cells, every axis with at least five: the 5 x 5 rows of five x-cells around
+        // the particle; the x cells of a row are consecutive in cell order, so a row is one run
+        // of candidates (two or three where it wraps, each with its own image shift)
+        for (int dz = -2; dz <= 2; ++dz)
+            for (int dy = -2; dy <= 2; ++dy)
+                {
+                const int uy = cy + dy, uz = cz + dz;
+                const int y = (uy + cg.ny) % cg.ny, z = (uz + cg.nz) % cg.nz;
+                const double sy = uy < 0 ? -cg.Ly : (uy >= cg.ny ? cg.Ly : 0.0);
+                const double sz = uz < 0 ? -cg.Lz : (uz >= cg.nz ? cg.Lz : 0.0);
+                const double by = ri.y - sy, bz = ri.z - sz;
+                const unsigned int rowc = unsigned(z * cg.ny + y) * unsigned(cg.nx);
+                const int lo = cx - 2, hi = cx + 2;
+                scan_run(start[rowc + unsigned(max(lo, 0))], start[rowc + unsigned(min(hi, cg.nx - 1)) + 1], ri.x, by, bz);
+                if (lo < 0)  // cells nx + lo .. nx - 1, seen one box length below
+                    scan_run(start[rowc + unsigned(cg.nx + lo)], start[rowc + unsigned(cg.nx)], ri.x + cg.Lx, by, bz);
+                if (hi >= cg.nx)  // cells 0 .. hi - nx, one box length above
+                    scan_run(start[rowc], start[rowc + unsigned(hi - cg.nx) + 1], ri.x - cg.Lx, by, bz);
+                }
+        }
+    else
+        {
+        // neighbour cells, each visited once even when an axis has fewer than 3 cells
+        const int rx = cg.nx >= 2 ? 1 : 0, ry = cg.ny >= 2 ? 1 : 0, rz = cg.nz >= 2 ? 1 : 0;
+        for (int dz = -rz; dz <= (cg.nz == 2 ? 0 : rz); ++dz)
+            for (int dy = -ry; dy <= (cg.ny == 2 ? 0 : ry); ++dy)
+                for (int dx = -rx; dx <= (cg.nx == 2 ? 0 : rx); ++dx)
+                    {
+                    const int ux = cx + dx, uy = cy + dy, uz = cz + dz;
+                    const int x = (ux + cg.nx) % cg.nx, y = (uy + cg.ny) % cg.ny, z = (uz + cg.nz) % cg.nz;
+                    // image of the neighbour cell next to particle i: r_j + s
+                    const double sx = shx ? (ux < 0 ? -cg.Lx : (ux >= cg.nx ? cg.Lx : 0.0)) : 0.0;
+                    const double sy = shy ? (uy < 0 ? -cg.Ly : (uy >= cg.ny ? cg.Ly : 0.0)) : 0.0;
+                    const double sz = shz ? (uz < 0 ? -cg.Lz : (uz >= cg.nz ? cg.Lz : 0.0)) : 0.0;
+                    const unsigned int c = unsigned((z * cg.ny + y) * cg.nx + x);
+                    scan_run(start[c], start[c + 1], ri.x - sx, ri.y - sy, ri.z - sz);
+                    }
+        }
     if (sub == 0)
         {
         n_neigh[i] = k < nmax ? k : nmax;

@@ -31,9 +31,11 @@ def test_nlist_matches_kdtree(N, L, r_cut, buf, inner):
     # symmetric full list
     assert sum(len(x) for x in lists) == 2 * len(exp)
     # candidates are listed cell by cell, each cell in ascending index (not in the order the
-    # binning atomics happened to run): at most 27 ascending runs per row, and a second list
-    # built from the same positions is identical
-    assert max(1 + int(np.sum(np.diff(x.astype(np.int64)) < 0)) for x in lists if len(x)) <= 27
+    # binning atomics happened to run): at most one ascending run per neighbour cell (27 cells
+    # of at least r_list, or 125 of at least r_list / 2 when every axis has five of those), and
+    # a second list built from the same positions is identical
+    half = min(int(L // ((r_cut + buf) / 2)) for _ in range(3)) >= 5
+    assert max(1 + int(np.sum(np.diff(x.astype(np.int64)) < 0)) for x in lists if len(x)) <= (125 if half else 27)
     nl2 = md.NeighborList(buffer=buf, inner=inner)
     st2 = md.State(pos, q, (L, L, L), exclusions=excl)
     nl2.update(st2, r_cut, force_rebuild=True)
